@@ -1,0 +1,252 @@
+/*
+ * fsi_hip.h - C ABI of libfsi_hip.so, the gfx950 implementation of the FSI
+ * substep that FSIDriver3D runs (lizhuoh9/MPM-LBM).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative FSI_E_* code on
+ *    failure; fsi_last_error() returns a thread-local message for the last
+ *    failure on the calling thread.
+ *  - Every buffer is caller-owned device memory (the Python host allocates it
+ *    as torch tensors).  The library never allocates or frees caller memory.
+ *  - Calls are stream-ordered and asynchronous on the hipStream_t passed as
+ *    `stream` (NULL = default stream).  Nothing synchronises except
+ *    fsi_sync().
+ *  - Lattice cell (i,j,k) has linear index c = (i*ny + j)*nz + k, the same
+ *    row-major order as the reference's Taichi fields.  Populations are SoA:
+ *    F[s*N + c].  Vector fields are SoA: v[d*N + c].
+ *  - Particle arrays are SoA: x[d*P + p], C and F as 9 planes [(r*3+col)*P + p].
+ *
+ * The reference has no FFI: its operator interface is the duck-typed Python
+ * methods of LBMFluid3D / MPMSolid3D / the couplers.  Each entry point below
+ * names the reference method(s) it replaces.
+ */
+#ifndef FSI_HIP_H
+#define FSI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSI_OK 0
+#define FSI_E_INVALID (-1)  /* bad argument or shape */
+#define FSI_E_HIP (-2)      /* HIP runtime error */
+
+/* Boundary-condition codes of LBMConfig.bc_x_* (lbm/config.py:61-66). */
+#define FSI_BC_PERIODIC 0
+#define FSI_BC_PRESSURE 1
+#define FSI_BC_VELOCITY 2
+
+/* Substep flags. */
+#define FSI_SUB_STORED_MOMENTS 1u /* collide with stored rho/vel instead of moments of F_src */
+#define FSI_SUB_MOVING_WALL 2u    /* moving-wall bounce-back (streaming_moving_bounceback) */
+#define FSI_SUB_GATHER_PREV 4u    /* wall cells gather the previous substep's link forces */
+#define FSI_SUB_DIAG 8u           /* record per-link corrections for the diagnostics */
+#define FSI_SUB_CELL_FORCE 16u    /* add cell_force to LBMConfig.force */
+
+typedef struct FsiLbmParams {
+  int32_t nx, ny, nz;
+  int32_t bc_x_left, bc_x_right;
+  float s_v, s_other, rho0;
+  float force[3];
+  float rho_bc_x_left, rho_bc_x_right;
+  float vel_bc_x_left[3], vel_bc_x_right[3];
+  float dynamic_solid_threshold;
+} FsiLbmParams;
+
+/* Device-side counters of the LBM (all reductions are order-independent:
+ * integer adds, max of non-negative floats, fp64 sums). */
+typedef struct FsiLbmCounters {
+  int32_t bb_link_count;
+  uint32_t bb_max_correction_bits; /* float bits, non-negative */
+  int32_t mb_sample_count;
+  uint32_t accum_norm_max_bits;
+  uint32_t mean_norm_max_bits;
+  int32_t pad0;
+  double bb_net_solid_force[3];
+} FsiLbmCounters;
+
+typedef struct FsiLbmState {
+  float* F[2];            /* 19*N each */
+  float* rho;             /* N */
+  float* vel;             /* 3*N */
+  int8_t* solid;          /* N */
+  int8_t* static_solid;   /* N */
+  int8_t* old_solid;      /* N */
+  uint32_t* cellinfo;     /* N: solid bit, link / incoming masks, moving-wall bit */
+  float* solid_phi;       /* N */
+  float* solid_mass;      /* N */
+  float* solid_vel;       /* 3*N */
+  float* cell_force;      /* 3*N */
+  float* hydro_force;     /* 3*N */
+  float* link_c[2];       /* 18*N each: per-link force scalar, double-buffered by substep parity */
+  float* link_corr;       /* 18*N: per-link moving-wall correction of the DIAG substep */
+  double* hydro_accum;    /* 3*N: subcycle force accumulator (fp64) */
+  float* accum_runmax;    /* N: running max of |accum| over the substeps */
+  FsiLbmCounters* counters;
+} FsiLbmState;
+
+/* LBM field reductions for one diagnostics row (fluid.py:1931-1968,
+ * fsi_diagnostics.py:8-97). */
+typedef struct FsiLbmStats {
+  double fluid_count;
+  double rho_min, rho_max, mass_total;
+  double max_v, force_norm_max;
+  double fluid_sum_v[3];
+  double phi_weight_sum, phi_weighted_v[3];
+  double far_count, far_sum_v[3];
+  double max_cell_force_norm_all, max_hydro_force_norm_all;
+  double net_cell_force[3], net_hydro_force[3];
+  double mass_rho_all;
+} FsiLbmStats;
+
+typedef struct FsiMpmParams {
+  int32_t gnx, gny, gnz;   /* MPM grid nodes per axis (n_grid each in the reference) */
+  int32_t n_particles;
+  int32_t bound;           /* sticky-wall width (mpm/config.py:21) */
+  int32_t damping_enabled, lock_z_enabled;
+  float dx, inv_dx, dt;
+  float mu, la;
+  float gravity[3];
+  float velocity_damping;
+  float domain_max[3];     /* particle clamp upper bound + dx (1.0 per axis in the reference) */
+} FsiMpmParams;
+
+typedef struct FsiMpmState {
+  float* x;        /* 3*P */
+  float* v;        /* 3*P */
+  float* C;        /* 9*P */
+  float* F;        /* 9*P */
+  float* Jp;       /* P */
+  float* mass;     /* P */
+  float* vol0;     /* P */
+  int8_t* fixed_mask; /* P */
+  float* fixed_x;  /* 3*P */
+  float* grid_v;   /* 3*G */
+  float* grid_m;   /* G */
+  float* grid_f;   /* 3*G */
+} FsiMpmState;
+
+/* Coupling constants (projection.py:9-15, penalty.py:27-33, moving_boundary.py:24-32). */
+typedef struct FsiCouplingParams {
+  float inv_dx_norm;
+  float cell_volume_norm;
+  float vel_scale_norm_to_lbm;
+  float eps_mass;
+  float beta_lbm, penalty_phi_min, force_cap_lbm;
+  float force_density_scale;      /* dx_norm / lbm_dt_phys^2 */
+  float reaction_scale;
+  float mb_phi_min, mb_force_cap_norm;
+} FsiCouplingParams;
+
+/* Device counters of projection and reaction transfer. */
+typedef struct FsiCouplingCounters {
+  double projected_mass, projected_volume_raw, projected_volume_clamped;
+  uint32_t max_phi_raw_bits;
+  int32_t active_cell_count;
+  int32_t active_reaction_particle_count;
+  int32_t active_force_cell_count;
+  uint32_t max_particle_reaction_bits;
+  uint32_t max_grid_reaction_bits;
+  uint32_t max_cell_force_bits, max_hydro_force_bits;
+  double net_particle_reaction[3];
+  double net_grid_reaction[3];
+  double net_cell_force[3], net_hydro_force[3];
+} FsiCouplingCounters;
+
+typedef struct FsiMpmStats {
+  double min_x[3], max_x[3];
+  double max_speed, min_J, max_J, total_mass;
+  double mass_weighted_v[3];
+} FsiMpmStats;
+
+const char* fsi_last_error(void);
+int fsi_version(void);
+/* Writes sizeof of FsiLbmParams, FsiLbmState, FsiLbmCounters, FsiLbmStats,
+ * FsiMpmParams, FsiMpmState, FsiCouplingParams, FsiCouplingCounters,
+ * FsiMpmStats (in that order, up to n entries) so bindings can verify their
+ * struct layouts.  Returns the number of entries written. */
+int fsi_abi_sizes(int32_t* out, int n);
+int fsi_sync(void* stream);
+
+/* LBMFluid3D.init / static_init (fluid.py:329-344, :357-373):
+ * F[0] = F[1] = feq(rho0, 0), rho = rho0, vel = 0 on every cell. */
+int fsi_lbm_init(const FsiLbmParams* p, const FsiLbmState* st, void* stream);
+
+/* update_dynamic_solid + reinitialize_new_fluid_cells (fluid.py:1848-1877),
+ * plus the per-cell link masks that the reference recomputes inside every
+ * streaming pass (fluid.py:432-440, :489-519).  `dynamic` = 0 keeps `solid`
+ * as-is (penalty / none modes); `cur` is the buffer holding the current F. */
+int fsi_lbm_rebuild(const FsiLbmParams* p, const FsiLbmState* st, int dynamic, int cur, void* stream);
+
+/* One fused LBM substep: colission (fluid.py:399-413), streaming1 /
+ * streaming_moving_bounceback (:432-440, :489-519), Boundary_condition_legacy
+ * (:927-1024, x faces), reading F[cur] and writing F[1-cur].  With
+ * FSI_SUB_MOVING_WALL the wall cells accumulate hydro forces
+ * (accumulate_moving_boundary_force_sample, :469-473) one substep late,
+ * deterministically.  `parity` selects link_c[parity] for writing. */
+int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity,
+                    uint32_t flags, void* stream);
+
+/* streaming3 moments (fluid.py:1795-1814): rho, vel from F[cur]. */
+int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags,
+                    void* stream);
+
+/* clear_moving_boundary_force_accumulator (fluid.py:461-466) and the
+ * clear of clear_moving_boundary_diagnostics (:444-458). */
+int fsi_lbm_mb_begin(const FsiLbmParams* p, const FsiLbmState* st, void* stream);
+
+/* finalize_moving_boundary_diagnostics + finalize_moving_boundary_force_accumulator
+ * (fluid.py:522-536, :476-486): wall cells gather the last substep
+ * (link_c[parity]); hydro_force = accum / n_samples when `subcycled`, else
+ * the last substep's link force. */
+int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int parity, int n_samples,
+                        int subcycled, void* stream);
+
+/* Number of per-block partial records the two-pass reductions use; their
+ * workspaces hold FSI_REDUCE_BLOCKS records of the reduced struct. */
+#define FSI_REDUCE_BLOCKS 256
+
+/* get_stats + FSIDiagnostics3D LBM reductions (fluid.py:1931-1968,
+ * fsi_diagnostics.py:8-97) into a device FsiLbmStats.  Deterministic: per-block
+ * partials in `workspace_dev` (FSI_REDUCE_BLOCKS * sizeof(FsiLbmStats) bytes),
+ * then a fixed-order final pass. */
+int fsi_lbm_reduce_stats(const FsiLbmParams* p, const FsiLbmState* st, FsiLbmStats* workspace_dev,
+                         FsiLbmStats* out_dev, void* stream);
+
+/* MPMToLBMProjector3D.project (projection.py:20-97). */
+int fsi_project(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                void* stream);
+
+/* PenaltyFSICoupler3D.clear_force_fields + build_penalty_force (penalty.py:42-88). */
+int fsi_penalty_force(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiCouplingParams* cp,
+                      FsiCouplingCounters* cc, void* stream);
+
+/* MPMSolid3D.clear_grid (solid.py:213-217): zero grid_v, grid_m, grid_f. */
+int fsi_mpm_clear_grid(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
+
+/* MPMSolid3D.p2g (solid.py:220-249); writes Jp = det F. */
+int fsi_mpm_p2g(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
+
+/* Reaction transfer onto grid_f: mode 1 = PenaltyFSICoupler3D.add_lbm_reaction_to_mpm_grid
+ * (penalty.py:90-129), mode 2 = MovingBoundaryFSICoupler3D.add_moving_boundary_reaction_to_mpm_grid
+ * (moving_boundary.py:58-111). */
+int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                     const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                     int mode, void* stream);
+
+/* MPMSolid3D.grid_update (solid.py:252-259) and g2p (:262-315). */
+int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
+int fsi_mpm_g2p(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
+
+/* MPMSolid3D.reduce_stats + FSIDiagnostics3D.solid_mean_velocity_norm (solid.py:342-357,
+ * fsi_diagnostics.py:117-124). */
+int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmStats* workspace_dev,
+                         FsiMpmStats* out_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSI_HIP_H */

@@ -1,0 +1,1251 @@
+// fsi_core.h - per-cell / per-particle bodies of the FSI substep kernels.
+//
+// Every function here is FSI_HD: the HIP build (fsi_kernels.hip) calls them
+// from __global__ kernels on gfx950, and the CPU emulation build
+// (fsi_emu.cpp, test-only) calls the very same bodies from serial loops so
+// the kernel arithmetic and indexing can be checked on a machine without a
+// GPU.  Atomic updates go through fsi_atomic_* which map to hardware atomics
+// on the GPU and plain read-modify-write in the serial emulation.
+//
+// Reference semantics followed (lizhuoh9/MPM-LBM, src/mpm_lbm/sim/...):
+//   D3Q19 ordering, weights, LR table ........ lbm/fluid.py:252, :359-373
+//   feq ....................................... lbm/fluid.py:321-326
+//   MRT moments M, meq, Guo forcing ........... lbm/fluid.py:228-277, :386-413
+//   streaming + half-way / moving-wall BB ..... lbm/fluid.py:432-440, :489-519
+//   legacy x-face boundary conditions ......... lbm/fluid.py:927-958
+//   streaming3 moments ........................ lbm/fluid.py:1795-1814
+//   dynamic solid / reinit .................... lbm/fluid.py:1848-1877
+//   MPM p2g / grid_update / g2p ............... mpm/solid.py:213-315
+//   projection / penalty / MB reaction ........ coupling/{projection,penalty,moving_boundary}.py
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+#include "../../include/fsi_hip.h"
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define FSI_HD __host__ __device__ __forceinline__
+#define FSI_DEV_ONLY __device__
+#else
+#define FSI_HD static inline
+#define FSI_DEV_ONLY
+#endif
+
+namespace fsi {
+
+constexpr int Q = 19;
+constexpr int NLINK = 18;  // link directions 1..18 (direction 0 never leaves the cell)
+
+// cellinfo bits
+constexpr uint32_t CI_SOLID = 1u << 31;   // cell is solid
+constexpr uint32_t CI_MOVING = 1u << 30;  // fluid cell with a link to a solid cell of nonzero velocity
+constexpr uint32_t CI_MASK = 0x7FFFEu;    // bits 1..18: link (fluid) / incoming (solid) directions
+
+// D3Q19 velocity set in the reference's order (fluid.py:359-369).
+constexpr int EX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int EY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int EZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int LR[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+FSI_HD float wgt(int s) {
+  return s == 0 ? (1.0f / 3.0f) : (s <= 6 ? (1.0f / 18.0f) : (1.0f / 36.0f));
+}
+
+// feq(k, rho, u) = w_k rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u)   (fluid.py:321-326)
+FSI_HD float feq(int s, float rho, float ux, float uy, float uz) {
+  const float eu = (float)EX[s] * ux + (float)EY[s] * uy + (float)EZ[s] * uz;
+  const float uv = ux * ux + uy * uy + uz * uz;
+  return wgt(s) * rho * (1.0f + 3.0f * eu + 4.5f * eu * eu - 1.5f * uv);
+}
+
+// ---------------------------------------------------------------------------
+// Moment transforms.  M (fluid.py:228-247) has entries in {0,+-1,+-2}; its
+// inverse has the same sparsity as M^T with entries k/72 (for k in Z), so both
+// products are written out over the 9 opposite-direction pairs
+// (a, LR[a]) as sums P_a = F_a + F_LR(a) and differences Q_a = F_a - F_LR(a).
+// Row by row these equal the dense products of the reference exactly in
+// rational arithmetic; in fp32 they differ only by rounding.
+// ---------------------------------------------------------------------------
+FSI_HD void moments_M(const float* F, float* m) {
+  const float P1 = F[1] + F[2], Q1 = F[1] - F[2];
+  const float P3 = F[3] + F[4], Q3 = F[3] - F[4];
+  const float P5 = F[5] + F[6], Q5 = F[5] - F[6];
+  const float P7 = F[7] + F[8], Q7 = F[7] - F[8];
+  const float P9 = F[9] + F[10], Q9 = F[9] - F[10];
+  const float P11 = F[11] + F[12], Q11 = F[11] - F[12];
+  const float P13 = F[13] + F[14], Q13 = F[13] - F[14];
+  const float P15 = F[15] + F[16], Q15 = F[15] - F[16];
+  const float P17 = F[17] + F[18], Q17 = F[17] - F[18];
+  const float axis = P1 + P3 + P5;
+  const float diag = P7 + P9 + P11 + P13 + P15 + P17;
+  const float dxy = P7 + P9, dxz = P11 + P13, dyz = P15 + P17;
+  m[0] = F[0] + axis + diag;
+  m[1] = -F[0] + diag;
+  m[2] = F[0] - 2.0f * axis + diag;
+  const float jx = Q7 + Q9 + Q11 + Q13;
+  const float jy = Q7 - Q9 + Q15 + Q17;
+  const float jz = Q11 - Q13 + Q15 - Q17;
+  m[3] = Q1 + jx;
+  m[4] = -2.0f * Q1 + jx;
+  m[5] = Q3 + jy;
+  m[6] = -2.0f * Q3 + jy;
+  m[7] = Q5 + jz;
+  m[8] = -2.0f * Q5 + jz;
+  m[9] = 2.0f * P1 - P3 - P5 + dxy + dxz - 2.0f * dyz;
+  m[10] = -2.0f * P1 + P3 + P5 + dxy + dxz - 2.0f * dyz;
+  m[11] = P3 - P5 + dxy - dxz;
+  m[12] = -P3 + P5 + dxy - dxz;
+  m[13] = P7 - P9;
+  m[14] = P15 - P17;
+  m[15] = P11 - P13;
+  m[16] = Q7 + Q9 - Q11 - Q13;
+  m[17] = -Q7 + Q9 + Q15 + Q17;
+  m[18] = Q11 - Q13 - Q15 + Q17;
+}
+
+// f += M^{-1} r   (M^{-1} written out exactly; see moments_M)
+FSI_HD void add_invM(const float* r, float* f) {
+  const float a18 = (r[0] - r[2]) * (1.0f / 18.0f);
+  const float c0 = r[0] * (1.0f / 36.0f) + r[1] * (1.0f / 24.0f) + r[2] * (1.0f / 72.0f);
+  const float t910a = (r[9] - r[10]) * (1.0f / 12.0f);
+  const float t910b = (r[9] - r[10]) * (1.0f / 24.0f);
+  const float s910 = (r[9] + r[10]) * (1.0f / 48.0f);
+  const float s910b = (r[9] + r[10]) * (1.0f / 24.0f);
+  const float d1112 = (r[11] - r[12]) * (1.0f / 8.0f);
+  const float s1112 = (r[11] + r[12]) * (1.0f / 16.0f);
+  const float jx = r[3] * (1.0f / 12.0f) + r[4] * (1.0f / 24.0f);
+  const float jy = r[5] * (1.0f / 12.0f) + r[6] * (1.0f / 24.0f);
+  const float jz = r[7] * (1.0f / 12.0f) + r[8] * (1.0f / 24.0f);
+  const float r16 = r[16] * 0.125f, r17 = r[17] * 0.125f, r18 = r[18] * 0.125f;
+
+  f[0] += r[0] * (1.0f / 3.0f) - r[1] * 0.5f + r[2] * (1.0f / 6.0f);
+  {  // (1,2)
+    const float S = a18 + t910a, D = (r[3] - r[4]) * (1.0f / 6.0f);
+    f[1] += S + D;
+    f[2] += S - D;
+  }
+  {  // (3,4)
+    const float S = a18 - t910b + d1112, D = (r[5] - r[6]) * (1.0f / 6.0f);
+    f[3] += S + D;
+    f[4] += S - D;
+  }
+  {  // (5,6)
+    const float S = a18 - t910b - d1112, D = (r[7] - r[8]) * (1.0f / 6.0f);
+    f[5] += S + D;
+    f[6] += S - D;
+  }
+  const float cxy = c0 + s910 + s1112;
+  const float cxz = c0 + s910 - s1112;
+  const float cyz = c0 - s910b;
+  {  // (7,8)
+    const float S = cxy + r[13] * 0.25f, D = jx + jy + r16 - r17;
+    f[7] += S + D;
+    f[8] += S - D;
+  }
+  {  // (9,10)
+    const float S = cxy - r[13] * 0.25f, D = jx - jy + r16 + r17;
+    f[9] += S + D;
+    f[10] += S - D;
+  }
+  {  // (11,12)
+    const float S = cxz + r[15] * 0.25f, D = jx + jz - r16 + r18;
+    f[11] += S + D;
+    f[12] += S - D;
+  }
+  {  // (13,14)
+    const float S = cxz - r[15] * 0.25f, D = jx - jz - r16 - r18;
+    f[13] += S + D;
+    f[14] += S - D;
+  }
+  {  // (15,16)
+    const float S = cyz + r[14] * 0.25f, D = jy + jz + r17 - r18;
+    f[15] += S + D;
+    f[16] += S - D;
+  }
+  {  // (17,18)
+    const float S = cyz - r[14] * 0.25f, D = jy - jz + r17 + r18;
+    f[17] += S + D;
+    f[18] += S - D;
+  }
+}
+
+// MRT collision with Guo forcing (fluid.py:399-413):
+//   m = M F;  m -= S (m - meq(rho,u));  m += (1 - S/2) M g;  f = M^{-1} m
+// evaluated as f = F + M^{-1} [ -S (M F - meq) + (1 - S/2) M g ], which is the
+// same map because M^{-1} M = I.  Relaxation vector (fluid.py:304):
+//   [0, sv, sv, 0, so, 0, so, 0, so, sv x7, so x3].
+// meq (fluid.py:386-392): m0 = rho, m1 = u.u, m3/5/7 = u, m9 = 2ux^2-uy^2-uz^2,
+//   m11 = uy^2-uz^2, m13 = ux uy, m14 = uy uz, m15 = ux uz, others 0.
+// Guo term as written in the reference (fluid.py:408):
+//   g_l = w_l ((e_l - u).F / 3 + (e_l.u)(e_l.F) / 9).
+FSI_HD void mrt_collide(const float* F, float rho, float ux, float uy, float uz, bool has_force,
+                        float gx, float gy, float gz, float sv, float so, float* f) {
+  float m[Q];
+  moments_M(F, m);
+  float r[Q];
+  const float uu = ux * ux + uy * uy + uz * uz;
+  r[0] = 0.0f;
+  r[1] = -sv * (m[1] - uu);
+  r[2] = -sv * m[2];
+  r[3] = 0.0f;
+  r[4] = -so * m[4];
+  r[5] = 0.0f;
+  r[6] = -so * m[6];
+  r[7] = 0.0f;
+  r[8] = -so * m[8];
+  r[9] = -sv * (m[9] - (2.0f * ux * ux - uy * uy - uz * uz));
+  r[10] = -sv * m[10];
+  r[11] = -sv * (m[11] - (uy * uy - uz * uz));
+  r[12] = -sv * m[12];
+  r[13] = -sv * (m[13] - ux * uy);
+  r[14] = -sv * (m[14] - uy * uz);
+  r[15] = -sv * (m[15] - ux * uz);
+  r[16] = -so * m[16];
+  r[17] = -so * m[17];
+  r[18] = -so * m[18];
+  if (has_force) {
+    float g[Q];
+    const float uF = ux * gx + uy * gy + uz * gz;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+      const float eF = (float)EX[l] * gx + (float)EY[l] * gy + (float)EZ[l] * gz;
+      const float eu = (float)EX[l] * ux + (float)EY[l] * uy + (float)EZ[l] * uz;
+      g[l] = wgt(l) * ((eF - uF) / 3.0f + (eu * eF) / 9.0f);
+    }
+    float mg[Q];
+    moments_M(g, mg);
+    const float hv = 1.0f - 0.5f * sv, ho = 1.0f - 0.5f * so;
+    r[0] += mg[0];
+    r[1] += hv * mg[1];
+    r[2] += hv * mg[2];
+    r[3] += mg[3];
+    r[4] += ho * mg[4];
+    r[5] += mg[5];
+    r[6] += ho * mg[6];
+    r[7] += mg[7];
+    r[8] += ho * mg[8];
+#pragma unroll
+    for (int k = 9; k <= 15; ++k) r[k] += hv * mg[k];
+    r[16] += ho * mg[16];
+    r[17] += ho * mg[17];
+    r[18] += ho * mg[18];
+  }
+#pragma unroll
+  for (int s = 0; s < Q; ++s) f[s] = F[s];
+  add_invM(r, f);
+}
+
+// streaming3 moments (fluid.py:1795-1814): rho = sum F, u = (sum e F)/rho + (force/2)/rho.
+FSI_HD void moments_of(const float* F, float gx, float gy, float gz, float& rho, float& ux,
+                       float& uy, float& uz) {
+  float r = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) {
+    r += F[s];
+    mx += (float)EX[s] * F[s];
+    my += (float)EY[s] * F[s];
+    mz += (float)EZ[s] * F[s];
+  }
+  rho = r;
+  ux = mx / r + (gx * 0.5f) / r;
+  uy = my / r + (gy * 0.5f) / r;
+  uz = mz / r + (gz * 0.5f) / r;
+}
+
+// ---------------------------------------------------------------------------
+// Lattice indexing
+// ---------------------------------------------------------------------------
+struct Grid {
+  int nx, ny, nz;
+  long long N;
+};
+
+FSI_HD Grid make_grid(const FsiLbmParams& p) {
+  Grid g;
+  g.nx = p.nx;
+  g.ny = p.ny;
+  g.nz = p.nz;
+  g.N = (long long)p.nx * p.ny * p.nz;
+  return g;
+}
+
+FSI_HD int wrap(int i, int n) { return i < 0 ? n - 1 : (i > n - 1 ? 0 : i); }
+
+FSI_HD long long cell_of(const Grid& g, int i, int j, int k) {
+  return ((long long)i * g.ny + j) * g.nz + k;
+}
+
+FSI_HD void coords_of(const Grid& g, long long c, int& i, int& j, int& k) {
+  k = (int)(c % g.nz);
+  const long long t = c / g.nz;
+  j = (int)(t % g.ny);
+  i = (int)(t / g.ny);
+}
+
+// periodic_index(i + e_s) (fluid.py:420-429)
+FSI_HD long long neighbor(const Grid& g, int i, int j, int k, int s) {
+  return cell_of(g, wrap(i + EX[s], g.nx), wrap(j + EY[s], g.ny), wrap(k + EZ[s], g.nz));
+}
+
+// a fluid cell on an x face whose populations the legacy BC overwrites
+FSI_HD bool bc_face(const FsiLbmParams& p, int i) {
+  return (i == 0 && p.bc_x_left != FSI_BC_PERIODIC) ||
+         (i == p.nx - 1 && p.bc_x_right != FSI_BC_PERIODIC);
+}
+
+// ---------------------------------------------------------------------------
+// Atomics / order-independent reductions
+// ---------------------------------------------------------------------------
+#ifdef __HIPCC__
+__device__ __forceinline__ void atomic_add_f32(float* a, float v) { atomicAdd(a, v); }
+__device__ __forceinline__ void atomic_add_f64(double* a, double v) { atomicAdd(a, v); }
+__device__ __forceinline__ void atomic_max_nonneg(uint32_t* a, float v) {
+  atomicMax(a, __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_add_i32(int32_t* a, int32_t v) { atomicAdd(a, v); }
+#define FSI_ATOMIC_HD __device__ __forceinline__
+#else
+static inline uint32_t f2u(float v) {
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  return u;
+}
+static inline void atomic_add_f32(float* a, float v) { *a += v; }
+static inline void atomic_add_f64(double* a, double v) { *a += v; }
+static inline void atomic_max_nonneg(uint32_t* a, float v) {
+  const uint32_t u = f2u(v);
+  if (u > *a) *a = u;
+}
+static inline void atomic_add_i32(int32_t* a, int32_t v) { *a += v; }
+#define FSI_ATOMIC_HD static inline
+#endif
+
+FSI_HD float norm3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+// ---------------------------------------------------------------------------
+// LBM per-cell bodies
+// ---------------------------------------------------------------------------
+
+// init (fluid.py:329-344): F = feq(rho0, 0) in both buffers, rho = rho0, u = 0.
+FSI_HD void lbm_init_cell(const FsiLbmParams& p, const FsiLbmState& st, long long c) {
+  const Grid g = make_grid(p);
+#pragma unroll
+  for (int s = 0; s < Q; ++s) {
+    const float v = feq(s, p.rho0, 0.0f, 0.0f, 0.0f);
+    st.F[0][s * g.N + c] = v;
+    st.F[1][s * g.N + c] = v;
+  }
+  st.rho[c] = p.rho0;
+  st.vel[c] = 0.0f;
+  st.vel[g.N + c] = 0.0f;
+  st.vel[2 * g.N + c] = 0.0f;
+}
+
+FSI_HD bool new_solid_flag(const FsiLbmParams& p, const FsiLbmState& st, long long c, int dynamic) {
+  if (!dynamic) return st.solid[c] != 0;
+  return st.static_solid[c] != 0 || st.solid_phi[c] >= p.dynamic_solid_threshold;
+}
+
+// update_dynamic_solid + reinitialize_new_fluid_cells (fluid.py:1848-1877) and
+// the link masks of the streaming passes.  Neighbour flags are evaluated from
+// static_solid / solid_phi directly, so every cell can be processed in one
+// pass without waiting for its neighbours' flag writes.
+FSI_HD void lbm_rebuild_cell(const FsiLbmParams& p, const FsiLbmState& st, int dynamic, int cur,
+                             long long c) {
+  const Grid g = make_grid(p);
+  int i, j, k;
+  coords_of(g, c, i, j, k);
+  const bool prev = st.solid[c] != 0;
+  const bool now = new_solid_flag(p, st, c, dynamic);
+  float* Fc = st.F[cur];
+  float* Fo = st.F[1 - cur];
+  if (dynamic) {
+    st.old_solid[c] = prev ? 1 : 0;
+    st.solid[c] = now ? 1 : 0;
+    if (prev && !now) {
+      // solid -> fluid: reset to feq(rho0, solid_vel or 0)
+      float vx = st.solid_vel[c], vy = st.solid_vel[g.N + c], vz = st.solid_vel[2 * g.N + c];
+      if (!(norm3(vx, vy, vz) > 0.0f)) vx = vy = vz = 0.0f;
+#pragma unroll
+      for (int s = 0; s < Q; ++s) {
+        const float v = feq(s, p.rho0, vx, vy, vz);
+        Fc[s * g.N + c] = v;
+        Fo[s * g.N + c] = v;
+      }
+      st.rho[c] = p.rho0;
+      st.vel[c] = vx;
+      st.vel[g.N + c] = vy;
+      st.vel[2 * g.N + c] = vz;
+    } else if (!prev && now) {
+      // fluid -> solid: the populations freeze at their current value
+#pragma unroll
+      for (int s = 0; s < Q; ++s) Fo[s * g.N + c] = Fc[s * g.N + c];
+    }
+  }
+  uint32_t info = 0;
+  if (now) {
+    info = CI_SOLID;
+#pragma unroll
+    for (int s = 1; s < Q; ++s) {
+      // incoming link: the fluid cell c - e_s streams direction s into c
+      const long long x = neighbor(g, i, j, k, LR[s]);
+      if (!new_solid_flag(p, st, x, dynamic)) info |= 1u << s;
+    }
+  } else {
+#pragma unroll
+    for (int s = 1; s < Q; ++s) {
+      const long long y = neighbor(g, i, j, k, s);
+      if (new_solid_flag(p, st, y, dynamic)) {
+        info |= 1u << s;
+        if (st.solid_vel[y] != 0.0f || st.solid_vel[g.N + y] != 0.0f ||
+            st.solid_vel[2 * g.N + y] != 0.0f)
+          info |= CI_MOVING;
+      }
+    }
+  }
+  st.cellinfo[c] = info;
+}
+
+// Wall cell y gathers the per-link force scalars written by its fluid
+// neighbours in the previous substep and adds the resulting hydro force to
+// the fp64 accumulator.  Link (x = y - e_s, s) carries c = f_s + bounced, and
+// its force on the solid is e_s * c (fluid.py:510-519).
+FSI_HD void wall_gather(const Grid& g, const FsiLbmState& st, const float* link, uint32_t info,
+                        int i, int j, int k, long long y, double& hx, double& hy, double& hz) {
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+  for (int s = 1; s < Q; ++s) {
+    if (info & (1u << s)) {
+      const long long x = neighbor(g, i, j, k, LR[s]);
+      const float cs = link[(long long)(s - 1) * g.N + x];
+      fx += (float)EX[s] * cs;
+      fy += (float)EY[s] * cs;
+      fz += (float)EZ[s] * cs;
+    }
+  }
+  hx = fx;
+  hy = fy;
+  hz = fz;
+}
+
+FSI_HD void wall_accumulate(const Grid& g, const FsiLbmState& st, long long y, double hx,
+                            double hy, double hz) {
+  const double ax = st.hydro_accum[y] + hx;
+  const double ay = st.hydro_accum[g.N + y] + hy;
+  const double az = st.hydro_accum[2 * g.N + y] + hz;
+  st.hydro_accum[y] = ax;
+  st.hydro_accum[g.N + y] = ay;
+  st.hydro_accum[2 * g.N + y] = az;
+  const float n = norm3((float)ax, (float)ay, (float)az);
+  if (n > st.accum_runmax[y]) st.accum_runmax[y] = n;
+}
+
+// Previous-step velocity of cell (i,j,k) as the reference's `v` field holds it
+// at Boundary_condition time: stored moments, or moments of F_src; 0 on solid.
+FSI_HD void prev_velocity(const FsiLbmParams& p, const FsiLbmState& st, const Grid& g,
+                          const float* Fs, uint32_t flags, long long c, float& ux, float& uy,
+                          float& uz) {
+  if (st.cellinfo[c] & CI_SOLID) {
+    ux = uy = uz = 0.0f;
+    return;
+  }
+  if (flags & FSI_SUB_STORED_MOMENTS) {
+    ux = st.vel[c];
+    uy = st.vel[g.N + c];
+    uz = st.vel[2 * g.N + c];
+    return;
+  }
+  float Fn[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) Fn[s] = Fs[s * g.N + c];
+  float gx = p.force[0], gy = p.force[1], gz = p.force[2];
+  if (flags & FSI_SUB_CELL_FORCE) {
+    gx += st.cell_force[c];
+    gy += st.cell_force[g.N + c];
+    gz += st.cell_force[2 * g.N + c];
+  }
+  float r;
+  moments_of(Fn, gx, gy, gz, r, ux, uy, uz);
+}
+
+
+// One fused LBM substep for fluid cell c (push form): collide F[cur] at c,
+// push the post-collision populations to the neighbours in F[1-cur], bounce
+// them back at solid neighbours (half-way, with the Ladd moving-wall
+// correction when FSI_SUB_MOVING_WALL), and apply the legacy x-face BCs.
+// Every population of every fluid cell of F[1-cur] is written exactly once:
+// by the upstream fluid neighbour, by the cell's own bounce-back, or by the BC.
+// Wall (solid) cells gather the previous substep's link forces instead.
+FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, int parity,
+                             uint32_t flags, long long c) {
+  const Grid g = make_grid(p);
+  const uint32_t info = st.cellinfo[c];
+  int i, j, k;
+  coords_of(g, c, i, j, k);
+  const bool mb = (flags & FSI_SUB_MOVING_WALL) != 0;
+  if (info & CI_SOLID) {
+    if (mb && (flags & FSI_SUB_GATHER_PREV) && (info & CI_MASK)) {
+      double hx, hy, hz;
+      wall_gather(g, st, st.link_c[1 - parity], info, i, j, k, c, hx, hy, hz);
+      wall_accumulate(g, st, c, hx, hy, hz);
+    }
+    return;
+  }
+  const float* Fs = st.F[cur];
+  float* Fd = st.F[1 - cur];
+  float F[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) F[s] = Fs[s * g.N + c];
+  float gx = p.force[0], gy = p.force[1], gz = p.force[2];
+  if (flags & FSI_SUB_CELL_FORCE) {
+    gx += st.cell_force[c];
+    gy += st.cell_force[g.N + c];
+    gz += st.cell_force[2 * g.N + c];
+  }
+  const bool has_force = (gx != 0.0f) || (gy != 0.0f) || (gz != 0.0f);
+  float rho, ux, uy, uz;
+  if (flags & FSI_SUB_STORED_MOMENTS) {
+    rho = st.rho[c];
+    ux = st.vel[c];
+    uy = st.vel[g.N + c];
+    uz = st.vel[2 * g.N + c];
+  } else {
+    moments_of(F, gx, gy, gz, rho, ux, uy, uz);
+  }
+  float f[Q];
+  mrt_collide(F, rho, ux, uy, uz, has_force, gx, gy, gz, p.s_v, p.s_other, f);
+
+  const bool self_face = bc_face(p, i);
+  float* link = mb ? st.link_c[parity] : nullptr;
+  const bool diag = mb && (flags & FSI_SUB_DIAG);
+#pragma unroll
+  for (int s = 0; s < Q; ++s) {
+    if (s != 0 && (info & (1u << s))) {
+      float corr = 0.0f;
+      if (mb && (info & CI_MOVING)) {
+        const long long y = neighbor(g, i, j, k, s);
+        const float wx = st.solid_vel[y], wy = st.solid_vel[g.N + y], wz = st.solid_vel[2 * g.N + y];
+        corr = -6.0f * wgt(s) * rho * ((float)EX[s] * wx + (float)EY[s] * wy + (float)EZ[s] * wz);
+      }
+      const float bounced = f[s] + corr;
+      if (!self_face) Fd[LR[s] * g.N + c] = bounced;
+      if (mb) {
+        link[(long long)(s - 1) * g.N + c] = f[s] + bounced;
+        if (diag) st.link_corr[(long long)(s - 1) * g.N + c] = corr;
+      }
+    } else {
+      const int ti = wrap(i + EX[s], g.nx);
+      if (!bc_face(p, ti)) {
+        const long long y = cell_of(g, ti, wrap(j + EY[s], g.ny), wrap(k + EZ[s], g.nz));
+        Fd[s * g.N + y] = f[s];
+      }
+    }
+  }
+
+  if (self_face) {
+    // Boundary_condition_legacy (fluid.py:927-958); right face applied last.
+    float brho = 1.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+    if (i == 0 && p.bc_x_left != FSI_BC_PERIODIC) {
+      if (p.bc_x_left == FSI_BC_VELOCITY) {
+        brho = 1.0f;
+        bx = p.vel_bc_x_left[0];
+        by = p.vel_bc_x_left[1];
+        bz = p.vel_bc_x_left[2];
+      } else {
+        brho = p.rho_bc_x_left;
+        const long long c1 = cell_of(g, (g.nx > 1 ? 1 : 0), j, k);
+        if (st.cellinfo[c1] & CI_SOLID) {
+          bx = by = bz = 0.0f;
+        } else {
+          bx = ux;
+          by = uy;
+          bz = uz;
+        }
+      }
+    }
+    if (i == g.nx - 1 && p.bc_x_right != FSI_BC_PERIODIC) {
+      if (p.bc_x_right == FSI_BC_VELOCITY) {
+        brho = 1.0f;
+        bx = p.vel_bc_x_right[0];
+        by = p.vel_bc_x_right[1];
+        bz = p.vel_bc_x_right[2];
+      } else {
+        brho = p.rho_bc_x_right;
+        const long long c2 = cell_of(g, (g.nx > 1 ? g.nx - 2 : 0), j, k);
+        prev_velocity(p, st, g, Fs, flags, c2, bx, by, bz);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < Q; ++s) Fd[s * g.N + c] = feq(s, brho, bx, by, bz);
+  }
+}
+
+// streaming3 (fluid.py:1795-1814) on the populations in F[cur].
+FSI_HD void lbm_moments_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, uint32_t flags,
+                             long long c) {
+  const Grid g = make_grid(p);
+  if (st.cellinfo[c] & CI_SOLID) {
+    st.rho[c] = p.rho0;
+    st.vel[c] = 0.0f;
+    st.vel[g.N + c] = 0.0f;
+    st.vel[2 * g.N + c] = 0.0f;
+    return;
+  }
+  float F[Q];
+  const float* Fs = st.F[cur];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) F[s] = Fs[s * g.N + c];
+  float gx = p.force[0], gy = p.force[1], gz = p.force[2];
+  if (flags & FSI_SUB_CELL_FORCE) {
+    gx += st.cell_force[c];
+    gy += st.cell_force[g.N + c];
+    gz += st.cell_force[2 * g.N + c];
+  }
+  float rho, ux, uy, uz;
+  moments_of(F, gx, gy, gz, rho, ux, uy, uz);
+  st.rho[c] = rho;
+  st.vel[c] = ux;
+  st.vel[g.N + c] = uy;
+  st.vel[2 * g.N + c] = uz;
+}
+
+// Values one cell contributes to the finalize reductions.
+struct FinalizeOut {
+  int links;
+  float max_corr;
+  float mean_norm;
+  float runmax;
+  double hx, hy, hz;  // last substep's hydro force on this cell
+};
+
+// finalize_moving_boundary_diagnostics / finalize_moving_boundary_force_accumulator
+// (fluid.py:522-536, :476-486) for cell y; writes hydro_force[y].
+FSI_HD FinalizeOut lbm_finalize_cell(const FsiLbmParams& p, const FsiLbmState& st, int parity,
+                                     int n_samples, int subcycled, long long y) {
+  const Grid g = make_grid(p);
+  FinalizeOut o;
+  o.links = 0;
+  o.max_corr = 0.0f;
+  o.mean_norm = 0.0f;
+  o.runmax = 0.0f;
+  o.hx = o.hy = o.hz = 0.0;
+  const uint32_t info = st.cellinfo[y];
+  if ((info & CI_SOLID) && (info & CI_MASK)) {
+    int i, j, k;
+    coords_of(g, y, i, j, k);
+    wall_gather(g, st, st.link_c[parity], info, i, j, k, y, o.hx, o.hy, o.hz);
+    if (subcycled) wall_accumulate(g, st, y, o.hx, o.hy, o.hz);
+#pragma unroll
+    for (int s = 1; s < Q; ++s) {
+      if (info & (1u << s)) {
+        ++o.links;
+        const long long x = neighbor(g, i, j, k, LR[s]);
+        const float a = fabsf(st.link_corr[(long long)(s - 1) * g.N + x]);
+        if (a > o.max_corr) o.max_corr = a;
+      }
+    }
+  }
+  if (subcycled) {
+    const double inv = 1.0 / (double)(n_samples > 0 ? n_samples : 1);
+    const float mx = (float)(st.hydro_accum[y] * inv);
+    const float my = (float)(st.hydro_accum[g.N + y] * inv);
+    const float mz = (float)(st.hydro_accum[2 * g.N + y] * inv);
+    st.hydro_force[y] = mx;
+    st.hydro_force[g.N + y] = my;
+    st.hydro_force[2 * g.N + y] = mz;
+    o.mean_norm = norm3(mx, my, mz);
+    o.runmax = st.accum_runmax[y];
+  } else {
+    st.hydro_force[y] = (float)o.hx;
+    st.hydro_force[g.N + y] = (float)o.hy;
+    st.hydro_force[2 * g.N + y] = (float)o.hz;
+  }
+  return o;
+}
+
+// clear_moving_boundary_force_accumulator + clear of hydro_force (fluid.py:444-466).
+FSI_HD void lbm_mb_begin_cell(const FsiLbmParams& p, const FsiLbmState& st, long long c) {
+  const Grid g = make_grid(p);
+  st.hydro_accum[c] = 0.0;
+  st.hydro_accum[g.N + c] = 0.0;
+  st.hydro_accum[2 * g.N + c] = 0.0;
+  st.accum_runmax[c] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// MPM (mpm/solid.py) per-particle / per-node bodies
+// ---------------------------------------------------------------------------
+struct MpmGrid {
+  int nx, ny, nz;
+  long long G;
+};
+FSI_HD MpmGrid make_mgrid(const FsiMpmParams& p) {
+  MpmGrid g;
+  g.nx = p.gnx;
+  g.ny = p.gny;
+  g.nz = p.gnz;
+  g.G = (long long)p.gnx * p.gny * p.gnz;
+  return g;
+}
+
+// Quadratic B-spline weights about base = int(X - 0.5) (truncation, as the
+// reference's ti.cast) with fx = X - base (solid.py:222-230).
+FSI_HD void bspline(float X, int& base, float& fx, float* w) {
+  base = (int)(X - 0.5f);
+  fx = X - (float)base;
+  const float a = 1.5f - fx, b = fx - 1.0f, d = fx - 0.5f;
+  w[0] = 0.5f * (a * a);
+  w[1] = 0.75f - b * b;
+  w[2] = 0.5f * (d * d);
+}
+
+FSI_HD float det3(const float* A) {
+  return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+         A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+
+// Rotation factor R = U V^T of the SVD F = U diag(s) V^T with det U = det V = +1
+// (sign of det F carried by the smallest singular value), the convention of
+// the reference's ti.svd.  Method: cyclic Jacobi eigen-decomposition of the
+// symmetric F^T F (Golub & Van Loan, Matrix Computations, 4th ed., sec. 8.5),
+// U's first two columns from F v_i / s_i (Gram-Schmidt), third = u0 x u1.
+FSI_HD void rotation_of(const float* Fm, float* R) {
+  float A[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      A[r * 3 + c] = Fm[0 * 3 + r] * Fm[0 * 3 + c] + Fm[1 * 3 + r] * Fm[1 * 3 + c] +
+                     Fm[2 * 3 + r] * Fm[2 * 3 + c];
+  float V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int sweep = 0; sweep < 6; ++sweep) {
+#pragma unroll
+    for (int pq = 0; pq < 3; ++pq) {
+      const int pi = pq == 2 ? 1 : 0;
+      const int qi = pq == 0 ? 1 : 2;
+      const float apq = A[pi * 3 + qi];
+      if (fabsf(apq) > 1e-30f) {
+        const float theta = (A[qi * 3 + qi] - A[pi * 3 + pi]) / (2.0f * apq);
+        const float t = (theta >= 0.0f ? 1.0f : -1.0f) / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
+        const float cs = 1.0f / sqrtf(t * t + 1.0f);
+        const float sn = t * cs;
+        // A <- J^T A J with J the (p,q) rotation
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const float arp = A[r * 3 + pi], arq = A[r * 3 + qi];
+          A[r * 3 + pi] = cs * arp - sn * arq;
+          A[r * 3 + qi] = sn * arp + cs * arq;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float apc = A[pi * 3 + c], aqc = A[qi * 3 + c];
+          A[pi * 3 + c] = cs * apc - sn * aqc;
+          A[qi * 3 + c] = sn * apc + cs * aqc;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const float vrp = V[r * 3 + pi], vrq = V[r * 3 + qi];
+          V[r * 3 + pi] = cs * vrp - sn * vrq;
+          V[r * 3 + qi] = sn * vrp + cs * vrq;
+        }
+      }
+    }
+  }
+  // order eigenpairs by decreasing eigenvalue
+  float lam[3] = {A[0], A[4], A[8]};
+  int o0 = 0, o1 = 1, o2 = 2;
+  if (lam[o0] < lam[o1]) { int t = o0; o0 = o1; o1 = t; }
+  if (lam[o1] < lam[o2]) { int t = o1; o1 = o2; o2 = t; }
+  if (lam[o0] < lam[o1]) { int t = o0; o0 = o1; o1 = t; }
+  float v0[3] = {V[0 * 3 + o0], V[1 * 3 + o0], V[2 * 3 + o0]};
+  float v1[3] = {V[0 * 3 + o1], V[1 * 3 + o1], V[2 * 3 + o1]};
+  float v2[3] = {V[0 * 3 + o2], V[1 * 3 + o2], V[2 * 3 + o2]};
+  // det V = +1
+  const float dv = v0[0] * (v1[1] * v2[2] - v1[2] * v2[1]) - v0[1] * (v1[0] * v2[2] - v1[2] * v2[0]) +
+                   v0[2] * (v1[0] * v2[1] - v1[1] * v2[0]);
+  if (dv < 0.0f) { v2[0] = -v2[0]; v2[1] = -v2[1]; v2[2] = -v2[2]; }
+  float u0[3], u1[3], u2[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    u0[r] = Fm[r * 3 + 0] * v0[0] + Fm[r * 3 + 1] * v0[1] + Fm[r * 3 + 2] * v0[2];
+    u1[r] = Fm[r * 3 + 0] * v1[0] + Fm[r * 3 + 1] * v1[1] + Fm[r * 3 + 2] * v1[2];
+  }
+  float n0 = sqrtf(u0[0] * u0[0] + u0[1] * u0[1] + u0[2] * u0[2]);
+  n0 = n0 > 1e-30f ? n0 : 1e-30f;
+  u0[0] /= n0; u0[1] /= n0; u0[2] /= n0;
+  const float d01 = u0[0] * u1[0] + u0[1] * u1[1] + u0[2] * u1[2];
+  u1[0] -= d01 * u0[0]; u1[1] -= d01 * u0[1]; u1[2] -= d01 * u0[2];
+  float n1 = sqrtf(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+  n1 = n1 > 1e-30f ? n1 : 1e-30f;
+  u1[0] /= n1; u1[1] /= n1; u1[2] /= n1;
+  u2[0] = u0[1] * u1[2] - u0[2] * u1[1];
+  u2[1] = u0[2] * u1[0] - u0[0] * u1[2];
+  u2[2] = u0[0] * u1[1] - u0[1] * u1[0];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      R[r * 3 + c] = u0[r] * v0[c] + u1[r] * v1[c] + u2[r] * v2[c];
+}
+
+FSI_HD void load_mat(const float* base, long long P, long long p, float* A) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) A[e] = base[e * P + p];
+}
+FSI_HD void store_mat(float* base, long long P, long long p, const float* A) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) base[e * P + p] = A[e];
+}
+
+FSI_HD bool inside3(int i, int j, int k, int nx, int ny, int nz) {
+  return 0 <= i && i < nx && 0 <= j && j < ny && 0 <= k && k < nz;
+}
+
+// p2g (solid.py:220-249) for particle p; Jp[p] = det F.
+FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p) {
+  const MpmGrid g = make_mgrid(mp);
+  const long long P = mp.n_particles;
+  int base[3];
+  float fx[3], w[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * mp.inv_dx, base[d], fx[d], w[d]);
+  float Fm[9], Cm[9], R[9];
+  load_mat(ms.F, P, p, Fm);
+  load_mat(ms.C, P, p, Cm);
+  rotation_of(Fm, R);
+  const float J = det3(Fm);
+  ms.Jp[p] = J;
+  // F^{-T} = cofactor(F) / det F
+  float FiT[9];
+  FiT[0] = (Fm[4] * Fm[8] - Fm[5] * Fm[7]) / J;
+  FiT[1] = (Fm[5] * Fm[6] - Fm[3] * Fm[8]) / J;
+  FiT[2] = (Fm[3] * Fm[7] - Fm[4] * Fm[6]) / J;
+  FiT[3] = (Fm[2] * Fm[7] - Fm[1] * Fm[8]) / J;
+  FiT[4] = (Fm[0] * Fm[8] - Fm[2] * Fm[6]) / J;
+  FiT[5] = (Fm[1] * Fm[6] - Fm[0] * Fm[7]) / J;
+  FiT[6] = (Fm[1] * Fm[5] - Fm[2] * Fm[4]) / J;
+  FiT[7] = (Fm[2] * Fm[3] - Fm[0] * Fm[5]) / J;
+  FiT[8] = (Fm[0] * Fm[4] - Fm[1] * Fm[3]) / J;
+  float Pk[9];
+  const float lj = mp.la * (J - 1.0f) * J;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Pk[e] = 2.0f * mp.mu * (Fm[e] - R[e]) + lj * FiT[e];
+  const float mass = ms.mass[p];
+  const float sc = -mp.dt * ms.vol0[p];
+  float aff[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      // (P F^T)_{rc}
+      const float pf = Pk[r * 3 + 0] * Fm[c * 3 + 0] + Pk[r * 3 + 1] * Fm[c * 3 + 1] + Pk[r * 3 + 2] * Fm[c * 3 + 2];
+      aff[r * 3 + c] = sc * pf * mp.inv_dx * mp.inv_dx + mass * Cm[r * 3 + c];
+    }
+  const float mvx = mass * ms.v[p], mvy = mass * ms.v[P + p], mvz = mass * ms.v[2 * P + p];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int cc = 0; cc < 3; ++cc) {
+        const int I = base[0] + a, Jn = base[1] + b, K = base[2] + cc;
+        if (!inside3(I, Jn, K, g.nx, g.ny, g.nz)) continue;
+        const float dpx = ((float)a - fx[0]) * mp.dx;
+        const float dpy = ((float)b - fx[1]) * mp.dx;
+        const float dpz = ((float)cc - fx[2]) * mp.dx;
+        const float wt = w[0][a] * w[1][b] * w[2][cc];
+        const float mx = mvx + aff[0] * dpx + aff[1] * dpy + aff[2] * dpz;
+        const float my = mvy + aff[3] * dpx + aff[4] * dpy + aff[5] * dpz;
+        const float mz = mvz + aff[6] * dpx + aff[7] * dpy + aff[8] * dpz;
+        const long long n = ((long long)I * g.ny + Jn) * g.nz + K;
+        atomic_add_f32(&ms.grid_v[n], wt * mx);
+        atomic_add_f32(&ms.grid_v[g.G + n], wt * my);
+        atomic_add_f32(&ms.grid_v[2 * g.G + n], wt * mz);
+        atomic_add_f32(&ms.grid_m[n], wt * mass);
+      }
+}
+
+// grid_update + apply_boundary_at_node (solid.py:252-259, :97-111).
+FSI_HD void mpm_grid_node(const FsiMpmParams& mp, const FsiMpmState& ms, long long n) {
+  const MpmGrid g = make_mgrid(mp);
+  const float m = ms.grid_m[n];
+  if (!(m > 0.0f)) return;
+  const int K = (int)(n % g.nz);
+  const int Jn = (int)((n / g.nz) % g.ny);
+  const int I = (int)(n / ((long long)g.nz * g.ny));
+  float v[3];
+  const int idx[3] = {I, Jn, K};
+  const int ext[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float vd = ms.grid_v[d * g.G + n] / m;
+    vd += mp.dt * mp.gravity[d];
+    vd += mp.dt * ms.grid_f[d * g.G + n] / m;
+    v[d] = vd;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (idx[d] < mp.bound && v[d] < 0.0f) v[d] = 0.0f;
+    if (idx[d] > ext[d] - mp.bound && v[d] > 0.0f) v[d] = 0.0f;
+    ms.grid_v[d * g.G + n] = v[d];
+  }
+}
+
+// g2p + clamp + dynamics controls + fixed pin (solid.py:262-315, :114-124).
+FSI_HD void mpm_g2p_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p) {
+  const MpmGrid g = make_mgrid(mp);
+  const long long P = mp.n_particles;
+  int base[3];
+  float fx[3], w[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * mp.inv_dx, base[d], fx[d], w[d]);
+  float nv[3] = {0.0f, 0.0f, 0.0f};
+  float nC[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int cc = 0; cc < 3; ++cc) {
+        const int I = base[0] + a, Jn = base[1] + b, K = base[2] + cc;
+        if (!inside3(I, Jn, K, g.nx, g.ny, g.nz)) continue;
+        const float dp[3] = {((float)a - fx[0]) * mp.dx, ((float)b - fx[1]) * mp.dx, ((float)cc - fx[2]) * mp.dx};
+        const float wt = w[0][a] * w[1][b] * w[2][cc];
+        const long long n = ((long long)I * g.ny + Jn) * g.nz + K;
+        const float gv[3] = {ms.grid_v[n], ms.grid_v[g.G + n], ms.grid_v[2 * g.G + n]};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          nv[r] += wt * gv[r];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) nC[r * 3 + c] += 4.0f * wt * gv[r] * dp[c] * mp.inv_dx * mp.inv_dx;
+        }
+      }
+  float x[3], v[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    v[d] = nv[d];
+    x[d] = ms.x[d * P + p] + mp.dt * nv[d];
+  }
+  const float eps = mp.dx;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (x[d] < eps) {
+      x[d] = eps;
+      if (v[d] < 0.0f) v[d] = 0.0f;
+    }
+    if (x[d] > mp.domain_max[d] - eps) {
+      x[d] = mp.domain_max[d] - eps;
+      if (v[d] > 0.0f) v[d] = 0.0f;
+    }
+  }
+  float Fm[9], Fn[9];
+  load_mat(ms.F, P, p, Fm);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float a0 = (r == 0 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 0];
+      const float a1 = (r == 1 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 1];
+      const float a2 = (r == 2 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 2];
+      Fn[r * 3 + c] = a0 * Fm[0 * 3 + c] + a1 * Fm[1 * 3 + c] + a2 * Fm[2 * 3 + c];
+    }
+  float Jn = det3(Fn);
+  if (mp.damping_enabled) {
+    float df = 1.0f - mp.velocity_damping * mp.dt;
+    df = df > 0.0f ? df : 0.0f;
+    v[0] *= df; v[1] *= df; v[2] *= df;
+  }
+  if (mp.lock_z_enabled) {
+    x[2] = ms.fixed_x[2 * P + p];
+    v[2] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      nC[2 * 3 + d] = 0.0f;
+      nC[d * 3 + 2] = 0.0f;
+      Fn[2 * 3 + d] = 0.0f;
+      Fn[d * 3 + 2] = 0.0f;
+    }
+    Fn[8] = 1.0f;
+    Jn = det3(Fn);
+  }
+  if (ms.fixed_mask[p] != 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[d] = ms.fixed_x[d * P + p];
+      v[d] = 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      nC[e] = 0.0f;
+      Fn[e] = (e == 0 || e == 4 || e == 8) ? 1.0f : 0.0f;
+    }
+    Jn = 1.0f;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    ms.x[d * P + p] = x[d];
+    ms.v[d * P + p] = v[d];
+  }
+  store_mat(ms.C, P, p, nC);
+  store_mat(ms.F, P, p, Fn);
+  ms.Jp[p] = Jn;
+}
+
+// ---------------------------------------------------------------------------
+// Coupling (coupling/projection.py, penalty.py, moving_boundary.py)
+// ---------------------------------------------------------------------------
+
+// project_particles (projection.py:38-64): scatter phi, mass, mass*velocity.
+FSI_HD void project_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                             const FsiMpmState& ms, const FsiCouplingParams& cp, long long p) {
+  const Grid g = make_grid(lp);
+  const long long P = mp.n_particles;
+  int base[3];
+  float fx[3], w[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * cp.inv_dx_norm, base[d], fx[d], w[d]);
+  float J = ms.Jp[p];
+  J = J > 0.0f ? J : 0.0f;
+  const float vol = ms.vol0[p] * J;
+  const float mass = ms.mass[p];
+  const float vx = ms.v[p] * cp.vel_scale_norm_to_lbm;
+  const float vy = ms.v[P + p] * cp.vel_scale_norm_to_lbm;
+  const float vz = ms.v[2 * P + p] * cp.vel_scale_norm_to_lbm;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int cc = 0; cc < 3; ++cc) {
+        const int I = base[0] + a, Jn = base[1] + b, K = base[2] + cc;
+        if (!inside3(I, Jn, K, g.nx, g.ny, g.nz)) continue;
+        const float wt = w[0][a] * w[1][b] * w[2][cc];
+        const float mc = wt * mass;
+        const float vf = wt * vol / cp.cell_volume_norm;
+        const long long c = cell_of(g, I, Jn, K);
+        atomic_add_f32(&ls.solid_phi[c], vf);
+        atomic_add_f32(&ls.solid_mass[c], mc);
+        atomic_add_f32(&ls.solid_vel[c], mc * vx);
+        atomic_add_f32(&ls.solid_vel[g.N + c], mc * vy);
+        atomic_add_f32(&ls.solid_vel[2 * g.N + c], mc * vz);
+      }
+}
+
+struct ProjOut {
+  float phi_raw, phi_clamped, mass;
+  int active;
+};
+
+// normalize_projection (projection.py:66-91) for cell c.
+FSI_HD ProjOut normalize_cell(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiCouplingParams& cp,
+                              long long c) {
+  const Grid g = make_grid(lp);
+  ProjOut o;
+  o.phi_raw = ls.solid_phi[c];
+  o.mass = ls.solid_mass[c];
+  if (o.mass > cp.eps_mass) {
+    ls.solid_vel[c] = ls.solid_vel[c] / o.mass;
+    ls.solid_vel[g.N + c] = ls.solid_vel[g.N + c] / o.mass;
+    ls.solid_vel[2 * g.N + c] = ls.solid_vel[2 * g.N + c] / o.mass;
+  } else {
+    ls.solid_vel[c] = 0.0f;
+    ls.solid_vel[g.N + c] = 0.0f;
+    ls.solid_vel[2 * g.N + c] = 0.0f;
+  }
+  float pc = o.phi_raw > 0.0f ? o.phi_raw : 0.0f;
+  pc = pc < 1.0f ? pc : 1.0f;
+  ls.solid_phi[c] = pc;
+  o.phi_clamped = pc;
+  o.active = pc > 1.0e-8f ? 1 : 0;
+  return o;
+}
+
+// build_penalty_force (penalty.py:59-88) for cell c; returns the force.
+FSI_HD void penalty_cell(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiCouplingParams& cp,
+                         long long c, float* force, int& active) {
+  const Grid g = make_grid(lp);
+  force[0] = force[1] = force[2] = 0.0f;
+  active = 0;
+  if (ls.solid[c] == 0) {
+    const float phi = ls.solid_phi[c];
+    if (phi > cp.penalty_phi_min) {
+      const float k = cp.beta_lbm * phi * ls.rho[c];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) force[d] = k * (ls.solid_vel[d * g.N + c] - ls.vel[d * g.N + c]);
+      const float n = norm3(force[0], force[1], force[2]);
+      if (n > cp.force_cap_lbm) {
+        const float s = cp.force_cap_lbm / n;
+        force[0] *= s; force[1] *= s; force[2] *= s;
+      }
+      active = 1;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    ls.cell_force[d * g.N + c] = force[d];
+    ls.hydro_force[d * g.N + c] = -force[d];
+  }
+}
+
+struct ReactOut {
+  int active;
+  float particle_norm;
+  float max_contrib;
+  float f[3];
+};
+
+// Reaction transfer for particle p: mode 1 = penalty (penalty.py:90-129),
+// mode 2 = moving boundary (moving_boundary.py:58-111).
+FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                                  const FsiMpmState& ms, const FsiCouplingParams& cp, int mode,
+                                  long long p) {
+  const Grid g = make_grid(lp);
+  const MpmGrid mg = make_mgrid(mp);
+  const long long P = mp.n_particles;
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = 0.0f;
+  o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  int base[3];
+  float fx[3], w[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * cp.inv_dx_norm, base[d], fx[d], w[d]);
+  float hx = 0.0f, hy = 0.0f, hz = 0.0f, sphi = 0.0f;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int cc = 0; cc < 3; ++cc) {
+        const int I = base[0] + a, Jn = base[1] + b, K = base[2] + cc;
+        if (!inside3(I, Jn, K, g.nx, g.ny, g.nz)) continue;
+        const float wt = w[0][a] * w[1][b] * w[2][cc];
+        const long long c = cell_of(g, I, Jn, K);
+        hx += wt * ls.hydro_force[c];
+        hy += wt * ls.hydro_force[g.N + c];
+        hz += wt * ls.hydro_force[2 * g.N + c];
+        if (mode == 2) sphi += wt * ls.solid_phi[c];
+      }
+  if (mode == 2 && !(sphi > cp.mb_phi_min)) return o;
+  float J = ms.Jp[p];
+  J = J > 0.0f ? J : 0.0f;
+  const float vol = ms.vol0[p] * J;
+  float pf[3] = {hx * cp.force_density_scale * vol * cp.reaction_scale,
+                 hy * cp.force_density_scale * vol * cp.reaction_scale,
+                 hz * cp.force_density_scale * vol * cp.reaction_scale};
+  if (mode == 2) {
+    float n = norm3(pf[0], pf[1], pf[2]);
+    if (n > cp.mb_force_cap_norm) {
+      const float s = cp.mb_force_cap_norm / n;
+      pf[0] *= s; pf[1] *= s; pf[2] *= s;
+      n = cp.mb_force_cap_norm;
+    }
+    o.particle_norm = n;
+    o.active = n > 0.0f ? 1 : 0;
+  }
+  o.f[0] = pf[0];
+  o.f[1] = pf[1];
+  o.f[2] = pf[2];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int cc = 0; cc < 3; ++cc) {
+        const int I = base[0] + a, Jn = base[1] + b, K = base[2] + cc;
+        if (!inside3(I, Jn, K, mg.nx, mg.ny, mg.nz)) continue;
+        const float wt = w[0][a] * w[1][b] * w[2][cc];
+        const long long n = ((long long)I * mg.ny + Jn) * mg.nz + K;
+        const float cx = wt * pf[0], cy = wt * pf[1], cz = wt * pf[2];
+        atomic_add_f32(&ms.grid_f[n], cx);
+        atomic_add_f32(&ms.grid_f[mg.G + n], cy);
+        atomic_add_f32(&ms.grid_f[2 * mg.G + n], cz);
+        const float cn = norm3(cx, cy, cz);
+        if (cn > o.max_contrib) o.max_contrib = cn;
+      }
+  return o;
+}
+
+
+// ---------------------------------------------------------------------------
+// Diagnostics reductions.  The stats structs are arrays of doubles; each field
+// has a combine op.  Per-cell / per-particle contributions are defined here so
+// the GPU two-pass reduction and the serial emulation agree on semantics.
+// ---------------------------------------------------------------------------
+enum RedOp { RED_SUM = 0, RED_MIN = 1, RED_MAX = 2 };
+constexpr int LBM_STATS_N = (int)(sizeof(FsiLbmStats) / sizeof(double));
+constexpr int MPM_STATS_N = (int)(sizeof(FsiMpmStats) / sizeof(double));
+
+FSI_HD int lbm_stats_op(int f) {
+  // fluid_count, rho_min, rho_max, mass_total, max_v, force_norm_max, fluid_sum_v[3],
+  // phi_weight_sum, phi_weighted_v[3], far_count, far_sum_v[3], max_cell_force_norm_all,
+  // max_hydro_force_norm_all, net_cell_force[3], net_hydro_force[3], mass_rho_all
+  return f == 1 ? RED_MIN : ((f == 2 || f == 4 || f == 5 || f == 17 || f == 18) ? RED_MAX : RED_SUM);
+}
+FSI_HD int mpm_stats_op(int f) {
+  // min_x[3], max_x[3], max_speed, min_J, max_J, total_mass, mass_weighted_v[3]
+  return (f <= 2 || f == 7) ? RED_MIN : ((f <= 6 || f == 8) ? RED_MAX : RED_SUM);
+}
+FSI_HD double red_identity(int op) {
+  return op == RED_SUM ? 0.0 : (op == RED_MIN ? 1.0e300 : -1.0e300);
+}
+FSI_HD double red_combine(int op, double a, double b) {
+  return op == RED_SUM ? a + b : (op == RED_MIN ? (a < b ? a : b) : (a > b ? a : b));
+}
+
+// One lattice cell's contribution (fluid.py:1931-1968; fsi_diagnostics.py:8-97).
+FSI_HD void lbm_stats_cell(const FsiLbmParams& p, const FsiLbmState& st, long long c, double* a) {
+  const Grid g = make_grid(p);
+  const bool fluid = st.solid[c] == 0;
+  const float vx = st.vel[c], vy = st.vel[g.N + c], vz = st.vel[2 * g.N + c];
+  const float cfx = st.cell_force[c], cfy = st.cell_force[g.N + c], cfz = st.cell_force[2 * g.N + c];
+  const float hfx = st.hydro_force[c], hfy = st.hydro_force[g.N + c], hfz = st.hydro_force[2 * g.N + c];
+  const float cfn = norm3(cfx, cfy, cfz);
+  const float phi = st.solid_phi[c];
+  if (fluid) {
+    const double r = st.rho[c];
+    a[0] += 1.0;
+    a[1] = r < a[1] ? r : a[1];
+    a[2] = r > a[2] ? r : a[2];
+    a[3] += r;
+    const double vn = norm3(vx, vy, vz);
+    a[4] = vn > a[4] ? vn : a[4];
+    a[5] = (double)cfn > a[5] ? (double)cfn : a[5];
+    a[6] += vx;
+    a[7] += vy;
+    a[8] += vz;
+    if (phi <= 1.0e-6f) {
+      a[13] += 1.0;
+      a[14] += vx;
+      a[15] += vy;
+      a[16] += vz;
+    }
+  }
+  if (phi > 1.0e-6f) {
+    a[9] += phi;
+    a[10] += (double)phi * vx;
+    a[11] += (double)phi * vy;
+    a[12] += (double)phi * vz;
+  }
+  a[17] = (double)cfn > a[17] ? (double)cfn : a[17];
+  const double hn = norm3(hfx, hfy, hfz);
+  a[18] = hn > a[18] ? hn : a[18];
+  a[19] += cfx;
+  a[20] += cfy;
+  a[21] += cfz;
+  a[22] += hfx;
+  a[23] += hfy;
+  a[24] += hfz;
+  a[25] += st.rho[c];
+}
+
+// One particle's contribution (solid.py:342-357; fsi_diagnostics.py:117-124).
+FSI_HD void mpm_stats_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p, double* a) {
+  const long long P = mp.n_particles;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double x = ms.x[d * P + p];
+    a[d] = x < a[d] ? x : a[d];
+    a[3 + d] = x > a[3 + d] ? x : a[3 + d];
+  }
+  const float vx = ms.v[p], vy = ms.v[P + p], vz = ms.v[2 * P + p];
+  const double sp = norm3(vx, vy, vz);
+  a[6] = sp > a[6] ? sp : a[6];
+  const double J = ms.Jp[p];
+  a[7] = J < a[7] ? J : a[7];
+  a[8] = J > a[8] ? J : a[8];
+  const double m = ms.mass[p];
+  a[9] += m;
+  a[10] += m * vx;
+  a[11] += m * vy;
+  a[12] += m * vz;
+}
+
+}  // namespace fsi

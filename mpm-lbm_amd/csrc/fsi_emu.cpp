@@ -1,0 +1,193 @@
+// fsi_emu.cpp - serial CPU emulation of libfsi_hip.so, for tests only.
+//
+// Exports the same C ABI as fsi_kernels.hip (include/fsi_hip.h) and runs the
+// very same per-cell / per-particle bodies of fsi_core.h in serial loops over
+// host memory.  The GPU library is the product; this build lets the CPU test
+// suite check the kernels' arithmetic, indexing and boundary handling against
+// the oracle on a machine without a GPU.  Built into libfsi_emu.so, which no
+// product code loads.
+#include <string.h>
+
+#include <string>
+
+#include "fsi_core.h"
+
+using namespace fsi;
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const char* m) {
+  g_err = m;
+  return code;
+}
+long long ncells(const FsiLbmParams* p) { return (long long)p->nx * p->ny * p->nz; }
+}  // namespace
+
+extern "C" {
+
+const char* fsi_last_error(void) { return g_err.c_str(); }
+int fsi_version(void) { return 1; }
+
+int fsi_abi_sizes(int32_t* out, int n) {
+  const int32_t sz[9] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
+                         (int32_t)sizeof(FsiLbmCounters),    (int32_t)sizeof(FsiLbmStats),
+                         (int32_t)sizeof(FsiMpmParams),      (int32_t)sizeof(FsiMpmState),
+                         (int32_t)sizeof(FsiCouplingParams), (int32_t)sizeof(FsiCouplingCounters),
+                         (int32_t)sizeof(FsiMpmStats)};
+  int k = 0;
+  for (; k < n && k < 9; ++k) out[k] = sz[k];
+  return k;
+}
+int fsi_sync(void*) { return FSI_OK; }
+
+int fsi_lbm_init(const FsiLbmParams* p, const FsiLbmState* st, void*) {
+  for (long long c = 0; c < ncells(p); ++c) lbm_init_cell(*p, *st, c);
+  return FSI_OK;
+}
+
+int fsi_lbm_rebuild(const FsiLbmParams* p, const FsiLbmState* st, int dynamic, int cur, void*) {
+  if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
+  // The GPU evaluates every cell from the pre-pass flags; emulate that by
+  // snapshotting `solid` (rebuild reads neighbours' new flags from
+  // static_solid / solid_phi only, and its own previous flag).
+  for (long long c = 0; c < ncells(p); ++c) lbm_rebuild_cell(*p, *st, dynamic, cur, c);
+  return FSI_OK;
+}
+
+int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity, uint32_t flags, void*) {
+  if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
+  for (long long c = 0; c < ncells(p); ++c) lbm_substep_cell(*p, *st, cur, parity, flags, c);
+  return FSI_OK;
+}
+
+int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags, void*) {
+  for (long long c = 0; c < ncells(p); ++c) lbm_moments_cell(*p, *st, cur, flags, c);
+  return FSI_OK;
+}
+
+int fsi_lbm_mb_begin(const FsiLbmParams* p, const FsiLbmState* st, void*) {
+  memset(st->counters, 0, sizeof(FsiLbmCounters));
+  for (long long c = 0; c < ncells(p); ++c) lbm_mb_begin_cell(*p, *st, c);
+  return FSI_OK;
+}
+
+int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int parity, int n_samples, int subcycled, void*) {
+  FsiLbmCounters* k = st->counters;
+  k->bb_link_count = 0;
+  k->bb_max_correction_bits = 0;
+  for (int d = 0; d < 3; ++d) k->bb_net_solid_force[d] = 0.0;
+  for (long long c = 0; c < ncells(p); ++c) {
+    FinalizeOut o = lbm_finalize_cell(*p, *st, parity, n_samples, subcycled, c);
+    k->bb_link_count += o.links;
+    atomic_max_nonneg(&k->bb_max_correction_bits, o.max_corr);
+    if (subcycled) {
+      atomic_max_nonneg(&k->mean_norm_max_bits, o.mean_norm);
+      atomic_max_nonneg(&k->accum_norm_max_bits, o.runmax);
+    }
+    k->bb_net_solid_force[0] += o.hx;
+    k->bb_net_solid_force[1] += o.hy;
+    k->bb_net_solid_force[2] += o.hz;
+  }
+  if (subcycled) k->mb_sample_count = n_samples;
+  return FSI_OK;
+}
+
+int fsi_lbm_reduce_stats(const FsiLbmParams* p, const FsiLbmState* st, FsiLbmStats*, FsiLbmStats* out, void*) {
+  double* a = reinterpret_cast<double*>(out);
+  for (int f = 0; f < LBM_STATS_N; ++f) a[f] = red_identity(lbm_stats_op(f));
+  for (long long c = 0; c < ncells(p); ++c) lbm_stats_cell(*p, *st, c, a);
+  return FSI_OK;
+}
+
+int fsi_project(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp, const FsiMpmState* ms,
+                const FsiCouplingParams* cp, FsiCouplingCounters* cc, void*) {
+  const long long N = ncells(lp);
+  for (long long c = 0; c < N; ++c) {
+    ls->solid_phi[c] = 0.0f;
+    ls->solid_mass[c] = 0.0f;
+    ls->solid_vel[c] = ls->solid_vel[N + c] = ls->solid_vel[2 * N + c] = 0.0f;
+  }
+  cc->projected_mass = cc->projected_volume_raw = cc->projected_volume_clamped = 0.0;
+  cc->max_phi_raw_bits = 0;
+  cc->active_cell_count = 0;
+  for (long long q = 0; q < mp->n_particles; ++q) project_particle(*lp, *ls, *mp, *ms, *cp, q);
+  for (long long c = 0; c < N; ++c) {
+    ProjOut o = normalize_cell(*lp, *ls, *cp, c);
+    atomic_max_nonneg(&cc->max_phi_raw_bits, o.phi_raw > 0.0f ? o.phi_raw : 0.0f);
+    cc->active_cell_count += o.active;
+    cc->projected_mass += o.mass;
+    cc->projected_volume_raw += (double)o.phi_raw * cp->cell_volume_norm;
+    cc->projected_volume_clamped += (double)o.phi_clamped * cp->cell_volume_norm;
+  }
+  return FSI_OK;
+}
+
+int fsi_penalty_force(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiCouplingParams* cp,
+                      FsiCouplingCounters* cc, void*) {
+  cc->active_force_cell_count = 0;
+  cc->max_cell_force_bits = cc->max_hydro_force_bits = 0;
+  for (int d = 0; d < 3; ++d) cc->net_cell_force[d] = cc->net_hydro_force[d] = 0.0;
+  for (long long c = 0; c < ncells(lp); ++c) {
+    float f[3];
+    int act;
+    penalty_cell(*lp, *ls, *cp, c, f, act);
+    cc->active_force_cell_count += act;
+    const float n = norm3(f[0], f[1], f[2]);
+    atomic_max_nonneg(&cc->max_cell_force_bits, n);
+    atomic_max_nonneg(&cc->max_hydro_force_bits, n);
+    for (int d = 0; d < 3; ++d) {
+      cc->net_cell_force[d] += f[d];
+      cc->net_hydro_force[d] -= f[d];
+    }
+  }
+  return FSI_OK;
+}
+
+int fsi_mpm_clear_grid(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
+  const long long G = (long long)mp->gnx * mp->gny * mp->gnz;
+  memset(ms->grid_v, 0, sizeof(float) * 3 * G);
+  memset(ms->grid_m, 0, sizeof(float) * G);
+  memset(ms->grid_f, 0, sizeof(float) * 3 * G);
+  return FSI_OK;
+}
+
+int fsi_mpm_p2g(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
+  for (long long q = 0; q < mp->n_particles; ++q) mpm_p2g_particle(*mp, *ms, q);
+  return FSI_OK;
+}
+
+int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp, const FsiMpmState* ms,
+                     const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode, void*) {
+  if (mode != 1 && mode != 2) return fail(FSI_E_INVALID, "reaction mode must be 1 (penalty) or 2 (moving boundary)");
+  cc->active_reaction_particle_count = 0;
+  cc->max_particle_reaction_bits = cc->max_grid_reaction_bits = 0;
+  for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] = cc->net_grid_reaction[d] = 0.0;
+  for (long long q = 0; q < mp->n_particles; ++q) {
+    ReactOut o = reaction_particle(*lp, *ls, *mp, *ms, *cp, mode, q);
+    cc->active_reaction_particle_count += o.active;
+    atomic_max_nonneg(&cc->max_particle_reaction_bits, o.particle_norm);
+    atomic_max_nonneg(&cc->max_grid_reaction_bits, o.max_contrib);
+    for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] += o.f[d];
+  }
+  return FSI_OK;
+}
+
+int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
+  const long long G = (long long)mp->gnx * mp->gny * mp->gnz;
+  for (long long n = 0; n < G; ++n) mpm_grid_node(*mp, *ms, n);
+  return FSI_OK;
+}
+
+int fsi_mpm_g2p(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
+  for (long long q = 0; q < mp->n_particles; ++q) mpm_g2p_particle(*mp, *ms, q);
+  return FSI_OK;
+}
+
+int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmStats*, FsiMpmStats* out, void*) {
+  double* a = reinterpret_cast<double*>(out);
+  for (int f = 0; f < MPM_STATS_N; ++f) a[f] = red_identity(mpm_stats_op(f));
+  for (long long q = 0; q < mp->n_particles; ++q) mpm_stats_particle(*mp, *ms, q, a);
+  return FSI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,625 @@
+// fsi_kernels.hip - gfx950 kernels and the extern "C" entry points of
+// libfsi_hip.so (declared in include/fsi_hip.h).
+//
+// Launch shapes: one work-item per lattice cell / MPM node / particle, 256-wide
+// workgroups (4 wave64s).  The fused LBM substep keeps a cell's 19 populations
+// in VGPRs between the coalesced SoA load of F[cur] and the pushes into
+// F[1-cur]; consecutive work-items own consecutive k, so each of the 19 push
+// streams is a contiguous (shifted) run of addresses.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+
+#include "fsi_core.h"
+
+using namespace fsi;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(FSI_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return FSI_OK;
+}
+
+constexpr int TPB = 256;
+
+inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_lbm(const FsiLbmParams* p, const FsiLbmState* st) {
+  if (!p || !st) return fail(FSI_E_INVALID, "null params/state");
+  if (p->nx < 3 || p->ny < 3 || p->nz < 3)
+    return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
+  if (p->bc_x_left < 0 || p->bc_x_left > 2 || p->bc_x_right < 0 || p->bc_x_right > 2)
+    return fail(FSI_E_INVALID, "bc_x_left/bc_x_right must be 0, 1 or 2");
+  if (!st->F[0] || !st->F[1] || !st->rho || !st->vel || !st->cellinfo || !st->solid)
+    return fail(FSI_E_INVALID, "null LBM buffer");
+  return FSI_OK;
+}
+
+int check_mpm(const FsiMpmParams* mp, const FsiMpmState* ms) {
+  if (!mp || !ms) return fail(FSI_E_INVALID, "null MPM params/state");
+  if (mp->gnx < 1 || mp->gny < 1 || mp->gnz < 1) return fail(FSI_E_INVALID, "empty MPM grid");
+  if (mp->n_particles < 0) return fail(FSI_E_INVALID, "negative particle count");
+  if (!ms->x || !ms->v || !ms->C || !ms->F || !ms->Jp || !ms->mass || !ms->vol0 || !ms->grid_v ||
+      !ms->grid_m || !ms->grid_f)
+    return fail(FSI_E_INVALID, "null MPM buffer");
+  return FSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions (wave64 shuffles, then one LDS slot per wave)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_reduce(double v, int op) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = red_combine(op, v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Result valid in thread 0.
+__device__ double block_reduce(double v, int op, double* smem) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_reduce(v, op);
+  __syncthreads();
+  if (lane == 0) smem[wave] = v;
+  __syncthreads();
+  double r = red_identity(op);
+  if (threadIdx.x == 0)
+    for (int w = 0; w < TPB / 64; ++w) r = red_combine(op, r, smem[w]);
+  return r;
+}
+
+__device__ __forceinline__ float block_max_f(float v, float* smem) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) smem[wave] = v;
+  __syncthreads();
+  float r = 0.0f;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < TPB / 64; ++w) r = smem[w] > r ? smem[w] : r;
+  return r;
+}
+
+__device__ __forceinline__ int block_sum_i(int v, int* smem) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) smem[wave] = v;
+  __syncthreads();
+  int r = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < TPB / 64; ++w) r += smem[w];
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// LBM kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_lbm_init(FsiLbmParams p, FsiLbmState st) {
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (c < (long long)p.nx * p.ny * p.nz) lbm_init_cell(p, st, c);
+}
+
+__global__ void __launch_bounds__(TPB) k_lbm_rebuild(FsiLbmParams p, FsiLbmState st, int dynamic, int cur) {
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (c < (long long)p.nx * p.ny * p.nz) lbm_rebuild_cell(p, st, dynamic, cur, c);
+}
+
+template <uint32_t FIXED>
+__global__ void __launch_bounds__(TPB) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
+                                                     uint32_t flags) {
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (c < (long long)p.nx * p.ny * p.nz)
+    lbm_substep_cell(p, st, cur, parity, (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED, c);
+}
+
+__global__ void __launch_bounds__(TPB) k_lbm_moments(FsiLbmParams p, FsiLbmState st, int cur, uint32_t flags) {
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (c < (long long)p.nx * p.ny * p.nz) lbm_moments_cell(p, st, cur, flags, c);
+}
+
+__global__ void __launch_bounds__(TPB) k_lbm_mb_begin(FsiLbmParams p, FsiLbmState st) {
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (c == 0) {
+    st.counters->bb_link_count = 0;
+    st.counters->bb_max_correction_bits = 0;
+    st.counters->mb_sample_count = 0;
+    st.counters->accum_norm_max_bits = 0;
+    st.counters->mean_norm_max_bits = 0;
+    st.counters->bb_net_solid_force[0] = 0.0;
+    st.counters->bb_net_solid_force[1] = 0.0;
+    st.counters->bb_net_solid_force[2] = 0.0;
+  }
+  if (c < (long long)p.nx * p.ny * p.nz) lbm_mb_begin_cell(p, st, c);
+}
+
+__global__ void __launch_bounds__(TPB) k_lbm_finalize_clear(FsiLbmState st) {
+  if (threadIdx.x == 0) {
+    st.counters->bb_link_count = 0;
+    st.counters->bb_max_correction_bits = 0;
+    st.counters->bb_net_solid_force[0] = 0.0;
+    st.counters->bb_net_solid_force[1] = 0.0;
+    st.counters->bb_net_solid_force[2] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(TPB) k_lbm_finalize(FsiLbmParams p, FsiLbmState st, int parity, int n_samples,
+                                                      int subcycled) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  __shared__ int si[TPB / 64];
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  FinalizeOut o;
+  o.links = 0;
+  o.max_corr = 0.0f;
+  o.mean_norm = 0.0f;
+  o.runmax = 0.0f;
+  o.hx = o.hy = o.hz = 0.0;
+  if (c < (long long)p.nx * p.ny * p.nz) o = lbm_finalize_cell(p, st, parity, n_samples, subcycled, c);
+  const int links = block_sum_i(o.links, si);
+  const float mc = block_max_f(o.max_corr, sf);
+  const float mn = block_max_f(o.mean_norm, sf);
+  const float rm = block_max_f(o.runmax, sf);
+  const double hx = block_reduce(o.hx, RED_SUM, sd);
+  const double hy = block_reduce(o.hy, RED_SUM, sd);
+  const double hz = block_reduce(o.hz, RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    FsiLbmCounters* k = st.counters;
+    if (links) atomicAdd(&k->bb_link_count, links);
+    atomicMax(&k->bb_max_correction_bits, __float_as_uint(mc));
+    if (subcycled) {
+      atomicMax(&k->mean_norm_max_bits, __float_as_uint(mn));
+      atomicMax(&k->accum_norm_max_bits, __float_as_uint(rm));
+    }
+    if (hx != 0.0) atomicAdd(&k->bb_net_solid_force[0], hx);
+    if (hy != 0.0) atomicAdd(&k->bb_net_solid_force[1], hy);
+    if (hz != 0.0) atomicAdd(&k->bb_net_solid_force[2], hz);
+    if (blockIdx.x == 0 && subcycled) k->mb_sample_count = n_samples;
+  }
+}
+
+__global__ void __launch_bounds__(TPB) k_lbm_stats_partial(FsiLbmParams p, FsiLbmState st, FsiLbmStats* part) {
+  __shared__ double sd[TPB / 64];
+  double a[LBM_STATS_N];
+#pragma unroll
+  for (int f = 0; f < LBM_STATS_N; ++f) a[f] = red_identity(lbm_stats_op(f));
+  const long long N = (long long)p.nx * p.ny * p.nz;
+  for (long long c = (long long)blockIdx.x * TPB + threadIdx.x; c < N; c += (long long)gridDim.x * TPB)
+    lbm_stats_cell(p, st, c, a);
+  double* out = reinterpret_cast<double*>(part + blockIdx.x);
+  for (int f = 0; f < LBM_STATS_N; ++f) {
+    const double r = block_reduce(a[f], lbm_stats_op(f), sd);
+    if (threadIdx.x == 0) out[f] = r;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_stats_final(const double* part, int nparts, int nfields, int which,
+                                                    double* out) {
+  // one work-item per field, fixed order over the partials: deterministic
+  const int f = threadIdx.x;
+  if (f >= nfields) return;
+  const int op = which == 0 ? lbm_stats_op(f) : mpm_stats_op(f);
+  double r = red_identity(op);
+  for (int b = 0; b < nparts; ++b) r = red_combine(op, r, part[(long long)b * nfields + f]);
+  out[f] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Projection / penalty
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_proj_clear(FsiLbmParams p, FsiLbmState st, FsiCouplingCounters* cc) {
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  const long long N = (long long)p.nx * p.ny * p.nz;
+  if (c == 0) {
+    cc->projected_mass = 0.0;
+    cc->projected_volume_raw = 0.0;
+    cc->projected_volume_clamped = 0.0;
+    cc->max_phi_raw_bits = 0;
+    cc->active_cell_count = 0;
+  }
+  if (c < N) {
+    st.solid_phi[c] = 0.0f;
+    st.solid_mass[c] = 0.0f;
+    st.solid_vel[c] = 0.0f;
+    st.solid_vel[N + c] = 0.0f;
+    st.solid_vel[2 * N + c] = 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(TPB) k_proj_particles(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+                                                        FsiMpmState ms, FsiCouplingParams cp) {
+  const long long q = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (q < mp.n_particles) project_particle(lp, ls, mp, ms, cp, q);
+}
+
+__global__ void __launch_bounds__(TPB) k_proj_normalize(FsiLbmParams lp, FsiLbmState ls, FsiCouplingParams cp,
+                                                        FsiCouplingCounters* cc) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  __shared__ int si[TPB / 64];
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  ProjOut o;
+  o.phi_raw = 0.0f;
+  o.phi_clamped = 0.0f;
+  o.mass = 0.0f;
+  o.active = 0;
+  if (c < (long long)lp.nx * lp.ny * lp.nz) o = normalize_cell(lp, ls, cp, c);
+  const float mp = block_max_f(o.phi_raw > 0.0f ? o.phi_raw : 0.0f, sf);
+  const int act = block_sum_i(o.active, si);
+  const double m = block_reduce((double)o.mass, RED_SUM, sd);
+  const double vr = block_reduce((double)o.phi_raw * cp.cell_volume_norm, RED_SUM, sd);
+  const double vc = block_reduce((double)o.phi_clamped * cp.cell_volume_norm, RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    atomicMax(&cc->max_phi_raw_bits, __float_as_uint(mp));
+    if (act) atomicAdd(&cc->active_cell_count, act);
+    if (m != 0.0) atomicAdd(&cc->projected_mass, m);
+    if (vr != 0.0) atomicAdd(&cc->projected_volume_raw, vr);
+    if (vc != 0.0) atomicAdd(&cc->projected_volume_clamped, vc);
+  }
+}
+
+__global__ void k_penalty_clear(FsiCouplingCounters* cc) {
+  if (threadIdx.x == 0) {
+    cc->active_force_cell_count = 0;
+    cc->max_cell_force_bits = 0;
+    cc->max_hydro_force_bits = 0;
+    for (int d = 0; d < 3; ++d) {
+      cc->net_cell_force[d] = 0.0;
+      cc->net_hydro_force[d] = 0.0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(TPB) k_penalty(FsiLbmParams lp, FsiLbmState ls, FsiCouplingParams cp,
+                                                 FsiCouplingCounters* cc) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  __shared__ int si[TPB / 64];
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  float f[3] = {0.0f, 0.0f, 0.0f};
+  int active = 0;
+  if (c < (long long)lp.nx * lp.ny * lp.nz) penalty_cell(lp, ls, cp, c, f, active);
+  const float fn = norm3(f[0], f[1], f[2]);
+  const float mfn = block_max_f(fn, sf);
+  const int act = block_sum_i(active, si);
+  double net[3];
+  for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)f[d], RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    if (act) atomicAdd(&cc->active_force_cell_count, act);
+    atomicMax(&cc->max_cell_force_bits, __float_as_uint(mfn));
+    atomicMax(&cc->max_hydro_force_bits, __float_as_uint(mfn));
+    for (int d = 0; d < 3; ++d)
+      if (net[d] != 0.0) {
+        atomicAdd(&cc->net_cell_force[d], net[d]);
+        atomicAdd(&cc->net_hydro_force[d], -net[d]);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MPM kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_mpm_clear(FsiMpmParams mp, FsiMpmState ms) {
+  const long long n = (long long)blockIdx.x * TPB + threadIdx.x;
+  const long long G = (long long)mp.gnx * mp.gny * mp.gnz;
+  if (n < G) {
+    ms.grid_v[n] = 0.0f;
+    ms.grid_v[G + n] = 0.0f;
+    ms.grid_v[2 * G + n] = 0.0f;
+    ms.grid_m[n] = 0.0f;
+    ms.grid_f[n] = 0.0f;
+    ms.grid_f[G + n] = 0.0f;
+    ms.grid_f[2 * G + n] = 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(TPB) k_mpm_p2g(FsiMpmParams mp, FsiMpmState ms) {
+  const long long q = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (q < mp.n_particles) mpm_p2g_particle(mp, ms, q);
+}
+
+__global__ void k_reaction_clear(FsiCouplingCounters* cc) {
+  if (threadIdx.x == 0) {
+    cc->active_reaction_particle_count = 0;
+    cc->max_particle_reaction_bits = 0;
+    cc->max_grid_reaction_bits = 0;
+    for (int d = 0; d < 3; ++d) {
+      cc->net_particle_reaction[d] = 0.0;
+      cc->net_grid_reaction[d] = 0.0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(TPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+                                                      FsiMpmState ms, FsiCouplingParams cp,
+                                                      FsiCouplingCounters* cc, int mode) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  __shared__ int si[TPB / 64];
+  const long long q = (long long)blockIdx.x * TPB + threadIdx.x;
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = 0.0f;
+  o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  if (q < mp.n_particles) o = reaction_particle(lp, ls, mp, ms, cp, mode, q);
+  const int act = block_sum_i(o.active, si);
+  const float mpn = block_max_f(o.particle_norm, sf);
+  const float mgc = block_max_f(o.max_contrib, sf);
+  double net[3];
+  for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)o.f[d], RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    if (act) atomicAdd(&cc->active_reaction_particle_count, act);
+    atomicMax(&cc->max_particle_reaction_bits, __float_as_uint(mpn));
+    atomicMax(&cc->max_grid_reaction_bits, __float_as_uint(mgc));
+    for (int d = 0; d < 3; ++d)
+      if (net[d] != 0.0) atomicAdd(&cc->net_particle_reaction[d], net[d]);
+  }
+}
+
+__global__ void __launch_bounds__(TPB) k_mpm_grid_update(FsiMpmParams mp, FsiMpmState ms) {
+  const long long n = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (n < (long long)mp.gnx * mp.gny * mp.gnz) mpm_grid_node(mp, ms, n);
+}
+
+__global__ void __launch_bounds__(TPB) k_mpm_g2p(FsiMpmParams mp, FsiMpmState ms) {
+  const long long q = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (q < mp.n_particles) mpm_g2p_particle(mp, ms, q);
+}
+
+__global__ void __launch_bounds__(TPB) k_mpm_stats_partial(FsiMpmParams mp, FsiMpmState ms, FsiMpmStats* part) {
+  __shared__ double sd[TPB / 64];
+  double a[MPM_STATS_N];
+#pragma unroll
+  for (int f = 0; f < MPM_STATS_N; ++f) a[f] = red_identity(mpm_stats_op(f));
+  for (long long q = (long long)blockIdx.x * TPB + threadIdx.x; q < mp.n_particles; q += (long long)gridDim.x * TPB)
+    mpm_stats_particle(mp, ms, q, a);
+  double* out = reinterpret_cast<double*>(part + blockIdx.x);
+  for (int f = 0; f < MPM_STATS_N; ++f) {
+    const double r = block_reduce(a[f], mpm_stats_op(f), sd);
+    if (threadIdx.x == 0) out[f] = r;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* fsi_last_error(void) { return g_last_error.c_str(); }
+
+int fsi_version(void) { return 1; }
+
+int fsi_abi_sizes(int32_t* out, int n) {
+  const int32_t sz[9] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
+                         (int32_t)sizeof(FsiLbmCounters),    (int32_t)sizeof(FsiLbmStats),
+                         (int32_t)sizeof(FsiMpmParams),      (int32_t)sizeof(FsiMpmState),
+                         (int32_t)sizeof(FsiCouplingParams), (int32_t)sizeof(FsiCouplingCounters),
+                         (int32_t)sizeof(FsiMpmStats)};
+  int k = 0;
+  for (; k < n && k < 9; ++k) out[k] = sz[k];
+  return k;
+}
+
+int fsi_sync(void* stream) {
+  hipError_t e = hipStreamSynchronize(as_stream(stream));
+  if (e != hipSuccess) return fail(FSI_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  return FSI_OK;
+}
+
+int fsi_lbm_init(const FsiLbmParams* p, const FsiLbmState* st, void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  const long long N = (long long)p->nx * p->ny * p->nz;
+  hipLaunchKernelGGL(k_lbm_init, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st);
+  return check_launch("k_lbm_init");
+}
+
+int fsi_lbm_rebuild(const FsiLbmParams* p, const FsiLbmState* st, int dynamic, int cur, void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if (cur != 0 && cur != 1) return fail(FSI_E_INVALID, "cur must be 0 or 1");
+  if (dynamic && (!st->static_solid || !st->old_solid || !st->solid_phi || !st->solid_vel))
+    return fail(FSI_E_INVALID, "dynamic rebuild needs static_solid, old_solid, solid_phi, solid_vel");
+  if (!st->solid_vel) return fail(FSI_E_INVALID, "rebuild needs solid_vel");
+  const long long N = (long long)p->nx * p->ny * p->nz;
+  hipLaunchKernelGGL(k_lbm_rebuild, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st, dynamic, cur);
+  return check_launch("k_lbm_rebuild");
+}
+
+int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity, uint32_t flags,
+                    void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if ((cur != 0 && cur != 1) || (parity != 0 && parity != 1))
+    return fail(FSI_E_INVALID, "cur and parity must be 0 or 1");
+  const bool mb = flags & FSI_SUB_MOVING_WALL;
+  if (mb && (!st->link_c[0] || !st->link_c[1] || !st->solid_vel || !st->hydro_accum || !st->accum_runmax))
+    return fail(FSI_E_INVALID, "moving-wall substep needs link_c, solid_vel, hydro_accum, accum_runmax");
+  if (mb && (flags & FSI_SUB_DIAG) && !st->link_corr)
+    return fail(FSI_E_INVALID, "diagnostic substep needs link_corr");
+  if ((flags & FSI_SUB_CELL_FORCE) && !st->cell_force) return fail(FSI_E_INVALID, "cell_force is null");
+  const long long N = (long long)p->nx * p->ny * p->nz;
+  const dim3 grid(blocks_for(N)), block(TPB);
+  hipStream_t s = as_stream(stream);
+  const uint32_t fixed = flags & (FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE);
+  switch (fixed) {
+#define FSI_CASE(V)                                                                   \
+  case (V):                                                                           \
+    hipLaunchKernelGGL(k_lbm_substep<(V)>, grid, block, 0, s, *p, *st, cur, parity, flags); \
+    break;
+    FSI_CASE(0u)
+    FSI_CASE(1u)
+    FSI_CASE(2u)
+    FSI_CASE(3u)
+    FSI_CASE(16u)
+    FSI_CASE(17u)
+    FSI_CASE(18u)
+    FSI_CASE(19u)
+#undef FSI_CASE
+    default:
+      return fail(FSI_E_INVALID, "unsupported substep flag combination");
+  }
+  return check_launch("k_lbm_substep");
+}
+
+int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags, void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if (cur != 0 && cur != 1) return fail(FSI_E_INVALID, "cur must be 0 or 1");
+  if ((flags & FSI_SUB_CELL_FORCE) && !st->cell_force) return fail(FSI_E_INVALID, "cell_force is null");
+  const long long N = (long long)p->nx * p->ny * p->nz;
+  hipLaunchKernelGGL(k_lbm_moments, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st, cur, flags);
+  return check_launch("k_lbm_moments");
+}
+
+int fsi_lbm_mb_begin(const FsiLbmParams* p, const FsiLbmState* st, void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if (!st->hydro_accum || !st->accum_runmax || !st->counters)
+    return fail(FSI_E_INVALID, "mb_begin needs hydro_accum, accum_runmax, counters");
+  const long long N = (long long)p->nx * p->ny * p->nz;
+  hipLaunchKernelGGL(k_lbm_mb_begin, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st);
+  return check_launch("k_lbm_mb_begin");
+}
+
+int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int parity, int n_samples, int subcycled,
+                        void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if (parity != 0 && parity != 1) return fail(FSI_E_INVALID, "parity must be 0 or 1");
+  if (!st->link_c[0] || !st->link_c[1] || !st->link_corr || !st->hydro_accum || !st->accum_runmax ||
+      !st->hydro_force || !st->counters)
+    return fail(FSI_E_INVALID, "mb_finalize needs link buffers, accumulator, hydro_force, counters");
+  if (n_samples < 1) return fail(FSI_E_INVALID, "n_samples must be positive");
+  const long long N = (long long)p->nx * p->ny * p->nz;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_lbm_finalize_clear, dim3(1), dim3(64), 0, s, *st);
+  hipLaunchKernelGGL(k_lbm_finalize, dim3(blocks_for(N)), dim3(TPB), 0, s, *p, *st, parity, n_samples, subcycled);
+  return check_launch("k_lbm_finalize");
+}
+
+int fsi_lbm_reduce_stats(const FsiLbmParams* p, const FsiLbmState* st, FsiLbmStats* workspace_dev,
+                         FsiLbmStats* out_dev, void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if (!workspace_dev || !out_dev || !st->solid_phi || !st->cell_force || !st->hydro_force)
+    return fail(FSI_E_INVALID, "reduce_stats needs workspace, out, solid_phi, cell_force, hydro_force");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_lbm_stats_partial, dim3(FSI_REDUCE_BLOCKS), dim3(TPB), 0, s, *p, *st, workspace_dev);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, reinterpret_cast<const double*>(workspace_dev),
+                     FSI_REDUCE_BLOCKS, LBM_STATS_N, 0, reinterpret_cast<double*>(out_dev));
+  return check_launch("k_lbm_stats");
+}
+
+int fsi_project(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp, const FsiMpmState* ms,
+                const FsiCouplingParams* cp, FsiCouplingCounters* cc, void* stream) {
+  int rc = check_lbm(lp, ls);
+  if (rc) return rc;
+  rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  if (!cp || !cc || !ls->solid_phi || !ls->solid_mass || !ls->solid_vel)
+    return fail(FSI_E_INVALID, "project needs coupling params/counters and projection fields");
+  const long long N = (long long)lp->nx * lp->ny * lp->nz;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_proj_clear, dim3(blocks_for(N)), dim3(TPB), 0, s, *lp, *ls, cc);
+  if (mp->n_particles > 0)
+    hipLaunchKernelGGL(k_proj_particles, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, s, *lp, *ls, *mp, *ms, *cp);
+  hipLaunchKernelGGL(k_proj_normalize, dim3(blocks_for(N)), dim3(TPB), 0, s, *lp, *ls, *cp, cc);
+  return check_launch("fsi_project");
+}
+
+int fsi_penalty_force(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiCouplingParams* cp,
+                      FsiCouplingCounters* cc, void* stream) {
+  int rc = check_lbm(lp, ls);
+  if (rc) return rc;
+  if (!cp || !cc || !ls->cell_force || !ls->hydro_force || !ls->solid_phi || !ls->solid_vel)
+    return fail(FSI_E_INVALID, "penalty needs coupling params/counters and force fields");
+  const long long N = (long long)lp->nx * lp->ny * lp->nz;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_penalty_clear, dim3(1), dim3(64), 0, s, cc);
+  hipLaunchKernelGGL(k_penalty, dim3(blocks_for(N)), dim3(TPB), 0, s, *lp, *ls, *cp, cc);
+  return check_launch("fsi_penalty_force");
+}
+
+int fsi_mpm_clear_grid(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {
+  int rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  const long long G = (long long)mp->gnx * mp->gny * mp->gnz;
+  hipLaunchKernelGGL(k_mpm_clear, dim3(blocks_for(G)), dim3(TPB), 0, as_stream(stream), *mp, *ms);
+  return check_launch("fsi_mpm_clear_grid");
+}
+
+int fsi_mpm_p2g(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {
+  int rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  if (mp->n_particles > 0)
+    hipLaunchKernelGGL(k_mpm_p2g, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, s, *mp, *ms);
+  return check_launch("fsi_mpm_p2g");
+}
+
+int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp, const FsiMpmState* ms,
+                     const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode, void* stream) {
+  int rc = check_lbm(lp, ls);
+  if (rc) return rc;
+  rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  if (mode != 1 && mode != 2) return fail(FSI_E_INVALID, "reaction mode must be 1 (penalty) or 2 (moving boundary)");
+  if (!cp || !cc || !ls->hydro_force || !ls->solid_phi) return fail(FSI_E_INVALID, "reaction needs hydro_force, solid_phi");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_reaction_clear, dim3(1), dim3(64), 0, s, cc);
+  if (mp->n_particles > 0)
+    hipLaunchKernelGGL(k_mpm_reaction, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, s, *lp, *ls, *mp, *ms, *cp, cc,
+                       mode);
+  return check_launch("fsi_mpm_reaction");
+}
+
+int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {
+  int rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  const long long G = (long long)mp->gnx * mp->gny * mp->gnz;
+  hipLaunchKernelGGL(k_mpm_grid_update, dim3(blocks_for(G)), dim3(TPB), 0, as_stream(stream), *mp, *ms);
+  return check_launch("fsi_mpm_grid_update");
+}
+
+int fsi_mpm_g2p(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {
+  int rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  if (!ms->fixed_mask || !ms->fixed_x) return fail(FSI_E_INVALID, "g2p needs fixed_mask and fixed_x");
+  if (mp->n_particles > 0)
+    hipLaunchKernelGGL(k_mpm_g2p, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, as_stream(stream), *mp, *ms);
+  return check_launch("fsi_mpm_g2p");
+}
+
+int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmStats* workspace_dev,
+                         FsiMpmStats* out_dev, void* stream) {
+  int rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  if (!workspace_dev || !out_dev) return fail(FSI_E_INVALID, "reduce_stats needs workspace and out");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_mpm_stats_partial, dim3(FSI_REDUCE_BLOCKS), dim3(TPB), 0, s, *mp, *ms, workspace_dev);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, reinterpret_cast<const double*>(workspace_dev),
+                     FSI_REDUCE_BLOCKS, MPM_STATS_N, 1, reinterpret_cast<double*>(out_dev));
+  return check_launch("k_mpm_stats");
+}
+
+}  // extern "C"

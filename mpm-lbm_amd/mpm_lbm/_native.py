@@ -1,0 +1,243 @@
+"""ctypes binding of libfsi_hip.so (C ABI declared in include/fsi_hip.h).
+
+The structs below mirror the header field for field; `Native.check_abi()`
+compares their sizes with the library's own `fsi_abi_sizes()` so a layout
+drift fails loudly instead of corrupting memory.
+
+Only `load_hip()` is used by the product path.  It raises if the gfx950
+library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+LIB_DIR = _HERE / "_lib"
+HIP_LIB_PATH = LIB_DIR / "libfsi_hip.so"
+
+f32 = ctypes.c_float
+i32 = ctypes.c_int32
+u32 = ctypes.c_uint32
+f64 = ctypes.c_double
+vp = ctypes.c_void_p
+
+FSI_BC_PERIODIC = 0
+FSI_BC_PRESSURE = 1
+FSI_BC_VELOCITY = 2
+
+SUB_STORED_MOMENTS = 1
+SUB_MOVING_WALL = 2
+SUB_GATHER_PREV = 4
+SUB_DIAG = 8
+SUB_CELL_FORCE = 16
+
+REDUCE_BLOCKS = 256
+
+
+class LbmParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", i32), ("ny", i32), ("nz", i32),
+        ("bc_x_left", i32), ("bc_x_right", i32),
+        ("s_v", f32), ("s_other", f32), ("rho0", f32),
+        ("force", f32 * 3),
+        ("rho_bc_x_left", f32), ("rho_bc_x_right", f32),
+        ("vel_bc_x_left", f32 * 3), ("vel_bc_x_right", f32 * 3),
+        ("dynamic_solid_threshold", f32),
+    ]
+
+
+class LbmCounters(ctypes.Structure):
+    _fields_ = [
+        ("bb_link_count", i32),
+        ("bb_max_correction_bits", u32),
+        ("mb_sample_count", i32),
+        ("accum_norm_max_bits", u32),
+        ("mean_norm_max_bits", u32),
+        ("pad0", i32),
+        ("bb_net_solid_force", f64 * 3),
+    ]
+
+
+class LbmState(ctypes.Structure):
+    _fields_ = [
+        ("F", vp * 2),
+        ("rho", vp), ("vel", vp),
+        ("solid", vp), ("static_solid", vp), ("old_solid", vp),
+        ("cellinfo", vp),
+        ("solid_phi", vp), ("solid_mass", vp), ("solid_vel", vp),
+        ("cell_force", vp), ("hydro_force", vp),
+        ("link_c", vp * 2), ("link_corr", vp),
+        ("hydro_accum", vp), ("accum_runmax", vp),
+        ("counters", vp),
+    ]
+
+
+LBM_STATS_FIELDS = [
+    "fluid_count", "rho_min", "rho_max", "mass_total", "max_v", "force_norm_max",
+    "fluid_sum_vx", "fluid_sum_vy", "fluid_sum_vz",
+    "phi_weight_sum", "phi_weighted_vx", "phi_weighted_vy", "phi_weighted_vz",
+    "far_count", "far_sum_vx", "far_sum_vy", "far_sum_vz",
+    "max_cell_force_norm_all", "max_hydro_force_norm_all",
+    "net_cell_force_x", "net_cell_force_y", "net_cell_force_z",
+    "net_hydro_force_x", "net_hydro_force_y", "net_hydro_force_z",
+    "mass_rho_all",
+]
+
+
+class LbmStats(ctypes.Structure):
+    _fields_ = [(name, f64) for name in LBM_STATS_FIELDS]
+
+
+class MpmParams(ctypes.Structure):
+    _fields_ = [
+        ("gnx", i32), ("gny", i32), ("gnz", i32),
+        ("n_particles", i32),
+        ("bound", i32),
+        ("damping_enabled", i32), ("lock_z_enabled", i32),
+        ("dx", f32), ("inv_dx", f32), ("dt", f32),
+        ("mu", f32), ("la", f32),
+        ("gravity", f32 * 3),
+        ("velocity_damping", f32),
+        ("domain_max", f32 * 3),
+    ]
+
+
+class MpmState(ctypes.Structure):
+    _fields_ = [
+        ("x", vp), ("v", vp), ("C", vp), ("F", vp), ("Jp", vp),
+        ("mass", vp), ("vol0", vp), ("fixed_mask", vp), ("fixed_x", vp),
+        ("grid_v", vp), ("grid_m", vp), ("grid_f", vp),
+    ]
+
+
+class CouplingParams(ctypes.Structure):
+    _fields_ = [
+        ("inv_dx_norm", f32), ("cell_volume_norm", f32), ("vel_scale_norm_to_lbm", f32),
+        ("eps_mass", f32),
+        ("beta_lbm", f32), ("penalty_phi_min", f32), ("force_cap_lbm", f32),
+        ("force_density_scale", f32), ("reaction_scale", f32),
+        ("mb_phi_min", f32), ("mb_force_cap_norm", f32),
+    ]
+
+
+class CouplingCounters(ctypes.Structure):
+    _fields_ = [
+        ("projected_mass", f64), ("projected_volume_raw", f64), ("projected_volume_clamped", f64),
+        ("max_phi_raw_bits", u32),
+        ("active_cell_count", i32),
+        ("active_reaction_particle_count", i32),
+        ("active_force_cell_count", i32),
+        ("max_particle_reaction_bits", u32),
+        ("max_grid_reaction_bits", u32),
+        ("max_cell_force_bits", u32), ("max_hydro_force_bits", u32),
+        ("net_particle_reaction", f64 * 3),
+        ("net_grid_reaction", f64 * 3),
+        ("net_cell_force", f64 * 3), ("net_hydro_force", f64 * 3),
+    ]
+
+
+MPM_STATS_FIELDS = [
+    "min_x0", "min_x1", "min_x2", "max_x0", "max_x1", "max_x2",
+    "max_speed", "min_J", "max_J", "total_mass", "mv_x", "mv_y", "mv_z",
+]
+
+
+class MpmStats(ctypes.Structure):
+    _fields_ = [(name, f64) for name in MPM_STATS_FIELDS]
+
+
+ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, CouplingParams,
+               CouplingCounters, MpmStats]
+
+EXPORTED_SYMBOLS = [
+    "fsi_last_error", "fsi_version", "fsi_abi_sizes", "fsi_sync",
+    "fsi_lbm_init", "fsi_lbm_rebuild", "fsi_lbm_substep", "fsi_lbm_moments",
+    "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats",
+    "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
+    "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_reduce_stats",
+]
+
+P = ctypes.POINTER
+
+
+class FsiError(RuntimeError):
+    """A libfsi_hip entry point returned a negative status."""
+
+
+class Native:
+    """Typed wrapper over one loaded copy of the C ABI."""
+
+    def __init__(self, path: os.PathLike | str, kind: str):
+        self.path = str(path)
+        self.kind = kind  # "hip" or "emulation"
+        self.lib = ctypes.CDLL(self.path)
+        L = self.lib
+        L.fsi_last_error.restype = ctypes.c_char_p
+        L.fsi_last_error.argtypes = []
+        L.fsi_version.restype = ctypes.c_int
+        L.fsi_abi_sizes.restype = ctypes.c_int
+        L.fsi_abi_sizes.argtypes = [P(i32), ctypes.c_int]
+        L.fsi_sync.argtypes = [vp]
+        L.fsi_lbm_init.argtypes = [P(LbmParams), P(LbmState), vp]
+        L.fsi_lbm_rebuild.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, vp]
+        L.fsi_lbm_substep.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, u32, vp]
+        L.fsi_lbm_moments.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, u32, vp]
+        L.fsi_lbm_mb_begin.argtypes = [P(LbmParams), P(LbmState), vp]
+        L.fsi_lbm_mb_finalize.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.fsi_lbm_reduce_stats.argtypes = [P(LbmParams), P(LbmState), vp, vp, vp]
+        L.fsi_project.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState), P(CouplingParams), vp, vp]
+        L.fsi_penalty_force.argtypes = [P(LbmParams), P(LbmState), P(CouplingParams), vp, vp]
+        L.fsi_mpm_clear_grid.argtypes = [P(MpmParams), P(MpmState), vp]
+        L.fsi_mpm_p2g.argtypes = [P(MpmParams), P(MpmState), vp]
+        L.fsi_mpm_reaction.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState), P(CouplingParams),
+                                       vp, ctypes.c_int, vp]
+        L.fsi_mpm_grid_update.argtypes = [P(MpmParams), P(MpmState), vp]
+        L.fsi_mpm_g2p.argtypes = [P(MpmParams), P(MpmState), vp]
+        L.fsi_mpm_reduce_stats.argtypes = [P(MpmParams), P(MpmState), vp, vp, vp]
+        for name in EXPORTED_SYMBOLS:
+            getattr(L, name).restype = ctypes.c_int if name != "fsi_last_error" else ctypes.c_char_p
+        self.check_abi()
+
+    def check_abi(self):
+        out = (i32 * len(ABI_STRUCTS))()
+        n = self.lib.fsi_abi_sizes(out, len(ABI_STRUCTS))
+        if n != len(ABI_STRUCTS):
+            raise FsiError(f"{self.path}: fsi_abi_sizes returned {n}")
+        for cls, size in zip(ABI_STRUCTS, out):
+            if ctypes.sizeof(cls) != size:
+                raise FsiError(
+                    f"{self.path}: struct {cls.__name__} is {size} bytes in C but {ctypes.sizeof(cls)} in ctypes"
+                )
+
+    def call(self, name: str, *args):
+        rc = getattr(self.lib, name)(*args)
+        if rc != 0:
+            msg = self.lib.fsi_last_error().decode("utf-8", "replace")
+            raise FsiError(f"{name} failed ({rc}): {msg}")
+        return rc
+
+
+_HIP: Native | None = None
+
+
+def load_hip() -> Native:
+    """Load the gfx950 library built in-tree; raise if it is missing."""
+    global _HIP
+    if _HIP is None:
+        if not HIP_LIB_PATH.is_file():
+            raise FsiError(
+                f"{HIP_LIB_PATH} is not built: run __graft_entry__.build() (hipcc --offload-arch=gfx950) first"
+            )
+        _HIP = Native(HIP_LIB_PATH, "hip")
+    return _HIP
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a contiguous torch tensor, or 0 for None."""
+    if t is None:
+        return 0
+    if not t.is_contiguous():
+        raise ValueError("buffers passed to libfsi_hip must be contiguous")
+    return int(t.data_ptr())

@@ -1,0 +1,1 @@
+"""Mirrors the reference package of the same name (src/mpm_lbm/sim)."""
