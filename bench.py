@@ -1,0 +1,225 @@
+#!/usr/bin/env python3
+"""Benchmark: plain FP32 moving-boundary FSI at 64^3 on one MI355X.
+
+Workload: 64^3 duct with a 4x32x24-cell elastic flap (24,576 particles,
+seeded jitter), moving-wall bounce-back with the subcycle force
+accumulator, 120 LBM substeps and 10 MPM substeps per FSI step.  A "step" of
+this benchmark is one FSI step.  `value` is MLUPS counted over all lattice
+cells: 64^3 * 120 * K / seconds / 1e6 for K timed FSI steps (projection, flag
+rebuild, 120 fused LBM substeps, finalize, moments and 10 MPM substeps each).
+
+Also reported:
+  roofline     the fused LBM substep kernel: algorithmic bytes per launch
+               (152 B per fluid cell + 4 B cell word per cell) over its
+               average duration, timed live with HIP events on the stream
+               the kernels run on, against 8 TB/s;
+  copy_ceiling device-to-device copy bandwidth measured in the same run;
+  cpu_baseline the reference-structured C++ oracle (OpenMP) on the host
+               cores, one FSI step of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "mpm-lbm_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "FSI MLUPS + % HBM peak at 64^3; batched/served vs plain at equal error; p99"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed FSI steps")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed FSI steps")
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--substeps", type=int, default=120)
+    ap.add_argument("--kernel-launches", type=int, default=600, help="substep launches timed for the roofline")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="FSI steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "lbm_substep_64cube_traffic.json"))
+    return ap.parse_args()
+
+
+def copy_ceiling_gbs(torch, nbytes=1 << 30, reps=20):
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    del a, b
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
+def time_substep_kernel(torch, lbm, launches):
+    """Average duration of the fused substep as the FSI loop launches it (moving wall + gather)."""
+    import ctypes
+
+    from mpm_lbm import _native
+
+    flags = _native.SUB_MOVING_WALL | _native.SUB_GATHER_PREV
+    stream = lbm._stream()
+    parity = 0
+
+    def launch():
+        nonlocal parity
+        lbm.native.call("fsi_lbm_substep", ctypes.byref(lbm._params), ctypes.byref(lbm._state), lbm.cur, parity,
+                        flags, stream)
+        lbm.cur = 1 - lbm.cur
+        parity = 1 - parity
+
+    for _ in range(20):
+        launch()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(launches):
+        launch()
+    e.record()
+    torch.cuda.synchronize()
+    lbm._moments_fresh = False
+    return s.elapsed_time(e) / launches
+
+
+def cpu_baseline(workload, steps):
+    import numpy as np
+
+    from mpm_lbm.sim.geometry.config import GeometryConfig
+    from mpm_lbm.sim.geometry.duct_flap_proxy import duct_flap_proxy_static_geometry
+    from oracle.oracle import OracleFSI
+
+    flags, _ = duct_flap_proxy_static_geometry(workload.shape, GeometryConfig.from_json(workload.geometry_json))
+    o = OracleFSI(workload.config, flags, workload.cloud, workload.material, dtype=np.float32)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step_once()
+    dt = time.perf_counter() - t0
+    cells = workload.n_cells * workload.config.lbm_substeps_per_fsi_step * steps
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return {
+        "value": round(cells / dt / 1e6, 3),
+        "unit": "MLUPS",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{steps} FSI step(s) of the same 64^3 workload ({workload.config.lbm_substeps_per_fsi_step} "
+                  f"LBM + {workload.config.mpm_substeps_per_lbm_step} MPM substeps each), C++/OpenMP "
+                  f"reference-structured oracle, FP32, {dt:.2f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the FSI path runs on a single "
+              "MI355X; see the sharding design notes)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from mpm_lbm import workloads
+    from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
+
+    n, sub = args.n, args.substeps
+    w = workloads.moving_boundary_flap(n=n, steps=args.warmup + args.steps, substeps=sub, out_dir="/tmp/fsi_bench")
+    log(f"[bench] workload {w.name}: {w.config.n_particles} particles, {sub} LBM substeps/FSI step")
+    d = FSIDriver3D(w.config, "/tmp/fsi_bench/run", particle_cloud=w.cloud, write_geometry_file=False)
+    d.initialize()
+    for i in range(args.warmup):
+        d.step_once()
+    torch.cuda.synchronize()
+    log(f"[bench] warmup {args.warmup} FSI steps done")
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for i in range(args.steps):
+        d.step_once()
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(end)
+    secs = gpu_ms / 1e3
+    mlups = n ** 3 * sub * args.steps / secs / 1e6
+    log(f"[bench] timed {args.steps} FSI steps: {gpu_ms:.2f} ms (wall {wall:.3f} s) -> {mlups:.1f} MLUPS")
+    row = d.collect_diagnostics(d.current_lbm_step)
+    if not all(np.isfinite(v) for v in row.values() if isinstance(v, float)):
+        raise RuntimeError("non-finite diagnostics after the timed steps")
+
+    fluid = int(np.count_nonzero(d.lbm.solid.to_numpy() == 0))
+    algo_bytes = fluid * 152 + n ** 3 * 4
+    k_ms = time_substep_kernel(torch, d.lbm, args.kernel_launches)
+    achieved = algo_bytes / (k_ms * 1e-3) / 1e9
+    log(f"[bench] fused substep: {k_ms * 1e3:.2f} us avg over {args.kernel_launches} launches, {achieved:.0f} GB/s")
+    copy_gbs = copy_ceiling_gbs(torch)
+    traffic = None
+    tp = Path(args.traffic_json)
+    if tp.is_file():
+        traffic = json.loads(tp.read_text()).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log("[bench] CPU baseline ...")
+        cpu = cpu_baseline(w, args.cpu_steps)
+        log(f"[bench] CPU baseline {cpu['value']} MLUPS on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(gpu_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain FP32 moving_boundary FSI, {n}^3 duct + 4x32x24-cell flap, "
+                        f"{w.config.n_particles} particles, {sub} LBM substeps + "
+                        f"{w.config.mpm_substeps_per_lbm_step} MPM substeps per FSI step",
+            "lattice": [n, n, n],
+            "fluid_cells": fluid,
+            "particles": w.config.n_particles,
+            "lbm_substeps_per_fsi_step": sub,
+            "mfluops": round(fluid * sub * args.steps / secs / 1e6, 2),
+            "particle_updates_per_s": round(w.config.n_particles * w.config.mpm_substeps_per_lbm_step * args.steps / secs, 1),
+            "copy_ceiling_gbs": round(copy_gbs, 1),
+            "substep_frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "k_lbm_substep (fused collide+stream+bounce-back+BC)",
+            "algorithmic_bytes_per_launch": algo_bytes,
+            "avg_launch_us": round(k_ms * 1e3, 3),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

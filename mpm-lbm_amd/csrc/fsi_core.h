@@ -705,32 +705,23 @@ FSI_HD float det3(const float* A) {
          A[2] * (A[3] * A[7] - A[4] * A[6]);
 }
 
-// Rotation factor R = U V^T of the SVD F = U diag(s) V^T with det U = det V = +1
-// (sign of det F carried by the smallest singular value), the convention of
-// the reference's ti.svd.  Method: cyclic Jacobi eigen-decomposition of the
-// symmetric F^T F (Golub & Van Loan, Matrix Computations, 4th ed., sec. 8.5),
-// U's first two columns from F v_i / s_i (Gram-Schmidt), third = u0 x u1.
-FSI_HD void rotation_of(const float* Fm, float* R) {
-  float A[9];
+// Symmetric 3x3 Jacobi eigen-decomposition A = V diag(A_ii) V^T, cyclic
+// sweeps (Golub & Van Loan, Matrix Computations, 4th ed., sec. 8.5).
+FSI_HD void sym_eig3(float* A, float* V) {
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      A[r * 3 + c] = Fm[0 * 3 + r] * Fm[0 * 3 + c] + Fm[1 * 3 + r] * Fm[1 * 3 + c] +
-                     Fm[2 * 3 + r] * Fm[2 * 3 + c];
-  float V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int e = 0; e < 9; ++e) V[e] = (e == 0 || e == 4 || e == 8) ? 1.0f : 0.0f;
   for (int sweep = 0; sweep < 6; ++sweep) {
 #pragma unroll
     for (int pq = 0; pq < 3; ++pq) {
       const int pi = pq == 2 ? 1 : 0;
       const int qi = pq == 0 ? 1 : 2;
       const float apq = A[pi * 3 + qi];
-      if (fabsf(apq) > 1e-30f) {
+      const float scale = fabsf(A[pi * 3 + pi]) + fabsf(A[qi * 3 + qi]);
+      if (apq != 0.0f && fabsf(apq) > 1e-12f * scale) {
         const float theta = (A[qi * 3 + qi] - A[pi * 3 + pi]) / (2.0f * apq);
         const float t = (theta >= 0.0f ? 1.0f : -1.0f) / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
         const float cs = 1.0f / sqrtf(t * t + 1.0f);
         const float sn = t * cs;
-        // A <- J^T A J with J the (p,q) rotation
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
           const float arp = A[r * 3 + pi], arq = A[r * 3 + qi];
@@ -752,7 +743,65 @@ FSI_HD void rotation_of(const float* Fm, float* R) {
       }
     }
   }
-  // order eigenpairs by decreasing eigenvalue
+}
+
+// Rotation R = U V^T of the reference's ti.svd(F) (det U = det V = +1, the
+// sign of det F carried by the smallest singular value), returned together
+// with D = F - R, the only part of R the fixed-corotated stress needs.
+//
+// det F > 0: R is the orthogonal polar factor, R = F S^{-1} with
+// S = (F^T F)^{1/2}, so D = F (I - S^{-1}).  I - S^{-1} is computed from the
+// Green-strain form F^T F - I = H + H^T + H^T H (H = F - I) so that D keeps
+// full fp32 relative precision at small strain:
+//   F^T F - I = V diag(m) V^T,  s = sqrt(1 + m),  I - S^{-1} = V diag(m / (s (s + 1))) V^T.
+// det F <= 0: SVD from the eigenvectors of F^T F (u_i = F v_i / s_i for the
+// two largest s_i, u_2 = u_0 x u_1), R = sum u_i v_i^T, D = F - R.
+FSI_HD void rotation_and_deviation(const float* Fm, float* R, float* D) {
+  const float detF = det3(Fm);
+  if (detF > 0.0f) {
+    float H[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) H[e] = Fm[e] - ((e == 0 || e == 4 || e == 8) ? 1.0f : 0.0f);
+    float A[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        A[r * 3 + c] = H[r * 3 + c] + H[c * 3 + r] +
+                       (H[0 * 3 + r] * H[0 * 3 + c] + H[1 * 3 + r] * H[1 * 3 + c] + H[2 * 3 + r] * H[2 * 3 + c]);
+    float V[9];
+    sym_eig3(A, V);
+    float d[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float m = A[i * 4];
+      const float s = sqrtf(1.0f + m);
+      d[i] = m / (s * (s + 1.0f));
+    }
+    float K[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        K[r * 3 + c] = V[r * 3 + 0] * d[0] * V[c * 3 + 0] + V[r * 3 + 1] * d[1] * V[c * 3 + 1] +
+                       V[r * 3 + 2] * d[2] * V[c * 3 + 2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        D[r * 3 + c] = Fm[r * 3 + 0] * K[0 * 3 + c] + Fm[r * 3 + 1] * K[1 * 3 + c] + Fm[r * 3 + 2] * K[2 * 3 + c];
+        R[r * 3 + c] = Fm[r * 3 + c] - D[r * 3 + c];
+      }
+    return;
+  }
+  float A[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      A[r * 3 + c] = Fm[0 * 3 + r] * Fm[0 * 3 + c] + Fm[1 * 3 + r] * Fm[1 * 3 + c] + Fm[2 * 3 + r] * Fm[2 * 3 + c];
+  float V[9];
+  sym_eig3(A, V);
   float lam[3] = {A[0], A[4], A[8]};
   int o0 = 0, o1 = 1, o2 = 2;
   if (lam[o0] < lam[o1]) { int t = o0; o0 = o1; o1 = t; }
@@ -761,7 +810,6 @@ FSI_HD void rotation_of(const float* Fm, float* R) {
   float v0[3] = {V[0 * 3 + o0], V[1 * 3 + o0], V[2 * 3 + o0]};
   float v1[3] = {V[0 * 3 + o1], V[1 * 3 + o1], V[2 * 3 + o1]};
   float v2[3] = {V[0 * 3 + o2], V[1 * 3 + o2], V[2 * 3 + o2]};
-  // det V = +1
   const float dv = v0[0] * (v1[1] * v2[2] - v1[2] * v2[1]) - v0[1] * (v1[0] * v2[2] - v1[2] * v2[0]) +
                    v0[2] * (v1[0] * v2[1] - v1[1] * v2[0]);
   if (dv < 0.0f) { v2[0] = -v2[0]; v2[1] = -v2[1]; v2[2] = -v2[2]; }
@@ -785,8 +833,10 @@ FSI_HD void rotation_of(const float* Fm, float* R) {
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
       R[r * 3 + c] = u0[r] * v0[c] + u1[r] * v1[c] + u2[r] * v2[c];
+      D[r * 3 + c] = Fm[r * 3 + c] - R[r * 3 + c];
+    }
 }
 
 FSI_HD void load_mat(const float* base, long long P, long long p, float* A) {
@@ -810,10 +860,10 @@ FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
   float fx[3], w[3][3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * mp.inv_dx, base[d], fx[d], w[d]);
-  float Fm[9], Cm[9], R[9];
+  float Fm[9], Cm[9], R[9], Dv[9];
   load_mat(ms.F, P, p, Fm);
   load_mat(ms.C, P, p, Cm);
-  rotation_of(Fm, R);
+  rotation_and_deviation(Fm, R, Dv);
   const float J = det3(Fm);
   ms.Jp[p] = J;
   // F^{-T} = cofactor(F) / det F
@@ -830,7 +880,7 @@ FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
   float Pk[9];
   const float lj = mp.la * (J - 1.0f) * J;
 #pragma unroll
-  for (int e = 0; e < 9; ++e) Pk[e] = 2.0f * mp.mu * (Fm[e] - R[e]) + lj * FiT[e];
+  for (int e = 0; e < 9; ++e) Pk[e] = 2.0f * mp.mu * Dv[e] + lj * FiT[e];
   const float mass = ms.mass[p];
   const float sc = -mp.dt * ms.vol0[p];
   float aff[9];

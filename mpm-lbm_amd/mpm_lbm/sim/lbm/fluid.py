@@ -412,13 +412,13 @@ class LBMFluid3D:
         return {"max_v": r["max_v"], "rho_min": r["rho_min"], "rho_max": r["rho_max"], "mass_total": r["mass_total"],
                 "force_norm_max": r["force_norm_max"]}
 
-    def _counters(self):
+    def _read_counters(self):
         return read_struct(self._counters, _native.LbmCounters)
 
     def get_moving_boundary_stats(self):
         """fluid.py:2067-2079 (last substep's link statistics)."""
         self._sync_hydro()
-        k = self._counters()
+        k = self._read_counters()
         solid_force = tuple(float(v) for v in k.bb_net_solid_force)
         return {
             "bb_link_count": int(k.bb_link_count),
@@ -429,7 +429,7 @@ class LBMFluid3D:
 
     def get_moving_boundary_accumulation_stats(self):
         """fluid.py:2093-2097."""
-        k = self._counters()
+        k = self._read_counters()
         return {
             "mb_subcycle_force_sample_count": int(k.mb_sample_count),
             "mb_subcycle_force_accum_norm_max": bits_to_float(k.accum_norm_max_bits),

@@ -529,8 +529,11 @@ void p2g(const MpmArgs<T>& a, const T* x, const T* v, const T* C, const T* F, T*
     double Fd[9], Rd[9];
     for (int e = 0; e < 9; ++e) Fd[e] = (double)Fp[e];
     polar_rotation(Fd, Rd);
-    T R[9];
-    for (int e = 0; e < 9; ++e) R[e] = (T)Rd[e];
+    // F - R from the exact polar factor (evaluated in double, then narrowed):
+    // the reference's ti.svd f32 iteration is not available, and at small
+    // strain an f32-rounded R would leave F - R with only a few valid digits.
+    T FmR[9];
+    for (int e = 0; e < 9; ++e) FmR[e] = (T)(Fd[e] - Rd[e]);
     const T J = det3<T>(Fp);
     Jp[p] = J;
     T FiT[9];
@@ -544,7 +547,7 @@ void p2g(const MpmArgs<T>& a, const T* x, const T* v, const T* C, const T* F, T*
     FiT[7] = (Fp[2] * Fp[3] - Fp[0] * Fp[5]) / J;
     FiT[8] = (Fp[0] * Fp[4] - Fp[1] * Fp[3]) / J;
     T P[9];
-    for (int e = 0; e < 9; ++e) P[e] = (T)2.0 * a.mu * (Fp[e] - R[e]) + a.la * (J - (T)1.0) * J * FiT[e];
+    for (int e = 0; e < 9; ++e) P[e] = (T)2.0 * a.mu * FmR[e] + a.la * (J - (T)1.0) * J * FiT[e];
     T aff[9];
     for (int r = 0; r < 3; ++r)
       for (int c = 0; c < 3; ++c) {
