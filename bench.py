@@ -70,29 +70,35 @@ def copy_ceiling_gbs(torch, nbytes=1 << 30, reps=20):
 
 
 def time_substep_kernel(torch, lbm, launches):
-    """Average duration of the fused substep as the FSI loop launches it (moving wall + gather)."""
+    """Average duration of the fused substep as the FSI loop issues it
+    (moving wall + gather of the previous substep): `launches` back-to-back
+    launches captured in one HIP graph, timed with HIP events around its
+    replay on the stream the kernels run on."""
     import ctypes
 
     from mpm_lbm import _native
 
+    launches += launches % 2  # even: the buffers return to their start
     flags = _native.SUB_MOVING_WALL | _native.SUB_GATHER_PREV
-    stream = lbm._stream()
-    parity = 0
 
-    def launch():
-        nonlocal parity
-        lbm.native.call("fsi_lbm_substep", ctypes.byref(lbm._params), ctypes.byref(lbm._state), lbm.cur, parity,
-                        flags, stream)
-        lbm.cur = 1 - lbm.cur
-        parity = 1 - parity
+    def issue():
+        parity, cur = 0, lbm.cur
+        stream = lbm._stream()
+        for _ in range(launches):
+            lbm.native.call("fsi_lbm_substep", ctypes.byref(lbm._params), ctypes.byref(lbm._state), cur, parity,
+                            flags, stream)
+            cur, parity = 1 - cur, 1 - parity
 
-    for _ in range(20):
-        launch()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    issue()  # warm
     torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        issue()
+    graph.replay()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.record()
-    for _ in range(launches):
-        launch()
+    graph.replay()
     e.record()
     torch.cuda.synchronize()
     lbm._moments_fresh = False

@@ -38,6 +38,16 @@ extern "C" {
 #define FSI_BC_PRESSURE 1
 #define FSI_BC_VELOCITY 2
 
+/* Population storage (FsiLbmParams.storage).  The 16-bit forms store the
+ * deviation F_s - w_s from the rest weights; collisions run in FP32. */
+#define FSI_STORE_F32 0
+#define FSI_STORE_F16_DEV 1
+#define FSI_STORE_BF16_DEV 2
+
+/* Collision operator (FsiLbmParams.collision). */
+#define FSI_COLLIDE_MRT 0         /* the reference's MRT with Guo forcing (fluid.py:399-413) */
+#define FSI_COLLIDE_REGULARIZED 1 /* regularised BGK, tau = 1/s_v (no reference counterpart) */
+
 /* Substep flags. */
 #define FSI_SUB_STORED_MOMENTS 1u /* collide with stored rho/vel instead of moments of F_src */
 #define FSI_SUB_MOVING_WALL 2u    /* moving-wall bounce-back (streaming_moving_bounceback) */
@@ -53,6 +63,8 @@ typedef struct FsiLbmParams {
   float rho_bc_x_left, rho_bc_x_right;
   float vel_bc_x_left[3], vel_bc_x_right[3];
   float dynamic_solid_threshold;
+  int32_t storage;   /* FSI_STORE_*: population storage in F[0], F[1] */
+  int32_t collision; /* FSI_COLLIDE_*: collision operator */
 } FsiLbmParams;
 
 /* Device-side counters of the LBM (all reductions are order-independent:

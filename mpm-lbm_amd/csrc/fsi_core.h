@@ -21,15 +21,17 @@
 
 #include <stdint.h>
 #include <math.h>
+#include <string.h>
 #include "../../include/fsi_hip.h"
 
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #define FSI_HD __host__ __device__ __forceinline__
-#define FSI_DEV_ONLY __device__
+#define FSI_HDM __host__ __device__ __forceinline__
 #else
 #define FSI_HD static inline
-#define FSI_DEV_ONLY
+#define FSI_HDM inline
 #endif
 
 namespace fsi {
@@ -57,6 +59,89 @@ FSI_HD float feq(int s, float rho, float ux, float uy, float uz) {
   const float eu = (float)EX[s] * ux + (float)EY[s] * uy + (float)EZ[s] * uz;
   const float uv = ux * ux + uy * uy + uz * uz;
   return wgt(s) * rho * (1.0f + 3.0f * eu + 4.5f * eu * eu - 1.5f * uv);
+}
+
+// ---------------------------------------------------------------------------
+// Population storage.  FP32 stores F_s; the 16-bit forms store the deviation
+// d_s = F_s - w_s from the rest-state weights, rounded to nearest even, and
+// every load / collision runs in FP32 (F_s = w_s + d_s).
+// ---------------------------------------------------------------------------
+FSI_HD uint32_t f32_bits(float v) {
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  return u;
+}
+FSI_HD float bits_f32(uint32_t u) {
+  float v;
+  memcpy(&v, &u, 4);
+  return v;
+}
+FSI_HD uint16_t f32_to_bf16(float v) {
+  const uint32_t u = f32_bits(v);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+FSI_HD float bf16_to_f32(uint16_t h) { return bits_f32((uint32_t)h << 16); }
+
+#ifdef __HIPCC__
+FSI_HD uint16_t f32_to_f16(float v) { return __half_as_ushort(__float2half_rn(v)); }
+FSI_HD float f16_to_f32(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+#else
+// IEEE binary16 with round-to-nearest-even (what v_cvt_f16_f32 does on gfx950).
+static inline uint16_t f32_to_f16(float v) {
+  const uint32_t u = f32_bits(v);
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const uint32_t ax = u & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return (uint16_t)(sign | 0x7C00u | (ax > 0x7F800000u ? 0x200u : 0u));
+  if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);  // rounds to infinity
+  if (ax < 0x38800000u) {                                   // subnormal or zero in binary16
+    if (ax < 0x33000000u) return (uint16_t)sign;            // below half the smallest subnormal
+    const uint32_t mant = (ax & 0x007FFFFFu) | 0x00800000u;
+    const int shift = 126 - (int)(ax >> 23);                // 14..24
+    const uint32_t q = mant >> shift, rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    const uint32_t r = q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u);
+    return (uint16_t)(sign | r);
+  }
+  const uint32_t e = (ax >> 23) - 112u;
+  const uint32_t m = ax & 0x007FFFFFu;
+  uint32_t h = (e << 10) | (m >> 13);
+  const uint32_t rem = m & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+  return (uint16_t)(sign | h);
+}
+static inline float f16_to_f32(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  if (e == 0) {
+    const float v = (float)m * 5.9604644775390625e-8f;  // m * 2^-24
+    return sign ? -v : v;
+  }
+  if (e == 31) return bits_f32(sign | 0x7F800000u | (m << 13));
+  return bits_f32(sign | ((e + 112u) << 23) | (m << 13));
+}
+#endif
+
+struct PopF32 {
+  float* p;
+  FSI_HDM float get(long long N, int s, long long c) const { return p[s * N + c]; }
+  FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = v; }
+};
+struct PopF16 {
+  uint16_t* p;
+  FSI_HDM float get(long long N, int s, long long c) const { return wgt(s) + f16_to_f32(p[s * N + c]); }
+  FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = f32_to_f16(v - wgt(s)); }
+};
+struct PopBF16 {
+  uint16_t* p;
+  FSI_HDM float get(long long N, int s, long long c) const { return wgt(s) + bf16_to_f32(p[s * N + c]); }
+  FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = f32_to_bf16(v - wgt(s)); }
+};
+
+template <typename Pop>
+FSI_HD Pop pop_of(void* base) {
+  Pop q;
+  q.p = reinterpret_cast<decltype(q.p)>(base);
+  return q;
 }
 
 // ---------------------------------------------------------------------------
@@ -236,6 +321,48 @@ FSI_HD void mrt_collide(const float* F, float rho, float ux, float uy, float uz,
   add_invM(r, f);
 }
 
+// Regularised BGK (Latt & Chopard, Math. Comput. Simul. 72 (2006) 165):
+// f = feq + (1 - 1/tau) f1, with f1_s = w_s 9/2 (e_s e_s - I/3) : Pi_neq and
+// Pi_neq = sum_s (F_s - feq_s) e_s e_s, plus Guo's BGK forcing
+// (1 - 1/(2 tau)) w_s [3 (e_s - u) + 9 (e_s . u) e_s] . G.  No reference
+// counterpart: an option of the precision form, tau = 1/s_v.
+FSI_HD void regularized_collide(const float* F, float rho, float ux, float uy, float uz, bool has_force,
+                                float gx, float gy, float gz, float sv, float* f) {
+  float fe[Q];
+  float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) {
+    fe[s] = feq(s, rho, ux, uy, uz);
+    const float d = F[s] - fe[s];
+    const float ex = (float)EX[s], ey = (float)EY[s], ez = (float)EZ[s];
+    pxx += ex * ex * d;
+    pyy += ey * ey * d;
+    pzz += ez * ez * d;
+    pxy += ex * ey * d;
+    pxz += ex * ez * d;
+    pyz += ey * ez * d;
+  }
+  const float tr3 = (pxx + pyy + pzz) * (1.0f / 3.0f);
+  const float omega = 1.0f - sv;
+  const float fo = 1.0f - 0.5f * sv;
+#pragma unroll
+  for (int s = 0; s < Q; ++s) {
+    const float ex = (float)EX[s], ey = (float)EY[s], ez = (float)EZ[s];
+    // (e e - I/3) : Pi_neq
+    const float qpi = ex * ex * pxx + ey * ey * pyy + ez * ez * pzz +
+                      2.0f * (ex * ey * pxy + ex * ez * pxz + ey * ez * pyz) - tr3;
+    float v = fe[s] + omega * wgt(s) * 4.5f * qpi;
+    if (has_force) {
+      const float eu = ex * ux + ey * uy + ez * uz;
+      const float gxs = 3.0f * (ex - ux) + 9.0f * eu * ex;
+      const float gys = 3.0f * (ey - uy) + 9.0f * eu * ey;
+      const float gzs = 3.0f * (ez - uz) + 9.0f * eu * ez;
+      v += fo * wgt(s) * (gxs * gx + gys * gy + gzs * gz);
+    }
+    f[s] = v;
+  }
+}
+
 // streaming3 moments (fluid.py:1795-1814): rho = sum F, u = (sum e F)/rho + (force/2)/rho.
 FSI_HD void moments_of(const float* F, float gx, float gy, float gz, float& rho, float& ux,
                        float& uy, float& uz) {
@@ -328,13 +455,15 @@ FSI_HD float norm3(float x, float y, float z) { return sqrtf(x * x + y * y + z *
 // ---------------------------------------------------------------------------
 
 // init (fluid.py:329-344): F = feq(rho0, 0) in both buffers, rho = rho0, u = 0.
+template <typename Pop>
 FSI_HD void lbm_init_cell(const FsiLbmParams& p, const FsiLbmState& st, long long c) {
   const Grid g = make_grid(p);
+  const Pop F0 = pop_of<Pop>(st.F[0]), F1 = pop_of<Pop>(st.F[1]);
 #pragma unroll
   for (int s = 0; s < Q; ++s) {
     const float v = feq(s, p.rho0, 0.0f, 0.0f, 0.0f);
-    st.F[0][s * g.N + c] = v;
-    st.F[1][s * g.N + c] = v;
+    F0.put(g.N, s, c, v);
+    F1.put(g.N, s, c, v);
   }
   st.rho[c] = p.rho0;
   st.vel[c] = 0.0f;
@@ -351,6 +480,7 @@ FSI_HD bool new_solid_flag(const FsiLbmParams& p, const FsiLbmState& st, long lo
 // the link masks of the streaming passes.  Neighbour flags are evaluated from
 // static_solid / solid_phi directly, so every cell can be processed in one
 // pass without waiting for its neighbours' flag writes.
+template <typename Pop>
 FSI_HD void lbm_rebuild_cell(const FsiLbmParams& p, const FsiLbmState& st, int dynamic, int cur,
                              long long c) {
   const Grid g = make_grid(p);
@@ -358,8 +488,8 @@ FSI_HD void lbm_rebuild_cell(const FsiLbmParams& p, const FsiLbmState& st, int d
   coords_of(g, c, i, j, k);
   const bool prev = st.solid[c] != 0;
   const bool now = new_solid_flag(p, st, c, dynamic);
-  float* Fc = st.F[cur];
-  float* Fo = st.F[1 - cur];
+  const Pop Fc = pop_of<Pop>(st.F[cur]);
+  const Pop Fo = pop_of<Pop>(st.F[1 - cur]);
   if (dynamic) {
     st.old_solid[c] = prev ? 1 : 0;
     st.solid[c] = now ? 1 : 0;
@@ -370,8 +500,8 @@ FSI_HD void lbm_rebuild_cell(const FsiLbmParams& p, const FsiLbmState& st, int d
 #pragma unroll
       for (int s = 0; s < Q; ++s) {
         const float v = feq(s, p.rho0, vx, vy, vz);
-        Fc[s * g.N + c] = v;
-        Fo[s * g.N + c] = v;
+        Fc.put(g.N, s, c, v);
+        Fo.put(g.N, s, c, v);
       }
       st.rho[c] = p.rho0;
       st.vel[c] = vx;
@@ -380,7 +510,7 @@ FSI_HD void lbm_rebuild_cell(const FsiLbmParams& p, const FsiLbmState& st, int d
     } else if (!prev && now) {
       // fluid -> solid: the populations freeze at their current value
 #pragma unroll
-      for (int s = 0; s < Q; ++s) Fo[s * g.N + c] = Fc[s * g.N + c];
+      for (int s = 0; s < Q; ++s) Fo.put(g.N, s, c, Fc.get(g.N, s, c));
     }
   }
   uint32_t info = 0;
@@ -443,8 +573,9 @@ FSI_HD void wall_accumulate(const Grid& g, const FsiLbmState& st, long long y, d
 
 // Previous-step velocity of cell (i,j,k) as the reference's `v` field holds it
 // at Boundary_condition time: stored moments, or moments of F_src; 0 on solid.
+template <typename Pop>
 FSI_HD void prev_velocity(const FsiLbmParams& p, const FsiLbmState& st, const Grid& g,
-                          const float* Fs, uint32_t flags, long long c, float& ux, float& uy,
+                          const Pop& Fs, uint32_t flags, long long c, float& ux, float& uy,
                           float& uz) {
   if (st.cellinfo[c] & CI_SOLID) {
     ux = uy = uz = 0.0f;
@@ -458,7 +589,7 @@ FSI_HD void prev_velocity(const FsiLbmParams& p, const FsiLbmState& st, const Gr
   }
   float Fn[Q];
 #pragma unroll
-  for (int s = 0; s < Q; ++s) Fn[s] = Fs[s * g.N + c];
+  for (int s = 0; s < Q; ++s) Fn[s] = Fs.get(g.N, s, c);
   float gx = p.force[0], gy = p.force[1], gz = p.force[2];
   if (flags & FSI_SUB_CELL_FORCE) {
     gx += st.cell_force[c];
@@ -477,6 +608,7 @@ FSI_HD void prev_velocity(const FsiLbmParams& p, const FsiLbmState& st, const Gr
 // Every population of every fluid cell of F[1-cur] is written exactly once:
 // by the upstream fluid neighbour, by the cell's own bounce-back, or by the BC.
 // Wall (solid) cells gather the previous substep's link forces instead.
+template <typename Pop>
 FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, int parity,
                              uint32_t flags, long long c) {
   const Grid g = make_grid(p);
@@ -492,11 +624,11 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
     }
     return;
   }
-  const float* Fs = st.F[cur];
-  float* Fd = st.F[1 - cur];
+  const Pop Fs = pop_of<Pop>(st.F[cur]);
+  const Pop Fd = pop_of<Pop>(st.F[1 - cur]);
   float F[Q];
 #pragma unroll
-  for (int s = 0; s < Q; ++s) F[s] = Fs[s * g.N + c];
+  for (int s = 0; s < Q; ++s) F[s] = Fs.get(g.N, s, c);
   float gx = p.force[0], gy = p.force[1], gz = p.force[2];
   if (flags & FSI_SUB_CELL_FORCE) {
     gx += st.cell_force[c];
@@ -514,7 +646,10 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
     moments_of(F, gx, gy, gz, rho, ux, uy, uz);
   }
   float f[Q];
-  mrt_collide(F, rho, ux, uy, uz, has_force, gx, gy, gz, p.s_v, p.s_other, f);
+  if (p.collision == FSI_COLLIDE_REGULARIZED)
+    regularized_collide(F, rho, ux, uy, uz, has_force, gx, gy, gz, p.s_v, f);
+  else
+    mrt_collide(F, rho, ux, uy, uz, has_force, gx, gy, gz, p.s_v, p.s_other, f);
 
   const bool self_face = bc_face(p, i);
   float* link = mb ? st.link_c[parity] : nullptr;
@@ -529,7 +664,7 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
         corr = -6.0f * wgt(s) * rho * ((float)EX[s] * wx + (float)EY[s] * wy + (float)EZ[s] * wz);
       }
       const float bounced = f[s] + corr;
-      if (!self_face) Fd[LR[s] * g.N + c] = bounced;
+      if (!self_face) Fd.put(g.N, LR[s], c, bounced);
       if (mb) {
         link[(long long)(s - 1) * g.N + c] = f[s] + bounced;
         if (diag) st.link_corr[(long long)(s - 1) * g.N + c] = corr;
@@ -538,7 +673,7 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
       const int ti = wrap(i + EX[s], g.nx);
       if (!bc_face(p, ti)) {
         const long long y = cell_of(g, ti, wrap(j + EY[s], g.ny), wrap(k + EZ[s], g.nz));
-        Fd[s * g.N + y] = f[s];
+        Fd.put(g.N, s, y, f[s]);
       }
     }
   }
@@ -577,11 +712,12 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
       }
     }
 #pragma unroll
-    for (int s = 0; s < Q; ++s) Fd[s * g.N + c] = feq(s, brho, bx, by, bz);
+    for (int s = 0; s < Q; ++s) Fd.put(g.N, s, c, feq(s, brho, bx, by, bz));
   }
 }
 
 // streaming3 (fluid.py:1795-1814) on the populations in F[cur].
+template <typename Pop>
 FSI_HD void lbm_moments_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, uint32_t flags,
                              long long c) {
   const Grid g = make_grid(p);
@@ -593,9 +729,9 @@ FSI_HD void lbm_moments_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
     return;
   }
   float F[Q];
-  const float* Fs = st.F[cur];
+  const Pop Fs = pop_of<Pop>(st.F[cur]);
 #pragma unroll
-  for (int s = 0; s < Q; ++s) F[s] = Fs[s * g.N + c];
+  for (int s = 0; s < Q; ++s) F[s] = Fs.get(g.N, s, c);
   float gx = p.force[0], gy = p.force[1], gz = p.force[2];
   if (flags & FSI_SUB_CELL_FORCE) {
     gx += st.cell_force[c];

@@ -21,6 +21,16 @@ int fail(int code, const char* m) {
   return code;
 }
 long long ncells(const FsiLbmParams* p) { return (long long)p->nx * p->ny * p->nz; }
+
+template <typename F>
+void by_storage(int code, F&& fn) {
+  if (code == FSI_STORE_F16_DEV)
+    fn(PopF16{});
+  else if (code == FSI_STORE_BF16_DEV)
+    fn(PopBF16{});
+  else
+    fn(PopF32{});
+}
 }  // namespace
 
 extern "C" {
@@ -41,7 +51,10 @@ int fsi_abi_sizes(int32_t* out, int n) {
 int fsi_sync(void*) { return FSI_OK; }
 
 int fsi_lbm_init(const FsiLbmParams* p, const FsiLbmState* st, void*) {
-  for (long long c = 0; c < ncells(p); ++c) lbm_init_cell(*p, *st, c);
+  by_storage(p->storage, [&](auto tag) {
+    using Pop = decltype(tag);
+    for (long long c = 0; c < ncells(p); ++c) lbm_init_cell<Pop>(*p, *st, c);
+  });
   return FSI_OK;
 }
 
@@ -50,18 +63,27 @@ int fsi_lbm_rebuild(const FsiLbmParams* p, const FsiLbmState* st, int dynamic, i
   // The GPU evaluates every cell from the pre-pass flags; emulate that by
   // snapshotting `solid` (rebuild reads neighbours' new flags from
   // static_solid / solid_phi only, and its own previous flag).
-  for (long long c = 0; c < ncells(p); ++c) lbm_rebuild_cell(*p, *st, dynamic, cur, c);
+  by_storage(p->storage, [&](auto tag) {
+    using Pop = decltype(tag);
+    for (long long c = 0; c < ncells(p); ++c) lbm_rebuild_cell<Pop>(*p, *st, dynamic, cur, c);
+  });
   return FSI_OK;
 }
 
 int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity, uint32_t flags, void*) {
   if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
-  for (long long c = 0; c < ncells(p); ++c) lbm_substep_cell(*p, *st, cur, parity, flags, c);
+  by_storage(p->storage, [&](auto tag) {
+    using Pop = decltype(tag);
+    for (long long c = 0; c < ncells(p); ++c) lbm_substep_cell<Pop>(*p, *st, cur, parity, flags, c);
+  });
   return FSI_OK;
 }
 
 int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags, void*) {
-  for (long long c = 0; c < ncells(p); ++c) lbm_moments_cell(*p, *st, cur, flags, c);
+  by_storage(p->storage, [&](auto tag) {
+    using Pop = decltype(tag);
+    for (long long c = 0; c < ncells(p); ++c) lbm_moments_cell<Pop>(*p, *st, cur, flags, c);
+  });
   return FSI_OK;
 }
 

@@ -36,12 +36,31 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB)
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Run STMT with POP bound to the population accessor of a storage code.
+#define FSI_BY_STORAGE(code, STMT)          \
+  do {                                      \
+    if ((code) == FSI_STORE_F16_DEV) {      \
+      using POP = PopF16;                   \
+      STMT;                                 \
+    } else if ((code) == FSI_STORE_BF16_DEV) { \
+      using POP = PopBF16;                  \
+      STMT;                                 \
+    } else {                                \
+      using POP = PopF32;                   \
+      STMT;                                 \
+    }                                       \
+  } while (0)
+
 int check_lbm(const FsiLbmParams* p, const FsiLbmState* st) {
   if (!p || !st) return fail(FSI_E_INVALID, "null params/state");
   if (p->nx < 3 || p->ny < 3 || p->nz < 3)
     return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
   if (p->bc_x_left < 0 || p->bc_x_left > 2 || p->bc_x_right < 0 || p->bc_x_right > 2)
     return fail(FSI_E_INVALID, "bc_x_left/bc_x_right must be 0, 1 or 2");
+  if (p->storage < FSI_STORE_F32 || p->storage > FSI_STORE_BF16_DEV)
+    return fail(FSI_E_INVALID, "storage must be FSI_STORE_F32, FSI_STORE_F16_DEV or FSI_STORE_BF16_DEV");
+  if (p->collision != FSI_COLLIDE_MRT && p->collision != FSI_COLLIDE_REGULARIZED)
+    return fail(FSI_E_INVALID, "collision must be FSI_COLLIDE_MRT or FSI_COLLIDE_REGULARIZED");
   if (!st->F[0] || !st->F[1] || !st->rho || !st->vel || !st->cellinfo || !st->solid)
     return fail(FSI_E_INVALID, "null LBM buffer");
   return FSI_OK;
@@ -111,27 +130,31 @@ __device__ __forceinline__ int block_sum_i(int v, int* smem) {
 // ---------------------------------------------------------------------------
 // LBM kernels
 // ---------------------------------------------------------------------------
+template <typename Pop>
 __global__ void __launch_bounds__(TPB) k_lbm_init(FsiLbmParams p, FsiLbmState st) {
   const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
-  if (c < (long long)p.nx * p.ny * p.nz) lbm_init_cell(p, st, c);
+  if (c < (long long)p.nx * p.ny * p.nz) lbm_init_cell<Pop>(p, st, c);
 }
 
+template <typename Pop>
 __global__ void __launch_bounds__(TPB) k_lbm_rebuild(FsiLbmParams p, FsiLbmState st, int dynamic, int cur) {
   const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
-  if (c < (long long)p.nx * p.ny * p.nz) lbm_rebuild_cell(p, st, dynamic, cur, c);
+  if (c < (long long)p.nx * p.ny * p.nz) lbm_rebuild_cell<Pop>(p, st, dynamic, cur, c);
 }
 
-template <uint32_t FIXED>
+template <typename Pop, uint32_t FIXED>
 __global__ void __launch_bounds__(TPB) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
                                                      uint32_t flags) {
   const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
   if (c < (long long)p.nx * p.ny * p.nz)
-    lbm_substep_cell(p, st, cur, parity, (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED, c);
+    lbm_substep_cell<Pop>(p, st, cur, parity,
+                          (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED, c);
 }
 
+template <typename Pop>
 __global__ void __launch_bounds__(TPB) k_lbm_moments(FsiLbmParams p, FsiLbmState st, int cur, uint32_t flags) {
   const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
-  if (c < (long long)p.nx * p.ny * p.nz) lbm_moments_cell(p, st, cur, flags, c);
+  if (c < (long long)p.nx * p.ny * p.nz) lbm_moments_cell<Pop>(p, st, cur, flags, c);
 }
 
 __global__ void __launch_bounds__(TPB) k_lbm_mb_begin(FsiLbmParams p, FsiLbmState st) {
@@ -429,7 +452,7 @@ int fsi_lbm_init(const FsiLbmParams* p, const FsiLbmState* st, void* stream) {
   int rc = check_lbm(p, st);
   if (rc) return rc;
   const long long N = (long long)p->nx * p->ny * p->nz;
-  hipLaunchKernelGGL(k_lbm_init, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st);
+  FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL(k_lbm_init<POP>, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st));
   return check_launch("k_lbm_init");
 }
 
@@ -441,7 +464,8 @@ int fsi_lbm_rebuild(const FsiLbmParams* p, const FsiLbmState* st, int dynamic, i
     return fail(FSI_E_INVALID, "dynamic rebuild needs static_solid, old_solid, solid_phi, solid_vel");
   if (!st->solid_vel) return fail(FSI_E_INVALID, "rebuild needs solid_vel");
   const long long N = (long long)p->nx * p->ny * p->nz;
-  hipLaunchKernelGGL(k_lbm_rebuild, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st, dynamic, cur);
+  FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL(k_lbm_rebuild<POP>, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p,
+                                                *st, dynamic, cur));
   return check_launch("k_lbm_rebuild");
 }
 
@@ -464,7 +488,7 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
   switch (fixed) {
 #define FSI_CASE(V)                                                                   \
   case (V):                                                                           \
-    hipLaunchKernelGGL(k_lbm_substep<(V)>, grid, block, 0, s, *p, *st, cur, parity, flags); \
+    FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V)>), grid, block, 0, s, *p, *st, cur, parity, flags)); \
     break;
     FSI_CASE(0u)
     FSI_CASE(1u)
@@ -487,7 +511,8 @@ int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint3
   if (cur != 0 && cur != 1) return fail(FSI_E_INVALID, "cur must be 0 or 1");
   if ((flags & FSI_SUB_CELL_FORCE) && !st->cell_force) return fail(FSI_E_INVALID, "cell_force is null");
   const long long N = (long long)p->nx * p->ny * p->nz;
-  hipLaunchKernelGGL(k_lbm_moments, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st, cur, flags);
+  FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL(k_lbm_moments<POP>, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p,
+                                                *st, cur, flags));
   return check_launch("k_lbm_moments");
 }
 

@@ -27,6 +27,12 @@ FSI_BC_PERIODIC = 0
 FSI_BC_PRESSURE = 1
 FSI_BC_VELOCITY = 2
 
+STORE_F32 = 0
+STORE_F16_DEV = 1
+STORE_BF16_DEV = 2
+COLLIDE_MRT = 0
+COLLIDE_REGULARIZED = 1
+
 SUB_STORED_MOMENTS = 1
 SUB_MOVING_WALL = 2
 SUB_GATHER_PREV = 4
@@ -45,6 +51,7 @@ class LbmParams(ctypes.Structure):
         ("rho_bc_x_left", f32), ("rho_bc_x_right", f32),
         ("vel_bc_x_left", f32 * 3), ("vel_bc_x_right", f32 * 3),
         ("dynamic_solid_threshold", f32),
+        ("storage", i32), ("collision", i32),
     ]
 
 
@@ -226,6 +233,10 @@ def load_hip() -> Native:
     """Load the gfx950 library built in-tree; raise if it is missing."""
     global _HIP
     if _HIP is None:
+        # torch first: its bundled libamdhip64.so.7 must be the one this
+        # library binds to, so both share one HIP runtime and its streams.
+        import torch  # noqa: F401
+
         if not HIP_LIB_PATH.is_file():
             raise FsiError(
                 f"{HIP_LIB_PATH} is not built: run __graft_entry__.build() (hipcc --offload-arch=gfx950) first"

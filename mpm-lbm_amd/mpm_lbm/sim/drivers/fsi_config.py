@@ -2,7 +2,9 @@
 
 Keys, defaults, validation order and error messages follow the reference
 (sim/drivers/fsi_config.py:67-497) so existing JSON configs and callers work
-unchanged.  `lattice_shape` is an extension for non-cubic lattices.
+unchanged.  Extensions: `lattice_shape` (non-cubic lattices),
+`lbm_population_storage` ("f32" | "f16" | "bf16" deviation storage) and
+`lbm_collision` ("mrt" | "regularized").
 """
 from dataclasses import asdict, dataclass
 import json
@@ -157,6 +159,8 @@ class FSIDriverConfig:
     quality_check_strict: bool = False
     quality_report_path: Optional[str] = None
     lattice_shape: Optional[Tuple[int, int, int]] = None
+    lbm_population_storage: str = "f32"
+    lbm_collision: str = "mrt"
 
     def __post_init__(self):
         for name in _TUPLE3_FIELDS:
@@ -300,6 +304,8 @@ class FSIDriverConfig:
         _validate_optional_positive(self.fluent_like_monitor_radius_m, "fluent_like_monitor_radius_m")
         if self.fluent_like_monitor_nearest_count <= 0:
             raise ValueError("fluent_like_monitor_nearest_count must be positive")
+        _one_of(self.lbm_population_storage, ("f32", "f16", "bf16"), "lbm_population_storage")
+        _one_of(self.lbm_collision, ("mrt", "regularized"), "lbm_collision")
         if self.lattice_shape is not None:
             if len(self.lattice_shape) != 3 or min(self.lattice_shape) <= 0:
                 raise ValueError("lattice_shape must hold three positive integers")
@@ -469,6 +475,8 @@ class FSIDriverConfig:
             open_boundary_noneq_cap=float(self.open_boundary_noneq_cap),
             open_boundary_population_floor=self.open_boundary_population_floor,
             lattice_shape=self.lattice_shape,
+            lbm_population_storage=self.lbm_population_storage,
+            lbm_collision=self.lbm_collision,
         )
 
     def make_mpm_control_overrides(self):

@@ -61,7 +61,7 @@ FLUENT_LIKE_FIELDS = [
 
 class FSIDriver3D:
     def __init__(self, config: FSIDriverConfig, out_dir: str, particle_cloud=None, native=None, device=None,
-                 write_geometry_file: bool = True):
+                 write_geometry_file: bool = True, use_graph: bool = True):
         self.config = config
         self.out_dir = out_dir
         self.sim = config.make_unified_sim_config()
@@ -99,6 +99,11 @@ class FSIDriver3D:
         self._device = device
         self._write_geometry_file = write_geometry_file
         self._geometry_config_cache = None
+        # The subcycled FSI step is replayed from a captured HIP graph after
+        # one eager step (allocations happen then); see _step_moving_boundary_subcycled.
+        self.use_graph = bool(use_graph)
+        self._fsi_graph = None
+        self._eager_subcycled_steps = 0
 
     # ------------------------------------------------------------- set-up
     def _unsupported_features(self):
@@ -357,14 +362,48 @@ class FSIDriver3D:
         self._advance_mpm_with_moving_boundary_reaction()
 
     def _step_moving_boundary_subcycled(self):
-        """fsi_driver.py:533-550, issued as one fused device sequence."""
+        """fsi_driver.py:533-550, issued as one fused device sequence.
+
+        Every FSI step launches the same kernels with the same arguments (the
+        population buffers swap an even number of times per step), so after
+        one eager step the whole step - projection, flag rebuild, the fused
+        substeps, finalize, moments and the MPM substeps - is captured once
+        as a HIP graph and replayed."""
+        n = self.config.lbm_substeps_per_fsi_step
+        if self._graph_ready():
+            t0 = time.perf_counter()
+            if self._fsi_graph is None:
+                self._capture_subcycled_step()
+            self._fsi_graph.replay()
+            self.total_lbm_substeps += n
+            self.total_mpm_substeps += self.config.mpm_substeps_per_lbm_step
+            self.timing["lbm_step_time"] += time.perf_counter() - t0
+            return
         self._project()
         t0 = time.perf_counter()
-        n = self.config.lbm_substeps_per_fsi_step
         self.lbm.run_moving_boundary_subcycle(n, self.config.dynamic_solid_threshold, accumulate=True)
         self.total_lbm_substeps += n
         self.timing["lbm_step_time"] += time.perf_counter() - t0
         self._advance_mpm_with_moving_boundary_reaction()
+        self._eager_subcycled_steps += 1
+
+    def _graph_ready(self) -> bool:
+        return (self.use_graph and self.lbm.device.type == "cuda" and self._eager_subcycled_steps >= 1
+                and self.config.lbm_substeps_per_fsi_step % 2 == 0)
+
+    def _capture_subcycled_step(self):
+        import torch
+
+        torch.cuda.synchronize(self.lbm.device)
+        graph = torch.cuda.CUDAGraph()
+        counted = self.total_mpm_substeps
+        with torch.cuda.graph(graph):
+            self.projector.project(self.solid, self.lbm)
+            self.lbm.run_moving_boundary_subcycle(self.config.lbm_substeps_per_fsi_step,
+                                                  self.config.dynamic_solid_threshold, accumulate=True)
+            self._advance_mpm_with_moving_boundary_reaction()
+        self.total_mpm_substeps = counted  # capture records the kernels without running them
+        self._fsi_graph = graph
 
     def _advance_mpm_with_moving_boundary_reaction(self):
         """fsi_driver.py:552-575 (engineering reaction transfer)."""

@@ -30,6 +30,8 @@ class UnifiedSimConfig:
     open_boundary_population_floor: Optional[float] = None
     lbm_dt_phys_override_s: Optional[float] = None
     lattice_shape: Optional[Tuple[int, int, int]] = None
+    lbm_population_storage: str = "f32"
+    lbm_collision: str = "mrt"
 
     def __post_init__(self):
         checks = (
@@ -99,6 +101,8 @@ class UnifiedSimConfig:
             "open_boundary_u_max": self.open_boundary_u_max,
             "open_boundary_noneq_cap": self.open_boundary_noneq_cap,
             "open_boundary_population_floor": self.open_boundary_population_floor,
+            "population_storage": self.lbm_population_storage,
+            "collision": self.lbm_collision,
         }
         values.update(overrides)
         return LBMConfig(**values)

@@ -28,6 +28,8 @@ VALID_OPEN_BOUNDARY_MASS_NEUTRAL_FLUX_CONTROL_MODES = (
     "outlet_population_projection_report_only",
 )
 VALID_OPEN_BOUNDARY_MASS_NEUTRAL_REFERENCE_MASS_MODES = ("initial",)
+VALID_POPULATION_STORAGE = ("f32", "f16", "bf16")
+VALID_COLLISIONS = ("mrt", "regularized")
 
 
 @dataclass
@@ -82,8 +84,15 @@ class LBMConfig:
     vel_bc_y_right: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     vel_bc_z_left: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     vel_bc_z_right: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    # Extensions (no reference counterpart): population storage and collision operator.
+    population_storage: str = "f32"  # "f32" | "f16" | "bf16" (16-bit: deviations F - w from the rest weights)
+    collision: str = "mrt"  # "mrt" (the reference's) | "regularized"
 
     def __post_init__(self):
+        if self.population_storage not in VALID_POPULATION_STORAGE:
+            raise ValueError(f"population_storage must be one of {VALID_POPULATION_STORAGE}")
+        if self.collision not in VALID_COLLISIONS:
+            raise ValueError(f"collision must be one of {VALID_COLLISIONS}")
         checks = (
             (self.relaxation_semantics in VALID_LBM_RELAXATION_SEMANTICS,
              f"relaxation_semantics must be one of {VALID_LBM_RELAXATION_SEMANTICS}"),

@@ -49,6 +49,39 @@ def mrt_rates(tau: float):
     return s_v, 8.0 * (2.0 - s_v) / (8.0 - s_v)
 
 
+class _PopulationView(FieldView):
+    """F / f (post-stream populations, (nx,ny,nz,19)) decoded from the storage format."""
+
+    def __init__(self, lbm, space_shape):
+        super().__init__(lambda: lbm._F[lbm.cur], space_shape, 19, after_write=lbm._populations_written)
+        self._lbm = lbm
+
+    def _decoded(self):
+        lbm = self._lbm
+        raw = lbm._F[lbm.cur]
+        if lbm.population_storage == "f32":
+            return raw
+        half = torch.float16 if lbm.population_storage == "f16" else torch.bfloat16
+        return raw.view(half).to(torch.float32) + lbm._w_col
+
+    def to_numpy(self):
+        t = self._decoded().reshape((19,) + self.space_shape)
+        return t.permute(1, 2, 3, 0).contiguous().cpu().numpy()
+
+    def from_numpy(self, arr):
+        a = np.asarray(arr, dtype=np.float32)
+        if a.shape != self.shape:
+            raise ValueError(f"expected array of shape {self.shape}, got {a.shape}")
+        lbm = self._lbm
+        src = torch.from_numpy(np.ascontiguousarray(a)).to(lbm.device).reshape(-1, 19).t()
+        if lbm.population_storage == "f32":
+            lbm._F[lbm.cur].copy_(src)
+        else:
+            half = torch.float16 if lbm.population_storage == "f16" else torch.bfloat16
+            lbm._F[lbm.cur].copy_((src - lbm._w_col).to(half).view(torch.int16))
+        lbm._populations_written()
+
+
 class LBMFluid3D:
     def __init__(self, config: LBMConfig, native: _native.Native | None = None, device=None):
         if config.sparse_storage:
@@ -86,7 +119,13 @@ class LBMFluid3D:
 
         N, dev = self.N, self.device
         f32 = dict(dtype=torch.float32, device=dev)
-        self._F = [torch.zeros((19, N), **f32), torch.zeros((19, N), **f32)]
+        self.population_storage = getattr(config, "population_storage", "f32")
+        self.collision = getattr(config, "collision", "mrt")
+        self._storage_code = {"f32": _native.STORE_F32, "f16": _native.STORE_F16_DEV,
+                              "bf16": _native.STORE_BF16_DEV}[self.population_storage]
+        pop_dtype = torch.float32 if self.population_storage == "f32" else torch.int16
+        self._F = [torch.zeros((19, N), dtype=pop_dtype, device=dev), torch.zeros((19, N), dtype=pop_dtype, device=dev)]
+        self._w_col = torch.tensor(W, dtype=torch.float32, device=dev)[:, None]
         self._rho = torch.full((N,), float(self.rho0), **f32)
         self._vel = torch.zeros((3, N), **f32)
         self._solid = torch.zeros((N,), dtype=torch.int8, device=dev)
@@ -128,8 +167,8 @@ class LBMFluid3D:
     # ------------------------------------------------------------------ setup
     def _make_views(self):
         sp = (self.nx, self.ny, self.nz)
-        self.F = FieldView(lambda: self._F[self.cur], sp, 19, after_write=self._populations_written)
-        self.f = FieldView(lambda: self._F[self.cur], sp, 19, after_write=self._populations_written)
+        self.F = _PopulationView(self, sp)
+        self.f = self.F
         self.rho = FieldView(lambda: self._rho, sp, before_read=self._sync_moments, after_write=self._moments_written)
         self.v = FieldView(lambda: self._vel, sp, 3, before_read=self._sync_moments, after_write=self._moments_written)
         self.solid = FieldView(lambda: self._solid, sp, after_write=self._flags_written, np_dtype=np.int8)
@@ -185,6 +224,8 @@ class LBMFluid3D:
         prm.vel_bc_x_left[:] = [float(v) for v in self.bc_vel_x_left]
         prm.vel_bc_x_right[:] = [float(v) for v in c.vel_bc_x_right]
         prm.dynamic_solid_threshold = 0.5
+        prm.storage = self._storage_code
+        prm.collision = _native.COLLIDE_REGULARIZED if self.collision == "regularized" else _native.COLLIDE_MRT
         self._params = prm
 
     def _stream(self):

@@ -68,3 +68,34 @@ def test_gpu_driver_run_writes_outputs(hip, tmp_path):
     assert rows[-1]["cell_force_max_norm"] == 0.0
     assert (tmp_path / "out" / "diagnostics_timeseries.csv").is_file()
     assert (tmp_path / "out" / "flap_tip_displacement_timeseries.csv").is_file()
+
+
+def test_gpu_graph_replay_matches_eager(hip, tmp_path):
+    """The captured FSI-step graph against eager launches, every step, with
+    eager diagnostics work between replays."""
+    from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
+    from tests.parity import rel_max_err
+
+    w = workloads.moving_boundary_flap(n=32, steps=5, substeps=20, flap=(2, 16, 12), out_dir=str(tmp_path))
+    eager = FSIDriver3D(w.config, str(tmp_path / "eager"), particle_cloud=w.cloud, use_graph=False,
+                        write_geometry_file=False)
+    graph = FSIDriver3D(w.config, str(tmp_path / "graph"), particle_cloud=w.cloud, use_graph=True,
+                        write_geometry_file=False)
+    eager.initialize()
+    graph.initialize()
+    for step in range(5):
+        eager.step_once()
+        graph.step_once()
+        assert (graph._fsi_graph is not None) == (step >= 1)
+        re, rg = eager.collect_diagnostics(step), graph.collect_diagnostics(step)
+        for key in ("bb_link_count", "active_cell_count", "active_reaction_particle_count", "total_mpm_substeps"):
+            assert re[key] == rg[key], (step, key)
+        errs = {
+            "F": rel_max_err(graph.lbm.F.to_numpy(), eager.lbm.F.to_numpy()),
+            "hydro": rel_max_err(graph.lbm.hydro_force.to_numpy(), eager.lbm.hydro_force.to_numpy()),
+            "x": rel_max_err(graph.solid.x.to_numpy(), eager.solid.x.to_numpy()),
+            "v": rel_max_err(graph.solid.v.to_numpy(), eager.solid.v.to_numpy()),
+        }
+        print("graph vs eager", step, errs, flush=True)
+        assert all(v <= 1e-6 for v in errs.values()), (step, errs)
+    assert graph.total_lbm_substeps == eager.total_lbm_substeps == 5 * 20
