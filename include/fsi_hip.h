@@ -173,11 +173,72 @@ typedef struct FsiMpmStats {
   double mass_weighted_v[3];
 } FsiMpmStats;
 
+/* ---------------------------------------------------------------------------
+ * Batched form: many independent FSI instances (mixed sizes and modes) in one
+ * launch per operation.  `inst` is a device array of descriptors; each launch
+ * covers the concatenated work items of the instances selected by `mask`,
+ * through a device table `block_start` (n_inst + 1 entries, exclusive prefix
+ * sum of each instance's workgroup count for that operation's domain: cells,
+ * MPM nodes or particles; built by fsi_batch_block_table).  No reference
+ * counterpart: the reference runs one instance per process.
+ * ------------------------------------------------------------------------- */
+#define FSI_BATCH_MODE_NONE 0
+#define FSI_BATCH_MODE_PENALTY 1
+#define FSI_BATCH_MODE_MOVING_BOUNDARY 2
+
+#define FSI_BATCH_DOMAIN_CELLS 0
+#define FSI_BATCH_DOMAIN_NODES 1
+#define FSI_BATCH_DOMAIN_PARTICLES 2
+
+typedef struct FsiBatchInstance {
+  FsiLbmParams lp;
+  FsiLbmState ls;
+  FsiMpmParams mp;
+  FsiMpmState ms;
+  FsiCouplingParams cp;
+  FsiCouplingCounters* cc;
+  int32_t mode;       /* FSI_BATCH_MODE_* */
+  int32_t cur0;       /* buffer holding F at substep 0 of the current exchange */
+  int32_t n_substeps; /* LBM substeps per exchange (1 for penalty / none) */
+  int32_t mpm_substeps;
+  int32_t active;     /* 0 = slot unused */
+  int32_t cell_force; /* 1 = add cell_force (penalty) */
+} FsiBatchInstance;
+
+/* Operations of fsi_batch_launch. */
+#define FSI_BATCH_OP_LBM_SUBSTEP 0  /* one fused substep (t = substep index within the exchange) */
+#define FSI_BATCH_OP_LBM_REBUILD 1  /* rebuild flags/links (dynamic for moving-boundary instances) */
+#define FSI_BATCH_OP_LBM_MOMENTS 2
+#define FSI_BATCH_OP_LBM_MB_BEGIN 3
+#define FSI_BATCH_OP_LBM_MB_FINALIZE 4
+#define FSI_BATCH_OP_PROJ_CLEAR 5
+#define FSI_BATCH_OP_PROJ_PARTICLES 6
+#define FSI_BATCH_OP_PROJ_NORMALIZE 7
+#define FSI_BATCH_OP_PENALTY 8
+#define FSI_BATCH_OP_MPM_CLEAR 9
+#define FSI_BATCH_OP_MPM_P2G 10
+#define FSI_BATCH_OP_MPM_REACTION 11
+#define FSI_BATCH_OP_MPM_GRID_UPDATE 12
+#define FSI_BATCH_OP_MPM_G2P 13
+#define FSI_BATCH_OP_COUNTERS_CLEAR 14 /* zero the coupling counters (one work item per instance) */
+
+/* Fill block_start (host memory, n_inst + 1 entries) for one domain from the
+ * host copy of the descriptors, counting only instances whose mode bit is in
+ * `mode_mask` (bit m = 1 << FSI_BATCH_MODE_m) and that are active.  Returns
+ * the total workgroup count. */
+int fsi_batch_block_table(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask,
+                          int32_t* block_start_host);
+
+/* One batched operation over the instances counted in block_start.  `t` is
+ * the substep index within the exchange (LBM_SUBSTEP) or unused. */
+int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const int32_t* block_start_dev,
+                     int n_blocks, int t, void* stream);
+
 const char* fsi_last_error(void);
 int fsi_version(void);
 /* Writes sizeof of FsiLbmParams, FsiLbmState, FsiLbmCounters, FsiLbmStats,
  * FsiMpmParams, FsiMpmState, FsiCouplingParams, FsiCouplingCounters,
- * FsiMpmStats (in that order, up to n entries) so bindings can verify their
+ * FsiMpmStats, FsiBatchInstance (in that order, up to n entries) so bindings can verify their
  * struct layouts.  Returns the number of entries written. */
 int fsi_abi_sizes(int32_t* out, int n);
 int fsi_sync(void* stream);

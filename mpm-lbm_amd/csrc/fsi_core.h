@@ -1434,4 +1434,33 @@ FSI_HD void mpm_stats_particle(const FsiMpmParams& mp, const FsiMpmState& ms, lo
   a[12] += m * vz;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched form: per-instance substep semantics shared by the GPU launch and
+// the serial emulation.
+// ---------------------------------------------------------------------------
+FSI_HD long long batch_domain_size(const FsiBatchInstance& I, int domain) {
+  if (domain == FSI_BATCH_DOMAIN_CELLS) return (long long)I.lp.nx * I.lp.ny * I.lp.nz;
+  if (domain == FSI_BATCH_DOMAIN_NODES) return (long long)I.mp.gnx * I.mp.gny * I.mp.gnz;
+  return (long long)I.mp.n_particles;
+}
+
+// Flags of instance I at substep t of the exchange: moving-boundary instances
+// collide with stored moments at t = 0 and gather the previous substep's wall
+// forces after it, recording link diagnostics on the last substep; penalty /
+// none instances refresh their moments after every substep.
+FSI_HD uint32_t batch_substep_flags(const FsiBatchInstance& I, int t) {
+  uint32_t f = I.cell_force ? FSI_SUB_CELL_FORCE : 0u;
+  if (I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY) {
+    f |= FSI_SUB_MOVING_WALL;
+    f |= t == 0 ? FSI_SUB_STORED_MOMENTS : FSI_SUB_GATHER_PREV;
+    if (t == I.n_substeps - 1) f |= FSI_SUB_DIAG;
+  } else {
+    f |= FSI_SUB_STORED_MOMENTS;
+  }
+  return f;
+}
+
+FSI_HD int batch_cur(const FsiBatchInstance& I, int t) { return I.cur0 ^ (t & 1); }
+
 }  // namespace fsi

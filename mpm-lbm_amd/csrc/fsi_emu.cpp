@@ -39,13 +39,13 @@ const char* fsi_last_error(void) { return g_err.c_str(); }
 int fsi_version(void) { return 1; }
 
 int fsi_abi_sizes(int32_t* out, int n) {
-  const int32_t sz[9] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
+  const int32_t sz[10] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
                          (int32_t)sizeof(FsiLbmCounters),    (int32_t)sizeof(FsiLbmStats),
                          (int32_t)sizeof(FsiMpmParams),      (int32_t)sizeof(FsiMpmState),
                          (int32_t)sizeof(FsiCouplingParams), (int32_t)sizeof(FsiCouplingCounters),
-                         (int32_t)sizeof(FsiMpmStats)};
+                         (int32_t)sizeof(FsiMpmStats),       (int32_t)sizeof(FsiBatchInstance)};
   int k = 0;
-  for (; k < n && k < 9; ++k) out[k] = sz[k];
+  for (; k < n && k < 10; ++k) out[k] = sz[k];
   return k;
 }
 int fsi_sync(void*) { return FSI_OK; }
@@ -209,6 +209,161 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
   double* a = reinterpret_cast<double*>(out);
   for (int f = 0; f < MPM_STATS_N; ++f) a[f] = red_identity(mpm_stats_op(f));
   for (long long q = 0; q < mp->n_particles; ++q) mpm_stats_particle(*mp, *ms, q, a);
+  return FSI_OK;
+}
+
+
+int fsi_batch_block_table(const FsiBatchInstance* inst, int n_inst, int domain, int mode_mask, int32_t* bs) {
+  const int TPB = 256;
+  long long total = 0;
+  bs[0] = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    long long nb = 0;
+    if (inst[i].active && (mode_mask & (1 << inst[i].mode))) nb = (batch_domain_size(inst[i], domain) + TPB - 1) / TPB;
+    total += nb;
+    bs[i + 1] = (int32_t)total;
+  }
+  return (int)total;
+}
+
+// Serial emulation of fsi_batch_launch: instance by instance, work item by
+// work item, in the order of the block table.
+int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int32_t* bs, int, int t, void*) {
+  if (op == FSI_BATCH_OP_COUNTERS_CLEAR) {
+    for (int i = 0; i < n_inst; ++i) {
+      if (!inst[i].active) continue;
+      FsiCouplingCounters* cc = inst[i].cc;
+      if (t == 0) {
+        cc->projected_mass = cc->projected_volume_raw = cc->projected_volume_clamped = 0.0;
+        cc->max_phi_raw_bits = 0;
+        cc->active_cell_count = 0;
+      } else if (t == 1) {
+        cc->active_force_cell_count = 0;
+        cc->max_cell_force_bits = cc->max_hydro_force_bits = 0;
+        for (int d = 0; d < 3; ++d) cc->net_cell_force[d] = cc->net_hydro_force[d] = 0.0;
+      } else if (t == 2) {
+        cc->active_reaction_particle_count = 0;
+        cc->max_particle_reaction_bits = cc->max_grid_reaction_bits = 0;
+        for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] = cc->net_grid_reaction[d] = 0.0;
+      } else {
+        FsiLbmCounters* k = inst[i].ls.counters;
+        k->bb_link_count = 0;
+        k->bb_max_correction_bits = 0;
+        for (int d = 0; d < 3; ++d) k->bb_net_solid_force[d] = 0.0;
+      }
+    }
+    return FSI_OK;
+  }
+  for (int i = 0; i < n_inst; ++i) {
+    if (bs[i + 1] == bs[i]) continue;
+    const FsiBatchInstance& I = inst[i];
+    const long long cells = batch_domain_size(I, FSI_BATCH_DOMAIN_CELLS);
+    const long long nodes = batch_domain_size(I, FSI_BATCH_DOMAIN_NODES);
+    const long long parts = batch_domain_size(I, FSI_BATCH_DOMAIN_PARTICLES);
+    switch (op) {
+      case FSI_BATCH_OP_LBM_SUBSTEP:
+        by_storage(I.lp.storage, [&](auto tag) {
+          using Pop = decltype(tag);
+          for (long long c = 0; c < cells; ++c)
+            lbm_substep_cell<Pop>(I.lp, I.ls, batch_cur(I, t), t & 1, batch_substep_flags(I, t), c);
+        });
+        break;
+      case FSI_BATCH_OP_LBM_REBUILD:
+        by_storage(I.lp.storage, [&](auto tag) {
+          using Pop = decltype(tag);
+          for (long long c = 0; c < cells; ++c)
+            lbm_rebuild_cell<Pop>(I.lp, I.ls, I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 1 : 0, I.cur0, c);
+        });
+        break;
+      case FSI_BATCH_OP_LBM_MOMENTS:
+        by_storage(I.lp.storage, [&](auto tag) {
+          using Pop = decltype(tag);
+          for (long long c = 0; c < cells; ++c)
+            lbm_moments_cell<Pop>(I.lp, I.ls, batch_cur(I, t + 1), I.cell_force ? FSI_SUB_CELL_FORCE : 0u, c);
+        });
+        break;
+      case FSI_BATCH_OP_LBM_MB_BEGIN:
+        memset(I.ls.counters, 0, sizeof(FsiLbmCounters));
+        for (long long c = 0; c < cells; ++c) lbm_mb_begin_cell(I.lp, I.ls, c);
+        break;
+      case FSI_BATCH_OP_LBM_MB_FINALIZE: {
+        FsiLbmCounters* k = I.ls.counters;
+        for (long long c = 0; c < cells; ++c) {
+          FinalizeOut o = lbm_finalize_cell(I.lp, I.ls, (I.n_substeps - 1) & 1, I.n_substeps, 1, c);
+          k->bb_link_count += o.links;
+          atomic_max_nonneg(&k->bb_max_correction_bits, o.max_corr);
+          atomic_max_nonneg(&k->mean_norm_max_bits, o.mean_norm);
+          atomic_max_nonneg(&k->accum_norm_max_bits, o.runmax);
+          k->bb_net_solid_force[0] += o.hx;
+          k->bb_net_solid_force[1] += o.hy;
+          k->bb_net_solid_force[2] += o.hz;
+        }
+        k->mb_sample_count = I.n_substeps;
+        break;
+      }
+      case FSI_BATCH_OP_PROJ_CLEAR:
+        for (long long c = 0; c < cells; ++c) {
+          I.ls.solid_phi[c] = 0.0f;
+          I.ls.solid_mass[c] = 0.0f;
+          I.ls.solid_vel[c] = I.ls.solid_vel[cells + c] = I.ls.solid_vel[2 * cells + c] = 0.0f;
+        }
+        break;
+      case FSI_BATCH_OP_PROJ_PARTICLES:
+        for (long long q = 0; q < parts; ++q) project_particle(I.lp, I.ls, I.mp, I.ms, I.cp, q);
+        break;
+      case FSI_BATCH_OP_PROJ_NORMALIZE:
+        for (long long c = 0; c < cells; ++c) {
+          ProjOut o = normalize_cell(I.lp, I.ls, I.cp, c);
+          atomic_max_nonneg(&I.cc->max_phi_raw_bits, o.phi_raw > 0.0f ? o.phi_raw : 0.0f);
+          I.cc->active_cell_count += o.active;
+          I.cc->projected_mass += o.mass;
+          I.cc->projected_volume_raw += (double)o.phi_raw * I.cp.cell_volume_norm;
+          I.cc->projected_volume_clamped += (double)o.phi_clamped * I.cp.cell_volume_norm;
+        }
+        break;
+      case FSI_BATCH_OP_PENALTY:
+        for (long long c = 0; c < cells; ++c) {
+          float f[3];
+          int act;
+          penalty_cell(I.lp, I.ls, I.cp, c, f, act);
+          I.cc->active_force_cell_count += act;
+          const float n = norm3(f[0], f[1], f[2]);
+          atomic_max_nonneg(&I.cc->max_cell_force_bits, n);
+          atomic_max_nonneg(&I.cc->max_hydro_force_bits, n);
+          for (int d = 0; d < 3; ++d) {
+            I.cc->net_cell_force[d] += f[d];
+            I.cc->net_hydro_force[d] -= f[d];
+          }
+        }
+        break;
+      case FSI_BATCH_OP_MPM_CLEAR:
+        memset(I.ms.grid_v, 0, sizeof(float) * 3 * nodes);
+        memset(I.ms.grid_m, 0, sizeof(float) * nodes);
+        memset(I.ms.grid_f, 0, sizeof(float) * 3 * nodes);
+        break;
+      case FSI_BATCH_OP_MPM_P2G:
+        for (long long q = 0; q < parts; ++q) mpm_p2g_particle(I.mp, I.ms, q);
+        break;
+      case FSI_BATCH_OP_MPM_REACTION:
+        if (I.mode == FSI_BATCH_MODE_NONE) break;
+        for (long long q = 0; q < parts; ++q) {
+          ReactOut o = reaction_particle(I.lp, I.ls, I.mp, I.ms, I.cp, I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1, q);
+          I.cc->active_reaction_particle_count += o.active;
+          atomic_max_nonneg(&I.cc->max_particle_reaction_bits, o.particle_norm);
+          atomic_max_nonneg(&I.cc->max_grid_reaction_bits, o.max_contrib);
+          for (int d = 0; d < 3; ++d) I.cc->net_particle_reaction[d] += o.f[d];
+        }
+        break;
+      case FSI_BATCH_OP_MPM_GRID_UPDATE:
+        for (long long n = 0; n < nodes; ++n) mpm_grid_node(I.mp, I.ms, n);
+        break;
+      case FSI_BATCH_OP_MPM_G2P:
+        for (long long q = 0; q < parts; ++q) mpm_g2p_particle(I.mp, I.ms, q);
+        break;
+      default:
+        return fail(FSI_E_INVALID, "unknown batch op");
+    }
+  }
   return FSI_OK;
 }
 

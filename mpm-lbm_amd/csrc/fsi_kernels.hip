@@ -420,6 +420,241 @@ __global__ void __launch_bounds__(TPB) k_mpm_stats_partial(FsiMpmParams mp, FsiM
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched kernels: workgroup b belongs to the instance i with
+// block_start[i] <= b < block_start[i+1]; its local work item is
+// (b - block_start[i]) * TPB + threadIdx.x.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int batch_instance(const int32_t* bs, int n) {
+  int lo = 0, hi = n;
+  const int b = (int)blockIdx.x;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bs[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+#define FSI_BATCH_PROLOGUE(DOMAIN)                                                   \
+  const int i_ = batch_instance(bs, n_inst);                                          \
+  const FsiBatchInstance& I = inst[i_];                                               \
+  const long long e_ = (long long)((int)blockIdx.x - bs[i_]) * TPB + threadIdx.x;     \
+  const bool in_ = e_ < batch_domain_size(I, DOMAIN);
+
+__global__ void __launch_bounds__(TPB) kb_substep(const FsiBatchInstance* inst, int n_inst, const int32_t* bs, int t) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  if (!in_) return;
+  const uint32_t flags = batch_substep_flags(I, t);
+  const int cur = batch_cur(I, t), parity = t & 1;
+  const FsiLbmParams p = I.lp;
+  const FsiLbmState st = I.ls;
+  if (p.storage == FSI_STORE_F16_DEV)
+    lbm_substep_cell<PopF16>(p, st, cur, parity, flags, e_);
+  else if (p.storage == FSI_STORE_BF16_DEV)
+    lbm_substep_cell<PopBF16>(p, st, cur, parity, flags, e_);
+  else
+    lbm_substep_cell<PopF32>(p, st, cur, parity, flags, e_);
+}
+
+__global__ void __launch_bounds__(TPB) kb_rebuild(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  if (!in_) return;
+  const int dyn = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 1 : 0;
+  FSI_BY_STORAGE(I.lp.storage, lbm_rebuild_cell<POP>(I.lp, I.ls, dyn, I.cur0, e_));
+}
+
+__global__ void __launch_bounds__(TPB) kb_moments(const FsiBatchInstance* inst, int n_inst, const int32_t* bs, int t) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  if (!in_) return;
+  // moments after substep t: the populations are in F[cur0 ^ ((t + 1) & 1)]
+  const uint32_t flags = I.cell_force ? FSI_SUB_CELL_FORCE : 0u;
+  FSI_BY_STORAGE(I.lp.storage, lbm_moments_cell<POP>(I.lp, I.ls, batch_cur(I, t + 1), flags, e_));
+}
+
+__global__ void __launch_bounds__(TPB) kb_mb_begin(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  if (e_ == 0) {
+    FsiLbmCounters* k = I.ls.counters;
+    k->bb_link_count = 0;
+    k->bb_max_correction_bits = 0;
+    k->mb_sample_count = 0;
+    k->accum_norm_max_bits = 0;
+    k->mean_norm_max_bits = 0;
+    k->bb_net_solid_force[0] = k->bb_net_solid_force[1] = k->bb_net_solid_force[2] = 0.0;
+  }
+  if (in_) lbm_mb_begin_cell(I.lp, I.ls, e_);
+}
+
+__global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  __shared__ int si[TPB / 64];
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  FinalizeOut o;
+  o.links = 0;
+  o.max_corr = o.mean_norm = o.runmax = 0.0f;
+  o.hx = o.hy = o.hz = 0.0;
+  const int last = I.n_substeps - 1;
+  if (in_) o = lbm_finalize_cell(I.lp, I.ls, last & 1, I.n_substeps, 1, e_);
+  const int links = block_sum_i(o.links, si);
+  const float mc = block_max_f(o.max_corr, sf);
+  const float mn = block_max_f(o.mean_norm, sf);
+  const float rm = block_max_f(o.runmax, sf);
+  const double hx = block_reduce(o.hx, RED_SUM, sd);
+  const double hy = block_reduce(o.hy, RED_SUM, sd);
+  const double hz = block_reduce(o.hz, RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    FsiLbmCounters* k = I.ls.counters;
+    if (links) atomicAdd(&k->bb_link_count, links);
+    atomicMax(&k->bb_max_correction_bits, __float_as_uint(mc));
+    atomicMax(&k->mean_norm_max_bits, __float_as_uint(mn));
+    atomicMax(&k->accum_norm_max_bits, __float_as_uint(rm));
+    if (hx != 0.0) atomicAdd(&k->bb_net_solid_force[0], hx);
+    if (hy != 0.0) atomicAdd(&k->bb_net_solid_force[1], hy);
+    if (hz != 0.0) atomicAdd(&k->bb_net_solid_force[2], hz);
+    if (blockIdx.x == (unsigned)bs[i_]) k->mb_sample_count = I.n_substeps;
+  }
+}
+
+__global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* inst, int n_inst, int which) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= n_inst || !inst[i].active) return;
+  const FsiBatchInstance& I = inst[i];
+  FsiCouplingCounters* cc = I.cc;
+  if (which == 0) {
+    cc->projected_mass = cc->projected_volume_raw = cc->projected_volume_clamped = 0.0;
+    cc->max_phi_raw_bits = 0;
+    cc->active_cell_count = 0;
+  } else if (which == 1) {
+    cc->active_force_cell_count = 0;
+    cc->max_cell_force_bits = cc->max_hydro_force_bits = 0;
+    for (int d = 0; d < 3; ++d) cc->net_cell_force[d] = cc->net_hydro_force[d] = 0.0;
+  } else if (which == 2) {
+    cc->active_reaction_particle_count = 0;
+    cc->max_particle_reaction_bits = cc->max_grid_reaction_bits = 0;
+    for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] = cc->net_grid_reaction[d] = 0.0;
+  } else {
+    FsiLbmCounters* k = I.ls.counters;
+    k->bb_link_count = 0;
+    k->bb_max_correction_bits = 0;
+    k->bb_net_solid_force[0] = k->bb_net_solid_force[1] = k->bb_net_solid_force[2] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  if (!in_) return;
+  const long long N = batch_domain_size(I, FSI_BATCH_DOMAIN_CELLS);
+  I.ls.solid_phi[e_] = 0.0f;
+  I.ls.solid_mass[e_] = 0.0f;
+  I.ls.solid_vel[e_] = I.ls.solid_vel[N + e_] = I.ls.solid_vel[2 * N + e_] = 0.0f;
+}
+
+__global__ void __launch_bounds__(TPB) kb_proj_particles(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
+  if (in_) project_particle(I.lp, I.ls, I.mp, I.ms, I.cp, e_);
+}
+
+__global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  __shared__ int si[TPB / 64];
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  ProjOut o;
+  o.phi_raw = o.phi_clamped = o.mass = 0.0f;
+  o.active = 0;
+  if (in_) o = normalize_cell(I.lp, I.ls, I.cp, e_);
+  const float mp = block_max_f(o.phi_raw > 0.0f ? o.phi_raw : 0.0f, sf);
+  const int act = block_sum_i(o.active, si);
+  const double m = block_reduce((double)o.mass, RED_SUM, sd);
+  const double vr = block_reduce((double)o.phi_raw * I.cp.cell_volume_norm, RED_SUM, sd);
+  const double vc = block_reduce((double)o.phi_clamped * I.cp.cell_volume_norm, RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    FsiCouplingCounters* cc = I.cc;
+    atomicMax(&cc->max_phi_raw_bits, __float_as_uint(mp));
+    if (act) atomicAdd(&cc->active_cell_count, act);
+    if (m != 0.0) atomicAdd(&cc->projected_mass, m);
+    if (vr != 0.0) atomicAdd(&cc->projected_volume_raw, vr);
+    if (vc != 0.0) atomicAdd(&cc->projected_volume_clamped, vc);
+  }
+}
+
+__global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  __shared__ int si[TPB / 64];
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  float f[3] = {0.0f, 0.0f, 0.0f};
+  int active = 0;
+  if (in_) penalty_cell(I.lp, I.ls, I.cp, e_, f, active);
+  const float mfn = block_max_f(norm3(f[0], f[1], f[2]), sf);
+  const int act = block_sum_i(active, si);
+  double net[3];
+  for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)f[d], RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    FsiCouplingCounters* cc = I.cc;
+    if (act) atomicAdd(&cc->active_force_cell_count, act);
+    atomicMax(&cc->max_cell_force_bits, __float_as_uint(mfn));
+    atomicMax(&cc->max_hydro_force_bits, __float_as_uint(mfn));
+    for (int d = 0; d < 3; ++d)
+      if (net[d] != 0.0) {
+        atomicAdd(&cc->net_cell_force[d], net[d]);
+        atomicAdd(&cc->net_hydro_force[d], -net[d]);
+      }
+  }
+}
+
+__global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES)
+  if (!in_) return;
+  const long long G = batch_domain_size(I, FSI_BATCH_DOMAIN_NODES);
+  I.ms.grid_v[e_] = I.ms.grid_v[G + e_] = I.ms.grid_v[2 * G + e_] = 0.0f;
+  I.ms.grid_m[e_] = 0.0f;
+  I.ms.grid_f[e_] = I.ms.grid_f[G + e_] = I.ms.grid_f[2 * G + e_] = 0.0f;
+}
+
+__global__ void __launch_bounds__(TPB) kb_mpm_p2g(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
+  if (in_) mpm_p2g_particle(I.mp, I.ms, e_);
+}
+
+__global__ void __launch_bounds__(TPB) kb_mpm_reaction(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  __shared__ int si[TPB / 64];
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  const int mode = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1;
+  if (in_ && I.mode != FSI_BATCH_MODE_NONE) o = reaction_particle(I.lp, I.ls, I.mp, I.ms, I.cp, mode, e_);
+  const int act = block_sum_i(o.active, si);
+  const float mpn = block_max_f(o.particle_norm, sf);
+  const float mgc = block_max_f(o.max_contrib, sf);
+  double net[3];
+  for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)o.f[d], RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    FsiCouplingCounters* cc = I.cc;
+    if (act) atomicAdd(&cc->active_reaction_particle_count, act);
+    atomicMax(&cc->max_particle_reaction_bits, __float_as_uint(mpn));
+    atomicMax(&cc->max_grid_reaction_bits, __float_as_uint(mgc));
+    for (int d = 0; d < 3; ++d)
+      if (net[d] != 0.0) atomicAdd(&cc->net_particle_reaction[d], net[d]);
+  }
+}
+
+__global__ void __launch_bounds__(TPB) kb_mpm_grid_update(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES)
+  if (in_) mpm_grid_node(I.mp, I.ms, e_);
+}
+
+__global__ void __launch_bounds__(TPB) kb_mpm_g2p(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
+  if (in_) mpm_g2p_particle(I.mp, I.ms, e_);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -432,13 +667,13 @@ const char* fsi_last_error(void) { return g_last_error.c_str(); }
 int fsi_version(void) { return 1; }
 
 int fsi_abi_sizes(int32_t* out, int n) {
-  const int32_t sz[9] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
+  const int32_t sz[10] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
                          (int32_t)sizeof(FsiLbmCounters),    (int32_t)sizeof(FsiLbmStats),
                          (int32_t)sizeof(FsiMpmParams),      (int32_t)sizeof(FsiMpmState),
                          (int32_t)sizeof(FsiCouplingParams), (int32_t)sizeof(FsiCouplingCounters),
-                         (int32_t)sizeof(FsiMpmStats)};
+                         (int32_t)sizeof(FsiMpmStats),       (int32_t)sizeof(FsiBatchInstance)};
   int k = 0;
-  for (; k < n && k < 9; ++k) out[k] = sz[k];
+  for (; k < n && k < 10; ++k) out[k] = sz[k];
   return k;
 }
 
@@ -645,6 +880,56 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
   hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, reinterpret_cast<const double*>(workspace_dev),
                      FSI_REDUCE_BLOCKS, MPM_STATS_N, 1, reinterpret_cast<double*>(out_dev));
   return check_launch("k_mpm_stats");
+}
+
+
+int fsi_batch_block_table(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask,
+                          int32_t* block_start_host) {
+  if (!inst_host || !block_start_host || n_inst < 0) return fail(FSI_E_INVALID, "null batch arguments");
+  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_PARTICLES)
+    return fail(FSI_E_INVALID, "unknown batch domain");
+  long long total = 0;
+  block_start_host[0] = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    const FsiBatchInstance& I = inst_host[i];
+    long long nb = 0;
+    if (I.active && (mode_mask & (1 << I.mode))) nb = (batch_domain_size(I, domain) + TPB - 1) / TPB;
+    total += nb;
+    if (total > 0x7fffffffLL) return fail(FSI_E_INVALID, "batch too large for one launch");
+    block_start_host[i + 1] = (int32_t)total;
+  }
+  return (int)total;
+}
+
+int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const int32_t* block_start_dev, int n_blocks,
+                     int t, void* stream) {
+  if (!inst_dev || n_inst <= 0) return fail(FSI_E_INVALID, "empty batch");
+  hipStream_t s = as_stream(stream);
+  if (op == FSI_BATCH_OP_COUNTERS_CLEAR) {
+    hipLaunchKernelGGL(kb_counters_clear, dim3((n_inst + 63) / 64), dim3(64), 0, s, inst_dev, n_inst, t);
+    return check_launch("kb_counters_clear");
+  }
+  if (!block_start_dev) return fail(FSI_E_INVALID, "null block table");
+  if (n_blocks <= 0) return FSI_OK;
+  const dim3 g(n_blocks), b(TPB);
+  switch (op) {
+    case FSI_BATCH_OP_LBM_SUBSTEP: hipLaunchKernelGGL(kb_substep, g, b, 0, s, inst_dev, n_inst, block_start_dev, t); break;
+    case FSI_BATCH_OP_LBM_REBUILD: hipLaunchKernelGGL(kb_rebuild, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_LBM_MOMENTS: hipLaunchKernelGGL(kb_moments, g, b, 0, s, inst_dev, n_inst, block_start_dev, t); break;
+    case FSI_BATCH_OP_LBM_MB_BEGIN: hipLaunchKernelGGL(kb_mb_begin, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_LBM_MB_FINALIZE: hipLaunchKernelGGL(kb_finalize, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_PROJ_CLEAR: hipLaunchKernelGGL(kb_proj_clear, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_PROJ_PARTICLES: hipLaunchKernelGGL(kb_proj_particles, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_PROJ_NORMALIZE: hipLaunchKernelGGL(kb_proj_normalize, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_PENALTY: hipLaunchKernelGGL(kb_penalty, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_MPM_CLEAR: hipLaunchKernelGGL(kb_mpm_clear, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_MPM_P2G: hipLaunchKernelGGL(kb_mpm_p2g, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_MPM_REACTION: hipLaunchKernelGGL(kb_mpm_reaction, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_MPM_GRID_UPDATE: hipLaunchKernelGGL(kb_mpm_grid_update, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    case FSI_BATCH_OP_MPM_G2P: hipLaunchKernelGGL(kb_mpm_g2p, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
+    default: return fail(FSI_E_INVALID, "unknown batch op");
+  }
+  return check_launch("fsi_batch_launch");
 }
 
 }  // extern "C"

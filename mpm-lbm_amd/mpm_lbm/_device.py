@@ -30,11 +30,13 @@ def stream_handle(device: torch.device):
 class FieldView:
     """to_numpy / from_numpy over a SoA tensor of shape (comps, *space) or (*space)."""
 
-    def __init__(self, get_tensor, space_shape, comps=None, before_read=None, after_write=None, np_dtype=np.float32):
+    def __init__(self, get_tensor, space_shape, comps=None, before_read=None, after_write=None, np_dtype=np.float32,
+                 before_write=None):
         self._get = get_tensor
         self.space_shape = tuple(space_shape)
         self.comps = comps  # None (scalar), int, or (3, 3)
         self._before_read = before_read
+        self._before_write = before_write
         self._after_write = after_write
         self.np_dtype = np_dtype
 
@@ -66,6 +68,8 @@ class FieldView:
         a = np.asarray(arr)
         if a.shape != self.shape:
             raise ValueError(f"expected array of shape {self.shape}, got {a.shape}")
+        if self._before_write is not None:
+            self._before_write()
         t = self._get()
         src = torch.from_numpy(np.ascontiguousarray(a.astype(self.np_dtype, copy=False)))
         if self.comps is not None:
@@ -76,6 +80,8 @@ class FieldView:
             self._after_write()
 
     def fill(self, value):
+        if self._before_write is not None:
+            self._before_write()
         self._get().fill_(value)
         if self._after_write is not None:
             self._after_write()

@@ -155,8 +155,29 @@ class MpmStats(ctypes.Structure):
     _fields_ = [(name, f64) for name in MPM_STATS_FIELDS]
 
 
+BATCH_MODE_NONE = 0
+BATCH_MODE_PENALTY = 1
+BATCH_MODE_MOVING_BOUNDARY = 2
+BATCH_DOMAIN_CELLS = 0
+BATCH_DOMAIN_NODES = 1
+BATCH_DOMAIN_PARTICLES = 2
+(BATCH_OP_LBM_SUBSTEP, BATCH_OP_LBM_REBUILD, BATCH_OP_LBM_MOMENTS, BATCH_OP_LBM_MB_BEGIN, BATCH_OP_LBM_MB_FINALIZE,
+ BATCH_OP_PROJ_CLEAR, BATCH_OP_PROJ_PARTICLES, BATCH_OP_PROJ_NORMALIZE, BATCH_OP_PENALTY, BATCH_OP_MPM_CLEAR,
+ BATCH_OP_MPM_P2G, BATCH_OP_MPM_REACTION, BATCH_OP_MPM_GRID_UPDATE, BATCH_OP_MPM_G2P,
+ BATCH_OP_COUNTERS_CLEAR) = range(15)
+
+
+class BatchInstance(ctypes.Structure):
+    _fields_ = [
+        ("lp", LbmParams), ("ls", LbmState), ("mp", MpmParams), ("ms", MpmState),
+        ("cp", CouplingParams), ("cc", vp),
+        ("mode", i32), ("cur0", i32), ("n_substeps", i32), ("mpm_substeps", i32), ("active", i32),
+        ("cell_force", i32),
+    ]
+
+
 ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, CouplingParams,
-               CouplingCounters, MpmStats]
+               CouplingCounters, MpmStats, BatchInstance]
 
 EXPORTED_SYMBOLS = [
     "fsi_last_error", "fsi_version", "fsi_abi_sizes", "fsi_sync",
@@ -164,6 +185,7 @@ EXPORTED_SYMBOLS = [
     "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats",
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
     "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_reduce_stats",
+    "fsi_batch_block_table", "fsi_batch_launch",
 ]
 
 P = ctypes.POINTER
@@ -203,6 +225,8 @@ class Native:
         L.fsi_mpm_grid_update.argtypes = [P(MpmParams), P(MpmState), vp]
         L.fsi_mpm_g2p.argtypes = [P(MpmParams), P(MpmState), vp]
         L.fsi_mpm_reduce_stats.argtypes = [P(MpmParams), P(MpmState), vp, vp, vp]
+        L.fsi_batch_block_table.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.fsi_batch_launch.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]
         for name in EXPORTED_SYMBOLS:
             getattr(L, name).restype = ctypes.c_int if name != "fsi_last_error" else ctypes.c_char_p
         self.check_abi()
@@ -220,7 +244,7 @@ class Native:
 
     def call(self, name: str, *args):
         rc = getattr(self.lib, name)(*args)
-        if rc != 0:
+        if rc < 0 or (rc != 0 and name != "fsi_batch_block_table"):
             msg = self.lib.fsi_last_error().decode("utf-8", "replace")
             raise FsiError(f"{name} failed ({rc}): {msg}")
         return rc

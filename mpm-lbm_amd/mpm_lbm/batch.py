@@ -1,0 +1,209 @@
+"""Batched form: many independent FSI instances advanced together.
+
+Each instance is a full FSIDriver3D (its own device buffers, geometry,
+particles and coupling constants).  `BatchedFSI` gathers their C-ABI
+descriptors into one device table and advances all of them with one launch
+per operation (`fsi_batch_launch`): every workgroup finds its instance in a
+block-prefix table, so instances of different sizes and modes share each
+launch.  One batched exchange - 120 LBM substeps for moving-boundary
+instances, 120 consecutive one-LBM-step FSI steps for penalty / none
+instances (the reference rejects penalty with subcycling,
+fsi_config.py:342-343) - is captured once as a HIP graph and replayed.
+
+Results are identical to running each instance alone: the kernels and their
+per-instance arguments are the same, only the launch is shared.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native
+from ._device import stream_handle, struct_buffer
+from .sim.drivers.fsi_driver import FSIDriver3D
+
+TPB = 256
+_MODE = {"none": _native.BATCH_MODE_NONE, "penalty": _native.BATCH_MODE_PENALTY,
+         "moving_boundary": _native.BATCH_MODE_MOVING_BOUNDARY}
+
+
+class BatchedFSI:
+    def __init__(self, workloads, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True):
+        if not workloads:
+            raise ValueError("BatchedFSI needs at least one workload")
+        self.workloads = list(workloads)
+        self.drivers = []
+        for k, w in enumerate(self.workloads):
+            d = FSIDriver3D(w.config, f"{out_dir}/inst{k:03d}", particle_cloud=w.cloud, native=native, device=device,
+                            write_geometry_file=False, use_graph=False)
+            d.initialize()
+            self.drivers.append(d)
+        self.native = self.drivers[0].lbm.native
+        self.device = self.drivers[0].lbm.device
+        mb = [d for d in self.drivers if d.config.coupling_mode == "moving_boundary"]
+        subs = {d.config.lbm_substeps_per_fsi_step for d in mb}
+        if len(subs) > 1:
+            raise ValueError("moving-boundary instances of one batch must share lbm_substeps_per_fsi_step")
+        self.n_substeps = subs.pop() if subs else 120
+        if self.n_substeps % 2:
+            raise ValueError("the batched exchange needs an even number of LBM substeps")
+        for d in self.drivers:
+            if d.config.coupling_mode == "moving_boundary" and d.config.fsi_exchange_mode != "lbm_subcycled_per_fsi_step":
+                raise ValueError("moving-boundary instances of a batch must be subcycled")
+        self._counters = [struct_buffer(_native.CouplingCounters, self.device) for _ in self.drivers]
+        self._prepare_instances()
+        self.use_graph = use_graph and self.device.type == "cuda"
+        self._graph = None
+        self.exchanges = 0
+
+    # ----------------------------------------------------------------- set-up
+    def _prepare_instances(self):
+        inst = (_native.BatchInstance * len(self.drivers))()
+        for k, d in enumerate(self.drivers):
+            lbm = d.lbm
+            lbm._ensure_ready()  # static link masks for penalty / none instances
+            if d.config.coupling_mode == "moving_boundary":
+                lbm._ensure_link_buffers()
+                lbm.update_dynamic_solid(d.config.dynamic_solid_threshold)
+            lbm._sync_moments()
+            I = inst[k]
+            I.lp = lbm._params
+            I.ls = lbm._state
+            I.mp = d.solid._params
+            I.ms = d.solid._state
+            coupler = d.mb_coupler or d.penalty_coupler or d.projector
+            I.cp = coupler.params
+            I.cc = _native.ptr(self._counters[k])
+            I.mode = _MODE[d.config.coupling_mode]
+            I.cur0 = lbm.cur
+            I.n_substeps = self.n_substeps if d.config.coupling_mode == "moving_boundary" else 1
+            I.mpm_substeps = d.config.mpm_substeps_per_lbm_step
+            I.active = 1
+            I.cell_force = 1 if d.config.coupling_mode == "penalty" else 0
+            lbm._cell_force_active = bool(I.cell_force)
+        self._inst_host = inst
+        raw = bytes(memoryview(inst).cast("B"))
+        self._inst_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
+        self._tables = {}
+
+    def _table(self, domain, modes, min_mpm=0):
+        """Device block table for instances of `modes` with at least min_mpm + 1 MPM substeps."""
+        key = (domain, tuple(sorted(modes)), min_mpm)
+        if key not in self._tables:
+            starts = [0]
+            for I in self._inst_host:
+                size = {_native.BATCH_DOMAIN_CELLS: I.lp.nx * I.lp.ny * I.lp.nz,
+                        _native.BATCH_DOMAIN_NODES: I.mp.gnx * I.mp.gny * I.mp.gnz,
+                        _native.BATCH_DOMAIN_PARTICLES: I.mp.n_particles}[domain]
+                use = I.active and I.mode in modes and I.mpm_substeps > min_mpm
+                starts.append(starts[-1] + ((size + TPB - 1) // TPB if use else 0))
+            t = torch.tensor(starts, dtype=torch.int32, device=self.device)
+            self._tables[key] = (t, starts[-1])
+        return self._tables[key]
+
+    def _launch(self, op, domain, modes, t=0, min_mpm=0):
+        table, nblocks = self._table(domain, modes, min_mpm)
+        if nblocks == 0:
+            return
+        self.native.call("fsi_batch_launch", op, _native.ptr(self._inst_dev), len(self.drivers), _native.ptr(table),
+                         nblocks, t, stream_handle(self.device))
+
+    def _clear_counters(self, which):
+        self.native.call("fsi_batch_launch", _native.BATCH_OP_COUNTERS_CLEAR, _native.ptr(self._inst_dev),
+                         len(self.drivers), None, 0, which, stream_handle(self.device))
+
+    def _mpm_substeps(self, modes, reaction):
+        C, N, P = _native.BATCH_DOMAIN_CELLS, _native.BATCH_DOMAIN_NODES, _native.BATCH_DOMAIN_PARTICLES
+        max_m = max(I.mpm_substeps for I in self._inst_host if I.mode in modes)
+        for j in range(max_m):
+            self._launch(_native.BATCH_OP_MPM_CLEAR, N, modes, min_mpm=j)
+            self._launch(_native.BATCH_OP_MPM_P2G, P, modes, min_mpm=j)
+            if reaction:
+                self._clear_counters(2)
+                self._launch(_native.BATCH_OP_MPM_REACTION, P, modes, min_mpm=j)
+            self._launch(_native.BATCH_OP_MPM_GRID_UPDATE, N, modes, min_mpm=j)
+            self._launch(_native.BATCH_OP_MPM_G2P, P, modes, min_mpm=j)
+
+    def _project(self, modes):
+        C, P = _native.BATCH_DOMAIN_CELLS, _native.BATCH_DOMAIN_PARTICLES
+        self._clear_counters(0)
+        self._launch(_native.BATCH_OP_PROJ_CLEAR, C, modes)
+        self._launch(_native.BATCH_OP_PROJ_PARTICLES, P, modes)
+        self._launch(_native.BATCH_OP_PROJ_NORMALIZE, C, modes)
+
+    def _issue_exchange(self):
+        """One batched exchange: n substeps of every instance (see the module docstring)."""
+        C = _native.BATCH_DOMAIN_CELLS
+        MB = (_native.BATCH_MODE_MOVING_BOUNDARY,)
+        ONE = (_native.BATCH_MODE_PENALTY, _native.BATCH_MODE_NONE)
+        PEN = (_native.BATCH_MODE_PENALTY,)
+        ALL = MB + ONE
+        has_mb = any(I.mode in MB for I in self._inst_host)
+        has_one = any(I.mode in ONE for I in self._inst_host)
+        if has_mb:
+            self._project(MB)
+            self._launch(_native.BATCH_OP_LBM_MB_BEGIN, C, MB)
+            self._launch(_native.BATCH_OP_LBM_REBUILD, C, MB)
+        for t in range(self.n_substeps):
+            if has_one:
+                self._project(ONE)
+                self._clear_counters(1)
+                self._launch(_native.BATCH_OP_PENALTY, C, PEN)
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP, C, ALL, t=t)
+            if has_one:
+                self._launch(_native.BATCH_OP_LBM_MOMENTS, C, ONE, t=t)
+                self._mpm_substeps(ONE, reaction=True)
+        if has_mb:
+            self._clear_counters(3)
+            self._launch(_native.BATCH_OP_LBM_MB_FINALIZE, C, MB)
+            self._launch(_native.BATCH_OP_LBM_MOMENTS, C, MB, t=self.n_substeps - 1)
+            self._mpm_substeps(MB, reaction=True)
+
+    # ------------------------------------------------------------------- run
+    def step(self):
+        """Advance every instance by one batched exchange."""
+        if self.use_graph:
+            if self._graph is None:
+                torch.cuda.synchronize(self.device)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    self._issue_exchange()
+                self._graph = g
+            self._graph.replay()
+        else:
+            self._issue_exchange()
+        self.exchanges += 1
+        self._sync_host_state()
+
+    def _sync_host_state(self):
+        """Host bookkeeping of each driver after a batched exchange."""
+        for d, I in zip(self.drivers, self._inst_host):
+            lbm = d.lbm
+            lbm.cur = I.cur0
+            lbm._moments_fresh = True
+            lbm._pending = None
+            lbm._hydro_from_links = False
+            if I.mode == _native.BATCH_MODE_MOVING_BOUNDARY:
+                lbm.mb_sample_count = self.n_substeps
+                d.current_lbm_step += 1
+                d.total_lbm_substeps += self.n_substeps
+                d.total_mpm_substeps += I.mpm_substeps
+                lbm._rebuilt_phi_version = None
+            else:
+                d.current_lbm_step += self.n_substeps
+                d.total_lbm_substeps += self.n_substeps
+                d.total_mpm_substeps += self.n_substeps * I.mpm_substeps
+
+    @property
+    def total_cells(self):
+        return sum(I.lp.nx * I.lp.ny * I.lp.nz for I in self._inst_host)
+
+    def cell_updates_per_exchange(self):
+        return self.total_cells * self.n_substeps
+
+    def coupling_stats(self, k):
+        from ._device import read_struct
+
+        return read_struct(self._counters[k], _native.CouplingCounters)

@@ -37,6 +37,9 @@ class PenaltyFSICoupler3D(CouplingBase):
         self._react_counters = None
 
     def clear_force_fields(self, lbm):
+        # v of the last step carries its cell force (streaming3 at the end of
+        # step(), fluid.py:2099-2103) and build_penalty_force reads that v
+        lbm._sync_moments()
         lbm._cell_force.zero_()
         lbm._hydro.zero_()
         lbm._hydro_from_links = False

@@ -178,7 +178,10 @@ class LBMFluid3D:
         self.solid_phi = FieldView(lambda: self._phi, sp, after_write=self._phi_written)
         self.solid_mass = FieldView(lambda: self._smass, sp)
         self.solid_vel = FieldView(lambda: self._svel, sp, 3, after_write=self._phi_written)
-        self.cell_force = FieldView(lambda: self._cell_force, sp, 3, after_write=self._cell_force_written)
+        # rho / v of the last step include that step's cell force (streaming3 runs
+        # at the end of step(), fluid.py:2099-2103): settle them before it changes
+        self.cell_force = FieldView(lambda: self._cell_force, sp, 3, before_write=self._sync_moments,
+                                    after_write=self._cell_force_written)
         self.hydro_force = FieldView(lambda: self._hydro, sp, 3, before_read=self._sync_hydro)
         self.hydro_force_accum = FieldView(self._accum_f32, sp, 3)
         self.bb_link_count = ScalarView(lambda: self.get_moving_boundary_stats()["bb_link_count"])
@@ -274,6 +277,7 @@ class LBMFluid3D:
             self._build_params()
 
     def set_uniform_cell_force(self, fx: float, fy: float, fz: float):
+        self._sync_moments()
         self._cell_force[0].fill_(float(fx))
         self._cell_force[1].fill_(float(fy))
         self._cell_force[2].fill_(float(fz))

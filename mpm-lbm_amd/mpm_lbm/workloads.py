@@ -153,3 +153,53 @@ def moving_boundary_unsubcycled(n=32, steps=10, **kw) -> Workload:
                              "lattice_shape": c.lattice_shape,
                              "fluid_kinematic_viscosity_m2_s": _viscosity_for_tau(w.notes["tau"], n, c.mpm_dt)})
     return Workload(w.name + "_unsubcycled", cfg, w.cloud, w.geometry_json, w.material, w.notes)
+
+
+def flap_case(mode, n, substeps=120, u_in=0.05, E_mod=1.0e5, thickness=4, seed=0, out_dir="/tmp/fsi_workloads",
+              rho_s=2.0, nu=0.35, tau=0.6, mpm_substeps=10, storage="f32", cfl=0.3) -> Workload:
+    """A flap case of either coupling mode, with the MPM step at `cfl` times the
+    elastic limit dx / sqrt(E / rho_s).  Moving-boundary instances subcycle
+    `substeps` LBM steps per FSI step; penalty instances take one LBM step and
+    one MPM substep per FSI step (the reference's only penalty schedule)."""
+    dx = 1.0 / n
+    dt_cfl = cfl * dx / np.sqrt(E_mod / rho_s)
+    flap = (thickness, max(2, n // 2), max(2, (3 * n) // 8))
+    name = f"{mode}_{n}_t{thickness}_s{seed}"
+    if mode == "moving_boundary":
+        w = moving_boundary_flap(n=n, steps=1, substeps=substeps, u_in=u_in, seed=seed, out_dir=out_dir, flap=flap,
+                                 E_mod=E_mod, nu=nu, rho_s=rho_s, tau=tau, mpm_dt=dt_cfl, mpm_substeps=mpm_substeps)
+    else:
+        shape = (n, n, n)
+        lo = (n // 2 - flap[0] // 2, 1, n // 2 - flap[2] // 2)
+        hi = (lo[0] + flap[0], lo[1] + flap[1], lo[2] + flap[2])
+        cloud = solid_box_cloud(shape, lo, hi, rho_s, seed, fixed_bottom=True, tip_top=True)
+        P = len(cloud["x"])
+        geo = _write_geometry(out_dir, name, duct_geometry_dict(shape, P, (E_mod, nu, rho_s)))
+        cfg = FSIDriverConfig(
+            coupling_mode=mode, geometry_type="duct_flap_proxy", geometry_config_path=geo, n_grid=n, n_particles=P,
+            n_lbm_steps=1, mpm_substeps_per_lbm_step=1, mpm_dt=dt_cfl, target_u_lbm=(u_in, 0.0, 0.0),
+            lbm_boundary_condition_mode="duct_velocity_inlet_pressure_outlet", lbm_viscosity_semantics="physical_nu_mapping",
+            fluid_kinematic_viscosity_m2_s=_viscosity_for_tau(tau, n, dt_cfl))
+        w = Workload(name, cfg, cloud, geo, (E_mod, nu, rho_s), {"tau": tau, "seed": seed})
+    if storage != "f32":
+        w = Workload(w.name, FSIDriverConfig(**{**w.config.to_dict(), "lbm_population_storage": storage}),
+                     w.cloud, w.geometry_json, w.material, w.notes)
+    return w
+
+
+def batch_sweep(count=64, seed=4, sizes=(32, 48, 64, 96), substeps=120, out_dir="/tmp/fsi_workloads",
+                storage="f32") -> list:
+    """Batched parameter sweep: grid from `sizes`, u_in ~ U(0.02, 0.08),
+    E ~ logU(1e4, 1e6), flap thickness in {2, 4, 6} cells, modes alternating
+    moving_boundary / penalty, all from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.choice(sizes))
+        u_in = float(rng.uniform(0.02, 0.08))
+        E_mod = float(10 ** rng.uniform(4.0, 6.0))
+        thickness = int(rng.choice([2, 4, 6]))
+        mode = "moving_boundary" if k % 2 == 0 else "penalty"
+        out.append(flap_case(mode, n, substeps=substeps, u_in=u_in, E_mod=E_mod, thickness=thickness, seed=seed * 1000 + k,
+                             out_dir=out_dir, storage=storage))
+    return out

@@ -207,6 +207,46 @@ void colission(const LbmArgs<T>& a, const T* F, T* f, const T* rho, const T* v, 
   }
 }
 
+// Regularised BGK collision, the precision form's option (no reference
+// counterpart), from J. Latt & B. Chopard, "Lattice Boltzmann method with
+// regularized pre-collision distribution functions", Math. Comput. Simul. 72
+// (2006) 165-168: f = feq + (1 - 1/tau) w (9/2) (e e - I/3) : Pi_neq, with
+// Guo's BGK forcing term (1 - 1/(2 tau)) w [3 (e - u) + 9 (e.u) e] . G.
+// tau = 1 / s_v; stored rho / v as in colission.
+template <typename T>
+void colission_regularized(const LbmArgs<T>& a, const T* F, T* f, const T* rho, const T* v, const int8_t* solid,
+                           const T* cell_force) {
+  const Tables<T>& tb = tables<T>();
+  const Dims d{a.nx, a.ny, a.nz};
+#pragma omp parallel for schedule(static)
+  for (long long c = 0; c < d.n(); ++c) {
+    if (solid[c] != 0) continue;
+    const T* Fc = F + c * Q;
+    const T* u = v + 3 * c;
+    T fe[Q], Pi[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int s = 0; s < Q; ++s) {
+      fe[s] = feq<T>(s, rho[c], u);
+      const T dn = Fc[s] - fe[s];
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) Pi[p][q] += tb.e[s][p] * tb.e[s][q] * dn;
+    }
+    const T tr = Pi[0][0] + Pi[1][1] + Pi[2][2];
+    T G[3];
+    for (int k = 0; k < 3; ++k) G[k] = a.fext[k] + (cell_force ? cell_force[3 * c + k] : (T)0);
+    for (int s = 0; s < Q; ++s) {
+      T qpi = -tr / (T)3;
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) qpi += tb.e[s][p] * tb.e[s][q] * Pi[p][q];
+      T out = fe[s] + ((T)1 - a.s_v) * tb.w[s] * (T)4.5 * qpi;
+      const T eu = tb.e[s][0] * u[0] + tb.e[s][1] * u[1] + tb.e[s][2] * u[2];
+      T gterm = 0;
+      for (int k = 0; k < 3; ++k) gterm += ((T)3 * (tb.e[s][k] - u[k]) + (T)9 * eu * tb.e[s][k]) * G[k];
+      out += ((T)1 - (T)0.5 * a.s_v) * tb.w[s] * gterm;
+      f[c * Q + s] = out;
+    }
+  }
+}
+
 // streaming1: push with static half-way bounce-back (fluid.py:432-440)
 template <typename T>
 void streaming1(const LbmArgs<T>& a, const T* f, T* F, const int8_t* solid) {
@@ -834,6 +874,10 @@ void reaction(const LbmArgs<T>& la, const MpmArgs<T>& ma, const CplArgs<T>& ca, 
   ORC_EXPORT void orc_colission_##SFX(const LbmArgs<T>* a, const T* F, T* f, const T* rho, const T* v,             \
                                       const int8_t* solid, const T* cell_force) {                                   \
     colission<T>(*a, F, f, rho, v, solid, cell_force);                                                            \
+  }                                                                                                                 \
+  ORC_EXPORT void orc_colission_regularized_##SFX(const LbmArgs<T>* a, const T* F, T* f, const T* rho,       \
+                                                  const T* v, const int8_t* solid, const T* cell_force) {         \
+    colission_regularized<T>(*a, F, f, rho, v, solid, cell_force);                                                \
   }                                                                                                                 \
   ORC_EXPORT void orc_streaming1_##SFX(const LbmArgs<T>* a, const T* f, T* F, const int8_t* solid) {              \
     streaming1<T>(*a, f, F, solid);                                                                               \

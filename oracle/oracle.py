@@ -212,7 +212,8 @@ class OracleFSI:
 
     # ---- LBM passes (fluid.py)
     def colission(self):
-        self._call("colission", ctypes.byref(self.la), _p(self.F), _p(self.f), _p(self.rho), _p(self.v),
+        name = "colission_regularized" if getattr(self.config, "lbm_collision", "mrt") == "regularized" else "colission"
+        self._call(name, ctypes.byref(self.la), _p(self.F), _p(self.f), _p(self.rho), _p(self.v),
                    _p(self.solid), _p(self.cell_force))
 
     def lbm_step(self):
