@@ -164,7 +164,9 @@ class BatchedFSI:
     # ------------------------------------------------------------------- run
     def step(self):
         """Advance every instance by one batched exchange."""
-        if self.use_graph:
+        if self.use_graph and self.exchanges > 0:
+            # the first exchange runs eagerly: it builds the block tables
+            # (host-to-device copies) that the captured launches then read
             if self._graph is None:
                 torch.cuda.synchronize(self.device)
                 g = torch.cuda.CUDAGraph()

@@ -6,17 +6,13 @@ import numpy as np
 import pytest
 
 from mpm_lbm import workloads
-from tests.parity import make_pair, run_and_compare
+from tests.parity import check_history, make_pair, run_and_compare
 
 TOL = 1.0e-5
 
 
 def _check(history):
-    for step, errs in enumerate(history):
-        assert errs["solid_equal"], f"solid flags differ at step {step}"
-        for k, v in errs.items():
-            if k != "solid_equal":
-                assert v <= TOL, f"step {step}: {k} relative error {v:.3e} > {TOL}"
+    check_history(history, "emulation", TOL, strict=True)
 
 
 def test_moving_boundary_subcycled(emu, tmp_path):

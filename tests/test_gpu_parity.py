@@ -2,10 +2,11 @@
 
 Bar (north star): FP32 fields within 1e-5 relative per FSI step, measured as
 max |gpu - oracle| / max |oracle| per field (C scaled by max(|C|, |v|/dx));
-solid flags bit-exact.  Velocity-level particle fields (particle v, APIC C,
-projected solid velocity) are FP32-ill-conditioned at small strain and are
-held to a quarter of the FP32-vs-FP64 oracle gap when that exceeds 1e-5
-(tests/parity.py)."""
+solid flags bit-exact.  The one-LBM-step schedules (penalty, unsubcycled
+moving boundary) are held to that bar against the FP32 oracle on every field.
+After a subcycled FSI step (40-120 LBM substeps) a field may instead be within
+1e-5 of the FP64 oracle, and the velocity-level particle fields within the
+FP32 method error (tests/parity.py states why)."""
 import sys
 
 import numpy as np
@@ -18,16 +19,16 @@ pytestmark = pytest.mark.gpu
 TOL = 1.0e-5
 
 
-def _check(history, label):
+def _check(history, label, strict=False):
     for step, errs in enumerate(history):
         print(label, step, errs, flush=True)
-    check_history(history, label, TOL)
+    check_history(history, label, TOL, strict=strict)
 
 
 def test_gpu_penalty_32(hip, tmp_path):
     w = workloads.penalty_block(n=32, steps=10, out_dir=str(tmp_path))
     d, o = make_pair(w, tmp_path / "run", hip, None, with_fp64=True)
-    _check(run_and_compare(d, o, 10), "penalty32")
+    _check(run_and_compare(d, o, 10), "penalty32", strict=True)
 
 
 def test_gpu_moving_boundary_subcycled_32(hip, tmp_path):
@@ -51,7 +52,7 @@ def test_gpu_moving_boundary_subcycled_64(hip, tmp_path):
 def test_gpu_moving_boundary_one_step(hip, tmp_path):
     w = workloads.moving_boundary_unsubcycled(n=32, steps=5, flap=(2, 16, 12), out_dir=str(tmp_path))
     d, o = make_pair(w, tmp_path / "run", hip, None, with_fp64=True)
-    _check(run_and_compare(d, o, 5), "mb32_1step")
+    _check(run_and_compare(d, o, 5), "mb32_1step", strict=True)
 
 
 def test_gpu_driver_run_writes_outputs(hip, tmp_path):
