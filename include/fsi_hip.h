@@ -92,10 +92,12 @@ typedef struct FsiLbmState {
   float* solid_vel;       /* 3*N */
   float* cell_force;      /* 3*N */
   float* hydro_force;     /* 3*N */
-  float* link_c[2];       /* 18*N each: per-link force scalar, double-buffered by substep parity */
+  float* link_c[2];       /* 18*N each: per-link force scalar f_s + bounced of links that bounce off a
+                             moving wall or leave a BC-face cell, double-buffered by substep parity
+                             (links into walls at rest are read back from the populations) */
   float* link_corr;       /* 18*N: per-link moving-wall correction of the DIAG substep */
-  double* hydro_accum;    /* 3*N: subcycle force accumulator (fp64) */
-  float* accum_runmax;    /* N: running max of |accum| over the substeps */
+  double* hydro_accum;    /* 4*N, per cell: subcycle force accumulator x, y, z (fp64) and the running
+                             max of its norm over the substeps */
   FsiLbmCounters* counters;
 } FsiLbmState;
 
@@ -271,11 +273,12 @@ int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint3
 int fsi_lbm_mb_begin(const FsiLbmParams* p, const FsiLbmState* st, void* stream);
 
 /* finalize_moving_boundary_diagnostics + finalize_moving_boundary_force_accumulator
- * (fluid.py:522-536, :476-486): wall cells gather the last substep
- * (link_c[parity]); hydro_force = accum / n_samples when `subcycled`, else
- * the last substep's link force. */
-int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int parity, int n_samples,
-                        int subcycled, void* stream);
+ * (fluid.py:522-536, :476-486): wall cells gather the link forces of the last
+ * substep, which wrote its populations to F[cur] and its link scalars to
+ * link_c[parity]; hydro_force = accum / n_samples when `subcycled`, else the
+ * last substep's link force. */
+int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity,
+                        int n_samples, int subcycled, void* stream);
 
 /* Number of per-block partial records the two-pass reductions use; their
  * workspaces hold FSI_REDUCE_BLOCKS records of the reduced struct. */
@@ -313,6 +316,25 @@ int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpm
 /* MPMSolid3D.grid_update (solid.py:252-259) and g2p (:262-315). */
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
 int fsi_mpm_g2p(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
+
+/* Workgroup size of the per-particle kernels, and the workspace of
+ * fsi_mpm_substep_coupled: 6 doubles per particle workgroup. */
+#define FSI_PARTICLE_BLOCK 64
+#define FSI_COUPLED_WORKSPACE_DOUBLES(n_particles) \
+  (6 * (((n_particles) + FSI_PARTICLE_BLOCK - 1) / FSI_PARTICLE_BLOCK))
+
+/* One MPM substep with its coupling reaction, as the driver sequences issue it
+ * (fsi_driver.py:497-517 and :552-575): clear_grid, then p2g together with
+ * the reaction transfer in one pass over the particles, then grid_update
+ * (which also folds the reaction counters, deterministically), then g2p.
+ * mode 0: plain MPMSolid3D.substep (lp, ls, cp, cc and workspace may be
+ * NULL); 1: with add_lbm_reaction_to_mpm_grid; 2: with
+ * add_moving_boundary_reaction_to_mpm_grid.  Results equal those of the
+ * separate entry points called in that order, up to the summation order of
+ * the float scatter.  workspace: FSI_COUPLED_WORKSPACE_DOUBLES(n_particles). */
+int fsi_mpm_substep_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                            const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                            int mode, double* workspace, void* stream);
 
 /* MPMSolid3D.reduce_stats + FSIDiagnostics3D.solid_mean_velocity_norm (solid.py:342-357,
  * fsi_diagnostics.py:117-124). */

@@ -41,7 +41,8 @@ constexpr int NLINK = 18;  // link directions 1..18 (direction 0 never leaves th
 
 // cellinfo bits
 constexpr uint32_t CI_SOLID = 1u << 31;   // cell is solid
-constexpr uint32_t CI_MOVING = 1u << 30;  // fluid cell with a link to a solid cell of nonzero velocity
+constexpr uint32_t CI_MOVING = 1u << 30;  // fluid cell: has a link to a solid cell of nonzero velocity;
+                                          // solid cell: its own solid velocity is nonzero
 constexpr uint32_t CI_MASK = 0x7FFFEu;    // bits 1..18: link (fluid) / incoming (solid) directions
 
 // D3Q19 velocity set in the reference's order (fluid.py:359-369).
@@ -121,20 +122,36 @@ static inline float f16_to_f32(uint16_t h) {
 }
 #endif
 
+// Element c of a population plane through a 32-bit byte offset.  A lattice
+// holds at most 2^30 cells (checked by the C ABI), so c * sizeof(T) < 2^32:
+// the plane base (uniform: buffer + s * N) stays in scalar registers and each
+// lane adds one zero-extended 32-bit offset, which gfx950 global loads and
+// stores take directly (saddr + voffset form).
+template <typename T>
+FSI_HD T& at32(T* plane, uint32_t c) {
+  return *(T*)((const char*)plane + (size_t)(uint32_t)(c * (uint32_t)sizeof(T)));
+}
+
 struct PopF32 {
   float* p;
   FSI_HDM float get(long long N, int s, long long c) const { return p[s * N + c]; }
   FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = v; }
+  FSI_HDM float ld(uint32_t N, int s, uint32_t c) const { return at32(p + (size_t)s * N, c); }
+  FSI_HDM void st(uint32_t N, int s, uint32_t c, float v) const { at32(p + (size_t)s * N, c) = v; }
 };
 struct PopF16 {
   uint16_t* p;
   FSI_HDM float get(long long N, int s, long long c) const { return wgt(s) + f16_to_f32(p[s * N + c]); }
   FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = f32_to_f16(v - wgt(s)); }
+  FSI_HDM float ld(uint32_t N, int s, uint32_t c) const { return wgt(s) + f16_to_f32(at32(p + (size_t)s * N, c)); }
+  FSI_HDM void st(uint32_t N, int s, uint32_t c, float v) const { at32(p + (size_t)s * N, c) = f32_to_f16(v - wgt(s)); }
 };
 struct PopBF16 {
   uint16_t* p;
   FSI_HDM float get(long long N, int s, long long c) const { return wgt(s) + bf16_to_f32(p[s * N + c]); }
   FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = f32_to_bf16(v - wgt(s)); }
+  FSI_HDM float ld(uint32_t N, int s, uint32_t c) const { return wgt(s) + bf16_to_f32(at32(p + (size_t)s * N, c)); }
+  FSI_HDM void st(uint32_t N, int s, uint32_t c, float v) const { at32(p + (size_t)s * N, c) = f32_to_bf16(v - wgt(s)); }
 };
 
 template <typename Pop>
@@ -264,10 +281,8 @@ FSI_HD void add_invM(const float* r, float* f) {
 //   m11 = uy^2-uz^2, m13 = ux uy, m14 = uy uz, m15 = ux uz, others 0.
 // Guo term as written in the reference (fluid.py:408):
 //   g_l = w_l ((e_l - u).F / 3 + (e_l.u)(e_l.F) / 9).
-FSI_HD void mrt_collide(const float* F, float rho, float ux, float uy, float uz, bool has_force,
-                        float gx, float gy, float gz, float sv, float so, float* f) {
-  float m[Q];
-  moments_M(F, m);
+FSI_HD void mrt_collide_m(const float* F, const float* m, float ux, float uy, float uz, bool has_force,
+                          float gx, float gy, float gz, float sv, float so, float* f) {
   float r[Q];
   const float uu = ux * ux + uy * uy + uz * uz;
   r[0] = 0.0f;
@@ -296,7 +311,7 @@ FSI_HD void mrt_collide(const float* F, float rho, float ux, float uy, float uz,
     for (int l = 0; l < Q; ++l) {
       const float eF = (float)EX[l] * gx + (float)EY[l] * gy + (float)EZ[l] * gz;
       const float eu = (float)EX[l] * ux + (float)EY[l] * uy + (float)EZ[l] * uz;
-      g[l] = wgt(l) * ((eF - uF) / 3.0f + (eu * eF) / 9.0f);
+      g[l] = wgt(l) * ((eF - uF) * (1.0f / 3.0f) + (eu * eF) * (1.0f / 9.0f));
     }
     float mg[Q];
     moments_M(g, mg);
@@ -319,6 +334,14 @@ FSI_HD void mrt_collide(const float* F, float rho, float ux, float uy, float uz,
 #pragma unroll
   for (int s = 0; s < Q; ++s) f[s] = F[s];
   add_invM(r, f);
+}
+
+FSI_HD void mrt_collide(const float* F, float rho, float ux, float uy, float uz, bool has_force,
+                        float gx, float gy, float gz, float sv, float so, float* f) {
+  (void)rho;
+  float m[Q];
+  moments_M(F, m);
+  mrt_collide_m(F, m, ux, uy, uz, has_force, gx, gy, gz, sv, so, f);
 }
 
 // Regularised BGK (Latt & Chopard, Math. Comput. Simul. 72 (2006) 165):
@@ -363,21 +386,19 @@ FSI_HD void regularized_collide(const float* F, float rho, float ux, float uy, f
   }
 }
 
-// streaming3 moments (fluid.py:1795-1814): rho = sum F, u = (sum e F)/rho + (force/2)/rho.
+// streaming3 moments (fluid.py:1795-1814): rho = sum F, u = (sum e F)/rho + (force/2)/rho,
+// evaluated as u = (sum e F + force/2) * (1/rho) with the sums taken from the
+// MRT moment vector (m0 = rho, m3/m5/m7 = momentum): the fused substep uses
+// the same expressions, so stored and recomputed moments agree bit for bit.
 FSI_HD void moments_of(const float* F, float gx, float gy, float gz, float& rho, float& ux,
                        float& uy, float& uz) {
-  float r = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
-#pragma unroll
-  for (int s = 0; s < Q; ++s) {
-    r += F[s];
-    mx += (float)EX[s] * F[s];
-    my += (float)EY[s] * F[s];
-    mz += (float)EZ[s] * F[s];
-  }
-  rho = r;
-  ux = mx / r + (gx * 0.5f) / r;
-  uy = my / r + (gy * 0.5f) / r;
-  uz = mz / r + (gz * 0.5f) / r;
+  float m[Q];
+  moments_M(F, m);
+  rho = m[0];
+  const float inv = 1.0f / rho;
+  ux = (m[3] + 0.5f * gx) * inv;
+  uy = (m[5] + 0.5f * gy) * inv;
+  uz = (m[7] + 0.5f * gz) * inv;
 }
 
 // ---------------------------------------------------------------------------
@@ -516,6 +537,8 @@ FSI_HD void lbm_rebuild_cell(const FsiLbmParams& p, const FsiLbmState& st, int d
   uint32_t info = 0;
   if (now) {
     info = CI_SOLID;
+    if (st.solid_vel && (st.solid_vel[c] != 0.0f || st.solid_vel[g.N + c] != 0.0f || st.solid_vel[2 * g.N + c] != 0.0f))
+      info |= CI_MOVING;
 #pragma unroll
     for (int s = 1; s < Q; ++s) {
       // incoming link: the fluid cell c - e_s streams direction s into c
@@ -537,38 +560,40 @@ FSI_HD void lbm_rebuild_cell(const FsiLbmParams& p, const FsiLbmState& st, int d
   st.cellinfo[c] = info;
 }
 
-// Wall cell y gathers the per-link force scalars written by its fluid
-// neighbours in the previous substep and adds the resulting hydro force to
-// the fp64 accumulator.  Link (x = y - e_s, s) carries c = f_s + bounced, and
-// its force on the solid is e_s * c (fluid.py:510-519).
-FSI_HD void wall_gather(const Grid& g, const FsiLbmState& st, const float* link, uint32_t info,
-                        int i, int j, int k, long long y, double& hx, double& hy, double& hz) {
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-#pragma unroll
-  for (int s = 1; s < Q; ++s) {
-    if (info & (1u << s)) {
-      const long long x = neighbor(g, i, j, k, LR[s]);
-      const float cs = link[(long long)(s - 1) * g.N + x];
-      fx += (float)EX[s] * cs;
-      fy += (float)EY[s] * cs;
-      fz += (float)EZ[s] * cs;
-    }
+// Link scalars of the previous substep, seen from wall cell y.  The fluid cell
+// x = y - e_s of link (x, s) bounced f_s into its own slot F_next[x][LR[s]],
+// which no other cell writes; in the following substep that buffer is the
+// read-only source.  The link's force on the wall is e_s c_s with
+// c_s = f_s + bounced = 2 bounced - corr (fluid.py:510-519), so a wall at rest
+// (corr = 0) reads c_s = 2 F_src[x][LR[s]] directly.  Fluid cells that bounce
+// off a moving wall, and fluid cells on a BC face (whose populations the BC
+// overwrites), store c_s in link_c instead; y takes it from there when it is
+// itself moving (CI_MOVING on a solid cell) or x lies on a BC face.
+// Per-axis BC-face flags of the planes i-1, i, i+1 select the case per link.
+struct FaceFlags {
+  bool m, o, p;
+  FSI_HDM bool of_source(int s) const {  // is x = y - e_s on a BC face?
+    return EX[s] > 0 ? m : (EX[s] < 0 ? p : o);
   }
-  hx = fx;
-  hy = fy;
-  hz = fz;
+};
+
+FSI_HD FaceFlags face_flags(const FsiLbmParams& p, int i) {
+  FaceFlags f;
+  f.m = bc_face(p, i == 0 ? p.nx - 1 : i - 1);
+  f.o = bc_face(p, i);
+  f.p = bc_face(p, i == p.nx - 1 ? 0 : i + 1);
+  return f;
 }
 
-FSI_HD void wall_accumulate(const Grid& g, const FsiLbmState& st, long long y, double hx,
-                            double hy, double hz) {
-  const double ax = st.hydro_accum[y] + hx;
-  const double ay = st.hydro_accum[g.N + y] + hy;
-  const double az = st.hydro_accum[2 * g.N + y] + hz;
-  st.hydro_accum[y] = ax;
-  st.hydro_accum[g.N + y] = ay;
-  st.hydro_accum[2 * g.N + y] = az;
-  const float n = norm3((float)ax, (float)ay, (float)az);
-  if (n > st.accum_runmax[y]) st.accum_runmax[y] = n;
+// fp64 subcycle accumulator of cell y: 4 doubles (x, y, z, running max of |accum|).
+FSI_HD void wall_accumulate(double* A, uint32_t y, float fx, float fy, float fz) {
+  double* a = A + 4 * (size_t)y;
+  const double ax = a[0] + (double)fx, ay = a[1] + (double)fy, az = a[2] + (double)fz;
+  a[0] = ax;
+  a[1] = ay;
+  a[2] = az;
+  const double n = (double)norm3((float)ax, (float)ay, (float)az);
+  if (n > a[3]) a[3] = n;
 }
 
 // Previous-step velocity of cell (i,j,k) as the reference's `v` field holds it
@@ -601,85 +626,143 @@ FSI_HD void prev_velocity(const FsiLbmParams& p, const FsiLbmState& st, const Gr
 }
 
 
-// One fused LBM substep for fluid cell c (push form): collide F[cur] at c,
-// push the post-collision populations to the neighbours in F[1-cur], bounce
-// them back at solid neighbours (half-way, with the Ladd moving-wall
+// Periodic neighbour offsets of cell (i,j,k) (fluid.py:420-429) as signed
+// 32-bit element deltas per axis; d(s) = index(i+ex, j+ey, k+ez) - index(i,j,k).
+struct Nbr {
+  int32_t xp, xm, yp, ym, zp, zm;
+  FSI_HDM int32_t d(int s) const {
+    return (EX[s] > 0 ? xp : (EX[s] < 0 ? xm : 0)) + (EY[s] > 0 ? yp : (EY[s] < 0 ? ym : 0)) +
+           (EZ[s] > 0 ? zp : (EZ[s] < 0 ? zm : 0));
+  }
+};
+
+FSI_HD Nbr nbr_of(const FsiLbmParams& p, int i, int j, int k) {
+  const int32_t sx = p.ny * p.nz, sy = p.nz;
+  Nbr n;
+  n.xp = i == p.nx - 1 ? -(p.nx - 1) * sx : sx;
+  n.xm = i == 0 ? (p.nx - 1) * sx : -sx;
+  n.yp = j == p.ny - 1 ? -(p.ny - 1) * sy : sy;
+  n.ym = j == 0 ? (p.ny - 1) * sy : -sy;
+  n.zp = k == p.nz - 1 ? -(p.nz - 1) : 1;
+  n.zm = k == 0 ? (p.nz - 1) : -1;
+  return n;
+}
+
+// One fused LBM substep for cell (i,j,k) (push form): collide F[cur] at the
+// cell, push the post-collision populations to the neighbours in F[1-cur],
+// bounce them back at solid neighbours (half-way, with the Ladd moving-wall
 // correction when FSI_SUB_MOVING_WALL), and apply the legacy x-face BCs.
 // Every population of every fluid cell of F[1-cur] is written exactly once:
 // by the upstream fluid neighbour, by the cell's own bounce-back, or by the BC.
 // Wall (solid) cells gather the previous substep's link forces instead.
+// All lattice addressing is 32-bit (at most 2^30 cells, checked by the ABI).
 template <typename Pop>
 FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, int parity,
-                             uint32_t flags, long long c) {
-  const Grid g = make_grid(p);
-  const uint32_t info = st.cellinfo[c];
-  int i, j, k;
-  coords_of(g, c, i, j, k);
+                             uint32_t flags, int i, int j, int k) {
+  const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
+  const uint32_t c = ((uint32_t)i * (uint32_t)p.ny + (uint32_t)j) * (uint32_t)p.nz + (uint32_t)k;
+  const uint32_t info = at32(st.cellinfo, c);
+  const Nbr nb = nbr_of(p, i, j, k);
   const bool mb = (flags & FSI_SUB_MOVING_WALL) != 0;
-  if (info & CI_SOLID) {
-    if (mb && (flags & FSI_SUB_GATHER_PREV) && (info & CI_MASK)) {
-      double hx, hy, hz;
-      wall_gather(g, st, st.link_c[1 - parity], info, i, j, k, c, hx, hy, hz);
-      wall_accumulate(g, st, c, hx, hy, hz);
+  const bool solid = (info & CI_SOLID) != 0;
+  const bool gather = solid && mb && (flags & FSI_SUB_GATHER_PREV) && (info & CI_MASK);
+  if (solid && !gather) return;
+  const Pop Fs = pop_of<Pop>(st.F[cur]);
+  const FaceFlags ff = face_flags(p, i);
+  // One load phase for both kinds of cell, so that a wave holding fluid and
+  // wall cells waits for memory once: fluid cells load their populations,
+  // wall cells the link scalars c_s of their links into the same registers.
+  float F[Q];
+  if (!solid) {
+#pragma unroll
+    for (int s = 0; s < Q; ++s) F[s] = Fs.ld(N, s, c);
+  } else {
+    const float* link = st.link_c[1 - parity];
+    const bool wall_moving = (info & CI_MOVING) != 0;
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+      F[s] = 0.0f;
+      if (s != 0 && (info & (1u << s))) {
+        const uint32_t x = c + (uint32_t)nb.d(LR[s]);
+        if (wall_moving || ff.of_source(s))
+          F[s] = at32(link + (size_t)(s - 1) * N, x);
+        else
+          F[s] = 2.0f * Fs.ld(N, LR[s], x);
+      }
     }
+  }
+  if (solid) {
+    // wall cell: the force of its links is sum_s e_s c_s; added to the accumulator
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+    for (int s = 1; s < Q; ++s) {
+      fx += (float)EX[s] * F[s];
+      fy += (float)EY[s] * F[s];
+      fz += (float)EZ[s] * F[s];
+    }
+    wall_accumulate(st.hydro_accum, c, fx, fy, fz);
     return;
   }
-  const Pop Fs = pop_of<Pop>(st.F[cur]);
   const Pop Fd = pop_of<Pop>(st.F[1 - cur]);
-  float F[Q];
-#pragma unroll
-  for (int s = 0; s < Q; ++s) F[s] = Fs.get(g.N, s, c);
   float gx = p.force[0], gy = p.force[1], gz = p.force[2];
   if (flags & FSI_SUB_CELL_FORCE) {
-    gx += st.cell_force[c];
-    gy += st.cell_force[g.N + c];
-    gz += st.cell_force[2 * g.N + c];
+    gx += at32(st.cell_force, c);
+    gy += at32(st.cell_force + N, c);
+    gz += at32(st.cell_force + 2 * (size_t)N, c);
   }
   const bool has_force = (gx != 0.0f) || (gy != 0.0f) || (gz != 0.0f);
+  float m[Q];
+  moments_M(F, m);
   float rho, ux, uy, uz;
   if (flags & FSI_SUB_STORED_MOMENTS) {
-    rho = st.rho[c];
-    ux = st.vel[c];
-    uy = st.vel[g.N + c];
-    uz = st.vel[2 * g.N + c];
+    rho = at32(st.rho, c);
+    ux = at32(st.vel, c);
+    uy = at32(st.vel + N, c);
+    uz = at32(st.vel + 2 * (size_t)N, c);
   } else {
-    moments_of(F, gx, gy, gz, rho, ux, uy, uz);
+    // streaming3 (fluid.py:1795-1814), from the moment vector as in moments_of
+    rho = m[0];
+    const float inv = 1.0f / rho;
+    ux = (m[3] + 0.5f * gx) * inv;
+    uy = (m[5] + 0.5f * gy) * inv;
+    uz = (m[7] + 0.5f * gz) * inv;
   }
   float f[Q];
   if (p.collision == FSI_COLLIDE_REGULARIZED)
     regularized_collide(F, rho, ux, uy, uz, has_force, gx, gy, gz, p.s_v, f);
   else
-    mrt_collide(F, rho, ux, uy, uz, has_force, gx, gy, gz, p.s_v, p.s_other, f);
+    mrt_collide_m(F, m, ux, uy, uz, has_force, gx, gy, gz, p.s_v, p.s_other, f);
 
-  const bool self_face = bc_face(p, i);
+  const bool self_face = ff.o;
+  const bool ok_xp = !ff.p;
+  const bool ok_xm = !ff.m;
   float* link = mb ? st.link_c[parity] : nullptr;
   const bool diag = mb && (flags & FSI_SUB_DIAG);
+  const bool moving = mb && (info & CI_MOVING);
+  const bool keep_links = mb && (moving || self_face);  // walls cannot read these links from F
 #pragma unroll
   for (int s = 0; s < Q; ++s) {
     if (s != 0 && (info & (1u << s))) {
       float corr = 0.0f;
-      if (mb && (info & CI_MOVING)) {
-        const long long y = neighbor(g, i, j, k, s);
-        const float wx = st.solid_vel[y], wy = st.solid_vel[g.N + y], wz = st.solid_vel[2 * g.N + y];
+      if (moving) {
+        const uint32_t y = c + (uint32_t)nb.d(s);
+        const float wx = at32(st.solid_vel, y), wy = at32(st.solid_vel + N, y),
+                    wz = at32(st.solid_vel + 2 * (size_t)N, y);
         corr = -6.0f * wgt(s) * rho * ((float)EX[s] * wx + (float)EY[s] * wy + (float)EZ[s] * wz);
       }
       const float bounced = f[s] + corr;
-      if (!self_face) Fd.put(g.N, LR[s], c, bounced);
-      if (mb) {
-        link[(long long)(s - 1) * g.N + c] = f[s] + bounced;
-        if (diag) st.link_corr[(long long)(s - 1) * g.N + c] = corr;
-      }
+      if (!self_face) Fd.st(N, LR[s], c, bounced);
+      if (keep_links) at32(link + (size_t)(s - 1) * N, c) = f[s] + bounced;
+      if (diag) at32(st.link_corr + (size_t)(s - 1) * N, c) = corr;
     } else {
-      const int ti = wrap(i + EX[s], g.nx);
-      if (!bc_face(p, ti)) {
-        const long long y = cell_of(g, ti, wrap(j + EY[s], g.ny), wrap(k + EZ[s], g.nz));
-        Fd.put(g.N, s, y, f[s]);
-      }
+      const bool ok = EX[s] > 0 ? ok_xp : (EX[s] < 0 ? ok_xm : !self_face);
+      if (ok) Fd.st(N, s, c + (uint32_t)nb.d(s), f[s]);
     }
   }
 
   if (self_face) {
     // Boundary_condition_legacy (fluid.py:927-958); right face applied last.
+    const Grid g = make_grid(p);
     float brho = 1.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
     if (i == 0 && p.bc_x_left != FSI_BC_PERIODIC) {
       if (p.bc_x_left == FSI_BC_VELOCITY) {
@@ -712,7 +795,7 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
       }
     }
 #pragma unroll
-    for (int s = 0; s < Q; ++s) Fd.put(g.N, s, c, feq(s, brho, bx, by, bz));
+    for (int s = 0; s < Q; ++s) Fd.st(N, s, c, feq(s, brho, bx, by, bz));
   }
 }
 
@@ -756,10 +839,13 @@ struct FinalizeOut {
 };
 
 // finalize_moving_boundary_diagnostics / finalize_moving_boundary_force_accumulator
-// (fluid.py:522-536, :476-486) for cell y; writes hydro_force[y].
-FSI_HD FinalizeOut lbm_finalize_cell(const FsiLbmParams& p, const FsiLbmState& st, int parity,
-                                     int n_samples, int subcycled, long long y) {
+// (fluid.py:522-536, :476-486) for cell y after a substep that wrote its
+// populations to F[cur] and its link scalars to link_c[parity]; writes hydro_force[y].
+template <typename Pop>
+FSI_HD FinalizeOut lbm_finalize_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, int parity,
+                                     int n_samples, int subcycled, long long yl) {
   const Grid g = make_grid(p);
+  const uint32_t N = (uint32_t)g.N, y = (uint32_t)yl;
   FinalizeOut o;
   o.links = 0;
   o.max_corr = 0.0f;
@@ -769,29 +855,43 @@ FSI_HD FinalizeOut lbm_finalize_cell(const FsiLbmParams& p, const FsiLbmState& s
   const uint32_t info = st.cellinfo[y];
   if ((info & CI_SOLID) && (info & CI_MASK)) {
     int i, j, k;
-    coords_of(g, y, i, j, k);
-    wall_gather(g, st, st.link_c[parity], info, i, j, k, y, o.hx, o.hy, o.hz);
-    if (subcycled) wall_accumulate(g, st, y, o.hx, o.hy, o.hz);
+    coords_of(g, yl, i, j, k);
+    const Nbr nb = nbr_of(p, i, j, k);
+    const FaceFlags ff = face_flags(p, i);
+    const Pop Fc = pop_of<Pop>(st.F[cur]);
+    const float* link = st.link_c[parity];
+    const bool wall_moving = (info & CI_MOVING) != 0;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
     for (int s = 1; s < Q; ++s) {
       if (info & (1u << s)) {
         ++o.links;
-        const long long x = neighbor(g, i, j, k, LR[s]);
-        const float a = fabsf(st.link_corr[(long long)(s - 1) * g.N + x]);
+        const uint32_t x = y + (uint32_t)nb.d(LR[s]);
+        const float cs = (wall_moving || ff.of_source(s)) ? at32(link + (size_t)(s - 1) * N, x)
+                                                            : 2.0f * Fc.ld(N, LR[s], x);
+        fx += (float)EX[s] * cs;
+        fy += (float)EY[s] * cs;
+        fz += (float)EZ[s] * cs;
+        const float a = fabsf(at32(st.link_corr + (size_t)(s - 1) * N, x));
         if (a > o.max_corr) o.max_corr = a;
       }
     }
+    o.hx = fx;
+    o.hy = fy;
+    o.hz = fz;
+    if (subcycled) wall_accumulate(st.hydro_accum, y, fx, fy, fz);
   }
   if (subcycled) {
     const double inv = 1.0 / (double)(n_samples > 0 ? n_samples : 1);
-    const float mx = (float)(st.hydro_accum[y] * inv);
-    const float my = (float)(st.hydro_accum[g.N + y] * inv);
-    const float mz = (float)(st.hydro_accum[2 * g.N + y] * inv);
+    const double* a = st.hydro_accum + 4 * (size_t)y;
+    const float mx = (float)(a[0] * inv);
+    const float my = (float)(a[1] * inv);
+    const float mz = (float)(a[2] * inv);
     st.hydro_force[y] = mx;
     st.hydro_force[g.N + y] = my;
     st.hydro_force[2 * g.N + y] = mz;
     o.mean_norm = norm3(mx, my, mz);
-    o.runmax = st.accum_runmax[y];
+    o.runmax = (float)a[3];
   } else {
     st.hydro_force[y] = (float)o.hx;
     st.hydro_force[g.N + y] = (float)o.hy;
@@ -800,13 +900,11 @@ FSI_HD FinalizeOut lbm_finalize_cell(const FsiLbmParams& p, const FsiLbmState& s
   return o;
 }
 
-// clear_moving_boundary_force_accumulator + clear of hydro_force (fluid.py:444-466).
+// clear_moving_boundary_force_accumulator (fluid.py:461-466).
 FSI_HD void lbm_mb_begin_cell(const FsiLbmParams& p, const FsiLbmState& st, long long c) {
-  const Grid g = make_grid(p);
-  st.hydro_accum[c] = 0.0;
-  st.hydro_accum[g.N + c] = 0.0;
-  st.hydro_accum[2 * g.N + c] = 0.0;
-  st.accum_runmax[c] = 0.0f;
+  (void)p;
+  double* a = st.hydro_accum + 4 * (size_t)c;
+  a[0] = a[1] = a[2] = a[3] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -989,7 +1087,34 @@ FSI_HD bool inside3(int i, int j, int k, int nx, int ny, int nz) {
 }
 
 // p2g (solid.py:220-249) for particle p; Jp[p] = det F.
-FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p) {
+// Scatter target of the particle-to-grid passes: add v to channel ch of grid
+// node n = (I, J, K).  GlobalSink adds atomically into the grid arrays; the
+// GPU kernels may also scatter into a workgroup-private LDS tile first.
+struct GlobalSink {
+  float* ch[4];
+  FSI_HDM void add(int c, long long n, int, int, int, float v) const { atomic_add_f32(ch[c] + n, v); }
+};
+
+FSI_HD GlobalSink p2g_sink(const FsiMpmState& ms, long long G) {
+  GlobalSink k;
+  k.ch[0] = ms.grid_v;
+  k.ch[1] = ms.grid_v + G;
+  k.ch[2] = ms.grid_v + 2 * G;
+  k.ch[3] = ms.grid_m;
+  return k;
+}
+
+FSI_HD GlobalSink reaction_sink(const FsiMpmState& ms, long long G) {
+  GlobalSink k;
+  k.ch[0] = ms.grid_f;
+  k.ch[1] = ms.grid_f + G;
+  k.ch[2] = ms.grid_f + 2 * G;
+  k.ch[3] = nullptr;
+  return k;
+}
+
+template <typename Sink>
+FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p, const Sink& sink) {
   const MpmGrid g = make_mgrid(mp);
   const long long P = mp.n_particles;
   int base[3];
@@ -1042,10 +1167,10 @@ FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
         const float my = mvy + aff[3] * dpx + aff[4] * dpy + aff[5] * dpz;
         const float mz = mvz + aff[6] * dpx + aff[7] * dpy + aff[8] * dpz;
         const long long n = ((long long)I * g.ny + Jn) * g.nz + K;
-        atomic_add_f32(&ms.grid_v[n], wt * mx);
-        atomic_add_f32(&ms.grid_v[g.G + n], wt * my);
-        atomic_add_f32(&ms.grid_v[2 * g.G + n], wt * mz);
-        atomic_add_f32(&ms.grid_m[n], wt * mass);
+        sink.add(0, n, I, Jn, K, wt * mx);
+        sink.add(1, n, I, Jn, K, wt * my);
+        sink.add(2, n, I, Jn, K, wt * mz);
+        sink.add(3, n, I, Jn, K, wt * mass);
       }
 }
 
@@ -1274,9 +1399,10 @@ struct ReactOut {
 
 // Reaction transfer for particle p: mode 1 = penalty (penalty.py:90-129),
 // mode 2 = moving boundary (moving_boundary.py:58-111).
+template <typename Sink>
 FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
                                   const FsiMpmState& ms, const FsiCouplingParams& cp, int mode,
-                                  long long p) {
+                                  long long p, const Sink& sink) {
   const Grid g = make_grid(lp);
   const MpmGrid mg = make_mgrid(mp);
   const long long P = mp.n_particles;
@@ -1330,9 +1456,9 @@ FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
         const float wt = w[0][a] * w[1][b] * w[2][cc];
         const long long n = ((long long)I * mg.ny + Jn) * mg.nz + K;
         const float cx = wt * pf[0], cy = wt * pf[1], cz = wt * pf[2];
-        atomic_add_f32(&ms.grid_f[n], cx);
-        atomic_add_f32(&ms.grid_f[mg.G + n], cy);
-        atomic_add_f32(&ms.grid_f[2 * mg.G + n], cz);
+        sink.add(0, n, I, Jn, K, cx);
+        sink.add(1, n, I, Jn, K, cy);
+        sink.add(2, n, I, Jn, K, cz);
         const float cn = norm3(cx, cy, cz);
         if (cn > o.max_contrib) o.max_contrib = cn;
       }

@@ -74,7 +74,9 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
   if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
   by_storage(p->storage, [&](auto tag) {
     using Pop = decltype(tag);
-    for (long long c = 0; c < ncells(p); ++c) lbm_substep_cell<Pop>(*p, *st, cur, parity, flags, c);
+    for (int i = 0; i < p->nx; ++i)
+      for (int j = 0; j < p->ny; ++j)
+        for (int k = 0; k < p->nz; ++k) lbm_substep_cell<Pop>(*p, *st, cur, parity, flags, i, j, k);
   });
   return FSI_OK;
 }
@@ -93,13 +95,18 @@ int fsi_lbm_mb_begin(const FsiLbmParams* p, const FsiLbmState* st, void*) {
   return FSI_OK;
 }
 
-int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int parity, int n_samples, int subcycled, void*) {
+int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity, int n_samples,
+                        int subcycled, void*) {
   FsiLbmCounters* k = st->counters;
   k->bb_link_count = 0;
   k->bb_max_correction_bits = 0;
   for (int d = 0; d < 3; ++d) k->bb_net_solid_force[d] = 0.0;
   for (long long c = 0; c < ncells(p); ++c) {
-    FinalizeOut o = lbm_finalize_cell(*p, *st, parity, n_samples, subcycled, c);
+    FinalizeOut o;
+    by_storage(p->storage, [&](auto tag) {
+      using Pop = decltype(tag);
+      o = lbm_finalize_cell<Pop>(*p, *st, cur, parity, n_samples, subcycled, c);
+    });
     k->bb_link_count += o.links;
     atomic_max_nonneg(&k->bb_max_correction_bits, o.max_corr);
     if (subcycled) {
@@ -174,7 +181,8 @@ int fsi_mpm_clear_grid(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
 }
 
 int fsi_mpm_p2g(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
-  for (long long q = 0; q < mp->n_particles; ++q) mpm_p2g_particle(*mp, *ms, q);
+  const GlobalSink sink = p2g_sink(*ms, (long long)mp->gnx * mp->gny * mp->gnz);
+  for (long long q = 0; q < mp->n_particles; ++q) mpm_p2g_particle(*mp, *ms, q, sink);
   return FSI_OK;
 }
 
@@ -185,13 +193,27 @@ int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpm
   cc->max_particle_reaction_bits = cc->max_grid_reaction_bits = 0;
   for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] = cc->net_grid_reaction[d] = 0.0;
   for (long long q = 0; q < mp->n_particles; ++q) {
-    ReactOut o = reaction_particle(*lp, *ls, *mp, *ms, *cp, mode, q);
+    ReactOut o = reaction_particle(*lp, *ls, *mp, *ms, *cp, mode, q, reaction_sink(*ms, (long long)mp->gnx * mp->gny * mp->gnz));
     cc->active_reaction_particle_count += o.active;
     atomic_max_nonneg(&cc->max_particle_reaction_bits, o.particle_norm);
     atomic_max_nonneg(&cc->max_grid_reaction_bits, o.max_contrib);
     for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] += o.f[d];
   }
   return FSI_OK;
+}
+
+int fsi_mpm_substep_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                            const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode,
+                            double* workspace, void* stream) {
+  if (mode < 0 || mode > 2) return fail(FSI_E_INVALID, "mode must be 0 (none), 1 (penalty) or 2 (moving boundary)");
+  if (mode && (!lp || !ls || !cp || !cc || !workspace))
+    return fail(FSI_E_INVALID, "the reaction needs coupling params, counters, workspace, hydro_force, solid_phi");
+  int rc = fsi_mpm_clear_grid(mp, ms, stream);
+  if (!rc) rc = fsi_mpm_p2g(mp, ms, stream);
+  if (!rc && mode) rc = fsi_mpm_reaction(lp, ls, mp, ms, cp, cc, mode, stream);
+  if (!rc) rc = fsi_mpm_grid_update(mp, ms, stream);
+  if (!rc) rc = fsi_mpm_g2p(mp, ms, stream);
+  return rc;
 }
 
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
@@ -264,8 +286,11 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
       case FSI_BATCH_OP_LBM_SUBSTEP:
         by_storage(I.lp.storage, [&](auto tag) {
           using Pop = decltype(tag);
-          for (long long c = 0; c < cells; ++c)
-            lbm_substep_cell<Pop>(I.lp, I.ls, batch_cur(I, t), t & 1, batch_substep_flags(I, t), c);
+          (void)cells;
+          for (int a = 0; a < I.lp.nx; ++a)
+            for (int b = 0; b < I.lp.ny; ++b)
+              for (int k = 0; k < I.lp.nz; ++k)
+                lbm_substep_cell<Pop>(I.lp, I.ls, batch_cur(I, t), t & 1, batch_substep_flags(I, t), a, b, k);
         });
         break;
       case FSI_BATCH_OP_LBM_REBUILD:
@@ -289,7 +314,12 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
       case FSI_BATCH_OP_LBM_MB_FINALIZE: {
         FsiLbmCounters* k = I.ls.counters;
         for (long long c = 0; c < cells; ++c) {
-          FinalizeOut o = lbm_finalize_cell(I.lp, I.ls, (I.n_substeps - 1) & 1, I.n_substeps, 1, c);
+          FinalizeOut o;
+          by_storage(I.lp.storage, [&](auto tag) {
+            using Pop = decltype(tag);
+            o = lbm_finalize_cell<Pop>(I.lp, I.ls, batch_cur(I, I.n_substeps), (I.n_substeps - 1) & 1, I.n_substeps,
+                                       1, c);
+          });
           k->bb_link_count += o.links;
           atomic_max_nonneg(&k->bb_max_correction_bits, o.max_corr);
           atomic_max_nonneg(&k->mean_norm_max_bits, o.mean_norm);
@@ -342,12 +372,13 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
         memset(I.ms.grid_f, 0, sizeof(float) * 3 * nodes);
         break;
       case FSI_BATCH_OP_MPM_P2G:
-        for (long long q = 0; q < parts; ++q) mpm_p2g_particle(I.mp, I.ms, q);
+        for (long long q = 0; q < parts; ++q) mpm_p2g_particle(I.mp, I.ms, q, p2g_sink(I.ms, nodes));
         break;
       case FSI_BATCH_OP_MPM_REACTION:
         if (I.mode == FSI_BATCH_MODE_NONE) break;
         for (long long q = 0; q < parts; ++q) {
-          ReactOut o = reaction_particle(I.lp, I.ls, I.mp, I.ms, I.cp, I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1, q);
+          ReactOut o = reaction_particle(I.lp, I.ls, I.mp, I.ms, I.cp, I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1, q,
+                                          reaction_sink(I.ms, nodes));
           I.cc->active_reaction_particle_count += o.active;
           atomic_max_nonneg(&I.cc->max_particle_reaction_bits, o.particle_norm);
           atomic_max_nonneg(&I.cc->max_grid_reaction_bits, o.max_contrib);

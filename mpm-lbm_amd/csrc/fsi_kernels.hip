@@ -34,6 +34,12 @@ constexpr int TPB = 256;
 
 inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// Workgroup size of the per-particle kernels.  Particle counts are small next
+// to lattice sizes (tens of thousands), so small workgroups spread them over
+// all 256 CUs instead of a few dozen.
+constexpr int PTPB = FSI_PARTICLE_BLOCK;
+inline unsigned pblocks_for(long long n) { return (unsigned)((n + PTPB - 1) / PTPB); }
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Run STMT with POP bound to the population accessor of a storage code.
@@ -55,6 +61,8 @@ int check_lbm(const FsiLbmParams* p, const FsiLbmState* st) {
   if (!p || !st) return fail(FSI_E_INVALID, "null params/state");
   if (p->nx < 3 || p->ny < 3 || p->nz < 3)
     return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
+  if ((long long)p->nx * p->ny * p->nz > (1LL << 30))
+    return fail(FSI_E_INVALID, "lattice must hold at most 2^30 cells (32-bit cell offsets)");
   if (p->bc_x_left < 0 || p->bc_x_left > 2 || p->bc_x_right < 0 || p->bc_x_right > 2)
     return fail(FSI_E_INVALID, "bc_x_left/bc_x_right must be 0, 1 or 2");
   if (p->storage < FSI_STORE_F32 || p->storage > FSI_STORE_BF16_DEV)
@@ -94,7 +102,7 @@ __device__ double block_reduce(double v, int op, double* smem) {
   __syncthreads();
   double r = red_identity(op);
   if (threadIdx.x == 0)
-    for (int w = 0; w < TPB / 64; ++w) r = red_combine(op, r, smem[w]);
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) r = red_combine(op, r, smem[w]);
   return r;
 }
 
@@ -110,7 +118,7 @@ __device__ __forceinline__ float block_max_f(float v, float* smem) {
   __syncthreads();
   float r = 0.0f;
   if (threadIdx.x == 0)
-    for (int w = 0; w < TPB / 64; ++w) r = smem[w] > r ? smem[w] : r;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) r = smem[w] > r ? smem[w] : r;
   return r;
 }
 
@@ -123,7 +131,7 @@ __device__ __forceinline__ int block_sum_i(int v, int* smem) {
   __syncthreads();
   int r = 0;
   if (threadIdx.x == 0)
-    for (int w = 0; w < TPB / 64; ++w) r += smem[w];
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) r += smem[w];
   return r;
 }
 
@@ -142,13 +150,33 @@ __global__ void __launch_bounds__(TPB) k_lbm_rebuild(FsiLbmParams p, FsiLbmState
   if (c < (long long)p.nx * p.ny * p.nz) lbm_rebuild_cell<Pop>(p, st, dynamic, cur, c);
 }
 
+// Substep launch geometry.  A workgroup covers TPB consecutive cells of one
+// i-plane (the (j,k) plane is contiguous in memory), so i, the x-neighbour
+// offsets and the x-face BC tests are uniform per workgroup.  Workgroups are
+// handed to the 8 XCDs round-robin by dispatch order; the logical tile of
+// dispatch slot b is chosen so that XCD x receives one contiguous run of
+// tiles, i.e. a slab of i-planes, and keeps that slab (and its L2 lines) in
+// every substep.
+constexpr unsigned NXCD = 8;
+
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
+  const unsigned per = n / NXCD, rem = n % NXCD, x = b % NXCD;
+  return x * per + (x < rem ? x : rem) + b / NXCD;
+}
+
 template <typename Pop, uint32_t FIXED>
 __global__ void __launch_bounds__(TPB) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
-                                                     uint32_t flags) {
-  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
-  if (c < (long long)p.nx * p.ny * p.nz)
-    lbm_substep_cell<Pop>(p, st, cur, parity,
-                          (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED, c);
+                                                     uint32_t flags, unsigned tiles_per_plane) {
+  const unsigned L = xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned i = L / tiles_per_plane;
+  const unsigned jk = (L - i * tiles_per_plane) * TPB + threadIdx.x;
+  const unsigned plane = (unsigned)p.ny * (unsigned)p.nz;
+  if (jk >= plane) return;
+  const unsigned j = jk / (unsigned)p.nz;
+  const unsigned k = jk - j * (unsigned)p.nz;
+  lbm_substep_cell<Pop>(p, st, cur, parity,
+                        (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED,
+                        (int)i, (int)j, (int)k);
 }
 
 template <typename Pop>
@@ -182,8 +210,9 @@ __global__ void __launch_bounds__(TPB) k_lbm_finalize_clear(FsiLbmState st) {
   }
 }
 
-__global__ void __launch_bounds__(TPB) k_lbm_finalize(FsiLbmParams p, FsiLbmState st, int parity, int n_samples,
-                                                      int subcycled) {
+template <typename Pop>
+__global__ void __launch_bounds__(TPB) k_lbm_finalize(FsiLbmParams p, FsiLbmState st, int cur, int parity,
+                                                      int n_samples, int subcycled) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
@@ -194,7 +223,7 @@ __global__ void __launch_bounds__(TPB) k_lbm_finalize(FsiLbmParams p, FsiLbmStat
   o.mean_norm = 0.0f;
   o.runmax = 0.0f;
   o.hx = o.hy = o.hz = 0.0;
-  if (c < (long long)p.nx * p.ny * p.nz) o = lbm_finalize_cell(p, st, parity, n_samples, subcycled, c);
+  if (c < (long long)p.nx * p.ny * p.nz) o = lbm_finalize_cell<Pop>(p, st, cur, parity, n_samples, subcycled, c);
   const int links = block_sum_i(o.links, si);
   const float mc = block_max_f(o.max_corr, sf);
   const float mn = block_max_f(o.mean_norm, sf);
@@ -265,9 +294,9 @@ __global__ void __launch_bounds__(TPB) k_proj_clear(FsiLbmParams p, FsiLbmState 
   }
 }
 
-__global__ void __launch_bounds__(TPB) k_proj_particles(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
-                                                        FsiMpmState ms, FsiCouplingParams cp) {
-  const long long q = (long long)blockIdx.x * TPB + threadIdx.x;
+__global__ void __launch_bounds__(PTPB) k_proj_particles(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+                                                         FsiMpmState ms, FsiCouplingParams cp) {
+  const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
   if (q < mp.n_particles) project_particle(lp, ls, mp, ms, cp, q);
 }
 
@@ -352,9 +381,172 @@ __global__ void __launch_bounds__(TPB) k_mpm_clear(FsiMpmParams mp, FsiMpmState 
   }
 }
 
-__global__ void __launch_bounds__(TPB) k_mpm_p2g(FsiMpmParams mp, FsiMpmState ms) {
-  const long long q = (long long)blockIdx.x * TPB + threadIdx.x;
-  if (q < mp.n_particles) mpm_p2g_particle(mp, ms, q);
+// ---------------------------------------------------------------------------
+// Workgroup-private scatter tiles for the particle-to-grid passes.  The nodes a
+// workgroup's particles touch lie in the box spanned by their B-spline base
+// nodes (+2 along each axis).  When that box holds at most TILE_NODES nodes the
+// workgroup accumulates its scatter in LDS (ds_add_f32) and then adds each
+// touched node to the grid once; otherwise it scatters with global atomics.
+// Particles stored in spatial order (as lattice-sampled clouds are) give
+// boxes of a few hundred nodes for 256 particles, replacing 27 global atomics
+// per particle and channel by a handful.
+// ---------------------------------------------------------------------------
+constexpr int TILE_NODES = 2048;
+
+struct TileBox {
+  int lx, ly, lz, ex, ey, ez, V;
+};
+
+// Box of the block's particles' stencils: base nodes int(x * inv - 0.5), as the
+// bodies compute them, plus 2.  Should a body's base differ from this one (a
+// different contraction of the expression), LdsSink sends that node to the
+// global grid, so the box never has to be exact.  V = 0: no valid particle or
+// the box does not fit.  Every thread of the block must call this.
+__device__ __forceinline__ TileBox block_tile_box(const float* x, long long P, long long q, bool valid, float inv, int* sbox,
+                                                  int cap = TILE_NODES) {
+  if (threadIdx.x == 0) {
+    sbox[0] = sbox[1] = sbox[2] = 0x7FFFFFFF;
+    sbox[3] = sbox[4] = sbox[5] = -0x7FFFFFFF;
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float X = x[d * P + q] * inv;
+      if (X > -1.0e6f && X < 1.0e6f) {
+        const int b = (int)(X - 0.5f);
+        atomicMin(&sbox[d], b);
+        atomicMax(&sbox[3 + d], b);
+      }
+    }
+  }
+  __syncthreads();
+  TileBox t;
+  t.lx = sbox[0];
+  t.ly = sbox[1];
+  t.lz = sbox[2];
+  t.V = 0;
+  t.ex = t.ey = t.ez = 0;
+  if (sbox[0] <= sbox[3] && sbox[1] <= sbox[4] && sbox[2] <= sbox[5]) {
+    const long long ex = (long long)sbox[3] - sbox[0] + 3, ey = (long long)sbox[4] - sbox[1] + 3,
+                    ez = (long long)sbox[5] - sbox[2] + 3;
+    if (ex * ey * ez <= cap) {
+      t.ex = (int)ex;
+      t.ey = (int)ey;
+      t.ez = (int)ez;
+      t.V = (int)(ex * ey * ez);
+    }
+  }
+  __syncthreads();  // sbox may be reused by the caller
+  return t;
+}
+
+// LDS scatter target; a node outside the box goes to the global grid.
+struct LdsSink {
+  float* tile;
+  TileBox b;
+  GlobalSink g;
+  __device__ __forceinline__ void add(int c, long long n, int I, int J, int K, float v) const {
+    const unsigned a = (unsigned)(I - b.lx), e = (unsigned)(J - b.ly), f = (unsigned)(K - b.lz);
+    // workgroup scope for the LDS add (ds_add_f32), device scope for the global one: the two stay
+    // separate instructions instead of one flat atomic on a merged pointer
+    if (a < (unsigned)b.ex && e < (unsigned)b.ey && f < (unsigned)b.ez)
+      __hip_atomic_fetch_add(tile + c * b.V + ((int)a * b.ey + (int)e) * b.ez + (int)f, v, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+      atomicAdd(g.ch[c] + n, v);
+  }
+};
+
+template <int NCH>
+__device__ __forceinline__ void tile_clear(float* tile, const TileBox& b) {
+  for (int i = threadIdx.x; i < NCH * b.V; i += (int)blockDim.x) tile[i] = 0.0f;
+  __syncthreads();
+}
+
+// Add the tile to the grid: one global atomic per touched node and channel.
+template <int NCH>
+__device__ __forceinline__ void tile_flush(const float* tile, const TileBox& b, const GlobalSink& g, int gnx, int gny, int gnz) {
+  __syncthreads();
+  const int plane = b.ey * b.ez;
+  for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
+    const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
+    const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
+    if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
+    const long long n = ((long long)I * gny + J) * gnz + K;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const float v = tile[c * b.V + i];
+      if (v != 0.0f) atomicAdd(g.ch[c] + n, v);
+    }
+  }
+}
+
+// Flush of a tile whose channels go to the grid arrays ch[0..nch).
+__device__ __forceinline__ void tile_flush_ch(const float* tile, const TileBox& b, float* const* ch, int nch, int gnx,
+                                              int gny, int gnz) {
+  __syncthreads();
+  const int plane = b.ey * b.ez;
+  for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
+    const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
+    const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
+    if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
+    const long long n = ((long long)I * gny + J) * gnz + K;
+    for (int c = 0; c < nch; ++c) {
+      const float v = tile[c * b.V + i];
+      if (v != 0.0f) atomicAdd(ch[c] + n, v);
+    }
+  }
+}
+
+// P2G of particle q (valid) by a whole workgroup.
+__device__ __forceinline__ void p2g_block(const FsiMpmParams& mp, const FsiMpmState& ms, long long q, bool valid, float* tile,
+                          int* sbox) {
+  const GlobalSink g = p2g_sink(ms, (long long)mp.gnx * mp.gny * mp.gnz);
+  const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, mp.inv_dx, sbox);
+  if (b.V > 0) {
+    tile_clear<4>(tile, b);
+    LdsSink k;
+    k.tile = tile;
+    k.b = b;
+    k.g = g;
+    if (valid) mpm_p2g_particle(mp, ms, q, k);
+    tile_flush<4>(tile, b, g, mp.gnx, mp.gny, mp.gnz);
+  } else if (valid) {
+    mpm_p2g_particle(mp, ms, q, g);
+  }
+}
+
+__global__ void __launch_bounds__(PTPB) k_mpm_p2g(FsiMpmParams mp, FsiMpmState ms) {
+  __shared__ float tile[4 * TILE_NODES];
+  __shared__ int sbox[6];
+  const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
+  p2g_block(mp, ms, q, q < mp.n_particles, tile, sbox);
+}
+
+// Reaction scatter of particle q (valid) by a whole workgroup.
+__device__ __forceinline__ ReactOut reaction_block(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                                   const FsiMpmState& ms, const FsiCouplingParams& cp, int mode, long long q,
+                                   bool valid, float* tile, int* sbox) {
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = 0.0f;
+  o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  const GlobalSink g = reaction_sink(ms, (long long)mp.gnx * mp.gny * mp.gnz);
+  const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, cp.inv_dx_norm, sbox);
+  if (b.V > 0) {
+    tile_clear<3>(tile, b);
+    LdsSink k;
+    k.tile = tile;
+    k.b = b;
+    k.g = g;
+    if (valid) o = reaction_particle(lp, ls, mp, ms, cp, mode, q, k);
+    tile_flush<3>(tile, b, g, mp.gnx, mp.gny, mp.gnz);
+  } else if (valid) {
+    o = reaction_particle(lp, ls, mp, ms, cp, mode, q, g);
+  }
+  return o;
 }
 
 __global__ void k_reaction_clear(FsiCouplingCounters* cc) {
@@ -369,19 +561,16 @@ __global__ void k_reaction_clear(FsiCouplingCounters* cc) {
   }
 }
 
-__global__ void __launch_bounds__(TPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+__global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
                                                       FsiMpmState ms, FsiCouplingParams cp,
                                                       FsiCouplingCounters* cc, int mode) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
-  const long long q = (long long)blockIdx.x * TPB + threadIdx.x;
-  ReactOut o;
-  o.active = 0;
-  o.particle_norm = 0.0f;
-  o.max_contrib = 0.0f;
-  o.f[0] = o.f[1] = o.f[2] = 0.0f;
-  if (q < mp.n_particles) o = reaction_particle(lp, ls, mp, ms, cp, mode, q);
+  __shared__ float tile[3 * TILE_NODES];
+  __shared__ int sbox[6];
+  const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
+  const ReactOut o = reaction_block(lp, ls, mp, ms, cp, mode, q, q < mp.n_particles, tile, sbox);
   const int act = block_sum_i(o.active, si);
   const float mpn = block_max_f(o.particle_norm, sf);
   const float mgc = block_max_f(o.max_contrib, sf);
@@ -396,13 +585,116 @@ __global__ void __launch_bounds__(TPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSta
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused MPM substep with the coupling reaction.  One pass over the particles
+// scatters P2G (grid_v, grid_m) and the reaction (grid_f) into one 7-channel
+// LDS tile; the reaction reads the J that P2G just stored for the particle, as
+// the reference's p2g -> add_*_reaction_to_mpm_grid order does.  Each block
+// leaves its reaction reductions in a workspace record; block 0 of the grid
+// update folds the records in block order into the counters (deterministic,
+// and no contended same-address atomics).
+// ---------------------------------------------------------------------------
+constexpr int FUSED_TILE_NODES = 1024;
+constexpr int REACT_REC = 6;  // active count, max particle norm, max grid contribution, net force x, y, z
+
+template <int MODE>  // 0: no reaction, 1: penalty reaction, 2: moving-boundary reaction
+__global__ void __launch_bounds__(PTPB) k_mpm_p2g_react(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+                                                        FsiMpmState ms, FsiCouplingParams cp, double* rec) {
+  __shared__ float tile[7 * FUSED_TILE_NODES];
+  __shared__ int sbox[6];
+  __shared__ double sd[PTPB / 64];
+  __shared__ float sf[PTPB / 64];
+  __shared__ int si[PTPB / 64];
+  const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
+  const bool valid = q < mp.n_particles;
+  const long long G = (long long)mp.gnx * mp.gny * mp.gnz;
+  const GlobalSink gp = p2g_sink(ms, G), gr = reaction_sink(ms, G);
+  const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, mp.inv_dx, sbox, FUSED_TILE_NODES);
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  if (b.V > 0) {
+    constexpr int NCH = MODE ? 7 : 4;
+    tile_clear<NCH>(tile, b);
+    LdsSink kp;
+    kp.tile = tile;
+    kp.b = b;
+    kp.g = gp;
+    if (valid) mpm_p2g_particle(mp, ms, q, kp);
+    if (MODE) {
+      LdsSink kr;
+      kr.tile = tile + 4 * b.V;
+      kr.b = b;
+      kr.g = gr;
+      if (valid) o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, kr);
+    }
+    float* ch[7] = {gp.ch[0], gp.ch[1], gp.ch[2], gp.ch[3], gr.ch[0], gr.ch[1], gr.ch[2]};
+    tile_flush_ch(tile, b, ch, NCH, mp.gnx, mp.gny, mp.gnz);
+  } else if (valid) {
+    mpm_p2g_particle(mp, ms, q, gp);
+    if (MODE) o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, gr);
+  }
+  if (MODE) {
+    const int act = block_sum_i(o.active, si);
+    const float mpn = block_max_f(o.particle_norm, sf);
+    const float mgc = block_max_f(o.max_contrib, sf);
+    double net[3];
+    for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)o.f[d], RED_SUM, sd);
+    if (threadIdx.x == 0) {
+      double* r = rec + (size_t)blockIdx.x * REACT_REC;
+      r[0] = (double)act;
+      r[1] = (double)mpn;
+      r[2] = (double)mgc;
+      r[3] = net[0];
+      r[4] = net[1];
+      r[5] = net[2];
+    }
+  }
+}
+
+// grid_update, and (block 0) the fold of the reaction records into the counters.
+__global__ void __launch_bounds__(TPB) k_mpm_grid_update_react(FsiMpmParams mp, FsiMpmState ms, const double* rec,
+                                                               int n_rec, FsiCouplingCounters* cc) {
+  __shared__ double sd[TPB / 64];
+  const long long n = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (n < (long long)mp.gnx * mp.gny * mp.gnz) mpm_grid_node(mp, ms, n);
+  if (rec && blockIdx.x == 0) {
+    double a[REACT_REC] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < n_rec; k += TPB) {
+      const double* r = rec + (size_t)k * REACT_REC;
+      a[0] += r[0];
+      a[1] = r[1] > a[1] ? r[1] : a[1];
+      a[2] = r[2] > a[2] ? r[2] : a[2];
+      a[3] += r[3];
+      a[4] += r[4];
+      a[5] += r[5];
+    }
+    const double act = block_reduce(a[0], RED_SUM, sd);
+    const double mpn = block_reduce(a[1], RED_MAX, sd);
+    const double mgc = block_reduce(a[2], RED_MAX, sd);
+    const double nx = block_reduce(a[3], RED_SUM, sd);
+    const double ny = block_reduce(a[4], RED_SUM, sd);
+    const double nz = block_reduce(a[5], RED_SUM, sd);
+    if (threadIdx.x == 0) {
+      cc->active_reaction_particle_count = (int32_t)act;
+      cc->max_particle_reaction_bits = __float_as_uint((float)mpn);
+      cc->max_grid_reaction_bits = __float_as_uint((float)mgc);
+      cc->net_particle_reaction[0] = nx;
+      cc->net_particle_reaction[1] = ny;
+      cc->net_particle_reaction[2] = nz;
+      cc->net_grid_reaction[0] = cc->net_grid_reaction[1] = cc->net_grid_reaction[2] = 0.0;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(TPB) k_mpm_grid_update(FsiMpmParams mp, FsiMpmState ms) {
   const long long n = (long long)blockIdx.x * TPB + threadIdx.x;
   if (n < (long long)mp.gnx * mp.gny * mp.gnz) mpm_grid_node(mp, ms, n);
 }
 
-__global__ void __launch_bounds__(TPB) k_mpm_g2p(FsiMpmParams mp, FsiMpmState ms) {
-  const long long q = (long long)blockIdx.x * TPB + threadIdx.x;
+__global__ void __launch_bounds__(PTPB) k_mpm_g2p(FsiMpmParams mp, FsiMpmState ms) {
+  const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
   if (q < mp.n_particles) mpm_g2p_particle(mp, ms, q);
 }
 
@@ -449,12 +741,14 @@ __global__ void __launch_bounds__(TPB) kb_substep(const FsiBatchInstance* inst, 
   const int cur = batch_cur(I, t), parity = t & 1;
   const FsiLbmParams p = I.lp;
   const FsiLbmState st = I.ls;
+  const unsigned e = (unsigned)e_, t1 = e / (unsigned)p.nz, k = e - t1 * (unsigned)p.nz;
+  const unsigned i = t1 / (unsigned)p.ny, j = t1 - i * (unsigned)p.ny;
   if (p.storage == FSI_STORE_F16_DEV)
-    lbm_substep_cell<PopF16>(p, st, cur, parity, flags, e_);
+    lbm_substep_cell<PopF16>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
   else if (p.storage == FSI_STORE_BF16_DEV)
-    lbm_substep_cell<PopBF16>(p, st, cur, parity, flags, e_);
+    lbm_substep_cell<PopBF16>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
   else
-    lbm_substep_cell<PopF32>(p, st, cur, parity, flags, e_);
+    lbm_substep_cell<PopF32>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
 }
 
 __global__ void __launch_bounds__(TPB) kb_rebuild(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
@@ -496,7 +790,9 @@ __global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* inst,
   o.max_corr = o.mean_norm = o.runmax = 0.0f;
   o.hx = o.hy = o.hz = 0.0;
   const int last = I.n_substeps - 1;
-  if (in_) o = lbm_finalize_cell(I.lp, I.ls, last & 1, I.n_substeps, 1, e_);
+  if (in_)
+    FSI_BY_STORAGE(I.lp.storage, o = lbm_finalize_cell<POP>(I.lp, I.ls, batch_cur(I, I.n_substeps), last & 1,
+                                                            I.n_substeps, 1, e_));
   const int links = block_sum_i(o.links, si);
   const float mc = block_max_f(o.max_corr, sf);
   const float mn = block_max_f(o.mean_norm, sf);
@@ -615,21 +911,22 @@ __global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst
 }
 
 __global__ void __launch_bounds__(TPB) kb_mpm_p2g(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+  __shared__ float tile[4 * TILE_NODES];
+  __shared__ int sbox[6];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
-  if (in_) mpm_p2g_particle(I.mp, I.ms, e_);
+  p2g_block(I.mp, I.ms, e_, in_, tile, sbox);
 }
 
 __global__ void __launch_bounds__(TPB) kb_mpm_reaction(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
+  __shared__ float tile[3 * TILE_NODES];
+  __shared__ int sbox[6];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
-  ReactOut o;
-  o.active = 0;
-  o.particle_norm = o.max_contrib = 0.0f;
-  o.f[0] = o.f[1] = o.f[2] = 0.0f;
   const int mode = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1;
-  if (in_ && I.mode != FSI_BATCH_MODE_NONE) o = reaction_particle(I.lp, I.ls, I.mp, I.ms, I.cp, mode, e_);
+  const ReactOut o = reaction_block(I.lp, I.ls, I.mp, I.ms, I.cp, mode, e_, in_ && I.mode != FSI_BATCH_MODE_NONE,
+                                    tile, sbox);
   const int act = block_sum_i(o.active, si);
   const float mpn = block_max_f(o.particle_norm, sf);
   const float mgc = block_max_f(o.max_contrib, sf);
@@ -711,19 +1008,19 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
   if ((cur != 0 && cur != 1) || (parity != 0 && parity != 1))
     return fail(FSI_E_INVALID, "cur and parity must be 0 or 1");
   const bool mb = flags & FSI_SUB_MOVING_WALL;
-  if (mb && (!st->link_c[0] || !st->link_c[1] || !st->solid_vel || !st->hydro_accum || !st->accum_runmax))
-    return fail(FSI_E_INVALID, "moving-wall substep needs link_c, solid_vel, hydro_accum, accum_runmax");
+  if (mb && (!st->link_c[0] || !st->link_c[1] || !st->solid_vel || !st->hydro_accum))
+    return fail(FSI_E_INVALID, "moving-wall substep needs link_c, solid_vel, hydro_accum");
   if (mb && (flags & FSI_SUB_DIAG) && !st->link_corr)
     return fail(FSI_E_INVALID, "diagnostic substep needs link_corr");
   if ((flags & FSI_SUB_CELL_FORCE) && !st->cell_force) return fail(FSI_E_INVALID, "cell_force is null");
-  const long long N = (long long)p->nx * p->ny * p->nz;
-  const dim3 grid(blocks_for(N)), block(TPB);
+  const unsigned tiles = blocks_for((long long)p->ny * p->nz);
+  const dim3 grid(tiles * (unsigned)p->nx), block(TPB);
   hipStream_t s = as_stream(stream);
   const uint32_t fixed = flags & (FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE);
   switch (fixed) {
 #define FSI_CASE(V)                                                                   \
   case (V):                                                                           \
-    FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V)>), grid, block, 0, s, *p, *st, cur, parity, flags)); \
+    FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V)>), grid, block, 0, s, *p, *st, cur, parity, flags, tiles)); \
     break;
     FSI_CASE(0u)
     FSI_CASE(1u)
@@ -754,26 +1051,27 @@ int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint3
 int fsi_lbm_mb_begin(const FsiLbmParams* p, const FsiLbmState* st, void* stream) {
   int rc = check_lbm(p, st);
   if (rc) return rc;
-  if (!st->hydro_accum || !st->accum_runmax || !st->counters)
-    return fail(FSI_E_INVALID, "mb_begin needs hydro_accum, accum_runmax, counters");
+  if (!st->hydro_accum || !st->counters)
+    return fail(FSI_E_INVALID, "mb_begin needs hydro_accum, counters");
   const long long N = (long long)p->nx * p->ny * p->nz;
   hipLaunchKernelGGL(k_lbm_mb_begin, dim3(blocks_for(N)), dim3(TPB), 0, as_stream(stream), *p, *st);
   return check_launch("k_lbm_mb_begin");
 }
 
-int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int parity, int n_samples, int subcycled,
-                        void* stream) {
+int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity, int n_samples,
+                        int subcycled, void* stream) {
   int rc = check_lbm(p, st);
   if (rc) return rc;
-  if (parity != 0 && parity != 1) return fail(FSI_E_INVALID, "parity must be 0 or 1");
-  if (!st->link_c[0] || !st->link_c[1] || !st->link_corr || !st->hydro_accum || !st->accum_runmax ||
-      !st->hydro_force || !st->counters)
+  if ((cur != 0 && cur != 1) || (parity != 0 && parity != 1))
+    return fail(FSI_E_INVALID, "cur and parity must be 0 or 1");
+  if (!st->link_c[0] || !st->link_c[1] || !st->link_corr || !st->hydro_accum || !st->hydro_force || !st->counters)
     return fail(FSI_E_INVALID, "mb_finalize needs link buffers, accumulator, hydro_force, counters");
   if (n_samples < 1) return fail(FSI_E_INVALID, "n_samples must be positive");
   const long long N = (long long)p->nx * p->ny * p->nz;
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(k_lbm_finalize_clear, dim3(1), dim3(64), 0, s, *st);
-  hipLaunchKernelGGL(k_lbm_finalize, dim3(blocks_for(N)), dim3(TPB), 0, s, *p, *st, parity, n_samples, subcycled);
+  FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL(k_lbm_finalize<POP>, dim3(blocks_for(N)), dim3(TPB), 0, s, *p, *st,
+                                                cur, parity, n_samples, subcycled));
   return check_launch("k_lbm_finalize");
 }
 
@@ -802,7 +1100,7 @@ int fsi_project(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParam
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(k_proj_clear, dim3(blocks_for(N)), dim3(TPB), 0, s, *lp, *ls, cc);
   if (mp->n_particles > 0)
-    hipLaunchKernelGGL(k_proj_particles, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, s, *lp, *ls, *mp, *ms, *cp);
+    hipLaunchKernelGGL(k_proj_particles, dim3(pblocks_for(mp->n_particles)), dim3(PTPB), 0, s, *lp, *ls, *mp, *ms, *cp);
   hipLaunchKernelGGL(k_proj_normalize, dim3(blocks_for(N)), dim3(TPB), 0, s, *lp, *ls, *cp, cc);
   return check_launch("fsi_project");
 }
@@ -833,7 +1131,7 @@ int fsi_mpm_p2g(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
   if (mp->n_particles > 0)
-    hipLaunchKernelGGL(k_mpm_p2g, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, s, *mp, *ms);
+    hipLaunchKernelGGL(k_mpm_p2g, dim3(pblocks_for(mp->n_particles)), dim3(PTPB), 0, s, *mp, *ms);
   return check_launch("fsi_mpm_p2g");
 }
 
@@ -848,9 +1146,50 @@ int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpm
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(k_reaction_clear, dim3(1), dim3(64), 0, s, cc);
   if (mp->n_particles > 0)
-    hipLaunchKernelGGL(k_mpm_reaction, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, s, *lp, *ls, *mp, *ms, *cp, cc,
+    hipLaunchKernelGGL(k_mpm_reaction, dim3(pblocks_for(mp->n_particles)), dim3(PTPB), 0, s, *lp, *ls, *mp, *ms, *cp, cc,
                        mode);
   return check_launch("fsi_mpm_reaction");
+}
+
+int fsi_mpm_substep_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                            const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode,
+                            double* workspace, void* stream) {
+  int rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  if (mode < 0 || mode > 2) return fail(FSI_E_INVALID, "mode must be 0 (none), 1 (penalty) or 2 (moving boundary)");
+  FsiLbmParams lpv;
+  FsiLbmState lsv;
+  FsiCouplingParams cpv;
+  memset(&lpv, 0, sizeof(lpv));
+  memset(&lsv, 0, sizeof(lsv));
+  memset(&cpv, 0, sizeof(cpv));
+  if (mode) {
+    rc = check_lbm(lp, ls);
+    if (rc) return rc;
+    if (!cp || !cc || !workspace || !ls->hydro_force || !ls->solid_phi)
+      return fail(FSI_E_INVALID, "the reaction needs coupling params, counters, workspace, hydro_force, solid_phi");
+    lpv = *lp;
+    lsv = *ls;
+    cpv = *cp;
+  }
+  hipStream_t s = as_stream(stream);
+  const long long G = (long long)mp->gnx * mp->gny * mp->gnz;
+  const unsigned pb = pblocks_for(mp->n_particles);
+  hipLaunchKernelGGL(k_mpm_clear, dim3(blocks_for(G)), dim3(TPB), 0, s, *mp, *ms);
+  if (mp->n_particles > 0) {
+    if (mode == 1)
+      hipLaunchKernelGGL(k_mpm_p2g_react<1>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, workspace);
+    else if (mode == 2)
+      hipLaunchKernelGGL(k_mpm_p2g_react<2>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, workspace);
+    else
+      hipLaunchKernelGGL(k_mpm_p2g_react<0>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, workspace);
+  }
+  hipLaunchKernelGGL(k_mpm_grid_update_react, dim3(blocks_for(G)), dim3(TPB), 0, s, *mp, *ms,
+                     (mode && mp->n_particles > 0) ? (const double*)workspace : (const double*)nullptr, (int)pb, cc);
+  if (mode && mp->n_particles == 0) hipLaunchKernelGGL(k_reaction_clear, dim3(1), dim3(64), 0, s, cc);
+  if (mp->n_particles > 0)
+    hipLaunchKernelGGL(k_mpm_g2p, dim3(pb), dim3(PTPB), 0, s, *mp, *ms);
+  return check_launch("fsi_mpm_substep_coupled");
 }
 
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {
@@ -866,7 +1205,7 @@ int fsi_mpm_g2p(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {
   if (rc) return rc;
   if (!ms->fixed_mask || !ms->fixed_x) return fail(FSI_E_INVALID, "g2p needs fixed_mask and fixed_x");
   if (mp->n_particles > 0)
-    hipLaunchKernelGGL(k_mpm_g2p, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, as_stream(stream), *mp, *ms);
+    hipLaunchKernelGGL(k_mpm_g2p, dim3(pblocks_for(mp->n_particles)), dim3(PTPB), 0, as_stream(stream), *mp, *ms);
   return check_launch("fsi_mpm_g2p");
 }
 

@@ -76,7 +76,7 @@ class LbmState(ctypes.Structure):
         ("solid_phi", vp), ("solid_mass", vp), ("solid_vel", vp),
         ("cell_force", vp), ("hydro_force", vp),
         ("link_c", vp * 2), ("link_corr", vp),
-        ("hydro_accum", vp), ("accum_runmax", vp),
+        ("hydro_accum", vp),
         ("counters", vp),
     ]
 
@@ -184,9 +184,16 @@ EXPORTED_SYMBOLS = [
     "fsi_lbm_init", "fsi_lbm_rebuild", "fsi_lbm_substep", "fsi_lbm_moments",
     "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats",
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
-    "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_reduce_stats",
+    "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_reduce_stats",
     "fsi_batch_block_table", "fsi_batch_launch",
 ]
+
+PARTICLE_BLOCK = 64  # FSI_PARTICLE_BLOCK
+
+
+def coupled_workspace_doubles(n_particles: int) -> int:
+    """FSI_COUPLED_WORKSPACE_DOUBLES(n_particles)."""
+    return 6 * ((int(n_particles) + PARTICLE_BLOCK - 1) // PARTICLE_BLOCK)
 
 P = ctypes.POINTER
 
@@ -214,7 +221,8 @@ class Native:
         L.fsi_lbm_substep.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, u32, vp]
         L.fsi_lbm_moments.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, u32, vp]
         L.fsi_lbm_mb_begin.argtypes = [P(LbmParams), P(LbmState), vp]
-        L.fsi_lbm_mb_finalize.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.fsi_lbm_mb_finalize.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, vp]
         L.fsi_lbm_reduce_stats.argtypes = [P(LbmParams), P(LbmState), vp, vp, vp]
         L.fsi_project.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState), P(CouplingParams), vp, vp]
         L.fsi_penalty_force.argtypes = [P(LbmParams), P(LbmState), P(CouplingParams), vp, vp]
@@ -224,6 +232,8 @@ class Native:
                                        vp, ctypes.c_int, vp]
         L.fsi_mpm_grid_update.argtypes = [P(MpmParams), P(MpmState), vp]
         L.fsi_mpm_g2p.argtypes = [P(MpmParams), P(MpmState), vp]
+        L.fsi_mpm_substep_coupled.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),
+                                              P(CouplingParams), vp, ctypes.c_int, vp, vp]
         L.fsi_mpm_reduce_stats.argtypes = [P(MpmParams), P(MpmState), vp, vp, vp]
         L.fsi_batch_block_table.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
         L.fsi_batch_launch.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]

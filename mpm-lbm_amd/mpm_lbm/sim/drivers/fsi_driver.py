@@ -325,7 +325,7 @@ class FSIDriver3D:
         self.timing["lbm_step_time"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         for _ in range(self.config.mpm_substeps_per_lbm_step):
-            self.solid.substep()
+            self.solid.substep_coupled()
             self.total_mpm_substeps += 1
         self.timing["mpm_substep_time"] += time.perf_counter() - t0
 
@@ -340,12 +340,9 @@ class FSIDriver3D:
         self.lbm.step()
         self.timing["lbm_step_time"] += time.perf_counter() - t0
         t0 = time.perf_counter()
+        # clear_grid, p2g, add_lbm_reaction_to_mpm_grid, grid_update, g2p - fused per substep
         for _ in range(self.config.mpm_substeps_per_lbm_step):
-            self.solid.clear_grid()
-            self.solid.p2g()
-            self.penalty_coupler.add_lbm_reaction_to_mpm_grid(self.solid, self.lbm)
-            self.solid.grid_update()
-            self.solid.g2p()
+            self.solid.substep_coupled(self.lbm, self.penalty_coupler, 1)
             self.total_mpm_substeps += 1
         self.timing["mpm_substep_time"] += time.perf_counter() - t0
 
@@ -408,13 +405,10 @@ class FSIDriver3D:
     def _advance_mpm_with_moving_boundary_reaction(self):
         """fsi_driver.py:552-575 (engineering reaction transfer)."""
         t0 = time.perf_counter()
+        # clear_grid, p2g, clear_reaction_diagnostics, add_moving_boundary_reaction_to_mpm_grid,
+        # grid_update, g2p - as one fused device sequence per substep
         for _ in range(self.config.mpm_substeps_per_lbm_step):
-            self.solid.clear_grid()
-            self.solid.p2g()
-            self.mb_coupler.clear_reaction_diagnostics()
-            self.mb_coupler.add_moving_boundary_reaction_to_mpm_grid(self.solid, self.lbm)
-            self.solid.grid_update()
-            self.solid.g2p()
+            self.solid.substep_coupled(self.lbm, self.mb_coupler, 2)
             self.total_mpm_substeps += 1
         self.timing["mpm_substep_time"] += time.perf_counter() - t0
 

@@ -53,7 +53,8 @@ class _PopulationView(FieldView):
     """F / f (post-stream populations, (nx,ny,nz,19)) decoded from the storage format."""
 
     def __init__(self, lbm, space_shape):
-        super().__init__(lambda: lbm._F[lbm.cur], space_shape, 19, after_write=lbm._populations_written)
+        super().__init__(lambda: lbm._F[lbm.cur], space_shape, 19, before_write=lbm._sync_hydro,
+                         after_write=lbm._populations_written)
         self._lbm = lbm
 
     def _decoded(self):
@@ -73,6 +74,7 @@ class _PopulationView(FieldView):
         if a.shape != self.shape:
             raise ValueError(f"expected array of shape {self.shape}, got {a.shape}")
         lbm = self._lbm
+        lbm._sync_hydro()
         src = torch.from_numpy(np.ascontiguousarray(a)).to(lbm.device).reshape(-1, 19).t()
         if lbm.population_storage == "f32":
             lbm._F[lbm.cur].copy_(src)
@@ -140,8 +142,7 @@ class LBMFluid3D:
         self._hydro = torch.zeros((3, N), **f32)
         self._link = None  # moving-wall link buffers, allocated on first use
         self._link_corr = None
-        self._accum = None
-        self._runmax = None
+        self._accum = None  # (N, 4) fp64: accumulated x, y, z and running max of the norm
         self._counters = struct_buffer(_native.LbmCounters, dev)
         self._stats_ws = struct_buffer(_native.LbmStats, dev, _native.REDUCE_BLOCKS)
         self._stats_out = struct_buffer(_native.LbmStats, dev)
@@ -191,15 +192,14 @@ class LBMFluid3D:
     def _accum_f32(self):
         if self._accum is None:
             return torch.zeros((3, self.N), dtype=torch.float32, device=self.device)
-        return self._accum.to(torch.float32)
+        return self._accum[:, :3].t().contiguous().to(torch.float32)
 
     def _ensure_link_buffers(self):
         if self._link is None:
             dev = self.device
             self._link = [torch.zeros((18, self.N), dtype=torch.float32, device=dev) for _ in range(2)]
             self._link_corr = torch.zeros((18, self.N), dtype=torch.float32, device=dev)
-            self._accum = torch.zeros((3, self.N), dtype=torch.float64, device=dev)
-            self._runmax = torch.zeros((self.N,), dtype=torch.float32, device=dev)
+            self._accum = torch.zeros((self.N, 4), dtype=torch.float64, device=dev)
             self._refresh_state()
 
     def _refresh_state(self):
@@ -213,7 +213,7 @@ class LBMFluid3D:
         s.cell_force, s.hydro_force = p(self._cell_force), p(self._hydro)
         if self._link is not None:
             s.link_c[0], s.link_c[1] = p(self._link[0]), p(self._link[1])
-            s.link_corr, s.hydro_accum, s.accum_runmax = p(self._link_corr), p(self._accum), p(self._runmax)
+            s.link_corr, s.hydro_accum = p(self._link_corr), p(self._accum)
         s.counters = p(self._counters)
 
     def _build_params(self):
@@ -314,6 +314,8 @@ class LBMFluid3D:
         if self._rebuilt_phi_version == key and self._cellinfo_valid:
             return
         self._sync_moments()
+        # the last substep's link forces are fixed by the flags they were made under
+        self._sync_hydro()
         self._params.dynamic_solid_threshold = threshold
         self._call("fsi_lbm_rebuild", ctypes.byref(self._params), ctypes.byref(self._state), 1, self.cur, self._stream())
         self._cellinfo_valid = True
@@ -340,8 +342,8 @@ class LBMFluid3D:
         """hydro_force = accumulated mean (fluid.py:476-486)."""
         if self._pending is None or not self._pending_accumulate:
             raise RuntimeError("finalize_moving_boundary_force_accumulator needs an accumulated substep")
-        self._call("fsi_lbm_mb_finalize", ctypes.byref(self._params), ctypes.byref(self._state), self._pending,
-                   self.mb_sample_count, 1, self._stream())
+        self._call("fsi_lbm_mb_finalize", ctypes.byref(self._params), ctypes.byref(self._state), self.cur,
+                   self._pending, self.mb_sample_count, 1, self._stream())
         self._pending = None
         self._pending_accumulate = False
         self._hydro_from_links = False
@@ -375,8 +377,8 @@ class LBMFluid3D:
         last = 1 - parity
         if accumulate:
             self.mb_sample_count = n
-            self._call("fsi_lbm_mb_finalize", ctypes.byref(self._params), ctypes.byref(self._state), last, n, 1,
-                       self._stream())
+            self._call("fsi_lbm_mb_finalize", ctypes.byref(self._params), ctypes.byref(self._state), self.cur, last,
+                       n, 1, self._stream())
             self._pending = None
             self._hydro_from_links = False
         else:
@@ -414,8 +416,8 @@ class LBMFluid3D:
 
     def _sync_hydro(self):
         if self._hydro_from_links and self._pending is not None:
-            self._call("fsi_lbm_mb_finalize", ctypes.byref(self._params), ctypes.byref(self._state), self._pending,
-                       1, 0, self._stream())
+            self._call("fsi_lbm_mb_finalize", ctypes.byref(self._params), ctypes.byref(self._state), self.cur,
+                       self._pending, 1, 0, self._stream())
             self._hydro_from_links = False
 
     def _populations_written(self):

@@ -240,6 +240,23 @@ class MPMSolid3D:
         self.grid_update()
         self.g2p()
 
+    def substep_coupled(self, lbm=None, coupler=None, mode: int = 0):
+        """One substep with the coupling reaction of `coupler` (mode 1: penalty,
+        2: moving boundary; 0: none) in one fused device sequence
+        (fsi_mpm_substep_coupled): clear_grid, p2g + reaction, grid_update, g2p."""
+        if getattr(self, "_coupled_ws", None) is None:
+            self._coupled_ws = torch.zeros(_native.coupled_workspace_doubles(self.n_particles), dtype=torch.float64,
+                                           device=self.device)
+        if mode:
+            stream = coupler._bind(lbm)
+            lbm._sync_hydro()
+            self.native.call("fsi_mpm_substep_coupled", ctypes.byref(lbm._params), ctypes.byref(lbm._state),
+                             ctypes.byref(self._params), ctypes.byref(self._state), ctypes.byref(coupler.params),
+                             _native.ptr(coupler._counters_buf), int(mode), _native.ptr(self._coupled_ws), stream)
+        else:
+            self.native.call("fsi_mpm_substep_coupled", None, None, ctypes.byref(self._params),
+                             ctypes.byref(self._state), None, None, 0, None, self._stream())
+
     # ----------------------------------------------------------- diagnostics
     def reduce_stats(self) -> dict:
         self.native.call("fsi_mpm_reduce_stats", ctypes.byref(self._params), ctypes.byref(self._state),

@@ -4,14 +4,14 @@ regularised collision, through the emulation build of the kernels.
 Errors are stated against the FP64 oracle (max |a - b| / max |b| over fluid
 cells after 3 FSI steps of 20 substeps, 16^3 flap case at tau = 0.55):
   FP16 storage: F <= 2e-4, u <= 2e-3;  BF16 storage: F <= 2e-3, u <= 2e-2.
-The regularised collision in FP32 storage meets the 1e-5 kernel bar against
-the FP32 oracle's restatement of it."""
+The regularised collision in FP32 storage meets the parity bar of
+tests/parity.py against the oracle's FP32 / FP64 restatements of it."""
 import numpy as np
 import pytest
 
 from mpm_lbm import workloads
 from mpm_lbm.sim.drivers.fsi_config import FSIDriverConfig
-from tests.parity import make_pair, run_and_compare
+from tests.parity import check_history, make_pair, run_and_compare
 
 BOUNDS = {"f16": {"F": 2e-4, "rho": 2e-4, "v": 2e-3}, "bf16": {"F": 2e-3, "rho": 2e-3, "v": 2e-2}}
 
@@ -34,11 +34,9 @@ def test_16bit_storage_error_vs_fp64(emu, tmp_path, storage, collision):
 
 
 def test_regularized_fp32_meets_kernel_bar(emu, tmp_path):
-    d, o = make_pair(_workload(tmp_path, "f32", "regularized"), tmp_path / "run", emu, "cpu", dtype=np.float32)
-    for step, errs in enumerate(run_and_compare(d, o, 3)):
-        assert errs["solid_equal"]
-        for key in ("F", "rho", "v", "hydro", "x", "pv", "Fp"):
-            assert errs[key] <= 1e-5, (step, key, errs[key])
+    d, o = make_pair(_workload(tmp_path, "f32", "regularized"), tmp_path / "run", emu, "cpu", dtype=np.float32,
+                     with_fp64=True)
+    check_history(run_and_compare(d, o, 3), "regularized f32", 1e-5)
 
 
 def test_16bit_round_trip_of_population_view(emu, tmp_path):
