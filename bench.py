@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--kernel-launches", type=int, default=600, help="substep launches timed for the roofline")
     ap.add_argument("--cpu-steps", type=int, default=1, help="FSI steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "lbm_substep_64cube_traffic.json"))
+    ap.add_argument("--batch", type=int, default=64, help="instances of the batched sweep (0 = skip)")
+    ap.add_argument("--batch-exchanges", type=int, default=3, help="timed batched exchanges")
     return ap.parse_args()
 
 
@@ -103,6 +105,66 @@ def time_substep_kernel(torch, lbm, launches):
     torch.cuda.synchronize()
     lbm._moments_fresh = False
     return s.elapsed_time(e) / launches
+
+
+def _timed(torch, fn):
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / 1e3
+
+
+def batched_form(torch, count, exchanges):
+    """Batched sweep (seeded mix of 32/48/64/96^3 moving-boundary and penalty
+    flap instances): aggregate MLUPS of one-launch-per-operation exchanges
+    replayed as a HIP graph, against the same instances run one at a time
+    through FSIDriver3D (graph-captured FSI step for moving-boundary instances,
+    120 one-LBM-step FSI steps for penalty instances)."""
+    from mpm_lbm import workloads
+    from mpm_lbm.batch import BatchedFSI
+
+    ws = workloads.batch_sweep(count=count, seed=4, out_dir="/tmp/fsi_bench_batch")
+    t0 = time.perf_counter()
+    b = BatchedFSI(ws, out_dir="/tmp/fsi_bench_batch/run")
+    setup = time.perf_counter() - t0
+    b.step()  # eager: builds the block tables
+    b.step()  # captures the exchange graph and replays it
+    torch.cuda.synchronize()
+    secs = _timed(torch, lambda: [b.step() for _ in range(exchanges)])
+    cells = b.total_cells
+    batched = cells * b.n_substeps * exchanges / secs / 1e6
+    log(f"[bench] batched: {count} instances, {cells} cells, {exchanges} exchanges in {secs * 1e3:.1f} ms "
+        f"-> {batched:.1f} MLUPS (set-up {setup:.1f} s)")
+    plain_s = 0.0
+    for d in b.drivers:
+        if d.config.coupling_mode == "moving_boundary":
+            d.use_graph = True
+            d.step_once()
+            d.step_once()
+            torch.cuda.synchronize()
+            plain_s += _timed(torch, d.step_once)
+        else:
+            d.step_once()
+            torch.cuda.synchronize()
+            plain_s += _timed(torch, lambda: [d.step_once() for _ in range(b.n_substeps)])
+    plain = cells * b.n_substeps / plain_s / 1e6
+    log(f"[bench] plain (one instance at a time): {plain_s * 1e3:.1f} ms per exchange -> {plain:.1f} MLUPS")
+    modes = [d.config.coupling_mode for d in b.drivers]
+    return {
+        "instances": count,
+        "moving_boundary": modes.count("moving_boundary"),
+        "penalty": modes.count("penalty"),
+        "total_cells": cells,
+        "substeps_per_exchange": b.n_substeps,
+        "batched_mlups": round(batched, 1),
+        "plain_mlups": round(plain, 1),
+        "speedup_vs_plain": round(batched / plain, 3),
+        "ms_per_exchange": round(secs * 1e3 / exchanges, 3),
+        "equal_error": "same FP32 kernels and per-instance arguments; batched runs checked against the oracle at "
+                       "the plain path's bar (tests/test_gpu_batched.py)",
+    }
 
 
 def cpu_baseline(workload, steps):
@@ -179,6 +241,10 @@ def main():
     if tp.is_file():
         traffic = json.loads(tp.read_text()).get("hbm_bytes_per_launch")
 
+    batched = None
+    if args.batch > 0:
+        batched = batched_form(torch, args.batch, args.batch_exchanges)
+
     cpu = None
     if args.cpu_steps > 0:
         log("[bench] CPU baseline ...")
@@ -210,6 +276,7 @@ def main():
             "particle_updates_per_s": round(w.config.n_particles * w.config.mpm_substeps_per_lbm_step * args.steps / secs, 1),
             "copy_ceiling_gbs": round(copy_gbs, 1),
             "substep_frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
+            "batched": batched,
         },
         "roofline": {
             "bound": "hbm",

@@ -114,7 +114,8 @@ def penalty_block(n=32, steps=100, u_in=0.04, seed=1, out_dir="/tmp/fsi_workload
 
 def moving_boundary_flap(n=64, steps=50, substeps=120, u_in=0.05, seed=2, out_dir="/tmp/fsi_workloads",
                          flap=(4, 32, 24), E_mod=1.0e5, nu=0.35, rho_s=2.0, tau=0.6, mpm_dt=2.0e-5, mpm_substeps=10,
-                         shape: Optional[Tuple[int, int, int]] = None, mb_cap=1.0e-4) -> Workload:
+                         shape: Optional[Tuple[int, int, int]] = None, mb_cap=1.0e-4,
+                         geometry_name: Optional[str] = None) -> Workload:
     """Subcycled moving_boundary FSI: a flap anchored on the y = 0 wall, `substeps` LBM substeps per FSI step.
     The LBM step is dt_lbm = mpm_substeps * mpm_dt / substeps so both solvers advance one FSI step per exchange."""
     shape = tuple(shape) if shape else (n, n, n)
@@ -125,7 +126,8 @@ def moving_boundary_flap(n=64, steps=50, substeps=120, u_in=0.05, seed=2, out_di
     material = (E_mod, nu, rho_s)
     cloud = solid_box_cloud(shape, lo, hi, rho_s, seed, fixed_bottom=True, tip_top=True)
     P = len(cloud["x"])
-    geo = _write_geometry(out_dir, "mb_{}x{}x{}".format(*shape), duct_geometry_dict(shape, P, material))
+    geo = _write_geometry(out_dir, geometry_name or "mb_{}x{}x{}_p{}".format(*shape, P),
+                          duct_geometry_dict(shape, P, material))
     dt_lbm = mpm_substeps * mpm_dt / substeps
     dx_phys = 1.0 / nx
     cfg = FSIDriverConfig(
@@ -167,7 +169,8 @@ def flap_case(mode, n, substeps=120, u_in=0.05, E_mod=1.0e5, thickness=4, seed=0
     name = f"{mode}_{n}_t{thickness}_s{seed}"
     if mode == "moving_boundary":
         w = moving_boundary_flap(n=n, steps=1, substeps=substeps, u_in=u_in, seed=seed, out_dir=out_dir, flap=flap,
-                                 E_mod=E_mod, nu=nu, rho_s=rho_s, tau=tau, mpm_dt=dt_cfl, mpm_substeps=mpm_substeps)
+                                 E_mod=E_mod, nu=nu, rho_s=rho_s, tau=tau, mpm_dt=dt_cfl, mpm_substeps=mpm_substeps,
+                                 geometry_name=name)
     else:
         shape = (n, n, n)
         lo = (n // 2 - flap[0] // 2, 1, n // 2 - flap[2] // 2)
