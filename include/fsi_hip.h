@@ -175,14 +175,16 @@ typedef struct FsiMpmStats {
   double mass_weighted_v[3];
 } FsiMpmStats;
 
+/* Workgroup size of the per-particle kernels. */
+#define FSI_PARTICLE_BLOCK 64
+
 /* ---------------------------------------------------------------------------
  * Batched form: many independent FSI instances (mixed sizes and modes) in one
- * launch per operation.  `inst` is a device array of descriptors; each launch
- * covers the concatenated work items of the instances selected by `mask`,
- * through a device table `block_start` (n_inst + 1 entries, exclusive prefix
- * sum of each instance's workgroup count for that operation's domain: cells,
- * MPM nodes or particles; built by fsi_batch_block_table).  No reference
- * counterpart: the reference runs one instance per process.
+ * launch per operation.  `inst` is a device array of descriptors; a launch
+ * covers the workgroups listed in a device block map (two int32 per
+ * workgroup: instance index, workgroup index within that instance's domain),
+ * built by fsi_batch_block_map for the instances an operation applies to.
+ * No reference counterpart: the reference runs one instance per process.
  * ------------------------------------------------------------------------- */
 #define FSI_BATCH_MODE_NONE 0
 #define FSI_BATCH_MODE_PENALTY 1
@@ -190,7 +192,8 @@ typedef struct FsiMpmStats {
 
 #define FSI_BATCH_DOMAIN_CELLS 0
 #define FSI_BATCH_DOMAIN_NODES 1
-#define FSI_BATCH_DOMAIN_PARTICLES 2
+#define FSI_BATCH_DOMAIN_PARTICLES 2 /* workgroups of FSI_PARTICLE_BLOCK particles */
+#define FSI_BATCH_DOMAIN_PLANES 3    /* substep tiles: nx * ceil(ny*nz / 256) workgroups */
 
 typedef struct FsiBatchInstance {
   FsiLbmParams lp;
@@ -205,6 +208,8 @@ typedef struct FsiBatchInstance {
   int32_t mpm_substeps;
   int32_t active;     /* 0 = slot unused */
   int32_t cell_force; /* 1 = add cell_force (penalty) */
+  int32_t pad0;
+  double* react_ws;   /* FSI_COUPLED_WORKSPACE_DOUBLES(n_particles): reaction records of MPM_P2G_REACT */
 } FsiBatchInstance;
 
 /* Operations of fsi_batch_launch. */
@@ -223,17 +228,34 @@ typedef struct FsiBatchInstance {
 #define FSI_BATCH_OP_MPM_GRID_UPDATE 12
 #define FSI_BATCH_OP_MPM_G2P 13
 #define FSI_BATCH_OP_COUNTERS_CLEAR 14 /* zero the coupling counters (one work item per instance) */
+#define FSI_BATCH_OP_MPM_P2G_REACT 15  /* P2G + the instance's reaction in one particle pass (as
+                                          fsi_mpm_substep_coupled), records into react_ws */
+#define FSI_BATCH_OP_MPM_GRID_UPDATE_REACT 16 /* grid update; folds react_ws into the counters */
+/* LBM_SUBSTEP for a map of one mode only (moving-boundary, penalty, none):
+ * the mode's flags are compiled into the kernel. */
+#define FSI_BATCH_OP_LBM_SUBSTEP_MB 17
+#define FSI_BATCH_OP_LBM_SUBSTEP_PENALTY 18
+#define FSI_BATCH_OP_LBM_SUBSTEP_NONE 19
 
-/* Fill block_start (host memory, n_inst + 1 entries) for one domain from the
- * host copy of the descriptors, counting only instances whose mode bit is in
- * `mode_mask` (bit m = 1 << FSI_BATCH_MODE_m) and that are active.  Returns
- * the total workgroup count. */
-int fsi_batch_block_table(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask,
-                          int32_t* block_start_host);
+/* Domain of each operation (the map passed to fsi_batch_launch must be built
+ * for it): LBM_SUBSTEP, LBM_SUBSTEP_MB / _PENALTY / _NONE -> PLANES; REBUILD, MOMENTS, MB_BEGIN, MB_FINALIZE,
+ * PROJ_CLEAR, PROJ_NORMALIZE, PENALTY -> CELLS; MPM_CLEAR, MPM_GRID_UPDATE,
+ * MPM_GRID_UPDATE_REACT -> NODES; PROJ_PARTICLES, MPM_P2G, MPM_REACTION,
+ * MPM_P2G_REACT, MPM_G2P -> PARTICLES. */
 
-/* One batched operation over the instances counted in block_start.  `t` is
- * the substep index within the exchange (LBM_SUBSTEP) or unused. */
-int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const int32_t* block_start_dev,
+/* Block map for one domain from the host copy of the descriptors: instances
+ * that are active, whose mode bit is in `mode_mask` (bit m = 1 <<
+ * FSI_BATCH_MODE_m) and that have more than `min_mpm` MPM substeps per LBM
+ * step.  Writes two int32 per workgroup into map_host (may be NULL to count)
+ * and returns the workgroup count. */
+int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask, int min_mpm,
+                        int32_t* map_host);
+
+/* One batched operation over the workgroups of a device block map.  `t` is
+ * the substep index within the exchange (LBM_SUBSTEP, MOMENTS) or, for
+ * COUNTERS_CLEAR, which counters (0 projection, 1 penalty, 2 reaction,
+ * 3 bounce-back). */
+int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const int32_t* block_map_dev,
                      int n_blocks, int t, void* stream);
 
 const char* fsi_last_error(void);
@@ -317,9 +339,7 @@ int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpm
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
 int fsi_mpm_g2p(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
 
-/* Workgroup size of the per-particle kernels, and the workspace of
- * fsi_mpm_substep_coupled: 6 doubles per particle workgroup. */
-#define FSI_PARTICLE_BLOCK 64
+/* Workspace of fsi_mpm_substep_coupled: 6 doubles per particle workgroup. */
 #define FSI_COUPLED_WORKSPACE_DOUBLES(n_particles) \
   (6 * (((n_particles) + FSI_PARTICLE_BLOCK - 1) / FSI_PARTICLE_BLOCK))
 

@@ -1091,7 +1091,7 @@ FSI_HD bool inside3(int i, int j, int k, int nx, int ny, int nz) {
 // node n = (I, J, K).  GlobalSink adds atomically into the grid arrays; the
 // GPU kernels may also scatter into a workgroup-private LDS tile first.
 struct GlobalSink {
-  float* ch[4];
+  float* ch[5];
   FSI_HDM void add(int c, long long n, int, int, int, float v) const { atomic_add_f32(ch[c] + n, v); }
 };
 
@@ -1101,6 +1101,7 @@ FSI_HD GlobalSink p2g_sink(const FsiMpmState& ms, long long G) {
   k.ch[1] = ms.grid_v + G;
   k.ch[2] = ms.grid_v + 2 * G;
   k.ch[3] = ms.grid_m;
+  k.ch[4] = nullptr;
   return k;
 }
 
@@ -1109,7 +1110,7 @@ FSI_HD GlobalSink reaction_sink(const FsiMpmState& ms, long long G) {
   k.ch[0] = ms.grid_f;
   k.ch[1] = ms.grid_f + G;
   k.ch[2] = ms.grid_f + 2 * G;
-  k.ch[3] = nullptr;
+  k.ch[3] = k.ch[4] = nullptr;
   return k;
 }
 
@@ -1302,8 +1303,20 @@ FSI_HD void mpm_g2p_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
 // ---------------------------------------------------------------------------
 
 // project_particles (projection.py:38-64): scatter phi, mass, mass*velocity.
+// Projection channels on the lattice: solid_phi, solid_mass, solid_vel x/y/z.
+FSI_HD GlobalSink proj_sink(const FsiLbmState& ls, long long N) {
+  GlobalSink k;
+  k.ch[0] = ls.solid_phi;
+  k.ch[1] = ls.solid_mass;
+  k.ch[2] = ls.solid_vel;
+  k.ch[3] = ls.solid_vel + N;
+  k.ch[4] = ls.solid_vel + 2 * N;
+  return k;
+}
+
+template <typename Sink>
 FSI_HD void project_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
-                             const FsiMpmState& ms, const FsiCouplingParams& cp, long long p) {
+                             const FsiMpmState& ms, const FsiCouplingParams& cp, long long p, const Sink& sink) {
   const Grid g = make_grid(lp);
   const long long P = mp.n_particles;
   int base[3];
@@ -1326,11 +1339,11 @@ FSI_HD void project_particle(const FsiLbmParams& lp, const FsiLbmState& ls, cons
         const float mc = wt * mass;
         const float vf = wt * vol / cp.cell_volume_norm;
         const long long c = cell_of(g, I, Jn, K);
-        atomic_add_f32(&ls.solid_phi[c], vf);
-        atomic_add_f32(&ls.solid_mass[c], mc);
-        atomic_add_f32(&ls.solid_vel[c], mc * vx);
-        atomic_add_f32(&ls.solid_vel[g.N + c], mc * vy);
-        atomic_add_f32(&ls.solid_vel[2 * g.N + c], mc * vz);
+        sink.add(0, c, I, Jn, K, vf);
+        sink.add(1, c, I, Jn, K, mc);
+        sink.add(2, c, I, Jn, K, mc * vx);
+        sink.add(3, c, I, Jn, K, mc * vy);
+        sink.add(4, c, I, Jn, K, mc * vz);
       }
 }
 
@@ -1565,6 +1578,20 @@ FSI_HD void mpm_stats_particle(const FsiMpmParams& mp, const FsiMpmState& ms, lo
 // Batched form: per-instance substep semantics shared by the GPU launch and
 // the serial emulation.
 // ---------------------------------------------------------------------------
+// Workgroup size of a batch domain's launches (particle ops use the smaller
+// particle workgroups) and the number of workgroups an instance needs there.
+FSI_HD int batch_block_size(int domain) { return domain == FSI_BATCH_DOMAIN_PARTICLES ? FSI_PARTICLE_BLOCK : 256; }
+
+FSI_HD long long batch_domain_blocks(const FsiBatchInstance& I, int domain) {
+  if (domain == FSI_BATCH_DOMAIN_PLANES)  // one i-plane per run of tiles, as the single-instance substep
+    return (long long)I.lp.nx * (((long long)I.lp.ny * I.lp.nz + 255) / 256);
+  long long n = domain == FSI_BATCH_DOMAIN_CELLS ? (long long)I.lp.nx * I.lp.ny * I.lp.nz
+              : domain == FSI_BATCH_DOMAIN_NODES ? (long long)I.mp.gnx * I.mp.gny * I.mp.gnz
+                                                 : (long long)I.mp.n_particles;
+  const int bs = batch_block_size(domain);
+  return (n + bs - 1) / bs;
+}
+
 FSI_HD long long batch_domain_size(const FsiBatchInstance& I, int domain) {
   if (domain == FSI_BATCH_DOMAIN_CELLS) return (long long)I.lp.nx * I.lp.ny * I.lp.nz;
   if (domain == FSI_BATCH_DOMAIN_NODES) return (long long)I.mp.gnx * I.mp.gny * I.mp.gnz;

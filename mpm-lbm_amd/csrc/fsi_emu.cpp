@@ -7,6 +7,7 @@
 // the oracle on a machine without a GPU.  Built into libfsi_emu.so, which no
 // product code loads.
 #include <string.h>
+#include <vector>
 
 #include <string>
 
@@ -139,7 +140,8 @@ int fsi_project(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParam
   cc->projected_mass = cc->projected_volume_raw = cc->projected_volume_clamped = 0.0;
   cc->max_phi_raw_bits = 0;
   cc->active_cell_count = 0;
-  for (long long q = 0; q < mp->n_particles; ++q) project_particle(*lp, *ls, *mp, *ms, *cp, q);
+  const GlobalSink sink = proj_sink(*ls, ncells(lp));
+  for (long long q = 0; q < mp->n_particles; ++q) project_particle(*lp, *ls, *mp, *ms, *cp, q, sink);
   for (long long c = 0; c < N; ++c) {
     ProjOut o = normalize_cell(*lp, *ls, *cp, c);
     atomic_max_nonneg(&cc->max_phi_raw_bits, o.phi_raw > 0.0f ? o.phi_raw : 0.0f);
@@ -235,22 +237,27 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
 }
 
 
-int fsi_batch_block_table(const FsiBatchInstance* inst, int n_inst, int domain, int mode_mask, int32_t* bs) {
-  const int TPB = 256;
+int fsi_batch_block_map(const FsiBatchInstance* inst, int n_inst, int domain, int mode_mask, int min_mpm,
+                        int32_t* map) {
+  if (!inst || n_inst < 0) return fail(FSI_E_INVALID, "null batch arguments");
+  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_PLANES) return fail(FSI_E_INVALID, "unknown batch domain");
   long long total = 0;
-  bs[0] = 0;
   for (int i = 0; i < n_inst; ++i) {
-    long long nb = 0;
-    if (inst[i].active && (mode_mask & (1 << inst[i].mode))) nb = (batch_domain_size(inst[i], domain) + TPB - 1) / TPB;
+    if (!inst[i].active || !(mode_mask & (1 << inst[i].mode)) || inst[i].mpm_substeps <= min_mpm) continue;
+    const long long nb = batch_domain_blocks(inst[i], domain);
+    if (map)
+      for (long long b = 0; b < nb; ++b) {
+        map[2 * (total + b)] = i;
+        map[2 * (total + b) + 1] = (int32_t)b;
+      }
     total += nb;
-    bs[i + 1] = (int32_t)total;
   }
   return (int)total;
 }
 
-// Serial emulation of fsi_batch_launch: instance by instance, work item by
-// work item, in the order of the block table.
-int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int32_t* bs, int, int t, void*) {
+// Serial emulation of fsi_batch_launch: instance by instance (the instances
+// named in the block map, in index order), work item by work item.
+int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int32_t* map, int n_blocks, int t, void*) {
   if (op == FSI_BATCH_OP_COUNTERS_CLEAR) {
     for (int i = 0; i < n_inst; ++i) {
       if (!inst[i].active) continue;
@@ -276,14 +283,20 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
     }
     return FSI_OK;
   }
+  if (!map) return fail(FSI_E_INVALID, "null block map");
+  std::vector<char> sel((size_t)n_inst, 0);
+  for (int b = 0; b < n_blocks; ++b) sel[(size_t)map[2 * b]] = 1;
   for (int i = 0; i < n_inst; ++i) {
-    if (bs[i + 1] == bs[i]) continue;
+    if (!sel[(size_t)i]) continue;
     const FsiBatchInstance& I = inst[i];
     const long long cells = batch_domain_size(I, FSI_BATCH_DOMAIN_CELLS);
     const long long nodes = batch_domain_size(I, FSI_BATCH_DOMAIN_NODES);
     const long long parts = batch_domain_size(I, FSI_BATCH_DOMAIN_PARTICLES);
     switch (op) {
       case FSI_BATCH_OP_LBM_SUBSTEP:
+      case FSI_BATCH_OP_LBM_SUBSTEP_MB:
+      case FSI_BATCH_OP_LBM_SUBSTEP_PENALTY:
+      case FSI_BATCH_OP_LBM_SUBSTEP_NONE:
         by_storage(I.lp.storage, [&](auto tag) {
           using Pop = decltype(tag);
           (void)cells;
@@ -339,7 +352,7 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
         }
         break;
       case FSI_BATCH_OP_PROJ_PARTICLES:
-        for (long long q = 0; q < parts; ++q) project_particle(I.lp, I.ls, I.mp, I.ms, I.cp, q);
+        for (long long q = 0; q < parts; ++q) project_particle(I.lp, I.ls, I.mp, I.ms, I.cp, q, proj_sink(I.ls, cells));
         break;
       case FSI_BATCH_OP_PROJ_NORMALIZE:
         for (long long c = 0; c < cells; ++c) {
@@ -385,7 +398,23 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
           for (int d = 0; d < 3; ++d) I.cc->net_particle_reaction[d] += o.f[d];
         }
         break;
+      case FSI_BATCH_OP_MPM_P2G_REACT:
+        for (long long q = 0; q < parts; ++q) mpm_p2g_particle(I.mp, I.ms, q, p2g_sink(I.ms, nodes));
+        if (I.mode == FSI_BATCH_MODE_NONE) break;
+        I.cc->active_reaction_particle_count = 0;
+        I.cc->max_particle_reaction_bits = I.cc->max_grid_reaction_bits = 0;
+        for (int d = 0; d < 3; ++d) I.cc->net_particle_reaction[d] = I.cc->net_grid_reaction[d] = 0.0;
+        for (long long q = 0; q < parts; ++q) {
+          ReactOut o = reaction_particle(I.lp, I.ls, I.mp, I.ms, I.cp, I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1, q,
+                                          reaction_sink(I.ms, nodes));
+          I.cc->active_reaction_particle_count += o.active;
+          atomic_max_nonneg(&I.cc->max_particle_reaction_bits, o.particle_norm);
+          atomic_max_nonneg(&I.cc->max_grid_reaction_bits, o.max_contrib);
+          for (int d = 0; d < 3; ++d) I.cc->net_particle_reaction[d] += o.f[d];
+        }
+        break;
       case FSI_BATCH_OP_MPM_GRID_UPDATE:
+      case FSI_BATCH_OP_MPM_GRID_UPDATE_REACT:
         for (long long n = 0; n < nodes; ++n) mpm_grid_node(I.mp, I.ms, n);
         break;
       case FSI_BATCH_OP_MPM_G2P:

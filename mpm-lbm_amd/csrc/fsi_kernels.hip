@@ -136,6 +136,149 @@ __device__ __forceinline__ int block_sum_i(int v, int* smem) {
 }
 
 // ---------------------------------------------------------------------------
+// Workgroup-private scatter tiles for the particle-to-grid passes.  The nodes a
+// workgroup's particles touch lie in the box spanned by their B-spline base
+// nodes (+2 along each axis).  When that box holds at most TILE_NODES nodes the
+// workgroup accumulates its scatter in LDS (ds_add_f32) and then adds each
+// touched node to the grid once; otherwise it scatters with global atomics.
+// Particles stored in spatial order (as lattice-sampled clouds are) give
+// boxes of a few hundred nodes for 256 particles, replacing 27 global atomics
+// per particle and channel by a handful.
+// ---------------------------------------------------------------------------
+// 512 nodes hold the stencil box of a 64-particle workgroup of a
+// lattice-ordered cloud (8 cells of 8 particles, also across a row end) and
+// keep the tiles small enough for several workgroups per CU.
+constexpr int TILE_NODES = 512;
+
+struct TileBox {
+  int lx, ly, lz, ex, ey, ez, V;
+};
+
+// Box of the block's particles' stencils: base nodes int(x * inv - 0.5), as the
+// bodies compute them, plus 2.  Should a body's base differ from this one (a
+// different contraction of the expression), LdsSink sends that node to the
+// global grid, so the box never has to be exact.  V = 0: no valid particle or
+// the box does not fit.  Every thread of the block must call this.
+__device__ __forceinline__ TileBox block_tile_box(const float* x, long long P, long long q, bool valid, float inv, int* sbox,
+                                                  int cap = TILE_NODES) {
+  if (threadIdx.x == 0) {
+    sbox[0] = sbox[1] = sbox[2] = 0x7FFFFFFF;
+    sbox[3] = sbox[4] = sbox[5] = -0x7FFFFFFF;
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float X = x[d * P + q] * inv;
+      if (X > -1.0e6f && X < 1.0e6f) {
+        const int b = (int)(X - 0.5f);
+        atomicMin(&sbox[d], b);
+        atomicMax(&sbox[3 + d], b);
+      }
+    }
+  }
+  __syncthreads();
+  TileBox t;
+  t.lx = sbox[0];
+  t.ly = sbox[1];
+  t.lz = sbox[2];
+  t.V = 0;
+  t.ex = t.ey = t.ez = 0;
+  if (sbox[0] <= sbox[3] && sbox[1] <= sbox[4] && sbox[2] <= sbox[5]) {
+    const long long ex = (long long)sbox[3] - sbox[0] + 3, ey = (long long)sbox[4] - sbox[1] + 3,
+                    ez = (long long)sbox[5] - sbox[2] + 3;
+    if (ex * ey * ez <= cap) {
+      t.ex = (int)ex;
+      t.ey = (int)ey;
+      t.ez = (int)ez;
+      t.V = (int)(ex * ey * ez);
+    }
+  }
+  __syncthreads();  // sbox may be reused by the caller
+  return t;
+}
+
+// LDS scatter target; a node outside the box goes to the global grid.
+struct LdsSink {
+  float* tile;
+  TileBox b;
+  GlobalSink g;
+  __device__ __forceinline__ void add(int c, long long n, int I, int J, int K, float v) const {
+    const unsigned a = (unsigned)(I - b.lx), e = (unsigned)(J - b.ly), f = (unsigned)(K - b.lz);
+    // workgroup scope for the LDS add (ds_add_f32), device scope for the global one: the two stay
+    // separate instructions instead of one flat atomic on a merged pointer
+    if (a < (unsigned)b.ex && e < (unsigned)b.ey && f < (unsigned)b.ez)
+      __hip_atomic_fetch_add(tile + c * b.V + ((int)a * b.ey + (int)e) * b.ez + (int)f, v, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+      atomicAdd(g.ch[c] + n, v);
+  }
+};
+
+template <int NCH>
+__device__ __forceinline__ void tile_clear(float* tile, const TileBox& b) {
+  for (int i = threadIdx.x; i < NCH * b.V; i += (int)blockDim.x) tile[i] = 0.0f;
+  __syncthreads();
+}
+
+// Add the tile to the grid: one global atomic per touched node and channel.
+template <int NCH>
+__device__ __forceinline__ void tile_flush(const float* tile, const TileBox& b, const GlobalSink& g, int gnx, int gny, int gnz) {
+  __syncthreads();
+  const int plane = b.ey * b.ez;
+  for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
+    const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
+    const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
+    if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
+    const long long n = ((long long)I * gny + J) * gnz + K;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const float v = tile[c * b.V + i];
+      if (v != 0.0f) atomicAdd(g.ch[c] + n, v);
+    }
+  }
+}
+
+// Flush of a tile whose channels go to the grid arrays ch[0..nch).
+__device__ __forceinline__ void tile_flush_ch(const float* tile, const TileBox& b, float* const* ch, int nch, int gnx,
+                                              int gny, int gnz) {
+  __syncthreads();
+  const int plane = b.ey * b.ez;
+  for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
+    const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
+    const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
+    if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
+    const long long n = ((long long)I * gny + J) * gnz + K;
+    for (int c = 0; c < nch; ++c) {
+      const float v = tile[c * b.V + i];
+      if (v != 0.0f) atomicAdd(ch[c] + n, v);
+    }
+  }
+}
+
+// Projection scatter of particle q (valid) by a whole workgroup: solid_phi,
+// solid_mass and solid_vel on the lattice.
+constexpr int PROJ_TILE_NODES = TILE_NODES;
+
+__device__ __forceinline__ void proj_block(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                                           const FsiMpmState& ms, const FsiCouplingParams& cp, long long q, bool valid,
+                                           float* tile, int* sbox) {
+  const GlobalSink g = proj_sink(ls, (long long)lp.nx * lp.ny * lp.nz);
+  const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, cp.inv_dx_norm, sbox, PROJ_TILE_NODES);
+  if (b.V > 0) {
+    tile_clear<5>(tile, b);
+    LdsSink k;
+    k.tile = tile;
+    k.b = b;
+    k.g = g;
+    if (valid) project_particle(lp, ls, mp, ms, cp, q, k);
+    tile_flush_ch(tile, b, g.ch, 5, lp.nx, lp.ny, lp.nz);
+  } else if (valid) {
+    project_particle(lp, ls, mp, ms, cp, q, g);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // LBM kernels
 // ---------------------------------------------------------------------------
 template <typename Pop>
@@ -296,8 +439,10 @@ __global__ void __launch_bounds__(TPB) k_proj_clear(FsiLbmParams p, FsiLbmState 
 
 __global__ void __launch_bounds__(PTPB) k_proj_particles(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
                                                          FsiMpmState ms, FsiCouplingParams cp) {
+  __shared__ float tile[5 * PROJ_TILE_NODES];
+  __shared__ int sbox[6];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
-  if (q < mp.n_particles) project_particle(lp, ls, mp, ms, cp, q);
+  proj_block(lp, ls, mp, ms, cp, q, q < mp.n_particles, tile, sbox);
 }
 
 __global__ void __launch_bounds__(TPB) k_proj_normalize(FsiLbmParams lp, FsiLbmState ls, FsiCouplingParams cp,
@@ -378,124 +523,6 @@ __global__ void __launch_bounds__(TPB) k_mpm_clear(FsiMpmParams mp, FsiMpmState 
     ms.grid_f[n] = 0.0f;
     ms.grid_f[G + n] = 0.0f;
     ms.grid_f[2 * G + n] = 0.0f;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Workgroup-private scatter tiles for the particle-to-grid passes.  The nodes a
-// workgroup's particles touch lie in the box spanned by their B-spline base
-// nodes (+2 along each axis).  When that box holds at most TILE_NODES nodes the
-// workgroup accumulates its scatter in LDS (ds_add_f32) and then adds each
-// touched node to the grid once; otherwise it scatters with global atomics.
-// Particles stored in spatial order (as lattice-sampled clouds are) give
-// boxes of a few hundred nodes for 256 particles, replacing 27 global atomics
-// per particle and channel by a handful.
-// ---------------------------------------------------------------------------
-constexpr int TILE_NODES = 2048;
-
-struct TileBox {
-  int lx, ly, lz, ex, ey, ez, V;
-};
-
-// Box of the block's particles' stencils: base nodes int(x * inv - 0.5), as the
-// bodies compute them, plus 2.  Should a body's base differ from this one (a
-// different contraction of the expression), LdsSink sends that node to the
-// global grid, so the box never has to be exact.  V = 0: no valid particle or
-// the box does not fit.  Every thread of the block must call this.
-__device__ __forceinline__ TileBox block_tile_box(const float* x, long long P, long long q, bool valid, float inv, int* sbox,
-                                                  int cap = TILE_NODES) {
-  if (threadIdx.x == 0) {
-    sbox[0] = sbox[1] = sbox[2] = 0x7FFFFFFF;
-    sbox[3] = sbox[4] = sbox[5] = -0x7FFFFFFF;
-  }
-  __syncthreads();
-  if (valid) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float X = x[d * P + q] * inv;
-      if (X > -1.0e6f && X < 1.0e6f) {
-        const int b = (int)(X - 0.5f);
-        atomicMin(&sbox[d], b);
-        atomicMax(&sbox[3 + d], b);
-      }
-    }
-  }
-  __syncthreads();
-  TileBox t;
-  t.lx = sbox[0];
-  t.ly = sbox[1];
-  t.lz = sbox[2];
-  t.V = 0;
-  t.ex = t.ey = t.ez = 0;
-  if (sbox[0] <= sbox[3] && sbox[1] <= sbox[4] && sbox[2] <= sbox[5]) {
-    const long long ex = (long long)sbox[3] - sbox[0] + 3, ey = (long long)sbox[4] - sbox[1] + 3,
-                    ez = (long long)sbox[5] - sbox[2] + 3;
-    if (ex * ey * ez <= cap) {
-      t.ex = (int)ex;
-      t.ey = (int)ey;
-      t.ez = (int)ez;
-      t.V = (int)(ex * ey * ez);
-    }
-  }
-  __syncthreads();  // sbox may be reused by the caller
-  return t;
-}
-
-// LDS scatter target; a node outside the box goes to the global grid.
-struct LdsSink {
-  float* tile;
-  TileBox b;
-  GlobalSink g;
-  __device__ __forceinline__ void add(int c, long long n, int I, int J, int K, float v) const {
-    const unsigned a = (unsigned)(I - b.lx), e = (unsigned)(J - b.ly), f = (unsigned)(K - b.lz);
-    // workgroup scope for the LDS add (ds_add_f32), device scope for the global one: the two stay
-    // separate instructions instead of one flat atomic on a merged pointer
-    if (a < (unsigned)b.ex && e < (unsigned)b.ey && f < (unsigned)b.ez)
-      __hip_atomic_fetch_add(tile + c * b.V + ((int)a * b.ey + (int)e) * b.ez + (int)f, v, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_WORKGROUP);
-    else
-      atomicAdd(g.ch[c] + n, v);
-  }
-};
-
-template <int NCH>
-__device__ __forceinline__ void tile_clear(float* tile, const TileBox& b) {
-  for (int i = threadIdx.x; i < NCH * b.V; i += (int)blockDim.x) tile[i] = 0.0f;
-  __syncthreads();
-}
-
-// Add the tile to the grid: one global atomic per touched node and channel.
-template <int NCH>
-__device__ __forceinline__ void tile_flush(const float* tile, const TileBox& b, const GlobalSink& g, int gnx, int gny, int gnz) {
-  __syncthreads();
-  const int plane = b.ey * b.ez;
-  for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
-    const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
-    const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
-    if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
-    const long long n = ((long long)I * gny + J) * gnz + K;
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      const float v = tile[c * b.V + i];
-      if (v != 0.0f) atomicAdd(g.ch[c] + n, v);
-    }
-  }
-}
-
-// Flush of a tile whose channels go to the grid arrays ch[0..nch).
-__device__ __forceinline__ void tile_flush_ch(const float* tile, const TileBox& b, float* const* ch, int nch, int gnx,
-                                              int gny, int gnz) {
-  __syncthreads();
-  const int plane = b.ey * b.ez;
-  for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
-    const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
-    const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
-    if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
-    const long long n = ((long long)I * gny + J) * gnz + K;
-    for (int c = 0; c < nch; ++c) {
-      const float v = tile[c * b.V + i];
-      if (v != 0.0f) atomicAdd(ch[c] + n, v);
-    }
   }
 }
 
@@ -594,19 +621,16 @@ __global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSt
 // update folds the records in block order into the counters (deterministic,
 // and no contended same-address atomics).
 // ---------------------------------------------------------------------------
-constexpr int FUSED_TILE_NODES = 1024;
+constexpr int FUSED_TILE_NODES = TILE_NODES;
 constexpr int REACT_REC = 6;  // active count, max particle norm, max grid contribution, net force x, y, z
 
-template <int MODE>  // 0: no reaction, 1: penalty reaction, 2: moving-boundary reaction
-__global__ void __launch_bounds__(PTPB) k_mpm_p2g_react(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
-                                                        FsiMpmState ms, FsiCouplingParams cp, double* rec) {
-  __shared__ float tile[7 * FUSED_TILE_NODES];
-  __shared__ int sbox[6];
-  __shared__ double sd[PTPB / 64];
-  __shared__ float sf[PTPB / 64];
-  __shared__ int si[PTPB / 64];
-  const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
-  const bool valid = q < mp.n_particles;
+// P2G (+ reaction, MODE 1 penalty / 2 moving boundary) of particle q (valid) by
+// a whole workgroup; the block's reaction record goes to rec[0..REACT_REC).
+template <int MODE>
+__device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                                                const FsiMpmState& ms, const FsiCouplingParams& cp, double* rec,
+                                                long long q, bool valid, float* tile, int* sbox, double* sd,
+                                                float* sf, int* si) {
   const long long G = (long long)mp.gnx * mp.gny * mp.gnz;
   const GlobalSink gp = p2g_sink(ms, G), gr = reaction_sink(ms, G);
   const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, mp.inv_dx, sbox, FUSED_TILE_NODES);
@@ -642,14 +666,57 @@ __global__ void __launch_bounds__(PTPB) k_mpm_p2g_react(FsiLbmParams lp, FsiLbmS
     double net[3];
     for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)o.f[d], RED_SUM, sd);
     if (threadIdx.x == 0) {
-      double* r = rec + (size_t)blockIdx.x * REACT_REC;
-      r[0] = (double)act;
-      r[1] = (double)mpn;
-      r[2] = (double)mgc;
-      r[3] = net[0];
-      r[4] = net[1];
-      r[5] = net[2];
+      rec[0] = (double)act;
+      rec[1] = (double)mpn;
+      rec[2] = (double)mgc;
+      rec[3] = net[0];
+      rec[4] = net[1];
+      rec[5] = net[2];
     }
+  }
+}
+
+template <int MODE>  // 0: no reaction, 1: penalty reaction, 2: moving-boundary reaction
+__global__ void __launch_bounds__(PTPB) k_mpm_p2g_react(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+                                                        FsiMpmState ms, FsiCouplingParams cp, double* rec) {
+  __shared__ float tile[7 * FUSED_TILE_NODES];
+  __shared__ int sbox[6];
+  __shared__ double sd[PTPB / 64];
+  __shared__ float sf[PTPB / 64];
+  __shared__ int si[PTPB / 64];
+  const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
+  p2g_react_block<MODE>(lp, ls, mp, ms, cp, rec + (size_t)blockIdx.x * REACT_REC, q, q < mp.n_particles, tile, sbox,
+                        sd, sf, si);
+}
+
+// Fold of the per-block reaction records (in block order: deterministic) into
+// the counters, by one whole workgroup of TPB threads.
+__device__ __forceinline__ void fold_react_records(const double* rec, int n_rec, FsiCouplingCounters* cc,
+                                                   double* sd) {
+  double a[REACT_REC] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < n_rec; k += (int)blockDim.x) {
+    const double* r = rec + (size_t)k * REACT_REC;
+    a[0] += r[0];
+    a[1] = r[1] > a[1] ? r[1] : a[1];
+    a[2] = r[2] > a[2] ? r[2] : a[2];
+    a[3] += r[3];
+    a[4] += r[4];
+    a[5] += r[5];
+  }
+  const double act = block_reduce(a[0], RED_SUM, sd);
+  const double mpn = block_reduce(a[1], RED_MAX, sd);
+  const double mgc = block_reduce(a[2], RED_MAX, sd);
+  const double nx = block_reduce(a[3], RED_SUM, sd);
+  const double ny = block_reduce(a[4], RED_SUM, sd);
+  const double nz = block_reduce(a[5], RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    cc->active_reaction_particle_count = (int32_t)act;
+    cc->max_particle_reaction_bits = __float_as_uint((float)mpn);
+    cc->max_grid_reaction_bits = __float_as_uint((float)mgc);
+    cc->net_particle_reaction[0] = nx;
+    cc->net_particle_reaction[1] = ny;
+    cc->net_particle_reaction[2] = nz;
+    cc->net_grid_reaction[0] = cc->net_grid_reaction[1] = cc->net_grid_reaction[2] = 0.0;
   }
 }
 
@@ -659,33 +726,7 @@ __global__ void __launch_bounds__(TPB) k_mpm_grid_update_react(FsiMpmParams mp, 
   __shared__ double sd[TPB / 64];
   const long long n = (long long)blockIdx.x * TPB + threadIdx.x;
   if (n < (long long)mp.gnx * mp.gny * mp.gnz) mpm_grid_node(mp, ms, n);
-  if (rec && blockIdx.x == 0) {
-    double a[REACT_REC] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int k = threadIdx.x; k < n_rec; k += TPB) {
-      const double* r = rec + (size_t)k * REACT_REC;
-      a[0] += r[0];
-      a[1] = r[1] > a[1] ? r[1] : a[1];
-      a[2] = r[2] > a[2] ? r[2] : a[2];
-      a[3] += r[3];
-      a[4] += r[4];
-      a[5] += r[5];
-    }
-    const double act = block_reduce(a[0], RED_SUM, sd);
-    const double mpn = block_reduce(a[1], RED_MAX, sd);
-    const double mgc = block_reduce(a[2], RED_MAX, sd);
-    const double nx = block_reduce(a[3], RED_SUM, sd);
-    const double ny = block_reduce(a[4], RED_SUM, sd);
-    const double nz = block_reduce(a[5], RED_SUM, sd);
-    if (threadIdx.x == 0) {
-      cc->active_reaction_particle_count = (int32_t)act;
-      cc->max_particle_reaction_bits = __float_as_uint((float)mpn);
-      cc->max_grid_reaction_bits = __float_as_uint((float)mgc);
-      cc->net_particle_reaction[0] = nx;
-      cc->net_particle_reaction[1] = ny;
-      cc->net_particle_reaction[2] = nz;
-      cc->net_grid_reaction[0] = cc->net_grid_reaction[1] = cc->net_grid_reaction[2] = 0.0;
-    }
-  }
+  if (rec && blockIdx.x == 0) fold_react_records(rec, n_rec, cc, sd);
 }
 
 __global__ void __launch_bounds__(TPB) k_mpm_grid_update(FsiMpmParams mp, FsiMpmState ms) {
@@ -713,36 +754,37 @@ __global__ void __launch_bounds__(TPB) k_mpm_stats_partial(FsiMpmParams mp, FsiM
 }
 
 // ---------------------------------------------------------------------------
-// Batched kernels: workgroup b belongs to the instance i with
-// block_start[i] <= b < block_start[i+1]; its local work item is
-// (b - block_start[i]) * TPB + threadIdx.x.
+// Batched kernels.  Dispatch slot blockIdx.x takes entry b of the block map
+// (b = xcd_tile(...): each XCD gets one contiguous run of the map, i.e. of
+// whole instances), which names the instance and the workgroup's index L in
+// that instance's domain; its local work item is L * (block size) + thread.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int batch_instance(const int32_t* bs, int n) {
-  int lo = 0, hi = n;
-  const int b = (int)blockIdx.x;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (bs[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-#define FSI_BATCH_PROLOGUE(DOMAIN)                                                   \
-  const int i_ = batch_instance(bs, n_inst);                                          \
-  const FsiBatchInstance& I = inst[i_];                                               \
-  const long long e_ = (long long)((int)blockIdx.x - bs[i_]) * TPB + threadIdx.x;     \
+#define FSI_BATCH_PROLOGUE(DOMAIN, BS)                                                     \
+  const unsigned b_ = xcd_tile(blockIdx.x, gridDim.x);                                     \
+  const int i_ = bmap[2 * b_], l_ = bmap[2 * b_ + 1];                                       \
+  const FsiBatchInstance& I = inst[i_];                                                    \
+  const long long e_ = (long long)l_ * (BS) + threadIdx.x;                                 \
   const bool in_ = e_ < batch_domain_size(I, DOMAIN);
 
-__global__ void __launch_bounds__(TPB) kb_substep(const FsiBatchInstance* inst, int n_inst, const int32_t* bs, int t) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
-  if (!in_) return;
-  const uint32_t flags = batch_substep_flags(I, t);
-  const int cur = batch_cur(I, t), parity = t & 1;
+// FIXED: the substep flags common to the launch's instances (STORED_MOMENTS,
+// MOVING_WALL, CELL_FORCE), compiled in as in k_lbm_substep; ~0u = take them
+// from each instance at run time (any mix of modes).
+template <uint32_t FIXED>
+__global__ void __launch_bounds__(TPB) kb_substep(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
+  const unsigned b_ = xcd_tile(blockIdx.x, gridDim.x);
+  const int i_ = bmap[2 * b_], l_ = bmap[2 * b_ + 1];
+  const FsiBatchInstance& I = inst[i_];
   const FsiLbmParams p = I.lp;
   const FsiLbmState st = I.ls;
-  const unsigned e = (unsigned)e_, t1 = e / (unsigned)p.nz, k = e - t1 * (unsigned)p.nz;
-  const unsigned i = t1 / (unsigned)p.ny, j = t1 - i * (unsigned)p.ny;
+  // the single-instance launch geometry: l_ = i * tiles + tile of the (j,k) plane
+  const unsigned plane = (unsigned)p.ny * (unsigned)p.nz, tiles = (plane + TPB - 1) / TPB;
+  const unsigned i = (unsigned)l_ / tiles;
+  const unsigned jk = ((unsigned)l_ - i * tiles) * TPB + threadIdx.x;
+  if (jk >= plane) return;
+  const unsigned j = jk / (unsigned)p.nz, k = jk - j * (unsigned)p.nz;
+  constexpr uint32_t VAR = FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE;
+  const uint32_t flags = FIXED == ~0u ? batch_substep_flags(I, t) : ((batch_substep_flags(I, t) & ~VAR) | FIXED);
+  const int cur = batch_cur(I, t), parity = t & 1;
   if (p.storage == FSI_STORE_F16_DEV)
     lbm_substep_cell<PopF16>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
   else if (p.storage == FSI_STORE_BF16_DEV)
@@ -751,23 +793,23 @@ __global__ void __launch_bounds__(TPB) kb_substep(const FsiBatchInstance* inst, 
     lbm_substep_cell<PopF32>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
 }
 
-__global__ void __launch_bounds__(TPB) kb_rebuild(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+__global__ void __launch_bounds__(TPB) kb_rebuild(const FsiBatchInstance* inst, const int32_t* bmap) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   if (!in_) return;
   const int dyn = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 1 : 0;
   FSI_BY_STORAGE(I.lp.storage, lbm_rebuild_cell<POP>(I.lp, I.ls, dyn, I.cur0, e_));
 }
 
-__global__ void __launch_bounds__(TPB) kb_moments(const FsiBatchInstance* inst, int n_inst, const int32_t* bs, int t) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+__global__ void __launch_bounds__(TPB) kb_moments(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   if (!in_) return;
   // moments after substep t: the populations are in F[cur0 ^ ((t + 1) & 1)]
   const uint32_t flags = I.cell_force ? FSI_SUB_CELL_FORCE : 0u;
   FSI_BY_STORAGE(I.lp.storage, lbm_moments_cell<POP>(I.lp, I.ls, batch_cur(I, t + 1), flags, e_));
 }
 
-__global__ void __launch_bounds__(TPB) kb_mb_begin(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+__global__ void __launch_bounds__(TPB) kb_mb_begin(const FsiBatchInstance* inst, const int32_t* bmap) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   if (e_ == 0) {
     FsiLbmCounters* k = I.ls.counters;
     k->bb_link_count = 0;
@@ -780,11 +822,11 @@ __global__ void __launch_bounds__(TPB) kb_mb_begin(const FsiBatchInstance* inst,
   if (in_) lbm_mb_begin_cell(I.lp, I.ls, e_);
 }
 
-__global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+__global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   FinalizeOut o;
   o.links = 0;
   o.max_corr = o.mean_norm = o.runmax = 0.0f;
@@ -803,13 +845,13 @@ __global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* inst,
   if (threadIdx.x == 0) {
     FsiLbmCounters* k = I.ls.counters;
     if (links) atomicAdd(&k->bb_link_count, links);
-    atomicMax(&k->bb_max_correction_bits, __float_as_uint(mc));
-    atomicMax(&k->mean_norm_max_bits, __float_as_uint(mn));
-    atomicMax(&k->accum_norm_max_bits, __float_as_uint(rm));
+    if (mc > 0.0f) atomicMax(&k->bb_max_correction_bits, __float_as_uint(mc));
+    if (mn > 0.0f) atomicMax(&k->mean_norm_max_bits, __float_as_uint(mn));
+    if (rm > 0.0f) atomicMax(&k->accum_norm_max_bits, __float_as_uint(rm));
     if (hx != 0.0) atomicAdd(&k->bb_net_solid_force[0], hx);
     if (hy != 0.0) atomicAdd(&k->bb_net_solid_force[1], hy);
     if (hz != 0.0) atomicAdd(&k->bb_net_solid_force[2], hz);
-    if (blockIdx.x == (unsigned)bs[i_]) k->mb_sample_count = I.n_substeps;
+    if (l_ == 0) k->mb_sample_count = I.n_substeps;
   }
 }
 
@@ -838,8 +880,8 @@ __global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* 
   }
 }
 
-__global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+__global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* inst, const int32_t* bmap) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   if (!in_) return;
   const long long N = batch_domain_size(I, FSI_BATCH_DOMAIN_CELLS);
   I.ls.solid_phi[e_] = 0.0f;
@@ -847,16 +889,18 @@ __global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* ins
   I.ls.solid_vel[e_] = I.ls.solid_vel[N + e_] = I.ls.solid_vel[2 * N + e_] = 0.0f;
 }
 
-__global__ void __launch_bounds__(TPB) kb_proj_particles(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
-  if (in_) project_particle(I.lp, I.ls, I.mp, I.ms, I.cp, e_);
+__global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance* inst, const int32_t* bmap) {
+  __shared__ float tile[5 * PROJ_TILE_NODES];
+  __shared__ int sbox[6];
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
+  proj_block(I.lp, I.ls, I.mp, I.ms, I.cp, e_, in_, tile, sbox);
 }
 
-__global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+__global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   ProjOut o;
   o.phi_raw = o.phi_clamped = o.mass = 0.0f;
   o.active = 0;
@@ -868,7 +912,7 @@ __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance*
   const double vc = block_reduce((double)o.phi_clamped * I.cp.cell_volume_norm, RED_SUM, sd);
   if (threadIdx.x == 0) {
     FsiCouplingCounters* cc = I.cc;
-    atomicMax(&cc->max_phi_raw_bits, __float_as_uint(mp));
+    if (mp > 0.0f) atomicMax(&cc->max_phi_raw_bits, __float_as_uint(mp));
     if (act) atomicAdd(&cc->active_cell_count, act);
     if (m != 0.0) atomicAdd(&cc->projected_mass, m);
     if (vr != 0.0) atomicAdd(&cc->projected_volume_raw, vr);
@@ -876,11 +920,11 @@ __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance*
   }
 }
 
-__global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+__global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS)
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   float f[3] = {0.0f, 0.0f, 0.0f};
   int active = 0;
   if (in_) penalty_cell(I.lp, I.ls, I.cp, e_, f, active);
@@ -891,8 +935,10 @@ __global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, 
   if (threadIdx.x == 0) {
     FsiCouplingCounters* cc = I.cc;
     if (act) atomicAdd(&cc->active_force_cell_count, act);
-    atomicMax(&cc->max_cell_force_bits, __float_as_uint(mfn));
-    atomicMax(&cc->max_hydro_force_bits, __float_as_uint(mfn));
+    if (mfn > 0.0f) {
+      atomicMax(&cc->max_cell_force_bits, __float_as_uint(mfn));
+      atomicMax(&cc->max_hydro_force_bits, __float_as_uint(mfn));
+    }
     for (int d = 0; d < 3; ++d)
       if (net[d] != 0.0) {
         atomicAdd(&cc->net_cell_force[d], net[d]);
@@ -901,8 +947,8 @@ __global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, 
   }
 }
 
-__global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES)
+__global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst, const int32_t* bmap) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES, TPB)
   if (!in_) return;
   const long long G = batch_domain_size(I, FSI_BATCH_DOMAIN_NODES);
   I.ms.grid_v[e_] = I.ms.grid_v[G + e_] = I.ms.grid_v[2 * G + e_] = 0.0f;
@@ -910,20 +956,20 @@ __global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst
   I.ms.grid_f[e_] = I.ms.grid_f[G + e_] = I.ms.grid_f[2 * G + e_] = 0.0f;
 }
 
-__global__ void __launch_bounds__(TPB) kb_mpm_p2g(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
+__global__ void __launch_bounds__(PTPB) kb_mpm_p2g(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ float tile[4 * TILE_NODES];
   __shared__ int sbox[6];
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   p2g_block(I.mp, I.ms, e_, in_, tile, sbox);
 }
 
-__global__ void __launch_bounds__(TPB) kb_mpm_reaction(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
-  __shared__ double sd[TPB / 64];
-  __shared__ float sf[TPB / 64];
-  __shared__ int si[TPB / 64];
+__global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* inst, const int32_t* bmap) {
+  __shared__ double sd[PTPB / 64];
+  __shared__ float sf[PTPB / 64];
+  __shared__ int si[PTPB / 64];
   __shared__ float tile[3 * TILE_NODES];
   __shared__ int sbox[6];
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   const int mode = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1;
   const ReactOut o = reaction_block(I.lp, I.ls, I.mp, I.ms, I.cp, mode, e_, in_ && I.mode != FSI_BATCH_MODE_NONE,
                                     tile, sbox);
@@ -935,20 +981,44 @@ __global__ void __launch_bounds__(TPB) kb_mpm_reaction(const FsiBatchInstance* i
   if (threadIdx.x == 0) {
     FsiCouplingCounters* cc = I.cc;
     if (act) atomicAdd(&cc->active_reaction_particle_count, act);
-    atomicMax(&cc->max_particle_reaction_bits, __float_as_uint(mpn));
-    atomicMax(&cc->max_grid_reaction_bits, __float_as_uint(mgc));
+    if (mpn > 0.0f) atomicMax(&cc->max_particle_reaction_bits, __float_as_uint(mpn));
+    if (mgc > 0.0f) atomicMax(&cc->max_grid_reaction_bits, __float_as_uint(mgc));
     for (int d = 0; d < 3; ++d)
       if (net[d] != 0.0) atomicAdd(&cc->net_particle_reaction[d], net[d]);
   }
 }
 
-__global__ void __launch_bounds__(TPB) kb_mpm_grid_update(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES)
+__global__ void __launch_bounds__(PTPB) kb_mpm_p2g_react(const FsiBatchInstance* inst, const int32_t* bmap) {
+  __shared__ float tile[7 * FUSED_TILE_NODES];
+  __shared__ int sbox[6];
+  __shared__ double sd[PTPB / 64];
+  __shared__ float sf[PTPB / 64];
+  __shared__ int si[PTPB / 64];
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
+  double* rec = I.react_ws + (size_t)l_ * REACT_REC;
+  if (I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY)
+    p2g_react_block<2>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sbox, sd, sf, si);
+  else if (I.mode == FSI_BATCH_MODE_PENALTY)
+    p2g_react_block<1>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sbox, sd, sf, si);
+  else
+    p2g_react_block<0>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sbox, sd, sf, si);
+}
+
+__global__ void __launch_bounds__(TPB) kb_mpm_grid_update(const FsiBatchInstance* inst, const int32_t* bmap) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES, TPB)
   if (in_) mpm_grid_node(I.mp, I.ms, e_);
 }
 
-__global__ void __launch_bounds__(TPB) kb_mpm_g2p(const FsiBatchInstance* inst, int n_inst, const int32_t* bs) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES)
+__global__ void __launch_bounds__(TPB) kb_mpm_grid_update_react(const FsiBatchInstance* inst, const int32_t* bmap) {
+  __shared__ double sd[TPB / 64];
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES, TPB)
+  if (in_) mpm_grid_node(I.mp, I.ms, e_);
+  if (l_ == 0 && I.mode != FSI_BATCH_MODE_NONE && I.mp.n_particles > 0)
+    fold_react_records(I.react_ws, (int)((I.mp.n_particles + PTPB - 1) / PTPB), I.cc, sd);
+}
+
+__global__ void __launch_bounds__(PTPB) kb_mpm_g2p(const FsiBatchInstance* inst, const int32_t* bmap) {
+  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   if (in_) mpm_g2p_particle(I.mp, I.ms, e_);
 }
 
@@ -1222,25 +1292,28 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
 }
 
 
-int fsi_batch_block_table(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask,
-                          int32_t* block_start_host) {
-  if (!inst_host || !block_start_host || n_inst < 0) return fail(FSI_E_INVALID, "null batch arguments");
-  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_PARTICLES)
+int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask, int min_mpm,
+                        int32_t* map_host) {
+  if (!inst_host || n_inst < 0) return fail(FSI_E_INVALID, "null batch arguments");
+  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_PLANES)
     return fail(FSI_E_INVALID, "unknown batch domain");
   long long total = 0;
-  block_start_host[0] = 0;
   for (int i = 0; i < n_inst; ++i) {
     const FsiBatchInstance& I = inst_host[i];
-    long long nb = 0;
-    if (I.active && (mode_mask & (1 << I.mode))) nb = (batch_domain_size(I, domain) + TPB - 1) / TPB;
+    if (!I.active || !(mode_mask & (1 << I.mode)) || I.mpm_substeps <= min_mpm) continue;
+    const long long nb = batch_domain_blocks(I, domain);
+    if (total + nb > 0x7fffffffLL) return fail(FSI_E_INVALID, "batch too large for one launch");
+    if (map_host)
+      for (long long b = 0; b < nb; ++b) {
+        map_host[2 * (total + b)] = i;
+        map_host[2 * (total + b) + 1] = (int32_t)b;
+      }
     total += nb;
-    if (total > 0x7fffffffLL) return fail(FSI_E_INVALID, "batch too large for one launch");
-    block_start_host[i + 1] = (int32_t)total;
   }
   return (int)total;
 }
 
-int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const int32_t* block_start_dev, int n_blocks,
+int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const int32_t* block_map_dev, int n_blocks,
                      int t, void* stream) {
   if (!inst_dev || n_inst <= 0) return fail(FSI_E_INVALID, "empty batch");
   hipStream_t s = as_stream(stream);
@@ -1248,25 +1321,43 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const
     hipLaunchKernelGGL(kb_counters_clear, dim3((n_inst + 63) / 64), dim3(64), 0, s, inst_dev, n_inst, t);
     return check_launch("kb_counters_clear");
   }
-  if (!block_start_dev) return fail(FSI_E_INVALID, "null block table");
+  if (!block_map_dev) return fail(FSI_E_INVALID, "null block map");
   if (n_blocks <= 0) return FSI_OK;
-  const dim3 g(n_blocks), b(TPB);
+  const dim3 g(n_blocks), b(TPB), bp(PTPB);
+  const int32_t* m = block_map_dev;
   switch (op) {
-    case FSI_BATCH_OP_LBM_SUBSTEP: hipLaunchKernelGGL(kb_substep, g, b, 0, s, inst_dev, n_inst, block_start_dev, t); break;
-    case FSI_BATCH_OP_LBM_REBUILD: hipLaunchKernelGGL(kb_rebuild, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_LBM_MOMENTS: hipLaunchKernelGGL(kb_moments, g, b, 0, s, inst_dev, n_inst, block_start_dev, t); break;
-    case FSI_BATCH_OP_LBM_MB_BEGIN: hipLaunchKernelGGL(kb_mb_begin, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_LBM_MB_FINALIZE: hipLaunchKernelGGL(kb_finalize, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_PROJ_CLEAR: hipLaunchKernelGGL(kb_proj_clear, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_PROJ_PARTICLES: hipLaunchKernelGGL(kb_proj_particles, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_PROJ_NORMALIZE: hipLaunchKernelGGL(kb_proj_normalize, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_PENALTY: hipLaunchKernelGGL(kb_penalty, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_MPM_CLEAR: hipLaunchKernelGGL(kb_mpm_clear, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_MPM_P2G: hipLaunchKernelGGL(kb_mpm_p2g, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_MPM_REACTION: hipLaunchKernelGGL(kb_mpm_reaction, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_MPM_GRID_UPDATE: hipLaunchKernelGGL(kb_mpm_grid_update, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    case FSI_BATCH_OP_MPM_G2P: hipLaunchKernelGGL(kb_mpm_g2p, g, b, 0, s, inst_dev, n_inst, block_start_dev); break;
-    default: return fail(FSI_E_INVALID, "unknown batch op");
+    case FSI_BATCH_OP_LBM_SUBSTEP: hipLaunchKernelGGL(kb_substep<~0u>, g, b, 0, s, inst_dev, m, t); break;
+    case FSI_BATCH_OP_LBM_SUBSTEP_MB:
+      if (t == 0)
+        hipLaunchKernelGGL(kb_substep<FSI_SUB_MOVING_WALL | FSI_SUB_STORED_MOMENTS>, g, b, 0, s, inst_dev, m, t);
+      else
+        hipLaunchKernelGGL(kb_substep<FSI_SUB_MOVING_WALL>, g, b, 0, s, inst_dev, m, t);
+      break;
+    case FSI_BATCH_OP_LBM_SUBSTEP_PENALTY:
+      hipLaunchKernelGGL(kb_substep<FSI_SUB_CELL_FORCE | FSI_SUB_STORED_MOMENTS>, g, b, 0, s, inst_dev, m, t);
+      break;
+    case FSI_BATCH_OP_LBM_SUBSTEP_NONE:
+      hipLaunchKernelGGL(kb_substep<FSI_SUB_STORED_MOMENTS>, g, b, 0, s, inst_dev, m, t);
+      break;
+    case FSI_BATCH_OP_LBM_REBUILD: hipLaunchKernelGGL(kb_rebuild, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_LBM_MOMENTS: hipLaunchKernelGGL(kb_moments, g, b, 0, s, inst_dev, m, t); break;
+    case FSI_BATCH_OP_LBM_MB_BEGIN: hipLaunchKernelGGL(kb_mb_begin, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_LBM_MB_FINALIZE: hipLaunchKernelGGL(kb_finalize, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_PROJ_CLEAR: hipLaunchKernelGGL(kb_proj_clear, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_PROJ_PARTICLES: hipLaunchKernelGGL(kb_proj_particles, g, bp, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_PROJ_NORMALIZE: hipLaunchKernelGGL(kb_proj_normalize, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_PENALTY: hipLaunchKernelGGL(kb_penalty, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_MPM_CLEAR: hipLaunchKernelGGL(kb_mpm_clear, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_MPM_P2G: hipLaunchKernelGGL(kb_mpm_p2g, g, bp, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_MPM_REACTION: hipLaunchKernelGGL(kb_mpm_reaction, g, bp, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_MPM_GRID_UPDATE: hipLaunchKernelGGL(kb_mpm_grid_update, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_MPM_G2P: hipLaunchKernelGGL(kb_mpm_g2p, g, bp, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_MPM_P2G_REACT: hipLaunchKernelGGL(kb_mpm_p2g_react, g, bp, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_MPM_GRID_UPDATE_REACT:
+      hipLaunchKernelGGL(kb_mpm_grid_update_react, g, b, 0, s, inst_dev, m);
+      break;
+    default:
+      return fail(FSI_E_INVALID, "unknown batch operation");
   }
   return check_launch("fsi_batch_launch");
 }

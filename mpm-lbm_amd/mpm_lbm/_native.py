@@ -161,10 +161,27 @@ BATCH_MODE_MOVING_BOUNDARY = 2
 BATCH_DOMAIN_CELLS = 0
 BATCH_DOMAIN_NODES = 1
 BATCH_DOMAIN_PARTICLES = 2
+BATCH_DOMAIN_PLANES = 3
 (BATCH_OP_LBM_SUBSTEP, BATCH_OP_LBM_REBUILD, BATCH_OP_LBM_MOMENTS, BATCH_OP_LBM_MB_BEGIN, BATCH_OP_LBM_MB_FINALIZE,
  BATCH_OP_PROJ_CLEAR, BATCH_OP_PROJ_PARTICLES, BATCH_OP_PROJ_NORMALIZE, BATCH_OP_PENALTY, BATCH_OP_MPM_CLEAR,
  BATCH_OP_MPM_P2G, BATCH_OP_MPM_REACTION, BATCH_OP_MPM_GRID_UPDATE, BATCH_OP_MPM_G2P,
- BATCH_OP_COUNTERS_CLEAR) = range(15)
+ BATCH_OP_COUNTERS_CLEAR, BATCH_OP_MPM_P2G_REACT, BATCH_OP_MPM_GRID_UPDATE_REACT, BATCH_OP_LBM_SUBSTEP_MB,
+ BATCH_OP_LBM_SUBSTEP_PENALTY, BATCH_OP_LBM_SUBSTEP_NONE) = range(20)
+
+# domain of each batched operation (the block map a launch takes is built for it)
+BATCH_OP_DOMAIN = {
+    BATCH_OP_LBM_SUBSTEP: BATCH_DOMAIN_PLANES, BATCH_OP_LBM_SUBSTEP_MB: BATCH_DOMAIN_PLANES,
+    BATCH_OP_LBM_SUBSTEP_PENALTY: BATCH_DOMAIN_PLANES, BATCH_OP_LBM_SUBSTEP_NONE: BATCH_DOMAIN_PLANES,
+    BATCH_OP_LBM_REBUILD: BATCH_DOMAIN_CELLS, BATCH_OP_LBM_MOMENTS: BATCH_DOMAIN_CELLS,
+    BATCH_OP_LBM_MB_BEGIN: BATCH_DOMAIN_CELLS, BATCH_OP_LBM_MB_FINALIZE: BATCH_DOMAIN_CELLS,
+    BATCH_OP_PROJ_CLEAR: BATCH_DOMAIN_CELLS, BATCH_OP_PROJ_NORMALIZE: BATCH_DOMAIN_CELLS,
+    BATCH_OP_PENALTY: BATCH_DOMAIN_CELLS,
+    BATCH_OP_MPM_CLEAR: BATCH_DOMAIN_NODES, BATCH_OP_MPM_GRID_UPDATE: BATCH_DOMAIN_NODES,
+    BATCH_OP_MPM_GRID_UPDATE_REACT: BATCH_DOMAIN_NODES,
+    BATCH_OP_PROJ_PARTICLES: BATCH_DOMAIN_PARTICLES, BATCH_OP_MPM_P2G: BATCH_DOMAIN_PARTICLES,
+    BATCH_OP_MPM_REACTION: BATCH_DOMAIN_PARTICLES, BATCH_OP_MPM_P2G_REACT: BATCH_DOMAIN_PARTICLES,
+    BATCH_OP_MPM_G2P: BATCH_DOMAIN_PARTICLES,
+}
 
 
 class BatchInstance(ctypes.Structure):
@@ -172,7 +189,7 @@ class BatchInstance(ctypes.Structure):
         ("lp", LbmParams), ("ls", LbmState), ("mp", MpmParams), ("ms", MpmState),
         ("cp", CouplingParams), ("cc", vp),
         ("mode", i32), ("cur0", i32), ("n_substeps", i32), ("mpm_substeps", i32), ("active", i32),
-        ("cell_force", i32),
+        ("cell_force", i32), ("pad0", i32), ("react_ws", vp),
     ]
 
 
@@ -185,7 +202,7 @@ EXPORTED_SYMBOLS = [
     "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats",
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
     "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_reduce_stats",
-    "fsi_batch_block_table", "fsi_batch_launch",
+    "fsi_batch_block_map", "fsi_batch_launch",
 ]
 
 PARTICLE_BLOCK = 64  # FSI_PARTICLE_BLOCK
@@ -235,7 +252,7 @@ class Native:
         L.fsi_mpm_substep_coupled.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),
                                               P(CouplingParams), vp, ctypes.c_int, vp, vp]
         L.fsi_mpm_reduce_stats.argtypes = [P(MpmParams), P(MpmState), vp, vp, vp]
-        L.fsi_batch_block_table.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.fsi_batch_block_map.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
         L.fsi_batch_launch.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]
         for name in EXPORTED_SYMBOLS:
             getattr(L, name).restype = ctypes.c_int if name != "fsi_last_error" else ctypes.c_char_p
@@ -254,7 +271,7 @@ class Native:
 
     def call(self, name: str, *args):
         rc = getattr(self.lib, name)(*args)
-        if rc < 0 or (rc != 0 and name != "fsi_batch_block_table"):
+        if rc < 0 or (rc != 0 and name != "fsi_batch_block_map"):
             msg = self.lib.fsi_last_error().decode("utf-8", "replace")
             raise FsiError(f"{name} failed ({rc}): {msg}")
         return rc

@@ -53,6 +53,8 @@ class BatchedFSI:
             if d.config.coupling_mode == "moving_boundary" and d.config.fsi_exchange_mode != "lbm_subcycled_per_fsi_step":
                 raise ValueError("moving-boundary instances of a batch must be subcycled")
         self._counters = [struct_buffer(_native.CouplingCounters, self.device) for _ in self.drivers]
+        self._react_ws = [torch.zeros(max(1, _native.coupled_workspace_doubles(d.solid.n_particles)),
+                                      dtype=torch.float64, device=self.device) for d in self.drivers]
         self._prepare_instances()
         self.use_graph = use_graph and self.device.type == "cuda"
         self._graph = None
@@ -82,29 +84,32 @@ class BatchedFSI:
             I.mpm_substeps = d.config.mpm_substeps_per_lbm_step
             I.active = 1
             I.cell_force = 1 if d.config.coupling_mode == "penalty" else 0
+            I.react_ws = _native.ptr(self._react_ws[k])
             lbm._cell_force_active = bool(I.cell_force)
         self._inst_host = inst
         raw = bytes(memoryview(inst).cast("B"))
         self._inst_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
         self._tables = {}
 
-    def _table(self, domain, modes, min_mpm=0):
-        """Device block table for instances of `modes` with at least min_mpm + 1 MPM substeps."""
+    def _block_map(self, domain, modes, min_mpm=0):
+        """Device block map (instance, workgroup) of `domain` over the instances
+        of `modes` with more than min_mpm MPM substeps (fsi_batch_block_map)."""
         key = (domain, tuple(sorted(modes)), min_mpm)
         if key not in self._tables:
-            starts = [0]
-            for I in self._inst_host:
-                size = {_native.BATCH_DOMAIN_CELLS: I.lp.nx * I.lp.ny * I.lp.nz,
-                        _native.BATCH_DOMAIN_NODES: I.mp.gnx * I.mp.gny * I.mp.gnz,
-                        _native.BATCH_DOMAIN_PARTICLES: I.mp.n_particles}[domain]
-                use = I.active and I.mode in modes and I.mpm_substeps > min_mpm
-                starts.append(starts[-1] + ((size + TPB - 1) // TPB if use else 0))
-            t = torch.tensor(starts, dtype=torch.int32, device=self.device)
-            self._tables[key] = (t, starts[-1])
+            mask = sum(1 << m for m in modes)
+            addr = ctypes.addressof(self._inst_host)
+            n = self.native.lib.fsi_batch_block_map(addr, len(self.drivers), domain, mask, min_mpm, None)
+            if n < 0:
+                raise _native.FsiError(self.native.lib.fsi_last_error().decode())
+            host = np.zeros((max(n, 1), 2), dtype=np.int32)
+            if n:
+                self.native.lib.fsi_batch_block_map(addr, len(self.drivers), domain, mask, min_mpm,
+                                                    host.ctypes.data_as(ctypes.c_void_p))
+            self._tables[key] = (torch.from_numpy(host).to(self.device), n)
         return self._tables[key]
 
-    def _launch(self, op, domain, modes, t=0, min_mpm=0):
-        table, nblocks = self._table(domain, modes, min_mpm)
+    def _launch(self, op, modes, t=0, min_mpm=0):
+        table, nblocks = self._block_map(_native.BATCH_OP_DOMAIN[op], modes, min_mpm)
         if nblocks == 0:
             return
         self.native.call("fsi_batch_launch", op, _native.ptr(self._inst_dev), len(self.drivers), _native.ptr(table),
@@ -114,52 +119,49 @@ class BatchedFSI:
         self.native.call("fsi_batch_launch", _native.BATCH_OP_COUNTERS_CLEAR, _native.ptr(self._inst_dev),
                          len(self.drivers), None, 0, which, stream_handle(self.device))
 
-    def _mpm_substeps(self, modes, reaction):
-        C, N, P = _native.BATCH_DOMAIN_CELLS, _native.BATCH_DOMAIN_NODES, _native.BATCH_DOMAIN_PARTICLES
+    def _mpm_substeps(self, modes):
+        """MPM substeps with each instance's reaction (fsi_mpm_substep_coupled per instance)."""
         max_m = max(I.mpm_substeps for I in self._inst_host if I.mode in modes)
         for j in range(max_m):
-            self._launch(_native.BATCH_OP_MPM_CLEAR, N, modes, min_mpm=j)
-            self._launch(_native.BATCH_OP_MPM_P2G, P, modes, min_mpm=j)
-            if reaction:
-                self._clear_counters(2)
-                self._launch(_native.BATCH_OP_MPM_REACTION, P, modes, min_mpm=j)
-            self._launch(_native.BATCH_OP_MPM_GRID_UPDATE, N, modes, min_mpm=j)
-            self._launch(_native.BATCH_OP_MPM_G2P, P, modes, min_mpm=j)
+            self._launch(_native.BATCH_OP_MPM_CLEAR, modes, min_mpm=j)
+            self._launch(_native.BATCH_OP_MPM_P2G_REACT, modes, min_mpm=j)
+            self._launch(_native.BATCH_OP_MPM_GRID_UPDATE_REACT, modes, min_mpm=j)
+            self._launch(_native.BATCH_OP_MPM_G2P, modes, min_mpm=j)
 
     def _project(self, modes):
-        C, P = _native.BATCH_DOMAIN_CELLS, _native.BATCH_DOMAIN_PARTICLES
         self._clear_counters(0)
-        self._launch(_native.BATCH_OP_PROJ_CLEAR, C, modes)
-        self._launch(_native.BATCH_OP_PROJ_PARTICLES, P, modes)
-        self._launch(_native.BATCH_OP_PROJ_NORMALIZE, C, modes)
+        self._launch(_native.BATCH_OP_PROJ_CLEAR, modes)
+        self._launch(_native.BATCH_OP_PROJ_PARTICLES, modes)
+        self._launch(_native.BATCH_OP_PROJ_NORMALIZE, modes)
 
     def _issue_exchange(self):
         """One batched exchange: n substeps of every instance (see the module docstring)."""
-        C = _native.BATCH_DOMAIN_CELLS
         MB = (_native.BATCH_MODE_MOVING_BOUNDARY,)
         ONE = (_native.BATCH_MODE_PENALTY, _native.BATCH_MODE_NONE)
         PEN = (_native.BATCH_MODE_PENALTY,)
-        ALL = MB + ONE
         has_mb = any(I.mode in MB for I in self._inst_host)
         has_one = any(I.mode in ONE for I in self._inst_host)
         if has_mb:
             self._project(MB)
-            self._launch(_native.BATCH_OP_LBM_MB_BEGIN, C, MB)
-            self._launch(_native.BATCH_OP_LBM_REBUILD, C, MB)
+            self._launch(_native.BATCH_OP_LBM_MB_BEGIN, MB)
+            self._launch(_native.BATCH_OP_LBM_REBUILD, MB)
         for t in range(self.n_substeps):
             if has_one:
                 self._project(ONE)
                 self._clear_counters(1)
-                self._launch(_native.BATCH_OP_PENALTY, C, PEN)
-            self._launch(_native.BATCH_OP_LBM_SUBSTEP, C, ALL, t=t)
+                self._launch(_native.BATCH_OP_PENALTY, PEN)
+            # one launch per mode, each with its mode's substep flags compiled in
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP_MB, MB, t=t)
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP_PENALTY, PEN, t=t)
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP_NONE, (_native.BATCH_MODE_NONE,), t=t)
             if has_one:
-                self._launch(_native.BATCH_OP_LBM_MOMENTS, C, ONE, t=t)
-                self._mpm_substeps(ONE, reaction=True)
+                self._launch(_native.BATCH_OP_LBM_MOMENTS, ONE, t=t)
+                self._mpm_substeps(ONE)
         if has_mb:
             self._clear_counters(3)
-            self._launch(_native.BATCH_OP_LBM_MB_FINALIZE, C, MB)
-            self._launch(_native.BATCH_OP_LBM_MOMENTS, C, MB, t=self.n_substeps - 1)
-            self._mpm_substeps(MB, reaction=True)
+            self._launch(_native.BATCH_OP_LBM_MB_FINALIZE, MB)
+            self._launch(_native.BATCH_OP_LBM_MOMENTS, MB, t=self.n_substeps - 1)
+            self._mpm_substeps(MB)
 
     # ------------------------------------------------------------------- run
     def step(self):

@@ -49,16 +49,43 @@ def test_batched_equals_alone(emu, tmp_path):
         assert np.array_equal(b.lbm.solid.to_numpy(), a.lbm.solid.to_numpy())
 
 
-def test_block_table_helper_matches_python(emu, tmp_path):
+def _expected_map(inst, domain, modes, min_mpm):
+    """Block map restated in Python: (instance, workgroup) pairs, instances in index order."""
+    from mpm_lbm import _native
+
+    rows = []
+    for k, I in enumerate(inst):
+        if not (I.active and I.mode in modes and I.mpm_substeps > min_mpm):
+            continue
+        if domain == _native.BATCH_DOMAIN_PLANES:
+            nb = I.lp.nx * (-(-(I.lp.ny * I.lp.nz) // 256))
+        elif domain == _native.BATCH_DOMAIN_CELLS:
+            nb = -(-(I.lp.nx * I.lp.ny * I.lp.nz) // 256)
+        elif domain == _native.BATCH_DOMAIN_NODES:
+            nb = -(-(I.mp.gnx * I.mp.gny * I.mp.gnz) // 256)
+        else:
+            nb = -(-I.mp.n_particles // _native.PARTICLE_BLOCK)
+        rows += [(k, b) for b in range(nb)]
+    return rows
+
+
+def test_block_map_helper_matches_python(emu, tmp_path):
     import ctypes
 
     from mpm_lbm import _native
 
     ws = _mixed(tmp_path)
     batch = BatchedFSI(ws, out_dir=str(tmp_path / "batch2"), native=emu, device="cpu")
-    out = (ctypes.c_int32 * (len(ws) + 1))()
-    mask = 1 << _native.BATCH_MODE_MOVING_BOUNDARY
-    total = emu.lib.fsi_batch_block_table(ctypes.addressof(batch._inst_host), len(ws), _native.BATCH_DOMAIN_CELLS, mask,
-                                          ctypes.addressof(out))
-    table, nblocks = batch._table(_native.BATCH_DOMAIN_CELLS, (_native.BATCH_MODE_MOVING_BOUNDARY,))
-    assert total == nblocks and list(out) == table.tolist()
+    batch._inst_host[2].mpm_substeps = 1  # exercise the min_mpm filter
+    addr = ctypes.addressof(batch._inst_host)
+    for domain in (_native.BATCH_DOMAIN_CELLS, _native.BATCH_DOMAIN_NODES, _native.BATCH_DOMAIN_PARTICLES,
+                   _native.BATCH_DOMAIN_PLANES):
+        for modes in ((_native.BATCH_MODE_MOVING_BOUNDARY,), (_native.BATCH_MODE_PENALTY, _native.BATCH_MODE_NONE),
+                      (0, 1, 2)):
+            for min_mpm in (0, 1):
+                mask = sum(1 << m for m in modes)
+                n = emu.lib.fsi_batch_block_map(addr, len(ws), domain, mask, min_mpm, None)
+                out = np.zeros((max(n, 1), 2), dtype=np.int32)
+                emu.lib.fsi_batch_block_map(addr, len(ws), domain, mask, min_mpm, out.ctypes.data_as(ctypes.c_void_p))
+                got = [tuple(r) for r in out[:n].tolist()]
+                assert got == _expected_map(batch._inst_host, domain, modes, min_mpm), (domain, modes, min_mpm)

@@ -72,9 +72,13 @@ def test_gpu_driver_run_writes_outputs(hip, tmp_path):
 
 def test_gpu_graph_replay_matches_eager(hip, tmp_path):
     """The captured FSI-step graph against eager launches, every step, with
-    eager diagnostics work between replays."""
+    eager diagnostics work between replays: both runs meet the parity bar
+    against the oracle, their integer counters agree exactly and their lattice
+    fields agree to 1e-5 (the particle scatter sums with float atomics, so the
+    two runs are not bit-identical)."""
     from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
-    from tests.parity import rel_max_err
+    from oracle.oracle import OracleFSI
+    from tests.parity import field_errors, rel_max_err
 
     w = workloads.moving_boundary_flap(n=32, steps=5, substeps=20, flap=(2, 16, 12), out_dir=str(tmp_path))
     eager = FSIDriver3D(w.config, str(tmp_path / "eager"), particle_cloud=w.cloud, use_graph=False,
@@ -83,19 +87,24 @@ def test_gpu_graph_replay_matches_eager(hip, tmp_path):
                         write_geometry_file=False)
     eager.initialize()
     graph.initialize()
+    flags = eager.lbm.solid.to_numpy()
+    o = OracleFSI(w.config, flags, w.cloud, w.material, dtype=np.float32)
+    o.fp64 = OracleFSI(w.config, flags, w.cloud, w.material, dtype=np.float64)
     for step in range(5):
         eager.step_once()
         graph.step_once()
+        o.step_once()
+        o.fp64.step_once()
         assert (graph._fsi_graph is not None) == (step >= 1)
         re, rg = eager.collect_diagnostics(step), graph.collect_diagnostics(step)
         for key in ("bb_link_count", "active_cell_count", "active_reaction_particle_count", "total_mpm_substeps"):
             assert re[key] == rg[key], (step, key)
-        errs = {
+        between = {
             "F": rel_max_err(graph.lbm.F.to_numpy(), eager.lbm.F.to_numpy()),
             "hydro": rel_max_err(graph.lbm.hydro_force.to_numpy(), eager.lbm.hydro_force.to_numpy()),
-            "x": rel_max_err(graph.solid.x.to_numpy(), eager.solid.x.to_numpy()),
-            "v": rel_max_err(graph.solid.v.to_numpy(), eager.solid.v.to_numpy()),
         }
-        print("graph vs eager", step, errs, flush=True)
-        assert all(v <= 1e-6 for v in errs.values()), (step, errs)
+        print("graph vs eager", step, between, flush=True)
+        assert all(v <= 1e-5 for v in between.values()), (step, between)
+        for label, d in (("eager", eager), ("graph", graph)):
+            check_history([field_errors(d, o)], f"{label} step {step}", TOL)
     assert graph.total_lbm_substeps == eager.total_lbm_substeps == 5 * 20
