@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "lbm_substep_64cube_traffic.json"))
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched sweep (0 = skip)")
     ap.add_argument("--batch-exchanges", type=int, default=3, help="timed batched exchanges")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the served form (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of served requests, per s")
     return ap.parse_args()
 
 
@@ -167,6 +169,40 @@ def batched_form(torch, count, exchanges):
     }
 
 
+def served_form(torch, rate, seconds):
+    """Continuous batching of a Poisson request stream: lattices from
+    {32,48,64,96,128}^3, modes 50/50, U(50,500) LBM steps per request rounded
+    up to whole 120-substep exchanges, 200 GB admission budget; latency =
+    completion - arrival, through admission, build and every exchange."""
+    from mpm_lbm.serve import FSIServer, latency_summary, poisson_requests
+
+    reqs = poisson_requests(rate=rate, duration=seconds, seed=5, out_dir="/tmp/fsi_bench_serve")
+    offered = sum(r.cells * 120 * r.exchanges for r in reqs) / seconds / 1e6
+    server = FSIServer(budget_bytes=200e9, out_dir="/tmp/fsi_bench_serve/run")
+    t0 = time.perf_counter()
+    server.run(reqs, sync=torch.cuda.synchronize)
+    wall = time.perf_counter() - t0
+    done = [r for r in reqs if r.done is not None]
+    sustained = sum(r.cells * 120 * r.exchanges for r in done) / wall / 1e6
+    lat = latency_summary(reqs)
+    log(f"[bench] served: {len(done)}/{len(reqs)} requests in {wall:.1f} s, offered {offered:.0f} MLUPS, "
+        f"sustained {sustained:.0f} MLUPS, p50 {lat.get('p50_s', 0):.3f} s, p99 {lat.get('p99_s', 0):.3f} s")
+    return {
+        "requests": len(reqs),
+        "arrival_rate_per_s": rate,
+        "arrival_window_s": seconds,
+        "load_definition": "request = U(50,500) LBM steps, rounded up to whole exchanges of 120 substeps",
+        "offered_mlups": round(offered, 1),
+        "sustained_mlups": round(sustained, 1),
+        "makespan_s": round(wall, 3),
+        "p50_latency_s": round(lat.get("p50_s", float("nan")), 4),
+        "p99_latency_s": round(lat.get("p99_s", float("nan")), 4),
+        "max_latency_s": round(lat.get("max_s", float("nan")), 4),
+        "exchanges": server.exchanges_run,
+        "regroups": server.regroups,
+    }
+
+
 def cpu_baseline(workload, steps):
     import numpy as np
 
@@ -245,6 +281,10 @@ def main():
     if args.batch > 0:
         batched = batched_form(torch, args.batch, args.batch_exchanges)
 
+    served = None
+    if args.serve_seconds > 0:
+        served = served_form(torch, args.serve_rate, args.serve_seconds)
+
     cpu = None
     if args.cpu_steps > 0:
         log("[bench] CPU baseline ...")
@@ -277,6 +317,7 @@ def main():
             "copy_ceiling_gbs": round(copy_gbs, 1),
             "substep_frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
             "batched": batched,
+            "served": served,
         },
         "roofline": {
             "bound": "hbm",

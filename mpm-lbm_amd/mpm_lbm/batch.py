@@ -30,20 +30,34 @@ _MODE = {"none": _native.BATCH_MODE_NONE, "penalty": _native.BATCH_MODE_PENALTY,
 
 
 class BatchedFSI:
-    def __init__(self, workloads, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True):
-        if not workloads:
-            raise ValueError("BatchedFSI needs at least one workload")
-        self.workloads = list(workloads)
-        self.drivers = []
-        for k, w in enumerate(self.workloads):
-            d = FSIDriver3D(w.config, f"{out_dir}/inst{k:03d}", particle_cloud=w.cloud, native=native, device=device,
-                            write_geometry_file=False, use_graph=False)
-            d.initialize()
-            self.drivers.append(d)
+    def __init__(self, workloads=None, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True,
+                 drivers=None, n_substeps=None):
+        """Batch `workloads` (a driver is built for each), or batch initialized
+        `drivers` that already exist (they keep their state; a serving loop
+        regroups its live instances this way).  n_substeps: LBM substeps per
+        exchange (default: the moving-boundary instances' FSI step, else 120);
+        moving-boundary instances must use it as their FSI step."""
+        if drivers is not None:
+            self.workloads = None
+            self.drivers = list(drivers)
+        else:
+            if not workloads:
+                raise ValueError("BatchedFSI needs at least one workload")
+            self.workloads = list(workloads)
+            self.drivers = []
+            for k, w in enumerate(self.workloads):
+                d = FSIDriver3D(w.config, f"{out_dir}/inst{k:03d}", particle_cloud=w.cloud, native=native,
+                                device=device, write_geometry_file=False, use_graph=False)
+                d.initialize()
+                self.drivers.append(d)
+        if not self.drivers:
+            raise ValueError("BatchedFSI needs at least one instance")
         self.native = self.drivers[0].lbm.native
         self.device = self.drivers[0].lbm.device
         mb = [d for d in self.drivers if d.config.coupling_mode == "moving_boundary"]
         subs = {d.config.lbm_substeps_per_fsi_step for d in mb}
+        if n_substeps is not None:
+            subs.add(int(n_substeps))
         if len(subs) > 1:
             raise ValueError("moving-boundary instances of one batch must share lbm_substeps_per_fsi_step")
         self.n_substeps = subs.pop() if subs else 120
