@@ -98,6 +98,7 @@ typedef struct FsiLbmState {
   float* link_corr;       /* 18*N: per-link moving-wall correction of the DIAG substep */
   double* hydro_accum;    /* 4*N, per cell: subcycle force accumulator x, y, z (fp64) and the running
                              max of its norm over the substeps */
+  double* reduce_ws;      /* FSI_LBM_REDUCE_WS_DOUBLES(N): per-workgroup records of the finalize */
   FsiLbmCounters* counters;
 } FsiLbmState;
 
@@ -175,6 +176,9 @@ typedef struct FsiMpmStats {
   double mass_weighted_v[3];
 } FsiMpmStats;
 
+/* Records of the moving-boundary finalize: 8 doubles per 256-cell workgroup. */
+#define FSI_LBM_REDUCE_WS_DOUBLES(n_cells) (8 * (((n_cells) + 255) / 256))
+
 /* Workgroup size of the per-particle kernels. */
 #define FSI_PARTICLE_BLOCK 64
 
@@ -194,6 +198,7 @@ typedef struct FsiMpmStats {
 #define FSI_BATCH_DOMAIN_NODES 1
 #define FSI_BATCH_DOMAIN_PARTICLES 2 /* workgroups of FSI_PARTICLE_BLOCK particles */
 #define FSI_BATCH_DOMAIN_PLANES 3    /* substep tiles: nx * ceil(ny*nz / 256) workgroups */
+#define FSI_BATCH_DOMAIN_INSTANCES 4 /* one workgroup per instance */
 
 typedef struct FsiBatchInstance {
   FsiLbmParams lp;
@@ -236,12 +241,13 @@ typedef struct FsiBatchInstance {
 #define FSI_BATCH_OP_LBM_SUBSTEP_MB 17
 #define FSI_BATCH_OP_LBM_SUBSTEP_PENALTY 18
 #define FSI_BATCH_OP_LBM_SUBSTEP_NONE 19
+#define FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD 20 /* after MB_FINALIZE: fold its records into the counters */
 
 /* Domain of each operation (the map passed to fsi_batch_launch must be built
  * for it): LBM_SUBSTEP, LBM_SUBSTEP_MB / _PENALTY / _NONE -> PLANES; REBUILD, MOMENTS, MB_BEGIN, MB_FINALIZE,
  * PROJ_CLEAR, PROJ_NORMALIZE, PENALTY -> CELLS; MPM_CLEAR, MPM_GRID_UPDATE,
  * MPM_GRID_UPDATE_REACT -> NODES; PROJ_PARTICLES, MPM_P2G, MPM_REACTION,
- * MPM_P2G_REACT, MPM_G2P -> PARTICLES. */
+ * MPM_P2G_REACT, MPM_G2P -> PARTICLES; LBM_MB_FINALIZE_FOLD -> INSTANCES. */
 
 /* Block map for one domain from the host copy of the descriptors: instances
  * that are active, whose mode bit is in `mode_mask` (bit m = 1 <<

@@ -1583,6 +1583,7 @@ FSI_HD void mpm_stats_particle(const FsiMpmParams& mp, const FsiMpmState& ms, lo
 FSI_HD int batch_block_size(int domain) { return domain == FSI_BATCH_DOMAIN_PARTICLES ? FSI_PARTICLE_BLOCK : 256; }
 
 FSI_HD long long batch_domain_blocks(const FsiBatchInstance& I, int domain) {
+  if (domain == FSI_BATCH_DOMAIN_INSTANCES) return 1;
   if (domain == FSI_BATCH_DOMAIN_PLANES)  // one i-plane per run of tiles, as the single-instance substep
     return (long long)I.lp.nx * (((long long)I.lp.ny * I.lp.nz + 255) / 256);
   long long n = domain == FSI_BATCH_DOMAIN_CELLS ? (long long)I.lp.nx * I.lp.ny * I.lp.nz

@@ -240,7 +240,7 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
 int fsi_batch_block_map(const FsiBatchInstance* inst, int n_inst, int domain, int mode_mask, int min_mpm,
                         int32_t* map) {
   if (!inst || n_inst < 0) return fail(FSI_E_INVALID, "null batch arguments");
-  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_PLANES) return fail(FSI_E_INVALID, "unknown batch domain");
+  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_INSTANCES) return fail(FSI_E_INVALID, "unknown batch domain");
   long long total = 0;
   for (int i = 0; i < n_inst; ++i) {
     if (!inst[i].active || !(mode_mask & (1 << inst[i].mode)) || inst[i].mpm_substeps <= min_mpm) continue;
@@ -344,6 +344,8 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
         k->mb_sample_count = I.n_substeps;
         break;
       }
+      case FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD:  // the serial finalize above sets the counters directly
+        break;
       case FSI_BATCH_OP_PROJ_CLEAR:
         for (long long c = 0; c < cells; ++c) {
           I.ls.solid_phi[c] = 0.0f;

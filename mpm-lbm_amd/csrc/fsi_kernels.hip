@@ -343,13 +343,67 @@ __global__ void __launch_bounds__(TPB) k_lbm_mb_begin(FsiLbmParams p, FsiLbmStat
   if (c < (long long)p.nx * p.ny * p.nz) lbm_mb_begin_cell(p, st, c);
 }
 
-__global__ void __launch_bounds__(TPB) k_lbm_finalize_clear(FsiLbmState st) {
+// The finalize reductions, two-pass: every workgroup leaves one record of
+// FIN_REC doubles in st.reduce_ws (links, max correction, max mean norm, max
+// running norm, net force x/y/z); one workgroup folds them in workgroup order
+// into the counters.  Deterministic, and free of the same-address atomics that
+// a thousand workgroups would otherwise serialise on.
+constexpr int FIN_REC = 8;
+
+__device__ __forceinline__ void finalize_block_record(const FinalizeOut& o, double* rec, double* sd, float* sf,
+                                                      int* si) {
+  const int links = block_sum_i(o.links, si);
+  const float mc = block_max_f(o.max_corr, sf);
+  const float mn = block_max_f(o.mean_norm, sf);
+  const float rm = block_max_f(o.runmax, sf);
+  const double hx = block_reduce(o.hx, RED_SUM, sd);
+  const double hy = block_reduce(o.hy, RED_SUM, sd);
+  const double hz = block_reduce(o.hz, RED_SUM, sd);
   if (threadIdx.x == 0) {
-    st.counters->bb_link_count = 0;
-    st.counters->bb_max_correction_bits = 0;
-    st.counters->bb_net_solid_force[0] = 0.0;
-    st.counters->bb_net_solid_force[1] = 0.0;
-    st.counters->bb_net_solid_force[2] = 0.0;
+    rec[0] = (double)links;
+    rec[1] = (double)mc;
+    rec[2] = (double)mn;
+    rec[3] = (double)rm;
+    rec[4] = hx;
+    rec[5] = hy;
+    rec[6] = hz;
+    rec[7] = 0.0;
+  }
+}
+
+// fold of n_rec finalize records into the counters, by one whole workgroup
+__device__ __forceinline__ void finalize_fold(const double* ws, int n_rec, FsiLbmCounters* k, int subcycled,
+                                              int n_samples, double* sd) {
+  double a[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int r = threadIdx.x; r < n_rec; r += (int)blockDim.x) {
+    const double* x = ws + (size_t)r * FIN_REC;
+    a[0] += x[0];
+    a[1] = x[1] > a[1] ? x[1] : a[1];
+    a[2] = x[2] > a[2] ? x[2] : a[2];
+    a[3] = x[3] > a[3] ? x[3] : a[3];
+    a[4] += x[4];
+    a[5] += x[5];
+    a[6] += x[6];
+  }
+  const double links = block_reduce(a[0], RED_SUM, sd);
+  const double mc = block_reduce(a[1], RED_MAX, sd);
+  const double mn = block_reduce(a[2], RED_MAX, sd);
+  const double rm = block_reduce(a[3], RED_MAX, sd);
+  const double hx = block_reduce(a[4], RED_SUM, sd);
+  const double hy = block_reduce(a[5], RED_SUM, sd);
+  const double hz = block_reduce(a[6], RED_SUM, sd);
+  if (threadIdx.x == 0) {
+    k->bb_link_count = (int32_t)links;
+    k->bb_max_correction_bits = __float_as_uint((float)mc);
+    if (subcycled) {
+      const uint32_t mb = __float_as_uint((float)mn), rb = __float_as_uint((float)rm);
+      if (mb > k->mean_norm_max_bits) k->mean_norm_max_bits = mb;
+      if (rb > k->accum_norm_max_bits) k->accum_norm_max_bits = rb;
+      k->mb_sample_count = n_samples;
+    }
+    k->bb_net_solid_force[0] = hx;
+    k->bb_net_solid_force[1] = hy;
+    k->bb_net_solid_force[2] = hz;
   }
 }
 
@@ -367,26 +421,12 @@ __global__ void __launch_bounds__(TPB) k_lbm_finalize(FsiLbmParams p, FsiLbmStat
   o.runmax = 0.0f;
   o.hx = o.hy = o.hz = 0.0;
   if (c < (long long)p.nx * p.ny * p.nz) o = lbm_finalize_cell<Pop>(p, st, cur, parity, n_samples, subcycled, c);
-  const int links = block_sum_i(o.links, si);
-  const float mc = block_max_f(o.max_corr, sf);
-  const float mn = block_max_f(o.mean_norm, sf);
-  const float rm = block_max_f(o.runmax, sf);
-  const double hx = block_reduce(o.hx, RED_SUM, sd);
-  const double hy = block_reduce(o.hy, RED_SUM, sd);
-  const double hz = block_reduce(o.hz, RED_SUM, sd);
-  if (threadIdx.x == 0) {
-    FsiLbmCounters* k = st.counters;
-    if (links) atomicAdd(&k->bb_link_count, links);
-    atomicMax(&k->bb_max_correction_bits, __float_as_uint(mc));
-    if (subcycled) {
-      atomicMax(&k->mean_norm_max_bits, __float_as_uint(mn));
-      atomicMax(&k->accum_norm_max_bits, __float_as_uint(rm));
-    }
-    if (hx != 0.0) atomicAdd(&k->bb_net_solid_force[0], hx);
-    if (hy != 0.0) atomicAdd(&k->bb_net_solid_force[1], hy);
-    if (hz != 0.0) atomicAdd(&k->bb_net_solid_force[2], hz);
-    if (blockIdx.x == 0 && subcycled) k->mb_sample_count = n_samples;
-  }
+  finalize_block_record(o, st.reduce_ws + (size_t)blockIdx.x * FIN_REC, sd, sf, si);
+}
+
+__global__ void __launch_bounds__(TPB) k_lbm_finalize_fold(FsiLbmState st, int n_rec, int subcycled, int n_samples) {
+  __shared__ double sd[TPB / 64];
+  finalize_fold(st.reduce_ws, n_rec, st.counters, subcycled, n_samples, sd);
 }
 
 __global__ void __launch_bounds__(TPB) k_lbm_stats_partial(FsiLbmParams p, FsiLbmState st, FsiLbmStats* part) {
@@ -835,24 +875,15 @@ __global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* inst,
   if (in_)
     FSI_BY_STORAGE(I.lp.storage, o = lbm_finalize_cell<POP>(I.lp, I.ls, batch_cur(I, I.n_substeps), last & 1,
                                                             I.n_substeps, 1, e_));
-  const int links = block_sum_i(o.links, si);
-  const float mc = block_max_f(o.max_corr, sf);
-  const float mn = block_max_f(o.mean_norm, sf);
-  const float rm = block_max_f(o.runmax, sf);
-  const double hx = block_reduce(o.hx, RED_SUM, sd);
-  const double hy = block_reduce(o.hy, RED_SUM, sd);
-  const double hz = block_reduce(o.hz, RED_SUM, sd);
-  if (threadIdx.x == 0) {
-    FsiLbmCounters* k = I.ls.counters;
-    if (links) atomicAdd(&k->bb_link_count, links);
-    if (mc > 0.0f) atomicMax(&k->bb_max_correction_bits, __float_as_uint(mc));
-    if (mn > 0.0f) atomicMax(&k->mean_norm_max_bits, __float_as_uint(mn));
-    if (rm > 0.0f) atomicMax(&k->accum_norm_max_bits, __float_as_uint(rm));
-    if (hx != 0.0) atomicAdd(&k->bb_net_solid_force[0], hx);
-    if (hy != 0.0) atomicAdd(&k->bb_net_solid_force[1], hy);
-    if (hz != 0.0) atomicAdd(&k->bb_net_solid_force[2], hz);
-    if (l_ == 0) k->mb_sample_count = I.n_substeps;
-  }
+  finalize_block_record(o, I.ls.reduce_ws + (size_t)l_ * FIN_REC, sd, sf, si);
+}
+
+// one workgroup per instance (INSTANCES domain): fold the finalize records
+__global__ void __launch_bounds__(TPB) kb_finalize_fold(const FsiBatchInstance* inst, const int32_t* bmap) {
+  __shared__ double sd[TPB / 64];
+  const FsiBatchInstance& I = inst[bmap[2 * blockIdx.x]];
+  const int n_rec = (int)batch_domain_blocks(I, FSI_BATCH_DOMAIN_CELLS);
+  finalize_fold(I.ls.reduce_ws, n_rec, I.ls.counters, 1, I.n_substeps, sd);
 }
 
 __global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* inst, int n_inst, int which) {
@@ -1134,14 +1165,15 @@ int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int cur, i
   if (rc) return rc;
   if ((cur != 0 && cur != 1) || (parity != 0 && parity != 1))
     return fail(FSI_E_INVALID, "cur and parity must be 0 or 1");
-  if (!st->link_c[0] || !st->link_c[1] || !st->link_corr || !st->hydro_accum || !st->hydro_force || !st->counters)
-    return fail(FSI_E_INVALID, "mb_finalize needs link buffers, accumulator, hydro_force, counters");
+  if (!st->link_c[0] || !st->link_c[1] || !st->link_corr || !st->hydro_accum || !st->hydro_force || !st->counters ||
+      !st->reduce_ws)
+    return fail(FSI_E_INVALID, "mb_finalize needs link buffers, accumulator, hydro_force, counters, reduce_ws");
   if (n_samples < 1) return fail(FSI_E_INVALID, "n_samples must be positive");
   const long long N = (long long)p->nx * p->ny * p->nz;
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(k_lbm_finalize_clear, dim3(1), dim3(64), 0, s, *st);
   FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL(k_lbm_finalize<POP>, dim3(blocks_for(N)), dim3(TPB), 0, s, *p, *st,
                                                 cur, parity, n_samples, subcycled));
+  hipLaunchKernelGGL(k_lbm_finalize_fold, dim3(1), dim3(TPB), 0, s, *st, (int)blocks_for(N), subcycled, n_samples);
   return check_launch("k_lbm_finalize");
 }
 
@@ -1295,7 +1327,7 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
 int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask, int min_mpm,
                         int32_t* map_host) {
   if (!inst_host || n_inst < 0) return fail(FSI_E_INVALID, "null batch arguments");
-  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_PLANES)
+  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_INSTANCES)
     return fail(FSI_E_INVALID, "unknown batch domain");
   long long total = 0;
   for (int i = 0; i < n_inst; ++i) {
@@ -1343,6 +1375,7 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const
     case FSI_BATCH_OP_LBM_MOMENTS: hipLaunchKernelGGL(kb_moments, g, b, 0, s, inst_dev, m, t); break;
     case FSI_BATCH_OP_LBM_MB_BEGIN: hipLaunchKernelGGL(kb_mb_begin, g, b, 0, s, inst_dev, m); break;
     case FSI_BATCH_OP_LBM_MB_FINALIZE: hipLaunchKernelGGL(kb_finalize, g, b, 0, s, inst_dev, m); break;
+    case FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD: hipLaunchKernelGGL(kb_finalize_fold, g, b, 0, s, inst_dev, m); break;
     case FSI_BATCH_OP_PROJ_CLEAR: hipLaunchKernelGGL(kb_proj_clear, g, b, 0, s, inst_dev, m); break;
     case FSI_BATCH_OP_PROJ_PARTICLES: hipLaunchKernelGGL(kb_proj_particles, g, bp, 0, s, inst_dev, m); break;
     case FSI_BATCH_OP_PROJ_NORMALIZE: hipLaunchKernelGGL(kb_proj_normalize, g, b, 0, s, inst_dev, m); break;

@@ -76,7 +76,7 @@ class LbmState(ctypes.Structure):
         ("solid_phi", vp), ("solid_mass", vp), ("solid_vel", vp),
         ("cell_force", vp), ("hydro_force", vp),
         ("link_c", vp * 2), ("link_corr", vp),
-        ("hydro_accum", vp),
+        ("hydro_accum", vp), ("reduce_ws", vp),
         ("counters", vp),
     ]
 
@@ -162,11 +162,12 @@ BATCH_DOMAIN_CELLS = 0
 BATCH_DOMAIN_NODES = 1
 BATCH_DOMAIN_PARTICLES = 2
 BATCH_DOMAIN_PLANES = 3
+BATCH_DOMAIN_INSTANCES = 4
 (BATCH_OP_LBM_SUBSTEP, BATCH_OP_LBM_REBUILD, BATCH_OP_LBM_MOMENTS, BATCH_OP_LBM_MB_BEGIN, BATCH_OP_LBM_MB_FINALIZE,
  BATCH_OP_PROJ_CLEAR, BATCH_OP_PROJ_PARTICLES, BATCH_OP_PROJ_NORMALIZE, BATCH_OP_PENALTY, BATCH_OP_MPM_CLEAR,
  BATCH_OP_MPM_P2G, BATCH_OP_MPM_REACTION, BATCH_OP_MPM_GRID_UPDATE, BATCH_OP_MPM_G2P,
  BATCH_OP_COUNTERS_CLEAR, BATCH_OP_MPM_P2G_REACT, BATCH_OP_MPM_GRID_UPDATE_REACT, BATCH_OP_LBM_SUBSTEP_MB,
- BATCH_OP_LBM_SUBSTEP_PENALTY, BATCH_OP_LBM_SUBSTEP_NONE) = range(20)
+ BATCH_OP_LBM_SUBSTEP_PENALTY, BATCH_OP_LBM_SUBSTEP_NONE, BATCH_OP_LBM_MB_FINALIZE_FOLD) = range(21)
 
 # domain of each batched operation (the block map a launch takes is built for it)
 BATCH_OP_DOMAIN = {
@@ -181,6 +182,7 @@ BATCH_OP_DOMAIN = {
     BATCH_OP_PROJ_PARTICLES: BATCH_DOMAIN_PARTICLES, BATCH_OP_MPM_P2G: BATCH_DOMAIN_PARTICLES,
     BATCH_OP_MPM_REACTION: BATCH_DOMAIN_PARTICLES, BATCH_OP_MPM_P2G_REACT: BATCH_DOMAIN_PARTICLES,
     BATCH_OP_MPM_G2P: BATCH_DOMAIN_PARTICLES,
+    BATCH_OP_LBM_MB_FINALIZE_FOLD: BATCH_DOMAIN_INSTANCES,
 }
 
 

@@ -172,8 +172,8 @@ class BatchedFSI:
                 self._launch(_native.BATCH_OP_LBM_MOMENTS, ONE, t=t)
                 self._mpm_substeps(ONE)
         if has_mb:
-            self._clear_counters(3)
             self._launch(_native.BATCH_OP_LBM_MB_FINALIZE, MB)
+            self._launch(_native.BATCH_OP_LBM_MB_FINALIZE_FOLD, MB)
             self._launch(_native.BATCH_OP_LBM_MOMENTS, MB, t=self.n_substeps - 1)
             self._mpm_substeps(MB)
 

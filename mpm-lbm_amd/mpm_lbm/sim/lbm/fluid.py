@@ -143,6 +143,7 @@ class LBMFluid3D:
         self._link = None  # moving-wall link buffers, allocated on first use
         self._link_corr = None
         self._accum = None  # (N, 4) fp64: accumulated x, y, z and running max of the norm
+        self._reduce_ws = None  # finalize records, 8 doubles per 256-cell workgroup
         self._counters = struct_buffer(_native.LbmCounters, dev)
         self._stats_ws = struct_buffer(_native.LbmStats, dev, _native.REDUCE_BLOCKS)
         self._stats_out = struct_buffer(_native.LbmStats, dev)
@@ -200,6 +201,7 @@ class LBMFluid3D:
             self._link = [torch.zeros((18, self.N), dtype=torch.float32, device=dev) for _ in range(2)]
             self._link_corr = torch.zeros((18, self.N), dtype=torch.float32, device=dev)
             self._accum = torch.zeros((self.N, 4), dtype=torch.float64, device=dev)
+            self._reduce_ws = torch.zeros((8 * ((self.N + 255) // 256),), dtype=torch.float64, device=dev)
             self._refresh_state()
 
     def _refresh_state(self):
@@ -213,7 +215,7 @@ class LBMFluid3D:
         s.cell_force, s.hydro_force = p(self._cell_force), p(self._hydro)
         if self._link is not None:
             s.link_c[0], s.link_c[1] = p(self._link[0]), p(self._link[1])
-            s.link_corr, s.hydro_accum = p(self._link_corr), p(self._accum)
+            s.link_corr, s.hydro_accum, s.reduce_ws = p(self._link_corr), p(self._accum), p(self._reduce_ws)
         s.counters = p(self._counters)
 
     def _build_params(self):
