@@ -596,6 +596,17 @@ FSI_HD void wall_accumulate(double* A, uint32_t y, float fx, float fy, float fz)
   if (n > a[3]) a[3] = n;
 }
 
+// The same update from an accumulator record `a` already loaded into registers.
+FSI_HD void wall_accumulate_loaded(double* A, uint32_t y, const double* a, float fx, float fy, float fz) {
+  double* o = A + 4 * (size_t)y;
+  const double ax = a[0] + (double)fx, ay = a[1] + (double)fy, az = a[2] + (double)fz;
+  const double n = (double)norm3((float)ax, (float)ay, (float)az);
+  o[0] = ax;
+  o[1] = ay;
+  o[2] = az;
+  o[3] = n > a[3] ? n : a[3];
+}
+
 // Previous-step velocity of cell (i,j,k) as the reference's `v` field holds it
 // at Boundary_condition time: stored moments, or moments of F_src; 0 on solid.
 template <typename Pop>
@@ -648,6 +659,11 @@ FSI_HD Nbr nbr_of(const FsiLbmParams& p, int i, int j, int k) {
   return n;
 }
 
+template <typename Pop>
+FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const Pop& Fs, int cur, int parity,
+                           uint32_t flags, int i, int j, int k, uint32_t N, uint32_t c, uint32_t info, const Nbr& nb,
+                           const FaceFlags& ff, const float* F);
+
 // One fused LBM substep for cell (i,j,k) (push form): collide F[cur] at the
 // cell, push the post-collision populations to the neighbours in F[1-cur],
 // bounce them back at solid neighbours (half-way, with the Ladd moving-wall
@@ -672,7 +688,10 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
   // One load phase for both kinds of cell, so that a wave holding fluid and
   // wall cells waits for memory once: fluid cells load their populations,
   // wall cells the link scalars c_s of their links into the same registers.
+  // (The wall cells' accumulator record is loaded here too, and the fluid
+  // work is issued before the wall work, so nothing waits on a second trip.)
   float F[Q];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (!solid) {
 #pragma unroll
     for (int s = 0; s < Q; ++s) F[s] = Fs.ld(N, s, c);
@@ -690,19 +709,34 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
           F[s] = 2.0f * Fs.ld(N, LR[s], x);
       }
     }
+    const double* a = st.hydro_accum + 4 * (size_t)c;
+    acc[0] = a[0];
+    acc[1] = a[1];
+    acc[2] = a[2];
+    acc[3] = a[3];
   }
-  if (solid) {
-    // wall cell: the force of its links is sum_s e_s c_s; added to the accumulator
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-#pragma unroll
-    for (int s = 1; s < Q; ++s) {
-      fx += (float)EX[s] * F[s];
-      fy += (float)EY[s] * F[s];
-      fz += (float)EZ[s] * F[s];
-    }
-    wall_accumulate(st.hydro_accum, c, fx, fy, fz);
+  if (!solid) {
+    lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, flags, i, j, k, N, c, info, nb, ff, F);
     return;
   }
+  // wall cell: the force of its links is sum_s e_s c_s; added to the accumulator
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+  for (int s = 1; s < Q; ++s) {
+    fx += (float)EX[s] * F[s];
+    fy += (float)EY[s] * F[s];
+    fz += (float)EZ[s] * F[s];
+  }
+  wall_accumulate_loaded(st.hydro_accum, c, acc, fx, fy, fz);
+}
+
+// Collide, push, bounce back and apply the x-face BCs for fluid cell c whose
+// post-stream populations F are already loaded (the rest of lbm_substep_cell).
+template <typename Pop>
+FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const Pop& Fs, int cur, int parity,
+                           uint32_t flags, int i, int j, int k, uint32_t N, uint32_t c, uint32_t info, const Nbr& nb,
+                           const FaceFlags& ff, const float* F) {
+  const bool mb = (flags & FSI_SUB_MOVING_WALL) != 0;
   const Pop Fd = pop_of<Pop>(st.F[1 - cur]);
   float gx = p.force[0], gy = p.force[1], gz = p.force[2];
   if (flags & FSI_SUB_CELL_FORCE) {
@@ -934,6 +968,20 @@ FSI_HD void bspline(float X, int& base, float& fx, float* w) {
   w[2] = 0.5f * (d * d);
 }
 
+// A particle's quadratic B-spline stencil: base node and weights per axis.
+struct Stencil {
+  int base[3];
+  float fx[3];
+  float w[3][3];
+};
+
+FSI_HD Stencil particle_stencil(const float* x, long long P, long long p, float inv) {
+  Stencil s;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline(x[d * P + p] * inv, s.base[d], s.fx[d], s.w[d]);
+  return s;
+}
+
 FSI_HD float det3(const float* A) {
   return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
          A[2] * (A[3] * A[7] - A[4] * A[6]);
@@ -1114,14 +1162,17 @@ FSI_HD GlobalSink reaction_sink(const FsiMpmState& ms, long long G) {
   return k;
 }
 
-template <typename Sink>
-FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p, const Sink& sink) {
-  const MpmGrid g = make_mgrid(mp);
+// Per-particle terms of p2g (solid.py:220-249): node momentum is
+// wt * (mv + aff * (x_node - x_p)), node mass wt * mass.  Writes Jp[p] = det F.
+struct P2gTerms {
+  float aff[9];
+  float mv[3];
+  float mass;
+};
+
+FSI_HD P2gTerms p2g_terms(const FsiMpmParams& mp, const FsiMpmState& ms, long long p) {
   const long long P = mp.n_particles;
-  int base[3];
-  float fx[3], w[3][3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * mp.inv_dx, base[d], fx[d], w[d]);
+  P2gTerms t;
   float Fm[9], Cm[9], R[9], Dv[9];
   load_mat(ms.F, P, p, Fm);
   load_mat(ms.C, P, p, Cm);
@@ -1145,16 +1196,30 @@ FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
   for (int e = 0; e < 9; ++e) Pk[e] = 2.0f * mp.mu * Dv[e] + lj * FiT[e];
   const float mass = ms.mass[p];
   const float sc = -mp.dt * ms.vol0[p];
-  float aff[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       // (P F^T)_{rc}
       const float pf = Pk[r * 3 + 0] * Fm[c * 3 + 0] + Pk[r * 3 + 1] * Fm[c * 3 + 1] + Pk[r * 3 + 2] * Fm[c * 3 + 2];
-      aff[r * 3 + c] = sc * pf * mp.inv_dx * mp.inv_dx + mass * Cm[r * 3 + c];
+      t.aff[r * 3 + c] = sc * pf * mp.inv_dx * mp.inv_dx + mass * Cm[r * 3 + c];
     }
-  const float mvx = mass * ms.v[p], mvy = mass * ms.v[P + p], mvz = mass * ms.v[2 * P + p];
+  t.mass = mass;
+  t.mv[0] = mass * ms.v[p];
+  t.mv[1] = mass * ms.v[P + p];
+  t.mv[2] = mass * ms.v[2 * P + p];
+  return t;
+}
+
+template <typename Sink>
+FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p, const Stencil& sn,
+                             const Sink& sink) {
+  const MpmGrid g = make_mgrid(mp);
+  const int* base = sn.base;
+  const float* fx = sn.fx;
+  const float(&w)[3][3] = sn.w;
+  const P2gTerms t = p2g_terms(mp, ms, p);
+  const float* aff = t.aff;
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b)
       for (int cc = 0; cc < 3; ++cc) {
@@ -1164,15 +1229,20 @@ FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
         const float dpy = ((float)b - fx[1]) * mp.dx;
         const float dpz = ((float)cc - fx[2]) * mp.dx;
         const float wt = w[0][a] * w[1][b] * w[2][cc];
-        const float mx = mvx + aff[0] * dpx + aff[1] * dpy + aff[2] * dpz;
-        const float my = mvy + aff[3] * dpx + aff[4] * dpy + aff[5] * dpz;
-        const float mz = mvz + aff[6] * dpx + aff[7] * dpy + aff[8] * dpz;
+        const float mx = t.mv[0] + aff[0] * dpx + aff[1] * dpy + aff[2] * dpz;
+        const float my = t.mv[1] + aff[3] * dpx + aff[4] * dpy + aff[5] * dpz;
+        const float mz = t.mv[2] + aff[6] * dpx + aff[7] * dpy + aff[8] * dpz;
         const long long n = ((long long)I * g.ny + Jn) * g.nz + K;
         sink.add(0, n, I, Jn, K, wt * mx);
         sink.add(1, n, I, Jn, K, wt * my);
         sink.add(2, n, I, Jn, K, wt * mz);
-        sink.add(3, n, I, Jn, K, wt * mass);
+        sink.add(3, n, I, Jn, K, wt * t.mass);
       }
+}
+
+template <typename Sink>
+FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p, const Sink& sink) {
+  mpm_p2g_particle(mp, ms, p, particle_stencil(ms.x, mp.n_particles, p, mp.inv_dx), sink);
 }
 
 // grid_update + apply_boundary_at_node (solid.py:252-259, :97-111).
@@ -1316,13 +1386,13 @@ FSI_HD GlobalSink proj_sink(const FsiLbmState& ls, long long N) {
 
 template <typename Sink>
 FSI_HD void project_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
-                             const FsiMpmState& ms, const FsiCouplingParams& cp, long long p, const Sink& sink) {
+                             const FsiMpmState& ms, const FsiCouplingParams& cp, long long p, const Stencil& sn,
+                             const Sink& sink) {
   const Grid g = make_grid(lp);
   const long long P = mp.n_particles;
-  int base[3];
-  float fx[3], w[3][3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * cp.inv_dx_norm, base[d], fx[d], w[d]);
+  const int* base = sn.base;
+  const float* fx = sn.fx;
+  const float(&w)[3][3] = sn.w;
   float J = ms.Jp[p];
   J = J > 0.0f ? J : 0.0f;
   const float vol = ms.vol0[p] * J;
@@ -1345,6 +1415,12 @@ FSI_HD void project_particle(const FsiLbmParams& lp, const FsiLbmState& ls, cons
         sink.add(3, c, I, Jn, K, mc * vy);
         sink.add(4, c, I, Jn, K, mc * vz);
       }
+}
+
+template <typename Sink>
+FSI_HD void project_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                             const FsiMpmState& ms, const FsiCouplingParams& cp, long long p, const Sink& sink) {
+  project_particle(lp, ls, mp, ms, cp, p, particle_stencil(ms.x, mp.n_particles, p, cp.inv_dx_norm), sink);
 }
 
 struct ProjOut {
@@ -1410,30 +1486,28 @@ struct ReactOut {
   float f[3];
 };
 
-// Reaction transfer for particle p: mode 1 = penalty (penalty.py:90-129),
-// mode 2 = moving boundary (moving_boundary.py:58-111).
-template <typename Sink>
-FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
-                                  const FsiMpmState& ms, const FsiCouplingParams& cp, int mode,
-                                  long long p, const Sink& sink) {
+// Reaction force on particle p: mode 1 = penalty (penalty.py:90-129), mode 2 =
+// moving boundary (moving_boundary.py:58-111).  Gathers hydro_force (and, for
+// mode 2, solid_phi) at the stencil's lattice cells; o.f is the particle's
+// reaction force, zero for a mode-2 particle outside the solid.  CHECKED =
+// false: the caller knows every stencil cell is inside the lattice.
+template <bool CHECKED = true>
+FSI_HD ReactOut reaction_force(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmState& ms,
+                               const FsiCouplingParams& cp, int mode, long long p, const Stencil& sn) {
   const Grid g = make_grid(lp);
-  const MpmGrid mg = make_mgrid(mp);
-  const long long P = mp.n_particles;
   ReactOut o;
   o.active = 0;
   o.particle_norm = 0.0f;
   o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
-  int base[3];
-  float fx[3], w[3][3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * cp.inv_dx_norm, base[d], fx[d], w[d]);
+  const int* base = sn.base;
+  const float(&w)[3][3] = sn.w;
   float hx = 0.0f, hy = 0.0f, hz = 0.0f, sphi = 0.0f;
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b)
       for (int cc = 0; cc < 3; ++cc) {
         const int I = base[0] + a, Jn = base[1] + b, K = base[2] + cc;
-        if (!inside3(I, Jn, K, g.nx, g.ny, g.nz)) continue;
+        if (CHECKED && !inside3(I, Jn, K, g.nx, g.ny, g.nz)) continue;
         const float wt = w[0][a] * w[1][b] * w[2][cc];
         const long long c = cell_of(g, I, Jn, K);
         hx += wt * ls.hydro_force[c];
@@ -1461,6 +1535,20 @@ FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
   o.f[0] = pf[0];
   o.f[1] = pf[1];
   o.f[2] = pf[2];
+  return o;
+}
+
+// Reaction transfer for particle p: the force, then its scatter to the MPM
+// grid nodes of the stencil (o.max_contrib = the largest node contribution).
+template <typename Sink>
+FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                                  const FsiMpmState& ms, const FsiCouplingParams& cp, int mode,
+                                  long long p, const Stencil& sn, const Sink& sink) {
+  const MpmGrid mg = make_mgrid(mp);
+  ReactOut o = reaction_force(lp, ls, ms, cp, mode, p, sn);  // zero force: adds of zero
+  const int* base = sn.base;
+  const float(&w)[3][3] = sn.w;
+  const float pf[3] = {o.f[0], o.f[1], o.f[2]};
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b)
       for (int cc = 0; cc < 3; ++cc) {
@@ -1478,6 +1566,14 @@ FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
   return o;
 }
 
+
+template <typename Sink>
+FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                                  const FsiMpmState& ms, const FsiCouplingParams& cp, int mode,
+                                  long long p, const Sink& sink) {
+  return reaction_particle(lp, ls, mp, ms, cp, mode, p, particle_stencil(ms.x, mp.n_particles, p, cp.inv_dx_norm),
+                           sink);
+}
 
 // ---------------------------------------------------------------------------
 // Diagnostics reductions.  The stats structs are arrays of doubles; each field

@@ -154,13 +154,11 @@ struct TileBox {
   int lx, ly, lz, ex, ey, ez, V;
 };
 
-// Box of the block's particles' stencils: base nodes int(x * inv - 0.5), as the
-// bodies compute them, plus 2.  Should a body's base differ from this one (a
-// different contraction of the expression), LdsSink sends that node to the
-// global grid, so the box never has to be exact.  V = 0: no valid particle or
-// the box does not fit.  Every thread of the block must call this.
-__device__ __forceinline__ TileBox block_tile_box(const float* x, long long P, long long q, bool valid, float inv, int* sbox,
-                                                  int cap = TILE_NODES) {
+// Box of the block's particles' stencils, [min base, max base + 2] per axis,
+// from the base nodes the bodies then use: every node a valid particle of the
+// block scatters to lies in it.  V = 0: no valid particle, or the box holds
+// more than `cap` nodes.  Every thread of the block must call this.
+__device__ __forceinline__ TileBox block_tile_box(const int* base, bool valid, int* sbox, int cap) {
   if (threadIdx.x == 0) {
     sbox[0] = sbox[1] = sbox[2] = 0x7FFFFFFF;
     sbox[3] = sbox[4] = sbox[5] = -0x7FFFFFFF;
@@ -169,12 +167,8 @@ __device__ __forceinline__ TileBox block_tile_box(const float* x, long long P, l
   if (valid) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const float X = x[d * P + q] * inv;
-      if (X > -1.0e6f && X < 1.0e6f) {
-        const int b = (int)(X - 0.5f);
-        atomicMin(&sbox[d], b);
-        atomicMax(&sbox[3 + d], b);
-      }
+      atomicMin(&sbox[d], base[d]);
+      atomicMax(&sbox[3 + d], base[d]);
     }
   }
   __syncthreads();
@@ -198,6 +192,43 @@ __device__ __forceinline__ TileBox block_tile_box(const float* x, long long P, l
   return t;
 }
 
+// LDS scatter target for particles whose stencils the box was built from:
+// every node is inside, so there is no test and no fallback path.
+//
+// The tile is kept in TILE_COPIES copies, lane parity choosing the copy, and
+// the flush sums them.  In a lattice-ordered cloud (8 particles per cell,
+// sub-cell index = 4 ox + 2 oy + oz) lane parity is oz, and the only two
+// lanes of a wave that hit the same node in one instruction are an oz = 1
+// particle of cell k and an oz = 0 particle of cell k + 1: in separate copies
+// no two lanes of a copy share a stencil base, which the fused P2G scatter
+// needs for its non-atomic read-modify-write (p2g_react_particle).  Any other
+// particle order stays correct (the adds are atomic then), only slower.
+constexpr int TILE_COPIES = 2;
+
+struct LdsTileSink {
+  float* tile;  // this lane's copy
+  TileBox b;
+  __device__ __forceinline__ void add(int c, long long, int I, int J, int K, float v) const {
+    __hip_atomic_fetch_add(tile + c * b.V + ((I - b.lx) * b.ey + (J - b.ly)) * b.ez + (K - b.lz), v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+};
+
+// this lane's copy of a tile of nch channels
+__device__ __forceinline__ float* lane_copy(float* tile, const TileBox& b, int nch) {
+  return tile + (threadIdx.x & (TILE_COPIES - 1)) * nch * b.V;
+}
+
+__device__ __forceinline__ Stencil stencil_if(bool valid, const float* x, long long P, long long q, float inv) {
+  Stencil sn;
+  if (valid) {
+    sn = particle_stencil(x, P, q, inv);
+  } else {
+    sn.base[0] = sn.base[1] = sn.base[2] = 0;
+  }
+  return sn;
+}
+
 // LDS scatter target; a node outside the box goes to the global grid.
 struct LdsSink {
   float* tile;
@@ -215,33 +246,10 @@ struct LdsSink {
   }
 };
 
-template <int NCH>
-__device__ __forceinline__ void tile_clear(float* tile, const TileBox& b) {
-  for (int i = threadIdx.x; i < NCH * b.V; i += (int)blockDim.x) tile[i] = 0.0f;
-  __syncthreads();
-}
-
-// Add the tile to the grid: one global atomic per touched node and channel.
-template <int NCH>
-__device__ __forceinline__ void tile_flush(const float* tile, const TileBox& b, const GlobalSink& g, int gnx, int gny, int gnz) {
-  __syncthreads();
-  const int plane = b.ey * b.ez;
-  for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
-    const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
-    const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
-    if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
-    const long long n = ((long long)I * gny + J) * gnz + K;
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      const float v = tile[c * b.V + i];
-      if (v != 0.0f) atomicAdd(g.ch[c] + n, v);
-    }
-  }
-}
-
-// Flush of a tile whose channels go to the grid arrays ch[0..nch).
-__device__ __forceinline__ void tile_flush_ch(const float* tile, const TileBox& b, float* const* ch, int nch, int gnx,
-                                              int gny, int gnz) {
+// Flush of a tile whose channels go to the grid arrays ch[0..nch): the
+// copies are summed, then one global atomic per touched node and channel.
+__device__ __forceinline__ void tile_flush_ch(const float* tile, const TileBox& b, float* const* ch, int nch,
+                                              int copies, int gnx, int gny, int gnz) {
   __syncthreads();
   const int plane = b.ey * b.ez;
   for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
@@ -250,10 +258,16 @@ __device__ __forceinline__ void tile_flush_ch(const float* tile, const TileBox& 
     if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
     const long long n = ((long long)I * gny + J) * gnz + K;
     for (int c = 0; c < nch; ++c) {
-      const float v = tile[c * b.V + i];
+      float v = tile[c * b.V + i];
+      for (int k = 1; k < copies; ++k) v += tile[(k * nch + c) * b.V + i];
       if (v != 0.0f) atomicAdd(ch[c] + n, v);
     }
   }
+}
+
+__device__ __forceinline__ void tile_clear_n(float* tile, int n) {
+  for (int i = threadIdx.x; i < n; i += (int)blockDim.x) tile[i] = 0.0f;
+  __syncthreads();
 }
 
 // Projection scatter of particle q (valid) by a whole workgroup: solid_phi,
@@ -264,17 +278,17 @@ __device__ __forceinline__ void proj_block(const FsiLbmParams& lp, const FsiLbmS
                                            const FsiMpmState& ms, const FsiCouplingParams& cp, long long q, bool valid,
                                            float* tile, int* sbox) {
   const GlobalSink g = proj_sink(ls, (long long)lp.nx * lp.ny * lp.nz);
-  const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, cp.inv_dx_norm, sbox, PROJ_TILE_NODES);
+  const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, cp.inv_dx_norm);
+  const TileBox b = block_tile_box(sn.base, valid, sbox, PROJ_TILE_NODES);
   if (b.V > 0) {
-    tile_clear<5>(tile, b);
-    LdsSink k;
-    k.tile = tile;
+    tile_clear_n(tile, TILE_COPIES * 5 * b.V);
+    LdsTileSink k;
+    k.tile = lane_copy(tile, b, 5);
     k.b = b;
-    k.g = g;
-    if (valid) project_particle(lp, ls, mp, ms, cp, q, k);
-    tile_flush_ch(tile, b, g.ch, 5, lp.nx, lp.ny, lp.nz);
+    if (valid) project_particle(lp, ls, mp, ms, cp, q, sn, k);
+    tile_flush_ch(tile, b, g.ch, 5, TILE_COPIES, lp.nx, lp.ny, lp.nz);
   } else if (valid) {
-    project_particle(lp, ls, mp, ms, cp, q, g);
+    project_particle(lp, ls, mp, ms, cp, q, sn, g);
   }
 }
 
@@ -479,7 +493,7 @@ __global__ void __launch_bounds__(TPB) k_proj_clear(FsiLbmParams p, FsiLbmState 
 
 __global__ void __launch_bounds__(PTPB) k_proj_particles(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
                                                          FsiMpmState ms, FsiCouplingParams cp) {
-  __shared__ float tile[5 * PROJ_TILE_NODES];
+  __shared__ float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
   __shared__ int sbox[6];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
   proj_block(lp, ls, mp, ms, cp, q, q < mp.n_particles, tile, sbox);
@@ -567,51 +581,52 @@ __global__ void __launch_bounds__(TPB) k_mpm_clear(FsiMpmParams mp, FsiMpmState 
 }
 
 // P2G of particle q (valid) by a whole workgroup.
-__device__ __forceinline__ void p2g_block(const FsiMpmParams& mp, const FsiMpmState& ms, long long q, bool valid, float* tile,
-                          int* sbox) {
+__device__ __forceinline__ void p2g_block(const FsiMpmParams& mp, const FsiMpmState& ms, long long q, bool valid,
+                                          float* tile, int* sbox) {
   const GlobalSink g = p2g_sink(ms, (long long)mp.gnx * mp.gny * mp.gnz);
-  const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, mp.inv_dx, sbox);
+  const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, mp.inv_dx);
+  const TileBox b = block_tile_box(sn.base, valid, sbox, TILE_NODES);
   if (b.V > 0) {
-    tile_clear<4>(tile, b);
-    LdsSink k;
-    k.tile = tile;
+    tile_clear_n(tile, TILE_COPIES * 4 * b.V);
+    LdsTileSink k;
+    k.tile = lane_copy(tile, b, 4);
     k.b = b;
-    k.g = g;
-    if (valid) mpm_p2g_particle(mp, ms, q, k);
-    tile_flush<4>(tile, b, g, mp.gnx, mp.gny, mp.gnz);
+    if (valid) mpm_p2g_particle(mp, ms, q, sn, k);
+    tile_flush_ch(tile, b, g.ch, 4, TILE_COPIES, mp.gnx, mp.gny, mp.gnz);
   } else if (valid) {
-    mpm_p2g_particle(mp, ms, q, g);
+    mpm_p2g_particle(mp, ms, q, sn, g);
   }
 }
 
 __global__ void __launch_bounds__(PTPB) k_mpm_p2g(FsiMpmParams mp, FsiMpmState ms) {
-  __shared__ float tile[4 * TILE_NODES];
+  __shared__ float tile[TILE_COPIES * 4 * TILE_NODES];
   __shared__ int sbox[6];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
   p2g_block(mp, ms, q, q < mp.n_particles, tile, sbox);
 }
 
 // Reaction scatter of particle q (valid) by a whole workgroup.
-__device__ __forceinline__ ReactOut reaction_block(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
-                                   const FsiMpmState& ms, const FsiCouplingParams& cp, int mode, long long q,
-                                   bool valid, float* tile, int* sbox) {
+__device__ __forceinline__ ReactOut reaction_block(const FsiLbmParams& lp, const FsiLbmState& ls,
+                                                   const FsiMpmParams& mp, const FsiMpmState& ms,
+                                                   const FsiCouplingParams& cp, int mode, long long q, bool valid,
+                                                   float* tile, int* sbox) {
   ReactOut o;
   o.active = 0;
   o.particle_norm = 0.0f;
   o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
   const GlobalSink g = reaction_sink(ms, (long long)mp.gnx * mp.gny * mp.gnz);
-  const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, cp.inv_dx_norm, sbox);
+  const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, cp.inv_dx_norm);
+  const TileBox b = block_tile_box(sn.base, valid, sbox, TILE_NODES);
   if (b.V > 0) {
-    tile_clear<3>(tile, b);
-    LdsSink k;
-    k.tile = tile;
+    tile_clear_n(tile, TILE_COPIES * 3 * b.V);
+    LdsTileSink k;
+    k.tile = lane_copy(tile, b, 3);
     k.b = b;
-    k.g = g;
-    if (valid) o = reaction_particle(lp, ls, mp, ms, cp, mode, q, k);
-    tile_flush<3>(tile, b, g, mp.gnx, mp.gny, mp.gnz);
+    if (valid) o = reaction_particle(lp, ls, mp, ms, cp, mode, q, sn, k);
+    tile_flush_ch(tile, b, g.ch, 3, TILE_COPIES, mp.gnx, mp.gny, mp.gnz);
   } else if (valid) {
-    o = reaction_particle(lp, ls, mp, ms, cp, mode, q, g);
+    o = reaction_particle(lp, ls, mp, ms, cp, mode, q, sn, g);
   }
   return o;
 }
@@ -634,7 +649,7 @@ __global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSt
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
-  __shared__ float tile[3 * TILE_NODES];
+  __shared__ float tile[TILE_COPIES * 3 * TILE_NODES];
   __shared__ int sbox[6];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
   const ReactOut o = reaction_block(lp, ls, mp, ms, cp, mode, q, q < mp.n_particles, tile, sbox);
@@ -654,15 +669,153 @@ __global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSt
 
 // ---------------------------------------------------------------------------
 // Fused MPM substep with the coupling reaction.  One pass over the particles
-// scatters P2G (grid_v, grid_m) and the reaction (grid_f) into one 7-channel
-// LDS tile; the reaction reads the J that P2G just stored for the particle, as
-// the reference's p2g -> add_*_reaction_to_mpm_grid order does.  Each block
-// leaves its reaction reductions in a workspace record; block 0 of the grid
-// update folds the records in block order into the counters (deterministic,
-// and no contended same-address atomics).
+// scatters P2G and the reaction into an LDS tile; the reaction reads the J
+// that P2G just stored for the particle, as the reference's p2g ->
+// add_*_reaction_to_mpm_grid order does.  Each block leaves its reaction
+// reductions in a workspace record; block 0 of the grid update folds the
+// records in block order into the counters (deterministic, and no contended
+// same-address atomics).
+//
+// The reaction's grid force enters the grid update only as dt * f / m next to
+// the momentum (solid.py:252-259), so the fused pass adds dt * f into the
+// particle's momentum before its scatter and leaves grid_f at zero: the update
+// is the same in exact arithmetic, and each node receives 4 floats (momentum
+// x, y, z and mass), kept together in the tile (array of float4).
 // ---------------------------------------------------------------------------
 constexpr int FUSED_TILE_NODES = TILE_NODES;
 constexpr int REACT_REC = 6;  // active count, max particle norm, max grid contribution, net force x, y, z
+
+// Keeps the compiler from moving LDS accesses across this point; the wave
+// itself issues its LDS instructions in program order.
+__device__ __forceinline__ void wave_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The fused tile updates a node with one 16-byte LDS load and one store
+// (read-modify-write) instead of four float atomics.  That is exact when the
+// workgroup is one wave and no two of its lanes that share a tile copy have
+// the same stencil base: in any one instruction the lanes of a copy then
+// address distinct nodes, and two updates of one node by different lanes come
+// from different iterations of the 27-node loop, which the wave issues in
+// order.  In a lattice-ordered cloud (8 particles per cell, lane parity = the
+// particle's z half of the cell, which is also its tile copy) the condition
+// always holds; otherwise the workgroup takes the atomic path.
+static_assert(PTPB == 64, "read-modify-write scatter assumes one wave per particle workgroup");
+
+// Whether the read-modify-write scatter is exact for this workgroup.  Scratch:
+// the tile, before it is cleared.
+__device__ __forceinline__ bool bases_exclusive(const Stencil& sn, bool valid, const TileBox& b, float* tile) {
+  int* own = reinterpret_cast<int*>(tile);
+  int key = 0;
+  if (valid) {
+    key = (int)(threadIdx.x & (TILE_COPIES - 1)) * b.V +
+          ((sn.base[0] - b.lx) * b.ey + (sn.base[1] - b.ly)) * b.ez + (sn.base[2] - b.lz);
+    own[key] = (int)threadIdx.x;
+  }
+  __syncthreads();
+  const int clash = valid && own[key] != (int)threadIdx.x;
+  return !__syncthreads_or(clash);
+}
+
+// P2G (+ reaction force folded into the momentum, MODE 1 penalty / 2 moving
+// boundary) of particle q into its tile copy `tile` (box b).  INTERIOR: every
+// stencil node of the workgroup is inside the MPM grid and (MODE > 0) the
+// lattice, so no node is tested; otherwise the gather tests each lattice cell
+// and nodes outside the grid land in the tile's margin, which the flush skips.
+template <int MODE, bool INTERIOR, bool RMW>
+__device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
+                                                       const FsiMpmParams& mp, const FsiMpmState& ms,
+                                                       const FsiCouplingParams& cp, long long q, const Stencil& sn,
+                                                       float4* tile, const TileBox& b) {
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  const P2gTerms t = p2g_terms(mp, ms, q);
+  float mv[3] = {t.mv[0], t.mv[1], t.mv[2]};
+  if (MODE) {
+    o = reaction_force<!INTERIOR>(lp, ls, ms, cp, MODE, q, sn);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) mv[d] += mp.dt * o.f[d];
+  }
+  const int* base = sn.base;
+  const float* fx = sn.fx;
+  const float(&w)[3][3] = sn.w;
+  float4* t0 = tile + ((base[0] - b.lx) * b.ey + (base[1] - b.ly)) * b.ez + (base[2] - b.lz);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float wt = w[0][a] * w[1][e] * w[2][c];
+        const float dpx = ((float)a - fx[0]) * mp.dx;
+        const float dpy = ((float)e - fx[1]) * mp.dx;
+        const float dpz = ((float)c - fx[2]) * mp.dx;
+        const float ax = wt * (mv[0] + t.aff[0] * dpx + t.aff[1] * dpy + t.aff[2] * dpz);
+        const float ay = wt * (mv[1] + t.aff[3] * dpx + t.aff[4] * dpy + t.aff[5] * dpz);
+        const float az = wt * (mv[2] + t.aff[6] * dpx + t.aff[7] * dpy + t.aff[8] * dpz);
+        const float am = wt * t.mass;
+        if (MODE && (INTERIOR || inside3(base[0] + a, base[1] + e, base[2] + c, mp.gnx, mp.gny, mp.gnz))) {
+          const float cn = norm3(wt * o.f[0], wt * o.f[1], wt * o.f[2]);
+          if (cn > o.max_contrib) o.max_contrib = cn;
+        }
+        float4* node = t0 + (a * b.ey + e) * b.ez + c;
+        if (RMW) {
+          float4 v = *node;
+          v.x += ax;
+          v.y += ay;
+          v.z += az;
+          v.w += am;
+          *node = v;
+          wave_order();
+        } else {
+          float* f = reinterpret_cast<float*>(node);
+          __hip_atomic_fetch_add(f + 0, ax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(f + 1, ay, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(f + 2, az, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(f + 3, am, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+  return o;
+}
+
+template <int MODE, bool INTERIOR>
+__device__ __forceinline__ ReactOut p2g_react_particle_rmw(bool rmw, const FsiLbmParams& lp, const FsiLbmState& ls,
+                                                           const FsiMpmParams& mp, const FsiMpmState& ms,
+                                                           const FsiCouplingParams& cp, long long q,
+                                                           const Stencil& sn, float4* tile, const TileBox& b) {
+  return rmw ? p2g_react_particle<MODE, INTERIOR, true>(lp, ls, mp, ms, cp, q, sn, tile, b)
+             : p2g_react_particle<MODE, INTERIOR, false>(lp, ls, mp, ms, cp, q, sn, tile, b);
+}
+
+// Flush of the fused tile: the copies are summed per node, then one global
+// atomic per touched node and channel into grid_v (x, y, z) and grid_m.
+__device__ __forceinline__ void tile_flush_fused(const float4* tile, const TileBox& b, const GlobalSink& g, int gnx,
+                                                 int gny, int gnz) {
+  __syncthreads();
+  const int plane = b.ey * b.ez;
+  for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
+    const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
+    const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
+    if (I < 0 || J < 0 || K < 0 || I >= gnx || J >= gny || K >= gnz) continue;
+    const long long n = ((long long)I * gny + J) * gnz + K;
+    float4 v = tile[i];
+    for (int k = 1; k < TILE_COPIES; ++k) {
+      const float4 u = tile[k * b.V + i];
+      v.x += u.x;
+      v.y += u.y;
+      v.z += u.z;
+      v.w += u.w;
+    }
+    if (v.x != 0.0f) atomicAdd(g.ch[0] + n, v.x);
+    if (v.y != 0.0f) atomicAdd(g.ch[1] + n, v.y);
+    if (v.z != 0.0f) atomicAdd(g.ch[2] + n, v.z);
+    if (v.w != 0.0f) atomicAdd(g.ch[3] + n, v.w);
+  }
+}
 
 // P2G (+ reaction, MODE 1 penalty / 2 moving boundary) of particle q (valid) by
 // a whole workgroup; the block's reaction record goes to rec[0..REACT_REC).
@@ -673,30 +826,40 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
                                                 float* sf, int* si) {
   const long long G = (long long)mp.gnx * mp.gny * mp.gnz;
   const GlobalSink gp = p2g_sink(ms, G), gr = reaction_sink(ms, G);
-  const TileBox b = block_tile_box(ms.x, mp.n_particles, q, valid, mp.inv_dx, sbox, FUSED_TILE_NODES);
+  const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, mp.inv_dx);
+  const TileBox b = block_tile_box(sn.base, valid, sbox, FUSED_TILE_NODES);
+  // the reaction samples the lattice with cp.inv_dx_norm; it rides on the P2G
+  // stencil when that spacing is the MPM grid's (the drivers' setting), and
+  // otherwise scatters on its own stencil into grid_f
+  const bool same = cp.inv_dx_norm == mp.inv_dx;
   ReactOut o;
   o.active = 0;
   o.particle_norm = o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
   if (b.V > 0) {
-    constexpr int NCH = MODE ? 7 : 4;
-    tile_clear<NCH>(tile, b);
-    LdsSink kp;
-    kp.tile = tile;
-    kp.b = b;
-    kp.g = gp;
-    if (valid) mpm_p2g_particle(mp, ms, q, kp);
-    if (MODE) {
-      LdsSink kr;
-      kr.tile = tile + 4 * b.V;
-      kr.b = b;
-      kr.g = gr;
-      if (valid) o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, kr);
+    const bool rmw = bases_exclusive(sn, valid, b, tile);
+    tile_clear_n(tile, TILE_COPIES * 4 * b.V);
+    float4* mine = reinterpret_cast<float4*>(tile) + (threadIdx.x & (TILE_COPIES - 1)) * b.V;
+    const bool in_grid = b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= mp.gnx && b.ly + b.ey <= mp.gny &&
+                         b.lz + b.ez <= mp.gnz;
+    const bool in_lat =
+        b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= lp.nx && b.ly + b.ey <= lp.ny && b.lz + b.ez <= lp.nz;
+    if (valid) {
+      if (MODE && !same) {
+        if (in_grid)
+          p2g_react_particle_rmw<0, true>(rmw, lp, ls, mp, ms, cp, q, sn, mine, b);
+        else
+          p2g_react_particle_rmw<0, false>(rmw, lp, ls, mp, ms, cp, q, sn, mine, b);
+        o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, gr);
+      } else if (in_grid && (MODE == 0 || in_lat)) {
+        o = p2g_react_particle_rmw<MODE, true>(rmw, lp, ls, mp, ms, cp, q, sn, mine, b);
+      } else {
+        o = p2g_react_particle_rmw<MODE, false>(rmw, lp, ls, mp, ms, cp, q, sn, mine, b);
+      }
     }
-    float* ch[7] = {gp.ch[0], gp.ch[1], gp.ch[2], gp.ch[3], gr.ch[0], gr.ch[1], gr.ch[2]};
-    tile_flush_ch(tile, b, ch, NCH, mp.gnx, mp.gny, mp.gnz);
+    tile_flush_fused(reinterpret_cast<const float4*>(tile), b, gp, mp.gnx, mp.gny, mp.gnz);
   } else if (valid) {
-    mpm_p2g_particle(mp, ms, q, gp);
+    mpm_p2g_particle(mp, ms, q, sn, gp);
     if (MODE) o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, gr);
   }
   if (MODE) {
@@ -719,7 +882,7 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
 template <int MODE>  // 0: no reaction, 1: penalty reaction, 2: moving-boundary reaction
 __global__ void __launch_bounds__(PTPB) k_mpm_p2g_react(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
                                                         FsiMpmState ms, FsiCouplingParams cp, double* rec) {
-  __shared__ float tile[7 * FUSED_TILE_NODES];
+  __shared__ __align__(16) float tile[TILE_COPIES * 4 * FUSED_TILE_NODES];
   __shared__ int sbox[6];
   __shared__ double sd[PTPB / 64];
   __shared__ float sf[PTPB / 64];
@@ -921,7 +1084,7 @@ __global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* ins
 }
 
 __global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance* inst, const int32_t* bmap) {
-  __shared__ float tile[5 * PROJ_TILE_NODES];
+  __shared__ float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
   __shared__ int sbox[6];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   proj_block(I.lp, I.ls, I.mp, I.ms, I.cp, e_, in_, tile, sbox);
@@ -988,7 +1151,7 @@ __global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst
 }
 
 __global__ void __launch_bounds__(PTPB) kb_mpm_p2g(const FsiBatchInstance* inst, const int32_t* bmap) {
-  __shared__ float tile[4 * TILE_NODES];
+  __shared__ float tile[TILE_COPIES * 4 * TILE_NODES];
   __shared__ int sbox[6];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   p2g_block(I.mp, I.ms, e_, in_, tile, sbox);
@@ -998,7 +1161,7 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* 
   __shared__ double sd[PTPB / 64];
   __shared__ float sf[PTPB / 64];
   __shared__ int si[PTPB / 64];
-  __shared__ float tile[3 * TILE_NODES];
+  __shared__ float tile[TILE_COPIES * 3 * TILE_NODES];
   __shared__ int sbox[6];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   const int mode = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1;
@@ -1020,7 +1183,7 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* 
 }
 
 __global__ void __launch_bounds__(PTPB) kb_mpm_p2g_react(const FsiBatchInstance* inst, const int32_t* bmap) {
-  __shared__ float tile[7 * FUSED_TILE_NODES];
+  __shared__ __align__(16) float tile[TILE_COPIES * 4 * FUSED_TILE_NODES];
   __shared__ int sbox[6];
   __shared__ double sd[PTPB / 64];
   __shared__ float sf[PTPB / 64];
