@@ -992,6 +992,7 @@ FSI_HD float det3(const float* A) {
 FSI_HD void sym_eig3(float* A, float* V) {
 #pragma unroll
   for (int e = 0; e < 9; ++e) V[e] = (e == 0 || e == 4 || e == 8) ? 1.0f : 0.0f;
+#pragma unroll 1
   for (int sweep = 0; sweep < 6; ++sweep) {
 #pragma unroll
     for (int pq = 0; pq < 3; ++pq) {
@@ -1040,19 +1041,24 @@ FSI_HD void sym_eig3(float* A, float* V) {
 // two largest s_i, u_2 = u_0 x u_1), R = sum u_i v_i^T, D = F - R.
 FSI_HD void rotation_and_deviation(const float* Fm, float* R, float* D) {
   const float detF = det3(Fm);
-  if (detF > 0.0f) {
-    float H[9];
+  const bool pos = detF > 0.0f;
+  // the matrix to diagonalise: F^T F - I (det F > 0) or F^T F, one eigen
+  // solve for both cases
+  float A[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) H[e] = Fm[e] - ((e == 0 || e == 4 || e == 8) ? 1.0f : 0.0f);
-    float A[9];
+  for (int r = 0; r < 3; ++r)
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        A[r * 3 + c] = H[r * 3 + c] + H[c * 3 + r] +
-                       (H[0 * 3 + r] * H[0 * 3 + c] + H[1 * 3 + r] * H[1 * 3 + c] + H[2 * 3 + r] * H[2 * 3 + c]);
-    float V[9];
-    sym_eig3(A, V);
+    for (int c = 0; c < 3; ++c) {
+      const float hr0 = Fm[0 * 3 + r] - (r == 0 ? 1.0f : 0.0f), hc0 = Fm[0 * 3 + c] - (c == 0 ? 1.0f : 0.0f);
+      const float hr1 = Fm[1 * 3 + r] - (r == 1 ? 1.0f : 0.0f), hc1 = Fm[1 * 3 + c] - (c == 1 ? 1.0f : 0.0f);
+      const float hr2 = Fm[2 * 3 + r] - (r == 2 ? 1.0f : 0.0f), hc2 = Fm[2 * 3 + c] - (c == 2 ? 1.0f : 0.0f);
+      const float hrc = Fm[r * 3 + c] - (r == c ? 1.0f : 0.0f), hcr = Fm[c * 3 + r] - (r == c ? 1.0f : 0.0f);
+      A[r * 3 + c] = pos ? hrc + hcr + (hr0 * hc0 + hr1 * hc1 + hr2 * hc2)
+                         : Fm[0 * 3 + r] * Fm[0 * 3 + c] + Fm[1 * 3 + r] * Fm[1 * 3 + c] + Fm[2 * 3 + r] * Fm[2 * 3 + c];
+    }
+  float V[9];
+  sym_eig3(A, V);
+  if (pos) {
     float d[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -1076,14 +1082,6 @@ FSI_HD void rotation_and_deviation(const float* Fm, float* R, float* D) {
       }
     return;
   }
-  float A[9];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      A[r * 3 + c] = Fm[0 * 3 + r] * Fm[0 * 3 + c] + Fm[1 * 3 + r] * Fm[1 * 3 + c] + Fm[2 * 3 + r] * Fm[2 * 3 + c];
-  float V[9];
-  sym_eig3(A, V);
   float lam[3] = {A[0], A[4], A[8]};
   int o0 = 0, o1 = 1, o2 = 2;
   if (lam[o0] < lam[o1]) { int t = o0; o0 = o1; o1 = t; }
@@ -1279,24 +1277,39 @@ FSI_HD void mpm_g2p_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
   float fx[3], w[3][3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * mp.inv_dx, base[d], fx[d], w[d]);
+  float Fm[9];
+  load_mat(ms.F, P, p, Fm);
   float nv[3] = {0.0f, 0.0f, 0.0f};
   float nC[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b)
-      for (int cc = 0; cc < 3; ++cc) {
-        const int I = base[0] + a, Jn = base[1] + b, K = base[2] + cc;
-        if (!inside3(I, Jn, K, g.nx, g.ny, g.nz)) continue;
-        const float dp[3] = {((float)a - fx[0]) * mp.dx, ((float)b - fx[1]) * mp.dx, ((float)cc - fx[2]) * mp.dx};
-        const float wt = w[0][a] * w[1][b] * w[2][cc];
-        const long long n = ((long long)I * g.ny + Jn) * g.nz + K;
-        const float gv[3] = {ms.grid_v[n], ms.grid_v[g.G + n], ms.grid_v[2 * g.G + n]};
+  // One plane of 9 nodes at a time, loads first (a node outside the grid
+  // reads node 0 and is then skipped): one round trip per plane.
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          nv[r] += wt * gv[r];
+  for (int a = 0; a < 3; ++a) {
+    float gv[9][3];
+    bool in[9];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) nC[r * 3 + c] += 4.0f * wt * gv[r] * dp[c] * mp.inv_dx * mp.inv_dx;
-        }
+    for (int e = 0; e < 9; ++e) {
+      const int I = base[0] + a, Jn = base[1] + e / 3, K = base[2] + e % 3;
+      in[e] = inside3(I, Jn, K, g.nx, g.ny, g.nz);
+      const long long n = in[e] ? ((long long)I * g.ny + Jn) * g.nz + K : 0;
+      gv[e][0] = ms.grid_v[n];
+      gv[e][1] = ms.grid_v[g.G + n];
+      gv[e][2] = ms.grid_v[2 * g.G + n];
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      if (!in[e]) continue;
+      const int b = e / 3, cc = e % 3;
+      const float dp[3] = {((float)a - fx[0]) * mp.dx, ((float)b - fx[1]) * mp.dx, ((float)cc - fx[2]) * mp.dx};
+      const float wt = w[0][a] * w[1][b] * w[2][cc];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        nv[r] += wt * gv[e][r];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nC[r * 3 + c] += 4.0f * wt * gv[e][r] * dp[c] * mp.inv_dx * mp.inv_dx;
       }
+    }
+  }
   float x[3], v[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -1315,8 +1328,7 @@ FSI_HD void mpm_g2p_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
       if (v[d] > 0.0f) v[d] = 0.0f;
     }
   }
-  float Fm[9], Fn[9];
-  load_mat(ms.F, P, p, Fm);
+  float Fn[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -1489,11 +1501,11 @@ struct ReactOut {
 // Reaction force on particle p: mode 1 = penalty (penalty.py:90-129), mode 2 =
 // moving boundary (moving_boundary.py:58-111).  Gathers hydro_force (and, for
 // mode 2, solid_phi) at the stencil's lattice cells; o.f is the particle's
-// reaction force, zero for a mode-2 particle outside the solid.  CHECKED =
+// reaction force, zero for a mode-2 particle outside the solid.  checked =
 // false: the caller knows every stencil cell is inside the lattice.
-template <bool CHECKED = true>
 FSI_HD ReactOut reaction_force(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmState& ms,
-                               const FsiCouplingParams& cp, int mode, long long p, const Stencil& sn) {
+                               const FsiCouplingParams& cp, int mode, long long p, const Stencil& sn,
+                               bool checked = true) {
   const Grid g = make_grid(lp);
   ReactOut o;
   o.active = 0;
@@ -1503,18 +1515,34 @@ FSI_HD ReactOut reaction_force(const FsiLbmParams& lp, const FsiLbmState& ls, co
   const int* base = sn.base;
   const float(&w)[3][3] = sn.w;
   float hx = 0.0f, hy = 0.0f, hz = 0.0f, sphi = 0.0f;
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b)
-      for (int cc = 0; cc < 3; ++cc) {
-        const int I = base[0] + a, Jn = base[1] + b, K = base[2] + cc;
-        if (CHECKED && !inside3(I, Jn, K, g.nx, g.ny, g.nz)) continue;
-        const float wt = w[0][a] * w[1][b] * w[2][cc];
-        const long long c = cell_of(g, I, Jn, K);
-        hx += wt * ls.hydro_force[c];
-        hy += wt * ls.hydro_force[g.N + c];
-        hz += wt * ls.hydro_force[2 * g.N + c];
-        if (mode == 2) sphi += wt * ls.solid_phi[c];
-      }
+  // One plane of 9 cells at a time: the loads of a plane carry no branch (a
+  // cell outside the lattice reads cell 0 and is then skipped), so they are
+  // all in flight together instead of one round trip per cell.
+#pragma unroll 1
+  for (int a = 0; a < 3; ++a) {
+    float h[9][4];
+    bool in[9];
+    const float wa = a == 0 ? w[0][0] : (a == 1 ? w[0][1] : w[0][2]);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      const int I = base[0] + a, Jn = base[1] + e / 3, K = base[2] + e % 3;
+      in[e] = !checked || inside3(I, Jn, K, g.nx, g.ny, g.nz);
+      const long long c = in[e] ? cell_of(g, I, Jn, K) : 0;
+      h[e][0] = ls.hydro_force[c];
+      h[e][1] = ls.hydro_force[g.N + c];
+      h[e][2] = ls.hydro_force[2 * g.N + c];
+      h[e][3] = mode == 2 ? ls.solid_phi[c] : 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      if (!in[e]) continue;
+      const float wt = wa * w[1][e / 3] * w[2][e % 3];
+      hx += wt * h[e][0];
+      hy += wt * h[e][1];
+      hz += wt * h[e][2];
+      if (mode == 2) sphi += wt * h[e][3];
+    }
+  }
   if (mode == 2 && !(sphi > cp.mb_phi_min)) return o;
   float J = ms.Jp[p];
   J = J > 0.0f ? J : 0.0f;

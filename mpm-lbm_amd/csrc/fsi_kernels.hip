@@ -38,6 +38,9 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB)
 // to lattice sizes (tens of thousands), so small workgroups spread them over
 // all 256 CUs instead of a few dozen.
 constexpr int PTPB = FSI_PARTICLE_BLOCK;
+// The particle workgroups are one wave: their tile boxes and reductions are
+// wave reductions, and the fused scatter's read-modify-write relies on it.
+static_assert(PTPB == 64, "particle workgroups are one wave64");
 inline unsigned pblocks_for(long long n) { return (unsigned)((n + PTPB - 1) / PTPB); }
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -93,10 +96,12 @@ __device__ __forceinline__ double wave_reduce(double v, int op) {
   return v;
 }
 
-// Result valid in thread 0.
+// Result valid in thread 0 (in every thread of a one-wave workgroup, which
+// skips the LDS stage and its barriers).
 __device__ double block_reduce(double v, int op, double* smem) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   v = wave_reduce(v, op);
+  if (blockDim.x == 64) return v;
   __syncthreads();
   if (lane == 0) smem[wave] = v;
   __syncthreads();
@@ -112,6 +117,7 @@ __device__ __forceinline__ float block_max_f(float v, float* smem) {
     const float o = __shfl_xor(v, off, 64);
     v = o > v ? o : v;
   }
+  if (blockDim.x == 64) return v > 0.0f ? v : 0.0f;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) smem[wave] = v;
@@ -125,6 +131,7 @@ __device__ __forceinline__ float block_max_f(float v, float* smem) {
 __device__ __forceinline__ int block_sum_i(int v, int* smem) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (blockDim.x == 64) return v;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) smem[wave] = v;
@@ -154,33 +161,44 @@ struct TileBox {
   int lx, ly, lz, ex, ey, ez, V;
 };
 
-// Box of the block's particles' stencils, [min base, max base + 2] per axis,
-// from the base nodes the bodies then use: every node a valid particle of the
-// block scatters to lies in it.  V = 0: no valid particle, or the box holds
-// more than `cap` nodes.  Every thread of the block must call this.
-__device__ __forceinline__ TileBox block_tile_box(const int* base, bool valid, int* sbox, int cap) {
-  if (threadIdx.x == 0) {
-    sbox[0] = sbox[1] = sbox[2] = 0x7FFFFFFF;
-    sbox[3] = sbox[4] = sbox[5] = -0x7FFFFFFF;
-  }
-  __syncthreads();
-  if (valid) {
+__device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      atomicMin(&sbox[d], base[d]);
-      atomicMax(&sbox[3 + d], base[d]);
-    }
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
   }
-  __syncthreads();
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Box of the workgroup's particles' stencils, [min base, max base + 2] per
+// axis, from the base nodes the bodies then use: every node a valid particle
+// of the workgroup scatters to lies in it.  V = 0: no valid particle, or the
+// box holds more than `cap` nodes.  The particle workgroups are one wave, so
+// the bounds are wave reductions (uniform in every lane; no LDS, no barrier).
+__device__ __forceinline__ TileBox block_tile_box(const int* base, bool valid, int cap) {
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = wave_min_i(valid ? base[d] : 0x7FFFFFFF);
+    hi[d] = wave_max_i(valid ? base[d] : -0x7FFFFFFF);
+  }
   TileBox t;
-  t.lx = sbox[0];
-  t.ly = sbox[1];
-  t.lz = sbox[2];
+  t.lx = lo[0];
+  t.ly = lo[1];
+  t.lz = lo[2];
   t.V = 0;
   t.ex = t.ey = t.ez = 0;
-  if (sbox[0] <= sbox[3] && sbox[1] <= sbox[4] && sbox[2] <= sbox[5]) {
-    const long long ex = (long long)sbox[3] - sbox[0] + 3, ey = (long long)sbox[4] - sbox[1] + 3,
-                    ez = (long long)sbox[5] - sbox[2] + 3;
+  if (lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2]) {
+    const long long ex = (long long)hi[0] - lo[0] + 3, ey = (long long)hi[1] - lo[1] + 3,
+                    ez = (long long)hi[2] - lo[2] + 3;
     if (ex * ey * ez <= cap) {
       t.ex = (int)ex;
       t.ey = (int)ey;
@@ -188,7 +206,6 @@ __device__ __forceinline__ TileBox block_tile_box(const int* base, bool valid, i
       t.V = (int)(ex * ey * ez);
     }
   }
-  __syncthreads();  // sbox may be reused by the caller
   return t;
 }
 
@@ -276,10 +293,10 @@ constexpr int PROJ_TILE_NODES = TILE_NODES;
 
 __device__ __forceinline__ void proj_block(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
                                            const FsiMpmState& ms, const FsiCouplingParams& cp, long long q, bool valid,
-                                           float* tile, int* sbox) {
+                                           float* tile) {
   const GlobalSink g = proj_sink(ls, (long long)lp.nx * lp.ny * lp.nz);
   const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, cp.inv_dx_norm);
-  const TileBox b = block_tile_box(sn.base, valid, sbox, PROJ_TILE_NODES);
+  const TileBox b = block_tile_box(sn.base, valid, PROJ_TILE_NODES);
   if (b.V > 0) {
     tile_clear_n(tile, TILE_COPIES * 5 * b.V);
     LdsTileSink k;
@@ -494,9 +511,8 @@ __global__ void __launch_bounds__(TPB) k_proj_clear(FsiLbmParams p, FsiLbmState 
 __global__ void __launch_bounds__(PTPB) k_proj_particles(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
                                                          FsiMpmState ms, FsiCouplingParams cp) {
   __shared__ float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
-  __shared__ int sbox[6];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
-  proj_block(lp, ls, mp, ms, cp, q, q < mp.n_particles, tile, sbox);
+  proj_block(lp, ls, mp, ms, cp, q, q < mp.n_particles, tile);
 }
 
 __global__ void __launch_bounds__(TPB) k_proj_normalize(FsiLbmParams lp, FsiLbmState ls, FsiCouplingParams cp,
@@ -582,10 +598,10 @@ __global__ void __launch_bounds__(TPB) k_mpm_clear(FsiMpmParams mp, FsiMpmState 
 
 // P2G of particle q (valid) by a whole workgroup.
 __device__ __forceinline__ void p2g_block(const FsiMpmParams& mp, const FsiMpmState& ms, long long q, bool valid,
-                                          float* tile, int* sbox) {
+                                          float* tile) {
   const GlobalSink g = p2g_sink(ms, (long long)mp.gnx * mp.gny * mp.gnz);
   const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, mp.inv_dx);
-  const TileBox b = block_tile_box(sn.base, valid, sbox, TILE_NODES);
+  const TileBox b = block_tile_box(sn.base, valid, TILE_NODES);
   if (b.V > 0) {
     tile_clear_n(tile, TILE_COPIES * 4 * b.V);
     LdsTileSink k;
@@ -600,16 +616,15 @@ __device__ __forceinline__ void p2g_block(const FsiMpmParams& mp, const FsiMpmSt
 
 __global__ void __launch_bounds__(PTPB) k_mpm_p2g(FsiMpmParams mp, FsiMpmState ms) {
   __shared__ float tile[TILE_COPIES * 4 * TILE_NODES];
-  __shared__ int sbox[6];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
-  p2g_block(mp, ms, q, q < mp.n_particles, tile, sbox);
+  p2g_block(mp, ms, q, q < mp.n_particles, tile);
 }
 
 // Reaction scatter of particle q (valid) by a whole workgroup.
 __device__ __forceinline__ ReactOut reaction_block(const FsiLbmParams& lp, const FsiLbmState& ls,
                                                    const FsiMpmParams& mp, const FsiMpmState& ms,
                                                    const FsiCouplingParams& cp, int mode, long long q, bool valid,
-                                                   float* tile, int* sbox) {
+                                                   float* tile) {
   ReactOut o;
   o.active = 0;
   o.particle_norm = 0.0f;
@@ -617,7 +632,7 @@ __device__ __forceinline__ ReactOut reaction_block(const FsiLbmParams& lp, const
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
   const GlobalSink g = reaction_sink(ms, (long long)mp.gnx * mp.gny * mp.gnz);
   const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, cp.inv_dx_norm);
-  const TileBox b = block_tile_box(sn.base, valid, sbox, TILE_NODES);
+  const TileBox b = block_tile_box(sn.base, valid, TILE_NODES);
   if (b.V > 0) {
     tile_clear_n(tile, TILE_COPIES * 3 * b.V);
     LdsTileSink k;
@@ -650,9 +665,8 @@ __global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSt
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
   __shared__ float tile[TILE_COPIES * 3 * TILE_NODES];
-  __shared__ int sbox[6];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
-  const ReactOut o = reaction_block(lp, ls, mp, ms, cp, mode, q, q < mp.n_particles, tile, sbox);
+  const ReactOut o = reaction_block(lp, ls, mp, ms, cp, mode, q, q < mp.n_particles, tile);
   const int act = block_sum_i(o.active, si);
   const float mpn = block_max_f(o.particle_norm, sf);
   const float mgc = block_max_f(o.max_contrib, sf);
@@ -702,7 +716,6 @@ __device__ __forceinline__ void wave_order() {
 // order.  In a lattice-ordered cloud (8 particles per cell, lane parity = the
 // particle's z half of the cell, which is also its tile copy) the condition
 // always holds; otherwise the workgroup takes the atomic path.
-static_assert(PTPB == 64, "read-modify-write scatter assumes one wave per particle workgroup");
 
 // Whether the read-modify-write scatter is exact for this workgroup.  Scratch:
 // the tile, before it is cleared.
@@ -719,76 +732,93 @@ __device__ __forceinline__ bool bases_exclusive(const Stencil& sn, bool valid, c
   return !__syncthreads_or(clash);
 }
 
-// P2G (+ reaction force folded into the momentum, MODE 1 penalty / 2 moving
-// boundary) of particle q into its tile copy `tile` (box b).  INTERIOR: every
-// stencil node of the workgroup is inside the MPM grid and (MODE > 0) the
-// lattice, so no node is tested; otherwise the gather tests each lattice cell
-// and nodes outside the grid land in the tile's margin, which the flush skips.
-template <int MODE, bool INTERIOR, bool RMW>
+// Where the fused P2G scatter puts a node's 4 floats.
+enum FusedScatter {
+  SCATTER_TILE_RMW = 0,     // LDS tile, read-modify-write (bases exclusive per copy)
+  SCATTER_TILE_ATOMIC = 1,  // LDS tile, float atomics
+  SCATTER_GLOBAL = 2        // grid arrays, float atomics (box too large for a tile)
+};
+
+__device__ __forceinline__ float pick3(int i, const float (&v)[3]) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
+
+// P2G of particle q (+ for MODE 1 penalty / 2 moving boundary with `fold`, the
+// reaction force, added into the momentum) with one scatter of 4 floats per
+// stencil node.  `checked` = false: every stencil node of the workgroup is
+// inside the MPM grid and (fold) the lattice, so nothing is tested; otherwise
+// the gather tests each lattice cell, and a node outside the grid lands in the
+// tile's margin (which the flush skips) or, for the global scatter, is
+// dropped.  The flags are uniform over the workgroup; the loops stay rolled
+// over the (a, e) node rows so the kernel's code stays small.
+template <int MODE>
 __device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
                                                        const FsiMpmParams& mp, const FsiMpmState& ms,
                                                        const FsiCouplingParams& cp, long long q, const Stencil& sn,
-                                                       float4* tile, const TileBox& b) {
+                                                       bool fold, bool checked, int kind, float4* tile,
+                                                       const TileBox& b, const GlobalSink& g) {
   ReactOut o;
   o.active = 0;
   o.particle_norm = o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
   const P2gTerms t = p2g_terms(mp, ms, q);
   float mv[3] = {t.mv[0], t.mv[1], t.mv[2]};
-  if (MODE) {
-    o = reaction_force<!INTERIOR>(lp, ls, ms, cp, MODE, q, sn);
+  if (MODE && fold) {
+    o = reaction_force(lp, ls, ms, cp, MODE, q, sn, checked);
 #pragma unroll
     for (int d = 0; d < 3; ++d) mv[d] += mp.dt * o.f[d];
   }
   const int* base = sn.base;
   const float* fx = sn.fx;
   const float(&w)[3][3] = sn.w;
-  float4* t0 = tile + ((base[0] - b.lx) * b.ey + (base[1] - b.ly)) * b.ez + (base[2] - b.lz);
+  const int gnx = mp.gnx, gny = mp.gny, gnz = mp.gnz;
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const int a = r / 3, e = r - 3 * (r / 3);
+    const float wae = pick3(a, w[0]) * pick3(e, w[1]);
+    const float dpx = ((float)a - fx[0]) * mp.dx;
+    const float dpy = ((float)e - fx[1]) * mp.dx;
+    const float bx = mv[0] + t.aff[0] * dpx + t.aff[1] * dpy;
+    const float by = mv[1] + t.aff[3] * dpx + t.aff[4] * dpy;
+    const float bz = mv[2] + t.aff[6] * dpx + t.aff[7] * dpy;
+    const int I = base[0] + a, J = base[1] + e;
+    const bool row_in = !checked || (0 <= I && I < gnx && 0 <= J && J < gny);
+    float4* row = tile + ((I - b.lx) * b.ey + (J - b.ly)) * b.ez + (base[2] - b.lz);
+    const long long nrow = ((long long)I * gny + J) * gnz + base[2];
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int e = 0; e < 3; ++e)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float wt = w[0][a] * w[1][e] * w[2][c];
-        const float dpx = ((float)a - fx[0]) * mp.dx;
-        const float dpy = ((float)e - fx[1]) * mp.dx;
-        const float dpz = ((float)c - fx[2]) * mp.dx;
-        const float ax = wt * (mv[0] + t.aff[0] * dpx + t.aff[1] * dpy + t.aff[2] * dpz);
-        const float ay = wt * (mv[1] + t.aff[3] * dpx + t.aff[4] * dpy + t.aff[5] * dpz);
-        const float az = wt * (mv[2] + t.aff[6] * dpx + t.aff[7] * dpy + t.aff[8] * dpz);
-        const float am = wt * t.mass;
-        if (MODE && (INTERIOR || inside3(base[0] + a, base[1] + e, base[2] + c, mp.gnx, mp.gny, mp.gnz))) {
-          const float cn = norm3(wt * o.f[0], wt * o.f[1], wt * o.f[2]);
-          if (cn > o.max_contrib) o.max_contrib = cn;
-        }
-        float4* node = t0 + (a * b.ey + e) * b.ez + c;
-        if (RMW) {
-          float4 v = *node;
-          v.x += ax;
-          v.y += ay;
-          v.z += az;
-          v.w += am;
-          *node = v;
-          wave_order();
-        } else {
-          float* f = reinterpret_cast<float*>(node);
-          __hip_atomic_fetch_add(f + 0, ax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(f + 1, ay, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(f + 2, az, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(f + 3, am, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
+    for (int c = 0; c < 3; ++c) {
+      const float wt = wae * w[2][c];
+      const float dpz = ((float)c - fx[2]) * mp.dx;
+      const float ax = wt * (bx + t.aff[2] * dpz);
+      const float ay = wt * (by + t.aff[5] * dpz);
+      const float az = wt * (bz + t.aff[8] * dpz);
+      const float am = wt * t.mass;
+      const bool in = row_in && (!checked || (0 <= base[2] + c && base[2] + c < gnz));
+      if (MODE && fold && in) {
+        const float cn = norm3(wt * o.f[0], wt * o.f[1], wt * o.f[2]);
+        if (cn > o.max_contrib) o.max_contrib = cn;
       }
+      if (kind == SCATTER_TILE_RMW) {
+        float4 v = row[c];
+        v.x += ax;
+        v.y += ay;
+        v.z += az;
+        v.w += am;
+        row[c] = v;
+        wave_order();
+      } else if (kind == SCATTER_TILE_ATOMIC) {
+        float* f = reinterpret_cast<float*>(row + c);
+        __hip_atomic_fetch_add(f + 0, ax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(f + 1, ay, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(f + 2, az, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(f + 3, am, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else if (in) {
+        atomicAdd(g.ch[0] + nrow + c, ax);
+        atomicAdd(g.ch[1] + nrow + c, ay);
+        atomicAdd(g.ch[2] + nrow + c, az);
+        atomicAdd(g.ch[3] + nrow + c, am);
+      }
+    }
+  }
   return o;
-}
-
-template <int MODE, bool INTERIOR>
-__device__ __forceinline__ ReactOut p2g_react_particle_rmw(bool rmw, const FsiLbmParams& lp, const FsiLbmState& ls,
-                                                           const FsiMpmParams& mp, const FsiMpmState& ms,
-                                                           const FsiCouplingParams& cp, long long q,
-                                                           const Stencil& sn, float4* tile, const TileBox& b) {
-  return rmw ? p2g_react_particle<MODE, INTERIOR, true>(lp, ls, mp, ms, cp, q, sn, tile, b)
-             : p2g_react_particle<MODE, INTERIOR, false>(lp, ls, mp, ms, cp, q, sn, tile, b);
 }
 
 // Flush of the fused tile: the copies are summed per node, then one global
@@ -822,12 +852,12 @@ __device__ __forceinline__ void tile_flush_fused(const float4* tile, const TileB
 template <int MODE>
 __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
                                                 const FsiMpmState& ms, const FsiCouplingParams& cp, double* rec,
-                                                long long q, bool valid, float* tile, int* sbox, double* sd,
+                                                long long q, bool valid, float* tile, double* sd,
                                                 float* sf, int* si) {
   const long long G = (long long)mp.gnx * mp.gny * mp.gnz;
   const GlobalSink gp = p2g_sink(ms, G), gr = reaction_sink(ms, G);
   const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, mp.inv_dx);
-  const TileBox b = block_tile_box(sn.base, valid, sbox, FUSED_TILE_NODES);
+  const TileBox b = block_tile_box(sn.base, valid, FUSED_TILE_NODES);
   // the reaction samples the lattice with cp.inv_dx_norm; it rides on the P2G
   // stencil when that spacing is the MPM grid's (the drivers' setting), and
   // otherwise scatters on its own stencil into grid_f
@@ -836,32 +866,24 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
   o.active = 0;
   o.particle_norm = o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  int kind = SCATTER_GLOBAL;
+  bool checked = true;
+  float4* mine = reinterpret_cast<float4*>(tile) + (threadIdx.x & (TILE_COPIES - 1)) * b.V;
   if (b.V > 0) {
-    const bool rmw = bases_exclusive(sn, valid, b, tile);
+    kind = bases_exclusive(sn, valid, b, tile) ? SCATTER_TILE_RMW : SCATTER_TILE_ATOMIC;
     tile_clear_n(tile, TILE_COPIES * 4 * b.V);
-    float4* mine = reinterpret_cast<float4*>(tile) + (threadIdx.x & (TILE_COPIES - 1)) * b.V;
     const bool in_grid = b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= mp.gnx && b.ly + b.ey <= mp.gny &&
                          b.lz + b.ez <= mp.gnz;
     const bool in_lat =
         b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= lp.nx && b.ly + b.ey <= lp.ny && b.lz + b.ez <= lp.nz;
-    if (valid) {
-      if (MODE && !same) {
-        if (in_grid)
-          p2g_react_particle_rmw<0, true>(rmw, lp, ls, mp, ms, cp, q, sn, mine, b);
-        else
-          p2g_react_particle_rmw<0, false>(rmw, lp, ls, mp, ms, cp, q, sn, mine, b);
-        o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, gr);
-      } else if (in_grid && (MODE == 0 || in_lat)) {
-        o = p2g_react_particle_rmw<MODE, true>(rmw, lp, ls, mp, ms, cp, q, sn, mine, b);
-      } else {
-        o = p2g_react_particle_rmw<MODE, false>(rmw, lp, ls, mp, ms, cp, q, sn, mine, b);
-      }
-    }
-    tile_flush_fused(reinterpret_cast<const float4*>(tile), b, gp, mp.gnx, mp.gny, mp.gnz);
-  } else if (valid) {
-    mpm_p2g_particle(mp, ms, q, sn, gp);
-    if (MODE) o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, gr);
+    checked = !(in_grid && (!(MODE && same) || in_lat));
   }
+  if (valid) {
+    o = p2g_react_particle<MODE>(lp, ls, mp, ms, cp, q, sn, same, checked, kind, mine, b, gp);
+    // a reaction on another spacing scatters on its own stencil into grid_f
+    if (MODE && !same) o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, gr);
+  }
+  if (b.V > 0) tile_flush_fused(reinterpret_cast<const float4*>(tile), b, gp, mp.gnx, mp.gny, mp.gnz);
   if (MODE) {
     const int act = block_sum_i(o.active, si);
     const float mpn = block_max_f(o.particle_norm, sf);
@@ -883,12 +905,11 @@ template <int MODE>  // 0: no reaction, 1: penalty reaction, 2: moving-boundary 
 __global__ void __launch_bounds__(PTPB) k_mpm_p2g_react(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
                                                         FsiMpmState ms, FsiCouplingParams cp, double* rec) {
   __shared__ __align__(16) float tile[TILE_COPIES * 4 * FUSED_TILE_NODES];
-  __shared__ int sbox[6];
   __shared__ double sd[PTPB / 64];
   __shared__ float sf[PTPB / 64];
   __shared__ int si[PTPB / 64];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
-  p2g_react_block<MODE>(lp, ls, mp, ms, cp, rec + (size_t)blockIdx.x * REACT_REC, q, q < mp.n_particles, tile, sbox,
+  p2g_react_block<MODE>(lp, ls, mp, ms, cp, rec + (size_t)blockIdx.x * REACT_REC, q, q < mp.n_particles, tile,
                         sd, sf, si);
 }
 
@@ -1085,9 +1106,8 @@ __global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* ins
 
 __global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
-  __shared__ int sbox[6];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
-  proj_block(I.lp, I.ls, I.mp, I.ms, I.cp, e_, in_, tile, sbox);
+  proj_block(I.lp, I.ls, I.mp, I.ms, I.cp, e_, in_, tile);
 }
 
 __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance* inst, const int32_t* bmap) {
@@ -1152,9 +1172,8 @@ __global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst
 
 __global__ void __launch_bounds__(PTPB) kb_mpm_p2g(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ float tile[TILE_COPIES * 4 * TILE_NODES];
-  __shared__ int sbox[6];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
-  p2g_block(I.mp, I.ms, e_, in_, tile, sbox);
+  p2g_block(I.mp, I.ms, e_, in_, tile);
 }
 
 __global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* inst, const int32_t* bmap) {
@@ -1162,11 +1181,10 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* 
   __shared__ float sf[PTPB / 64];
   __shared__ int si[PTPB / 64];
   __shared__ float tile[TILE_COPIES * 3 * TILE_NODES];
-  __shared__ int sbox[6];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   const int mode = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1;
   const ReactOut o = reaction_block(I.lp, I.ls, I.mp, I.ms, I.cp, mode, e_, in_ && I.mode != FSI_BATCH_MODE_NONE,
-                                    tile, sbox);
+                                    tile);
   const int act = block_sum_i(o.active, si);
   const float mpn = block_max_f(o.particle_norm, sf);
   const float mgc = block_max_f(o.max_contrib, sf);
@@ -1184,18 +1202,17 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* 
 
 __global__ void __launch_bounds__(PTPB) kb_mpm_p2g_react(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ __align__(16) float tile[TILE_COPIES * 4 * FUSED_TILE_NODES];
-  __shared__ int sbox[6];
   __shared__ double sd[PTPB / 64];
   __shared__ float sf[PTPB / 64];
   __shared__ int si[PTPB / 64];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   double* rec = I.react_ws + (size_t)l_ * REACT_REC;
   if (I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY)
-    p2g_react_block<2>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sbox, sd, sf, si);
+    p2g_react_block<2>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sd, sf, si);
   else if (I.mode == FSI_BATCH_MODE_PENALTY)
-    p2g_react_block<1>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sbox, sd, sf, si);
+    p2g_react_block<1>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sd, sf, si);
   else
-    p2g_react_block<0>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sbox, sd, sf, si);
+    p2g_react_block<0>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sd, sf, si);
 }
 
 __global__ void __launch_bounds__(TPB) kb_mpm_grid_update(const FsiBatchInstance* inst, const int32_t* bmap) {
