@@ -1119,6 +1119,36 @@ FSI_HD void rotation_and_deviation(const float* Fm, float* R, float* D) {
     }
 }
 
+// Three consecutive floats p[0..2] (a stencil row along k) as one load.
+struct Row3 {
+  float v[3];
+};
+FSI_HD Row3 load_row3(const float* p) {
+  Row3 r;
+  __builtin_memcpy(&r, p, sizeof(r));
+  return r;
+}
+
+// Start k0 of the 3-float row load covering cells K0..K0+2 along an axis of
+// n >= 3 cells, kept inside [0, n - 3].
+FSI_HD int row_start(int K0, int n) { return K0 < 0 ? 0 : (K0 > n - 3 ? n - 3 : K0); }
+// Cells K0, K0 + 1, K0 + 2 of a row loaded from k0 = K0 + d: cell K0 + c is
+// element c - d, and a cell outside the loaded row (c - d outside 0..2)
+// comes back as 0, to be masked off by the caller.  A bitwise select: no
+// branch, and the values are the loaded ones bit for bit.
+FSI_HD Row3 row_shift(const Row3& r, int d) {
+  uint32_t u[3];
+  __builtin_memcpy(u, r.v, sizeof(u));
+  Row3 o;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int e = c - d;
+    const uint32_t b = (u[0] & (e == 0 ? ~0u : 0u)) | (u[1] & (e == 1 ? ~0u : 0u)) | (u[2] & (e == 2 ? ~0u : 0u));
+    __builtin_memcpy(&o.v[c], &b, sizeof(b));
+  }
+  return o;
+}
+
 FSI_HD void load_mat(const float* base, long long P, long long p, float* A) {
 #pragma unroll
   for (int e = 0; e < 9; ++e) A[e] = base[e * P + p];
@@ -1160,23 +1190,39 @@ FSI_HD GlobalSink reaction_sink(const FsiMpmState& ms, long long G) {
   return k;
 }
 
-// Per-particle terms of p2g (solid.py:220-249): node momentum is
-// wt * (mv + aff * (x_node - x_p)), node mass wt * mass.  Writes Jp[p] = det F.
+// Per-particle terms of p2g (solid.py:220-249).
 struct P2gTerms {
   float aff[9];
   float mv[3];
   float mass;
+  float J;  // det F
 };
 
-FSI_HD P2gTerms p2g_terms(const FsiMpmParams& mp, const FsiMpmState& ms, long long p) {
-  const long long P = mp.n_particles;
+// The particle state p2g reads.
+struct P2gIn {
+  float F[9], C[9], v[3], mass, vol0;
+};
+
+FSI_HD P2gIn p2g_load(const FsiMpmState& ms, long long P, long long p) {
+  P2gIn in;
+  load_mat(ms.F, P, p, in.F);
+  load_mat(ms.C, P, p, in.C);
+  in.v[0] = ms.v[p];
+  in.v[1] = ms.v[P + p];
+  in.v[2] = ms.v[2 * P + p];
+  in.mass = ms.mass[p];
+  in.vol0 = ms.vol0[p];
+  return in;
+}
+
+FSI_HD P2gTerms p2g_terms_of(const FsiMpmParams& mp, const P2gIn& in) {
   P2gTerms t;
-  float Fm[9], Cm[9], R[9], Dv[9];
-  load_mat(ms.F, P, p, Fm);
-  load_mat(ms.C, P, p, Cm);
+  const float* Fm = in.F;
+  const float* Cm = in.C;
+  float R[9], Dv[9];
   rotation_and_deviation(Fm, R, Dv);
   const float J = det3(Fm);
-  ms.Jp[p] = J;
+  t.J = J;
   // F^{-T} = cofactor(F) / det F
   float FiT[9];
   FiT[0] = (Fm[4] * Fm[8] - Fm[5] * Fm[7]) / J;
@@ -1192,8 +1238,8 @@ FSI_HD P2gTerms p2g_terms(const FsiMpmParams& mp, const FsiMpmState& ms, long lo
   const float lj = mp.la * (J - 1.0f) * J;
 #pragma unroll
   for (int e = 0; e < 9; ++e) Pk[e] = 2.0f * mp.mu * Dv[e] + lj * FiT[e];
-  const float mass = ms.mass[p];
-  const float sc = -mp.dt * ms.vol0[p];
+  const float mass = in.mass;
+  const float sc = -mp.dt * in.vol0;
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -1203,9 +1249,17 @@ FSI_HD P2gTerms p2g_terms(const FsiMpmParams& mp, const FsiMpmState& ms, long lo
       t.aff[r * 3 + c] = sc * pf * mp.inv_dx * mp.inv_dx + mass * Cm[r * 3 + c];
     }
   t.mass = mass;
-  t.mv[0] = mass * ms.v[p];
-  t.mv[1] = mass * ms.v[P + p];
-  t.mv[2] = mass * ms.v[2 * P + p];
+  t.mv[0] = mass * in.v[0];
+  t.mv[1] = mass * in.v[1];
+  t.mv[2] = mass * in.v[2];
+  return t;
+}
+
+// Per-particle terms of p2g (solid.py:220-249): node momentum is
+// wt * (mv + aff * (x_node - x_p)), node mass wt * mass.  Writes Jp[p] = det F.
+FSI_HD P2gTerms p2g_terms(const FsiMpmParams& mp, const FsiMpmState& ms, long long p) {
+  const P2gTerms t = p2g_terms_of(mp, p2g_load(ms, mp.n_particles, p));
+  ms.Jp[p] = t.J;
   return t;
 }
 
@@ -1281,33 +1335,49 @@ FSI_HD void mpm_g2p_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
   load_mat(ms.F, P, p, Fm);
   float nv[3] = {0.0f, 0.0f, 0.0f};
   float nC[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  // One plane of 9 nodes at a time, loads first (a node outside the grid
-  // reads node 0 and is then skipped): one round trip per plane.
+  // Loads first, then the sums in node order: 9 rows of 3 nodes along k, one
+  // 3-float load per row and channel from a row start kept inside the grid
+  // (a row outside in i or j reads row 0), nodes outside flagged off.
+  const int k0 = row_start(base[2], g.nz);
+  const int dk = k0 - base[2];
+  Row3 rv[9][3];
+  bool row_ok[9];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    float gv[9][3];
-    bool in[9];
+  for (int r = 0; r < 9; ++r) {
+    const int I = base[0] + r / 3, Jn = base[1] + r % 3;
+    row_ok[r] = 0 <= I && I < g.nx && 0 <= Jn && Jn < g.ny;
+    const long long n0 = row_ok[r] ? ((long long)I * g.ny + Jn) * g.nz + k0 : 0;
+    rv[r][0] = load_row3(ms.grid_v + n0);
+    rv[r][1] = load_row3(ms.grid_v + g.G + n0);
+    rv[r][2] = load_row3(ms.grid_v + 2 * g.G + n0);
+  }
+  float gv[27][3];
+  uint32_t inm = 0u;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) {
-      const int I = base[0] + a, Jn = base[1] + e / 3, K = base[2] + e % 3;
-      in[e] = inside3(I, Jn, K, g.nx, g.ny, g.nz);
-      const long long n = in[e] ? ((long long)I * g.ny + Jn) * g.nz + K : 0;
-      gv[e][0] = ms.grid_v[n];
-      gv[e][1] = ms.grid_v[g.G + n];
-      gv[e][2] = ms.grid_v[2 * g.G + n];
+  for (int r = 0; r < 9; ++r) {
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const Row3 v = row_shift(rv[r][ch], dk);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) gv[3 * r + c][ch] = v.v[c];
     }
 #pragma unroll
-    for (int e = 0; e < 9; ++e) {
-      if (!in[e]) continue;
-      const int b = e / 3, cc = e % 3;
-      const float dp[3] = {((float)a - fx[0]) * mp.dx, ((float)b - fx[1]) * mp.dx, ((float)cc - fx[2]) * mp.dx};
-      const float wt = w[0][a] * w[1][b] * w[2][cc];
+    for (int c = 0; c < 3; ++c) {
+      const int K = base[2] + c;
+      inm |= (row_ok[r] && 0 <= K && K < g.nz) ? (1u << (3 * r + c)) : 0u;
+    }
+  }
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        nv[r] += wt * gv[e][r];
+  for (int k = 0; k < 27; ++k) {
+    if (!((inm >> k) & 1u)) continue;
+    const int a = k / 9, b = (k / 3) % 3, cc = k % 3;
+    const float dp[3] = {((float)a - fx[0]) * mp.dx, ((float)b - fx[1]) * mp.dx, ((float)cc - fx[2]) * mp.dx};
+    const float wt = w[0][a] * w[1][b] * w[2][cc];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) nC[r * 3 + c] += 4.0f * wt * gv[e][r] * dp[c] * mp.inv_dx * mp.inv_dx;
-      }
+    for (int r = 0; r < 3; ++r) {
+      nv[r] += wt * gv[k][r];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) nC[r * 3 + c] += 4.0f * wt * gv[k][r] * dp[c] * mp.inv_dx * mp.inv_dx;
     }
   }
   float x[3], v[3];
@@ -1498,55 +1568,76 @@ struct ReactOut {
   float f[3];
 };
 
-// Reaction force on particle p: mode 1 = penalty (penalty.py:90-129), mode 2 =
-// moving boundary (moving_boundary.py:58-111).  Gathers hydro_force (and, for
-// mode 2, solid_phi) at the stencil's lattice cells; o.f is the particle's
-// reaction force, zero for a mode-2 particle outside the solid.  checked =
-// false: the caller knows every stencil cell is inside the lattice.
-FSI_HD ReactOut reaction_force(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmState& ms,
-                               const FsiCouplingParams& cp, int mode, long long p, const Stencil& sn,
-                               bool checked = true) {
+// Lattice values the reaction force reads at the 27 stencil cells: 9 rows
+// (a-major, then b) of 3 cells along k, each row one 3-float load per array
+// (hydro_force x, y, z and, for mode 2, solid_phi) from a start kept inside
+// the lattice; cell K0 + c of a row is element c of row_shift(row, d).  `in`
+// flags the cells inside the lattice (bit 3 row + c).
+struct HydroGather {
+  Row3 h[9][4];
+  int d;
+  uint32_t in;
+};
+
+// Issues the loads and returns at once: no loaded value is used here, so the
+// loads stay in flight while the caller works until reaction_from_gather.
+// checked = false: the caller knows every stencil cell is inside the lattice.
+FSI_HD void hydro_gather_load(const FsiLbmParams& lp, const FsiLbmState& ls, int mode, const Stencil& sn,
+                              bool checked, HydroGather& G) {
   const Grid g = make_grid(lp);
+  const int* base = sn.base;
+  const int k0 = checked ? row_start(base[2], g.nz) : base[2];
+  G.d = k0 - base[2];
+  G.in = 0u;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int I = base[0] + r / 3, Jn = base[1] + r % 3;
+    const bool row_ok = !checked || (0 <= I && I < g.nx && 0 <= Jn && Jn < g.ny);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int K = base[2] + c;
+      G.in |= (row_ok && (!checked || (0 <= K && K < g.nz))) ? (1u << (3 * r + c)) : 0u;
+    }
+    const long long c0 = row_ok ? cell_of(g, I, Jn, k0) : 0;  // a row outside reads row 0
+    G.h[r][0] = load_row3(ls.hydro_force + c0);
+    G.h[r][1] = load_row3(ls.hydro_force + g.N + c0);
+    G.h[r][2] = load_row3(ls.hydro_force + 2 * g.N + c0);
+    if (mode == 2) G.h[r][3] = load_row3(ls.solid_phi + c0);
+    else G.h[r][3].v[0] = G.h[r][3].v[1] = G.h[r][3].v[2] = 0.0f;
+  }
+}
+
+// Reaction force on a particle of deformation J and rest volume vol0 from its
+// gathered lattice values: mode 1 = penalty (penalty.py:90-129), mode 2 =
+// moving boundary (moving_boundary.py:58-111).  o.f is the particle's
+// reaction force, zero for a mode-2 particle outside the solid.
+FSI_HD ReactOut reaction_from_gather(const FsiCouplingParams& cp, int mode, float J, float vol0, const Stencil& sn,
+                                     const HydroGather& G) {
   ReactOut o;
   o.active = 0;
   o.particle_norm = 0.0f;
   o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
-  const int* base = sn.base;
   const float(&w)[3][3] = sn.w;
   float hx = 0.0f, hy = 0.0f, hz = 0.0f, sphi = 0.0f;
-  // One plane of 9 cells at a time: the loads of a plane carry no branch (a
-  // cell outside the lattice reads cell 0 and is then skipped), so they are
-  // all in flight together instead of one round trip per cell.
-#pragma unroll 1
-  for (int a = 0; a < 3; ++a) {
-    float h[9][4];
-    bool in[9];
-    const float wa = a == 0 ? w[0][0] : (a == 1 ? w[0][1] : w[0][2]);
 #pragma unroll
-    for (int e = 0; e < 9; ++e) {
-      const int I = base[0] + a, Jn = base[1] + e / 3, K = base[2] + e % 3;
-      in[e] = !checked || inside3(I, Jn, K, g.nx, g.ny, g.nz);
-      const long long c = in[e] ? cell_of(g, I, Jn, K) : 0;
-      h[e][0] = ls.hydro_force[c];
-      h[e][1] = ls.hydro_force[g.N + c];
-      h[e][2] = ls.hydro_force[2 * g.N + c];
-      h[e][3] = mode == 2 ? ls.solid_phi[c] : 0.0f;
-    }
+  for (int r = 0; r < 9; ++r) {
+    const Row3 rx = row_shift(G.h[r][0], G.d), ry = row_shift(G.h[r][1], G.d), rz = row_shift(G.h[r][2], G.d),
+               rp = row_shift(G.h[r][3], G.d);
 #pragma unroll
-    for (int e = 0; e < 9; ++e) {
-      if (!in[e]) continue;
-      const float wt = wa * w[1][e / 3] * w[2][e % 3];
-      hx += wt * h[e][0];
-      hy += wt * h[e][1];
-      hz += wt * h[e][2];
-      if (mode == 2) sphi += wt * h[e][3];
+    for (int c = 0; c < 3; ++c) {
+      const int k = 3 * r + c;
+      if (!((G.in >> k) & 1u)) continue;
+      const float wt = w[0][k / 9] * w[1][(k / 3) % 3] * w[2][k % 3];
+      hx += wt * rx.v[c];
+      hy += wt * ry.v[c];
+      hz += wt * rz.v[c];
+      if (mode == 2) sphi += wt * rp.v[c];
     }
   }
   if (mode == 2 && !(sphi > cp.mb_phi_min)) return o;
-  float J = ms.Jp[p];
   J = J > 0.0f ? J : 0.0f;
-  const float vol = ms.vol0[p] * J;
+  const float vol = vol0 * J;
   float pf[3] = {hx * cp.force_density_scale * vol * cp.reaction_scale,
                  hy * cp.force_density_scale * vol * cp.reaction_scale,
                  hz * cp.force_density_scale * vol * cp.reaction_scale};
@@ -1564,6 +1655,16 @@ FSI_HD ReactOut reaction_force(const FsiLbmParams& lp, const FsiLbmState& ls, co
   o.f[1] = pf[1];
   o.f[2] = pf[2];
   return o;
+}
+
+// Reaction force on particle p (its Jp as p2g left it).  checked = false: the
+// caller knows every stencil cell is inside the lattice.
+FSI_HD ReactOut reaction_force(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmState& ms,
+                               const FsiCouplingParams& cp, int mode, long long p, const Stencil& sn,
+                               bool checked = true) {
+  HydroGather G;
+  hydro_gather_load(lp, ls, mode, sn, checked, G);
+  return reaction_from_gather(cp, mode, ms.Jp[p], ms.vol0[p], sn, G);
 }
 
 // Reaction transfer for particle p: the force, then its scatter to the MPM

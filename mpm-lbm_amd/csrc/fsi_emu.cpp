@@ -183,6 +183,8 @@ int fsi_mpm_clear_grid(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
 }
 
 int fsi_mpm_p2g(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
+  if (mp->gnx < 3 || mp->gny < 3 || mp->gnz < 3)
+    return fail(FSI_E_INVALID, "MPM grid must be at least 3 nodes along every axis");
   const GlobalSink sink = p2g_sink(*ms, (long long)mp->gnx * mp->gny * mp->gnz);
   for (long long q = 0; q < mp->n_particles; ++q) mpm_p2g_particle(*mp, *ms, q, sink);
   return FSI_OK;
@@ -225,6 +227,8 @@ int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
 }
 
 int fsi_mpm_g2p(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
+  if (mp->gnx < 3 || mp->gny < 3 || mp->gnz < 3)
+    return fail(FSI_E_INVALID, "MPM grid must be at least 3 nodes along every axis");
   for (long long q = 0; q < mp->n_particles; ++q) mpm_g2p_particle(*mp, *ms, q);
   return FSI_OK;
 }

@@ -79,7 +79,8 @@ int check_lbm(const FsiLbmParams* p, const FsiLbmState* st) {
 
 int check_mpm(const FsiMpmParams* mp, const FsiMpmState* ms) {
   if (!mp || !ms) return fail(FSI_E_INVALID, "null MPM params/state");
-  if (mp->gnx < 1 || mp->gny < 1 || mp->gnz < 1) return fail(FSI_E_INVALID, "empty MPM grid");
+  if (mp->gnx < 3 || mp->gny < 3 || mp->gnz < 3)
+    return fail(FSI_E_INVALID, "MPM grid must be at least 3 nodes along every axis");
   if (mp->n_particles < 0) return fail(FSI_E_INVALID, "negative particle count");
   if (!ms->x || !ms->v || !ms->C || !ms->F || !ms->Jp || !ms->mass || !ms->vol0 || !ms->grid_v ||
       !ms->grid_m || !ms->grid_f)
@@ -741,35 +742,25 @@ enum FusedScatter {
 
 __device__ __forceinline__ float pick3(int i, const float (&v)[3]) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
 
-// P2G of particle q (+ for MODE 1 penalty / 2 moving boundary with `fold`, the
-// reaction force, added into the momentum) with one scatter of 4 floats per
-// stencil node.  `checked` = false: every stencil node of the workgroup is
-// inside the MPM grid and (fold) the lattice, so nothing is tested; otherwise
-// the gather tests each lattice cell, and a node outside the grid lands in the
-// tile's margin (which the flush skips) or, for the global scatter, is
-// dropped.  The flags are uniform over the workgroup; the loops stay rolled
-// over the (a, e) node rows so the kernel's code stays small.
-template <int MODE>
-__device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
-                                                       const FsiMpmParams& mp, const FsiMpmState& ms,
-                                                       const FsiCouplingParams& cp, long long q, const Stencil& sn,
-                                                       bool fold, bool checked, int kind, float4* tile,
-                                                       const TileBox& b, const GlobalSink& g) {
-  ReactOut o;
-  o.active = 0;
-  o.particle_norm = o.max_contrib = 0.0f;
-  o.f[0] = o.f[1] = o.f[2] = 0.0f;
-  const P2gTerms t = p2g_terms(mp, ms, q);
-  float mv[3] = {t.mv[0], t.mv[1], t.mv[2]};
-  if (MODE && fold) {
-    o = reaction_force(lp, ls, ms, cp, MODE, q, sn, checked);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) mv[d] += mp.dt * o.f[d];
-  }
+// The 27-node scatter of p2g_react_particle for one scatter kind: node (a, e,
+// c) receives wt * (m + aff (x_node - x_p)) and wt * mass.  Returns the
+// largest weight of a node inside the grid.
+template <int KIND>
+__device__ __forceinline__ float fused_scatter(const FsiMpmParams& mp, const Stencil& sn, const P2gTerms& t,
+                                               const float (&mv)[3], bool checked, float4* tile, const TileBox& b,
+                                               const GlobalSink& g) {
   const int* base = sn.base;
   const float* fx = sn.fx;
   const float(&w)[3][3] = sn.w;
   const int gnx = mp.gnx, gny = mp.gny, gnz = mp.gnz;
+  float wmax = 0.0f;
+  float dpz[3];
+  bool zin[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    dpz[c] = ((float)c - fx[2]) * mp.dx;
+    zin[c] = !checked || (0 <= base[2] + c && base[2] + c < gnz);
+  }
 #pragma unroll 1
   for (int r = 0; r < 9; ++r) {
     const int a = r / 3, e = r - 3 * (r / 3);
@@ -786,17 +777,13 @@ __device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, c
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float wt = wae * w[2][c];
-      const float dpz = ((float)c - fx[2]) * mp.dx;
-      const float ax = wt * (bx + t.aff[2] * dpz);
-      const float ay = wt * (by + t.aff[5] * dpz);
-      const float az = wt * (bz + t.aff[8] * dpz);
+      const float ax = wt * (bx + t.aff[2] * dpz[c]);
+      const float ay = wt * (by + t.aff[5] * dpz[c]);
+      const float az = wt * (bz + t.aff[8] * dpz[c]);
       const float am = wt * t.mass;
-      const bool in = row_in && (!checked || (0 <= base[2] + c && base[2] + c < gnz));
-      if (MODE && fold && in) {
-        const float cn = norm3(wt * o.f[0], wt * o.f[1], wt * o.f[2]);
-        if (cn > o.max_contrib) o.max_contrib = cn;
-      }
-      if (kind == SCATTER_TILE_RMW) {
+      const bool in = row_in && zin[c];
+      wmax = in && wt > wmax ? wt : wmax;
+      if (KIND == SCATTER_TILE_RMW) {
         float4 v = row[c];
         v.x += ax;
         v.y += ay;
@@ -804,7 +791,7 @@ __device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, c
         v.w += am;
         row[c] = v;
         wave_order();
-      } else if (kind == SCATTER_TILE_ATOMIC) {
+      } else if (KIND == SCATTER_TILE_ATOMIC) {
         float* f = reinterpret_cast<float*>(row + c);
         __hip_atomic_fetch_add(f + 0, ax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         __hip_atomic_fetch_add(f + 1, ay, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -817,6 +804,55 @@ __device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, c
         atomicAdd(g.ch[3] + nrow + c, am);
       }
     }
+  }
+  return wmax;
+}
+
+// P2G of particle q (+ for MODE 1 penalty / 2 moving boundary with `fold`, the
+// reaction force, added into the momentum) with one scatter of 4 floats per
+// stencil node.  `checked` = false: every stencil node of the workgroup is
+// inside the MPM grid and (fold) the lattice, so nothing is tested; otherwise
+// the gather tests each lattice cell, and a node outside the grid lands in the
+// tile's margin (which the flush skips) or, for the global scatter, is
+// dropped.  `kind` and `checked` are uniform over the workgroup.
+//
+// The largest node contribution norm3(wt * f) over the nodes inside the grid
+// is norm3(wmax * f) for the largest such weight wmax: each rounded step
+// (product, squares, sum, square root) is monotonic in wt.
+//
+// The particle's state `in` was loaded at the start of the kernel, and the
+// lattice gather is issued before the polar decomposition: both memory round
+// trips overlap work instead of following it.
+template <int MODE>
+__device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
+                                                       const FsiMpmParams& mp, const FsiMpmState& ms,
+                                                       const FsiCouplingParams& cp, long long q, const P2gIn& in,
+                                                       const Stencil& sn, bool fold, bool checked, int kind,
+                                                       float4* tile, const TileBox& b, const GlobalSink& g) {
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  HydroGather G;
+  if (MODE && fold) hydro_gather_load(lp, ls, MODE, sn, checked, G);
+  const P2gTerms t = p2g_terms_of(mp, in);
+  ms.Jp[q] = t.J;
+  float mv[3] = {t.mv[0], t.mv[1], t.mv[2]};
+  if (MODE && fold) {
+    o = reaction_from_gather(cp, MODE, t.J, in.vol0, sn, G);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) mv[d] += mp.dt * o.f[d];
+  }
+  float wmax;
+  if (kind == SCATTER_TILE_RMW)
+    wmax = fused_scatter<SCATTER_TILE_RMW>(mp, sn, t, mv, checked, tile, b, g);
+  else if (kind == SCATTER_TILE_ATOMIC)
+    wmax = fused_scatter<SCATTER_TILE_ATOMIC>(mp, sn, t, mv, checked, tile, b, g);
+  else
+    wmax = fused_scatter<SCATTER_GLOBAL>(mp, sn, t, mv, checked, tile, b, g);
+  if (MODE && fold) {
+    const float cn = norm3(wmax * o.f[0], wmax * o.f[1], wmax * o.f[2]);
+    o.max_contrib = cn > 0.0f ? cn : 0.0f;
   }
   return o;
 }
@@ -856,7 +892,12 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
                                                 float* sf, int* si) {
   const long long G = (long long)mp.gnx * mp.gny * mp.gnz;
   const GlobalSink gp = p2g_sink(ms, G), gr = reaction_sink(ms, G);
-  const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, mp.inv_dx);
+  // x, then the particle's p2g inputs, without a branch (an idle lane reads
+  // particle 0), so the stencil waits for x alone and the rest stays in flight
+  const long long qs = valid ? q : 0;
+  Stencil sn = particle_stencil(ms.x, mp.n_particles, qs, mp.inv_dx);
+  const P2gIn in = p2g_load(ms, mp.n_particles, qs);
+  if (!valid) sn.base[0] = sn.base[1] = sn.base[2] = 0;
   const TileBox b = block_tile_box(sn.base, valid, FUSED_TILE_NODES);
   // the reaction samples the lattice with cp.inv_dx_norm; it rides on the P2G
   // stencil when that spacing is the MPM grid's (the drivers' setting), and
@@ -878,8 +919,11 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
         b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= lp.nx && b.ly + b.ey <= lp.ny && b.lz + b.ez <= lp.nz;
     checked = !(in_grid && (!(MODE && same) || in_lat));
   }
+  // uniform over the wave: scalar branches
+  kind = __builtin_amdgcn_readfirstlane(kind);
+  checked = __builtin_amdgcn_readfirstlane((int)checked) != 0;
   if (valid) {
-    o = p2g_react_particle<MODE>(lp, ls, mp, ms, cp, q, sn, same, checked, kind, mine, b, gp);
+    o = p2g_react_particle<MODE>(lp, ls, mp, ms, cp, q, in, sn, same, checked, kind, mine, b, gp);
     // a reaction on another spacing scatters on its own stencil into grid_f
     if (MODE && !same) o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, gr);
   }

@@ -29,6 +29,19 @@ def test_moving_boundary_subcycled(emu, tmp_path):
     assert mbs["bb_max_correction"] == pytest.approx(o.mbstats.max_correction, rel=1e-5, abs=1e-12)
 
 
+def test_moving_boundary_flap_spanning_z(emu, tmp_path):
+    """A flap across the whole z extent: particle stencils leave the lattice
+    and the MPM grid through both z faces, so the row gathers read shifted
+    rows and mask the cells outside."""
+    w = workloads.moving_boundary_flap(n=16, steps=2, substeps=4, flap=(2, 6, 16), mpm_substeps=2,
+                                       out_dir=str(tmp_path))
+    x = np.asarray(w.cloud["x"]) * 16
+    base = np.floor(x[:, 2] - 0.5).astype(int)
+    assert base.min() == -1 and base.max() == 15  # shifts of +1, -1 and -2 against the 16-cell lattice
+    d, o = make_pair(w, tmp_path / "run", emu, "cpu")
+    _check(run_and_compare(d, o, 2))
+
+
 def test_moving_boundary_one_step_per_fsi_step(emu, tmp_path):
     w = workloads.moving_boundary_unsubcycled(n=16, steps=3, flap=(2, 6, 6), out_dir=str(tmp_path))
     d, o = make_pair(w, tmp_path / "run", emu, "cpu")
