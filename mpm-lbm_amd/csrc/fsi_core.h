@@ -127,10 +127,24 @@ static inline float f16_to_f32(uint16_t h) {
 // the plane base (uniform: buffer + s * N) stays in scalar registers and each
 // lane adds one zero-extended 32-bit offset, which gfx950 global loads and
 // stores take directly (saddr + voffset form).
+//
+// Every plane lives in device global memory.  On the GPU the access is made
+// through a global-address-space pointer, so a plane base the kernel read
+// from memory (the batched kernels' instance table) still gets global
+// loads and stores instead of flat ones.
+#if defined(__HIP_DEVICE_COMPILE__)
 template <typename T>
-FSI_HD T& at32(T* plane, uint32_t c) {
+__device__ __forceinline__ __attribute__((address_space(1))) T& at32(T* plane, uint32_t c) {
+  typedef __attribute__((address_space(1))) T GT;
+  typedef __attribute__((address_space(1))) const char GC;
+  return *(GT*)((GC*)plane + (size_t)(uint32_t)(c * (uint32_t)sizeof(T)));
+}
+#else
+template <typename T>
+inline T& at32(T* plane, uint32_t c) {
   return *(T*)((const char*)plane + (size_t)(uint32_t)(c * (uint32_t)sizeof(T)));
 }
+#endif
 
 struct PopF32 {
   float* p;

@@ -1042,8 +1042,10 @@ __global__ void __launch_bounds__(TPB) kb_substep(const FsiBatchInstance* inst, 
   const unsigned b_ = xcd_tile(blockIdx.x, gridDim.x);
   const int i_ = bmap[2 * b_], l_ = bmap[2 * b_ + 1];
   const FsiBatchInstance& I = inst[i_];
-  const FsiLbmParams p = I.lp;
-  const FsiLbmState st = I.ls;
+  // references into the instance table, not local copies: a copy indexed at
+  // run time (F[cur], link_c[parity]) would live in scratch memory
+  const FsiLbmParams& p = I.lp;
+  const FsiLbmState& st = I.ls;
   // the single-instance launch geometry: l_ = i * tiles + tile of the (j,k) plane
   const unsigned plane = (unsigned)p.ny * (unsigned)p.nz, tiles = (plane + TPB - 1) / TPB;
   const unsigned i = (unsigned)l_ / tiles;
