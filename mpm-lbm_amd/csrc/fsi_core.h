@@ -460,7 +460,12 @@ FSI_HD bool bc_face(const FsiLbmParams& p, int i) {
 // Atomics / order-independent reductions
 // ---------------------------------------------------------------------------
 #ifdef __HIPCC__
-__device__ __forceinline__ void atomic_add_f32(float* a, float v) { atomicAdd(a, v); }
+// a lands in device global memory (grid and lattice arrays): addressed as
+// such, the add is a global atomic even when the pointer was read from memory
+__device__ __forceinline__ void atomic_add_f32(float* a, float v) {
+  typedef __attribute__((address_space(1))) float GF;
+  __hip_atomic_fetch_add((GF*)a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ void atomic_add_f64(double* a, double v) { atomicAdd(a, v); }
 __device__ __forceinline__ void atomic_max_nonneg(uint32_t* a, float v) {
   atomicMax(a, __float_as_uint(v));

@@ -260,7 +260,7 @@ struct LdsSink {
       __hip_atomic_fetch_add(tile + c * b.V + ((int)a * b.ey + (int)e) * b.ez + (int)f, v, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
     else
-      atomicAdd(g.ch[c] + n, v);
+      atomic_add_f32(g.ch[c] + n, v);
   }
 };
 
@@ -278,7 +278,7 @@ __device__ __forceinline__ void tile_flush_ch(const float* tile, const TileBox& 
     for (int c = 0; c < nch; ++c) {
       float v = tile[c * b.V + i];
       for (int k = 1; k < copies; ++k) v += tile[(k * nch + c) * b.V + i];
-      if (v != 0.0f) atomicAdd(ch[c] + n, v);
+      if (v != 0.0f) atomic_add_f32(ch[c] + n, v);
     }
   }
 }
@@ -288,25 +288,177 @@ __device__ __forceinline__ void tile_clear_n(float* tile, int n) {
   __syncthreads();
 }
 
-// Projection scatter of particle q (valid) by a whole workgroup: solid_phi,
-// solid_mass and solid_vel on the lattice.
+// Keeps the compiler from moving LDS accesses across this point; the wave
+// itself issues its LDS instructions in program order.
+__device__ __forceinline__ void wave_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The projection and fused P2G tiles update a node with LDS loads and stores
+// (read-modify-write) instead of one float atomic per channel.  That is exact
+// when the workgroup is one wave and no two of its lanes that share a tile
+// copy have the same stencil base: in any one instruction the lanes of a copy
+// then address distinct nodes, and two updates of one node by different lanes
+// come from different iterations of the 27-node loop, which the wave issues in
+// order.  In a lattice-ordered cloud (8 particles per cell, lane parity = the
+// particle's z half of the cell, which is also its tile copy) the condition
+// always holds; otherwise the workgroup takes the atomic path.
+
+// Whether the read-modify-write scatter is exact for this workgroup.  Scratch:
+// the tile, before it is cleared.
+__device__ __forceinline__ bool bases_exclusive(const Stencil& sn, bool valid, const TileBox& b, float* tile) {
+  int* own = reinterpret_cast<int*>(tile);
+  int key = 0;
+  if (valid) {
+    key = (int)(threadIdx.x & (TILE_COPIES - 1)) * b.V +
+          ((sn.base[0] - b.lx) * b.ey + (sn.base[1] - b.ly)) * b.ez + (sn.base[2] - b.lz);
+    own[key] = (int)threadIdx.x;
+  }
+  __syncthreads();
+  const int clash = valid && own[key] != (int)threadIdx.x;
+  return !__syncthreads_or(clash);
+}
+
+// Where the fused P2G scatter puts a node's 4 floats.
+enum FusedScatter {
+  SCATTER_TILE_RMW = 0,     // LDS tile, read-modify-write (bases exclusive per copy)
+  SCATTER_TILE_ATOMIC = 1,  // LDS tile, float atomics
+  SCATTER_GLOBAL = 2        // grid arrays, float atomics (box too large for a tile)
+};
+
+__device__ __forceinline__ float pick3(int i, const float (&v)[3]) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
+
+// Projection (projection.py:24-64) of the workgroup's particles onto the
+// lattice: solid_phi, solid_mass and solid_vel.  A node's 5 floats sit in the
+// tile as a float4 (phi, mass, momentum x, y) and a float (momentum z), and
+// are updated by read-modify-write when the stencil bases are exclusive per
+// tile copy (as in the fused P2G scatter), else with LDS atomics; a box too
+// large for the tile scatters with global atomics.  The arithmetic per node
+// is project_particle's.
 constexpr int PROJ_TILE_NODES = TILE_NODES;
+
+template <int KIND>
+__device__ __forceinline__ void proj_scatter(const FsiLbmParams& lp, const FsiCouplingParams& cp, const Stencil& sn,
+                                             float vol, float mass, float vx, float vy, float vz, bool checked,
+                                             float4* t4, float* t1, const TileBox& b, const GlobalSink& g) {
+  const int* base = sn.base;
+  const float(&w)[3][3] = sn.w;
+  const int nx = lp.nx, ny = lp.ny, nz = lp.nz;
+  bool zin[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) zin[c] = !checked || (0 <= base[2] + c && base[2] + c < nz);
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const int a = r / 3, e = r - 3 * (r / 3);
+    const float wae = pick3(a, w[0]) * pick3(e, w[1]);
+    const int I = base[0] + a, J = base[1] + e;
+    const bool row_in = !checked || (0 <= I && I < nx && 0 <= J && J < ny);
+    const int row = ((I - b.lx) * b.ey + (J - b.ly)) * b.ez + (base[2] - b.lz);
+    const long long crow = ((long long)I * ny + J) * nz + base[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float wt = wae * w[2][c];
+      const float mc = wt * mass;
+      const float vf = wt * vol / cp.cell_volume_norm;
+      const float p2 = mc * vx, p3 = mc * vy, p4 = mc * vz;
+      if (KIND == SCATTER_TILE_RMW) {
+        float4 v = t4[row + c];
+        float u = t1[row + c];
+        v.x += vf;
+        v.y += mc;
+        v.z += p2;
+        v.w += p3;
+        u += p4;
+        t4[row + c] = v;
+        t1[row + c] = u;
+        wave_order();
+      } else if (KIND == SCATTER_TILE_ATOMIC) {
+        float* f = reinterpret_cast<float*>(t4 + row + c);
+        __hip_atomic_fetch_add(f + 0, vf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(f + 1, mc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(f + 2, p2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(f + 3, p3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(t1 + row + c, p4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else if (row_in && zin[c]) {
+        atomic_add_f32(g.ch[0] + crow + c, vf);
+        atomic_add_f32(g.ch[1] + crow + c, mc);
+        atomic_add_f32(g.ch[2] + crow + c, p2);
+        atomic_add_f32(g.ch[3] + crow + c, p3);
+        atomic_add_f32(g.ch[4] + crow + c, p4);
+      }
+    }
+  }
+}
 
 __device__ __forceinline__ void proj_block(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
                                            const FsiMpmState& ms, const FsiCouplingParams& cp, long long q, bool valid,
                                            float* tile) {
   const GlobalSink g = proj_sink(ls, (long long)lp.nx * lp.ny * lp.nz);
-  const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, cp.inv_dx_norm);
+  const long long P = mp.n_particles;
+  // the particle's data, all loaded at once and without a branch (an idle
+  // lane reads particle 0)
+  const long long qs = valid ? q : 0;
+  Stencil sn = particle_stencil(ms.x, P, qs, cp.inv_dx_norm);
+  float J = ms.Jp[qs];
+  J = J > 0.0f ? J : 0.0f;
+  const float vol = ms.vol0[qs] * J;
+  const float mass = ms.mass[qs];
+  const float vx = ms.v[qs] * cp.vel_scale_norm_to_lbm;
+  const float vy = ms.v[P + qs] * cp.vel_scale_norm_to_lbm;
+  const float vz = ms.v[2 * P + qs] * cp.vel_scale_norm_to_lbm;
+  if (!valid) sn.base[0] = sn.base[1] = sn.base[2] = 0;
   const TileBox b = block_tile_box(sn.base, valid, PROJ_TILE_NODES);
+  int kind = SCATTER_GLOBAL;
+  bool checked = true;
+  float4* t4 = reinterpret_cast<float4*>(tile);
+  float* t1 = reinterpret_cast<float*>(t4 + TILE_COPIES * b.V);
+  const int copy = (int)(threadIdx.x & (TILE_COPIES - 1));
   if (b.V > 0) {
+    kind = bases_exclusive(sn, valid, b, tile) ? SCATTER_TILE_RMW : SCATTER_TILE_ATOMIC;
     tile_clear_n(tile, TILE_COPIES * 5 * b.V);
-    LdsTileSink k;
-    k.tile = lane_copy(tile, b, 5);
-    k.b = b;
-    if (valid) project_particle(lp, ls, mp, ms, cp, q, sn, k);
-    tile_flush_ch(tile, b, g.ch, 5, TILE_COPIES, lp.nx, lp.ny, lp.nz);
-  } else if (valid) {
-    project_particle(lp, ls, mp, ms, cp, q, sn, g);
+    checked = !(b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= lp.nx && b.ly + b.ey <= lp.ny &&
+                b.lz + b.ez <= lp.nz);
+  }
+  kind = __builtin_amdgcn_readfirstlane(kind);
+  checked = __builtin_amdgcn_readfirstlane((int)checked) != 0;
+  if (valid) {
+    float4* m4 = t4 + copy * b.V;
+    float* m1 = t1 + copy * b.V;
+    if (kind == SCATTER_TILE_RMW)
+      proj_scatter<SCATTER_TILE_RMW>(lp, cp, sn, vol, mass, vx, vy, vz, checked, m4, m1, b, g);
+    else if (kind == SCATTER_TILE_ATOMIC)
+      proj_scatter<SCATTER_TILE_ATOMIC>(lp, cp, sn, vol, mass, vx, vy, vz, checked, m4, m1, b, g);
+    else
+      proj_scatter<SCATTER_GLOBAL>(lp, cp, sn, vol, mass, vx, vy, vz, checked, m4, m1, b, g);
+  }
+  if (b.V > 0) {
+    // flush: the copies summed per node, one global atomic per touched node
+    // and channel
+    __syncthreads();
+    const int plane = b.ey * b.ez;
+    for (int i = threadIdx.x; i < b.V; i += (int)blockDim.x) {
+      const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
+      const int I = b.lx + a, Jn = b.ly + e, K = b.lz + f;
+      if (I < 0 || Jn < 0 || K < 0 || I >= lp.nx || Jn >= lp.ny || K >= lp.nz) continue;
+      const long long n = ((long long)I * lp.ny + Jn) * lp.nz + K;
+      float4 v = t4[i];
+      float u = t1[i];
+      for (int k = 1; k < TILE_COPIES; ++k) {
+        const float4 o = t4[k * b.V + i];
+        v.x += o.x;
+        v.y += o.y;
+        v.z += o.z;
+        v.w += o.w;
+        u += t1[k * b.V + i];
+      }
+      if (v.x != 0.0f) atomic_add_f32(g.ch[0] + n, v.x);
+      if (v.y != 0.0f) atomic_add_f32(g.ch[1] + n, v.y);
+      if (v.z != 0.0f) atomic_add_f32(g.ch[2] + n, v.z);
+      if (v.w != 0.0f) atomic_add_f32(g.ch[3] + n, v.w);
+      if (u != 0.0f) atomic_add_f32(g.ch[4] + n, u);
+    }
   }
 }
 
@@ -511,7 +663,7 @@ __global__ void __launch_bounds__(TPB) k_proj_clear(FsiLbmParams p, FsiLbmState 
 
 __global__ void __launch_bounds__(PTPB) k_proj_particles(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
                                                          FsiMpmState ms, FsiCouplingParams cp) {
-  __shared__ float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
+  __shared__ __align__(16) float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
   const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
   proj_block(lp, ls, mp, ms, cp, q, q < mp.n_particles, tile);
 }
@@ -700,48 +852,6 @@ __global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSt
 constexpr int FUSED_TILE_NODES = TILE_NODES;
 constexpr int REACT_REC = 6;  // active count, max particle norm, max grid contribution, net force x, y, z
 
-// Keeps the compiler from moving LDS accesses across this point; the wave
-// itself issues its LDS instructions in program order.
-__device__ __forceinline__ void wave_order() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// The fused tile updates a node with one 16-byte LDS load and one store
-// (read-modify-write) instead of four float atomics.  That is exact when the
-// workgroup is one wave and no two of its lanes that share a tile copy have
-// the same stencil base: in any one instruction the lanes of a copy then
-// address distinct nodes, and two updates of one node by different lanes come
-// from different iterations of the 27-node loop, which the wave issues in
-// order.  In a lattice-ordered cloud (8 particles per cell, lane parity = the
-// particle's z half of the cell, which is also its tile copy) the condition
-// always holds; otherwise the workgroup takes the atomic path.
-
-// Whether the read-modify-write scatter is exact for this workgroup.  Scratch:
-// the tile, before it is cleared.
-__device__ __forceinline__ bool bases_exclusive(const Stencil& sn, bool valid, const TileBox& b, float* tile) {
-  int* own = reinterpret_cast<int*>(tile);
-  int key = 0;
-  if (valid) {
-    key = (int)(threadIdx.x & (TILE_COPIES - 1)) * b.V +
-          ((sn.base[0] - b.lx) * b.ey + (sn.base[1] - b.ly)) * b.ez + (sn.base[2] - b.lz);
-    own[key] = (int)threadIdx.x;
-  }
-  __syncthreads();
-  const int clash = valid && own[key] != (int)threadIdx.x;
-  return !__syncthreads_or(clash);
-}
-
-// Where the fused P2G scatter puts a node's 4 floats.
-enum FusedScatter {
-  SCATTER_TILE_RMW = 0,     // LDS tile, read-modify-write (bases exclusive per copy)
-  SCATTER_TILE_ATOMIC = 1,  // LDS tile, float atomics
-  SCATTER_GLOBAL = 2        // grid arrays, float atomics (box too large for a tile)
-};
-
-__device__ __forceinline__ float pick3(int i, const float (&v)[3]) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
-
 // The 27-node scatter of p2g_react_particle for one scatter kind: node (a, e,
 // c) receives wt * (m + aff (x_node - x_p)) and wt * mass.  Returns the
 // largest weight of a node inside the grid.
@@ -798,10 +908,10 @@ __device__ __forceinline__ float fused_scatter(const FsiMpmParams& mp, const Ste
         __hip_atomic_fetch_add(f + 2, az, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         __hip_atomic_fetch_add(f + 3, am, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       } else if (in) {
-        atomicAdd(g.ch[0] + nrow + c, ax);
-        atomicAdd(g.ch[1] + nrow + c, ay);
-        atomicAdd(g.ch[2] + nrow + c, az);
-        atomicAdd(g.ch[3] + nrow + c, am);
+        atomic_add_f32(g.ch[0] + nrow + c, ax);
+        atomic_add_f32(g.ch[1] + nrow + c, ay);
+        atomic_add_f32(g.ch[2] + nrow + c, az);
+        atomic_add_f32(g.ch[3] + nrow + c, am);
       }
     }
   }
@@ -876,10 +986,10 @@ __device__ __forceinline__ void tile_flush_fused(const float4* tile, const TileB
       v.z += u.z;
       v.w += u.w;
     }
-    if (v.x != 0.0f) atomicAdd(g.ch[0] + n, v.x);
-    if (v.y != 0.0f) atomicAdd(g.ch[1] + n, v.y);
-    if (v.z != 0.0f) atomicAdd(g.ch[2] + n, v.z);
-    if (v.w != 0.0f) atomicAdd(g.ch[3] + n, v.w);
+    if (v.x != 0.0f) atomic_add_f32(g.ch[0] + n, v.x);
+    if (v.y != 0.0f) atomic_add_f32(g.ch[1] + n, v.y);
+    if (v.z != 0.0f) atomic_add_f32(g.ch[2] + n, v.z);
+    if (v.w != 0.0f) atomic_add_f32(g.ch[3] + n, v.w);
   }
 }
 
@@ -1151,7 +1261,7 @@ __global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* ins
 }
 
 __global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance* inst, const int32_t* bmap) {
-  __shared__ float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
+  __shared__ __align__(16) float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   proj_block(I.lp, I.ls, I.mp, I.ms, I.cp, e_, in_, tile);
 }
