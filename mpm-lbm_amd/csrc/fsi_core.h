@@ -793,16 +793,35 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
   const bool diag = mb && (flags & FSI_SUB_DIAG);
   const bool moving = mb && (info & CI_MOVING);
   const bool keep_links = mb && (moving || self_face);  // walls cannot read these links from F
+  // Moving-wall corrections of the links, before the pushes: the wall
+  // velocities are loaded 6 links at a time, each load branch-free (a
+  // direction without a link reads the cell itself), so a cell next to the
+  // moving solid waits for memory 3 times rather than once per link.
+  float cr[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) cr[s] = 0.0f;
+  if (moving) {
+#pragma unroll
+    for (int g0 = 1; g0 < Q; g0 += 6) {
+      float w3[6][3];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const uint32_t y = (info & (1u << (g0 + q))) ? c + (uint32_t)nb.d(g0 + q) : c;
+        w3[q][0] = at32(st.solid_vel, y);
+        w3[q][1] = at32(st.solid_vel + N, y);
+        w3[q][2] = at32(st.solid_vel + 2 * (size_t)N, y);
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int s = g0 + q;
+        cr[s] = -6.0f * wgt(s) * rho * ((float)EX[s] * w3[q][0] + (float)EY[s] * w3[q][1] + (float)EZ[s] * w3[q][2]);
+      }
+    }
+  }
 #pragma unroll
   for (int s = 0; s < Q; ++s) {
     if (s != 0 && (info & (1u << s))) {
-      float corr = 0.0f;
-      if (moving) {
-        const uint32_t y = c + (uint32_t)nb.d(s);
-        const float wx = at32(st.solid_vel, y), wy = at32(st.solid_vel + N, y),
-                    wz = at32(st.solid_vel + 2 * (size_t)N, y);
-        corr = -6.0f * wgt(s) * rho * ((float)EX[s] * wx + (float)EY[s] * wy + (float)EZ[s] * wz);
-      }
+      const float corr = cr[s];
       const float bounced = f[s] + corr;
       if (!self_face) Fd.st(N, LR[s], c, bounced);
       if (keep_links) at32(link + (size_t)(s - 1) * N, c) = f[s] + bounced;

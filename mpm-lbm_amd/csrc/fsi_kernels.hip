@@ -34,6 +34,21 @@ constexpr int TPB = 256;
 
 inline unsigned blocks_for(long long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// SIMDs of the current device (4 per CU), queried once per device.
+inline long long simds_on_device() {
+  static thread_local int dev_cached = -1;
+  static thread_local long long simds = 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  if (dev != dev_cached) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    simds = 4LL * cus;
+    dev_cached = dev;
+  }
+  return simds;
+}
+
 // Workgroup size of the per-particle kernels.  Particle counts are small next
 // to lattice sizes (tens of thousands), so small workgroups spread them over
 // all 256 CUs instead of a few dozen.
@@ -491,9 +506,13 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
   return x * per + (x < rem ? x : rem) + b / NXCD;
 }
 
-template <typename Pop, uint32_t FIXED>
-__global__ void __launch_bounds__(TPB) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
-                                                     uint32_t flags, unsigned tiles_per_plane) {
+// MINW: the waves per SIMD the register budget must allow.  A grid that fits
+// on the chip in one wave generation at 4 waves per SIMD runs without a cap
+// (MINW 1: more registers, fewer waits); a larger grid keeps 5 waves per SIMD
+// resident to hide memory latency.
+template <typename Pop, uint32_t FIXED, int MINW>
+__global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
+                                                           uint32_t flags, unsigned tiles_per_plane) {
   const unsigned L = xcd_tile(blockIdx.x, gridDim.x);
   const unsigned i = L / tiles_per_plane;
   const unsigned jk = (L - i * tiles_per_plane) * TPB + threadIdx.x;
@@ -1148,7 +1167,7 @@ __global__ void __launch_bounds__(TPB) k_mpm_stats_partial(FsiMpmParams mp, FsiM
 // MOVING_WALL, CELL_FORCE), compiled in as in k_lbm_substep; ~0u = take them
 // from each instance at run time (any mix of modes).
 template <uint32_t FIXED>
-__global__ void __launch_bounds__(TPB) kb_substep(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
+__global__ void __launch_bounds__(TPB, 5) kb_substep(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
   const unsigned b_ = xcd_tile(blockIdx.x, gridDim.x);
   const int i_ = bmap[2 * b_], l_ = bmap[2 * b_ + 1];
   const FsiBatchInstance& I = inst[i_];
@@ -1454,10 +1473,14 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
   const dim3 grid(tiles * (unsigned)p->nx), block(TPB);
   hipStream_t s = as_stream(stream);
   const uint32_t fixed = flags & (FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE);
+  const bool one_generation = (long long)grid.x * (TPB / 64) <= 4LL * simds_on_device();
   switch (fixed) {
 #define FSI_CASE(V)                                                                   \
   case (V):                                                                           \
-    FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V)>), grid, block, 0, s, *p, *st, cur, parity, flags, tiles)); \
+    if (one_generation)                                                               \
+      FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V), 1>), grid, block, 0, s, *p, *st, cur, parity, flags, tiles)); \
+    else                                                                              \
+      FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V), 5>), grid, block, 0, s, *p, *st, cur, parity, flags, tiles)); \
     break;
     FSI_CASE(0u)
     FSI_CASE(1u)
