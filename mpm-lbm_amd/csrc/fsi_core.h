@@ -147,6 +147,7 @@ inline T& at32(T* plane, uint32_t c) {
 #endif
 
 struct PopF32 {
+  static constexpr bool kF32 = true;
   float* p;
   FSI_HDM float get(long long N, int s, long long c) const { return p[s * N + c]; }
   FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = v; }
@@ -154,6 +155,7 @@ struct PopF32 {
   FSI_HDM void st(uint32_t N, int s, uint32_t c, float v) const { at32(p + (size_t)s * N, c) = v; }
 };
 struct PopF16 {
+  static constexpr bool kF32 = false;
   uint16_t* p;
   FSI_HDM float get(long long N, int s, long long c) const { return wgt(s) + f16_to_f32(p[s * N + c]); }
   FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = f32_to_f16(v - wgt(s)); }
@@ -161,12 +163,19 @@ struct PopF16 {
   FSI_HDM void st(uint32_t N, int s, uint32_t c, float v) const { at32(p + (size_t)s * N, c) = f32_to_f16(v - wgt(s)); }
 };
 struct PopBF16 {
+  static constexpr bool kF32 = false;
   uint16_t* p;
   FSI_HDM float get(long long N, int s, long long c) const { return wgt(s) + bf16_to_f32(p[s * N + c]); }
   FSI_HDM void put(long long N, int s, long long c, float v) const { p[s * N + c] = f32_to_bf16(v - wgt(s)); }
   FSI_HDM float ld(uint32_t N, int s, uint32_t c) const { return wgt(s) + bf16_to_f32(at32(p + (size_t)s * N, c)); }
   FSI_HDM void st(uint32_t N, int s, uint32_t c, float v) const { at32(p + (size_t)s * N, c) = f32_to_bf16(v - wgt(s)); }
 };
+
+// Plane s of fp32 populations (other storages have no float planes).
+template <typename Pop>
+FSI_HD const float* pop_plane_f32(const Pop& P, uint32_t N, int s) {
+  return reinterpret_cast<const float*>(P.p) + (size_t)s * N;
+}
 
 template <typename Pop>
 FSI_HD Pop pop_of(void* base) {
@@ -717,15 +726,31 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
   } else {
     const float* link = st.link_c[1 - parity];
     const bool wall_moving = (info & CI_MOVING) != 0;
+    if constexpr (Pop::kF32) {
+      // fp32 populations: each link's source is a float plane either way, so
+      // one branch-free load per direction (a direction without a link reads
+      // the cell itself; the factor 2 of a population read back is applied
+      // at the sum) and no load waits for another
+      F[0] = 0.0f;
 #pragma unroll
-    for (int s = 0; s < Q; ++s) {
-      F[s] = 0.0f;
-      if (s != 0 && (info & (1u << s))) {
-        const uint32_t x = c + (uint32_t)nb.d(LR[s]);
-        if (wall_moving || ff.of_source(s))
-          F[s] = at32(link + (size_t)(s - 1) * N, x);
-        else
-          F[s] = 2.0f * Fs.ld(N, LR[s], x);
+      for (int s = 1; s < Q; ++s) {
+        const bool has = (info & (1u << s)) != 0;
+        const uint32_t x = has ? c + (uint32_t)nb.d(LR[s]) : c;
+        const float* src = (wall_moving || ff.of_source(s)) ? link + (size_t)(s - 1) * N
+                                                            : pop_plane_f32(Fs, N, LR[s]);
+        F[s] = at32(src, x);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < Q; ++s) {
+        F[s] = 0.0f;
+        if (s != 0 && (info & (1u << s))) {
+          const uint32_t x = c + (uint32_t)nb.d(LR[s]);
+          if (wall_moving || ff.of_source(s))
+            F[s] = at32(link + (size_t)(s - 1) * N, x);
+          else
+            F[s] = 2.0f * Fs.ld(N, LR[s], x);
+        }
       }
     }
     const double* a = st.hydro_accum + 4 * (size_t)c;
@@ -739,12 +764,15 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
     return;
   }
   // wall cell: the force of its links is sum_s e_s c_s; added to the accumulator
+  const bool wall_moving = (info & CI_MOVING) != 0;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
   for (int s = 1; s < Q; ++s) {
-    fx += (float)EX[s] * F[s];
-    fy += (float)EY[s] * F[s];
-    fz += (float)EZ[s] * F[s];
+    float cs = F[s];
+    if constexpr (Pop::kF32) cs = !(info & (1u << s)) ? 0.0f : ((wall_moving || ff.of_source(s)) ? cs : 2.0f * cs);
+    fx += (float)EX[s] * cs;
+    fy += (float)EY[s] * cs;
+    fz += (float)EZ[s] * cs;
   }
   wall_accumulate_loaded(st.hydro_accum, c, acc, fx, fy, fz);
 }
