@@ -700,29 +700,44 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
 // by the upstream fluid neighbour, by the cell's own bounce-back, or by the BC.
 // Wall (solid) cells gather the previous substep's link forces instead.
 // All lattice addressing is 32-bit (at most 2^30 cells, checked by the ABI).
-template <typename Pop>
+template <typename Pop, bool SPEC = false>
 FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, int parity,
                              uint32_t flags, int i, int j, int k) {
   const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
   const uint32_t c = ((uint32_t)i * (uint32_t)p.ny + (uint32_t)j) * (uint32_t)p.nz + (uint32_t)k;
   const uint32_t info = at32(st.cellinfo, c);
+  const Pop Fs = pop_of<Pop>(st.F[cur]);
+  // SPEC: the populations are loaded with the flags, before the flags say
+  // whether they are needed (a solid cell drops them): one memory round trip
+  // for a fluid cell instead of two, for some extra traffic.  It pays where
+  // several wave generations share the chip (a large lattice, the batched
+  // form); a lattice that fits in one generation is faster without it.
+  float F[Q];
+  if (SPEC) {
+#pragma unroll
+    for (int s = 0; s < Q; ++s) F[s] = Fs.ld(N, s, c);
+  }
   const Nbr nb = nbr_of(p, i, j, k);
   const bool mb = (flags & FSI_SUB_MOVING_WALL) != 0;
   const bool solid = (info & CI_SOLID) != 0;
   const bool gather = solid && mb && (flags & FSI_SUB_GATHER_PREV) && (info & CI_MASK);
   if (solid && !gather) return;
-  const Pop Fs = pop_of<Pop>(st.F[cur]);
   const FaceFlags ff = face_flags(p, i);
   // One load phase for both kinds of cell, so that a wave holding fluid and
   // wall cells waits for memory once: fluid cells load their populations,
   // wall cells the link scalars c_s of their links into the same registers.
   // (The wall cells' accumulator record is loaded here too, and the fluid
   // work is issued before the wall work, so nothing waits on a second trip.)
-  float F[Q];
+  // (with SPEC the wall cells' loads go to registers of their own, Fw, so
+  // they do not wait for the populations' loads into F to land)
+  float Fw_[SPEC ? Q : 1];
+  float* Fw = SPEC ? Fw_ : F;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (!solid) {
+    if (!SPEC) {
 #pragma unroll
-    for (int s = 0; s < Q; ++s) F[s] = Fs.ld(N, s, c);
+      for (int s = 0; s < Q; ++s) F[s] = Fs.ld(N, s, c);
+    }
   } else {
     const float* link = st.link_c[1 - parity];
     const bool wall_moving = (info & CI_MOVING) != 0;
@@ -731,25 +746,25 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
       // one branch-free load per direction (a direction without a link reads
       // the cell itself; the factor 2 of a population read back is applied
       // at the sum) and no load waits for another
-      F[0] = 0.0f;
+      Fw[0] = 0.0f;
 #pragma unroll
       for (int s = 1; s < Q; ++s) {
         const bool has = (info & (1u << s)) != 0;
         const uint32_t x = has ? c + (uint32_t)nb.d(LR[s]) : c;
         const float* src = (wall_moving || ff.of_source(s)) ? link + (size_t)(s - 1) * N
                                                             : pop_plane_f32(Fs, N, LR[s]);
-        F[s] = at32(src, x);
+        Fw[s] = at32(src, x);
       }
     } else {
 #pragma unroll
       for (int s = 0; s < Q; ++s) {
-        F[s] = 0.0f;
+        Fw[s] = 0.0f;
         if (s != 0 && (info & (1u << s))) {
           const uint32_t x = c + (uint32_t)nb.d(LR[s]);
           if (wall_moving || ff.of_source(s))
-            F[s] = at32(link + (size_t)(s - 1) * N, x);
+            Fw[s] = at32(link + (size_t)(s - 1) * N, x);
           else
-            F[s] = 2.0f * Fs.ld(N, LR[s], x);
+            Fw[s] = 2.0f * Fs.ld(N, LR[s], x);
         }
       }
     }
@@ -768,7 +783,7 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
   for (int s = 1; s < Q; ++s) {
-    float cs = F[s];
+    float cs = Fw[s];
     if constexpr (Pop::kF32) cs = !(info & (1u << s)) ? 0.0f : ((wall_moving || ff.of_source(s)) ? cs : 2.0f * cs);
     fx += (float)EX[s] * cs;
     fy += (float)EY[s] * cs;

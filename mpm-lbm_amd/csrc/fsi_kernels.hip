@@ -509,7 +509,7 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
 // MINW: the waves per SIMD the register budget must allow.  A grid that fits
 // on the chip in one wave generation at 4 waves per SIMD runs without a cap
 // (MINW 1: more registers, fewer waits); a larger grid keeps 5 waves per SIMD
-// resident to hide memory latency.
+// resident to hide memory latency and loads populations speculatively.
 template <typename Pop, uint32_t FIXED, int MINW>
 __global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
                                                            uint32_t flags, unsigned tiles_per_plane) {
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLb
   if (jk >= plane) return;
   const unsigned j = jk / (unsigned)p.nz;
   const unsigned k = jk - j * (unsigned)p.nz;
-  lbm_substep_cell<Pop>(p, st, cur, parity,
+  lbm_substep_cell<Pop, (MINW > 1)>(p, st, cur, parity,
                         (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED,
                         (int)i, (int)j, (int)k);
 }
@@ -1185,11 +1185,11 @@ __global__ void __launch_bounds__(TPB, 5) kb_substep(const FsiBatchInstance* ins
   const uint32_t flags = FIXED == ~0u ? batch_substep_flags(I, t) : ((batch_substep_flags(I, t) & ~VAR) | FIXED);
   const int cur = batch_cur(I, t), parity = t & 1;
   if (p.storage == FSI_STORE_F16_DEV)
-    lbm_substep_cell<PopF16>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
+    lbm_substep_cell<PopF16, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
   else if (p.storage == FSI_STORE_BF16_DEV)
-    lbm_substep_cell<PopBF16>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
+    lbm_substep_cell<PopBF16, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
   else
-    lbm_substep_cell<PopF32>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
+    lbm_substep_cell<PopF32, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
 }
 
 __global__ void __launch_bounds__(TPB) kb_rebuild(const FsiBatchInstance* inst, const int32_t* bmap) {
