@@ -55,6 +55,54 @@ def test_gpu_moving_boundary_one_step(hip, tmp_path):
     _check(run_and_compare(d, o, 5), "mb32_1step", strict=True)
 
 
+def _reordered(w, how, seed=7):
+    """`w` with its particles in another storage order.  "local" shuffles each
+    64-particle workgroup's particles among themselves: the tile boxes stay
+    those of the lattice order, but lanes of one tile copy then share stencil
+    bases, so the scatters take their LDS-atomic path.  "global" shuffles the
+    whole cloud: the boxes outgrow the tile and the scatters go to the grid
+    with global atomics."""
+    import dataclasses
+
+    P = w.config.n_particles
+    rng = np.random.default_rng(seed)
+    if how == "local":
+        perm = np.concatenate([b + rng.permutation(min(64, P - b)) for b in range(0, P, 64)])
+    else:
+        perm = rng.permutation(P)
+    return dataclasses.replace(w, cloud={k: np.asarray(v)[perm] for k, v in w.cloud.items()})
+
+
+def _groups(w, n):
+    """Per 64-particle group: (tile-box nodes, bases exclusive per lane parity)."""
+    base = np.floor(np.asarray(w.cloud["x"], np.float64) * n - 0.5).astype(int)
+    out = []
+    for b in range(0, len(base), 64):
+        g = base[b:b + 64]
+        keys = {(l & 1, *map(int, g[l])) for l in range(len(g))}
+        out.append((int(np.prod(g.max(0) - g.min(0) + 3)), len(keys) == len(g)))
+    return out
+
+
+@pytest.mark.parametrize("how", ["local", "global"])
+@pytest.mark.parametrize("mode", ["penalty", "moving_boundary"])
+def test_gpu_scatter_fallback_paths(hip, tmp_path, how, mode):
+    """The projection and fused P2G scatters off their read-modify-write path
+    (particles not in lattice order), against the oracle on the same cloud."""
+    if mode == "penalty":
+        w = workloads.penalty_block(n=32, steps=4, out_dir=str(tmp_path))
+    else:
+        w = workloads.moving_boundary_unsubcycled(n=32, steps=4, flap=(2, 16, 12), out_dir=str(tmp_path))
+    w = _reordered(w, how)
+    g = _groups(w, 32)
+    if how == "local":
+        assert all(v <= 512 for v, _ in g) and not all(ex for _, ex in g)  # LDS tile, atomic updates
+    else:
+        assert sum(v > 512 for v, _ in g) > len(g) // 2  # mostly too large for a tile
+    d, o = make_pair(w, tmp_path / "run", hip, None, with_fp64=True)
+    _check(run_and_compare(d, o, 4), f"{mode}_{how}")
+
+
 def test_gpu_driver_run_writes_outputs(hip, tmp_path):
     w = workloads.moving_boundary_flap(n=32, steps=4, substeps=20, flap=(2, 16, 12), out_dir=str(tmp_path))
     from mpm_lbm.sim.drivers.fsi_driver import DIAGNOSTIC_FIELDS, FSIDriver3D
