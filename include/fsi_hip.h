@@ -56,7 +56,7 @@ extern "C" {
 #define FSI_SUB_CELL_FORCE 16u    /* add cell_force to LBMConfig.force */
 
 typedef struct FsiLbmParams {
-  int32_t nx, ny, nz;
+  int32_t nx, ny, nz;  /* at least 3 each, at most 2^30 cells in all */
   int32_t bc_x_left, bc_x_right;
   float s_v, s_other, rho0;
   float force[3];
@@ -117,7 +117,7 @@ typedef struct FsiLbmStats {
 } FsiLbmStats;
 
 typedef struct FsiMpmParams {
-  int32_t gnx, gny, gnz;   /* MPM grid nodes per axis (n_grid each in the reference) */
+  int32_t gnx, gny, gnz;   /* MPM grid nodes per axis (n_grid each in the reference); at least 3 each */
   int32_t n_particles;
   int32_t bound;           /* sticky-wall width (mpm/config.py:21) */
   int32_t damping_enabled, lock_z_enabled;
