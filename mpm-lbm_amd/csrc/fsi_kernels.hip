@@ -104,12 +104,37 @@ int check_mpm(const FsiMpmParams* mp, const FsiMpmState* ms) {
 }
 
 // ---------------------------------------------------------------------------
-// Block reductions (wave64 shuffles, then one LDS slot per wave)
+// Block reductions: a wave64 reduction through DPP lane moves (register to
+// register: quad permutes, row mirrors, then the row broadcasts of lanes 15
+// and 31; the full result lands in lane 63), then one LDS slot per wave.  A
+// lane whose DPP source is masked off takes the identity.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_reduce(double v, int op) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = red_combine(op, v, __shfl_xor(v, off, 64));
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_dpp_reduce(T v, T id, Op op) {
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xF, 0xF, false));  // row_mirror
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false));  // row_bcast:15 into rows 1, 3
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xC, 0xF, false));  // row_bcast:31 into rows 2, 3
   return v;
+}
+
+// Lane 63's value, in every lane (uniform).
+__device__ __forceinline__ int lane63(int v) { return __builtin_amdgcn_readlane(v, 63); }
+__device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xFFFFFFFFll), 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Reduction over the wave, uniform in every lane.
+__device__ __forceinline__ double wave_reduce(double v, int op) {
+  if (op == RED_SUM) return lane63(wave_dpp_reduce(v, 0.0, [](double a, double b) { return a + b; }));
+  if (op == RED_MIN) return lane63(wave_dpp_reduce(v, 1.0e300, [](double a, double b) { return b < a ? b : a; }));
+  return lane63(wave_dpp_reduce(v, -1.0e300, [](double a, double b) { return b > a ? b : a; }));
 }
 
 // Result valid in thread 0 (in every thread of a one-wave workgroup, which
@@ -128,11 +153,7 @@ __device__ double block_reduce(double v, int op, double* smem) {
 }
 
 __device__ __forceinline__ float block_max_f(float v, float* smem) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const float o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
+  v = lane63(wave_dpp_reduce(v, -__builtin_inff(), [](float a, float b) { return b > a ? b : a; }));
   if (blockDim.x == 64) return v > 0.0f ? v : 0.0f;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
@@ -145,8 +166,7 @@ __device__ __forceinline__ float block_max_f(float v, float* smem) {
 }
 
 __device__ __forceinline__ int block_sum_i(int v, int* smem) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  v = lane63(wave_dpp_reduce(v, 0, [](int a, int b) { return a + b; }));
   if (blockDim.x == 64) return v;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
@@ -178,20 +198,10 @@ struct TileBox {
 };
 
 __device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_xor(v, off, 64);
-    v = o < v ? o : v;
-  }
-  return v;
+  return lane63(wave_dpp_reduce(v, 0x7FFFFFFF, [](int a, int b) { return b < a ? b : a; }));
 }
 __device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
+  return lane63(wave_dpp_reduce(v, -0x7FFFFFFF - 1, [](int a, int b) { return b > a ? b : a; }));
 }
 
 // Box of the workgroup's particles' stencils, [min base, max base + 2] per
