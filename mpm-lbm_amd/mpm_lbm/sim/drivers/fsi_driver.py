@@ -17,6 +17,7 @@ import json
 import os
 import time
 from dataclasses import replace
+from pathlib import Path
 
 import numpy as np
 
@@ -29,6 +30,7 @@ from ..geometry.duct_flap_proxy import duct_flap_proxy_static_geometry
 from ..geometry.sampler import GeometrySampler3D
 from ..io.run_utils import assert_no_nan_inf_array, ensure_output_dir, save_csv_rows, save_json_config
 from ..lbm.fluid import LBMFluid3D
+from ..lbm.restart import expected_restart_metadata_from_config, load_lbm_restart_to_lbm
 from ..monitoring.fluent_like import official_point_like_displacement
 from ..mpm.solid import MPMSolid3D
 from ..units.mapper import GridUnitMapper
@@ -114,8 +116,6 @@ class FSIDriver3D:
             raise NotImplementedError("prescribed wall-velocity / geometry-motion application is not implemented on the GPU path")
         if c.boundary_motion_report_enabled:
             raise NotImplementedError("boundary-motion interface reports are not implemented on the GPU path")
-        if c.lbm_restart_path:
-            raise NotImplementedError("lbm_restart_path is not implemented on the GPU path")
         if c.quality_check_enabled:
             raise NotImplementedError("geometry quality checks are not implemented on the GPU path")
         if c.write_vtk:
@@ -136,8 +136,7 @@ class FSIDriver3D:
         self.lbm = LBMFluid3D(lbm_config, native=self._native, device=self._device)
         self.lbm.set_solid_flags(solid_flags)
         self.lbm.init_simulation()
-        self.lbm_restart_load_report = {"restart_loaded": False, "restart_path": "",
-                                        "restart_required": bool(self.config.lbm_restart_required)}
+        self._load_lbm_restart_if_configured()
 
         overrides = {"gravity": self.config.gravity, "box_min": self.config.box_min, "box_max": self.config.box_max}
         overrides.update(self.config.make_mpm_control_overrides())
@@ -170,6 +169,26 @@ class FSIDriver3D:
         self.initialized = True
         self.start_time = time.perf_counter()
         self.timing["init_time"] += self.start_time - t0
+
+    def _load_lbm_restart_if_configured(self):
+        """fsi_driver.py:192-216.  A relative lbm_restart_path is taken from the
+        current working directory (the reference's is from its repository root)."""
+        if not self.config.lbm_restart_path:
+            self.lbm_restart_load_report = {"restart_loaded": False, "restart_path": "",
+                                            "restart_required": bool(self.config.lbm_restart_required)}
+            return self.lbm_restart_load_report
+        path = Path(self.config.lbm_restart_path)
+        if not path.is_absolute():
+            path = Path.cwd() / path
+        report = load_lbm_restart_to_lbm(self.lbm, path, expected_restart_metadata_from_config(self.config),
+                                         required=bool(self.config.lbm_restart_required))
+        report["restart_required"] = bool(self.config.lbm_restart_required)
+        report["restart_scope"] = self.config.lbm_restart_scope
+        self.lbm_restart_load_report = report
+        with open(os.path.join(self.out_dir, "lbm_restart_load_report.json"), "w", encoding="utf-8") as f:
+            json.dump(report, f, indent=2, sort_keys=True)
+            f.write("\n")
+        return report
 
     def _static_lbm_geometry(self, geometry_config):
         """fsi_driver.py:219-236: duct flags (written as a Fortran-order .dat like the reference) or all fluid."""
@@ -384,6 +403,12 @@ class FSIDriver3D:
         self._advance_mpm_with_moving_boundary_reaction()
         self._eager_subcycled_steps += 1
 
+    def _drop_step_graph(self):
+        """Forget the captured FSI-step graph (its launches hold the state the
+        driver had at capture); the next step runs eagerly and recaptures."""
+        self._fsi_graph = None
+        self._eager_subcycled_steps = 0
+
     def _graph_ready(self) -> bool:
         return (self.use_graph and self.lbm.device.type == "cuda" and self._eager_subcycled_steps >= 1
                 and self.config.lbm_substeps_per_fsi_step % 2 == 0)
@@ -412,21 +437,66 @@ class FSIDriver3D:
             self.total_mpm_substeps += 1
         self.timing["mpm_substep_time"] += time.perf_counter() - t0
 
-    def run(self):
-        """fsi_driver.py:577-595."""
+    def run(self, checkpoint_dir=None, checkpoint_every: int = 0, resume: bool = False,
+            stop_after_steps=None, keep_last: int = 3):
+        """fsi_driver.py:577-595, with checkpoint and resume (no reference
+        counterpart in the driver; the contract of the reference's
+        checkpointed rows, tests/test_step120_checkpoint_restart_contract.py):
+
+        * checkpoint_every > 0 writes a checkpoint to `checkpoint_dir` every
+          that many FSI steps (keeping the newest `keep_last`);
+        * resume=True continues from the newest checkpoint in `checkpoint_dir`
+          (a fresh run when there is none) up to n_lbm_steps;
+        * stop_after_steps ends the run after that many FSI steps of this
+          call, without the final outputs (an interrupted run), unless those
+          were its last steps."""
+        from ...checkpoint import latest_fsi_checkpoint, load_fsi_checkpoint, save_fsi_checkpoint
+
         total_start = time.perf_counter()
         if not self.initialized:
             self.initialize()
-        self.collect_diagnostics(0)
-        for _ in range(self.config.n_lbm_steps):
+        resumed = False
+        if resume:
+            if checkpoint_dir is None:
+                raise ValueError("resume=True needs checkpoint_dir")
+            latest = latest_fsi_checkpoint(checkpoint_dir)
+            if latest is not None:
+                load_fsi_checkpoint(self, latest)
+                resumed = True
+        if checkpoint_every and checkpoint_dir is None:
+            raise ValueError("checkpoint_every needs checkpoint_dir")
+        if not resumed:
+            self.collect_diagnostics(0)
+        # a fresh run takes n_lbm_steps steps (as the reference's run); a
+        # resumed one the steps its checkpoint had left
+        steps = (max(0, self.config.n_lbm_steps - self.current_lbm_step) if resumed
+                 else self.config.n_lbm_steps)
+        for done in range(1, steps + 1):
             self.step_once()
             if (self.current_lbm_step % self.config.output_interval == 0
                     or self.current_lbm_step == self.config.n_lbm_steps):
                 self.collect_diagnostics(self.current_lbm_step)
+            if checkpoint_every and self.current_lbm_step % int(checkpoint_every) == 0:
+                save_fsi_checkpoint(self, checkpoint_dir, keep_last=keep_last)
+            if stop_after_steps is not None and done >= int(stop_after_steps) and done < steps:
+                self.timing["total_time"] = time.perf_counter() - total_start
+                return self.diagnostics_rows
         self.export_outputs(self.current_lbm_step)
         self.save_timeseries()
         self.timing["total_time"] = time.perf_counter() - total_start
         return self.diagnostics_rows
+
+    def save_checkpoint(self, directory, keep_last: int = 3):
+        """Checkpoint after the current FSI step (mpm_lbm.checkpoint)."""
+        from ...checkpoint import save_fsi_checkpoint
+
+        return save_fsi_checkpoint(self, directory, keep_last=keep_last)
+
+    def load_checkpoint(self, path):
+        """Resume from a checkpoint (mpm_lbm.checkpoint)."""
+        from ...checkpoint import load_fsi_checkpoint
+
+        return load_fsi_checkpoint(self, path)
 
     # ----------------------------------------------------------- diagnostics
     def collect_diagnostics(self, step: int):

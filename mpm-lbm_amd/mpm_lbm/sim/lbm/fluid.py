@@ -422,6 +422,13 @@ class LBMFluid3D:
                        self._pending, 1, 0, self._stream())
             self._hydro_from_links = False
 
+    def drop_pending_links(self):
+        """Forget the wall links of the last substep without gathering them
+        (state loaded from a file replaces what they would give)."""
+        self._pending = None
+        self._pending_accumulate = False
+        self._hydro_from_links = False
+
     def _populations_written(self):
         other = self._F[1 - self.cur]
         other.copy_(self._F[self.cur])

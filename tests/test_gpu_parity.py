@@ -156,3 +156,38 @@ def test_gpu_graph_replay_matches_eager(hip, tmp_path):
         for label, d in (("eager", eager), ("graph", graph)):
             check_history([field_errors(d, o)], f"{label} step {step}", TOL)
     assert graph.total_lbm_substeps == eager.total_lbm_substeps == 5 * 20
+
+
+@pytest.mark.parametrize("mode", ["penalty", "moving_boundary"])
+def test_gpu_checkpoint_resume(hip, tmp_path, mode):
+    """A run stopped after 2 FSI steps and resumed from its checkpoint in a new
+    driver (the step graph recaptured) meets the parity bar after 4 steps;
+    the loaded device state equals the checkpoint's arrays bit for bit."""
+    from mpm_lbm.checkpoint import latest_fsi_checkpoint
+    from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
+    from oracle.oracle import OracleFSI
+    from tests.parity import field_errors
+
+    if mode == "penalty":
+        w = workloads.penalty_block(n=32, steps=4, out_dir=str(tmp_path))
+    else:
+        w = workloads.moving_boundary_flap(n=32, steps=4, substeps=20, flap=(2, 16, 12), out_dir=str(tmp_path))
+    ck = tmp_path / "ck"
+    first = FSIDriver3D(w.config, str(tmp_path / "run"), particle_cloud=w.cloud, write_geometry_file=False)
+    first.run(checkpoint_dir=ck, checkpoint_every=1, stop_after_steps=2)
+    path = latest_fsi_checkpoint(ck)
+    second = FSIDriver3D(w.config, str(tmp_path / "run"), particle_cloud=w.cloud, write_geometry_file=False)
+    second.load_checkpoint(path)
+    with np.load(path, allow_pickle=False) as z:
+        for k in ("F", "rho", "v", "solid", "solid_phi", "solid_vel", "hydro_force"):
+            np.testing.assert_array_equal(getattr(second.lbm, k).to_numpy(), z[f"lbm_{k}"], err_msg=k)
+        for k in ("x", "v", "C", "F", "Jp"):
+            np.testing.assert_array_equal(getattr(second.solid, k).to_numpy(), z[f"mpm_{k}"], err_msg=k)
+    second.run(checkpoint_dir=ck, checkpoint_every=1, resume=True)
+    assert second.current_lbm_step == 4
+    o = OracleFSI(w.config, second.lbm.static_solid.to_numpy(), w.cloud, w.material, dtype=np.float32)
+    o.fp64 = OracleFSI(w.config, second.lbm.static_solid.to_numpy(), w.cloud, w.material, dtype=np.float64)
+    for _ in range(4):
+        o.step_once()
+        o.fp64.step_once()
+    _check([field_errors(second, o)], f"resume_{mode}", strict=(mode == "penalty"))
