@@ -122,6 +122,18 @@ static inline float f16_to_f32(uint16_t h) {
 }
 #endif
 
+// A value that every lane of the wave holds (the lattice size), shown to be
+// uniform.  A uniform value that reaches a use through divergent branches can
+// end up in vector registers, and the plane bases derived from it then take
+// 64-bit per-lane address arithmetic at every load and store.
+FSI_HD uint32_t wave_uniform(uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+#else
+  return v;
+#endif
+}
+
 // Element c of a population plane through a 32-bit byte offset.  A lattice
 // holds at most 2^30 cells (checked by the C ABI), so c * sizeof(T) < 2^32:
 // the plane base (uniform: buffer + s * N) stays in scalar registers and each
@@ -692,6 +704,67 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
                            uint32_t flags, int i, int j, int k, uint32_t N, uint32_t c, uint32_t info, const Nbr& nb,
                            const FaceFlags& ff, const float* F);
 
+FSI_HD uint32_t cell_info(const FsiLbmState& st, uint32_t c) { return at32(st.cellinfo, c); }
+
+// Wall cell c (solid, with links) of a moving-wall substep: load the link
+// scalars c_s of the previous substep into Fw[1..18] and the cell's
+// accumulator record into acc.
+template <typename Pop>
+FSI_HD void wall_links_load(const FsiLbmState& st, const Pop& Fs, int parity, uint32_t N, uint32_t c,
+                            uint32_t info, const Nbr& nb, const FaceFlags& ff, float* Fw, double* acc) {
+  const float* link = st.link_c[1 - parity];
+  const bool wall_moving = (info & CI_MOVING) != 0;
+  Fw[0] = 0.0f;
+  if constexpr (Pop::kF32) {
+    // fp32 populations: each link's source is a float plane either way, so
+    // one branch-free load per direction (a direction without a link reads
+    // the cell itself; the factor 2 of a population read back is applied
+    // at the sum) and no load waits for another
+#pragma unroll
+    for (int s = 1; s < Q; ++s) {
+      const bool has = (info & (1u << s)) != 0;
+      const uint32_t x = has ? c + (uint32_t)nb.d(LR[s]) : c;
+      const float* src = (wall_moving || ff.of_source(s)) ? link + (size_t)(s - 1) * N
+                                                          : pop_plane_f32(Fs, N, LR[s]);
+      Fw[s] = at32(src, x);
+    }
+  } else {
+#pragma unroll
+    for (int s = 1; s < Q; ++s) {
+      Fw[s] = 0.0f;
+      if (info & (1u << s)) {
+        const uint32_t x = c + (uint32_t)nb.d(LR[s]);
+        if (wall_moving || ff.of_source(s))
+          Fw[s] = at32(link + (size_t)(s - 1) * N, x);
+        else
+          Fw[s] = 2.0f * Fs.ld(N, LR[s], x);
+      }
+    }
+  }
+  const double* a = st.hydro_accum + 4 * (size_t)c;
+  acc[0] = a[0];
+  acc[1] = a[1];
+  acc[2] = a[2];
+  acc[3] = a[3];
+}
+
+// The force of wall cell c's links, sum_s e_s c_s, added to its accumulator.
+template <typename Pop>
+FSI_HD void wall_links_accumulate(const FsiLbmState& st, uint32_t c, uint32_t info, const FaceFlags& ff,
+                                  const float* Fw, const double* acc) {
+  const bool wall_moving = (info & CI_MOVING) != 0;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+  for (int s = 1; s < Q; ++s) {
+    float cs = Fw[s];
+    if constexpr (Pop::kF32) cs = !(info & (1u << s)) ? 0.0f : ((wall_moving || ff.of_source(s)) ? cs : 2.0f * cs);
+    fx += (float)EX[s] * cs;
+    fy += (float)EY[s] * cs;
+    fz += (float)EZ[s] * cs;
+  }
+  wall_accumulate_loaded(st.hydro_accum, c, acc, fx, fy, fz);
+}
+
 // One fused LBM substep for cell (i,j,k) (push form): collide F[cur] at the
 // cell, push the post-collision populations to the neighbours in F[1-cur],
 // bounce them back at solid neighbours (half-way, with the Ladd moving-wall
@@ -725,11 +798,10 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
   const FaceFlags ff = face_flags(p, i);
   // One load phase for both kinds of cell, so that a wave holding fluid and
   // wall cells waits for memory once: fluid cells load their populations,
-  // wall cells the link scalars c_s of their links into the same registers.
-  // (The wall cells' accumulator record is loaded here too, and the fluid
-  // work is issued before the wall work, so nothing waits on a second trip.)
-  // (with SPEC the wall cells' loads go to registers of their own, Fw, so
-  // they do not wait for the populations' loads into F to land)
+  // wall cells the link scalars c_s of their links (and their accumulator
+  // record), and the fluid work is issued before the wall work.  With SPEC
+  // the wall cells' loads go to registers of their own, Fw, so they do not
+  // wait for the populations' loads into F to land.
   float Fw_[SPEC ? Q : 1];
   float* Fw = SPEC ? Fw_ : F;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -739,57 +811,13 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
       for (int s = 0; s < Q; ++s) F[s] = Fs.ld(N, s, c);
     }
   } else {
-    const float* link = st.link_c[1 - parity];
-    const bool wall_moving = (info & CI_MOVING) != 0;
-    if constexpr (Pop::kF32) {
-      // fp32 populations: each link's source is a float plane either way, so
-      // one branch-free load per direction (a direction without a link reads
-      // the cell itself; the factor 2 of a population read back is applied
-      // at the sum) and no load waits for another
-      Fw[0] = 0.0f;
-#pragma unroll
-      for (int s = 1; s < Q; ++s) {
-        const bool has = (info & (1u << s)) != 0;
-        const uint32_t x = has ? c + (uint32_t)nb.d(LR[s]) : c;
-        const float* src = (wall_moving || ff.of_source(s)) ? link + (size_t)(s - 1) * N
-                                                            : pop_plane_f32(Fs, N, LR[s]);
-        Fw[s] = at32(src, x);
-      }
-    } else {
-#pragma unroll
-      for (int s = 0; s < Q; ++s) {
-        Fw[s] = 0.0f;
-        if (s != 0 && (info & (1u << s))) {
-          const uint32_t x = c + (uint32_t)nb.d(LR[s]);
-          if (wall_moving || ff.of_source(s))
-            Fw[s] = at32(link + (size_t)(s - 1) * N, x);
-          else
-            Fw[s] = 2.0f * Fs.ld(N, LR[s], x);
-        }
-      }
-    }
-    const double* a = st.hydro_accum + 4 * (size_t)c;
-    acc[0] = a[0];
-    acc[1] = a[1];
-    acc[2] = a[2];
-    acc[3] = a[3];
+    wall_links_load<Pop>(st, Fs, parity, N, c, info, nb, ff, Fw, acc);
   }
   if (!solid) {
     lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, flags, i, j, k, N, c, info, nb, ff, F);
     return;
   }
-  // wall cell: the force of its links is sum_s e_s c_s; added to the accumulator
-  const bool wall_moving = (info & CI_MOVING) != 0;
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-#pragma unroll
-  for (int s = 1; s < Q; ++s) {
-    float cs = Fw[s];
-    if constexpr (Pop::kF32) cs = !(info & (1u << s)) ? 0.0f : ((wall_moving || ff.of_source(s)) ? cs : 2.0f * cs);
-    fx += (float)EX[s] * cs;
-    fy += (float)EY[s] * cs;
-    fz += (float)EZ[s] * cs;
-  }
-  wall_accumulate_loaded(st.hydro_accum, c, acc, fx, fy, fz);
+  wall_links_accumulate<Pop>(st, c, info, ff, Fw, acc);
 }
 
 // Collide, push, bounce back and apply the x-face BCs for fluid cell c whose
@@ -861,6 +889,7 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
       }
     }
   }
+  N = wave_uniform(N);
 #pragma unroll
   for (int s = 0; s < Q; ++s) {
     if (s != 0 && (info & (1u << s))) {

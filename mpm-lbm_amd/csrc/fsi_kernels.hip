@@ -520,6 +520,15 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
 // on the chip in one wave generation at 4 waves per SIMD runs without a cap
 // (MINW 1: more registers, fewer waits); a larger grid keeps 5 waves per SIMD
 // resident to hide memory latency and loads populations speculatively.
+//
+// Wall cells of a gathering moving-wall substep, in a one-generation grid:
+// a wave's row of cells typically holds a wall cell at each end, and run in
+// place each of them would cost its whole wave the wall work with the other
+// lanes idle.  Instead the workgroup queues its wall cells in LDS and one
+// wave runs them (one more wave per further 64), chosen by the workgroup
+// index so that the four SIMDs share the wall work.  That wave issues the
+// queued cells' link loads together with its own cells' population loads,
+// so it still waits for memory once.
 template <typename Pop, uint32_t FIXED, int MINW>
 __global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
                                                            uint32_t flags, unsigned tiles_per_plane) {
@@ -527,12 +536,69 @@ __global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLb
   const unsigned i = L / tiles_per_plane;
   const unsigned jk = (L - i * tiles_per_plane) * TPB + threadIdx.x;
   const unsigned plane = (unsigned)p.ny * (unsigned)p.nz;
-  if (jk >= plane) return;
-  const unsigned j = jk / (unsigned)p.nz;
-  const unsigned k = jk - j * (unsigned)p.nz;
-  lbm_substep_cell<Pop, (MINW > 1)>(p, st, cur, parity,
-                        (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED,
-                        (int)i, (int)j, (int)k);
+  const uint32_t fl = (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED;
+  if (MINW > 1 || !(FIXED & FSI_SUB_MOVING_WALL) || !(fl & FSI_SUB_GATHER_PREV)) {
+    if (jk >= plane) return;
+    const unsigned j = jk / (unsigned)p.nz;
+    const unsigned k = jk - j * (unsigned)p.nz;
+    lbm_substep_cell<Pop, (MINW > 1)>(p, st, cur, parity, fl, (int)i, (int)j, (int)k);
+    return;
+  }
+  constexpr unsigned WAVES = TPB / 64;
+  __shared__ uint32_t q_jk[TPB], q_info[TPB];
+  __shared__ uint32_t q_n[WAVES];
+  const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
+  const bool in = jk < plane;
+  const unsigned j = in ? jk / (unsigned)p.nz : 0u;
+  const unsigned k = in ? jk - j * (unsigned)p.nz : 0u;
+  const uint32_t c = ((uint32_t)i * (uint32_t)p.ny + j) * (uint32_t)p.nz + k;
+  const uint32_t info = in ? cell_info(st, c) : CI_SOLID;  // outside: solid without links
+  const bool wall = (info & CI_SOLID) && (info & CI_MASK);
+  const uint64_t bal = __ballot(wall);
+  const unsigned wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (wall) {
+    const unsigned at = wave * 64 + (unsigned)__popcll(bal & ((1ull << lane) - 1ull));
+    q_jk[at] = jk;
+    q_info[at] = info;
+  }
+  if (lane == 0) q_n[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t n[WAVES], total = 0;
+#pragma unroll
+  for (unsigned w = 0; w < WAVES; ++w) total += (n[w] = q_n[w]);
+  // queue entries 64t .. 64t+63 are run by wave (blockIdx.x + t) % WAVES
+  const unsigned trip = (wave + WAVES - blockIdx.x % WAVES) % WAVES;
+  uint32_t e = trip * 64 + lane;
+  const bool queued = e < total;
+  const Pop Fs = pop_of<Pop>(st.F[cur]);
+  const FaceFlags ff = face_flags(p, (int)i);
+  const bool fluid = !(info & CI_SOLID);
+  float F[Q];
+  if (fluid) {
+#pragma unroll
+    for (int s = 0; s < Q; ++s) F[s] = Fs.ld(N, s, c);
+  }
+  float Fw[Q];
+  double acc[4];
+  uint32_t wc = 0, winfo = 0;
+  if (queued) {
+    unsigned w = 0;
+#pragma unroll
+    for (unsigned v = 0; v + 1 < WAVES; ++v)
+      if (w == v && e >= n[v]) {
+        e -= n[v];
+        w = v + 1;
+      }
+    const uint32_t wjk = q_jk[w * 64 + e];
+    winfo = q_info[w * 64 + e];
+    const unsigned wj = wjk / (unsigned)p.nz, wk = wjk - wj * (unsigned)p.nz;
+    wc = ((uint32_t)i * (uint32_t)p.ny + wj) * (uint32_t)p.nz + wk;
+    wall_links_load<Pop>(st, Fs, parity, N, wc, winfo, nbr_of(p, (int)i, (int)wj, (int)wk), ff, Fw, acc);
+  }
+  if (fluid)
+    lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, fl, (int)i, (int)j, (int)k, N, c, info, nbr_of(p, (int)i, (int)j, (int)k),
+                        ff, F);
+  if (queued) wall_links_accumulate<Pop>(st, wc, winfo, ff, Fw, acc);
 }
 
 template <typename Pop>
