@@ -48,6 +48,14 @@ extern "C" {
 #define FSI_COLLIDE_MRT 0         /* the reference's MRT with Guo forcing (fluid.py:399-413) */
 #define FSI_COLLIDE_REGULARIZED 1 /* regularised BGK, tau = 1/s_v (no reference counterpart) */
 
+/* Open-boundary treatment of the x faces (FsiLbmParams.open_bc), the
+ * reference's lbm_open_boundary_semantics (fluid.py:538-570). */
+#define FSI_OPEN_BC_EQUILIBRIUM 0         /* equilibrium_all_population_reset: Boundary_condition_legacy,
+                                             applied by the face cells inside fsi_lbm_substep */
+#define FSI_OPEN_BC_REGULARIZED 1         /* regularized_velocity_pressure */
+#define FSI_OPEN_BC_REGULARIZED_LIMITED 2 /* regularized_velocity_pressure_limited */
+#define FSI_OPEN_BC_CONVECTIVE 3          /* convective_pressure_outlet_experimental */
+
 /* Substep flags. */
 #define FSI_SUB_STORED_MOMENTS 1u /* collide with stored rho/vel instead of moments of F_src */
 #define FSI_SUB_MOVING_WALL 2u    /* moving-wall bounce-back (streaming_moving_bounceback) */
@@ -65,6 +73,14 @@ typedef struct FsiLbmParams {
   float dynamic_solid_threshold;
   int32_t storage;   /* FSI_STORE_*: population storage in F[0], F[1] */
   int32_t collision; /* FSI_COLLIDE_*: collision operator */
+  int32_t open_bc;   /* FSI_OPEN_BC_*: x-face treatment; other than FSI_OPEN_BC_EQUILIBRIUM the face
+                        cells stream like inner cells and fsi_lbm_open_bc rewrites their unknown
+                        populations after every substep */
+  int32_t ob_limiter;       /* open_boundary_limiter_enabled: the limited mode clips rho, |u| and the
+                               non-equilibrium part and counts; the limited and convective modes
+                               apply the population floor when it is set */
+  int32_t ob_floor_enabled; /* open_boundary_population_floor is set */
+  float ob_rho_min, ob_rho_max, ob_u_max, ob_noneq_cap, ob_floor;
 } FsiLbmParams;
 
 /* Device-side counters of the LBM (all reductions are order-independent:
@@ -77,6 +93,13 @@ typedef struct FsiLbmCounters {
   uint32_t mean_norm_max_bits;
   int32_t pad0;
   double bb_net_solid_force[3];
+  /* open-boundary limiter counters, in the order rho clips, velocity clips,
+   * non-equilibrium clips, population floors, reconstructed populations:
+   * of the last fsi_lbm_open_bc, and summed over the run (cleared by the
+   * caller) */
+  uint32_t ob_step[5];
+  uint32_t pad1;
+  uint64_t ob_run[5];
 } FsiLbmCounters;
 
 typedef struct FsiLbmState {
@@ -291,6 +314,17 @@ int fsi_lbm_rebuild(const FsiLbmParams* p, const FsiLbmState* st, int dynamic, i
  * deterministically.  `parity` selects link_c[parity] for writing. */
 int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity,
                     uint32_t flags, void* stream);
+
+/* Boundary_condition on the x faces (fluid.py:538-570, :927-958,
+ * :1027-1111, :1349-1493) for F[buf] in place: p->open_bc selects the legacy
+ * reset of all populations or the regularized / limited / convective
+ * reconstruction of the unknown populations of the fluid face cells from
+ * their inner neighbours.  The neighbours' previous moments are the stored
+ * rho, vel with FSI_SUB_STORED_MOMENTS, else the moments of F[1-buf] (with
+ * cell_force under FSI_SUB_CELL_FORCE): after fsi_lbm_substep(cur, ...) call
+ * it with buf = 1 - cur and the substep's flags.  Sets ob_step and adds to
+ * ob_run of the counters. */
+int fsi_lbm_open_bc(const FsiLbmParams* p, const FsiLbmState* st, int buf, uint32_t flags, void* stream);
 
 /* streaming3 moments (fluid.py:1795-1814): rho, vel from F[cur]. */
 int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags,

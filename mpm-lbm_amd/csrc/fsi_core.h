@@ -857,13 +857,18 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
   else
     mrt_collide_m(F, m, ux, uy, uz, has_force, gx, gy, gz, p.s_v, p.s_other, f);
 
-  const bool self_face = ff.o;
-  const bool ok_xp = !ff.p;
-  const bool ok_xm = !ff.m;
+  // The legacy reset rewrites every population of a face cell here; the open
+  // modes leave the face cells streaming like inner cells (their unknown
+  // populations are rewritten by the open-boundary pass after the substep).
+  const bool legacy_bc = p.open_bc == FSI_OPEN_BC_EQUILIBRIUM;
+  const bool self_face = ff.o && legacy_bc;
+  const bool ok_xp = !(ff.p && legacy_bc);
+  const bool ok_xm = !(ff.m && legacy_bc);
   float* link = mb ? st.link_c[parity] : nullptr;
   const bool diag = mb && (flags & FSI_SUB_DIAG);
   const bool moving = mb && (info & CI_MOVING);
-  const bool keep_links = mb && (moving || self_face);  // walls cannot read these links from F
+  // walls cannot read these links back from F (a boundary rewrites them)
+  const bool keep_links = mb && (moving || ff.o);
   // Moving-wall corrections of the links, before the pushes: the wall
   // velocities are loaded 6 links at a time, each load branch-free (a
   // direction without a link reads the cell itself), so a cell next to the
@@ -940,6 +945,166 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
     }
 #pragma unroll
     for (int s = 0; s < Q; ++s) Fd.st(N, s, c, feq(s, brho, bx, by, bz));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary_condition on the x faces as a pass of its own (fluid.py:538-570)
+// ---------------------------------------------------------------------------
+constexpr int UNKNOWN_X_MIN[5] = {1, 7, 9, 11, 13};  // streamed in across x = 0 (fluid.py:26)
+constexpr int UNKNOWN_X_MAX[5] = {2, 8, 10, 12, 14};  // streamed in across x = nx-1 (fluid.py:27)
+
+// The moments of cell c that the reference's rho, v hold at Boundary_condition
+// time (those of the previous streaming3): the stored ones, or the moments of
+// the populations F_src the substep started from (rho0 and 0 on a solid cell).
+template <typename Pop>
+FSI_HD void prev_moments(const FsiLbmParams& p, const FsiLbmState& st, const Pop& Fs, uint32_t N, uint32_t flags,
+                         uint32_t c, float& rho, float* u) {
+  if (flags & FSI_SUB_STORED_MOMENTS) {
+    rho = at32(st.rho, c);
+    u[0] = at32(st.vel, c);
+    u[1] = at32(st.vel + N, c);
+    u[2] = at32(st.vel + 2 * (size_t)N, c);
+    return;
+  }
+  if (at32(st.cellinfo, c) & CI_SOLID) {
+    rho = p.rho0;
+    u[0] = u[1] = u[2] = 0.0f;
+    return;
+  }
+  float Fn[Q];
+#pragma unroll
+  for (int s = 0; s < Q; ++s) Fn[s] = Fs.ld(N, s, c);
+  float gx = p.force[0], gy = p.force[1], gz = p.force[2];
+  if (flags & FSI_SUB_CELL_FORCE) {
+    gx += at32(st.cell_force, c);
+    gy += at32(st.cell_force + N, c);
+    gz += at32(st.cell_force + 2 * (size_t)N, c);
+  }
+  moments_of(Fn, gx, gy, gz, rho, u[0], u[1], u[2]);
+}
+
+// _regularized_population and _limited_regularized_population (fluid.py:
+// 1027-1030, :1060-1104): feq(target) plus the neighbour's non-equilibrium
+// part Fn_s - feq(rho_n, u_n); the limited form clips the target density and
+// speed, the non-equilibrium part and the result, counting each clip in
+// cnt = {rho, velocity, non-equilibrium, floor, reconstructed}.
+FSI_HD float ob_regularized(const FsiLbmParams& p, bool limited, int s, float target_rho, const float* tu, float Fn_s,
+                            float rho_n, const float* un, uint32_t* cnt) {
+  float rl = target_rho;
+  float ul[3] = {tu[0], tu[1], tu[2]};
+  const bool lim = limited && p.ob_limiter;
+  if (lim) {
+    cnt[4] += 1;
+    if (rl < p.ob_rho_min) {
+      rl = p.ob_rho_min;
+      cnt[0] += 1;
+    }
+    if (rl > p.ob_rho_max) {
+      rl = p.ob_rho_max;
+      cnt[0] += 1;
+    }
+    const float norm = sqrtf(ul[0] * ul[0] + ul[1] * ul[1] + ul[2] * ul[2]);
+    if (norm > p.ob_u_max) {
+      const float sc = p.ob_u_max / norm;
+      ul[0] *= sc;
+      ul[1] *= sc;
+      ul[2] *= sc;
+      cnt[1] += 1;
+    }
+  }
+  float noneq = Fn_s - feq(s, rho_n, un[0], un[1], un[2]);
+  if (lim) {
+    if (noneq > p.ob_noneq_cap) {
+      noneq = p.ob_noneq_cap;
+      cnt[2] += 1;
+    }
+    if (noneq < -p.ob_noneq_cap) {
+      noneq = -p.ob_noneq_cap;
+      cnt[2] += 1;
+    }
+  }
+  float out = feq(s, rl, ul[0], ul[1], ul[2]) + noneq;
+  if (lim && p.ob_floor_enabled && out < p.ob_floor) {
+    out = p.ob_floor;
+    cnt[3] += 1;
+  }
+  return out;
+}
+
+// Boundary_condition for the face cell (j, k) of the left (right = false) or
+// right x face, on F[buf] in place: the legacy reset of all populations
+// (Boundary_condition_legacy, fluid.py:927-958) or the open modes' rewrite of
+// the unknown populations (fluid.py:1349-1493).  The inner neighbour is the
+// next cell inwards, or the face cell itself when that one is solid.  Each
+// unknown population reads only its own slot before writing it, so the
+// in-place rewrite is safe when the neighbour is the face cell.
+template <typename Pop>
+FSI_HD void lbm_open_bc_cell(const FsiLbmParams& p, const FsiLbmState& st, int buf, uint32_t flags, bool right, int j,
+                             int k, uint32_t* cnt) {
+  const int bc = right ? p.bc_x_right : p.bc_x_left;
+  if (bc != FSI_BC_PRESSURE && bc != FSI_BC_VELOCITY) return;
+  const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
+  const uint32_t jk = (uint32_t)j * (uint32_t)p.nz + (uint32_t)k, plane = (uint32_t)p.ny * (uint32_t)p.nz;
+  const uint32_t c0 = (right ? (uint32_t)(p.nx - 1) : 0u) * plane + jk;
+  if (at32(st.cellinfo, c0) & CI_SOLID) return;
+  const uint32_t c1 = (right ? (uint32_t)(p.nx - 2) : 1u) * plane + jk;
+  const bool inner_solid = (at32(st.cellinfo, c1) & CI_SOLID) != 0;
+  const Pop Fd = pop_of<Pop>(st.F[buf]), Fs = pop_of<Pop>(st.F[1 - buf]);
+  if (p.open_bc == FSI_OPEN_BC_EQUILIBRIUM) {
+    float brho = 1.0f, u[3];
+    if (bc == FSI_BC_PRESSURE) {
+      // the left face takes v[1] when cell 1 is solid and its own v
+      // otherwise; the right face takes v[nx-2] (fluid.py:928-950)
+      brho = right ? p.rho_bc_x_right : p.rho_bc_x_left;
+      float r;
+      prev_moments(p, st, Fs, N, flags, (right || inner_solid) ? c1 : c0, r, u);
+    } else {
+      const float* bv = right ? p.vel_bc_x_right : p.vel_bc_x_left;
+      u[0] = bv[0];
+      u[1] = bv[1];
+      u[2] = bv[2];
+    }
+#pragma unroll
+    for (int s = 0; s < Q; ++s) Fd.st(N, s, c0, feq(s, brho, u[0], u[1], u[2]));
+    return;
+  }
+  const uint32_t cn = inner_solid ? c0 : c1;
+  if (right && bc == FSI_BC_PRESSURE && p.open_bc == FSI_OPEN_BC_CONVECTIVE) {
+    // _convective_outlet_population (fluid.py:1106-1111): linear
+    // extrapolation from the two inner cells, then the population floor
+    const uint32_t c2 = inner_solid ? c0 : (uint32_t)(p.nx - 3) * plane + jk;
+    const bool c2_fluid = !(at32(st.cellinfo, c2) & CI_SOLID);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const int s = UNKNOWN_X_MAX[q];
+      float x = Fd.ld(N, s, cn);
+      if (c2_fluid) x = 2.0f * x - Fd.ld(N, s, c2);
+      if (p.ob_limiter && p.ob_floor_enabled && x < p.ob_floor) x = p.ob_floor;
+      Fd.st(N, s, c0, x);
+    }
+    return;
+  }
+  float rho_n, un[3];
+  prev_moments(p, st, Fs, N, flags, cn, rho_n, un);
+  float target_rho, tu[3];
+  if (bc == FSI_BC_PRESSURE) {
+    target_rho = right ? p.rho_bc_x_right : p.rho_bc_x_left;
+    tu[0] = un[0];
+    tu[1] = un[1];
+    tu[2] = un[2];
+  } else {
+    target_rho = rho_n <= 1.0e-6f ? p.rho0 : rho_n;
+    const float* bv = right ? p.vel_bc_x_right : p.vel_bc_x_left;
+    tu[0] = bv[0];
+    tu[1] = bv[1];
+    tu[2] = bv[2];
+  }
+  const bool limited = p.open_bc == FSI_OPEN_BC_REGULARIZED_LIMITED;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const int s = right ? UNKNOWN_X_MAX[q] : UNKNOWN_X_MIN[q];
+    Fd.st(N, s, c0, ob_regularized(p, limited, s, target_rho, tu, Fd.ld(N, s, cn), rho_n, un, cnt));
   }
 }
 

@@ -82,6 +82,26 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
   return FSI_OK;
 }
 
+int fsi_lbm_open_bc(const FsiLbmParams* p, const FsiLbmState* st, int buf, uint32_t flags, void*) {
+  if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
+  if (p->open_bc < FSI_OPEN_BC_EQUILIBRIUM || p->open_bc > FSI_OPEN_BC_CONVECTIVE)
+    return fail(FSI_E_INVALID, "open_bc must be one of FSI_OPEN_BC_*");
+  if (buf != 0 && buf != 1) return fail(FSI_E_INVALID, "buf must be 0 or 1");
+  if (!st->counters) return fail(FSI_E_INVALID, "open_bc needs counters");
+  uint32_t cnt[5] = {0u, 0u, 0u, 0u, 0u};
+  by_storage(p->storage, [&](auto tag) {
+    using Pop = decltype(tag);
+    for (int side = 0; side < 2; ++side)
+      for (int j = 0; j < p->ny; ++j)
+        for (int k = 0; k < p->nz; ++k) lbm_open_bc_cell<Pop>(*p, *st, buf, flags, side == 1, j, k, cnt);
+  });
+  for (int q = 0; q < 5; ++q) {
+    st->counters->ob_step[q] = cnt[q];
+    st->counters->ob_run[q] += cnt[q];
+  }
+  return FSI_OK;
+}
+
 int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags, void*) {
   by_storage(p->storage, [&](auto tag) {
     using Pop = decltype(tag);
@@ -90,8 +110,18 @@ int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint3
   return FSI_OK;
 }
 
+// the moving-boundary counters (the open-boundary ones run on)
+static void clear_mb_counters(FsiLbmCounters* k) {
+  k->bb_link_count = 0;
+  k->bb_max_correction_bits = 0;
+  k->mb_sample_count = 0;
+  k->accum_norm_max_bits = 0;
+  k->mean_norm_max_bits = 0;
+  k->bb_net_solid_force[0] = k->bb_net_solid_force[1] = k->bb_net_solid_force[2] = 0.0;
+}
+
 int fsi_lbm_mb_begin(const FsiLbmParams* p, const FsiLbmState* st, void*) {
-  memset(st->counters, 0, sizeof(FsiLbmCounters));
+  clear_mb_counters(st->counters);
   for (long long c = 0; c < ncells(p); ++c) lbm_mb_begin_cell(*p, *st, c);
   return FSI_OK;
 }
@@ -247,6 +277,8 @@ int fsi_batch_block_map(const FsiBatchInstance* inst, int n_inst, int domain, in
   if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_INSTANCES) return fail(FSI_E_INVALID, "unknown batch domain");
   long long total = 0;
   for (int i = 0; i < n_inst; ++i) {
+    if (inst[i].lp.open_bc != FSI_OPEN_BC_EQUILIBRIUM)
+      return fail(FSI_E_INVALID, "the batched form runs the legacy x-face reset only (open_bc 0)");
     if (!inst[i].active || !(mode_mask & (1 << inst[i].mode)) || inst[i].mpm_substeps <= min_mpm) continue;
     const long long nb = batch_domain_blocks(inst[i], domain);
     if (map)
@@ -325,7 +357,7 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
         });
         break;
       case FSI_BATCH_OP_LBM_MB_BEGIN:
-        memset(I.ls.counters, 0, sizeof(FsiLbmCounters));
+        clear_mb_counters(I.ls.counters);
         for (long long c = 0; c < cells; ++c) lbm_mb_begin_cell(I.lp, I.ls, c);
         break;
       case FSI_BATCH_OP_LBM_MB_FINALIZE: {

@@ -87,6 +87,8 @@ int check_lbm(const FsiLbmParams* p, const FsiLbmState* st) {
     return fail(FSI_E_INVALID, "storage must be FSI_STORE_F32, FSI_STORE_F16_DEV or FSI_STORE_BF16_DEV");
   if (p->collision != FSI_COLLIDE_MRT && p->collision != FSI_COLLIDE_REGULARIZED)
     return fail(FSI_E_INVALID, "collision must be FSI_COLLIDE_MRT or FSI_COLLIDE_REGULARIZED");
+  if (p->open_bc < FSI_OPEN_BC_EQUILIBRIUM || p->open_bc > FSI_OPEN_BC_CONVECTIVE)
+    return fail(FSI_E_INVALID, "open_bc must be one of FSI_OPEN_BC_*");
   if (!st->F[0] || !st->F[1] || !st->rho || !st->vel || !st->cellinfo || !st->solid)
     return fail(FSI_E_INVALID, "null LBM buffer");
   return FSI_OK;
@@ -599,6 +601,33 @@ __global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLb
     lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, fl, (int)i, (int)j, (int)k, N, c, info, nbr_of(p, (int)i, (int)j, (int)k),
                         ff, F);
   if (queued) wall_links_accumulate<Pop>(st, wc, winfo, ff, Fw, acc);
+}
+
+__global__ void __launch_bounds__(64) k_open_bc_clear(FsiLbmState st) {
+  if (threadIdx.x < 5) st.counters->ob_step[threadIdx.x] = 0u;
+}
+
+// x-face boundary pass: one thread per (j, k) of a face, blockIdx.y = face;
+// the limited mode's clip counters are summed per workgroup, then added
+template <typename Pop>
+__global__ void __launch_bounds__(TPB) k_lbm_open_bc(FsiLbmParams p, FsiLbmState st, int buf, uint32_t flags) {
+  __shared__ int si[TPB / 64];
+  const unsigned plane = (unsigned)p.ny * (unsigned)p.nz;
+  const unsigned t = blockIdx.x * TPB + threadIdx.x;
+  uint32_t cnt[5] = {0u, 0u, 0u, 0u, 0u};
+  if (t < plane) {
+    const unsigned j = t / (unsigned)p.nz;
+    lbm_open_bc_cell<Pop>(p, st, buf, flags, blockIdx.y == 1, (int)j, (int)(t - j * (unsigned)p.nz), cnt);
+  }
+  if (p.open_bc != FSI_OPEN_BC_REGULARIZED_LIMITED || !p.ob_limiter) return;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const int n = block_sum_i((int)cnt[q], si);
+    if (threadIdx.x == 0 && n > 0) {
+      atomicAdd(&st.counters->ob_step[q], (uint32_t)n);
+      atomicAdd((unsigned long long*)&st.counters->ob_run[q], (unsigned long long)n);
+    }
+  }
 }
 
 template <typename Pop>
@@ -1573,6 +1602,19 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
   return check_launch("k_lbm_substep");
 }
 
+int fsi_lbm_open_bc(const FsiLbmParams* p, const FsiLbmState* st, int buf, uint32_t flags, void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if (buf != 0 && buf != 1) return fail(FSI_E_INVALID, "buf must be 0 or 1");
+  if (!st->counters) return fail(FSI_E_INVALID, "open_bc needs counters");
+  if ((flags & FSI_SUB_CELL_FORCE) && !st->cell_force) return fail(FSI_E_INVALID, "cell_force is null");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_open_bc_clear, dim3(1), dim3(64), 0, s, *st);
+  const dim3 grid(blocks_for((long long)p->ny * p->nz), 2);
+  FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL(k_lbm_open_bc<POP>, grid, dim3(TPB), 0, s, *p, *st, buf, flags));
+  return check_launch("k_lbm_open_bc");
+}
+
 int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags, void* stream) {
   int rc = check_lbm(p, st);
   if (rc) return rc;
@@ -1767,6 +1809,8 @@ int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domai
   long long total = 0;
   for (int i = 0; i < n_inst; ++i) {
     const FsiBatchInstance& I = inst_host[i];
+    if (I.lp.open_bc != FSI_OPEN_BC_EQUILIBRIUM)
+      return fail(FSI_E_INVALID, "the batched form runs the legacy x-face reset only (open_bc 0)");
     if (!I.active || !(mode_mask & (1 << I.mode)) || I.mpm_substeps <= min_mpm) continue;
     const long long nb = batch_domain_blocks(I, domain);
     if (total + nb > 0x7fffffffLL) return fail(FSI_E_INVALID, "batch too large for one launch");

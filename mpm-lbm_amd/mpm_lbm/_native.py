@@ -32,6 +32,10 @@ STORE_F16_DEV = 1
 STORE_BF16_DEV = 2
 COLLIDE_MRT = 0
 COLLIDE_REGULARIZED = 1
+OPEN_BC_EQUILIBRIUM = 0
+OPEN_BC_REGULARIZED = 1
+OPEN_BC_REGULARIZED_LIMITED = 2
+OPEN_BC_CONVECTIVE = 3
 
 SUB_STORED_MOMENTS = 1
 SUB_MOVING_WALL = 2
@@ -52,6 +56,8 @@ class LbmParams(ctypes.Structure):
         ("vel_bc_x_left", f32 * 3), ("vel_bc_x_right", f32 * 3),
         ("dynamic_solid_threshold", f32),
         ("storage", i32), ("collision", i32),
+        ("open_bc", i32), ("ob_limiter", i32), ("ob_floor_enabled", i32),
+        ("ob_rho_min", f32), ("ob_rho_max", f32), ("ob_u_max", f32), ("ob_noneq_cap", f32), ("ob_floor", f32),
     ]
 
 
@@ -64,6 +70,8 @@ class LbmCounters(ctypes.Structure):
         ("mean_norm_max_bits", u32),
         ("pad0", i32),
         ("bb_net_solid_force", f64 * 3),
+        ("ob_step", u32 * 5), ("pad1", u32),
+        ("ob_run", ctypes.c_uint64 * 5),
     ]
 
 
@@ -200,7 +208,7 @@ ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, 
 
 EXPORTED_SYMBOLS = [
     "fsi_last_error", "fsi_version", "fsi_abi_sizes", "fsi_sync",
-    "fsi_lbm_init", "fsi_lbm_rebuild", "fsi_lbm_substep", "fsi_lbm_moments",
+    "fsi_lbm_init", "fsi_lbm_rebuild", "fsi_lbm_substep", "fsi_lbm_open_bc", "fsi_lbm_moments",
     "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats",
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
     "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_reduce_stats",
@@ -238,6 +246,7 @@ class Native:
         L.fsi_lbm_init.argtypes = [P(LbmParams), P(LbmState), vp]
         L.fsi_lbm_rebuild.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, vp]
         L.fsi_lbm_substep.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, u32, vp]
+        L.fsi_lbm_open_bc.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, u32, vp]
         L.fsi_lbm_moments.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, u32, vp]
         L.fsi_lbm_mb_begin.argtypes = [P(LbmParams), P(LbmState), vp]
         L.fsi_lbm_mb_finalize.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, ctypes.c_int,

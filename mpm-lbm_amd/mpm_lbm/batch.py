@@ -52,6 +52,10 @@ class BatchedFSI:
                 self.drivers.append(d)
         if not self.drivers:
             raise ValueError("BatchedFSI needs at least one instance")
+        for d in self.drivers:
+            if d.config.lbm_open_boundary_semantics != "equilibrium_all_population_reset":
+                raise NotImplementedError("the batched form runs the legacy x-face reset only "
+                                          f"(lbm_open_boundary_semantics={d.config.lbm_open_boundary_semantics!r})")
         self.native = self.drivers[0].lbm.native
         self.device = self.drivers[0].lbm.device
         mb = [d for d in self.drivers if d.config.coupling_mode == "moving_boundary"]

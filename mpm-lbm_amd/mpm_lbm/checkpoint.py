@@ -12,7 +12,8 @@ uninterrupted run would:
   hydrodynamic force of the last step;
 * MPM: particle positions, velocities, APIC C, deformation gradient F, Jp,
   mass, rest volume, the fixed-particle mask and pinned positions;
-* the driver's step counters and its diagnostics and monitor rows.
+* the driver's step counters, the open-boundary limiter's run counters and
+  its diagnostics and monitor rows.
 
 What is not saved is rebuilt or recomputed before it is read: the cell words
 (link masks), the MPM grid (cleared every substep), the moving-wall link
@@ -42,7 +43,7 @@ _PREFIX = "ckpt_"
 # the run that loads it.
 _GUARDED = ("coupling_mode", "fsi_exchange_mode", "n_grid", "lattice_shape", "n_particles",
             "lbm_substeps_per_fsi_step", "mpm_substeps_per_lbm_step", "lbm_population_storage",
-            "lbm_collision", "lbm_boundary_condition_mode", "target_u_lbm")
+            "lbm_collision", "lbm_boundary_condition_mode", "lbm_open_boundary_semantics", "target_u_lbm")
 
 _LBM_FIELDS = ("rho", "v", "F", "solid", "static_solid", "old_solid", "solid_phi", "solid_mass", "solid_vel",
                "cell_force", "hydro_force")
@@ -92,6 +93,7 @@ def save_fsi_checkpoint(driver, directory: str | Path, keep_last: int = 3) -> Pa
         "total_mpm_substeps": int(driver.total_mpm_substeps),
         "lbm_cell_force_active": bool(lbm._cell_force_active),
         "lbm_mb_sample_count": int(lbm.mb_sample_count),
+        "lbm_open_boundary_run_counters": lbm._ob_counters()[1],
         "diagnostics_rows": driver.diagnostics_rows,
         "flap_tip_monitor_rows": driver.flap_tip_monitor_rows,
         "fluent_like_monitor_rows": driver.fluent_like_monitor_rows,
@@ -150,6 +152,7 @@ def load_fsi_checkpoint(driver, path: str | Path) -> dict:
     lbm.drop_pending_links()
     lbm.hydro_force.from_numpy(arrays["lbm_hydro_force"])
     lbm.mb_sample_count = int(meta["lbm_mb_sample_count"])
+    lbm.set_open_boundary_limiter_run_counters(*meta.get("lbm_open_boundary_run_counters", [0] * 5))
     for k in _MPM_FIELDS:
         getattr(solid, k).from_numpy(arrays[f"mpm_{k}"])
     driver.current_lbm_step = int(meta["current_lbm_step"])

@@ -32,6 +32,14 @@ from .relaxation_semantics import (
 )
 
 EQUILIBRIUM_ALL_POPULATION_RESET = "equilibrium_all_population_reset"
+# lbm_open_boundary_semantics with a GPU form (fluid.py:14-19, :538-570)
+OPEN_BC_CODES = {
+    EQUILIBRIUM_ALL_POPULATION_RESET: _native.OPEN_BC_EQUILIBRIUM,
+    "regularized_velocity_pressure": _native.OPEN_BC_REGULARIZED,
+    "regularized_velocity_pressure_limited": _native.OPEN_BC_REGULARIZED_LIMITED,
+    "convective_pressure_outlet_experimental": _native.OPEN_BC_CONVECTIVE,
+}
+OB_COUNTERS = ("rho_clip", "velocity_clip", "noneq_clip", "population_floor", "reconstructed_population")
 UNKNOWN_X_MIN_POPULATIONS = (1, 7, 9, 11, 13)
 UNKNOWN_X_MAX_POPULATIONS = (2, 8, 10, 12, 14)
 
@@ -90,10 +98,10 @@ class LBMFluid3D:
             raise NotImplementedError(
                 "Step 2 only supports dense storage. Enable sparse storage after FSI coupling is validated."
             )
-        if config.open_boundary_semantics != EQUILIBRIUM_ALL_POPULATION_RESET:
+        if config.open_boundary_semantics not in OPEN_BC_CODES:
             raise NotImplementedError(
                 f"open_boundary_semantics={config.open_boundary_semantics!r} is not implemented on the GPU path; "
-                "only equilibrium_all_population_reset is"
+                f"implemented: {', '.join(OPEN_BC_CODES)}"
             )
         for side in ("bc_y_left", "bc_y_right", "bc_z_left", "bc_z_right"):
             if getattr(config, side) != 0:
@@ -113,6 +121,9 @@ class LBMFluid3D:
         self.rho0 = config.rho0
         self.relaxation_semantics = config.relaxation_semantics
         self.open_boundary_semantics = config.open_boundary_semantics
+        self._open_bc = OPEN_BC_CODES[config.open_boundary_semantics]
+        self.open_boundary_limiter_enabled = bool(config.open_boundary_limiter_enabled)
+        self.open_boundary_population_floor_enabled = config.open_boundary_population_floor is not None
         self.sparse_storage = False
         self.bc_x_left, self.bc_x_right = int(config.bc_x_left), int(config.bc_x_right)
         self.e = E.copy()
@@ -231,6 +242,12 @@ class LBMFluid3D:
         prm.dynamic_solid_threshold = 0.5
         prm.storage = self._storage_code
         prm.collision = _native.COLLIDE_REGULARIZED if self.collision == "regularized" else _native.COLLIDE_MRT
+        prm.open_bc = self._open_bc
+        prm.ob_limiter = int(self.open_boundary_limiter_enabled)
+        prm.ob_floor_enabled = int(self.open_boundary_population_floor_enabled)
+        prm.ob_rho_min, prm.ob_rho_max = float(c.open_boundary_rho_min), float(c.open_boundary_rho_max)
+        prm.ob_u_max, prm.ob_noneq_cap = float(c.open_boundary_u_max), float(c.open_boundary_noneq_cap)
+        prm.ob_floor = float(c.open_boundary_population_floor) if self.open_boundary_population_floor_enabled else -1.0e30
         self._params = prm
 
     def _stream(self):
@@ -374,6 +391,7 @@ class LBMFluid3D:
             self._call("fsi_lbm_substep", ctypes.byref(self._params), ctypes.byref(self._state), self.cur, parity,
                        flags, self._stream())
             self.cur = 1 - self.cur
+            self._open_bc_after_substep(flags)
             self._moments_fresh = False
             parity = 1 - parity
         last = 1 - parity
@@ -388,6 +406,56 @@ class LBMFluid3D:
             self._pending_accumulate = False
             self._hydro_from_links = True
         self._sync_moments()
+
+    # --------------------------------------------------------- open boundaries
+    def Boundary_condition(self):
+        """Boundary_condition (fluid.py:538-570) on the current populations in
+        place, with the stored rho, v as the neighbours' moments; clears the
+        step counters first (fluid.py:539).  Substeps apply it themselves."""
+        self._ensure_ready()
+        self._call("fsi_lbm_open_bc", ctypes.byref(self._params), ctypes.byref(self._state), self.cur,
+                   _native.SUB_STORED_MOMENTS, self._stream())
+
+    def _ob_counters(self):
+        k = read_struct(self._counters, _native.LbmCounters)
+        return [int(v) for v in k.ob_step], [int(v) for v in k.ob_run]
+
+    def _write_ob_counters(self, step=None, run=None):
+        k = read_struct(self._counters, _native.LbmCounters)
+        if step is not None:
+            k.ob_step[:] = [int(v) for v in step]
+        if run is not None:
+            k.ob_run[:] = [int(v) for v in run]
+        raw = np.frombuffer(bytes(k), dtype=np.uint8).copy()
+        self._counters.copy_(torch.from_numpy(raw).to(self._counters.device))
+
+    def clear_open_boundary_limiter_step_counters(self):
+        """fluid.py:572-587 (the limiter counters)."""
+        self._write_ob_counters(step=[0] * 5)
+
+    def clear_open_boundary_limiter_run_counters(self):
+        """fluid.py:590-637 (the limiter counters)."""
+        self._write_ob_counters(step=[0] * 5, run=[0] * 5)
+
+    def set_open_boundary_limiter_run_counters(self, rho_clip_count, velocity_clip_count, noneq_clip_count,
+                                               population_floor_count, reconstructed_population_count):
+        """fluid.py:639-693 (restart of the run counters)."""
+        self._write_ob_counters(step=[0] * 5, run=[rho_clip_count, velocity_clip_count, noneq_clip_count,
+                                                  population_floor_count, reconstructed_population_count])
+
+    def get_open_boundary_limiter_stats(self):
+        """The limiter part of fluid.py:804-925 (the flux-controller and
+        mass-neutral entries belong to open-boundary modes without a GPU form)."""
+        step, run = self._ob_counters()
+        out = {}
+        for name, a, b in zip(OB_COUNTERS, step, run):
+            out[f"{name}_count_step"] = a
+            out[f"{name}_count_run"] = b
+        out["limiter_activation_count_step"] = sum(step[:4])
+        out["limiter_activation_count"] = sum(run[:4])
+        out["limiter_activation_denominator"] = run[4]
+        out["limiter_activation_fraction"] = float(sum(run[:4]) / run[4] if run[4] else 0.0)
+        return out
 
     # -------------------------------------------------------- lazy state sync
     def _ensure_ready(self):
@@ -404,7 +472,15 @@ class LBMFluid3D:
         self._call("fsi_lbm_substep", ctypes.byref(self._params), ctypes.byref(self._state), self.cur, parity, flags,
                    self._stream())
         self.cur = 1 - self.cur
+        self._open_bc_after_substep(flags)
         self._moments_fresh = False
+
+    def _open_bc_after_substep(self, flags):
+        """The open modes' Boundary_condition on the substep's output F[cur]
+        (the legacy reset runs inside the substep)."""
+        if self._open_bc != _native.OPEN_BC_EQUILIBRIUM:
+            self._call("fsi_lbm_open_bc", ctypes.byref(self._params), ctypes.byref(self._state), self.cur,
+                       flags & (_native.SUB_STORED_MOMENTS | _native.SUB_CELL_FORCE), self._stream())
 
     def _sync_moments(self):
         if self._moments_fresh or self._params is None:

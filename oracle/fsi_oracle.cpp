@@ -363,6 +363,133 @@ void bc_legacy(const LbmArgs<T>& a, T* F, const T* v, const int8_t* solid) {
     }
 }
 
+// Open boundaries of Boundary_condition (fluid.py:538-570) other than the
+// legacy reset, on the x faces: the regularized reconstruction of the unknown
+// populations (the ones streamed in across the face, fluid.py:26-27) from the
+// inner neighbour's non-equilibrium part (fluid.py:1027-1030, :1349-1395), its
+// limited form with clip counters (:1060-1104, :1397-1442), and the convective
+// pressure outlet (:1106-1111 with :1051-1058, :1445-1493).  The neighbour's
+// rho, v are those of the previous streaming3; its populations are the
+// post-stream ones this step.
+enum { OB_LEGACY = 0, OB_REGULARIZED = 1, OB_REGULARIZED_LIMITED = 2, OB_CONVECTIVE = 3 };
+
+template <typename T>
+struct ObArgs {
+  int mode, limiter, floor_enabled;
+  T rho_min, rho_max, u_max, noneq_cap, floor;
+};
+
+// rho clips, velocity clips, non-equilibrium clips, population floors,
+// reconstructed populations (the reference's ob_*_count_step)
+struct ObCounts {
+  long long c[5];
+};
+
+static const int UNKNOWN_X_MIN[5] = {1, 7, 9, 11, 13};
+static const int UNKNOWN_X_MAX[5] = {2, 8, 10, 12, 14};
+
+// _regularized_population / _limited_regularized_population
+template <typename T>
+T ob_regularized(const ObArgs<T>& ob, bool limited, int s, T target_rho, const T* target_u, const T* Fn, T rho_n,
+                 const T* v_n, ObCounts* k) {
+  T rl = target_rho;
+  T ul[3] = {target_u[0], target_u[1], target_u[2]};
+  const bool lim = limited && ob.limiter;
+  if (lim) {
+    k->c[4] += 1;
+    if (rl < ob.rho_min) {
+      rl = ob.rho_min;
+      k->c[0] += 1;
+    }
+    if (rl > ob.rho_max) {
+      rl = ob.rho_max;
+      k->c[0] += 1;
+    }
+    const T norm = sqrt(ul[0] * ul[0] + ul[1] * ul[1] + ul[2] * ul[2]);
+    if (norm > ob.u_max) {
+      const T sc = ob.u_max / norm;
+      for (int q = 0; q < 3; ++q) ul[q] = ul[q] * sc;
+      k->c[1] += 1;
+    }
+  }
+  T noneq = Fn[s] - feq<T>(s, rho_n, v_n);
+  if (lim) {
+    if (noneq > ob.noneq_cap) {
+      noneq = ob.noneq_cap;
+      k->c[2] += 1;
+    }
+    if (noneq < -ob.noneq_cap) {
+      noneq = -ob.noneq_cap;
+      k->c[2] += 1;
+    }
+  }
+  T out = feq<T>(s, rl, ul) + noneq;
+  if (lim && ob.floor_enabled && out < ob.floor) {
+    out = ob.floor;
+    k->c[3] += 1;
+  }
+  return out;
+}
+
+template <typename T>
+void bc_open(const LbmArgs<T>& a, const ObArgs<T>& ob, T* F, const T* rho, const T* v, const int8_t* solid,
+             ObCounts* k) {
+  const Dims d{a.nx, a.ny, a.nz};
+  const bool limited = ob.mode == OB_REGULARIZED_LIMITED;
+  for (int q = 0; q < 5; ++q) k->c[q] = 0;
+  for (int j = 0; j < d.ny; ++j)
+    for (int kk = 0; kk < d.nz; ++kk) {
+      const long long c0 = d.idx(0, j, kk);
+      if ((a.bc_x_left != 1 && a.bc_x_left != 2) || solid[c0] != 0) continue;
+      const long long cn = solid[d.idx(1, j, kk)] > 0 ? c0 : d.idx(1, j, kk);
+      T target_rho;
+      const T* target_u;
+      if (a.bc_x_left == 1) {
+        target_rho = a.rho_bc_x_left;
+        target_u = v + 3 * cn;
+      } else {
+        target_rho = rho[cn] <= (T)1.0e-6 ? a.rho0 : rho[cn];
+        target_u = a.vel_bc_x_left;
+      }
+      for (int u = 0; u < 5; ++u) {
+        const int s = UNKNOWN_X_MIN[u];
+        F[c0 * Q + s] = ob_regularized<T>(ob, limited, s, target_rho, target_u, F + cn * Q, rho[cn], v + 3 * cn, k);
+      }
+    }
+  for (int j = 0; j < d.ny; ++j)
+    for (int kk = 0; kk < d.nz; ++kk) {
+      const long long c0 = d.idx(d.nx - 1, j, kk);
+      if ((a.bc_x_right != 1 && a.bc_x_right != 2) || solid[c0] != 0) continue;
+      const bool inner_solid = solid[d.idx(d.nx - 2, j, kk)] > 0;
+      const long long cn = inner_solid ? c0 : d.idx(d.nx - 2, j, kk);
+      if (a.bc_x_right == 1 && ob.mode == OB_CONVECTIVE) {
+        // _convective_outlet_population: linear extrapolation, then the floor
+        const long long c2 = inner_solid ? c0 : d.idx(d.nx - 3, j, kk);
+        for (int u = 0; u < 5; ++u) {
+          const int s = UNKNOWN_X_MAX[u];
+          T x = F[cn * Q + s];
+          if (solid[c2] == 0) x = (T)2.0 * F[cn * Q + s] - F[c2 * Q + s];
+          if (ob.limiter && ob.floor_enabled && x < ob.floor) x = ob.floor;
+          F[c0 * Q + s] = x;
+        }
+        continue;
+      }
+      T target_rho;
+      const T* target_u;
+      if (a.bc_x_right == 1) {
+        target_rho = a.rho_bc_x_right;
+        target_u = v + 3 * cn;
+      } else {
+        target_rho = rho[cn] <= (T)1.0e-6 ? a.rho0 : rho[cn];
+        target_u = a.vel_bc_x_right;
+      }
+      for (int u = 0; u < 5; ++u) {
+        const int s = UNKNOWN_X_MAX[u];
+        F[c0 * Q + s] = ob_regularized<T>(ob, limited, s, target_rho, target_u, F + cn * Q, rho[cn], v + 3 * cn, k);
+      }
+    }
+}
+
 // streaming3 (fluid.py:1795-1814)
 template <typename T>
 void streaming3(const LbmArgs<T>& a, const T* F, T* f, T* rho, T* v, const int8_t* solid, const T* cell_force) {
@@ -888,6 +1015,10 @@ void reaction(const LbmArgs<T>& la, const MpmArgs<T>& ma, const CplArgs<T>& ca, 
   }                                                                                                                 \
   ORC_EXPORT void orc_bc_legacy_##SFX(const LbmArgs<T>* a, T* F, const T* v, const int8_t* solid) {               \
     bc_legacy<T>(*a, F, v, solid);                                                                                \
+  }                                                                                                                 \
+  ORC_EXPORT void orc_bc_open_##SFX(const LbmArgs<T>* a, const ObArgs<T>* ob, T* F, const T* rho, const T* v,     \
+                                    const int8_t* solid, ObCounts* k) {                                             \
+    bc_open<T>(*a, *ob, F, rho, v, solid, k);                                                                      \
   }                                                                                                                 \
   ORC_EXPORT void orc_streaming3_##SFX(const LbmArgs<T>* a, const T* F, T* f, T* rho, T* v, const int8_t* solid,   \
                                        const T* cell_force) {                                                       \

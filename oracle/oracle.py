@@ -70,6 +70,45 @@ def _structs(T):
     return LbmArgs, MpmArgs, CplArgs, MbStats, ProjStats
 
 
+def _ob_args(T):
+    class ObArgs(ctypes.Structure):
+        _fields_ = [("mode", ctypes.c_int), ("limiter", ctypes.c_int), ("floor_enabled", ctypes.c_int),
+                    ("rho_min", T), ("rho_max", T), ("u_max", T), ("noneq_cap", T), ("floor", T)]
+    return ObArgs
+
+
+class ObCounts(ctypes.Structure):
+    """rho clips, velocity clips, non-equilibrium clips, population floors,
+    reconstructed populations of one Boundary_condition call."""
+    _fields_ = [("c", ctypes.c_longlong * 5)]
+
+
+OB_COUNT_NAMES = ("rho_clip_count", "velocity_clip_count", "noneq_clip_count", "population_floor_count",
+                  "reconstructed_population_count")
+# lbm_open_boundary_semantics -> the oracle's mode (fsi_oracle.cpp OB_*)
+OB_MODES = {"equilibrium_all_population_reset": 0, "regularized_velocity_pressure": 1,
+            "regularized_velocity_pressure_limited": 2, "convective_pressure_outlet_experimental": 3}
+_OB_ARGS = {np.float32: _ob_args(ctypes.c_float), np.float64: _ob_args(ctypes.c_double)}
+
+
+def ob_args(config, dtype=np.float32):
+    """The open-boundary options of an FSIDriverConfig or LBMConfig (the
+    reference's LBMFluid3D.__init__, fluid.py:49-81)."""
+    sem = (getattr(config, "lbm_open_boundary_semantics", None) or getattr(config, "open_boundary_semantics", None)
+           or "equilibrium_all_population_reset")
+    ob = _OB_ARGS[dtype]()
+    ob.mode = OB_MODES[sem]
+    ob.limiter = int(bool(getattr(config, "open_boundary_limiter_enabled", False)))
+    floor = getattr(config, "open_boundary_population_floor", None)
+    ob.floor_enabled = int(floor is not None)
+    ob.rho_min = getattr(config, "open_boundary_rho_min", 0.8)  # fsi_config.py:86-89 defaults
+    ob.rho_max = getattr(config, "open_boundary_rho_max", 1.2)
+    ob.u_max = getattr(config, "open_boundary_u_max", 0.1)
+    ob.noneq_cap = getattr(config, "open_boundary_noneq_cap", 0.05)
+    ob.floor = float(floor) if floor is not None else -1.0e30
+    return ob
+
+
 _STRUCTS = {np.float32: _structs(ctypes.c_float), np.float64: _structs(ctypes.c_double)}
 
 
@@ -139,6 +178,9 @@ class OracleFSI:
             for name in ("s_v", "s_other"):
                 setattr(la, name, float(np.float32(getattr(la, name))))
         self.la = la
+        self.ob = ob_args(config, dtype)
+        self.ob_step = ObCounts()
+        self.ob_run = np.zeros(5, np.int64)
         # lattice fields
         self.F = np.zeros(N * Q, T)
         self.f = np.zeros(N * Q, T)
@@ -216,11 +258,20 @@ class OracleFSI:
         self._call(name, ctypes.byref(self.la), _p(self.F), _p(self.f), _p(self.rho), _p(self.v),
                    _p(self.solid), _p(self.cell_force))
 
+    def boundary_condition(self):
+        """Boundary_condition (fluid.py:538-570): the legacy reset or an open-boundary reconstruction."""
+        if self.ob.mode == 0:
+            self._call("bc_legacy", ctypes.byref(self.la), _p(self.F), _p(self.v), _p(self.solid))
+            return
+        self._call("bc_open", ctypes.byref(self.la), ctypes.byref(self.ob), _p(self.F), _p(self.rho), _p(self.v),
+                   _p(self.solid), ctypes.byref(self.ob_step))
+        self.ob_run += np.asarray(self.ob_step.c[:], np.int64)
+
     def lbm_step(self):
         """step(): colission, streaming1, Boundary_condition, streaming3 (fluid.py:2099-2103)."""
         self.colission()
         self._call("streaming1", ctypes.byref(self.la), _p(self.f), _p(self.F), _p(self.solid))
-        self._call("bc_legacy", ctypes.byref(self.la), _p(self.F), _p(self.v), _p(self.solid))
+        self.boundary_condition()
         self._call("streaming3", ctypes.byref(self.la), _p(self.F), _p(self.f), _p(self.rho), _p(self.v),
                    _p(self.solid), _p(self.cell_force))
 
@@ -229,7 +280,7 @@ class OracleFSI:
         self.colission()
         self._call("streaming_mb", ctypes.byref(self.la), _p(self.f), _p(self.F), _p(self.solid), _p(self.rho),
                    _p(self.svel), _p(self.hydro), ctypes.byref(self.mbstats))
-        self._call("bc_legacy", ctypes.byref(self.la), _p(self.F), _p(self.v), _p(self.solid))
+        self.boundary_condition()
         self._call("streaming3", ctypes.byref(self.la), _p(self.F), _p(self.f), _p(self.rho), _p(self.v),
                    _p(self.solid), _p(self.cell_force))
 
