@@ -176,6 +176,8 @@ typedef struct FsiCouplingParams {
   float force_density_scale;      /* dx_norm / lbm_dt_phys^2 */
   float reaction_scale;
   float mb_phi_min, mb_force_cap_norm;
+  float area_scale;               /* moving-boundary reaction factor of the link-area transfer
+                                     (link_area.py:150-157); 1 for the engineering transfer */
 } FsiCouplingParams;
 
 /* Device counters of projection and reaction transfer. */
@@ -341,6 +343,21 @@ int fsi_lbm_mb_begin(const FsiLbmParams* p, const FsiLbmState* st, void* stream)
  * last substep's link force. */
 int fsi_lbm_mb_finalize(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity,
                         int n_samples, int subcycled, void* stream);
+
+/* Per-direction statistics of the links of the last moving-wall substep
+ * (bb_*_by_dir of streaming_moving_bounceback, fluid.py:489-519; read by
+ * get_moving_boundary_directional_stats, fluid.py:2080-2090), which wrote
+ * its populations to F[cur] and its link scalars to link_c[parity] with
+ * FSI_SUB_DIAG.  out_dev: 19 records of FSI_LINK_STATS_VALUES doubles (link
+ * count, solid force x, y, z, sum and max of |correction|), direction 0 all
+ * zero; the fluid impulse is the negated solid force.  workspace_dev holds
+ * FSI_LINK_STATS_WS_DOUBLES doubles.  Summed per workgroup in cell order,
+ * then in workgroup order: deterministic. */
+#define FSI_LINK_STATS_VALUES 6
+#define FSI_LINK_STATS_BLOCKS 64
+#define FSI_LINK_STATS_WS_DOUBLES (18 * FSI_LINK_STATS_BLOCKS * FSI_LINK_STATS_VALUES)
+int fsi_lbm_link_stats(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity, double* workspace_dev,
+                       double* out_dev, void* stream);
 
 /* Number of per-block partial records the two-pass reductions use; their
  * workspaces hold FSI_REDUCE_BLOCKS records of the reduced struct. */

@@ -948,6 +948,48 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
   }
 }
 
+// Link statistics of direction s for wall cell y after the last moving-wall
+// substep (bb_*_by_dir, fluid.py:489-519): if y has an incoming link from
+// x = y - e_s, add 1 link, the solid force e_s c_s (c_s = f_s + bounced, read
+// back as the wall gathers it) and |correction| to acc = {links, fx, fy, fz,
+// sum |corr|, max |corr|}.  s is a run-time direction here.
+template <typename Pop>
+FSI_HD void link_stats_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, int parity, int s, uint32_t y,
+                            double* acc) {
+  const uint32_t info = at32(st.cellinfo, y);
+  if (!(info & CI_SOLID) || !(info & (1u << s))) return;
+  int ex = 0, ey = 0, ez = 0, lr = 0;
+#pragma unroll
+  for (int q = 1; q < Q; ++q)
+    if (q == s) {
+      ex = EX[q];
+      ey = EY[q];
+      ez = EZ[q];
+      lr = LR[q];
+    }
+  const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
+  const uint32_t plane = (uint32_t)p.ny * (uint32_t)p.nz;
+  const uint32_t i = y / plane, jk = y - i * plane;
+  const uint32_t j = jk / (uint32_t)p.nz, k = jk - j * (uint32_t)p.nz;
+  const Nbr nb = nbr_of(p, (int)i, (int)j, (int)k);
+  const FaceFlags ff = face_flags(p, (int)i);
+  // x = y - e_s = y + e_LR(s)
+  const int32_t d = (ex < 0 ? nb.xp : (ex > 0 ? nb.xm : 0)) + (ey < 0 ? nb.yp : (ey > 0 ? nb.ym : 0)) +
+                    (ez < 0 ? nb.zp : (ez > 0 ? nb.zm : 0));
+  const uint32_t x = y + (uint32_t)d;
+  const bool x_face = ex > 0 ? ff.m : (ex < 0 ? ff.p : ff.o);
+  const Pop Fc = pop_of<Pop>(st.F[cur]);
+  const float cs = ((info & CI_MOVING) || x_face) ? at32(st.link_c[parity] + (size_t)(s - 1) * N, x)
+                                                   : 2.0f * Fc.ld(N, lr, x);
+  const float ca = fabsf(at32(st.link_corr + (size_t)(s - 1) * N, x));
+  acc[0] += 1.0;
+  acc[1] += (double)((float)ex * cs);
+  acc[2] += (double)((float)ey * cs);
+  acc[3] += (double)((float)ez * cs);
+  acc[4] += (double)ca;
+  if ((double)ca > acc[5]) acc[5] = (double)ca;
+}
+
 // ---------------------------------------------------------------------------
 // Boundary_condition on the x faces as a pass of its own (fluid.py:538-570)
 // ---------------------------------------------------------------------------
@@ -1144,7 +1186,8 @@ struct FinalizeOut {
   float max_corr;
   float mean_norm;
   float runmax;
-  double hx, hy, hz;  // last substep's hydro force on this cell
+  double hx, hy, hz;  // last substep's link forces on this cell, summed link by link in fp64 (net force)
+  float fx, fy, fz;   // the same summed in fp32 in link order: the cell's hydro force
 };
 
 // finalize_moving_boundary_diagnostics / finalize_moving_boundary_force_accumulator
@@ -1161,6 +1204,7 @@ FSI_HD FinalizeOut lbm_finalize_cell(const FsiLbmParams& p, const FsiLbmState& s
   o.mean_norm = 0.0f;
   o.runmax = 0.0f;
   o.hx = o.hy = o.hz = 0.0;
+  o.fx = o.fy = o.fz = 0.0f;
   const uint32_t info = st.cellinfo[y];
   if ((info & CI_SOLID) && (info & CI_MASK)) {
     int i, j, k;
@@ -1181,13 +1225,18 @@ FSI_HD FinalizeOut lbm_finalize_cell(const FsiLbmParams& p, const FsiLbmState& s
         fx += (float)EX[s] * cs;
         fy += (float)EY[s] * cs;
         fz += (float)EZ[s] * cs;
+        // the net force sums link by link (the reference's bb_net_solid_force
+        // is the sum of its per-direction link sums, fluid.py:522-536)
+        o.hx += (double)((float)EX[s] * cs);
+        o.hy += (double)((float)EY[s] * cs);
+        o.hz += (double)((float)EZ[s] * cs);
         const float a = fabsf(at32(st.link_corr + (size_t)(s - 1) * N, x));
         if (a > o.max_corr) o.max_corr = a;
       }
     }
-    o.hx = fx;
-    o.hy = fy;
-    o.hz = fz;
+    o.fx = fx;
+    o.fy = fy;
+    o.fz = fz;
     if (subcycled) wall_accumulate(st.hydro_accum, y, fx, fy, fz);
   }
   if (subcycled) {
@@ -1202,9 +1251,9 @@ FSI_HD FinalizeOut lbm_finalize_cell(const FsiLbmParams& p, const FsiLbmState& s
     o.mean_norm = norm3(mx, my, mz);
     o.runmax = (float)a[3];
   } else {
-    st.hydro_force[y] = (float)o.hx;
-    st.hydro_force[g.N + y] = (float)o.hy;
-    st.hydro_force[2 * g.N + y] = (float)o.hz;
+    st.hydro_force[y] = o.fx;
+    st.hydro_force[g.N + y] = o.fy;
+    st.hydro_force[2 * g.N + y] = o.fz;
   }
   return o;
 }
@@ -1917,6 +1966,11 @@ FSI_HD ReactOut reaction_from_gather(const FsiCouplingParams& cp, int mode, floa
                  hy * cp.force_density_scale * vol * cp.reaction_scale,
                  hz * cp.force_density_scale * vol * cp.reaction_scale};
   if (mode == 2) {
+    // the link-area transfer's global factor, last as in link_area.py:150-157
+    // (1 for the engineering transfer, an exact no-op)
+    pf[0] *= cp.area_scale;
+    pf[1] *= cp.area_scale;
+    pf[2] *= cp.area_scale;
     float n = norm3(pf[0], pf[1], pf[2]);
     if (n > cp.mb_force_cap_norm) {
       const float s = cp.mb_force_cap_norm / n;

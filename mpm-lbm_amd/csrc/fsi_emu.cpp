@@ -102,6 +102,21 @@ int fsi_lbm_open_bc(const FsiLbmParams* p, const FsiLbmState* st, int buf, uint3
   return FSI_OK;
 }
 
+int fsi_lbm_link_stats(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity, double* ws, double* out,
+                       void*) {
+  if ((cur != 0 && cur != 1) || (parity != 0 && parity != 1)) return fail(FSI_E_INVALID, "cur and parity must be 0 or 1");
+  if (!st->link_c[0] || !st->link_c[1] || !st->link_corr || !ws || !out)
+    return fail(FSI_E_INVALID, "link_stats needs link buffers, link_corr, workspace, out");
+  for (int q = 0; q < 19 * FSI_LINK_STATS_VALUES; ++q) out[q] = 0.0;
+  by_storage(p->storage, [&](auto tag) {
+    using Pop = decltype(tag);
+    for (int s = 1; s < 19; ++s)
+      for (long long y = 0; y < ncells(p); ++y)
+        link_stats_cell<Pop>(*p, *st, cur, parity, s, (uint32_t)y, out + s * FSI_LINK_STATS_VALUES);
+  });
+  return FSI_OK;
+}
+
 int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags, void*) {
   by_storage(p->storage, [&](auto tag) {
     using Pop = decltype(tag);

@@ -630,6 +630,43 @@ __global__ void __launch_bounds__(TPB) k_lbm_open_bc(FsiLbmParams p, FsiLbmState
   }
 }
 
+// Per-direction link statistics: blockIdx.y + 1 is the direction; workgroup
+// b sums its contiguous share of the cells, thread-strided, into one record.
+template <typename Pop>
+__global__ void __launch_bounds__(TPB) k_link_stats_partial(FsiLbmParams p, FsiLbmState st, int cur, int parity,
+                                                            double* ws) {
+  __shared__ double sd[TPB / 64];
+  __shared__ float sf[TPB / 64];
+  const int s = (int)blockIdx.y + 1;
+  const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
+  const uint32_t per = (N + gridDim.x - 1) / gridDim.x;
+  const uint32_t c0 = blockIdx.x * per, c1 = c0 + per < N ? c0 + per : N;
+  double acc[FSI_LINK_STATS_VALUES] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (uint32_t y = c0 + threadIdx.x; y < c1; y += TPB) link_stats_cell<Pop>(p, st, cur, parity, s, y, acc);
+  double* rec = ws + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * FSI_LINK_STATS_VALUES;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double v = block_reduce(acc[q], RED_SUM, sd);
+    if (threadIdx.x == 0) rec[q] = v;
+  }
+  const float m = block_max_f((float)acc[5], sf);
+  if (threadIdx.x == 0) rec[5] = (double)m;
+}
+
+// fold: thread s-1 sums direction s's records in workgroup order
+__global__ void __launch_bounds__(64) k_link_stats_fold(const double* ws, int blocks, double* out) {
+  const int t = threadIdx.x;
+  if (t < FSI_LINK_STATS_VALUES) out[t] = 0.0;  // direction 0
+  if (t >= 18) return;
+  double a[FSI_LINK_STATS_VALUES] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < blocks; ++b) {
+    const double* r = ws + ((size_t)t * blocks + b) * FSI_LINK_STATS_VALUES;
+    for (int q = 0; q < 5; ++q) a[q] += r[q];
+    a[5] = r[5] > a[5] ? r[5] : a[5];
+  }
+  for (int q = 0; q < FSI_LINK_STATS_VALUES; ++q) out[(t + 1) * FSI_LINK_STATS_VALUES + q] = a[q];
+}
+
 template <typename Pop>
 __global__ void __launch_bounds__(TPB) k_lbm_moments(FsiLbmParams p, FsiLbmState st, int cur, uint32_t flags) {
   const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
@@ -1613,6 +1650,22 @@ int fsi_lbm_open_bc(const FsiLbmParams* p, const FsiLbmState* st, int buf, uint3
   const dim3 grid(blocks_for((long long)p->ny * p->nz), 2);
   FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL(k_lbm_open_bc<POP>, grid, dim3(TPB), 0, s, *p, *st, buf, flags));
   return check_launch("k_lbm_open_bc");
+}
+
+int fsi_lbm_link_stats(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity, double* workspace_dev,
+                       double* out_dev, void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if ((cur != 0 && cur != 1) || (parity != 0 && parity != 1))
+    return fail(FSI_E_INVALID, "cur and parity must be 0 or 1");
+  if (!st->link_c[0] || !st->link_c[1] || !st->link_corr || !workspace_dev || !out_dev)
+    return fail(FSI_E_INVALID, "link_stats needs link buffers, link_corr, workspace, out");
+  hipStream_t s = as_stream(stream);
+  FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL(k_link_stats_partial<POP>, dim3(FSI_LINK_STATS_BLOCKS, 18), dim3(TPB), 0,
+                                                s, *p, *st, cur, parity, workspace_dev));
+  hipLaunchKernelGGL(k_link_stats_fold, dim3(1), dim3(64), 0, s, (const double*)workspace_dev,
+                     (int)FSI_LINK_STATS_BLOCKS, out_dev);
+  return check_launch("k_link_stats");
 }
 
 int fsi_lbm_moments(const FsiLbmParams* p, const FsiLbmState* st, int cur, uint32_t flags, void* stream) {

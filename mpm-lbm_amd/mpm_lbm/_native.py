@@ -134,7 +134,13 @@ class CouplingParams(ctypes.Structure):
         ("beta_lbm", f32), ("penalty_phi_min", f32), ("force_cap_lbm", f32),
         ("force_density_scale", f32), ("reaction_scale", f32),
         ("mb_phi_min", f32), ("mb_force_cap_norm", f32),
+        ("area_scale", f32),
     ]
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        if "area_scale" not in kw:
+            self.area_scale = 1.0  # the engineering transfer's reaction is unscaled
 
 
 class CouplingCounters(ctypes.Structure):
@@ -208,7 +214,7 @@ ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, 
 
 EXPORTED_SYMBOLS = [
     "fsi_last_error", "fsi_version", "fsi_abi_sizes", "fsi_sync",
-    "fsi_lbm_init", "fsi_lbm_rebuild", "fsi_lbm_substep", "fsi_lbm_open_bc", "fsi_lbm_moments",
+    "fsi_lbm_init", "fsi_lbm_rebuild", "fsi_lbm_substep", "fsi_lbm_open_bc", "fsi_lbm_link_stats", "fsi_lbm_moments",
     "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats",
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
     "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_reduce_stats",
@@ -216,6 +222,8 @@ EXPORTED_SYMBOLS = [
 ]
 
 PARTICLE_BLOCK = 64  # FSI_PARTICLE_BLOCK
+LINK_STATS_VALUES = 6  # FSI_LINK_STATS_VALUES
+LINK_STATS_WS_DOUBLES = 18 * 64 * LINK_STATS_VALUES  # FSI_LINK_STATS_WS_DOUBLES
 
 
 def coupled_workspace_doubles(n_particles: int) -> int:
@@ -247,6 +255,7 @@ class Native:
         L.fsi_lbm_rebuild.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, vp]
         L.fsi_lbm_substep.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, u32, vp]
         L.fsi_lbm_open_bc.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, u32, vp]
+        L.fsi_lbm_link_stats.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, vp, vp, vp]
         L.fsi_lbm_moments.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, u32, vp]
         L.fsi_lbm_mb_begin.argtypes = [P(LbmParams), P(LbmState), vp]
         L.fsi_lbm_mb_finalize.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, ctypes.c_int,

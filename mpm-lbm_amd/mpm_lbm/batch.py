@@ -53,6 +53,9 @@ class BatchedFSI:
         if not self.drivers:
             raise ValueError("BatchedFSI needs at least one instance")
         for d in self.drivers:
+            if d.config.coupling_mode == "moving_boundary" and d.config.reaction_transfer_mode != "engineering":
+                raise NotImplementedError("the batched form runs the engineering reaction transfer only "
+                                          f"(reaction_transfer_mode={d.config.reaction_transfer_mode!r})")
             if d.config.lbm_open_boundary_semantics != "equilibrium_all_population_reset":
                 raise NotImplementedError("the batched form runs the legacy x-face reset only "
                                           f"(lbm_open_boundary_semantics={d.config.lbm_open_boundary_semantics!r})")

@@ -22,6 +22,7 @@ from pathlib import Path
 import numpy as np
 
 from ...diagnostics.fsi_diagnostics import FSIDiagnostics3D
+from ..coupling.link_area import LinkAreaMovingBoundaryCoupler3D
 from ..coupling.moving_boundary import MovingBoundaryFSICoupler3D
 from ..coupling.penalty import PenaltyFSICoupler3D
 from ..coupling.projection import MPMToLBMProjector3D
@@ -110,8 +111,6 @@ class FSIDriver3D:
     # ------------------------------------------------------------- set-up
     def _unsupported_features(self):
         c = self.config
-        if c.reaction_transfer_mode == "link_area_experimental":
-            raise NotImplementedError("reaction_transfer_mode='link_area_experimental' is not implemented on the GPU path")
         if c.wall_velocity_application_mode != "disabled" or c.geometry_motion_application_mode != "disabled":
             raise NotImplementedError("prescribed wall-velocity / geometry-motion application is not implemented on the GPU path")
         if c.boundary_motion_report_enabled:
@@ -166,6 +165,12 @@ class FSIDriver3D:
         if self.config.coupling_mode == "moving_boundary" and self.config.reaction_transfer_mode == "engineering":
             self.mb_coupler = MovingBoundaryFSICoupler3D(self.sim, reaction_scale=self.config.mb_reaction_scale,
                                                          force_cap_norm=self.config.mb_force_cap_norm, phi_min=1.0e-6)
+        if (self.config.coupling_mode == "moving_boundary"
+                and self.config.reaction_transfer_mode == "link_area_experimental"):
+            self.link_area_coupler = LinkAreaMovingBoundaryCoupler3D(
+                self.sim, area_policy=self.config.link_area_policy, reaction_scale=self.config.mb_reaction_scale,
+                force_cap_norm=self.config.mb_force_cap_norm, phi_min=1.0e-6,
+                area_scale_min=self.config.link_area_scale_min, area_scale_max=self.config.link_area_scale_max)
         self.initialized = True
         self.start_time = time.perf_counter()
         self.timing["init_time"] += self.start_time - t0
@@ -410,8 +415,10 @@ class FSIDriver3D:
         self._eager_subcycled_steps = 0
 
     def _graph_ready(self) -> bool:
+        # the link-area transfer sets its scale on the host from the step's
+        # link statistics, between the LBM and MPM parts: no graph
         return (self.use_graph and self.lbm.device.type == "cuda" and self._eager_subcycled_steps >= 1
-                and self.config.lbm_substeps_per_fsi_step % 2 == 0)
+                and self.config.lbm_substeps_per_fsi_step % 2 == 0 and self.link_area_coupler is None)
 
     def _capture_subcycled_step(self):
         import torch
@@ -428,12 +435,16 @@ class FSIDriver3D:
         self._fsi_graph = graph
 
     def _advance_mpm_with_moving_boundary_reaction(self):
-        """fsi_driver.py:552-575 (engineering reaction transfer)."""
+        """fsi_driver.py:552-575 (engineering or link-area reaction transfer)."""
         t0 = time.perf_counter()
-        # clear_grid, p2g, clear_reaction_diagnostics, add_moving_boundary_reaction_to_mpm_grid,
+        coupler = self.mb_coupler
+        if self.link_area_coupler is not None:
+            self.link_area_coupler.update_area_scale_from_lbm(self.lbm)
+            coupler = self.link_area_coupler
+        # clear_grid, p2g, clear_reaction_diagnostics, add_*_reaction_to_mpm_grid,
         # grid_update, g2p - as one fused device sequence per substep
         for _ in range(self.config.mpm_substeps_per_lbm_step):
-            self.solid.substep_coupled(self.lbm, self.mb_coupler, 2)
+            self.solid.substep_coupled(self.lbm, coupler, 2)
             self.total_mpm_substeps += 1
         self.timing["mpm_substep_time"] += time.perf_counter() - t0
 
@@ -519,8 +530,8 @@ class FSIDriver3D:
         acc_mode = ("mean_force_accumulator"
                     if self.config.fsi_exchange_mode == "lbm_subcycled_per_fsi_step" and samples > 0 else "not_active")
         active_reaction, max_grid_reaction = 0, 0.0
-        if self.config.coupling_mode == "moving_boundary" and self.mb_coupler is not None:
-            st = self.mb_coupler.get_stats()
+        if self.config.coupling_mode == "moving_boundary" and (self.mb_coupler or self.link_area_coupler) is not None:
+            st = (self.mb_coupler or self.link_area_coupler).get_stats()
             active_reaction, max_grid_reaction = st["active_reaction_particle_count"], st["max_grid_reaction_norm"]
         elif self.config.coupling_mode == "penalty" and self.penalty_coupler is not None:
             max_grid_reaction = self.penalty_coupler.get_stats()["max_reaction_grid_force_norm"]

@@ -169,6 +169,8 @@ class LBMFluid3D:
         self._pending_accumulate = False
         self._hydro_from_links = False
         self.mb_sample_count = 0
+        self._last_link_parity = None  # link_c parity of the last moving-wall substep, while its links are current
+        self._link_stats_ws = None
         self.tau_f = None
         self.s_v = None
         self.s_other = None
@@ -324,6 +326,7 @@ class LBMFluid3D:
         self._pending = parity
         self._pending_accumulate = False
         self._hydro_from_links = True
+        self._last_link_parity = parity
 
     def update_dynamic_solid(self, threshold: float):
         """update_dynamic_solid + reinitialize_new_fluid_cells (fluid.py:1848-1877).
@@ -336,6 +339,7 @@ class LBMFluid3D:
         # the last substep's link forces are fixed by the flags they were made under
         self._sync_hydro()
         self._params.dynamic_solid_threshold = threshold
+        self._last_link_parity = None  # the flags no longer describe the last substep's links
         self._call("fsi_lbm_rebuild", ctypes.byref(self._params), ctypes.byref(self._state), 1, self.cur, self._stream())
         self._cellinfo_valid = True
         self._rebuilt_phi_version = key
@@ -395,6 +399,7 @@ class LBMFluid3D:
             self._moments_fresh = False
             parity = 1 - parity
         last = 1 - parity
+        self._last_link_parity = last
         if accumulate:
             self.mb_sample_count = n
             self._call("fsi_lbm_mb_finalize", ctypes.byref(self._params), ctypes.byref(self._state), self.cur, last,
@@ -462,11 +467,13 @@ class LBMFluid3D:
         if self._params is None:
             raise RuntimeError("call init_simulation() first")
         if not self._cellinfo_valid:
+            self._last_link_parity = None
             self._call("fsi_lbm_rebuild", ctypes.byref(self._params), ctypes.byref(self._state), 0, self.cur,
                        self._stream())
             self._cellinfo_valid = True
 
     def _launch_substep(self, flags, parity=0):
+        self._last_link_parity = None
         if self._cell_force_active:
             flags |= _native.SUB_CELL_FORCE
         self._call("fsi_lbm_substep", ctypes.byref(self._params), ctypes.byref(self._state), self.cur, parity, flags,
@@ -557,6 +564,32 @@ class LBMFluid3D:
             "bb_max_correction": bits_to_float(k.bb_max_correction_bits),
             "bb_net_fluid_impulse": tuple(-v for v in solid_force),
             "bb_net_solid_force": solid_force,
+        }
+
+    def get_moving_boundary_directional_stats(self):
+        """fluid.py:2080-2090: per-direction link statistics of the last
+        moving-wall substep (zeros before one, or once a flag rebuild has
+        followed it); the fluid impulse of a link is minus its solid force."""
+        out = np.zeros((19, _native.LINK_STATS_VALUES), dtype=np.float64)
+        if self._last_link_parity is not None and self._link is not None:
+            if self._link_stats_ws is None:
+                self._link_stats_ws = torch.zeros((_native.LINK_STATS_WS_DOUBLES + out.size,), dtype=torch.float64,
+                                                  device=self.device)
+            ws = self._link_stats_ws
+            res = ws[_native.LINK_STATS_WS_DOUBLES:]
+            self._call("fsi_lbm_link_stats", ctypes.byref(self._params), ctypes.byref(self._state), self.cur,
+                       self._last_link_parity, _native.ptr(ws), _native.ptr(res), self._stream())
+            out = res.cpu().numpy().reshape(19, _native.LINK_STATS_VALUES)
+        # fp64 sums (the reference's tables are fp32 atomics): the net force of
+        # get_moving_boundary_stats is summed in fp64 link by link too, so
+        # the two stay consistent where the directions' forces cancel
+        solid = out[:, 1:4].copy()
+        return {
+            "link_count_by_dir": out[:, 0].astype(np.int32),
+            "fluid_impulse_by_dir": -solid,
+            "solid_force_by_dir": solid,
+            "correction_abs_sum_by_dir": out[:, 4].copy(),
+            "correction_abs_max_by_dir": out[:, 5].copy(),
         }
 
     def get_moving_boundary_accumulation_stats(self):

@@ -849,6 +849,7 @@ struct CplArgs {
   T beta_lbm, penalty_phi_min, force_cap_lbm;
   T force_density_scale, reaction_scale;
   T mb_phi_min, mb_force_cap_norm;
+  T area_scale;  // link-area transfer's global factor (1 for the engineering transfer)
 };
 
 template <typename T>
@@ -961,6 +962,9 @@ void reaction(const LbmArgs<T>& la, const MpmArgs<T>& ma, const CplArgs<T>& ca, 
     T pf[3];
     for (int q = 0; q < 3; ++q) pf[q] = h[q] * ca.force_density_scale * vol * ca.reaction_scale;
     if (mode == 2) {
+      // add_link_area_reaction_to_mpm_grid (link_area.py:150-157) multiplies
+      // the engineering force by its area scale last
+      for (int q = 0; q < 3; ++q) pf[q] = pf[q] * ca.area_scale;
       T n = sqrt(pf[0] * pf[0] + pf[1] * pf[1] + pf[2] * pf[2]);
       if (n > ca.mb_force_cap_norm) {
         for (int q = 0; q < 3; ++q) pf[q] = pf[q] * (ca.mb_force_cap_norm / n);

@@ -54,7 +54,7 @@ def _structs(T):
     class CplArgs(ctypes.Structure):
         _fields_ = [(n, T) for n in ("inv_dx_norm", "cell_volume_norm", "vel_scale", "eps_mass", "beta_lbm",
                                      "penalty_phi_min", "force_cap_lbm", "force_density_scale",
-                                     "reaction_scale", "mb_phi_min", "mb_force_cap_norm")]
+                                     "reaction_scale", "mb_phi_min", "mb_force_cap_norm", "area_scale")]
 
     class MbStats(ctypes.Structure):
         _fields_ = [("link_count", ctypes.c_longlong), ("max_correction", T),
@@ -242,6 +242,7 @@ class OracleFSI:
         ca.force_density_scale = prm["dx_norm"] / prm["dt_lbm"] ** 2
         ca.reaction_scale = config.mb_reaction_scale if config.coupling_mode == "moving_boundary" else 1.0
         ca.mb_phi_min, ca.mb_force_cap_norm = 1.0e-6, config.mb_force_cap_norm
+        ca.area_scale = 1.0
         self.ca = ca
         self.proj_stats = ProjStats()
         self.reaction_stats = np.zeros(9, T)
@@ -346,6 +347,33 @@ class OracleFSI:
             self.g2p()
             self.total_mpm_substeps += 1
 
+    def update_area_scale(self):
+        """LinkAreaMovingBoundaryCoupler3D.update_area_scale_from_lbm
+        (link_area.py:84-109 with link_area_accounting.py:42-60, :86-110): the
+        area-weighted x solid force of the last substep's links over their
+        plain x solid force, clipped; stored as the reaction's float scale."""
+        c = self.config
+        length = np.linalg.norm(E.astype(np.float64), axis=1)
+        moving = length > 0
+        w = np.zeros(Q)
+        w[moving] = {"uniform": np.ones(Q), "inverse_length": 1.0 / np.where(moving, length, 1.0),
+                     "length": length}[c.link_area_policy][moving]
+        by_dir = np.asarray(self.mbstats.solid_force_by_dir[:], dtype=self.dtype).reshape(Q, 3)
+        weighted_x = np.sum(by_dir * w[:, None], axis=0, dtype=np.float64)[0]
+        net_x = float(self.mbstats.net_solid_force[0])
+        raw = abs(float(weighted_x)) / (abs(net_x) + 1.0e-12)
+        scale = float(np.clip(raw, c.link_area_scale_min, c.link_area_scale_max))
+        if not np.isfinite(scale):
+            scale = c.link_area_scale_min
+        self.ca.area_scale = float(self.dtype(scale))
+        self.raw_area_scale = raw
+        return scale
+
+    def _mb_reaction_substeps(self):
+        if getattr(self.config, "reaction_transfer_mode", "engineering") == "link_area_experimental":
+            self.update_area_scale()
+        self.mpm_substeps(2)
+
     # ---- driver step paths
     def step_once(self):
         c = self.config
@@ -358,7 +386,7 @@ class OracleFSI:
                 self.accumulate()
                 self.total_lbm_substeps += 1
             self.finalize_accumulator()
-            self.mpm_substeps(2)
+            self._mb_reaction_substeps()
             return
         if c.coupling_mode == "none":
             self.project()
@@ -373,7 +401,7 @@ class OracleFSI:
             self.project()
             self.update_dynamic_solid(c.dynamic_solid_threshold)
             self.lbm_step_moving_bounceback()
-            self.mpm_substeps(2)
+            self._mb_reaction_substeps()
         self.total_lbm_substeps += 1
 
     # ---- views in reference shapes
