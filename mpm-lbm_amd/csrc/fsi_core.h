@@ -708,22 +708,25 @@ FSI_HD uint32_t cell_info(const FsiLbmState& st, uint32_t c) { return at32(st.ce
 
 // Wall cell c (solid, with links) of a moving-wall substep: load the link
 // scalars c_s of the previous substep into Fw[1..18] and the cell's
-// accumulator record into acc.
+// accumulator record into acc.  A direction without a link reads element
+// idle of its plane instead (masked at the sum): the cell itself when the
+// wave's other lanes load its row anyway, else element 0, one line that all
+// such lanes share.
 template <typename Pop>
 FSI_HD void wall_links_load(const FsiLbmState& st, const Pop& Fs, int parity, uint32_t N, uint32_t c,
-                            uint32_t info, const Nbr& nb, const FaceFlags& ff, float* Fw, double* acc) {
+                            uint32_t info, const Nbr& nb, const FaceFlags& ff, float* Fw, double* acc,
+                            uint32_t idle) {
   const float* link = st.link_c[1 - parity];
   const bool wall_moving = (info & CI_MOVING) != 0;
   Fw[0] = 0.0f;
   if constexpr (Pop::kF32) {
     // fp32 populations: each link's source is a float plane either way, so
-    // one branch-free load per direction (a direction without a link reads
-    // the cell itself; the factor 2 of a population read back is applied
-    // at the sum) and no load waits for another
+    // one branch-free load per direction (the factor 2 of a population read
+    // back is applied at the sum) and no load waits for another
 #pragma unroll
     for (int s = 1; s < Q; ++s) {
       const bool has = (info & (1u << s)) != 0;
-      const uint32_t x = has ? c + (uint32_t)nb.d(LR[s]) : c;
+      const uint32_t x = has ? c + (uint32_t)nb.d(LR[s]) : idle;
       const float* src = (wall_moving || ff.of_source(s)) ? link + (size_t)(s - 1) * N
                                                           : pop_plane_f32(Fs, N, LR[s]);
       Fw[s] = at32(src, x);
@@ -811,7 +814,7 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
       for (int s = 0; s < Q; ++s) F[s] = Fs.ld(N, s, c);
     }
   } else {
-    wall_links_load<Pop>(st, Fs, parity, N, c, info, nb, ff, Fw, acc);
+    wall_links_load<Pop>(st, Fs, parity, N, c, info, nb, ff, Fw, acc, c);
   }
   if (!solid) {
     lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, flags, i, j, k, N, c, info, nb, ff, F);

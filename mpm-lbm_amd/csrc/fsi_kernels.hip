@@ -595,7 +595,7 @@ __global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLb
     winfo = q_info[w * 64 + e];
     const unsigned wj = wjk / (unsigned)p.nz, wk = wjk - wj * (unsigned)p.nz;
     wc = ((uint32_t)i * (uint32_t)p.ny + wj) * (uint32_t)p.nz + wk;
-    wall_links_load<Pop>(st, Fs, parity, N, wc, winfo, nbr_of(p, (int)i, (int)wj, (int)wk), ff, Fw, acc);
+    wall_links_load<Pop>(st, Fs, parity, N, wc, winfo, nbr_of(p, (int)i, (int)wj, (int)wk), ff, Fw, acc, 0u);
   }
   if (fluid)
     lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, fl, (int)i, (int)j, (int)k, N, c, info, nbr_of(p, (int)i, (int)j, (int)k),
