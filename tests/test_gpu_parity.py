@@ -49,6 +49,17 @@ def test_gpu_moving_boundary_subcycled_64(hip, tmp_path):
     _check(run_and_compare(d, o, 1), "mb64")
 
 
+def test_gpu_moving_boundary_subcycled_ragged(hip, tmp_path):
+    """A lattice whose (j, k) planes end in a partial workgroup tile (22 x 30
+    = 660 cells: two full tiles of 256 and one of 148), odd in every axis
+    length, so the substep's wall queue and the edge tiles are exercised."""
+    w = workloads.moving_boundary_flap(n=40, shape=(40, 22, 30), steps=2, substeps=10, flap=(2, 10, 8),
+                                       out_dir=str(tmp_path))
+    d, o = make_pair(w, tmp_path / "run", hip, None, with_fp64=True)
+    _check(run_and_compare(d, o, 2), "mb_ragged")
+    assert d.lbm.get_moving_boundary_stats()["bb_link_count"] == o.mbstats.link_count
+
+
 def test_gpu_moving_boundary_one_step(hip, tmp_path):
     w = workloads.moving_boundary_unsubcycled(n=32, steps=5, flap=(2, 16, 12), out_dir=str(tmp_path))
     d, o = make_pair(w, tmp_path / "run", hip, None, with_fp64=True)
