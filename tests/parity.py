@@ -28,8 +28,11 @@ def rel_max_err(a, b, mask=None, floor=0.0):
 #       rho / v (a 19-term sum of O(1) populations in reference order) reaches
 #       ~1.6e-5 of |v|, while the fused kernel stays ~1e-6 from FP64.
 #   (c) velocity-level particle fields only (particle v, APIC C, projected
-#       solid velocity): within the FP32 method error m = |FP32 - FP64 oracle|
-#       of the FP32 oracle, or within 2 m of the FP64 oracle (two FP32
+#       solid velocity, and the two forces linear in that velocity: the
+#       penalty cell force beta phi rho (u_s - u) and the hydrodynamic force,
+#       whose moving-wall terms carry u_s): within the FP32 method error
+#       m = |FP32 - FP64 oracle| of the FP32 oracle, or within 2 m of the
+#       FP64 oracle (two FP32
 #       realisations, each m from the exact answer, are up to 2 m apart).
 #       F is stored in FP32 as I + H with strains ~1e-6, so one rounding
 #       difference in grid momentum moves H by an ulp and the fixed-corotated
@@ -38,7 +41,7 @@ def rel_max_err(a, b, mask=None, floor=0.0):
 #       sequential sums, or the reference's own atomics run to run) differ
 #       there by ~1e-4 - 1e-3, always below |FP32 - FP64|.
 # Solid flags, counts and link counts are compared exactly.
-CONDITIONED_FIELDS = ("pv", "C", "solid_vel")
+CONDITIONED_FIELDS = ("pv", "C", "solid_vel", "cell_force", "hydro")
 
 
 def make_pair(workload, out_dir, native, device, dtype=np.float32, with_fp64=False):
