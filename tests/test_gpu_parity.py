@@ -129,17 +129,20 @@ def test_gpu_driver_run_writes_outputs(hip, tmp_path):
     assert (tmp_path / "out" / "flap_tip_displacement_timeseries.csv").is_file()
 
 
-def test_gpu_graph_replay_matches_eager(hip, tmp_path):
+@pytest.mark.parametrize("mpm_substeps", [10, 3])
+def test_gpu_graph_replay_matches_eager(hip, tmp_path, mpm_substeps):
     """The captured FSI-step graph against eager launches, every step, with
     eager diagnostics work between replays: both runs meet the parity bar
     against the oracle, their integer counters agree exactly and their lattice
     fields agree to 1e-5 (the particle scatter sums with float atomics, so the
-    two runs are not bit-identical)."""
+    two runs are not bit-identical).  Even and odd MPM substep counts per
+    captured step."""
     from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
     from oracle.oracle import OracleFSI
     from tests.parity import field_errors, rel_max_err
 
-    w = workloads.moving_boundary_flap(n=32, steps=5, substeps=20, flap=(2, 16, 12), out_dir=str(tmp_path))
+    w = workloads.moving_boundary_flap(n=32, steps=5, substeps=20, flap=(2, 16, 12), mpm_substeps=mpm_substeps,
+                                       out_dir=str(tmp_path))
     eager = FSIDriver3D(w.config, str(tmp_path / "eager"), particle_cloud=w.cloud, use_graph=False,
                         write_geometry_file=False)
     graph = FSIDriver3D(w.config, str(tmp_path / "graph"), particle_cloud=w.cloud, use_graph=True,
@@ -167,6 +170,7 @@ def test_gpu_graph_replay_matches_eager(hip, tmp_path):
         for label, d in (("eager", eager), ("graph", graph)):
             check_history([field_errors(d, o)], f"{label} step {step}", TOL)
     assert graph.total_lbm_substeps == eager.total_lbm_substeps == 5 * 20
+    assert graph.total_mpm_substeps == eager.total_mpm_substeps == 5 * mpm_substeps
 
 
 @pytest.mark.parametrize("mode", ["penalty", "moving_boundary"])
