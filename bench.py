@@ -52,9 +52,7 @@ def parse():
     ap.add_argument("--batch-exchanges", type=int, default=3, help="timed batched exchanges")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the served form (0 = skip)")
     ap.add_argument("--precision", type=int, default=0,
-                    help="FSI steps of the 192x96x96 precision comparison against the FP64 oracle (0 = skip)")
-    ap.add_argument("--precision-long", type=int, default=17,
-                    help="FSI steps of its GPU-only drift leg (16-bit storage against FP32)")
+                    help="FSI steps of the 192x96x96 precision form, 16-bit storage against FP32 (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of served requests, per s")
     return ap.parse_args()
 
@@ -207,96 +205,32 @@ def served_form(torch, rate, seconds):
     }
 
 
-def precision_form(torch, fsi_steps, long_steps):
+def precision_form(torch, long_steps):
     """192x96x96 duct with a 6x48x36-cell flap (82,944 particles, seed 3),
     regularised collision at tau = 0.55, subcycled moving boundary (120 LBM
-    substeps per FSI step).  FP32, FP16-deviation and BF16-deviation storage on
-    the GPU against the FP64 oracle after `fsi_steps` FSI steps, and the 16-bit
-    forms against FP32 after `long_steps` (GPU only)."""
-    import numpy as np
+    substeps per FSI step): FP16- and BF16-deviation storage against FP32
+    storage after `long_steps` FSI steps, all on the GPU.  (The comparison of
+    each storage against the FP64 oracle is tests/precision_report.py.)"""
+    from mpm_lbm import precision_case
 
-    from mpm_lbm import workloads
-    from mpm_lbm.sim.drivers.fsi_config import FSIDriverConfig
+    base, config, state, compare = precision_case.setup(max(1, long_steps))
     from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
-    from oracle.oracle import OracleFSI
-
-    shape, E, rho_s = (192, 96, 96), 1.0e5, 2.0
-    mpm_dt = 0.3 * (1.0 / shape[0]) / np.sqrt(E / rho_s)
-    base = workloads.moving_boundary_flap(shape=shape, flap=(6, 48, 36), seed=3, tau=0.55, substeps=120,
-                                          steps=max(fsi_steps, long_steps), mpm_dt=mpm_dt, E_mod=E, rho_s=rho_s,
-                                          out_dir="/tmp/fsi_bench_prec")
-    tip = base.cloud["free_tip_proxy_mask"] != 0
-    x0 = base.cloud["x"][tip].astype(np.float64).mean(0)
-
-    def config(storage):
-        return FSIDriverConfig(**{**base.config.to_dict(), "lbm_population_storage": storage,
-                                  "lbm_collision": "regularized"})
-
-    def state(rho, v, solid, x):
-        fluid = solid == 0
-        return {"rho": np.asarray(rho, np.float64), "v": np.asarray(v, np.float64), "fluid": fluid,
-                "mass": float(np.asarray(rho, np.float64)[fluid].sum()),
-                "tip": np.asarray(x, np.float64)[tip].mean(0) - x0}
 
     def run(storage, steps):
         d = FSIDriver3D(config(storage), f"/tmp/fsi_bench_prec/{storage}", particle_cloud=base.cloud,
                         write_geometry_file=False)
         d.initialize()
         s0 = state(d.lbm.rho.to_numpy(), d.lbm.v.to_numpy(), d.lbm.solid.to_numpy(), d.solid.x.to_numpy())
-        flags = d.lbm.solid.to_numpy()
         for _ in range(steps):
             d.step_once()
         torch.cuda.synchronize()
-        return flags, s0, state(d.lbm.rho.to_numpy(), d.lbm.v.to_numpy(), d.lbm.solid.to_numpy(), d.solid.x.to_numpy())
+        return s0, state(d.lbm.rho.to_numpy(), d.lbm.v.to_numpy(), d.lbm.solid.to_numpy(), d.solid.x.to_numpy())
 
-    def compare(a, ref, m0_ref, m0_a):
-        both = a["fluid"] & ref["fluid"]
-        # cells whose dynamic-solid test (phi >= threshold) came out differently, dilated by 2 cells:
-        # a tie at phi ~ threshold changes the bounce-back around it, whatever the precision
-        near = a["fluid"] != ref["fluid"]
-        for _ in range(2):
-            grown = near.copy()
-            for ax in range(3):
-                grown |= np.roll(near, 1, ax) | np.roll(near, -1, ax)
-            near = grown
-        away = both & ~near
-        du = np.abs(a["v"][both] - ref["v"][both])
-        du_away = np.abs(a["v"][away] - ref["v"][away])
-        umax = float(np.max(np.abs(ref["v"][both])))
-        drho = np.abs(a["rho"][both] - ref["rho"][both])
-        tipn = float(np.linalg.norm(ref["tip"]))
-        return {
-            "rho_max_err": float(drho.max()), "rho_mean_err": float(drho.mean()),
-            "rho_max_err_away_from_flag_flips": float(np.abs(a["rho"][away] - ref["rho"][away]).max()),
-            "u_max_rel_err": float(du.max() / umax), "u_mean_rel_err": float(du.mean() / umax),
-            "u_max_rel_err_away_from_flag_flips": float(du_away.max() / umax),
-            "tip_disp_rel_err": float(np.linalg.norm(a["tip"] - ref["tip"]) / tipn) if tipn > 0 else None,
-            "mass_drift": (a["mass"] - m0_a) / m0_a, "mass_drift_ref": (ref["mass"] - m0_ref) / m0_ref,
-            "flag_mismatch_cells": int(np.count_nonzero(a["fluid"] != ref["fluid"])),
-        }
-
-    out = {"workload": "192x96x96 duct, 6x48x36 flap, 82,944 particles, regularised collision tau=0.55, "
-                       "moving boundary, 120 substeps per FSI step", "fsi_steps_vs_fp64": fsi_steps,
-           "fsi_steps_16bit_vs_fp32": long_steps}
-    if fsi_steps > 0:
-        runs = {st: run(st, fsi_steps) for st in ("f32", "f16", "bf16")}
-        flags = runs["f32"][0]
-        t0 = time.perf_counter()
-        o = OracleFSI(config("f32"), flags, base.cloud, base.material, dtype=np.float64)
-        f = o.fields()
-        o0 = state(f["rho"], f["v"], f["solid"], f["x"])
-        for _ in range(fsi_steps):
-            o.step_once()
-        f = o.fields()
-        ref = state(f["rho"], f["v"], f["solid"], f["x"])
-        log(f"[bench] precision: FP64 oracle {fsi_steps} FSI steps in {time.perf_counter() - t0:.1f} s")
-        out["vs_fp64"] = {st: compare(r[2], ref, o0["mass"], r[1]["mass"]) for st, r in runs.items()}
-        out["vs_fp64"]["fp64_mass_drift"] = (ref["mass"] - o0["mass"]) / o0["mass"]
-    if long_steps > 0:
-        long = {st: run(st, long_steps) for st in ("f32", "f16", "bf16")}
-        ref32 = long["f32"][2]
-        out["16bit_vs_fp32"] = {st: compare(long[st][2], ref32, long["f32"][1]["mass"], long[st][1]["mass"])
-                                for st in ("f16", "bf16")}
+    out = {"workload": precision_case.WORKLOAD, "fsi_steps_16bit_vs_fp32": long_steps}
+    long = {st: run(st, long_steps) for st in ("f32", "f16", "bf16")}
+    ref0, ref32 = long["f32"]
+    out["16bit_vs_fp32"] = {st: compare(long[st][1], ref32, ref0["mass"], long[st][0]["mass"])
+                            for st in ("f16", "bf16")}
     log("[bench] precision: " + json.dumps(out))
     return out
 
@@ -385,7 +319,7 @@ def main():
 
     precision = None
     if args.precision > 0:
-        precision = precision_form(torch, args.precision, args.precision_long)
+        precision = precision_form(torch, args.precision)
 
     cpu = None
     if args.cpu_steps > 0:
