@@ -166,6 +166,7 @@ def batched_form(torch, count, exchanges):
         "plain_mlups": round(plain, 1),
         "speedup_vs_plain": round(batched / plain, 3),
         "ms_per_exchange": round(secs * 1e3 / exchanges, 3),
+        "substep_plan_tuning": b.tuning,
         "equal_error": "same FP32 kernels and per-instance arguments; batched runs checked against the oracle at "
                        "the plain path's bar (tests/test_gpu_batched.py)",
     }

@@ -25,18 +25,33 @@ from ._device import stream_handle, struct_buffer
 from .sim.drivers.fsi_driver import FSIDriver3D
 
 TPB = 256
+# Launch plans of the batched LBM substep.  Both run the same cell function
+# with the same flags per instance, so their results are identical; which is
+# faster depends on the mix (one launch per mode compiles the mode's flags in
+# but leaves each launch a share of the chip).
+SUBSTEP_PLANS = ("split", "mixed")
+# plan chosen per batch signature (instance shapes, modes, storage), this process
+_TUNED = {}
 _MODE = {"none": _native.BATCH_MODE_NONE, "penalty": _native.BATCH_MODE_PENALTY,
          "moving_boundary": _native.BATCH_MODE_MOVING_BOUNDARY}
 
 
 class BatchedFSI:
     def __init__(self, workloads=None, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True,
-                 drivers=None, n_substeps=None):
+                 drivers=None, n_substeps=None, substep_plan=None):
         """Batch `workloads` (a driver is built for each), or batch initialized
         `drivers` that already exist (they keep their state; a serving loop
         regroups its live instances this way).  n_substeps: LBM substeps per
         exchange (default: the moving-boundary instances' FSI step, else 120);
-        moving-boundary instances must use it as their FSI step."""
+        moving-boundary instances must use it as their FSI step.
+        substep_plan: how each LBM substep is launched, "split" (one launch
+        per coupling mode, the mode's flags compiled in) or "mixed" (one
+        launch over every instance, flags read per instance); None tunes it
+        on the first exchange (`SUBSTEP_PLANS`)."""
+        if substep_plan is not None and substep_plan not in SUBSTEP_PLANS:
+            raise ValueError(f"substep_plan must be one of {SUBSTEP_PLANS} or None")
+        self.substep_plan = substep_plan
+        self.tuning = None
         if drivers is not None:
             self.workloads = None
             self.drivers = list(drivers)
@@ -171,10 +186,7 @@ class BatchedFSI:
                 self._project(ONE)
                 self._clear_counters(1)
                 self._launch(_native.BATCH_OP_PENALTY, PEN)
-            # one launch per mode, each with its mode's substep flags compiled in
-            self._launch(_native.BATCH_OP_LBM_SUBSTEP_MB, MB, t=t)
-            self._launch(_native.BATCH_OP_LBM_SUBSTEP_PENALTY, PEN, t=t)
-            self._launch(_native.BATCH_OP_LBM_SUBSTEP_NONE, (_native.BATCH_MODE_NONE,), t=t)
+            self._issue_substep(t)
             if has_one:
                 self._launch(_native.BATCH_OP_LBM_MOMENTS, ONE, t=t)
                 self._mpm_substeps(ONE)
@@ -183,6 +195,58 @@ class BatchedFSI:
             self._launch(_native.BATCH_OP_LBM_MB_FINALIZE_FOLD, MB)
             self._launch(_native.BATCH_OP_LBM_MOMENTS, MB, t=self.n_substeps - 1)
             self._mpm_substeps(MB)
+
+    def _issue_substep(self, t):
+        plan = self.substep_plan
+        timer = None
+        if plan is None:
+            # tuning exchange: the plans take turns (t = 0 runs the moving-boundary
+            # instances' first-substep variant and is not timed)
+            plan = SUBSTEP_PLANS[t % len(SUBSTEP_PLANS)]
+            if t > 0:
+                timer = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                timer[0].record()
+        if plan == "mixed":
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP, tuple(_MODE.values()), t=t)
+        else:
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP_MB, (_native.BATCH_MODE_MOVING_BOUNDARY,), t=t)
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP_PENALTY, (_native.BATCH_MODE_PENALTY,), t=t)
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP_NONE, (_native.BATCH_MODE_NONE,), t=t)
+        if timer is not None:
+            timer[1].record()
+            self._timers.append((plan, timer))
+
+    def _signature(self):
+        return tuple((I.lp.nx, I.lp.ny, I.lp.nz, int(I.mode), int(I.lp.storage)) for I in self._inst_host)
+
+    def _tune_begin(self):
+        """Before an exchange: take a plan chosen earlier for the same batch, or
+        tune on this exchange (GPU only; on the CPU the plan is "split")."""
+        if self.substep_plan is not None:
+            return
+        key = self._signature()
+        if key in _TUNED:
+            self.substep_plan = _TUNED[key]
+            self.tuning = {"plan": self.substep_plan, "cached": True}
+        elif self.device.type != "cuda":
+            self.substep_plan = "split"
+        else:
+            self._timers = []
+
+    def _tune_end(self):
+        if self.substep_plan is not None:
+            return
+        torch.cuda.synchronize(self.device)
+        per = {p: [] for p in SUBSTEP_PLANS}
+        for plan, (a, b) in self._timers:
+            per[plan].append(a.elapsed_time(b) * 1000.0)
+        self._timers = []
+        mean = {p: float(np.mean(v)) if v else float("inf") for p, v in per.items()}
+        self.substep_plan = min(SUBSTEP_PLANS, key=lambda p: mean[p])
+        _TUNED[self._signature()] = self.substep_plan
+        self.tuning = {"plan": self.substep_plan, "cached": False,
+                       "us_per_substep": {p: round(v, 2) for p, v in mean.items()},
+                       "samples": {p: len(v) for p, v in per.items()}}
 
     # ------------------------------------------------------------------- run
     def step(self):
@@ -198,7 +262,9 @@ class BatchedFSI:
                 self._graph = g
             self._graph.replay()
         else:
+            self._tune_begin()
             self._issue_exchange()
+            self._tune_end()
         self.exchanges += 1
         self._sync_host_state()
 

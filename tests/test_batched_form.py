@@ -91,3 +91,25 @@ def test_block_map_helper_matches_python(emu, tmp_path):
                 emu.lib.fsi_batch_block_map(addr, len(ws), domain, mask, min_mpm, out.ctypes.data_as(ctypes.c_void_p))
                 got = [tuple(r) for r in out[:n].tolist()]
                 assert got == _expected_map(batch._inst_host, domain, modes, min_mpm), (domain, modes, min_mpm)
+
+
+def test_substep_launch_plans_agree(emu, tmp_path):
+    """The two launch plans of the batched substep (one launch per coupling
+    mode, flags compiled in; or one launch over every instance, flags read
+    per instance) give bit-identical instances; an unknown plan is refused."""
+    import pytest
+
+    ws = _mixed(tmp_path)
+    runs = {}
+    for plan in ("split", "mixed"):
+        b = BatchedFSI(ws, out_dir=str(tmp_path / plan), native=emu, device="cpu", substep_plan=plan)
+        b.step()
+        b.step()
+        runs[plan] = b
+    for da, db in zip(runs["split"].drivers, runs["mixed"].drivers):
+        for name in ("F", "rho", "v", "hydro_force", "cell_force", "solid_phi"):
+            assert np.array_equal(getattr(da.lbm, name).to_numpy(), getattr(db.lbm, name).to_numpy()), name
+        for name in ("x", "v", "F", "C"):
+            assert np.array_equal(getattr(da.solid, name).to_numpy(), getattr(db.solid, name).to_numpy()), name
+    with pytest.raises(ValueError, match="substep_plan"):
+        BatchedFSI(ws, out_dir=str(tmp_path / "bad"), native=emu, device="cpu", substep_plan="fast")

@@ -34,9 +34,13 @@ def test_gpu_batched_matches_oracle(hip, tmp_path):
         o = OracleFSI(w.config, flags, w.cloud, w.material, dtype=np.float32)
         o.fp64 = OracleFSI(w.config, flags, w.cloud, w.material, dtype=np.float64)
         oracles.append(o)
-    n_exchanges = 3  # eager, capture + replay, replay
+    n_exchanges = 3  # eager (tunes the substep launch plan), capture + replay, replay
     for _ in range(n_exchanges):
         batch.step()
+    from mpm_lbm.batch import SUBSTEP_PLANS
+
+    print("substep plan tuning", batch.tuning, flush=True)
+    assert batch.substep_plan in SUBSTEP_PLANS and batch.tuning["plan"] == batch.substep_plan
     for k, (d, o) in enumerate(zip(batch.drivers, oracles)):
         steps = n_exchanges if d.config.coupling_mode == "moving_boundary" else n_exchanges * batch.n_substeps
         for _ in range(steps):
