@@ -459,11 +459,15 @@ FSI_HD long long cell_of(const Grid& g, int i, int j, int k) {
   return ((long long)i * g.ny + j) * g.nz + k;
 }
 
+// Cell c of the lattice (0 <= c < N <= 2^30, checked by the C ABI) in 32-bit
+// division: a 64-bit division is a long software sequence on the GPU.
 FSI_HD void coords_of(const Grid& g, long long c, int& i, int& j, int& k) {
-  k = (int)(c % g.nz);
-  const long long t = c / g.nz;
-  j = (int)(t % g.ny);
-  i = (int)(t / g.ny);
+  const uint32_t cc = (uint32_t)c, nz = (uint32_t)g.nz, ny = (uint32_t)g.ny;
+  const uint32_t t = cc / nz;
+  k = (int)(cc - t * nz);
+  const uint32_t u = t / ny;
+  j = (int)(t - u * ny);
+  i = (int)u;
 }
 
 // periodic_index(i + e_s) (fluid.py:420-429)
