@@ -1,6 +1,6 @@
 """GPU parity: libfsi_hip.so (gfx950) through FSIDriver3D against the oracle.
 
-Bar (north star): FP32 fields within 1e-5 relative per FSI step, measured as
+Bar: FP32 fields within 1e-5 relative per FSI step, measured as
 max |gpu - oracle| / max |oracle| per field (C scaled by max(|C|, |v|/dx));
 solid flags bit-exact.  The one-LBM-step schedules (penalty, unsubcycled
 moving boundary) are held to that bar against the FP32 oracle on every field.
