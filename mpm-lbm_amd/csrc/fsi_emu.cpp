@@ -2,7 +2,7 @@
 //
 // Exports the same C ABI as fsi_kernels.hip (include/fsi_hip.h) and runs the
 // very same per-cell / per-particle bodies of fsi_core.h in serial loops over
-// host memory.  The GPU library is the product; this build lets the CPU test
+// host memory.  The GPU library is the product; this emulation lets the CPU test
 // suite check the kernels' arithmetic, indexing and boundary handling against
 // the oracle on a machine without a GPU.  Built into libfsi_emu.so, which no
 // product code loads.

@@ -1,7 +1,7 @@
 // fsi_oracle.cpp - CPU restatement of the reference FSI substep.
 //
 // TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
-// bench.py's cpu_baseline leg load this library, as the checker or as the
+// bench.py's cpu_baseline leg load this library, as the reference answer or the
 // timed CPU baseline; the product path never calls it.
 //
 // It restates lizhuoh9/MPM-LBM's Taichi kernels pass by pass, in the
