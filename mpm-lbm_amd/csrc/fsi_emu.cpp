@@ -480,4 +480,25 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
   return FSI_OK;
 }
 
+// Test-only entries of the emulation build: the per-point bodies the kernels
+// inline, exposed so the closed-form checks can call them directly.
+int fsi_emu_bspline(int n, const float* X, int32_t* base, float* fx, float* w) {
+  for (int i = 0; i < n; ++i) {
+    int b;
+    bspline(X[i], b, fx[i], w + 3 * i);
+    base[i] = b;
+  }
+  return FSI_OK;
+}
+
+int fsi_emu_regularized_collide(int n, const float* F, const float* rho, const float* u, const float* g, float sv,
+                                float* f) {
+  for (int c = 0; c < n; ++c) {
+    const bool has_force = g[3 * c] != 0.0f || g[3 * c + 1] != 0.0f || g[3 * c + 2] != 0.0f;
+    regularized_collide(F + Q * c, rho[c], u[3 * c], u[3 * c + 1], u[3 * c + 2], has_force, g[3 * c], g[3 * c + 1],
+                        g[3 * c + 2], sv, f + Q * c);
+  }
+  return FSI_OK;
+}
+
 }  // extern "C"

@@ -22,6 +22,19 @@
 #include <algorithm>
 #include <vector>
 
+// ORACLE_MUTANT plants one known mistake for the tests that show the
+// closed-form checks catch it (tests/test_oracle_regularized_bspline.py builds
+// each variant into a temporary directory).  0, the default, is the oracle.
+//   1  regularised collision: sign of the trace term of (e e - I/3) : Pi flipped
+//   2  regularised collision: 9/2 Hermite factor written as 3
+//   3  regularised collision: non-equilibrium relaxed by (1 - s_v/2)
+//   4  regularised collision: Guo prefactor (1 - s_v/2) dropped
+//   5  B-spline: middle weight centred on fx - 0.5
+//   6  B-spline: outer weights swapped
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 namespace {
 
 constexpr int Q = 19;
@@ -233,15 +246,18 @@ void colission_regularized(const LbmArgs<T>& a, const T* F, T* f, const T* rho, 
     const T tr = Pi[0][0] + Pi[1][1] + Pi[2][2];
     T G[3];
     for (int k = 0; k < 3; ++k) G[k] = a.fext[k] + (cell_force ? cell_force[3 * c + k] : (T)0);
+    const T h2 = ORACLE_MUTANT == 2 ? (T)3 : (T)4.5;
+    const T omega = ORACLE_MUTANT == 3 ? (T)1 - (T)0.5 * a.s_v : (T)1 - a.s_v;
+    const T guo = ORACLE_MUTANT == 4 ? (T)1 : (T)1 - (T)0.5 * a.s_v;
     for (int s = 0; s < Q; ++s) {
-      T qpi = -tr / (T)3;
+      T qpi = ORACLE_MUTANT == 1 ? tr / (T)3 : -tr / (T)3;
       for (int p = 0; p < 3; ++p)
         for (int q = 0; q < 3; ++q) qpi += tb.e[s][p] * tb.e[s][q] * Pi[p][q];
-      T out = fe[s] + ((T)1 - a.s_v) * tb.w[s] * (T)4.5 * qpi;
+      T out = fe[s] + omega * tb.w[s] * h2 * qpi;
       const T eu = tb.e[s][0] * u[0] + tb.e[s][1] * u[1] + tb.e[s][2] * u[2];
       T gterm = 0;
       for (int k = 0; k < 3; ++k) gterm += ((T)3 * (tb.e[s][k] - u[k]) + (T)9 * eu * tb.e[s][k]) * G[k];
-      out += ((T)1 - (T)0.5 * a.s_v) * tb.w[s] * gterm;
+      out += guo * tb.w[s] * gterm;
       f[c * Q + s] = out;
     }
   }
@@ -571,8 +587,10 @@ void weights(T X, int& base, T& fx, T w[3]) {
   base = (int)(X - (T)0.5);  // ti.cast(f32 -> i32) truncates
   fx = X - (T)base;
   w[0] = (T)0.5 * ((T)1.5 - fx) * ((T)1.5 - fx);
-  w[1] = (T)0.75 - (fx - (T)1.0) * (fx - (T)1.0);
+  const T c1 = ORACLE_MUTANT == 5 ? fx - (T)0.5 : fx - (T)1.0;
+  w[1] = (T)0.75 - c1 * c1;
   w[2] = (T)0.5 * (fx - (T)0.5) * (fx - (T)0.5);
+  if (ORACLE_MUTANT == 6) std::swap(w[0], w[2]);
 }
 
 template <typename T>
@@ -1065,6 +1083,9 @@ void reaction(const LbmArgs<T>& la, const MpmArgs<T>& ma, const CplArgs<T>& ca, 
     polar_rotation(Fd, Rd);                                                                                       \
     for (int e = 0; e < 9; ++e) R[e] = (T)Rd[e];                                                                  \
   }                                                                                                                 \
+  ORC_EXPORT void orc_bspline_##SFX(int n, const T* X, int* base, T* fx, T* w) {                                   \
+    for (int i = 0; i < n; ++i) weights<T>(X[i], base[i], fx[i], w + 3 * i);                                        \
+  }                                                                                                                 \
   ORC_EXPORT void orc_inv_M_##SFX(T* out) {                                                                        \
     const Tables<T>& tb = tables<T>();                                                                            \
     for (int r = 0; r < Q; ++r)                                                                                   \
@@ -1076,3 +1097,4 @@ ORC_DEFINE(double, f64)
 
 ORC_EXPORT int orc_mbstats_size_f32() { return (int)sizeof(MbStats<float>); }
 ORC_EXPORT int orc_mbstats_size_f64() { return (int)sizeof(MbStats<double>); }
+ORC_EXPORT int orc_mutant() { return ORACLE_MUTANT; }

@@ -163,11 +163,3 @@ def test_polar_rotation_against_numpy_svd(seed):
         S = R.T @ F
         assert np.allclose(S, S.T, atol=1e-12) and np.all(np.linalg.eigvalsh((S + S.T) / 2) > 0)
 
-
-def test_bspline_weights_partition_of_unity_and_first_moment():
-    for X in np.linspace(0.51, 30.49, 57):
-        base = int(X - 0.5)
-        fx = X - base
-        w = np.array([0.5 * (1.5 - fx) ** 2, 0.75 - (fx - 1) ** 2, 0.5 * (fx - 0.5) ** 2])
-        assert abs(w.sum() - 1) < 1e-14
-        assert abs((w * (np.arange(3) - fx)).sum()) < 1e-14
