@@ -396,9 +396,9 @@ int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpm
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
 int fsi_mpm_g2p(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream);
 
-/* Workspace of fsi_mpm_substep_coupled: 6 doubles per particle workgroup. */
+/* Workspace of fsi_mpm_substep_coupled: 9 doubles per particle workgroup. */
 #define FSI_COUPLED_WORKSPACE_DOUBLES(n_particles) \
-  (6 * (((n_particles) + FSI_PARTICLE_BLOCK - 1) / FSI_PARTICLE_BLOCK))
+  (9 * (((n_particles) + FSI_PARTICLE_BLOCK - 1) / FSI_PARTICLE_BLOCK))
 
 /* One MPM substep with its coupling reaction, as the driver sequences issue it
  * (fsi_driver.py:497-517 and :552-575): clear_grid, then p2g together with

@@ -1896,8 +1896,27 @@ struct ReactOut {
   int active;
   float particle_norm;
   float max_contrib;
-  float f[3];
+  float f[3];  // the particle's reaction force
+  float g[3];  // what its scatter adds to the grid nodes inside the grid: f times the in-grid weight
 };
+
+// Sum of a stencil's weights over the nodes inside an nx x ny x nz grid: the
+// product of the per-axis sums, as a node is inside iff it is along every axis.
+FSI_HD float inside_weight(const Stencil& sn, int nx, int ny, int nz) {
+  const int n[3] = {nx, ny, nz};
+  float prod = 1.0f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float s = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      const int I = sn.base[d] + o;
+      if (0 <= I && I < n[d]) s += sn.w[d][o];
+    }
+    prod *= s;
+  }
+  return prod;
+}
 
 // Lattice values the reaction force reads at the 27 stencil cells: 9 rows
 // (a-major, then b) of 3 cells along k, each row one 3-float load per array
@@ -1949,6 +1968,7 @@ FSI_HD ReactOut reaction_from_gather(const FsiCouplingParams& cp, int mode, floa
   o.particle_norm = 0.0f;
   o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  o.g[0] = o.g[1] = o.g[2] = 0.0f;
   const float(&w)[3][3] = sn.w;
   float hx = 0.0f, hy = 0.0f, hz = 0.0f, sphi = 0.0f;
 #pragma unroll
@@ -2028,6 +2048,10 @@ FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
         const float cn = norm3(cx, cy, cz);
         if (cn > o.max_contrib) o.max_contrib = cn;
       }
+  const float win = inside_weight(sn, mg.nx, mg.ny, mg.nz);
+  o.g[0] = pf[0] * win;
+  o.g[1] = pf[1] * win;
+  o.g[2] = pf[2] * win;
   return o;
 }
 

@@ -246,7 +246,10 @@ int fsi_mpm_reaction(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpm
     cc->active_reaction_particle_count += o.active;
     atomic_max_nonneg(&cc->max_particle_reaction_bits, o.particle_norm);
     atomic_max_nonneg(&cc->max_grid_reaction_bits, o.max_contrib);
-    for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] += o.f[d];
+    for (int d = 0; d < 3; ++d) {
+      cc->net_particle_reaction[d] += o.f[d];
+      cc->net_grid_reaction[d] += o.g[d];
+    }
   }
   return FSI_OK;
 }
@@ -448,7 +451,10 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
           I.cc->active_reaction_particle_count += o.active;
           atomic_max_nonneg(&I.cc->max_particle_reaction_bits, o.particle_norm);
           atomic_max_nonneg(&I.cc->max_grid_reaction_bits, o.max_contrib);
-          for (int d = 0; d < 3; ++d) I.cc->net_particle_reaction[d] += o.f[d];
+          for (int d = 0; d < 3; ++d) {
+            I.cc->net_particle_reaction[d] += o.f[d];
+            I.cc->net_grid_reaction[d] += o.g[d];
+          }
         }
         break;
       case FSI_BATCH_OP_MPM_P2G_REACT:
@@ -463,7 +469,10 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
           I.cc->active_reaction_particle_count += o.active;
           atomic_max_nonneg(&I.cc->max_particle_reaction_bits, o.particle_norm);
           atomic_max_nonneg(&I.cc->max_grid_reaction_bits, o.max_contrib);
-          for (int d = 0; d < 3; ++d) I.cc->net_particle_reaction[d] += o.f[d];
+          for (int d = 0; d < 3; ++d) {
+            I.cc->net_particle_reaction[d] += o.f[d];
+            I.cc->net_grid_reaction[d] += o.g[d];
+          }
         }
         break;
       case FSI_BATCH_OP_MPM_GRID_UPDATE:

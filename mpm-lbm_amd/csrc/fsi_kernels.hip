@@ -944,6 +944,7 @@ __device__ __forceinline__ ReactOut reaction_block(const FsiLbmParams& lp, const
   o.particle_norm = 0.0f;
   o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  o.g[0] = o.g[1] = o.g[2] = 0.0f;
   const GlobalSink g = reaction_sink(ms, (long long)mp.gnx * mp.gny * mp.gnz);
   const Stencil sn = stencil_if(valid, ms.x, mp.n_particles, q, cp.inv_dx_norm);
   const TileBox b = block_tile_box(sn.base, valid, TILE_NODES);
@@ -984,14 +985,17 @@ __global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSt
   const int act = block_sum_i(o.active, si);
   const float mpn = block_max_f(o.particle_norm, sf);
   const float mgc = block_max_f(o.max_contrib, sf);
-  double net[3];
+  double net[3], netg[3];
   for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)o.f[d], RED_SUM, sd);
+  for (int d = 0; d < 3; ++d) netg[d] = block_reduce((double)o.g[d], RED_SUM, sd);
   if (threadIdx.x == 0) {
     if (act) atomicAdd(&cc->active_reaction_particle_count, act);
     atomicMax(&cc->max_particle_reaction_bits, __float_as_uint(mpn));
     atomicMax(&cc->max_grid_reaction_bits, __float_as_uint(mgc));
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < 3; ++d) {
       if (net[d] != 0.0) atomicAdd(&cc->net_particle_reaction[d], net[d]);
+      if (netg[d] != 0.0) atomicAdd(&cc->net_grid_reaction[d], netg[d]);
+    }
   }
 }
 
@@ -1011,7 +1015,8 @@ __global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSt
 // x, y, z and mass), kept together in the tile (array of float4).
 // ---------------------------------------------------------------------------
 constexpr int FUSED_TILE_NODES = TILE_NODES;
-constexpr int REACT_REC = 6;  // active count, max particle norm, max grid contribution, net force x, y, z
+constexpr int REACT_REC = 9;  // active count, max particle norm, max grid contribution, net particle force x, y, z,
+                              // net in-grid force x, y, z
 
 // The 27-node scatter of p2g_react_particle for one scatter kind: node (a, e,
 // c) receives wt * (m + aff (x_node - x_p)) and wt * mass.  Returns the
@@ -1104,6 +1109,7 @@ __device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, c
   o.active = 0;
   o.particle_norm = o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  o.g[0] = o.g[1] = o.g[2] = 0.0f;
   HydroGather G;
   if (MODE && fold) hydro_gather_load(lp, ls, MODE, sn, checked, G);
   const P2gTerms t = p2g_terms_of(mp, in);
@@ -1111,8 +1117,12 @@ __device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, c
   float mv[3] = {t.mv[0], t.mv[1], t.mv[2]};
   if (MODE && fold) {
     o = reaction_from_gather(cp, MODE, t.J, in.vol0, sn, G);
+    const float win = inside_weight(sn, mp.gnx, mp.gny, mp.gnz);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) mv[d] += mp.dt * o.f[d];
+    for (int d = 0; d < 3; ++d) {
+      mv[d] += mp.dt * o.f[d];
+      o.g[d] = o.f[d] * win;
+    }
   }
   float wmax;
   if (kind == SCATTER_TILE_RMW)
@@ -1178,6 +1188,7 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
   o.active = 0;
   o.particle_norm = o.max_contrib = 0.0f;
   o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  o.g[0] = o.g[1] = o.g[2] = 0.0f;
   int kind = SCATTER_GLOBAL;
   bool checked = true;
   float4* mine = reinterpret_cast<float4*>(tile) + (threadIdx.x & (TILE_COPIES - 1)) * b.V;
@@ -1203,15 +1214,17 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
     const int act = block_sum_i(o.active, si);
     const float mpn = block_max_f(o.particle_norm, sf);
     const float mgc = block_max_f(o.max_contrib, sf);
-    double net[3];
+    double net[3], netg[3];
     for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)o.f[d], RED_SUM, sd);
+    for (int d = 0; d < 3; ++d) netg[d] = block_reduce((double)o.g[d], RED_SUM, sd);
     if (threadIdx.x == 0) {
       rec[0] = (double)act;
       rec[1] = (double)mpn;
       rec[2] = (double)mgc;
-      rec[3] = net[0];
-      rec[4] = net[1];
-      rec[5] = net[2];
+      for (int d = 0; d < 3; ++d) {
+        rec[3 + d] = net[d];
+        rec[6 + d] = netg[d];
+      }
     }
   }
 }
@@ -1232,30 +1245,28 @@ __global__ void __launch_bounds__(PTPB) k_mpm_p2g_react(FsiLbmParams lp, FsiLbmS
 // the counters, by one whole workgroup of TPB threads.
 __device__ __forceinline__ void fold_react_records(const double* rec, int n_rec, FsiCouplingCounters* cc,
                                                    double* sd) {
-  double a[REACT_REC] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double a[REACT_REC];
+#pragma unroll
+  for (int e = 0; e < REACT_REC; ++e) a[e] = 0.0;
   for (int k = threadIdx.x; k < n_rec; k += (int)blockDim.x) {
     const double* r = rec + (size_t)k * REACT_REC;
     a[0] += r[0];
     a[1] = r[1] > a[1] ? r[1] : a[1];
     a[2] = r[2] > a[2] ? r[2] : a[2];
-    a[3] += r[3];
-    a[4] += r[4];
-    a[5] += r[5];
+#pragma unroll
+    for (int e = 3; e < REACT_REC; ++e) a[e] += r[e];
   }
-  const double act = block_reduce(a[0], RED_SUM, sd);
-  const double mpn = block_reduce(a[1], RED_MAX, sd);
-  const double mgc = block_reduce(a[2], RED_MAX, sd);
-  const double nx = block_reduce(a[3], RED_SUM, sd);
-  const double ny = block_reduce(a[4], RED_SUM, sd);
-  const double nz = block_reduce(a[5], RED_SUM, sd);
+  double t[REACT_REC];
+#pragma unroll
+  for (int e = 0; e < REACT_REC; ++e) t[e] = block_reduce(a[e], (e == 1 || e == 2) ? RED_MAX : RED_SUM, sd);
   if (threadIdx.x == 0) {
-    cc->active_reaction_particle_count = (int32_t)act;
-    cc->max_particle_reaction_bits = __float_as_uint((float)mpn);
-    cc->max_grid_reaction_bits = __float_as_uint((float)mgc);
-    cc->net_particle_reaction[0] = nx;
-    cc->net_particle_reaction[1] = ny;
-    cc->net_particle_reaction[2] = nz;
-    cc->net_grid_reaction[0] = cc->net_grid_reaction[1] = cc->net_grid_reaction[2] = 0.0;
+    cc->active_reaction_particle_count = (int32_t)t[0];
+    cc->max_particle_reaction_bits = __float_as_uint((float)t[1]);
+    cc->max_grid_reaction_bits = __float_as_uint((float)t[2]);
+    for (int d = 0; d < 3; ++d) {
+      cc->net_particle_reaction[d] = t[3 + d];
+      cc->net_grid_reaction[d] = t[6 + d];
+    }
   }
 }
 

@@ -62,7 +62,8 @@ class FieldView:
             out = out.permute(*range(1, len(self.space_shape) + 1), 0)
             if isinstance(self.comps, tuple):
                 out = out.reshape(self.space_shape + self.comps)
-        return out.contiguous().cpu().numpy().astype(self.np_dtype, copy=False)
+        # a fresh array, as Taichi's to_numpy gives: host tensors would otherwise share their memory
+        return np.array(out.contiguous().cpu().numpy(), dtype=self.np_dtype, copy=True)
 
     def from_numpy(self, arr):
         a = np.asarray(arr)

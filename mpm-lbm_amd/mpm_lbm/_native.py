@@ -228,7 +228,7 @@ LINK_STATS_WS_DOUBLES = 18 * 64 * LINK_STATS_VALUES  # FSI_LINK_STATS_WS_DOUBLES
 
 def coupled_workspace_doubles(n_particles: int) -> int:
     """FSI_COUPLED_WORKSPACE_DOUBLES(n_particles)."""
-    return 6 * ((int(n_particles) + PARTICLE_BLOCK - 1) // PARTICLE_BLOCK)
+    return 9 * ((int(n_particles) + PARTICLE_BLOCK - 1) // PARTICLE_BLOCK)
 
 P = ctypes.POINTER
 

@@ -53,5 +53,5 @@ class MovingBoundaryFSICoupler3D(CouplingBase):
             "max_particle_reaction_norm": bits_to_float(k.max_particle_reaction_bits),
             "max_grid_reaction_norm": bits_to_float(k.max_grid_reaction_bits),
             "net_particle_reaction_force": tuple(float(v) for v in k.net_particle_reaction),
-            "net_grid_reaction_force": tuple(float(v) for v in k.net_particle_reaction),
+            "net_grid_reaction_force": tuple(float(v) for v in k.net_grid_reaction),
         }

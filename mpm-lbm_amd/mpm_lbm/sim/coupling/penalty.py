@@ -75,5 +75,5 @@ class PenaltyFSICoupler3D(CouplingBase):
             "net_cell_force": tuple(float(v) for v in k.net_cell_force),
             "net_hydro_force": tuple(float(v) for v in k.net_hydro_force),
             "max_reaction_grid_force_norm": bits_to_float(k.max_grid_reaction_bits),
-            "net_reaction_grid_force": tuple(float(v) for v in k.net_particle_reaction),
+            "net_reaction_grid_force": tuple(float(v) for v in k.net_grid_reaction),
         }

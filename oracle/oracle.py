@@ -186,7 +186,7 @@ class OracleFSI:
         self.f = np.zeros(N * Q, T)
         self.rho = np.zeros(N, T)
         self.v = np.zeros(N * 3, T)
-        self.solid = np.ascontiguousarray(np.asarray(solid_flags, dtype=np.int8).reshape(-1))
+        self.solid = np.array(solid_flags, dtype=np.int8).reshape(-1)  # a copy
         self.static_solid = self.solid.copy()
         self.old_solid = self.solid.copy()
         self.reinit = np.zeros(N, np.int8)
@@ -216,7 +216,7 @@ class OracleFSI:
         ma.velocity_damping = 0.0
         ma.domain_max[:] = [nx * prm["dx_norm"], ny * prm["dx_norm"], nz * prm["dx_norm"]]
         self.ma = ma
-        self.x = np.ascontiguousarray(np.asarray(particles["x"], T).reshape(-1))
+        self.x = np.array(particles["x"], T).reshape(-1)  # a copy: the caller's cloud stays as given
         self.fixed_x = self.x.copy()
         self.pv = np.zeros(P * 3, T)
         v0 = np.asarray(config.initial_solid_velocity_norm, T)
@@ -226,8 +226,8 @@ class OracleFSI:
         self.C = np.zeros(P * 9, T)
         self.Fp = np.tile(np.eye(3, dtype=T).reshape(-1), P)
         self.Jp = np.ones(P, T)
-        self.mass = np.ascontiguousarray(np.asarray(particles["mass"], T))
-        self.vol0 = np.ascontiguousarray(np.asarray(particles["vol0"], T))
+        self.mass = np.array(particles["mass"], T)
+        self.vol0 = np.array(particles["vol0"], T)
         G = nx * ny * nz
         self.grid_v = np.zeros(G * 3, T)
         self.grid_m = np.zeros(G, T)

@@ -79,6 +79,13 @@ def oracle_row(o, mode):
         "active_reaction_particle_count": int(o.reaction_stats[0]) if mode == "moving_boundary" else 0,
         "max_grid_reaction_norm": float(o.reaction_stats[2]) if mode != "none" else 0.0,
         "total_mpm_substeps": o.total_mpm_substeps,
+        # the subcycle accumulator columns (fsi_driver.py:640-652)
+        "mb_subcycle_force_accumulation_mode": ("mean_force_accumulator" if o.mb_sample_count > 0 and
+                                                o.config.fsi_exchange_mode == "lbm_subcycled_per_fsi_step"
+                                                else "not_active"),
+        "mb_subcycle_force_sample_count": int(o.mb_sample_count),
+        "mb_subcycle_force_accum_norm_max": float(o.accum_norm_max),
+        "mb_subcycle_force_mean_norm_max": float(o.mean_norm_max),
     }
     return row
 
