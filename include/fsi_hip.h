@@ -413,6 +413,19 @@ int fsi_mpm_substep_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const
                             const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
                             int mode, double* workspace, void* stream);
 
+/* fsi_mpm_substep_coupled without the caller's workspace, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The reaction records live in a fixed scratch array of the library, one per
+ * GPU, sized for FSI_NOALLOC_MAX_PARTICLES particles (2.25 MiB); larger
+ * particle counts are rejected with FSI_E_INVALID.  Calls no hipMalloc or
+ * hipFree.  Every call on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (other streams, other host threads) race. */
+#define FSI_NOALLOC_MAX_PARTICLES (1 << 21)
+int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                                    const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                                    int mode, void* stream);
+
 /* MPMSolid3D.reduce_stats + FSIDiagnostics3D.solid_mean_velocity_norm (solid.py:342-357,
  * fsi_diagnostics.py:117-124). */
 int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmStats* workspace_dev,

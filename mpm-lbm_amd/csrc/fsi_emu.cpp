@@ -268,6 +268,16 @@ int fsi_mpm_substep_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const
   return rc;
 }
 
+int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                                    const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                                    int mode, void* stream) {
+  if (!mp) return fail(FSI_E_INVALID, "null MPM params");
+  if (mp->n_particles > FSI_NOALLOC_MAX_PARTICLES)
+    return fail(FSI_E_INVALID, "fsi_mpm_substep_coupled_noalloc: particles exceed its scratch limit");
+  static std::vector<double> scratch((size_t)FSI_COUPLED_WORKSPACE_DOUBLES(FSI_NOALLOC_MAX_PARTICLES));
+  return fsi_mpm_substep_coupled(lp, ls, mp, ms, cp, cc, mode, mode ? scratch.data() : nullptr, stream);
+}
+
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {
   const long long G = (long long)mp->gnx * mp->gny * mp->gnz;
   for (long long n = 0; n < G; ++n) mpm_grid_node(*mp, *ms, n);

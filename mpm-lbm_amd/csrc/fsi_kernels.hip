@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
+#include <mutex>
 #include <string>
 
 #include "fsi_core.h"
@@ -1566,6 +1567,38 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_g2p(const FsiBatchInstance* inst,
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// Scratch of fsi_mpm_substep_coupled_noalloc: room for the reaction records
+// (FSI_COUPLED_WORKSPACE_DOUBLES) of FSI_NOALLOC_MAX_PARTICLES particles,
+// 2.25 MiB, held by every GPU the library runs on until the process ends.  In
+// host code `scratch` only names the symbol; its address on each device comes
+// from hipGetSymbolAddress, kept per device below.
+static __device__ __align__(256) double scratch[FSI_COUPLED_WORKSPACE_DOUBLES(FSI_NOALLOC_MAX_PARTICLES)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device (with it current) and kept.
+int noalloc_scratch_ptr(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(FSI_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(FSI_E_INVALID, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FSI_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return FSI_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* fsi_last_error(void) { return g_last_error.c_str(); }
@@ -1833,6 +1866,23 @@ int fsi_mpm_substep_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const
   if (mp->n_particles > 0)
     hipLaunchKernelGGL(k_mpm_g2p, dim3(pb), dim3(PTPB), 0, s, *mp, *ms);
   return check_launch("fsi_mpm_substep_coupled");
+}
+
+int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                                    const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                                    int mode, void* stream) {
+  if (!mp) return fail(FSI_E_INVALID, "null MPM params");
+  if (mp->n_particles > FSI_NOALLOC_MAX_PARTICLES)
+    return fail(FSI_E_INVALID, "fsi_mpm_substep_coupled_noalloc: " + std::to_string(mp->n_particles) +
+                                   " particles exceed its scratch limit of " +
+                                   std::to_string(FSI_NOALLOC_MAX_PARTICLES) +
+                                   " (use fsi_mpm_substep_coupled with a caller workspace)");
+  double* ws = nullptr;
+  if (mode) {
+    const int rc = noalloc_scratch_ptr(&ws);
+    if (rc) return rc;
+  }
+  return fsi_mpm_substep_coupled(lp, ls, mp, ms, cp, cc, mode, ws, stream);
 }
 
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {

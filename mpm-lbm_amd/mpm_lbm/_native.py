@@ -217,11 +217,13 @@ EXPORTED_SYMBOLS = [
     "fsi_lbm_init", "fsi_lbm_rebuild", "fsi_lbm_substep", "fsi_lbm_open_bc", "fsi_lbm_link_stats", "fsi_lbm_moments",
     "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats",
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
-    "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_reduce_stats",
+    "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_substep_coupled_noalloc",
+    "fsi_mpm_reduce_stats",
     "fsi_batch_block_map", "fsi_batch_launch",
 ]
 
 PARTICLE_BLOCK = 64  # FSI_PARTICLE_BLOCK
+NOALLOC_MAX_PARTICLES = 1 << 21  # FSI_NOALLOC_MAX_PARTICLES
 LINK_STATS_VALUES = 6  # FSI_LINK_STATS_VALUES
 LINK_STATS_WS_DOUBLES = 18 * 64 * LINK_STATS_VALUES  # FSI_LINK_STATS_WS_DOUBLES
 
@@ -271,6 +273,8 @@ class Native:
         L.fsi_mpm_g2p.argtypes = [P(MpmParams), P(MpmState), vp]
         L.fsi_mpm_substep_coupled.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),
                                               P(CouplingParams), vp, ctypes.c_int, vp, vp]
+        L.fsi_mpm_substep_coupled_noalloc.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),
+                                                      P(CouplingParams), vp, ctypes.c_int, vp]
         L.fsi_mpm_reduce_stats.argtypes = [P(MpmParams), P(MpmState), vp, vp, vp]
         L.fsi_batch_block_map.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
         L.fsi_batch_launch.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]
