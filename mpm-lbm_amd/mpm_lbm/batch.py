@@ -217,7 +217,11 @@ class BatchedFSI:
             self._timers.append((plan, timer))
 
     def _signature(self):
-        return tuple((I.lp.nx, I.lp.ny, I.lp.nz, int(I.mode), int(I.lp.storage)) for I in self._inst_host)
+        """What the substep timings depend on: per instance its lattice, mode,
+        storage, collision, particle count and MPM substeps, and the exchange length."""
+        return (self.n_substeps,) + tuple(
+            (I.lp.nx, I.lp.ny, I.lp.nz, int(I.mode), int(I.lp.storage), int(I.lp.collision), int(I.mp.n_particles),
+             int(I.mpm_substeps)) for I in self._inst_host)
 
     def _tune_begin(self):
         """Before an exchange: take a plan chosen earlier for the same batch, or

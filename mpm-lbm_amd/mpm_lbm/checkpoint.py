@@ -43,11 +43,22 @@ _PREFIX = "ckpt_"
 # the run that loads it.
 _GUARDED = ("coupling_mode", "fsi_exchange_mode", "n_grid", "lattice_shape", "n_particles",
             "lbm_substeps_per_fsi_step", "mpm_substeps_per_lbm_step", "lbm_population_storage",
-            "lbm_collision", "lbm_boundary_condition_mode", "lbm_open_boundary_semantics", "target_u_lbm")
+            "lbm_collision", "lbm_boundary_condition_mode", "lbm_open_boundary_semantics", "target_u_lbm",
+            "mpm_dt", "lbm_dt_phys_override_s", "gravity", "dynamic_solid_threshold", "beta_lbm",
+            "penalty_force_cap_lbm", "mb_reaction_scale", "mb_force_cap_norm", "reaction_transfer_mode",
+            "lbm_viscosity_semantics")
 
 _LBM_FIELDS = ("rho", "v", "F", "solid", "static_solid", "old_solid", "solid_phi", "solid_mass", "solid_vel",
                "cell_force", "hydro_force")
 _MPM_FIELDS = ("x", "v", "C", "F", "Jp", "mass", "vol0", "fixed_mask", "fixed_x")
+
+
+def _resolved(driver) -> dict:
+    """Parameters the driver resolved from its config and geometry files (the
+    material may come from a geometry file): they must match too."""
+    sp, lp = driver.solid._params, driver.lbm._params
+    return {"mpm_mu": float(sp.mu), "mpm_lambda": float(sp.la), "mpm_dt": float(sp.dt), "lbm_s_v": float(lp.s_v),
+            "lbm_s_other": float(lp.s_other)}
 
 
 def _guard(config) -> dict:
@@ -88,6 +99,7 @@ def save_fsi_checkpoint(driver, directory: str | Path, keep_last: int = 3) -> Pa
     meta = {
         "format": CHECKPOINT_FORMAT,
         "config": _guard(driver.config),
+        "resolved": _resolved(driver),
         "current_lbm_step": int(driver.current_lbm_step),
         "total_lbm_substeps": int(driver.total_lbm_substeps),
         "total_mpm_substeps": int(driver.total_mpm_substeps),
@@ -137,6 +149,11 @@ def load_fsi_checkpoint(driver, path: str | Path) -> dict:
         raise ValueError(f"FSI checkpoint config mismatch (checkpoint, run): {mismatch}")
     if not driver.initialized:
         driver.initialize()
+    resolved = _resolved(driver)
+    mismatch = {k: (meta.get("resolved", {}).get(k), v) for k, v in resolved.items()
+                if meta.get("resolved", {}).get(k) != v}
+    if mismatch:
+        raise ValueError(f"FSI checkpoint material/time-step mismatch (checkpoint, run): {mismatch}")
     lbm, solid = driver.lbm, driver.solid
     # populations first (they mark the moments stale), then the moments
     # (fresh); the flag writes make the cell words rebuild before the next step
@@ -145,8 +162,9 @@ def load_fsi_checkpoint(driver, path: str | Path) -> dict:
     lbm.v.from_numpy(arrays["lbm_v"])
     for k in ("solid", "static_solid", "old_solid", "solid_phi", "solid_mass", "solid_vel"):
         getattr(lbm, k).from_numpy(arrays[f"lbm_{k}"])
-    if meta["lbm_cell_force_active"]:
-        lbm.cell_force.from_numpy(arrays["lbm_cell_force"])
+    # always: a driver that already ran penalty steps must not keep its own force
+    lbm.cell_force.from_numpy(arrays["lbm_cell_force"])
+    lbm._cell_force_active = bool(meta["lbm_cell_force_active"])
     # the hydrodynamic force of the last step (recomputed by the next
     # moving-boundary step; read by diagnostics and reports until then)
     lbm.drop_pending_links()

@@ -12,7 +12,10 @@ penalty schedule).  The server loop, on one device stream:
      built - geometry, particles, device state - when it is admitted);
   3. regroups the live instances into a BatchedFSI when the set changed;
   4. advances every live instance by one batched exchange;
-  5. retires the requests that have run all their exchanges.
+  5. retires the requests that have run all their exchanges: their results
+     go to the host (the caller's `on_done` callback, or the final diagnostics
+     row in `Request.result`) and their device state is released, so the
+     admission budget bounds what is resident.
 
 A request's results are those of running it alone: batching shares launches,
 not state.  Latency is completion time minus arrival time.  No reference
@@ -59,6 +62,7 @@ class Request:
     driver: Any = field(default=None, repr=False)
     remaining: int = 0
     bytes: int = 0
+    result: Any = field(default=None, repr=False)
 
     @property
     def latency(self) -> Optional[float]:
@@ -90,8 +94,12 @@ class FSIServer:
         d.initialize()
         return d
 
-    def run(self, requests: List[Request], sim_exchange_seconds: Optional[float] = None, sync=None):
+    def run(self, requests: List[Request], sim_exchange_seconds: Optional[float] = None, sync=None, on_done=None):
         """Serve `requests` (arrival times in seconds from the start) to completion.
+
+        A retiring request's driver is handed to on_done(request, driver) (or,
+        without a callback, its final diagnostics row is kept in
+        request.result) and then dropped, which frees its device buffers.
 
         With sim_exchange_seconds the clock is simulated - it advances by that
         much per batched exchange and jumps to the next arrival when the
@@ -157,12 +165,21 @@ class FSIServer:
                 if r.remaining <= 0:
                     r.done = t
                     used -= r.bytes
+                    self._retire(r, on_done)
                 else:
                     still.append(r)
             if len(still) != len(active):
                 batch = None
             active = still
         return requests
+
+    @staticmethod
+    def _retire(r: Request, on_done):
+        if on_done is not None:
+            on_done(r, r.driver)
+        else:
+            r.result = r.driver.collect_diagnostics(r.driver.current_lbm_step)
+        r.driver = None
 
 
 def poisson_requests(rate=20.0, duration=10.0, seed=5, sizes=(32, 48, 64, 96, 128), lbm_steps=(50, 500),

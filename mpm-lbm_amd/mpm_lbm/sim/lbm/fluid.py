@@ -75,7 +75,7 @@ class _PopulationView(FieldView):
 
     def to_numpy(self):
         t = self._decoded().reshape((19,) + self.space_shape)
-        return t.permute(1, 2, 3, 0).contiguous().cpu().numpy()
+        return np.array(t.permute(1, 2, 3, 0).contiguous().cpu().numpy(), copy=True)
 
     def from_numpy(self, arr):
         a = np.asarray(arr, dtype=np.float32)
@@ -83,6 +83,9 @@ class _PopulationView(FieldView):
             raise ValueError(f"expected array of shape {self.shape}, got {a.shape}")
         lbm = self._lbm
         lbm._sync_hydro()
+        # rho and v are fields of their own in the reference: writing F leaves
+        # them as they were, so settle any lazy moments of the old F first
+        lbm._sync_moments()
         src = torch.from_numpy(np.ascontiguousarray(a)).to(lbm.device).reshape(-1, 19).t()
         if lbm.population_storage == "f32":
             lbm._F[lbm.cur].copy_(src)
@@ -418,6 +421,8 @@ class LBMFluid3D:
         place, with the stored rho, v as the neighbours' moments; clears the
         step counters first (fluid.py:539).  Substeps apply it themselves."""
         self._ensure_ready()
+        # the neighbours' moments are those streaming3 gave the last step (fluid.py:1027-1030)
+        self._sync_moments()
         self._call("fsi_lbm_open_bc", ctypes.byref(self._params), ctypes.byref(self._state), self.cur,
                    _native.SUB_STORED_MOMENTS, self._stream())
 
@@ -515,7 +520,6 @@ class LBMFluid3D:
     def _populations_written(self):
         other = self._F[1 - self.cur]
         other.copy_(self._F[self.cur])
-        self._moments_fresh = False
 
     def _moments_written(self):
         self._moments_fresh = True

@@ -88,8 +88,13 @@ def errors(a, b, inv_dx):
 def field_errors(d, o):
     """Driver against the oracle `o`; with `o.fp64` set, also against the FP64
     oracle ("vs_fp64") and the FP32 oracle against the FP64 one ("method")."""
+    return fields_errors(driver_fields(d), o)
+
+
+def fields_errors(a, o):
+    """field_errors for driver fields `a` already copied out (driver_fields)."""
     inv_dx = float(o.ma.inv_dx)
-    a, b = driver_fields(d), o.fields()
+    b = o.fields()
     errs = errors(a, b, inv_dx)
     if getattr(o, "fp64", None) is not None:
         c = o.fp64.fields()

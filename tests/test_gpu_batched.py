@@ -22,6 +22,8 @@ def _mixed(tmp_path):
         workloads.flap_case("moving_boundary", 16, substeps=8, thickness=4, seed=3, mpm_substeps=3, E_mod=3e4,
                             out_dir=str(tmp_path)),
         workloads.flap_case("penalty", 20, substeps=8, thickness=4, seed=4, E_mod=2e5, out_dir=str(tmp_path)),
+        # coupling "none": the split plan's NONE launch
+        workloads.flap_case("none", 12, substeps=8, thickness=2, seed=5, out_dir=str(tmp_path)),
     ]
 
 

@@ -18,7 +18,7 @@ import pytest
 from mpm_lbm import workloads
 from mpm_lbm.sim.drivers.fsi_config import FSIDriverConfig
 from oracle.oracle import OracleFSI
-from tests.parity import check_history, field_errors, make_pair, run_and_compare
+from tests.parity import check_history, driver_fields, field_errors, fields_errors, make_pair, run_and_compare
 from tests.test_precision_forms import BOUNDS, _workload
 
 pytestmark = pytest.mark.gpu
@@ -87,18 +87,23 @@ def test_gpu_served_requests_match_oracle(hip, tmp_path):
         Request(3, mk("penalty", 12, substeps=8, thickness=2, seed=14, E_mod=2e5, out_dir=str(tmp_path)), 2, 3.5),
     ]
     server = FSIServer(native=hip, out_dir=str(tmp_path / "srv"), n_substeps=8)
-    server.run(reqs, sim_exchange_seconds=1.0)
-    assert all(r.done is not None for r in reqs) and server.regroups >= 3
+    kept = {}
+
+    def keep(r, d):  # copy the results out before the server releases the request's device state
+        kept[r.rid] = (driver_fields(d), d.lbm.static_solid.to_numpy())
+
+    server.run(reqs, sim_exchange_seconds=1.0, on_done=keep)
+    assert all(r.done is not None and r.driver is None for r in reqs) and server.regroups >= 3
     for r in reqs:
-        w, d = r.workload, r.driver
-        flags = d.lbm.static_solid.to_numpy()
+        w = r.workload
+        fields, flags = kept[r.rid]
         o = OracleFSI(w.config, flags, w.cloud, w.material, dtype=np.float32)
         o.fp64 = OracleFSI(w.config, flags, w.cloud, w.material, dtype=np.float64)
         steps = r.exchanges if w.config.coupling_mode == "moving_boundary" else r.exchanges * 8
         for _ in range(steps):
             o.step_once()
             o.fp64.step_once()
-        errs = field_errors(d, o)
+        errs = fields_errors(fields, o)
         print("served", r.rid, {k: v for k, v in errs.items() if k not in ("vs_fp64", "method")}, flush=True)
         check_history([errs], f"served request {r.rid}", TOL)
     s = latency_summary(reqs)

@@ -262,6 +262,31 @@ def test_lbm_boundary_condition_matches_oracle_on_random_state(emu, sem, kw):
                                                 "reconstructed_population")] == list(k)
 
 
+@pytest.mark.parametrize("sem,kw", MODES, ids=[m[0] for m in MODES])
+def test_step_then_boundary_condition_uses_the_steps_moments(emu, sem, kw):
+    """step() and then a standalone Boundary_condition(): the pass reads the
+    rho, v that the step's streaming3 left (fluid.py:1027-1030, 2099-2103),
+    i.e. the moments of the populations after the step, not older ones."""
+    shape = (6, 5, 4)
+    rng = np.random.default_rng(5)
+    F0 = (W * (1.0 + 0.05 * rng.standard_normal(shape + (Q,)))).astype(np.float32)
+    lbm = LBMFluid3D(LBMConfig(nx=6, ny=5, nz=4, open_boundary_semantics=sem, bc_x_left=2, bc_x_right=1,
+                               rho_bc_x_right=1.0, vel_bc_x_left=(0.04, 0.0, 0.0), **kw), native=emu, device="cpu")
+    lbm.init_simulation()
+    lbm.F.from_numpy(F0)
+    lbm.rho.from_numpy(F0.sum(-1))
+    lbm.v.from_numpy((F0 @ orc.E.astype(np.float32)) / F0.sum(-1)[..., None])
+    lbm.step()
+    F1 = lbm.F.to_numpy()
+    lbm.Boundary_condition()
+    rho1 = F1.astype(np.float64).sum(-1)
+    v1 = (F1.astype(np.float64) @ orc.E.astype(np.float64)) / rho1[..., None]
+    want, _ = _bc_open(shape, sem, F1, rho1, v1, bc=(2, 1), rho_bc=(1.0, 1.0),
+                       vel_bc=((0.04, 0.0, 0.0), (0.0, 0.0, 0.0)), **kw)
+    np.testing.assert_allclose(lbm.F.to_numpy(), want, rtol=0, atol=2e-6)
+    assert np.abs(want - F1).max() > 1e-4  # the pass rewrote the faces
+
+
 def _with_ob(w, sem, kw):
     return dataclasses.replace(w, config=FSIDriverConfig(**{**w.config.to_dict(), "lbm_open_boundary_semantics": sem,
                                                             **kw}))

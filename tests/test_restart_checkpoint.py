@@ -249,3 +249,42 @@ def test_checkpoint_guard(emu, tmp_path):
         load_fsi_checkpoint(other, tmp_path / "ck" / "ckpt_00000099.npz")
     with pytest.raises(ValueError, match="resume=True needs checkpoint_dir"):
         _driver(w, tmp_path / "c", emu).run(resume=True)
+
+
+def test_checkpoint_guards_material_and_time_step(emu, tmp_path):
+    """The material comes from the geometry file, not a config entry: a run
+    whose resolved material differs refuses the checkpoint, and so does one
+    with another MPM time step or reaction scale."""
+    w = _penalty(tmp_path / "w1", steps=2)
+    d = _driver(w, tmp_path / "a", emu)
+    d.initialize()
+    d.step_once()
+    path = save_fsi_checkpoint(d, tmp_path / "ck")
+    stiffer = workloads.penalty_block(n=12, steps=2, block=(4, 4, 2), seed=3, E_mod=9.0e4, out_dir=str(tmp_path / "w2"))
+    with pytest.raises(ValueError, match="material/time-step mismatch"):
+        load_fsi_checkpoint(_driver(stiffer, tmp_path / "b", emu), path)
+    with pytest.raises(ValueError, match="config mismatch"):
+        load_fsi_checkpoint(_driver(w, tmp_path / "c", emu, mpm_dt=2.0e-4), path)
+    with pytest.raises(ValueError, match="config mismatch"):
+        load_fsi_checkpoint(_driver(w, tmp_path / "e", emu, beta_lbm=2.0e-3), path)
+
+
+def test_checkpoint_restores_an_inactive_cell_force(emu, tmp_path):
+    """A checkpoint written before any penalty force existed, loaded into a
+    driver that has run penalty steps, leaves no stale force behind."""
+    w = _penalty(tmp_path, steps=3)
+    fresh = _driver(w, tmp_path / "a", emu)
+    fresh.initialize()
+    path = save_fsi_checkpoint(fresh, tmp_path / "ck")
+    busy = _driver(w, tmp_path / "b", emu)
+    busy.initialize()
+    busy.solid.set_uniform_velocity(0.05, 0.0, 0.0)  # a moving block: a nonzero penalty force
+    for _ in range(2):
+        busy.step_once()
+    assert np.abs(busy.lbm.cell_force.to_numpy()).max() > 0
+    load_fsi_checkpoint(busy, path)
+    assert np.abs(busy.lbm.cell_force.to_numpy()).max() == 0
+    assert not busy.lbm._cell_force_active
+    busy.step_once()
+    fresh.step_once()
+    _same(_state(busy), _state(fresh))

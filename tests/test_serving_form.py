@@ -37,7 +37,14 @@ def _alone(r, emu, tmp_path):
 def test_served_requests_equal_alone_runs(emu, tmp_path):
     reqs = _requests(tmp_path)
     server = FSIServer(native=emu, device="cpu", out_dir=str(tmp_path / "srv"), n_substeps=4)
-    server.run(reqs, sim_exchange_seconds=1.0)
+    snap = {}
+
+    def keep(r, d):  # the caller's copy of the results; the server then drops the driver
+        snap[r.rid] = {("lbm", n): getattr(d.lbm, n).to_numpy() for n in ("F", "rho", "v", "hydro_force", "solid_phi")}
+        snap[r.rid].update({("solid", n): getattr(d.solid, n).to_numpy() for n in ("x", "v", "F", "C")})
+
+    server.run(reqs, sim_exchange_seconds=1.0, on_done=keep)
+    assert all(r.driver is None for r in reqs)  # retired requests hold no device state
     # schedule: every request done, in a mix that changed as requests came and went
     assert all(r.done is not None for r in reqs)
     assert server.regroups >= 3
@@ -46,13 +53,9 @@ def test_served_requests_equal_alone_runs(emu, tmp_path):
         assert r.latency >= r.exchanges * 1.0 - 1e-12
     for r in reqs:
         a = _alone(r, emu, tmp_path)
-        b = r.driver
-        for name in ("F", "rho", "v", "hydro_force", "solid_phi"):
-            fa, fb = getattr(a.lbm, name).to_numpy(), getattr(b.lbm, name).to_numpy()
-            assert np.array_equal(fa, fb), (r.rid, name)
-        for name in ("x", "v", "F", "C"):
-            fa, fb = getattr(a.solid, name).to_numpy(), getattr(b.solid, name).to_numpy()
-            assert np.array_equal(fa, fb), (r.rid, name)
+        for (part, name), fb in snap[r.rid].items():
+            fa = getattr(getattr(a, part), name).to_numpy()
+            assert np.array_equal(fa, fb), (r.rid, part, name)
     s = latency_summary(reqs)
     assert s["completed"] == 4 and s["p50_s"] <= s["p99_s"] <= s["max_s"]
 
@@ -65,6 +68,8 @@ def test_admission_respects_budget_fifo(emu, tmp_path):
     budget = max(sizes) + min(sizes)  # never room for more than two at once
     server = FSIServer(budget_bytes=budget, native=emu, device="cpu", out_dir=str(tmp_path / "srv2"), n_substeps=4)
     server.run(reqs, sim_exchange_seconds=1.0)
+    # without a callback each request keeps its final diagnostics row
+    assert all(r.driver is None and r.result["step"] > 0 for r in reqs)
     order = sorted(reqs, key=lambda r: (r.admitted, r.rid))
     assert [r.rid for r in order] == [0, 1, 2, 3]  # first in, first out
     events = sorted([(r.admitted, r.bytes) for r in reqs] + [(r.done, -r.bytes) for r in reqs],
