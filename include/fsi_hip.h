@@ -293,7 +293,7 @@ const char* fsi_last_error(void);
 int fsi_version(void);
 /* Writes sizeof of FsiLbmParams, FsiLbmState, FsiLbmCounters, FsiLbmStats,
  * FsiMpmParams, FsiMpmState, FsiCouplingParams, FsiCouplingCounters,
- * FsiMpmStats, FsiBatchInstance (in that order, up to n entries) so bindings can verify their
+ * FsiMpmStats, FsiBatchInstance, FsiBins (in that order, up to n entries) so bindings can verify their
  * struct layouts.  Returns the number of entries written. */
 int fsi_abi_sizes(int32_t* out, int n);
 int fsi_sync(void* stream);
@@ -425,6 +425,57 @@ int fsi_mpm_substep_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const
 int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
                                     const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
                                     int mode, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Particle binning and the particle-ordered (deterministic) transfers.
+ *
+ * A particle's bin is its quadratic B-spline stencil base per axis,
+ * base = trunc(x * inv_dx - 0.5) (solid.py:222-224), offset by 2 over an
+ * (nx + 3) x (ny + 3) x (nz + 3) box that holds every base whose stencil
+ * reaches the nx x ny x nz node grid; bin FSI_BIN_COUNT(...) (the last) is the
+ * overflow bin of any other particle.  start[b] .. start[b + 1] index order[],
+ * which lists each bin's particles in increasing particle index: a counting
+ * sort, deterministic whatever the thread schedule.
+ * ------------------------------------------------------------------------- */
+#define FSI_BIN_COUNT(nx, ny, nz) ((long long)((nx) + 3) * ((ny) + 3) * ((nz) + 3))
+/* ints of FsiBins.work for `nbins` = FSI_BIN_COUNT(...) bins */
+#define FSI_BIN_WS_INTS(nbins) ((nbins) + 1 + ((nbins) + 1 + 1023) / 1024)
+typedef struct FsiBins {
+  int32_t* key;   /* n_particles: each particle's bin */
+  int32_t* start; /* nbins + 2 */
+  int32_t* order; /* n_particles */
+  int32_t* work;  /* FSI_BIN_WS_INTS(nbins) */
+} FsiBins;
+
+/* Bins the particles at positions x (SoA, 3 x n_particles) on spacing
+ * 1 / inv_dx over an nx x ny x nz node grid (the binning key of p2g and of the
+ * projection's scatter, solid.py:222-224, projection.py:41-44). */
+int fsi_bin_particles(const float* x, int32_t n_particles, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
+                      const FsiBins* bins, void* stream);
+
+/* Floats per particle of fsi_mpm_substep_ordered's `terms` buffer. */
+#define FSI_ORDERED_TERMS_FLOATS 28
+
+/* fsi_mpm_substep_coupled with every node sum taken in particle order: the
+ * particles are binned (`bins`, sized for the MPM grid), a particle pass
+ * stores p2g's terms and the reaction force in `terms`, and each grid node
+ * then sums its contributors in increasing particle index - the order of the
+ * reference's kernels run sequentially - and applies grid_update; g2p
+ * follows.  Deterministic: two runs give identical bits.  The reaction is
+ * scattered into grid_f as in the reference (solid.py:213-315,
+ * penalty.py:90-129, moving_boundary.py:58-111); the lattice spacing must be
+ * the MPM grid's (cp->inv_dx_norm == mp->inv_dx).  workspace:
+ * FSI_COUPLED_WORKSPACE_DOUBLES(n_particles); terms: FSI_ORDERED_TERMS_FLOATS
+ * per particle. */
+int fsi_mpm_substep_ordered(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                            const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode,
+                            const FsiBins* bins, float* terms, double* workspace, void* stream);
+
+/* MPMToLBMProjector3D.project (projection.py:20-97) with each lattice cell's
+ * sums taken in particle order (`bins` sized for the lattice); deterministic. */
+int fsi_project_ordered(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                        const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                        const FsiBins* bins, void* stream);
 
 /* MPMSolid3D.reduce_stats + FSIDiagnostics3D.solid_mean_velocity_norm (solid.py:342-357,
  * fsi_diagnostics.py:117-124). */

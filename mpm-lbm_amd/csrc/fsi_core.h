@@ -29,9 +29,19 @@
 #include <hip/hip_fp16.h>
 #define FSI_HD __host__ __device__ __forceinline__
 #define FSI_HDM __host__ __device__ __forceinline__
+// Evaluate each float operation rounded on its own, in the written order, as
+// the oracle's plain C++ does: the library is built with
+// -ffp-contract=fast-honor-pragmas, so these turn a*b+c -> fma fusion off
+// (inside a function body, or from BEGIN to END at file scope) and back on.
+#define FSI_NO_CONTRACT _Pragma("clang fp contract(off)")
+#define FSI_NO_CONTRACT_BEGIN _Pragma("clang fp contract(off)")
+#define FSI_NO_CONTRACT_END _Pragma("clang fp contract(fast)")
 #else
 #define FSI_HD static inline
 #define FSI_HDM inline
+#define FSI_NO_CONTRACT
+#define FSI_NO_CONTRACT_BEGIN
+#define FSI_NO_CONTRACT_END
 #endif
 
 namespace fsi {
@@ -1273,8 +1283,14 @@ FSI_HD void lbm_mb_begin_cell(const FsiLbmParams& p, const FsiLbmState& st, long
 }
 
 // ---------------------------------------------------------------------------
-// MPM (mpm/solid.py) per-particle / per-node bodies
+// MPM (mpm/solid.py) per-particle / per-node bodies.
+//
+// From here to the diagnostics, no fma fusion: the elastic stress of strains
+// near 1e-6 turns any rounding difference in the particle path into a
+// visible velocity difference, so the particle and coupling arithmetic keeps
+// the oracle's rounding step by step.  The lattice substep above keeps fma.
 // ---------------------------------------------------------------------------
+FSI_NO_CONTRACT_BEGIN
 struct MpmGrid {
   int nx, ny, nz;
   long long G;
@@ -1656,6 +1672,7 @@ FSI_HD void mpm_grid_node(const FsiMpmParams& mp, const FsiMpmState& ms, long lo
 
 // g2p + clamp + dynamics controls + fixed pin (solid.py:262-315, :114-124).
 FSI_HD void mpm_g2p_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long long p) {
+  FSI_NO_CONTRACT
   const MpmGrid g = make_mgrid(mp);
   const long long P = mp.n_particles;
   int base[3];
@@ -2063,6 +2080,237 @@ FSI_HD ReactOut reaction_particle(const FsiLbmParams& lp, const FsiLbmState& ls,
   return reaction_particle(lp, ls, mp, ms, cp, mode, p, particle_stencil(ms.x, mp.n_particles, p, cp.inv_dx_norm),
                            sink);
 }
+
+// ---------------------------------------------------------------------------
+// Particle binning and particle-ordered transfers.
+//
+// A particle's bin is its quadratic B-spline stencil base, base_d =
+// trunc(x_d * inv_dx - 0.5) per axis (solid.py:222-224): the node where its
+// 3 x 3 x 3 stencil starts.  Bins cover bases -2 .. n per axis, i.e. every base
+// whose stencil reaches a node inside an n-node axis, offset by 2; one overflow
+// bin (index nb) takes any other particle.  bin_start (nb + 2 entries) and
+// bin_order (P entries) list each bin's particles in increasing index.  The
+// contributors of node (I, J, K) are the particles of the 27 bins
+// (I - a, J - b, K - c), a, b, c in {0, 1, 2}; walking those 27 lists merged
+// in increasing particle index visits them in the order of a sequential
+// scatter over the particles (the oracle's), whatever the thread schedule, so
+// the node sums are deterministic and in that order.
+// ---------------------------------------------------------------------------
+struct BinGrid {
+  int nx, ny, nz;  // node grid
+  int by, bz;      // bins along y and z (n + 3)
+  long long nb;    // bins without the overflow bin
+};
+
+FSI_HD BinGrid make_bins(int nx, int ny, int nz) {
+  BinGrid b;
+  b.nx = nx;
+  b.ny = ny;
+  b.nz = nz;
+  b.by = ny + 3;
+  b.bz = nz + 3;
+  b.nb = (long long)(nx + 3) * b.by * b.bz;
+  return b;
+}
+
+// Bin of a stencil base (the overflow bin nb when its stencil misses the grid).
+FSI_HD int32_t bin_of_base(const BinGrid& b, const int* base) {
+  if (base[0] < -2 || base[0] > b.nx || base[1] < -2 || base[1] > b.ny || base[2] < -2 || base[2] > b.nz)
+    return (int32_t)b.nb;
+  return (int32_t)(((long long)(base[0] + 2) * b.by + (base[1] + 2)) * b.bz + (base[2] + 2));
+}
+
+// Bin of particle p: its stencil base on spacing 1/inv, as bspline() takes it.
+FSI_HD int32_t particle_bin(const BinGrid& b, const float* x, long long P, long long p, float inv) {
+  int base[3];
+  float fx, w[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline(x[d * P + p] * inv, base[d], fx, w);
+  return bin_of_base(b, base);
+}
+
+// Calls fn(p, a, b, c) for every particle p whose stencil covers node (I, J, K)
+// at offset (a, b, c), in increasing p.  start/order: the binning above.
+template <typename Fn>
+FSI_HD void for_each_contributor(const BinGrid& bg, const int32_t* start, const int32_t* order, int I, int J, int K,
+                                 Fn&& fn) {
+  constexpr int32_t kNone = 0x7fffffff;
+  int32_t cur[27], end[27], head[27];
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    const int a = o / 9, b = (o / 3) % 3, c = o % 3;
+    const long long bin = ((long long)(I - a + 2) * bg.by + (J - b + 2)) * bg.bz + (K - c + 2);
+    cur[o] = start[bin];
+    end[o] = start[bin + 1];
+    head[o] = cur[o] < end[o] ? order[cur[o]] : kNone;
+  }
+  for (;;) {
+    int32_t best = kNone;
+    int bo = 0;
+#pragma unroll
+    for (int o = 0; o < 27; ++o)
+      if (head[o] < best) {
+        best = head[o];
+        bo = o;
+      }
+    if (best == kNone) break;
+    fn(best, bo / 9, (bo / 3) % 3, bo % 3);
+#pragma unroll
+    for (int o = 0; o < 27; ++o)
+      if (o == bo) {
+        ++cur[o];
+        head[o] = cur[o] < end[o] ? order[cur[o]] : kNone;
+      }
+  }
+}
+
+// Per-particle record of the ordered MPM substep (floats): the affine momentum
+// terms of p2g, the reaction force, and the stencil (fx, weights).
+constexpr int OT_AFF = 0, OT_MV = 9, OT_MASS = 12, OT_PF = 13, OT_FX = 16, OT_W = 19;
+constexpr int ORDERED_TERMS = 28;
+static_assert(ORDERED_TERMS == FSI_ORDERED_TERMS_FLOATS, "ordered-substep record size");
+
+// The particle pass of the ordered substep: p2g's terms (writes Jp = det F,
+// solid.py:226-240) and, for mode 1 / 2, the reaction force of
+// add_lbm_reaction_to_mpm_grid / add_moving_boundary_reaction_to_mpm_grid,
+// sampled on the same stencil (the lattice spacing is the MPM grid's).
+// o.max_contrib is the largest in-grid node contribution, o.g the in-grid sum.
+FSI_HD ReactOut mpm_ordered_terms_particle(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                                           const FsiMpmState& ms, const FsiCouplingParams& cp, int mode, long long p,
+                                           float* terms) {
+  const Stencil sn = particle_stencil(ms.x, mp.n_particles, p, mp.inv_dx);
+  const P2gTerms t = p2g_terms(mp, ms, p);
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  o.g[0] = o.g[1] = o.g[2] = 0.0f;
+  if (mode) {
+    o = reaction_force(lp, ls, ms, cp, mode, p, sn);
+    const int n[3] = {mp.gnx, mp.gny, mp.gnz};
+    float wmax = 1.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float m = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int I = sn.base[d] + q;
+        if (0 <= I && I < n[d] && sn.w[d][q] > m) m = sn.w[d][q];
+      }
+      wmax *= m;
+    }
+    const float win = inside_weight(sn, mp.gnx, mp.gny, mp.gnz);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o.g[d] = o.f[d] * win;
+    if (wmax > 0.0f) o.max_contrib = norm3(wmax * o.f[0], wmax * o.f[1], wmax * o.f[2]);
+  }
+  float* T = terms + (long long)ORDERED_TERMS * p;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) T[OT_AFF + e] = t.aff[e];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    T[OT_MV + d] = t.mv[d];
+    T[OT_PF + d] = o.f[d];
+    T[OT_FX + d] = sn.fx[d];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) T[OT_W + 3 * d + q] = sn.w[d][q];
+  }
+  T[OT_MASS] = t.mass;
+  return o;
+}
+
+// Node n of the ordered substep: clear_grid, p2g and the reaction scatter
+// (solid.py:213-249, penalty.py:111-129, moving_boundary.py:100-111) summed in
+// particle order with the oracle's operation order, then grid_update
+// (solid.py:252-259).  Every node is written, so no clear pass is needed.
+FSI_HD void mpm_ordered_node(const FsiMpmParams& mp, const FsiMpmState& ms, const BinGrid& bg, const int32_t* start,
+                             const int32_t* order, const float* terms, bool reaction, long long n) {
+  FSI_NO_CONTRACT
+  const MpmGrid g = make_mgrid(mp);
+  const int K = (int)(n % g.nz);
+  const int Jn = (int)((n / g.nz) % g.ny);
+  const int I = (int)(n / ((long long)g.nz * g.ny));
+  float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, m = 0.0f, f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+  for_each_contributor(bg, start, order, I, Jn, K, [&](int32_t p, int a, int b, int c) {
+    const float* T = terms + (long long)ORDERED_TERMS * p;
+    const float wt = T[OT_W + a] * T[OT_W + 3 + b] * T[OT_W + 6 + c];
+    const float dp0 = ((float)a - T[OT_FX]) * mp.dx;
+    const float dp1 = ((float)b - T[OT_FX + 1]) * mp.dx;
+    const float dp2 = ((float)c - T[OT_FX + 2]) * mp.dx;
+    const float* A = T + OT_AFF;
+    v0 += wt * (T[OT_MV] + (A[0] * dp0 + A[1] * dp1 + A[2] * dp2));
+    v1 += wt * (T[OT_MV + 1] + (A[3] * dp0 + A[4] * dp1 + A[5] * dp2));
+    v2 += wt * (T[OT_MV + 2] + (A[6] * dp0 + A[7] * dp1 + A[8] * dp2));
+    m += wt * T[OT_MASS];
+    if (reaction) {
+      f0 += wt * T[OT_PF];
+      f1 += wt * T[OT_PF + 1];
+      f2 += wt * T[OT_PF + 2];
+    }
+  });
+  float v[3] = {v0, v1, v2};
+  const float f[3] = {f0, f1, f2};
+  if (m > 0.0f) {
+    const int idx[3] = {I, Jn, K};
+    const int ext[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float vd = v[d] / m;
+      vd += mp.dt * mp.gravity[d];
+      vd += mp.dt * f[d] / m;
+      v[d] = vd;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (idx[d] < mp.bound && v[d] < 0.0f) v[d] = 0.0f;
+      if (idx[d] > ext[d] - mp.bound && v[d] > 0.0f) v[d] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    ms.grid_v[d * g.G + n] = v[d];
+    ms.grid_f[d * g.G + n] = f[d];
+  }
+  ms.grid_m[n] = m;
+}
+
+// Lattice cell c of the ordered projection (projection.py:38-64): phi, mass
+// and mass-weighted velocity summed in particle order with the oracle's
+// operation order; normalize_cell follows.  Bins on the lattice spacing.
+FSI_HD void project_ordered_cell(const FsiLbmParams& lp, const FsiLbmState& ls, const FsiMpmParams& mp,
+                                 const FsiMpmState& ms, const FsiCouplingParams& cp, const BinGrid& bg,
+                                 const int32_t* start, const int32_t* order, long long c) {
+  FSI_NO_CONTRACT
+  const Grid g = make_grid(lp);
+  int I, Jn, K;
+  coords_of(g, c, I, Jn, K);
+  const long long P = mp.n_particles;
+  float phi = 0.0f, sm = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  for_each_contributor(bg, start, order, I, Jn, K, [&](int32_t p, int a, int b, int cc) {
+    int base[3];
+    float fx[3], w[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * cp.inv_dx_norm, base[d], fx[d], w[d]);
+    float J = ms.Jp[p];
+    J = J > 0.0f ? J : 0.0f;
+    const float vol = ms.vol0[p] * J;
+    const float mass = ms.mass[p];
+    const float wt = w[0][a] * w[1][b] * w[2][cc];
+    const float mc = wt * mass;
+    phi += wt * vol / cp.cell_volume_norm;
+    sm += mc;
+    s0 += mc * (ms.v[p] * cp.vel_scale_norm_to_lbm);
+    s1 += mc * (ms.v[P + p] * cp.vel_scale_norm_to_lbm);
+    s2 += mc * (ms.v[2 * P + p] * cp.vel_scale_norm_to_lbm);
+  });
+  ls.solid_phi[c] = phi;
+  ls.solid_mass[c] = sm;
+  ls.solid_vel[c] = s0;
+  ls.solid_vel[g.N + c] = s1;
+  ls.solid_vel[2 * g.N + c] = s2;
+}
+
+FSI_NO_CONTRACT_END
 
 // ---------------------------------------------------------------------------
 // Diagnostics reductions.  The stats structs are arrays of doubles; each field

@@ -40,13 +40,14 @@ const char* fsi_last_error(void) { return g_err.c_str(); }
 int fsi_version(void) { return 1; }
 
 int fsi_abi_sizes(int32_t* out, int n) {
-  const int32_t sz[10] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
+  const int32_t sz[11] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
                          (int32_t)sizeof(FsiLbmCounters),    (int32_t)sizeof(FsiLbmStats),
                          (int32_t)sizeof(FsiMpmParams),      (int32_t)sizeof(FsiMpmState),
                          (int32_t)sizeof(FsiCouplingParams), (int32_t)sizeof(FsiCouplingCounters),
-                         (int32_t)sizeof(FsiMpmStats),       (int32_t)sizeof(FsiBatchInstance)};
+                         (int32_t)sizeof(FsiMpmStats),       (int32_t)sizeof(FsiBatchInstance),
+                         (int32_t)sizeof(FsiBins)};
   int k = 0;
-  for (; k < n && k < 10; ++k) out[k] = sz[k];
+  for (; k < n && k < 11; ++k) out[k] = sz[k];
   return k;
 }
 int fsi_sync(void*) { return FSI_OK; }
@@ -276,6 +277,101 @@ int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* l
     return fail(FSI_E_INVALID, "fsi_mpm_substep_coupled_noalloc: particles exceed its scratch limit");
   static std::vector<double> scratch((size_t)FSI_COUPLED_WORKSPACE_DOUBLES(FSI_NOALLOC_MAX_PARTICLES));
   return fsi_mpm_substep_coupled(lp, ls, mp, ms, cp, cc, mode, mode ? scratch.data() : nullptr, stream);
+}
+
+// Binning, serially: keys in particle order, counts, exclusive scan, and
+// slots taken in particle order (so each bin's list is already increasing).
+static void bin_serial(const float* x, int32_t P, float inv, const BinGrid& bg, const FsiBins& b) {
+  const long long n = bg.nb + 1;
+  int32_t* count = b.work;
+  for (long long i = 0; i < n; ++i) count[i] = 0;
+  for (int32_t p = 0; p < P; ++p) {
+    b.key[p] = particle_bin(bg, x, P, p, inv);
+    ++count[b.key[p]];
+  }
+  int32_t run = 0;
+  for (long long i = 0; i < n; ++i) {
+    b.start[i] = run;
+    run += count[i];
+  }
+  b.start[n] = run;
+  for (long long i = 0; i < n; ++i) count[i] = 0;
+  for (int32_t p = 0; p < P; ++p) b.order[b.start[b.key[p]] + count[b.key[p]]++] = p;
+  for (long long i = 0; i < n; ++i) count[i] = 0;
+}
+
+int fsi_bin_particles(const float* x, int32_t n_particles, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
+                      const FsiBins* bins, void*) {
+  if (nx < 1 || ny < 1 || nz < 1 || n_particles < 0 || !bins)
+    return fail(FSI_E_INVALID, "fsi_bin_particles: bad grid, particles or buffers");
+  bin_serial(x, n_particles, inv_dx, make_bins(nx, ny, nz), *bins);
+  return FSI_OK;
+}
+
+int fsi_mpm_substep_ordered(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                            const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode,
+                            const FsiBins* bins, float* terms, double* workspace, void*) {
+  if (mode < 0 || mode > 2) return fail(FSI_E_INVALID, "mode must be 0 (none), 1 (penalty) or 2 (moving boundary)");
+  if (!bins || (mp->n_particles > 0 && !terms)) return fail(FSI_E_INVALID, "null binning or terms buffers");
+  if (mode && (!lp || !ls || !cp || !cc || !workspace))
+    return fail(FSI_E_INVALID, "the reaction needs coupling params, counters, workspace, hydro_force, solid_phi");
+  if (mode && cp->inv_dx_norm != mp->inv_dx)
+    return fail(FSI_E_INVALID, "fsi_mpm_substep_ordered: the lattice spacing must be the MPM grid's");
+  FsiLbmParams lpv;
+  FsiLbmState lsv;
+  FsiCouplingParams cpv;
+  memset(&lpv, 0, sizeof(lpv));
+  memset(&lsv, 0, sizeof(lsv));
+  memset(&cpv, 0, sizeof(cpv));
+  if (mode) {
+    lpv = *lp;
+    lsv = *ls;
+    cpv = *cp;
+    cc->active_reaction_particle_count = 0;
+    cc->max_particle_reaction_bits = cc->max_grid_reaction_bits = 0;
+    for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] = cc->net_grid_reaction[d] = 0.0;
+  }
+  for (long long q = 0; q < mp->n_particles; ++q) {
+    const ReactOut o = mpm_ordered_terms_particle(lpv, lsv, *mp, *ms, cpv, mode, q, terms);
+    if (mode) {
+      cc->active_reaction_particle_count += o.active;
+      atomic_max_nonneg(&cc->max_particle_reaction_bits, o.particle_norm);
+      atomic_max_nonneg(&cc->max_grid_reaction_bits, o.max_contrib);
+      for (int d = 0; d < 3; ++d) {
+        cc->net_particle_reaction[d] += o.f[d];
+        cc->net_grid_reaction[d] += o.g[d];
+      }
+    }
+  }
+  const BinGrid bg = make_bins(mp->gnx, mp->gny, mp->gnz);
+  bin_serial(ms->x, mp->n_particles, mp->inv_dx, bg, *bins);
+  const long long G = (long long)mp->gnx * mp->gny * mp->gnz;
+  for (long long n = 0; n < G; ++n)
+    mpm_ordered_node(*mp, *ms, bg, bins->start, bins->order, terms, mode != 0 && mp->n_particles > 0, n);
+  for (long long q = 0; q < mp->n_particles; ++q) mpm_g2p_particle(*mp, *ms, q);
+  return FSI_OK;
+}
+
+int fsi_project_ordered(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                        const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                        const FsiBins* bins, void*) {
+  if (!bins) return fail(FSI_E_INVALID, "null binning buffers");
+  const BinGrid bg = make_bins(lp->nx, lp->ny, lp->nz);
+  bin_serial(ms->x, mp->n_particles, cp->inv_dx_norm, bg, *bins);
+  const long long N = ncells(lp);
+  for (long long c = 0; c < N; ++c) project_ordered_cell(*lp, *ls, *mp, *ms, *cp, bg, bins->start, bins->order, c);
+  cc->projected_mass = cc->projected_volume_raw = cc->projected_volume_clamped = 0.0;
+  cc->max_phi_raw_bits = 0;
+  cc->active_cell_count = 0;
+  for (long long c = 0; c < N; ++c) {
+    ProjOut o = normalize_cell(*lp, *ls, *cp, c);
+    atomic_max_nonneg(&cc->max_phi_raw_bits, o.phi_raw > 0.0f ? o.phi_raw : 0.0f);
+    cc->active_cell_count += o.active;
+    cc->projected_mass += o.mass;
+    cc->projected_volume_raw += (double)o.phi_raw * cp->cell_volume_norm;
+    cc->projected_volume_clamped += (double)o.phi_clamped * cp->cell_volume_norm;
+  }
+  return FSI_OK;
 }
 
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void*) {

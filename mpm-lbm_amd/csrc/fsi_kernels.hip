@@ -1280,6 +1280,180 @@ __global__ void __launch_bounds__(TPB) k_mpm_grid_update_react(FsiMpmParams mp, 
   if (rec && blockIdx.x == 0) fold_react_records(rec, n_rec, cc, sd);
 }
 
+// ---------------------------------------------------------------------------
+// Particle binning (counting sort by stencil base) and the particle-ordered
+// transfers.  Bin counts are integer atomics (order-free); each bin's list is
+// then sorted by particle index, so start/order do not depend on the schedule.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_fill_i32(int32_t* a, long long n, int32_t v) {
+  const long long i = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(TPB) k_bin_keys(const float* x, int32_t P, float inv, BinGrid bg, int32_t* count,
+                                                  int32_t* key) {
+  const long long p = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (p >= P) return;
+  const int32_t b = particle_bin(bg, x, P, p, inv);
+  key[p] = b;
+  atomicAdd(count + b, 1);
+}
+
+constexpr int SCAN_TILE = 1024;  // elements per workgroup of the scan (4 per thread)
+
+// Exclusive scan within 1024-element tiles of count[0, n) into start[], and
+// each tile's total into part[tile].
+__global__ void __launch_bounds__(TPB) k_scan_tiles(const int32_t* count, long long n, int32_t* start,
+                                                    int32_t* part) {
+  __shared__ int32_t sh[TPB];
+  const long long base = (long long)blockIdx.x * SCAN_TILE + 4 * threadIdx.x;
+  int32_t v[4];
+  int32_t tsum = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v[e] = base + e < n ? count[base + e] : 0;
+    tsum += v[e];
+  }
+  sh[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int off = 1; off < TPB; off <<= 1) {  // inclusive Hillis-Steele over the thread sums
+    const int32_t add = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  int32_t run = sh[threadIdx.x] - tsum;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (base + e < n) start[base + e] = run;
+    run += v[e];
+  }
+  if (threadIdx.x == TPB - 1) part[blockIdx.x] = sh[TPB - 1];
+}
+
+// Exclusive scan of the tile totals in place (one workgroup); start[n] = total.
+__global__ void __launch_bounds__(TPB) k_scan_parts(int32_t* part, int n_tiles, int32_t* start, long long n) {
+  __shared__ int32_t sh[TPB];
+  int32_t carry = 0;
+  for (int c0 = 0; c0 < n_tiles; c0 += TPB) {
+    const int i = c0 + (int)threadIdx.x;
+    const int32_t v = i < n_tiles ? part[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < TPB; off <<= 1) {
+      const int32_t add = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (i < n_tiles) part[i] = carry + sh[threadIdx.x] - v;
+    const int32_t total = sh[TPB - 1];
+    __syncthreads();
+    carry += total;
+  }
+  if (threadIdx.x == 0) start[n] = carry;
+}
+
+__global__ void __launch_bounds__(TPB) k_scan_add(int32_t* start, long long n, const int32_t* part) {
+  const long long i = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (i < n) start[i] += part[i / SCAN_TILE];
+}
+
+// Each particle takes a slot of its bin (count returns to zero).
+__global__ void __launch_bounds__(TPB) k_bin_fill(const int32_t* key, int32_t P, const int32_t* start, int32_t* count,
+                                                  int32_t* order) {
+  const long long p = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (p >= P) return;
+  const int32_t b = key[p];
+  order[start[b] + atomicSub(count + b, 1) - 1] = (int32_t)p;
+}
+
+// Each bin's list in increasing particle index (insertion sort: a bin holds a
+// few particles).
+__global__ void __launch_bounds__(TPB) k_bin_sort(const int32_t* start, long long n_bins, int32_t* order) {
+  const long long b = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (b >= n_bins) return;
+  const int32_t s = start[b], e = start[b + 1];
+  for (int32_t i = s + 1; i < e; ++i) {
+    const int32_t v = order[i];
+    int32_t j = i - 1;
+    while (j >= s && order[j] > v) {
+      order[j + 1] = order[j];
+      --j;
+    }
+    order[j + 1] = v;
+  }
+}
+
+// The particle pass of the ordered substep: terms, bin key and count, and the
+// workgroup's reaction record (MODE 1 penalty, 2 moving boundary).
+template <int MODE>
+__global__ void __launch_bounds__(PTPB) k_ordered_terms(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+                                                        FsiMpmState ms, FsiCouplingParams cp, float* terms, double* rec,
+                                                        BinGrid bg, int32_t* count, int32_t* key) {
+  __shared__ double sd[PTPB / 64];
+  __shared__ float sf[PTPB / 64];
+  __shared__ int si[PTPB / 64];
+  const long long p = (long long)blockIdx.x * PTPB + threadIdx.x;
+  ReactOut o;
+  o.active = 0;
+  o.particle_norm = o.max_contrib = 0.0f;
+  o.f[0] = o.f[1] = o.f[2] = 0.0f;
+  o.g[0] = o.g[1] = o.g[2] = 0.0f;
+  if (p < mp.n_particles) {
+    o = mpm_ordered_terms_particle(lp, ls, mp, ms, cp, MODE, p, terms);
+    const int32_t b = particle_bin(bg, ms.x, mp.n_particles, p, mp.inv_dx);
+    key[p] = b;
+    atomicAdd(count + b, 1);
+  }
+  if (MODE) {
+    const int act = block_sum_i(o.active, si);
+    const float mpn = block_max_f(o.particle_norm, sf);
+    const float mgc = block_max_f(o.max_contrib, sf);
+    double net[3], netg[3];
+    for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)o.f[d], RED_SUM, sd);
+    for (int d = 0; d < 3; ++d) netg[d] = block_reduce((double)o.g[d], RED_SUM, sd);
+    if (threadIdx.x == 0) {
+      double* r = rec + (size_t)blockIdx.x * REACT_REC;
+      r[0] = (double)act;
+      r[1] = (double)mpn;
+      r[2] = (double)mgc;
+      for (int d = 0; d < 3; ++d) {
+        r[3 + d] = net[d];
+        r[6 + d] = netg[d];
+      }
+    }
+  }
+}
+
+// Node pass of the ordered substep; block 0 also folds the reaction records.
+template <bool REACT>
+__global__ void __launch_bounds__(TPB) k_ordered_nodes(FsiMpmParams mp, FsiMpmState ms, BinGrid bg,
+                                                       const int32_t* start, const int32_t* order, const float* terms,
+                                                       const double* rec, int n_rec, FsiCouplingCounters* cc) {
+  __shared__ double sd[TPB / 64];
+  const long long n = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (n < (long long)mp.gnx * mp.gny * mp.gnz) mpm_ordered_node(mp, ms, bg, start, order, terms, REACT, n);
+  if (REACT && blockIdx.x == 0) fold_react_records(rec, n_rec, cc, sd);
+}
+
+__global__ void __launch_bounds__(TPB) k_project_ordered(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+                                                         FsiMpmState ms, FsiCouplingParams cp, BinGrid bg,
+                                                         const int32_t* start, const int32_t* order) {
+  const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
+  if (c < (long long)lp.nx * lp.ny * lp.nz) project_ordered_cell(lp, ls, mp, ms, cp, bg, start, order, c);
+}
+
+__global__ void k_proj_counters_clear(FsiCouplingCounters* cc) {
+  if (threadIdx.x == 0) {
+    cc->projected_mass = 0.0;
+    cc->projected_volume_raw = 0.0;
+    cc->projected_volume_clamped = 0.0;
+    cc->max_phi_raw_bits = 0;
+    cc->active_cell_count = 0;
+  }
+}
+
 __global__ void __launch_bounds__(TPB) k_mpm_grid_update(FsiMpmParams mp, FsiMpmState ms) {
   const long long n = (long long)blockIdx.x * TPB + threadIdx.x;
   if (n < (long long)mp.gnx * mp.gny * mp.gnz) mpm_grid_node(mp, ms, n);
@@ -1567,6 +1741,37 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_g2p(const FsiBatchInstance* inst,
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+namespace {
+// Checks of a binning's buffers; the bin count must fit the int32 keys.
+int check_bins(const FsiBins* b, long long nbins) {
+  if (!b || !b->key || !b->start || !b->order || !b->work) return fail(FSI_E_INVALID, "null binning buffers");
+  if (nbins + 2 > 0x7fffffffLL) return fail(FSI_E_INVALID, "grid too large for 32-bit bin keys");
+  return FSI_OK;
+}
+
+// Clears the bin counts (before the keys are counted).
+void bins_begin(long long nbins, const FsiBins& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(nbins + 1)), dim3(TPB), 0, s, b.work, nbins + 1, 0);
+}
+
+// After the keys are counted (b.work[0 .. nbins]): scan into start, fill
+// order, sort each bin by particle index.
+void bins_finish(long long nbins, int32_t P, const FsiBins& b, hipStream_t s) {
+  const long long n = nbins + 1;  // with the overflow bin
+  const int tiles = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
+  int32_t* count = b.work;
+  int32_t* part = b.work + n;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(TPB), 0, s, count, n, b.start, part);
+  hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(TPB), 0, s, part, tiles, b.start, n);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n)), dim3(TPB), 0, s, b.start, n, (const int32_t*)part);
+  if (P > 0) {
+    hipLaunchKernelGGL(k_bin_fill, dim3(blocks_for(P)), dim3(TPB), 0, s, (const int32_t*)b.key, P,
+                       (const int32_t*)b.start, count, b.order);
+    hipLaunchKernelGGL(k_bin_sort, dim3(blocks_for(n)), dim3(TPB), 0, s, (const int32_t*)b.start, n, b.order);
+  }
+}
+}  // namespace
+
 // Scratch of fsi_mpm_substep_coupled_noalloc: room for the reaction records
 // (FSI_COUPLED_WORKSPACE_DOUBLES) of FSI_NOALLOC_MAX_PARTICLES particles,
 // 2.25 MiB, held by every GPU the library runs on until the process ends.  In
@@ -1606,13 +1811,14 @@ const char* fsi_last_error(void) { return g_last_error.c_str(); }
 int fsi_version(void) { return 1; }
 
 int fsi_abi_sizes(int32_t* out, int n) {
-  const int32_t sz[10] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
+  const int32_t sz[11] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
                          (int32_t)sizeof(FsiLbmCounters),    (int32_t)sizeof(FsiLbmStats),
                          (int32_t)sizeof(FsiMpmParams),      (int32_t)sizeof(FsiMpmState),
                          (int32_t)sizeof(FsiCouplingParams), (int32_t)sizeof(FsiCouplingCounters),
-                         (int32_t)sizeof(FsiMpmStats),       (int32_t)sizeof(FsiBatchInstance)};
+                         (int32_t)sizeof(FsiMpmStats),       (int32_t)sizeof(FsiBatchInstance),
+                         (int32_t)sizeof(FsiBins)};
   int k = 0;
-  for (; k < n && k < 10; ++k) out[k] = sz[k];
+  for (; k < n && k < 11; ++k) out[k] = sz[k];
   return k;
 }
 
@@ -1883,6 +2089,104 @@ int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* l
     if (rc) return rc;
   }
   return fsi_mpm_substep_coupled(lp, ls, mp, ms, cp, cc, mode, ws, stream);
+}
+
+int fsi_bin_particles(const float* x, int32_t n_particles, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
+                      const FsiBins* bins, void* stream) {
+  if (nx < 1 || ny < 1 || nz < 1 || n_particles < 0 || (n_particles > 0 && !x))
+    return fail(FSI_E_INVALID, "fsi_bin_particles: bad grid or particles");
+  const BinGrid bg = make_bins(nx, ny, nz);
+  int rc = check_bins(bins, bg.nb);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  bins_begin(bg.nb, *bins, s);
+  if (n_particles > 0)
+    hipLaunchKernelGGL(k_bin_keys, dim3(blocks_for(n_particles)), dim3(TPB), 0, s, x, n_particles, inv_dx, bg,
+                       bins->work, bins->key);
+  bins_finish(bg.nb, n_particles, *bins, s);
+  return check_launch("fsi_bin_particles");
+}
+
+int fsi_mpm_substep_ordered(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                            const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode,
+                            const FsiBins* bins, float* terms, double* workspace, void* stream) {
+  int rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  if (mode < 0 || mode > 2) return fail(FSI_E_INVALID, "mode must be 0 (none), 1 (penalty) or 2 (moving boundary)");
+  const BinGrid bg = make_bins(mp->gnx, mp->gny, mp->gnz);
+  rc = check_bins(bins, bg.nb);
+  if (rc) return rc;
+  if (mp->n_particles > 0 && !terms) return fail(FSI_E_INVALID, "fsi_mpm_substep_ordered: null terms buffer");
+  FsiLbmParams lpv;
+  FsiLbmState lsv;
+  FsiCouplingParams cpv;
+  memset(&lpv, 0, sizeof(lpv));
+  memset(&lsv, 0, sizeof(lsv));
+  memset(&cpv, 0, sizeof(cpv));
+  if (mode) {
+    rc = check_lbm(lp, ls);
+    if (rc) return rc;
+    if (!cp || !cc || !workspace || !ls->hydro_force || !ls->solid_phi)
+      return fail(FSI_E_INVALID, "the reaction needs coupling params, counters, workspace, hydro_force, solid_phi");
+    if (cp->inv_dx_norm != mp->inv_dx)
+      return fail(FSI_E_INVALID, "fsi_mpm_substep_ordered: the lattice spacing must be the MPM grid's");
+    lpv = *lp;
+    lsv = *ls;
+    cpv = *cp;
+  }
+  hipStream_t s = as_stream(stream);
+  const long long G = (long long)mp->gnx * mp->gny * mp->gnz;
+  const unsigned pb = pblocks_for(mp->n_particles);
+  bins_begin(bg.nb, *bins, s);
+  if (mp->n_particles > 0) {
+    if (mode == 1)
+      hipLaunchKernelGGL(k_ordered_terms<1>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, terms, workspace, bg,
+                         bins->work, bins->key);
+    else if (mode == 2)
+      hipLaunchKernelGGL(k_ordered_terms<2>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, terms, workspace, bg,
+                         bins->work, bins->key);
+    else
+      hipLaunchKernelGGL(k_ordered_terms<0>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, terms, workspace, bg,
+                         bins->work, bins->key);
+  }
+  bins_finish(bg.nb, mp->n_particles, *bins, s);
+  if (mode && mp->n_particles > 0)
+    hipLaunchKernelGGL(k_ordered_nodes<true>, dim3(blocks_for(G)), dim3(TPB), 0, s, *mp, *ms, bg,
+                       (const int32_t*)bins->start, (const int32_t*)bins->order, (const float*)terms,
+                       (const double*)workspace, (int)pb, cc);
+  else
+    hipLaunchKernelGGL(k_ordered_nodes<false>, dim3(blocks_for(G)), dim3(TPB), 0, s, *mp, *ms, bg,
+                       (const int32_t*)bins->start, (const int32_t*)bins->order, (const float*)terms,
+                       (const double*)nullptr, 0, cc);
+  if (mode && mp->n_particles == 0) hipLaunchKernelGGL(k_reaction_clear, dim3(1), dim3(64), 0, s, cc);
+  if (mp->n_particles > 0) hipLaunchKernelGGL(k_mpm_g2p, dim3(pb), dim3(PTPB), 0, s, *mp, *ms);
+  return check_launch("fsi_mpm_substep_ordered");
+}
+
+int fsi_project_ordered(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                        const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc,
+                        const FsiBins* bins, void* stream) {
+  int rc = check_lbm(lp, ls);
+  if (rc) return rc;
+  rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  if (!cp || !cc || !ls->solid_phi || !ls->solid_mass || !ls->solid_vel)
+    return fail(FSI_E_INVALID, "project needs coupling params/counters and projection fields");
+  const BinGrid bg = make_bins(lp->nx, lp->ny, lp->nz);
+  rc = check_bins(bins, bg.nb);
+  if (rc) return rc;
+  const long long N = (long long)lp->nx * lp->ny * lp->nz;
+  hipStream_t s = as_stream(stream);
+  bins_begin(bg.nb, *bins, s);
+  if (mp->n_particles > 0)
+    hipLaunchKernelGGL(k_bin_keys, dim3(blocks_for(mp->n_particles)), dim3(TPB), 0, s, (const float*)ms->x,
+                       mp->n_particles, cp->inv_dx_norm, bg, bins->work, bins->key);
+  bins_finish(bg.nb, mp->n_particles, *bins, s);
+  hipLaunchKernelGGL(k_proj_counters_clear, dim3(1), dim3(64), 0, s, cc);
+  hipLaunchKernelGGL(k_project_ordered, dim3(blocks_for(N)), dim3(TPB), 0, s, *lp, *ls, *mp, *ms, *cp, bg,
+                     (const int32_t*)bins->start, (const int32_t*)bins->order);
+  hipLaunchKernelGGL(k_proj_normalize, dim3(blocks_for(N)), dim3(TPB), 0, s, *lp, *ls, *cp, cc);
+  return check_launch("fsi_project_ordered");
 }
 
 int fsi_mpm_grid_update(const FsiMpmParams* mp, const FsiMpmState* ms, void* stream) {

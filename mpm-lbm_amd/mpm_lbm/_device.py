@@ -114,3 +114,22 @@ def struct_buffer(cls, device, count: int = 1) -> torch.Tensor:
 
 def bits_to_float(u: int) -> float:
     return float(np.array([u], dtype=np.uint32).view(np.float32)[0])
+
+
+class BinBuffers:
+    """Device buffers of one particle binning (the FsiBins struct) for
+    `n_particles` particles over a node grid of `shape`."""
+
+    def __init__(self, n_particles: int, shape, device):
+        from . import _native
+
+        nb = _native.bin_count(*shape)
+        i32 = dict(dtype=torch.int32, device=device)
+        self.shape = tuple(int(v) for v in shape)
+        self.key = torch.zeros(max(1, int(n_particles)), **i32)
+        self.start = torch.zeros(nb + 2, **i32)
+        self.order = torch.zeros(max(1, int(n_particles)), **i32)
+        self.work = torch.zeros(_native.bin_work_ints(nb), **i32)
+        self.struct = _native.Bins(self.key.data_ptr(), self.start.data_ptr(), self.order.data_ptr(),
+                                   self.work.data_ptr())
+

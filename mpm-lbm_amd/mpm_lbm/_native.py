@@ -5,7 +5,10 @@ compares their sizes with the library's own `fsi_abi_sizes()` so a layout
 drift fails loudly instead of corrupting memory.
 
 Only `load_hip()` is used by the product path.  It raises if the gfx950
-library has not been built: there is no CPU fallback.
+library has not been built: there is no CPU fallback.  `load_hip(exact=True)`
+loads libfsi_hip_exact.so, the same kernels built with -ffp-contract=off
+(every float operation rounded on its own, as the serial CPU build and the
+oracle evaluate them), which the particle-ordered transfer mode uses.
 """
 from __future__ import annotations
 
@@ -16,6 +19,7 @@ from pathlib import Path
 _HERE = Path(__file__).resolve().parent
 LIB_DIR = _HERE / "_lib"
 HIP_LIB_PATH = LIB_DIR / "libfsi_hip.so"
+HIP_EXACT_LIB_PATH = LIB_DIR / "libfsi_hip_exact.so"
 
 f32 = ctypes.c_float
 i32 = ctypes.c_int32
@@ -209,8 +213,13 @@ class BatchInstance(ctypes.Structure):
     ]
 
 
+class Bins(ctypes.Structure):
+    """FsiBins: a particle binning (key, start, order) and its work ints."""
+    _fields_ = [("key", vp), ("start", vp), ("order", vp), ("work", vp)]
+
+
 ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, CouplingParams,
-               CouplingCounters, MpmStats, BatchInstance]
+               CouplingCounters, MpmStats, BatchInstance, Bins]
 
 EXPORTED_SYMBOLS = [
     "fsi_last_error", "fsi_version", "fsi_abi_sizes", "fsi_sync",
@@ -219,6 +228,7 @@ EXPORTED_SYMBOLS = [
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
     "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_substep_coupled_noalloc",
     "fsi_mpm_reduce_stats",
+    "fsi_bin_particles", "fsi_mpm_substep_ordered", "fsi_project_ordered",
     "fsi_batch_block_map", "fsi_batch_launch",
 ]
 
@@ -226,6 +236,19 @@ PARTICLE_BLOCK = 64  # FSI_PARTICLE_BLOCK
 NOALLOC_MAX_PARTICLES = 1 << 21  # FSI_NOALLOC_MAX_PARTICLES
 LINK_STATS_VALUES = 6  # FSI_LINK_STATS_VALUES
 LINK_STATS_WS_DOUBLES = 18 * 64 * LINK_STATS_VALUES  # FSI_LINK_STATS_WS_DOUBLES
+
+
+ORDERED_TERMS_FLOATS = 28  # FSI_ORDERED_TERMS_FLOATS
+
+
+def bin_count(nx: int, ny: int, nz: int) -> int:
+    """FSI_BIN_COUNT(nx, ny, nz): bins without the overflow bin."""
+    return (int(nx) + 3) * (int(ny) + 3) * (int(nz) + 3)
+
+
+def bin_work_ints(n_bins: int) -> int:
+    """FSI_BIN_WS_INTS(n_bins)."""
+    return int(n_bins) + 1 + (int(n_bins) + 1 + 1023) // 1024
 
 
 def coupled_workspace_doubles(n_particles: int) -> int:
@@ -276,6 +299,11 @@ class Native:
         L.fsi_mpm_substep_coupled_noalloc.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),
                                                       P(CouplingParams), vp, ctypes.c_int, vp]
         L.fsi_mpm_reduce_stats.argtypes = [P(MpmParams), P(MpmState), vp, vp, vp]
+        L.fsi_bin_particles.argtypes = [vp, i32, ctypes.c_float, i32, i32, i32, P(Bins), vp]
+        L.fsi_mpm_substep_ordered.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),
+                                              P(CouplingParams), vp, ctypes.c_int, P(Bins), vp, vp, vp]
+        L.fsi_project_ordered.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState), P(CouplingParams),
+                                          vp, P(Bins), vp]
         L.fsi_batch_block_map.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
         L.fsi_batch_launch.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]
         for name in EXPORTED_SYMBOLS:
@@ -301,23 +329,24 @@ class Native:
         return rc
 
 
-_HIP: Native | None = None
+_HIP: dict = {}
 
 
-def load_hip() -> Native:
-    """Load the gfx950 library built in-tree; raise if it is missing."""
-    global _HIP
-    if _HIP is None:
+def load_hip(exact: bool = False) -> Native:
+    """Load the gfx950 library built in-tree (exact=True: its build without
+    fma fusion); raise if it is missing."""
+    path = HIP_EXACT_LIB_PATH if exact else HIP_LIB_PATH
+    if path not in _HIP:
         # torch first: its bundled libamdhip64.so.7 must be the one this
         # library binds to, so both share one HIP runtime and its streams.
         import torch  # noqa: F401
 
-        if not HIP_LIB_PATH.is_file():
+        if not path.is_file():
             raise FsiError(
-                f"{HIP_LIB_PATH} is not built: run __graft_entry__.build() (hipcc --offload-arch=gfx950) first"
+                f"{path} is not built: run __graft_entry__.build() (hipcc --offload-arch=gfx950) first"
             )
-        _HIP = Native(HIP_LIB_PATH, "hip")
-    return _HIP
+        _HIP[path] = Native(path, "hip")
+    return _HIP[path]
 
 
 def ptr(t) -> int:

@@ -7,7 +7,9 @@ velocity by mass (mass > 1e-12) and clamps phi to [0, 1].
 """
 from __future__ import annotations
 
-from ..._device import bits_to_float
+import ctypes
+
+from ..._device import BinBuffers, bits_to_float
 from ... import _native
 from ._common import CouplingBase
 
@@ -18,13 +20,23 @@ class MPMToLBMProjector3D(CouplingBase):
         self.cell_volume_norm = sim_config.dx_norm ** 3
         self.vel_scale_norm_to_lbm = sim_config.lbm_dt_phys / sim_config.dx_norm
         self.eps_mass = 1.0e-12
+        self.transfer_order = "atomic"  # or "particle": binned, sums in particle order
+        self._bins = None
 
     def project(self, solid, lbm, clear=True):
         if not clear:
             raise NotImplementedError("project(clear=False) is not supported")
         stream = self._bind(lbm)
-        self._call("fsi_project", *self._refs(lbm._params, lbm._state, solid._params, solid._state, self.params),
-                   _native.ptr(self._counters_buf), stream)
+        refs = self._refs(lbm._params, lbm._state, solid._params, solid._state, self.params)
+        if self.transfer_order == "particle":
+            # each cell's sums in particle order (deterministic)
+            shape = (lbm.nx, lbm.ny, lbm.nz)
+            if self._bins is None or self._bins.shape != shape or self._bins.key.numel() < solid.n_particles:
+                self._bins = BinBuffers(solid.n_particles, shape, lbm.device)
+            self._call("fsi_project_ordered", *refs, _native.ptr(self._counters_buf), ctypes.byref(self._bins.struct),
+                       stream)
+        else:
+            self._call("fsi_project", *refs, _native.ptr(self._counters_buf), stream)
         lbm.mark_projection_updated()
 
     def get_stats(self):

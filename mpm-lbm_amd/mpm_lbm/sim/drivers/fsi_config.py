@@ -4,7 +4,9 @@ Keys, defaults, validation order and error messages follow the reference
 (sim/drivers/fsi_config.py:67-497) so existing JSON configs and callers work
 unchanged.  Extensions: `lattice_shape` (non-cubic lattices),
 `lbm_population_storage` ("f32" | "f16" | "bf16" deviation storage) and
-`lbm_collision` ("mrt" | "regularized").
+`lbm_collision` ("mrt" | "regularized") and `mpm_transfer_order` ("atomic" |
+"particle": the order of the projection's and p2g's node sums, "particle"
+being the deterministic order of a sequential scatter).
 """
 from dataclasses import asdict, dataclass
 import json
@@ -161,6 +163,7 @@ class FSIDriverConfig:
     lattice_shape: Optional[Tuple[int, int, int]] = None
     lbm_population_storage: str = "f32"
     lbm_collision: str = "mrt"
+    mpm_transfer_order: str = "atomic"
 
     def __post_init__(self):
         for name in _TUPLE3_FIELDS:
@@ -306,6 +309,7 @@ class FSIDriverConfig:
             raise ValueError("fluent_like_monitor_nearest_count must be positive")
         _one_of(self.lbm_population_storage, ("f32", "f16", "bf16"), "lbm_population_storage")
         _one_of(self.lbm_collision, ("mrt", "regularized"), "lbm_collision")
+        _one_of(self.mpm_transfer_order, ("atomic", "particle"), "mpm_transfer_order")
         if self.lattice_shape is not None:
             if len(self.lattice_shape) != 3 or min(self.lattice_shape) <= 0:
                 raise ValueError("lattice_shape must hold three positive integers")

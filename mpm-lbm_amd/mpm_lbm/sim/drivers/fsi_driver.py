@@ -98,6 +98,12 @@ class FSIDriver3D:
         self.material_reference_used_for_mpm_config = False
         self.lbm_restart_load_report = None
         self._particle_cloud = particle_cloud
+        if native is None and config.mpm_transfer_order == "particle":
+            # the particle-ordered mode runs the kernels built without fma
+            # fusion: the same arithmetic as the serial CPU build and the oracle
+            from ... import _native as _nat
+
+            native = _nat.load_hip(exact=True)
         self._native = native
         self._device = device
         self._write_geometry_file = write_geometry_file
@@ -158,7 +164,9 @@ class FSIDriver3D:
         v0 = self.config.initial_solid_velocity_norm
         self.solid.set_uniform_velocity(float(v0[0]), float(v0[1]), float(v0[2]))
 
+        self.solid.transfer_order = self.config.mpm_transfer_order
         self.projector = MPMToLBMProjector3D(self.sim)
+        self.projector.transfer_order = self.config.mpm_transfer_order
         if self.config.coupling_mode == "penalty":
             self.penalty_coupler = PenaltyFSICoupler3D(self.sim, beta_lbm=self.config.beta_lbm,
                                                        force_cap_lbm=self.config.penalty_force_cap_lbm)

@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from ... import _native
-from ..._device import FieldView, default_device, read_struct, stream_handle, struct_buffer
+from ..._device import BinBuffers, FieldView, default_device, read_struct, stream_handle, struct_buffer
 from .config import MPMConfig
 
 
@@ -49,6 +49,10 @@ class MPMSolid3D:
         self.n_particles = int(n_particles)
         self.bound = config.bound
         self.use_apic = config.use_apic
+        # order of the grid-node sums of substep_coupled: "atomic" (LDS tiles,
+        # float atomics; fastest) or "particle" (binned, in particle order;
+        # deterministic)
+        self.transfer_order = "atomic"
         self.gravity_x, self.gravity_y, self.gravity_z = config.gravity
         self.box_min_x, self.box_min_y, self.box_min_z = config.box_min
         self.box_max_x, self.box_max_y, self.box_max_z = config.box_max
@@ -247,6 +251,9 @@ class MPMSolid3D:
         if getattr(self, "_coupled_ws", None) is None:
             self._coupled_ws = torch.zeros(_native.coupled_workspace_doubles(self.n_particles), dtype=torch.float64,
                                            device=self.device)
+        if self.transfer_order == "particle":
+            self._substep_ordered(lbm, coupler, mode)
+            return
         if mode:
             stream = coupler._bind(lbm)
             lbm._sync_hydro()
@@ -256,6 +263,26 @@ class MPMSolid3D:
         else:
             self.native.call("fsi_mpm_substep_coupled", None, None, ctypes.byref(self._params),
                              ctypes.byref(self._state), None, None, 0, None, self._stream())
+
+    def _substep_ordered(self, lbm, coupler, mode):
+        """substep_coupled with every grid-node sum in particle order
+        (fsi_mpm_substep_ordered): deterministic, and in the order of the
+        reference's kernels run sequentially."""
+        if getattr(self, "_bins", None) is None:
+            self._bins = BinBuffers(self.n_particles, self.grid_shape, self.device)
+            self._ordered_terms = torch.zeros(_native.ORDERED_TERMS_FLOATS * self.n_particles, dtype=torch.float32,
+                                              device=self.device)
+        if mode:
+            stream = coupler._bind(lbm)
+            lbm._sync_hydro()
+            self.native.call("fsi_mpm_substep_ordered", ctypes.byref(lbm._params), ctypes.byref(lbm._state),
+                             ctypes.byref(self._params), ctypes.byref(self._state), ctypes.byref(coupler.params),
+                             _native.ptr(coupler._counters_buf), int(mode), ctypes.byref(self._bins.struct),
+                             _native.ptr(self._ordered_terms), _native.ptr(self._coupled_ws), stream)
+        else:
+            self.native.call("fsi_mpm_substep_ordered", None, None, ctypes.byref(self._params),
+                             ctypes.byref(self._state), None, None, 0, ctypes.byref(self._bins.struct),
+                             _native.ptr(self._ordered_terms), None, self._stream())
 
     # ----------------------------------------------------------- diagnostics
     def reduce_stats(self) -> dict:

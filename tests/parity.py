@@ -128,14 +128,15 @@ def field_passes(errs, k, tol=1.0e-5):
     return False, None
 
 
-def check_history(history, label, tol=1.0e-5, strict=False):
-    """Assert the parity bar on every step; strict=True allows only (a)."""
+def check_history(history, label, tol=1.0e-5, strict=False, strict_fields=()):
+    """Assert the parity bar on every step; strict=True allows only (a), on
+    every field or (strict_fields) on the named ones."""
     for step, errs in enumerate(history):
         assert errs["solid_equal"], f"{label}: solid flags differ at step {step}"
         for k, v in errs.items():
             if k in ("solid_equal", "vs_fp64", "method"):
                 continue
-            ok, _ = (v <= tol, "fp32") if strict else field_passes(errs, k, tol)
+            ok, _ = (v <= tol, "fp32") if strict or k in strict_fields else field_passes(errs, k, tol)
             extra = ""
             if "vs_fp64" in errs:
                 extra = f" (vs FP64 {errs['vs_fp64'][k]:.3e}, FP32 method error {errs['method'][k]:.3e})"

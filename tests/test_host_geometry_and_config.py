@@ -105,21 +105,24 @@ def test_hip_library_exports_every_declared_symbol():
 
     from mpm_lbm import _native
 
-    assert _native.HIP_LIB_PATH.is_file(), "libfsi_hip.so not built"
-    lib = ctypes.CDLL(str(_native.HIP_LIB_PATH))
     declared = _declared_symbols()
     assert set(declared) == set(_native.EXPORTED_SYMBOLS)
-    for name in declared:
-        assert hasattr(lib, name), name
-    # struct layouts: the library reports its sizeofs without touching a GPU
-    native = _native.Native(_native.HIP_LIB_PATH, "hip")
-    native.check_abi()
+    for path in (_native.HIP_LIB_PATH, _native.HIP_EXACT_LIB_PATH):
+        assert path.is_file(), f"{path.name} not built"
+        lib = ctypes.CDLL(str(path))
+        for name in declared:
+            assert hasattr(lib, name), (path.name, name)
+        # struct layouts: the library reports its sizeofs without touching a GPU
+        _native.Native(path, "hip").check_abi()
 
 
 def test_product_path_refuses_missing_library(monkeypatch, tmp_path):
     from mpm_lbm import _native
 
     monkeypatch.setattr(_native, "HIP_LIB_PATH", tmp_path / "missing.so")
-    monkeypatch.setattr(_native, "_HIP", None)
+    monkeypatch.setattr(_native, "HIP_EXACT_LIB_PATH", tmp_path / "missing_exact.so")
+    monkeypatch.setattr(_native, "_HIP", {})
     with pytest.raises(_native.FsiError, match="not built"):
         _native.load_hip()
+    with pytest.raises(_native.FsiError, match="not built"):
+        _native.load_hip(exact=True)
