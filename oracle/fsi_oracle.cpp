@@ -263,6 +263,15 @@ void colission_regularized(const LbmArgs<T>& a, const T* F, T* f, const T* rho, 
   }
 }
 
+// The streaming target of fluid cell (i,j,k) in direction s: its periodic
+// neighbour (fluid.py:420-429, :434-436, :493-495).  The link bounces back
+// when that cell is solid (fluid.py:437-440, :496-519).
+inline long long stream_target(const Dims& d, int i, int j, int k, int s) {
+  int ii = i + E[s][0], jj = j + E[s][1], kk = k + E[s][2];
+  periodic(d, ii, jj, kk);
+  return d.idx(ii, jj, kk);
+}
+
 // streaming1: push with static half-way bounce-back (fluid.py:432-440)
 template <typename T>
 void streaming1(const LbmArgs<T>& a, const T* f, T* F, const int8_t* solid) {
@@ -273,9 +282,7 @@ void streaming1(const LbmArgs<T>& a, const T* f, T* F, const int8_t* solid) {
         const long long c = d.idx(i, j, k);
         if (solid[c] != 0) continue;
         for (int s = 0; s < Q; ++s) {
-          int ii = i + E[s][0], jj = j + E[s][1], kk = k + E[s][2];
-          periodic(d, ii, jj, kk);
-          const long long cp = d.idx(ii, jj, kk);
+          const long long cp = stream_target(d, i, j, k, s);
           if (solid[cp] == 0)
             F[cp * Q + s] = f[c * Q + s];
           else
@@ -313,9 +320,7 @@ void streaming_mb(const LbmArgs<T>& a, const T* f, T* F, const int8_t* solid, co
         const long long c = d.idx(i, j, k);
         if (solid[c] != 0) continue;
         for (int s = 0; s < Q; ++s) {
-          int ii = i + E[s][0], jj = j + E[s][1], kk = k + E[s][2];
-          periodic(d, ii, jj, kk);
-          const long long cp = d.idx(ii, jj, kk);
+          const long long cp = stream_target(d, i, j, k, s);
           if (solid[cp] == 0) {
             F[cp * Q + s] = f[c * Q + s];
           } else {
@@ -1098,3 +1103,20 @@ ORC_DEFINE(double, f64)
 ORC_EXPORT int orc_mbstats_size_f32() { return (int)sizeof(MbStats<float>); }
 ORC_EXPORT int orc_mbstats_size_f64() { return (int)sizeof(MbStats<double>); }
 ORC_EXPORT int orc_mutant() { return ORACLE_MUTANT; }
+
+// The bounce-back links of every fluid cell, as the streaming passes classify
+// them: bit s (1..18) of mask[c] is set when the streaming target of fluid
+// cell c in direction s is solid; solid cells get 0.
+ORC_EXPORT void orc_link_masks(int nx, int ny, int nz, const int8_t* solid, uint32_t* mask) {
+  const Dims d{nx, ny, nz};
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const long long c = d.idx(i, j, k);
+        uint32_t m = 0;
+        if (solid[c] == 0)
+          for (int s = 1; s < Q; ++s)
+            if (solid[stream_target(d, i, j, k, s)] != 0) m |= 1u << s;
+        mask[c] = m;
+      }
+}
