@@ -661,35 +661,34 @@ FSI_HD void wall_accumulate_loaded(double* A, uint32_t y, const double* a, float
   o[3] = n > a[3] ? n : a[3];
 }
 
-// Previous-step velocity of cell (i,j,k) as the reference's `v` field holds it
-// at Boundary_condition time: stored moments, or moments of F_src; 0 on solid.
+// Previous-step velocity of cell c as the reference's `v` field holds it at
+// Boundary_condition time: the stored moments, or the moments of F_src; 0 on
+// a solid cell.  Every load is issued at once and none waits on another (the
+// solid flag decides only at the end); 32-bit addressing.
 template <typename Pop>
-FSI_HD void prev_velocity(const FsiLbmParams& p, const FsiLbmState& st, const Grid& g,
-                          const Pop& Fs, uint32_t flags, long long c, float& ux, float& uy,
-                          float& uz) {
-  if (st.cellinfo[c] & CI_SOLID) {
-    ux = uy = uz = 0.0f;
-    return;
-  }
+FSI_HD void prev_velocity_flat(const FsiLbmParams& p, const FsiLbmState& st, const Pop& Fs, uint32_t flags,
+                               uint32_t c, float& ux, float& uy, float& uz) {
+  const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
+  const bool solid = (at32(st.cellinfo, c) & CI_SOLID) != 0;
   if (flags & FSI_SUB_STORED_MOMENTS) {
-    ux = st.vel[c];
-    uy = st.vel[g.N + c];
-    uz = st.vel[2 * g.N + c];
-    return;
-  }
-  float Fn[Q];
+    ux = at32(st.vel, c);
+    uy = at32(st.vel + N, c);
+    uz = at32(st.vel + 2 * (size_t)N, c);
+  } else {
+    float Fn[Q];
 #pragma unroll
-  for (int s = 0; s < Q; ++s) Fn[s] = Fs.get(g.N, s, c);
-  float gx = p.force[0], gy = p.force[1], gz = p.force[2];
-  if (flags & FSI_SUB_CELL_FORCE) {
-    gx += st.cell_force[c];
-    gy += st.cell_force[g.N + c];
-    gz += st.cell_force[2 * g.N + c];
+    for (int s = 0; s < Q; ++s) Fn[s] = Fs.ld(N, s, c);
+    float gx = p.force[0], gy = p.force[1], gz = p.force[2];
+    if (flags & FSI_SUB_CELL_FORCE) {
+      gx += at32(st.cell_force, c);
+      gy += at32(st.cell_force + N, c);
+      gz += at32(st.cell_force + 2 * (size_t)N, c);
+    }
+    float r;
+    moments_of(Fn, gx, gy, gz, r, ux, uy, uz);
   }
-  float r;
-  moments_of(Fn, gx, gy, gz, r, ux, uy, uz);
+  if (solid) ux = uy = uz = 0.0f;
 }
-
 
 // Periodic neighbour offsets of cell (i,j,k) (fluid.py:420-429) as signed
 // 32-bit element deltas per axis; d(s) = index(i+ex, j+ey, k+ez) - index(i,j,k).
@@ -852,6 +851,13 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
     gz += at32(st.cell_force + 2 * (size_t)N, c);
   }
   const bool has_force = (gx != 0.0f) || (gy != 0.0f) || (gz != 0.0f);
+  // The pressure outlet's reset needs the velocity of the cell behind it
+  // (fluid.py:944-951): its loads go out now, with nothing waiting on them
+  // until the reset, instead of as a second memory round trip at the end.
+  const bool legacy_reset = p.open_bc == FSI_OPEN_BC_EQUILIBRIUM;
+  const bool outlet_p = legacy_reset && i == p.nx - 1 && p.bc_x_right == FSI_BC_PRESSURE;
+  float obx = 0.0f, oby = 0.0f, obz = 0.0f;
+  if (outlet_p) prev_velocity_flat(p, st, Fs, flags, c - (uint32_t)(p.ny * p.nz), obx, oby, obz);
   float m[Q];
   moments_M(F, m);
   float rho, ux, uy, uz;
@@ -956,8 +962,9 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
         bz = p.vel_bc_x_right[2];
       } else {
         brho = p.rho_bc_x_right;
-        const long long c2 = cell_of(g, (g.nx > 1 ? g.nx - 2 : 0), j, k);
-        prev_velocity(p, st, g, Fs, flags, c2, bx, by, bz);
+        bx = obx;  // the velocity of cell (nx - 2, j, k), loaded above
+        by = oby;
+        bz = obz;
       }
     }
 #pragma unroll
