@@ -5,8 +5,10 @@ Workload: 64^3 duct with a 4x32x24-cell elastic flap (24,576 particles,
 seeded jitter), moving-wall bounce-back with the subcycle force
 accumulator, 120 LBM substeps and 10 MPM substeps per FSI step.  A "step" of
 this benchmark is one FSI step.  `value` is MLUPS counted over all lattice
-cells: 64^3 * 120 * K / seconds / 1e6 for K timed FSI steps (projection, flag
-rebuild, 120 fused LBM substeps, finalize, moments and 10 MPM substeps each).
+cells, 64^3 * 120 * steps / seconds / 1e6, of the median of `--repeats`
+blocks into which the K timed FSI steps are split (the blocks run back to
+back between two synchronisations; each FSI step is projection, flag
+rebuild, 120 fused LBM substeps, finalize, moments and 10 MPM substeps).
 
 Also reported:
   roofline     the fused LBM substep kernel: algorithmic bytes per launch
@@ -15,7 +17,9 @@ Also reported:
                the kernels run on, against 8 TB/s;
   copy_ceiling device-to-device copy bandwidth measured in the same run;
   cpu_baseline the reference-structured C++ oracle (OpenMP) on the host
-               cores, one FSI step of the same workload.
+               cores: median of 3 FSI steps of the same workload after one
+               warm-up step;
+  batched / served / precision   the three forms beside the plain path.
 """
 from __future__ import annotations
 
@@ -46,13 +50,19 @@ def parse():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--substeps", type=int, default=120)
     ap.add_argument("--kernel-launches", type=int, default=600, help="substep launches timed for the roofline")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="FSI steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="timed FSI steps of the CPU baseline sample, after one warm-up step; median (0 = skip)")
     ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "lbm_substep_64cube_traffic.json"))
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched sweep (0 = skip)")
     ap.add_argument("--batch-exchanges", type=int, default=3, help="timed batched exchanges")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the served form (0 = skip)")
-    ap.add_argument("--precision", type=int, default=0,
-                    help="FSI steps of the 192x96x96 precision form, 16-bit storage against FP32 (0 = skip)")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed blocks the K timed steps are split into; value = their median")
+    ap.add_argument("--precision", type=int, default=17,
+                    help="FSI steps of the 192x96x96 precision form (17 = 2,040 LBM steps), 16-bit storage against "
+                         "FP32 on the GPU (0 = skip)")
+    ap.add_argument("--precision-fp64-steps", type=int, default=2,
+                    help="FSI steps of the precision form compared with the FP64 oracle on the host (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of served requests, per s")
     return ap.parse_args()
 
@@ -206,37 +216,80 @@ def served_form(torch, rate, seconds):
     }
 
 
-def precision_form(torch, long_steps):
+def precision_form(torch, long_steps, fp64_steps):
     """192x96x96 duct with a 6x48x36-cell flap (82,944 particles, seed 3),
     regularised collision at tau = 0.55, subcycled moving boundary (120 LBM
-    substeps per FSI step): FP16- and BF16-deviation storage against FP32
-    storage after `long_steps` FSI steps, all on the GPU.  (The comparison of
-    each storage against the FP64 oracle is tests/precision_report.py.)"""
+    substeps per FSI step), in FP32, FP16- and BF16-deviation storage on the
+    GPU: each storage against the FP64 oracle (C++ on the host cores) after
+    `fp64_steps` FSI steps, and the 16-bit storages against FP32 after
+    `long_steps` FSI steps (17 = 2,040 LBM steps), with the total fluid mass
+    drift of every run."""
+    import numpy as np
+
     from mpm_lbm import precision_case
-
-    base, config, state, compare = precision_case.setup(max(1, long_steps))
     from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
+    from oracle.oracle import OracleFSI
 
-    def run(storage, steps):
-        d = FSIDriver3D(config(storage), f"/tmp/fsi_bench_prec/{storage}", particle_cloud=base.cloud,
-                        write_geometry_file=False)
+    long_steps = max(long_steps, fp64_steps)
+    base, config, state, compare = precision_case.setup(long_steps)
+
+    def snap(d):
+        return state(d.lbm.rho.to_numpy(), d.lbm.v.to_numpy(), d.lbm.solid.to_numpy(), d.solid.x.to_numpy())
+
+    runs, flags, gpu_ms = {}, None, {}
+    for st in ("f32", "f16", "bf16"):
+        d = FSIDriver3D(config(st), f"/tmp/fsi_bench_prec/{st}", particle_cloud=base.cloud, write_geometry_file=False)
         d.initialize()
-        s0 = state(d.lbm.rho.to_numpy(), d.lbm.v.to_numpy(), d.lbm.solid.to_numpy(), d.solid.x.to_numpy())
-        for _ in range(steps):
+        flags = d.lbm.solid.to_numpy() if flags is None else flags
+        s0, mid = snap(d), None
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(long_steps):
             d.step_once()
+            if k + 1 == fp64_steps:
+                mid = snap(d)
+        e1.record()
         torch.cuda.synchronize()
-        return s0, state(d.lbm.rho.to_numpy(), d.lbm.v.to_numpy(), d.lbm.solid.to_numpy(), d.solid.x.to_numpy())
-
-    out = {"workload": precision_case.WORKLOAD, "fsi_steps_16bit_vs_fp32": long_steps}
-    long = {st: run(st, long_steps) for st in ("f32", "f16", "bf16")}
-    ref0, ref32 = long["f32"]
-    out["16bit_vs_fp32"] = {st: compare(long[st][1], ref32, ref0["mass"], long[st][0]["mass"])
+        gpu_ms[st] = round(e0.elapsed_time(e1) / long_steps, 3)
+        runs[st] = (s0, mid, snap(d))
+        del d
+        torch.cuda.empty_cache()
+    out = {"workload": precision_case.WORKLOAD, "lbm_steps_16bit_vs_fp32": 120 * long_steps,
+           "ms_per_fsi_step_with_snapshots": gpu_ms}
+    ref0, _, ref32 = runs["f32"]
+    out["16bit_vs_fp32"] = {st: compare(runs[st][2], ref32, ref0["mass"], runs[st][0]["mass"])
                             for st in ("f16", "bf16")}
+    out["fp32_mass_drift"] = (ref32["mass"] - ref0["mass"]) / ref0["mass"]
+    if fp64_steps > 0:
+        t0 = time.perf_counter()
+        o = OracleFSI(config("f32"), flags, base.cloud, base.material, dtype=np.float64)
+        f = o.fields()
+        o0 = state(f["rho"], f["v"], f["solid"], f["x"])
+        for _ in range(fp64_steps):
+            o.step_once()
+        f = o.fields()
+        ref = state(f["rho"], f["v"], f["solid"], f["x"])
+        out["lbm_steps_vs_fp64"] = 120 * fp64_steps
+        out["vs_fp64"] = {st: compare(r[1], ref, o0["mass"], r[0]["mass"]) for st, r in runs.items()}
+        out["fp64_mass_drift"] = (ref["mass"] - o0["mass"]) / o0["mass"]
+        out["fp64_oracle_seconds"] = round(time.perf_counter() - t0, 1)
     log("[bench] precision: " + json.dumps(out))
     return out
 
 
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(workload, steps):
+    """The reference-structured C++/OpenMP oracle on the host cores: one
+    untimed warm-up FSI step, then the median of `steps` timed FSI steps."""
     import numpy as np
 
     from mpm_lbm.sim.geometry.config import GeometryConfig
@@ -245,20 +298,25 @@ def cpu_baseline(workload, steps):
 
     flags, _ = duct_flap_proxy_static_geometry(workload.shape, GeometryConfig.from_json(workload.geometry_json))
     o = OracleFSI(workload.config, flags, workload.cloud, workload.material, dtype=np.float32)
-    t0 = time.perf_counter()
+    o.step_once()  # warm-up: first-touch of the arrays, thread pool start
+    times = []
     for _ in range(steps):
+        t0 = time.perf_counter()
         o.step_once()
-    dt = time.perf_counter() - t0
-    cells = workload.n_cells * workload.config.lbm_substeps_per_fsi_step * steps
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    cells = workload.n_cells * workload.config.lbm_substeps_per_fsi_step
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     return {
         "value": round(cells / dt / 1e6, 3),
         "unit": "MLUPS",
         "cores": cores,
         "kind": "port",
-        "sample": f"{steps} FSI step(s) of the same 64^3 workload ({workload.config.lbm_substeps_per_fsi_step} "
-                  f"LBM + {workload.config.mpm_substeps_per_lbm_step} MPM substeps each), C++/OpenMP "
-                  f"reference-structured oracle, FP32, {dt:.2f} s",
+        "cpu_model": _cpu_model(),
+        "sample": f"median of {steps} FSI steps after 1 untimed warm-up step, of the same 64^3 workload "
+                  f"({workload.config.lbm_substeps_per_fsi_step} LBM + {workload.config.mpm_substeps_per_lbm_step} "
+                  f"MPM substeps each); C++/OpenMP reference-structured oracle, FP32; step times "
+                  f"{', '.join(f'{t:.2f}' for t in times)} s",
     }
 
 
@@ -283,18 +341,31 @@ def main():
         d.step_once()
     torch.cuda.synchronize()
     log(f"[bench] warmup {args.warmup} FSI steps done")
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # the K timed steps run back to back, split into `repeats` blocks each
+    # bracketed by events; value = the median block's rate
+    nblk = max(1, min(args.repeats, args.steps))
+    per = [args.steps // nblk + (1 if b < args.steps % nblk else 0) for b in range(nblk)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(nblk + 1)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    start.record()
-    for i in range(args.steps):
-        d.step_once()
-    end.record()
+    ev[0].record()
+    for b, nb in enumerate(per):
+        for i in range(nb):
+            d.step_once()
+        ev[b + 1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(end)
-    secs = gpu_ms / 1e3
-    mlups = n ** 3 * sub * args.steps / secs / 1e6
-    log(f"[bench] timed {args.steps} FSI steps: {gpu_ms:.2f} ms (wall {wall:.3f} s) -> {mlups:.1f} MLUPS")
+    gpu_ms = ev[0].elapsed_time(ev[-1])
+    blocks = [{"steps": nb, "ms": round(ev[b].elapsed_time(ev[b + 1]), 4)} for b, nb in enumerate(per)]
+    for blk in blocks:
+        blk["mlups"] = round(n ** 3 * sub * blk["steps"] / (blk["ms"] * 1e-3) / 1e6, 2)
+    med = sorted(blocks, key=lambda blk: blk["mlups"])[len(blocks) // 2]
+    mlups = med["mlups"]
+    ms_step = med["ms"] / med["steps"]
+    secs = ms_step * args.steps / 1e3
+    whole = n ** 3 * sub * args.steps / (gpu_ms * 1e-3) / 1e6
+    log(f"[bench] timed {args.steps} FSI steps in {nblk} blocks: {gpu_ms:.2f} ms (wall {wall:.3f} s); median block "
+        f"{mlups:.1f} MLUPS, whole region {whole:.1f} MLUPS; blocks {[blk['mlups'] for blk in blocks]}")
     row = d.collect_diagnostics(d.current_lbm_step)
     if not all(np.isfinite(v) for v in row.values() if isinstance(v, float)):
         raise RuntimeError("non-finite diagnostics after the timed steps")
@@ -320,7 +391,7 @@ def main():
 
     precision = None
     if args.precision > 0:
-        precision = precision_form(torch, args.precision)
+        precision = precision_form(torch, args.precision, args.precision_fp64_steps)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -335,7 +406,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(gpu_ms / args.steps, 4),
+        "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -346,6 +417,8 @@ def main():
                         f"{w.config.n_particles} particles, {sub} LBM substeps + "
                         f"{w.config.mpm_substeps_per_lbm_step} MPM substeps per FSI step",
             "lattice": [n, n, n],
+            "timed_blocks": blocks,
+            "value_whole_timed_region": round(whole, 2),
             "fluid_cells": fluid,
             "particles": w.config.n_particles,
             "lbm_substeps_per_fsi_step": sub,
