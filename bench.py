@@ -191,7 +191,7 @@ def served_form(torch, rate, seconds):
 
     reqs = poisson_requests(rate=rate, duration=seconds, seed=5, out_dir="/tmp/fsi_bench_serve")
     offered = sum(r.cells * 120 * r.exchanges for r in reqs) / seconds / 1e6
-    server = FSIServer(budget_bytes=200e9, out_dir="/tmp/fsi_bench_serve/run")
+    server = FSIServer(budget_bytes=200e9, out_dir="/tmp/fsi_bench_serve/run")  # slot table, requests built ahead
     t0 = time.perf_counter()
     server.run(reqs, sync=torch.cuda.synchronize)
     wall = time.perf_counter() - t0
@@ -199,7 +199,8 @@ def served_form(torch, rate, seconds):
     sustained = sum(r.cells * 120 * r.exchanges for r in done) / wall / 1e6
     lat = latency_summary(reqs)
     log(f"[bench] served: {len(done)}/{len(reqs)} requests in {wall:.1f} s, offered {offered:.0f} MLUPS, "
-        f"sustained {sustained:.0f} MLUPS, p50 {lat.get('p50_s', 0):.3f} s, p99 {lat.get('p99_s', 0):.3f} s")
+        f"sustained {sustained:.0f} MLUPS, p50 {lat.get('p50_s', 0):.3f} s, p99 {lat.get('p99_s', 0):.3f} s, "
+        f"{server.stats()}")
     return {
         "requests": len(reqs),
         "arrival_rate_per_s": rate,
@@ -211,8 +212,7 @@ def served_form(torch, rate, seconds):
         "p50_latency_s": round(lat.get("p50_s", float("nan")), 4),
         "p99_latency_s": round(lat.get("p99_s", float("nan")), 4),
         "max_latency_s": round(lat.get("max_s", float("nan")), 4),
-        "exchanges": server.exchanges_run,
-        "regroups": server.regroups,
+        **server.stats(),
     }
 
 

@@ -282,6 +282,7 @@ typedef struct FsiBatchInstance {
 int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask, int min_mpm,
                         int32_t* map_host);
 
+
 /* One batched operation over the workgroups of a device block map.  `t` is
  * the substep index within the exchange (LBM_SUBSTEP, MOMENTS) or, for
  * COUNTERS_CLEAR, which counters (0 projection, 1 penalty, 2 reaction,
@@ -289,11 +290,34 @@ int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domai
 int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const int32_t* block_map_dev,
                      int n_blocks, int t, void* stream);
 
+/* A batched exchange as a HIP graph built from its launch list (no stream
+ * capture): one kernel node per operation, in order.  An operation is what
+ * fsi_batch_launch takes (n_blocks > 0; COUNTERS_CLEAR ignores block_map and
+ * n_blocks).  When instances come and go, fsi_batch_graph_update rewrites the
+ * block maps and workgroup counts of the instantiated graph in place
+ * (hipGraphExecKernelNodeSetParams) for a launch list of the same operations
+ * and substep indices; it fails (FSI_E_INVALID) on any other list.  The
+ * instance table `inst_dev` and its slot count are fixed for the graph's
+ * life: rewrite the table's contents, not its address. */
+typedef struct FsiBatchOp {
+  int32_t op;
+  int32_t t;
+  const int32_t* block_map; /* device */
+  int32_t n_blocks;
+  int32_t pad0;
+} FsiBatchOp;
+
+int fsi_batch_graph_create(const FsiBatchOp* ops, int n_ops, const FsiBatchInstance* inst_dev, int n_inst,
+                           void** graph_out);
+int fsi_batch_graph_update(void* graph, const FsiBatchOp* ops, int n_ops);
+int fsi_batch_graph_launch(void* graph, void* stream);
+int fsi_batch_graph_destroy(void* graph);
+
 const char* fsi_last_error(void);
 int fsi_version(void);
 /* Writes sizeof of FsiLbmParams, FsiLbmState, FsiLbmCounters, FsiLbmStats,
  * FsiMpmParams, FsiMpmState, FsiCouplingParams, FsiCouplingCounters,
- * FsiMpmStats, FsiBatchInstance, FsiBins (in that order, up to n entries) so bindings can verify their
+ * FsiMpmStats, FsiBatchInstance, FsiBins, FsiBatchOp (in that order, up to n entries) so bindings can verify their
  * struct layouts.  Returns the number of entries written. */
 int fsi_abi_sizes(int32_t* out, int n);
 int fsi_sync(void* stream);

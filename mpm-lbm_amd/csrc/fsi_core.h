@@ -2457,4 +2457,14 @@ FSI_HD uint32_t batch_substep_flags(const FsiBatchInstance& I, int t) {
 
 FSI_HD int batch_cur(const FsiBatchInstance& I, int t) { return I.cur0 ^ (t & 1); }
 
+// Workgroups are handed to the 8 XCDs round-robin by dispatch order; the
+// logical tile of dispatch slot b of an n-workgroup launch is chosen so that
+// XCD x receives one contiguous run of tiles (and keeps their L2 lines).
+constexpr unsigned NXCD = 8;
+
+FSI_HD unsigned xcd_tile(unsigned b, unsigned n) {
+  const unsigned per = n / NXCD, rem = n % NXCD, x = b % NXCD;
+  return x * per + (x < rem ? x : rem) + b / NXCD;
+}
+
 }  // namespace fsi

@@ -40,14 +40,14 @@ const char* fsi_last_error(void) { return g_err.c_str(); }
 int fsi_version(void) { return 1; }
 
 int fsi_abi_sizes(int32_t* out, int n) {
-  const int32_t sz[11] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
+  const int32_t sz[12] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
                          (int32_t)sizeof(FsiLbmCounters),    (int32_t)sizeof(FsiLbmStats),
                          (int32_t)sizeof(FsiMpmParams),      (int32_t)sizeof(FsiMpmState),
                          (int32_t)sizeof(FsiCouplingParams), (int32_t)sizeof(FsiCouplingCounters),
                          (int32_t)sizeof(FsiMpmStats),       (int32_t)sizeof(FsiBatchInstance),
-                         (int32_t)sizeof(FsiBins)};
+                         (int32_t)sizeof(FsiBins),           (int32_t)sizeof(FsiBatchOp)};
   int k = 0;
-  for (; k < n && k < 11; ++k) out[k] = sz[k];
+  for (; k < n && k < 12; ++k) out[k] = sz[k];
   return k;
 }
 int fsi_sync(void*) { return FSI_OK; }
@@ -592,6 +592,63 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
         return fail(FSI_E_INVALID, "unknown batch op");
     }
   }
+  return FSI_OK;
+}
+
+// The batched exchange graph of the emulation: its launch list, run in order.
+namespace {
+struct EmuBatchGraph {
+  const FsiBatchInstance* inst;
+  int n_inst;
+  std::vector<FsiBatchOp> ops;
+};
+
+int check_ops(const FsiBatchOp* ops, int n_ops, const EmuBatchGraph* g) {
+  if (g && (size_t)n_ops != g->ops.size()) return fail(FSI_E_INVALID, "batch graph update with another number of operations");
+  for (int i = 0; i < n_ops; ++i) {
+    if (ops[i].op < FSI_BATCH_OP_LBM_SUBSTEP || ops[i].op > FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD)
+      return fail(FSI_E_INVALID, "unknown batch operation");
+    if (ops[i].op != FSI_BATCH_OP_COUNTERS_CLEAR && (!ops[i].block_map || ops[i].n_blocks <= 0))
+      return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");
+    if (g && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t))
+      return fail(FSI_E_INVALID, "batch graph update with other operations");
+  }
+  return FSI_OK;
+}
+}  // namespace
+
+int fsi_batch_graph_create(const FsiBatchOp* ops, int n_ops, const FsiBatchInstance* inst, int n_inst,
+                           void** graph_out) {
+  if (!graph_out) return fail(FSI_E_INVALID, "null graph handle");
+  *graph_out = nullptr;
+  if (!ops || n_ops <= 0 || !inst || n_inst <= 0) return fail(FSI_E_INVALID, "empty batch graph");
+  const int rc = check_ops(ops, n_ops, nullptr);
+  if (rc != FSI_OK) return rc;
+  *graph_out = new EmuBatchGraph{inst, n_inst, std::vector<FsiBatchOp>(ops, ops + n_ops)};
+  return FSI_OK;
+}
+
+int fsi_batch_graph_update(void* graph, const FsiBatchOp* ops, int n_ops) {
+  EmuBatchGraph* g = static_cast<EmuBatchGraph*>(graph);
+  if (!g || !ops) return fail(FSI_E_INVALID, "null batch graph");
+  const int rc = check_ops(ops, n_ops, g);
+  if (rc != FSI_OK) return rc;
+  g->ops.assign(ops, ops + n_ops);
+  return FSI_OK;
+}
+
+int fsi_batch_graph_launch(void* graph, void* stream) {
+  EmuBatchGraph* g = static_cast<EmuBatchGraph*>(graph);
+  if (!g) return fail(FSI_E_INVALID, "null batch graph");
+  for (const FsiBatchOp& o : g->ops) {
+    const int rc = fsi_batch_launch(o.op, g->inst, g->n_inst, o.block_map, o.n_blocks, o.t, stream);
+    if (rc != FSI_OK) return rc;
+  }
+  return FSI_OK;
+}
+
+int fsi_batch_graph_destroy(void* graph) {
+  delete static_cast<EmuBatchGraph*>(graph);
   return FSI_OK;
 }
 

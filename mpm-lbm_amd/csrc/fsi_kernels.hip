@@ -11,6 +11,7 @@
 #include <string.h>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "fsi_core.h"
 
@@ -507,17 +508,9 @@ __global__ void __launch_bounds__(TPB) k_lbm_rebuild(FsiLbmParams p, FsiLbmState
 
 // Substep launch geometry.  A workgroup covers TPB consecutive cells of one
 // i-plane (the (j,k) plane is contiguous in memory), so i, the x-neighbour
-// offsets and the x-face BC tests are uniform per workgroup.  Workgroups are
-// handed to the 8 XCDs round-robin by dispatch order; the logical tile of
-// dispatch slot b is chosen so that XCD x receives one contiguous run of
-// tiles, i.e. a slab of i-planes, and keeps that slab (and its L2 lines) in
-// every substep.
-constexpr unsigned NXCD = 8;
-
-__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
-  const unsigned per = n / NXCD, rem = n % NXCD, x = b % NXCD;
-  return x * per + (x < rem ? x : rem) + b / NXCD;
-}
+// offsets and the x-face BC tests are uniform per workgroup.  With xcd_tile
+// each XCD receives one contiguous run of tiles, i.e. a slab of i-planes, and
+// keeps that slab (and its L2 lines) in every substep.
 
 // MINW: the waves per SIMD the register budget must allow.  A grid that fits
 // on the chip in one wave generation at 4 waves per SIMD runs without a cap
@@ -1742,6 +1735,112 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_g2p(const FsiBatchInstance* inst,
 // C ABI
 // ---------------------------------------------------------------------------
 namespace {
+// Kernel of a batched operation: its entry and workgroup size.
+struct BatchKernel {
+  const void* fn;
+  int block;
+};
+
+int batch_kernel(int op, int t, BatchKernel& k) {
+  k.block = TPB;
+  switch (op) {
+    case FSI_BATCH_OP_COUNTERS_CLEAR: k.fn = (const void*)&kb_counters_clear; k.block = 64; break;
+    case FSI_BATCH_OP_LBM_SUBSTEP: k.fn = (const void*)&kb_substep<~0u>; break;
+    case FSI_BATCH_OP_LBM_SUBSTEP_MB:
+      k.fn = t == 0 ? (const void*)&kb_substep<FSI_SUB_MOVING_WALL | FSI_SUB_STORED_MOMENTS>
+                    : (const void*)&kb_substep<FSI_SUB_MOVING_WALL>;
+      break;
+    case FSI_BATCH_OP_LBM_SUBSTEP_PENALTY: k.fn = (const void*)&kb_substep<FSI_SUB_CELL_FORCE | FSI_SUB_STORED_MOMENTS>; break;
+    case FSI_BATCH_OP_LBM_SUBSTEP_NONE: k.fn = (const void*)&kb_substep<FSI_SUB_STORED_MOMENTS>; break;
+    case FSI_BATCH_OP_LBM_REBUILD: k.fn = (const void*)&kb_rebuild; break;
+    case FSI_BATCH_OP_LBM_MOMENTS: k.fn = (const void*)&kb_moments; break;
+    case FSI_BATCH_OP_LBM_MB_BEGIN: k.fn = (const void*)&kb_mb_begin; break;
+    case FSI_BATCH_OP_LBM_MB_FINALIZE: k.fn = (const void*)&kb_finalize; break;
+    case FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD: k.fn = (const void*)&kb_finalize_fold; break;
+    case FSI_BATCH_OP_PROJ_CLEAR: k.fn = (const void*)&kb_proj_clear; break;
+    case FSI_BATCH_OP_PROJ_PARTICLES: k.fn = (const void*)&kb_proj_particles; k.block = PTPB; break;
+    case FSI_BATCH_OP_PROJ_NORMALIZE: k.fn = (const void*)&kb_proj_normalize; break;
+    case FSI_BATCH_OP_PENALTY: k.fn = (const void*)&kb_penalty; break;
+    case FSI_BATCH_OP_MPM_CLEAR: k.fn = (const void*)&kb_mpm_clear; break;
+    case FSI_BATCH_OP_MPM_P2G: k.fn = (const void*)&kb_mpm_p2g; k.block = PTPB; break;
+    case FSI_BATCH_OP_MPM_REACTION: k.fn = (const void*)&kb_mpm_reaction; k.block = PTPB; break;
+    case FSI_BATCH_OP_MPM_GRID_UPDATE: k.fn = (const void*)&kb_mpm_grid_update; break;
+    case FSI_BATCH_OP_MPM_G2P: k.fn = (const void*)&kb_mpm_g2p; k.block = PTPB; break;
+    case FSI_BATCH_OP_MPM_P2G_REACT: k.fn = (const void*)&kb_mpm_p2g_react; k.block = PTPB; break;
+    case FSI_BATCH_OP_MPM_GRID_UPDATE_REACT: k.fn = (const void*)&kb_mpm_grid_update_react; break;
+    default: return fail(FSI_E_INVALID, "unknown batch operation");
+  }
+  return FSI_OK;
+}
+
+dim3 batch_grid(int op, int n_inst, int n_blocks) {
+  return op == FSI_BATCH_OP_COUNTERS_CLEAR ? dim3((unsigned)(n_inst + 63) / 64) : dim3((unsigned)n_blocks);
+}
+
+// Argument values of a batched kernel and the pointer array a launch takes:
+// (inst, map[, t]) or, for COUNTERS_CLEAR, (inst, n_inst, which).
+struct BatchArgs {
+  const FsiBatchInstance* inst;
+  const int32_t* map;
+  int n_inst, t;
+  void* ptrs[3];
+  BatchArgs(const FsiBatchInstance* i, int n, const int32_t* m, int t_) : inst(i), map(m), n_inst(n), t(t_) {
+    ptrs[0] = &inst;
+    ptrs[1] = &map;
+    ptrs[2] = &t;
+  }
+  void counters_layout() { ptrs[1] = &n_inst; }
+};
+
+struct BatchGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const FsiBatchInstance* inst = nullptr;
+  int n_inst = 0;
+  std::vector<hipGraphNode_t> nodes;
+  std::vector<FsiBatchOp> ops;
+};
+
+void destroy_graph(BatchGraph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
+
+// Kernel node parameters of op `o`; `a` holds the argument values (and must
+// outlive the call that takes `p`).
+void node_params(const FsiBatchOp& o, int n_inst, BatchArgs& a, hipKernelNodeParams& p) {
+  BatchKernel k;
+  batch_kernel(o.op, o.t, k);
+  if (o.op == FSI_BATCH_OP_COUNTERS_CLEAR) a.counters_layout();
+  p.func = const_cast<void*>(k.fn);
+  p.gridDim = batch_grid(o.op, n_inst, o.n_blocks);
+  p.blockDim = dim3(k.block);
+  p.sharedMemBytes = 0;
+  p.kernelParams = a.ptrs;
+  p.extra = nullptr;
+}
+
+// A launch list the graph can take: known operations with a block map and
+// workgroups; for an update, the same operations and substep indices as the
+// graph was built from.
+int check_graph_ops(const BatchGraph* g, const FsiBatchOp* ops, int n_ops, bool update) {
+  if (update && (size_t)n_ops != g->ops.size())
+    return fail(FSI_E_INVALID, "batch graph update with another number of operations");
+  for (int i = 0; i < n_ops; ++i) {
+    BatchKernel k;
+    if (batch_kernel(ops[i].op, ops[i].t, k) != FSI_OK) return FSI_E_INVALID;
+    if (ops[i].op != FSI_BATCH_OP_COUNTERS_CLEAR && (!ops[i].block_map || ops[i].n_blocks <= 0))
+      return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");
+    if (update && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t))
+      return fail(FSI_E_INVALID, "batch graph update with other operations");
+  }
+  return FSI_OK;
+}
+}  // namespace
+
+namespace {
 // Checks of a binning's buffers; the bin count must fit the int32 keys.
 int check_bins(const FsiBins* b, long long nbins) {
   if (!b || !b->key || !b->start || !b->order || !b->work) return fail(FSI_E_INVALID, "null binning buffers");
@@ -1811,14 +1910,14 @@ const char* fsi_last_error(void) { return g_last_error.c_str(); }
 int fsi_version(void) { return 1; }
 
 int fsi_abi_sizes(int32_t* out, int n) {
-  const int32_t sz[11] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
+  const int32_t sz[12] = {(int32_t)sizeof(FsiLbmParams),      (int32_t)sizeof(FsiLbmState),
                          (int32_t)sizeof(FsiLbmCounters),    (int32_t)sizeof(FsiLbmStats),
                          (int32_t)sizeof(FsiMpmParams),      (int32_t)sizeof(FsiMpmState),
                          (int32_t)sizeof(FsiCouplingParams), (int32_t)sizeof(FsiCouplingCounters),
                          (int32_t)sizeof(FsiMpmStats),       (int32_t)sizeof(FsiBatchInstance),
-                         (int32_t)sizeof(FsiBins)};
+                         (int32_t)sizeof(FsiBins),           (int32_t)sizeof(FsiBatchOp)};
   int k = 0;
-  for (; k < n && k < 11; ++k) out[k] = sz[k];
+  for (; k < n && k < 12; ++k) out[k] = sz[k];
   return k;
 }
 
@@ -2245,51 +2344,78 @@ int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domai
 int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const int32_t* block_map_dev, int n_blocks,
                      int t, void* stream) {
   if (!inst_dev || n_inst <= 0) return fail(FSI_E_INVALID, "empty batch");
-  hipStream_t s = as_stream(stream);
-  if (op == FSI_BATCH_OP_COUNTERS_CLEAR) {
-    hipLaunchKernelGGL(kb_counters_clear, dim3((n_inst + 63) / 64), dim3(64), 0, s, inst_dev, n_inst, t);
-    return check_launch("kb_counters_clear");
+  BatchKernel k;
+  if (batch_kernel(op, t, k) != FSI_OK) return FSI_E_INVALID;
+  if (op != FSI_BATCH_OP_COUNTERS_CLEAR) {
+    if (!block_map_dev) return fail(FSI_E_INVALID, "null block map");
+    if (n_blocks <= 0) return FSI_OK;
   }
-  if (!block_map_dev) return fail(FSI_E_INVALID, "null block map");
-  if (n_blocks <= 0) return FSI_OK;
-  const dim3 g(n_blocks), b(TPB), bp(PTPB);
-  const int32_t* m = block_map_dev;
-  switch (op) {
-    case FSI_BATCH_OP_LBM_SUBSTEP: hipLaunchKernelGGL(kb_substep<~0u>, g, b, 0, s, inst_dev, m, t); break;
-    case FSI_BATCH_OP_LBM_SUBSTEP_MB:
-      if (t == 0)
-        hipLaunchKernelGGL(kb_substep<FSI_SUB_MOVING_WALL | FSI_SUB_STORED_MOMENTS>, g, b, 0, s, inst_dev, m, t);
-      else
-        hipLaunchKernelGGL(kb_substep<FSI_SUB_MOVING_WALL>, g, b, 0, s, inst_dev, m, t);
-      break;
-    case FSI_BATCH_OP_LBM_SUBSTEP_PENALTY:
-      hipLaunchKernelGGL(kb_substep<FSI_SUB_CELL_FORCE | FSI_SUB_STORED_MOMENTS>, g, b, 0, s, inst_dev, m, t);
-      break;
-    case FSI_BATCH_OP_LBM_SUBSTEP_NONE:
-      hipLaunchKernelGGL(kb_substep<FSI_SUB_STORED_MOMENTS>, g, b, 0, s, inst_dev, m, t);
-      break;
-    case FSI_BATCH_OP_LBM_REBUILD: hipLaunchKernelGGL(kb_rebuild, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_LBM_MOMENTS: hipLaunchKernelGGL(kb_moments, g, b, 0, s, inst_dev, m, t); break;
-    case FSI_BATCH_OP_LBM_MB_BEGIN: hipLaunchKernelGGL(kb_mb_begin, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_LBM_MB_FINALIZE: hipLaunchKernelGGL(kb_finalize, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD: hipLaunchKernelGGL(kb_finalize_fold, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_PROJ_CLEAR: hipLaunchKernelGGL(kb_proj_clear, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_PROJ_PARTICLES: hipLaunchKernelGGL(kb_proj_particles, g, bp, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_PROJ_NORMALIZE: hipLaunchKernelGGL(kb_proj_normalize, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_PENALTY: hipLaunchKernelGGL(kb_penalty, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_MPM_CLEAR: hipLaunchKernelGGL(kb_mpm_clear, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_MPM_P2G: hipLaunchKernelGGL(kb_mpm_p2g, g, bp, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_MPM_REACTION: hipLaunchKernelGGL(kb_mpm_reaction, g, bp, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_MPM_GRID_UPDATE: hipLaunchKernelGGL(kb_mpm_grid_update, g, b, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_MPM_G2P: hipLaunchKernelGGL(kb_mpm_g2p, g, bp, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_MPM_P2G_REACT: hipLaunchKernelGGL(kb_mpm_p2g_react, g, bp, 0, s, inst_dev, m); break;
-    case FSI_BATCH_OP_MPM_GRID_UPDATE_REACT:
-      hipLaunchKernelGGL(kb_mpm_grid_update_react, g, b, 0, s, inst_dev, m);
-      break;
-    default:
-      return fail(FSI_E_INVALID, "unknown batch operation");
-  }
+  BatchArgs a(inst_dev, n_inst, block_map_dev, t);
+  const hipError_t e = hipLaunchKernel(k.fn, batch_grid(op, n_inst, n_blocks), dim3(k.block), a.ptrs, 0,
+                                       as_stream(stream));
+  if (e != hipSuccess) return fail(FSI_E_HIP, hipGetErrorString(e));
   return check_launch("fsi_batch_launch");
+}
+
+int fsi_batch_graph_create(const FsiBatchOp* ops, int n_ops, const FsiBatchInstance* inst_dev, int n_inst,
+                           void** graph_out) {
+  if (!graph_out) return fail(FSI_E_INVALID, "null graph handle");
+  *graph_out = nullptr;
+  if (!ops || n_ops <= 0 || !inst_dev || n_inst <= 0) return fail(FSI_E_INVALID, "empty batch graph");
+  BatchGraph* g = new BatchGraph();
+  g->inst = inst_dev;
+  g->n_inst = n_inst;
+  g->ops.assign(ops, ops + n_ops);
+  int rc = check_graph_ops(g, ops, n_ops, false);
+  hipError_t e = hipSuccess;
+  if (rc == FSI_OK) e = hipGraphCreate(&g->graph, 0);
+  for (int i = 0; rc == FSI_OK && e == hipSuccess && i < n_ops; ++i) {
+    hipKernelNodeParams p;
+    BatchArgs a(inst_dev, n_inst, ops[i].block_map, ops[i].t);
+    node_params(ops[i], n_inst, a, p);
+    hipGraphNode_t node;
+    e = hipGraphAddKernelNode(&node, g->graph, i ? &g->nodes.back() : nullptr, i ? 1 : 0, &p);
+    if (e == hipSuccess) g->nodes.push_back(node);
+  }
+  if (rc == FSI_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (rc == FSI_OK && e != hipSuccess) rc = fail(FSI_E_HIP, hipGetErrorString(e));
+  if (rc != FSI_OK) {
+    destroy_graph(g);
+    return rc;
+  }
+  *graph_out = g;
+  return FSI_OK;
+}
+
+int fsi_batch_graph_update(void* graph, const FsiBatchOp* ops, int n_ops) {
+  BatchGraph* g = static_cast<BatchGraph*>(graph);
+  if (!g || !ops) return fail(FSI_E_INVALID, "null batch graph");
+  const int rc = check_graph_ops(g, ops, n_ops, true);
+  if (rc != FSI_OK) return rc;
+  for (int i = 0; i < n_ops; ++i) {
+    const FsiBatchOp& o = g->ops[(size_t)i];
+    if (o.block_map == ops[i].block_map && o.n_blocks == ops[i].n_blocks) continue;
+    hipKernelNodeParams p;
+    BatchArgs a(g->inst, g->n_inst, ops[i].block_map, ops[i].t);
+    node_params(ops[i], g->n_inst, a, p);
+    const hipError_t e = hipGraphExecKernelNodeSetParams(g->exec, g->nodes[(size_t)i], &p);
+    if (e != hipSuccess) return fail(FSI_E_HIP, hipGetErrorString(e));
+    g->ops[(size_t)i] = ops[i];
+  }
+  return FSI_OK;
+}
+
+int fsi_batch_graph_launch(void* graph, void* stream) {
+  BatchGraph* g = static_cast<BatchGraph*>(graph);
+  if (!g) return fail(FSI_E_INVALID, "null batch graph");
+  const hipError_t e = hipGraphLaunch(g->exec, as_stream(stream));
+  if (e != hipSuccess) return fail(FSI_E_HIP, hipGetErrorString(e));
+  return FSI_OK;
+}
+
+int fsi_batch_graph_destroy(void* graph) {
+  destroy_graph(static_cast<BatchGraph*>(graph));
+  return FSI_OK;
 }
 
 }  // extern "C"

@@ -218,8 +218,13 @@ class Bins(ctypes.Structure):
     _fields_ = [("key", vp), ("start", vp), ("order", vp), ("work", vp)]
 
 
+class BatchOp(ctypes.Structure):
+    """FsiBatchOp: one operation of a batched exchange graph."""
+    _fields_ = [("op", i32), ("t", i32), ("block_map", vp), ("n_blocks", i32), ("pad0", i32)]
+
+
 ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, CouplingParams,
-               CouplingCounters, MpmStats, BatchInstance, Bins]
+               CouplingCounters, MpmStats, BatchInstance, Bins, BatchOp]
 
 EXPORTED_SYMBOLS = [
     "fsi_last_error", "fsi_version", "fsi_abi_sizes", "fsi_sync",
@@ -230,6 +235,7 @@ EXPORTED_SYMBOLS = [
     "fsi_mpm_reduce_stats",
     "fsi_bin_particles", "fsi_mpm_substep_ordered", "fsi_project_ordered",
     "fsi_batch_block_map", "fsi_batch_launch",
+    "fsi_batch_graph_create", "fsi_batch_graph_update", "fsi_batch_graph_launch", "fsi_batch_graph_destroy",
 ]
 
 PARTICLE_BLOCK = 64  # FSI_PARTICLE_BLOCK
@@ -305,6 +311,10 @@ class Native:
         L.fsi_project_ordered.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState), P(CouplingParams),
                                           vp, P(Bins), vp]
         L.fsi_batch_block_map.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.fsi_batch_graph_create.argtypes = [P(BatchOp), ctypes.c_int, vp, ctypes.c_int, P(vp)]
+        L.fsi_batch_graph_update.argtypes = [vp, P(BatchOp), ctypes.c_int]
+        L.fsi_batch_graph_launch.argtypes = [vp, vp]
+        L.fsi_batch_graph_destroy.argtypes = [vp]
         L.fsi_batch_launch.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp]
         for name in EXPORTED_SYMBOLS:
             getattr(L, name).restype = ctypes.c_int if name != "fsi_last_error" else ctypes.c_char_p

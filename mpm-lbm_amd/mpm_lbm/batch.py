@@ -4,11 +4,22 @@ Each instance is a full FSIDriver3D (its own device buffers, geometry,
 particles and coupling constants).  `BatchedFSI` gathers their C-ABI
 descriptors into one device table and advances all of them with one launch
 per operation (`fsi_batch_launch`): every workgroup finds its instance in a
-block-prefix table, so instances of different sizes and modes share each
-launch.  One batched exchange - 120 LBM substeps for moving-boundary
-instances, 120 consecutive one-LBM-step FSI steps for penalty / none
-instances (the reference rejects penalty with subcycling,
-fsi_config.py:342-343) - is captured once as a HIP graph and replayed.
+block map, so instances of different sizes and modes share each launch.  One
+batched exchange - 120 LBM substeps for moving-boundary instances, 120
+consecutive one-LBM-step FSI steps for penalty / none instances (the
+reference rejects penalty with subcycling, fsi_config.py:342-343) - is
+captured once as a HIP graph and replayed.
+
+The table is a fixed array of slots.  A fixed batch fills every slot once; a
+slot table (`BatchedFSI.slots`, the serving form's) admits and releases
+instances in place: the instance table lives at a fixed device address and is
+rewritten between exchanges, and the block maps are rebuilt for the live
+instances.  The exchange graph is built from its launch list in the native
+library (fsi_batch_graph_create, no stream capture), and a change of the
+instance set rewrites the block maps and workgroup counts of its kernel nodes
+in place (fsi_batch_graph_update) as long as the launch plan - which
+operations run, over which instance groups - is the same; one graph is kept
+per launch plan.
 
 Results are identical to running each instance alone: the kernels and their
 per-instance arguments are the same, only the launch is shared.
@@ -34,49 +45,75 @@ SUBSTEP_PLANS = ("split", "mixed")
 _TUNED = {}
 _MODE = {"none": _native.BATCH_MODE_NONE, "penalty": _native.BATCH_MODE_PENALTY,
          "moving_boundary": _native.BATCH_MODE_MOVING_BOUNDARY}
+MB = (_native.BATCH_MODE_MOVING_BOUNDARY,)
+ONE = (_native.BATCH_MODE_PENALTY, _native.BATCH_MODE_NONE)
+PEN = (_native.BATCH_MODE_PENALTY,)
+
+def _full_device(device):
+    """`device` with its index (cuda -> cuda:<current>); None -> the GPU if there is one."""
+    if device is None:
+        device = "cuda" if torch.cuda.is_available() else "cpu"
+    device = torch.device(device)
+    if device.type == "cuda" and device.index is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    return device
+
+
+def check_batchable(d):
+    """Raise unless driver `d` can run in a batched exchange."""
+    if d.config.coupling_mode == "moving_boundary" and d.config.reaction_transfer_mode != "engineering":
+        raise NotImplementedError("the batched form runs the engineering reaction transfer only "
+                                  f"(reaction_transfer_mode={d.config.reaction_transfer_mode!r})")
+    if d.config.lbm_open_boundary_semantics != "equilibrium_all_population_reset":
+        raise NotImplementedError("the batched form runs the legacy x-face reset only "
+                                  f"(lbm_open_boundary_semantics={d.config.lbm_open_boundary_semantics!r})")
+    if d.config.coupling_mode == "moving_boundary" and d.config.fsi_exchange_mode != "lbm_subcycled_per_fsi_step":
+        raise ValueError("moving-boundary instances of a batch must be subcycled")
 
 
 class BatchedFSI:
     def __init__(self, workloads=None, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True,
-                 drivers=None, n_substeps=None, substep_plan=None):
+                 drivers=None, n_substeps=None, substep_plan=None, capacity=None):
         """Batch `workloads` (a driver is built for each), or batch initialized
-        `drivers` that already exist (they keep their state; a serving loop
-        regroups its live instances this way).  n_substeps: LBM substeps per
-        exchange (default: the moving-boundary instances' FSI step, else 120);
-        moving-boundary instances must use it as their FSI step.
-        substep_plan: how each LBM substep is launched, "split" (one launch
-        per coupling mode, the mode's flags compiled in) or "mixed" (one
-        launch over every instance, flags read per instance); None tunes it
-        on the first exchange (`SUBSTEP_PLANS`)."""
+        `drivers` that already exist (they keep their state).  n_substeps: LBM
+        substeps per exchange (default: the moving-boundary instances' FSI
+        step, else 120); moving-boundary instances must use it as their FSI
+        step.  substep_plan: how each LBM substep is launched, "split" (one
+        launch per coupling mode, the mode's flags compiled in) or "mixed"
+        (one launch over every instance, flags read per instance); None tunes
+        it on the first exchange (`SUBSTEP_PLANS`).  capacity: slots of the
+        table (default: exactly the instances given; more leaves free slots
+        for `admit`).  use_graph: run each exchange as a native graph (the
+        tuning exchange runs eagerly)."""
         if substep_plan is not None and substep_plan not in SUBSTEP_PLANS:
             raise ValueError(f"substep_plan must be one of {SUBSTEP_PLANS} or None")
         self.substep_plan = substep_plan
         self.tuning = None
         if drivers is not None:
             self.workloads = None
-            self.drivers = list(drivers)
+            given = list(drivers)
         else:
-            if not workloads:
-                raise ValueError("BatchedFSI needs at least one workload")
-            self.workloads = list(workloads)
-            self.drivers = []
+            self.workloads = list(workloads or [])
+            given = []
             for k, w in enumerate(self.workloads):
                 d = FSIDriver3D(w.config, f"{out_dir}/inst{k:03d}", particle_cloud=w.cloud, native=native,
                                 device=device, write_geometry_file=False, use_graph=False)
                 d.initialize()
-                self.drivers.append(d)
-        if not self.drivers:
+                given.append(d)
+        if capacity is None and not given:
             raise ValueError("BatchedFSI needs at least one instance")
-        for d in self.drivers:
-            if d.config.coupling_mode == "moving_boundary" and d.config.reaction_transfer_mode != "engineering":
-                raise NotImplementedError("the batched form runs the engineering reaction transfer only "
-                                          f"(reaction_transfer_mode={d.config.reaction_transfer_mode!r})")
-            if d.config.lbm_open_boundary_semantics != "equilibrium_all_population_reset":
-                raise NotImplementedError("the batched form runs the legacy x-face reset only "
-                                          f"(lbm_open_boundary_semantics={d.config.lbm_open_boundary_semantics!r})")
-        self.native = self.drivers[0].lbm.native
-        self.device = self.drivers[0].lbm.device
-        mb = [d for d in self.drivers if d.config.coupling_mode == "moving_boundary"]
+        self.capacity = int(capacity) if capacity is not None else len(given)
+        if self.capacity < max(1, len(given)):
+            raise ValueError("capacity below the number of instances given")
+        for d in given:
+            check_batchable(d)
+        if given:
+            native, device = given[0].lbm.native, given[0].lbm.device
+        if native is None:
+            native = _native.load_hip()
+        self.native = native
+        self.device = _full_device(device)
+        mb = [d for d in given if d.config.coupling_mode == "moving_boundary"]
         subs = {d.config.lbm_substeps_per_fsi_step for d in mb}
         if n_substeps is not None:
             subs.add(int(n_substeps))
@@ -85,79 +122,157 @@ class BatchedFSI:
         self.n_substeps = subs.pop() if subs else 120
         if self.n_substeps % 2:
             raise ValueError("the batched exchange needs an even number of LBM substeps")
-        for d in self.drivers:
-            if d.config.coupling_mode == "moving_boundary" and d.config.fsi_exchange_mode != "lbm_subcycled_per_fsi_step":
-                raise ValueError("moving-boundary instances of a batch must be subcycled")
-        self._counters = [struct_buffer(_native.CouplingCounters, self.device) for _ in self.drivers]
-        self._react_ws = [torch.zeros(max(1, _native.coupled_workspace_doubles(d.solid.n_particles)),
-                                      dtype=torch.float64, device=self.device) for d in self.drivers]
-        self._prepare_instances()
-        self.use_graph = use_graph and self.device.type == "cuda"
-        self._graph = None
+        self.drivers = [None] * self.capacity
+        self._counters = [None] * self.capacity
+        self._react_ws = [None] * self.capacity
+        self._inst_host = (_native.BatchInstance * self.capacity)()
+        nbytes = ctypes.sizeof(self._inst_host)
+        self._inst_dev = torch.zeros(nbytes, dtype=torch.uint8, device=self.device)
+        self._maps = {}  # key -> [device map, live workgroups]
+        self._graphs = {}  # launch plan -> [native graph handle, its launch list]
+        self._plan = None
+        self._dirty = True
+        self.graph_builds = 0  # exchanges run by a newly built graph
+        self.graph_updates = 0  # graphs rewritten in place for a new instance set
+        self.replays = 0  # exchanges run by a graph built earlier (updated or not)
+        for d in given:
+            self._fill_slot(self.drivers.index(None), d)
+        self.use_graph = bool(use_graph)
+        self._recording = None
+        self._plan_for = None
         self.exchanges = 0
 
-    # ----------------------------------------------------------------- set-up
-    def _prepare_instances(self):
-        inst = (_native.BatchInstance * len(self.drivers))()
-        for k, d in enumerate(self.drivers):
-            lbm = d.lbm
-            lbm._ensure_ready()  # static link masks for penalty / none instances
-            if d.config.coupling_mode == "moving_boundary":
-                lbm._ensure_link_buffers()
-                lbm.update_dynamic_solid(d.config.dynamic_solid_threshold)
-            lbm._sync_moments()
-            I = inst[k]
-            I.lp = lbm._params
-            I.ls = lbm._state
-            I.mp = d.solid._params
-            I.ms = d.solid._state
-            coupler = d.mb_coupler or d.penalty_coupler or d.projector
-            I.cp = coupler.params
-            I.cc = _native.ptr(self._counters[k])
-            I.mode = _MODE[d.config.coupling_mode]
-            I.cur0 = lbm.cur
-            I.n_substeps = self.n_substeps if d.config.coupling_mode == "moving_boundary" else 1
-            I.mpm_substeps = d.config.mpm_substeps_per_lbm_step
-            I.active = 1
-            I.cell_force = 1 if d.config.coupling_mode == "penalty" else 0
-            I.react_ws = _native.ptr(self._react_ws[k])
-            lbm._cell_force_active = bool(I.cell_force)
-        self._inst_host = inst
-        raw = bytes(memoryview(inst).cast("B"))
-        self._inst_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
-        self._tables = {}
+    @classmethod
+    def slots(cls, capacity, native=None, device=None, n_substeps=120, substep_plan="split", use_graph=True):
+        """An empty slot table of `capacity` instances (the serving form)."""
+        return cls(drivers=[], native=native, device=device, n_substeps=n_substeps, substep_plan=substep_plan,
+                   use_graph=use_graph, capacity=capacity)
+
+    # ----------------------------------------------------------------- slots
+    def _fill_slot(self, k, d):
+        lbm = d.lbm
+        lbm._ensure_ready()  # static link masks for penalty / none instances
+        if d.config.coupling_mode == "moving_boundary":
+            lbm._ensure_link_buffers()
+            lbm.update_dynamic_solid(d.config.dynamic_solid_threshold)
+        lbm._sync_moments()
+        self._counters[k] = struct_buffer(_native.CouplingCounters, self.device)
+        self._react_ws[k] = torch.zeros(max(1, _native.coupled_workspace_doubles(d.solid.n_particles)),
+                                        dtype=torch.float64, device=self.device)
+        I = self._inst_host[k]
+        I.lp = lbm._params
+        I.ls = lbm._state
+        I.mp = d.solid._params
+        I.ms = d.solid._state
+        coupler = d.mb_coupler or d.penalty_coupler or d.projector
+        I.cp = coupler.params
+        I.cc = _native.ptr(self._counters[k])
+        I.mode = _MODE[d.config.coupling_mode]
+        I.cur0 = lbm.cur
+        I.n_substeps = self.n_substeps if d.config.coupling_mode == "moving_boundary" else 1
+        I.mpm_substeps = d.config.mpm_substeps_per_lbm_step
+        I.active = 1
+        I.cell_force = 1 if d.config.coupling_mode == "penalty" else 0
+        I.react_ws = _native.ptr(self._react_ws[k])
+        lbm._cell_force_active = bool(I.cell_force)
+        self.drivers[k] = d
+        self._dirty = True
+
+    def admit(self, d) -> int:
+        """Put initialized driver `d` into a free slot; returns the slot.  It
+        runs from the next exchange on; the other instances are untouched."""
+        check_batchable(d)
+        if _full_device(d.lbm.device) != self.device:
+            raise ValueError("the instance lives on another device than the batch")
+        if d.config.coupling_mode == "moving_boundary" and d.config.lbm_substeps_per_fsi_step != self.n_substeps:
+            raise ValueError(f"moving-boundary FSI step of {d.config.lbm_substeps_per_fsi_step} substeps, "
+                             f"the batch runs exchanges of {self.n_substeps}")
+        try:
+            k = self.drivers.index(None)
+        except ValueError:
+            raise RuntimeError(f"all {self.capacity} slots are taken") from None
+        self._fill_slot(k, d)
+        return k
+
+    def release(self, k):
+        """Free slot k (its instance leaves the batch; the driver keeps its state)."""
+        if self.drivers[k] is None:
+            raise ValueError(f"slot {k} is free")
+        self._inst_host[k] = _native.BatchInstance()
+        self.drivers[k] = None
+        self._counters[k] = None
+        self._react_ws[k] = None
+        self._dirty = True
+
+    @property
+    def live_slots(self):
+        return [k for k, d in enumerate(self.drivers) if d is not None]
+
+    def _refresh(self):
+        """After admissions / releases: the instance table rewritten in place,
+        every known block map rebuilt for the live instances, and the launch
+        plan recorded again on the next exchange."""
+        if self.device.type == "cuda":
+            torch.cuda.synchronize(self.device)  # the last exchange has read the old tables
+        raw = torch.frombuffer(bytearray(memoryview(self._inst_host).cast("B")), dtype=torch.uint8)
+        self._inst_dev.copy_(raw)
+        for key in list(self._maps):
+            self._write_map(key)
+        self._plan = None
+        self._dirty = False
+
+    def _write_map(self, key):
+        """Block map of `key` for the live instances, into its device buffer
+        (grown when too small: the graphs take the new address on update)."""
+        domain, modes, min_mpm = key
+        mask = sum(1 << m for m in modes)
+        addr = ctypes.addressof(self._inst_host)
+        lib = self.native.lib
+        n = lib.fsi_batch_block_map(addr, self.capacity, domain, mask, min_mpm, None)
+        if n < 0:
+            raise _native.FsiError(lib.fsi_last_error().decode())
+        entry = self._maps.get(key)
+        if entry is None or entry[0].shape[0] < n:
+            entry = [torch.empty((max(n, 1), 2), dtype=torch.int32, device=self.device), 0]
+            self._maps[key] = entry
+        if n:
+            host = np.empty((n, 2), dtype=np.int32)
+            lib.fsi_batch_block_map(addr, self.capacity, domain, mask, min_mpm, host.ctypes.data_as(ctypes.c_void_p))
+            entry[0][:n].copy_(torch.from_numpy(host))
+        entry[1] = n
 
     def _block_map(self, domain, modes, min_mpm=0):
-        """Device block map (instance, workgroup) of `domain` over the instances
-        of `modes` with more than min_mpm MPM substeps (fsi_batch_block_map)."""
+        """(device map, workgroups) of `domain` over the live instances of
+        `modes` with more than min_mpm MPM substeps."""
         key = (domain, tuple(sorted(modes)), min_mpm)
-        if key not in self._tables:
-            mask = sum(1 << m for m in modes)
-            addr = ctypes.addressof(self._inst_host)
-            n = self.native.lib.fsi_batch_block_map(addr, len(self.drivers), domain, mask, min_mpm, None)
-            if n < 0:
-                raise _native.FsiError(self.native.lib.fsi_last_error().decode())
-            host = np.zeros((max(n, 1), 2), dtype=np.int32)
-            if n:
-                self.native.lib.fsi_batch_block_map(addr, len(self.drivers), domain, mask, min_mpm,
-                                                    host.ctypes.data_as(ctypes.c_void_p))
-            self._tables[key] = (torch.from_numpy(host).to(self.device), n)
-        return self._tables[key]
+        if key not in self._maps:
+            self._write_map(key)
+        return self._maps[key]
 
+    # ------------------------------------------------------------ launches
     def _launch(self, op, modes, t=0, min_mpm=0):
-        table, nblocks = self._block_map(_native.BATCH_OP_DOMAIN[op], modes, min_mpm)
-        if nblocks == 0:
+        table, live = self._block_map(_native.BATCH_OP_DOMAIN[op], modes, min_mpm)
+        if live == 0:
             return
-        self.native.call("fsi_batch_launch", op, _native.ptr(self._inst_dev), len(self.drivers), _native.ptr(table),
-                         nblocks, t, stream_handle(self.device))
+        if self._recording is not None:
+            self._recording.append((op, t, (_native.BATCH_OP_DOMAIN[op], tuple(sorted(modes)), min_mpm)))
+            return
+        self.native.call("fsi_batch_launch", op, _native.ptr(self._inst_dev), self.capacity, _native.ptr(table),
+                         live, t, stream_handle(self.device))
 
     def _clear_counters(self, which):
+        if self._recording is not None:
+            self._recording.append((_native.BATCH_OP_COUNTERS_CLEAR, which, None))
+            return
         self.native.call("fsi_batch_launch", _native.BATCH_OP_COUNTERS_CLEAR, _native.ptr(self._inst_dev),
-                         len(self.drivers), None, 0, which, stream_handle(self.device))
+                         self.capacity, None, 0, which, stream_handle(self.device))
+
+    def _live_inst(self):
+        return [self._inst_host[k] for k in self.live_slots]
 
     def _mpm_substeps(self, modes):
         """MPM substeps with each instance's reaction (fsi_mpm_substep_coupled per instance)."""
-        max_m = max(I.mpm_substeps for I in self._inst_host if I.mode in modes)
+        max_m = max(I.mpm_substeps for I in self._live_inst() if I.mode in modes)
         for j in range(max_m):
             self._launch(_native.BATCH_OP_MPM_CLEAR, modes, min_mpm=j)
             self._launch(_native.BATCH_OP_MPM_P2G_REACT, modes, min_mpm=j)
@@ -172,11 +287,9 @@ class BatchedFSI:
 
     def _issue_exchange(self):
         """One batched exchange: n substeps of every instance (see the module docstring)."""
-        MB = (_native.BATCH_MODE_MOVING_BOUNDARY,)
-        ONE = (_native.BATCH_MODE_PENALTY, _native.BATCH_MODE_NONE)
-        PEN = (_native.BATCH_MODE_PENALTY,)
-        has_mb = any(I.mode in MB for I in self._inst_host)
-        has_one = any(I.mode in ONE for I in self._inst_host)
+        live = self._live_inst()
+        has_mb = any(I.mode in MB for I in live)
+        has_one = any(I.mode in ONE for I in live)
         if has_mb:
             self._project(MB)
             self._launch(_native.BATCH_OP_LBM_MB_BEGIN, MB)
@@ -203,7 +316,7 @@ class BatchedFSI:
             # tuning exchange: the plans take turns (t = 0 runs the moving-boundary
             # instances' first-substep variant and is not timed)
             plan = SUBSTEP_PLANS[t % len(SUBSTEP_PLANS)]
-            if t > 0:
+            if t > 0 and self._recording is None:
                 timer = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 timer[0].record()
         if plan == "mixed":
@@ -216,12 +329,69 @@ class BatchedFSI:
             timer[1].record()
             self._timers.append((plan, timer))
 
+    def _record_plan(self):
+        """The launch plan of one exchange - (operation, substep index, block
+        map key) per launch, for the live instances - from a dry run that also
+        builds every block map it needs."""
+        self._recording = []
+        try:
+            self._issue_exchange()
+            return tuple(self._recording)
+        finally:
+            self._recording = None
+
+    def _launch_list(self):
+        """The plan's launches with the current block maps (FsiBatchOp array)."""
+        ops = (_native.BatchOp * len(self._plan))()
+        for o, (op, t, key) in zip(ops, self._plan):
+            o.op, o.t = op, t
+            if key is not None:
+                table, live = self._maps[key]
+                o.block_map, o.n_blocks = _native.ptr(table), live
+        return ops
+
+    def _run_graph(self):
+        """The exchange as a native graph: built once per launch plan, its
+        block maps and workgroup counts rewritten in place when they changed."""
+        ops = self._launch_list()
+        raw = bytes(ops)
+        entry = self._graphs.get(self._plan)
+        if entry is None:
+            handle = ctypes.c_void_p()
+            self.native.call("fsi_batch_graph_create", ops, len(ops), _native.ptr(self._inst_dev), self.capacity,
+                             ctypes.byref(handle))
+            entry = [handle, raw]
+            self._graphs[self._plan] = entry
+            self.graph_builds += 1
+        else:
+            if entry[1] != raw:
+                if self.device.type == "cuda":
+                    torch.cuda.synchronize(self.device)  # no launch of the graph in flight while it changes
+                self.native.call("fsi_batch_graph_update", entry[0], ops, len(ops))
+                entry[1] = raw
+                self.graph_updates += 1
+            self.replays += 1
+        self.native.call("fsi_batch_graph_launch", entry[0], stream_handle(self.device))
+
+    def close(self):
+        """Release the native graphs (also done when the batch is collected)."""
+        for handle, _ in self._graphs.values():
+            self.native.lib.fsi_batch_graph_destroy(handle)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --------------------------------------------------------------- tuning
     def _signature(self):
         """What the substep timings depend on: per instance its lattice, mode,
         storage, collision, particle count and MPM substeps, and the exchange length."""
         return (self.n_substeps,) + tuple(
             (I.lp.nx, I.lp.ny, I.lp.nz, int(I.mode), int(I.lp.storage), int(I.lp.collision), int(I.mp.n_particles),
-             int(I.mpm_substeps)) for I in self._inst_host)
+             int(I.mpm_substeps)) for I in self._live_inst())
 
     def _tune_begin(self):
         """Before an exchange: take a plan chosen earlier for the same batch, or
@@ -252,20 +422,25 @@ class BatchedFSI:
                        "us_per_substep": {p: round(v, 2) for p, v in mean.items()},
                        "samples": {p: len(v) for p, v in per.items()}}
 
+    def _ensure_plan(self):
+        if self.substep_plan is None and self.device.type != "cuda":
+            self.substep_plan = "split"  # nothing to time on the CPU
+        if self._plan is None or self._plan_for != self.substep_plan:
+            self._plan = self._record_plan()
+            self._plan_for = self.substep_plan
+
     # ------------------------------------------------------------------- run
     def step(self):
         """Advance every instance by one batched exchange."""
-        if self.use_graph and self.exchanges > 0:
-            # the first exchange runs eagerly: it builds the block tables
-            # (host-to-device copies) that the captured launches then read
-            if self._graph is None:
-                torch.cuda.synchronize(self.device)
-                g = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(g):
-                    self._issue_exchange()
-                self._graph = g
-            self._graph.replay()
+        if self._dirty:
+            self._refresh()
+        if not self.live_slots:
+            return
+        if self.use_graph and self.substep_plan is not None:
+            self._ensure_plan()
+            self._run_graph()
         else:
+            # eager: use_graph=False, or the exchange that tunes the substep plan
             self._tune_begin()
             self._issue_exchange()
             self._tune_end()
@@ -274,7 +449,8 @@ class BatchedFSI:
 
     def _sync_host_state(self):
         """Host bookkeeping of each driver after a batched exchange."""
-        for d, I in zip(self.drivers, self._inst_host):
+        for k in self.live_slots:
+            d, I = self.drivers[k], self._inst_host[k]
             lbm = d.lbm
             lbm.cur = I.cur0
             lbm._moments_fresh = True
@@ -293,7 +469,7 @@ class BatchedFSI:
 
     @property
     def total_cells(self):
-        return sum(I.lp.nx * I.lp.ny * I.lp.nz for I in self._inst_host)
+        return sum(I.lp.nx * I.lp.ny * I.lp.nz for I in self._live_inst())
 
     def cell_updates_per_exchange(self):
         return self.total_cells * self.n_substeps

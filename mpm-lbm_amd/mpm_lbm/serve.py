@@ -4,18 +4,25 @@ A request is an FSI case (a Workload: moving-boundary or penalty flap of any
 lattice size) plus a number of batched exchanges to run; one exchange is one
 FSI step of `n_substeps` LBM substeps for a moving-boundary case and
 `n_substeps` one-LBM-step FSI steps for a penalty case (the reference's only
-penalty schedule).  The server loop, on one device stream:
+penalty schedule).  All live requests sit in one fixed-capacity slot table
+(BatchedFSI.slots) whose instance table and block maps are rewritten in place,
+so a change of the live set updates the exchange graph instead of building a
+new batch.  The server loop, on one device stream:
 
   1. takes in the requests that have arrived;
-  2. admits queued requests first-in first-out while the estimated HBM
-     footprint of the admitted set stays within the budget (a request is
-     built - geometry, particles, device state - when it is admitted);
-  3. regroups the live instances into a BatchedFSI when the set changed;
-  4. advances every live instance by one batched exchange;
+  2. admits queued requests first-in first-out into free slots while the
+     estimated HBM footprint of the admitted set stays within the budget;
+  3. launches one batched exchange of every live request: a replay of the
+     exchange graph of the current launch plan, its block maps rewritten in
+     place when the live set changed (a new graph only for a new plan);
+  4. while the device runs it, builds the next queued requests (geometry,
+     particles, device state: at most `prebuild` built ahead of admission),
+     so admission itself only fills a slot;
   5. retires the requests that have run all their exchanges: their results
      go to the host (the caller's `on_done` callback, or the final diagnostics
-     row in `Request.result`) and their device state is released, so the
-     admission budget bounds what is resident.
+     row in `Request.result`), their slot is freed and their device state
+     released, so the admission budget bounds what is resident (plus the
+     requests built ahead).
 
 A request's results are those of running it alone: batching shares launches,
 not state.  Latency is completion time minus arrival time.  No reference
@@ -77,22 +84,47 @@ class Request:
 
 class FSIServer:
     def __init__(self, budget_bytes: float = 200e9, native=None, device=None, out_dir="/tmp/fsi_serve",
-                 use_graph: bool = True, n_substeps: int = 120):
+                 use_graph: bool = True, n_substeps: int = 120, capacity: int = 256, prebuild: int = 8,
+                 substep_plan: str = "split"):
         self.budget = float(budget_bytes)
         self.n_substeps = int(n_substeps)  # LBM substeps per exchange, whatever the mix
         self.native = native
         self.device = device
         self.out_dir = out_dir
         self.use_graph = use_graph
+        self.capacity = int(capacity)  # slots of the table: requests live at once
+        self.prebuild = int(prebuild)  # requests built ahead of their admission
+        self.substep_plan = substep_plan
+        self.table = None
         self.exchanges_run = 0
-        self.regroups = 0
+        self.regroups = 0  # exchanges whose live set differs from the previous exchange's
+        self.build_seconds = 0.0  # host time spent building requests
+        self.build_inline = 0  # requests built at admission (none ready ahead)
 
     def _build(self, r: Request):
+        t0 = time.perf_counter()
         w = r.workload
         d = FSIDriver3D(w.config, f"{self.out_dir}/req{r.rid:05d}", particle_cloud=w.cloud, native=self.native,
                         device=self.device, write_geometry_file=False, use_graph=False)
         d.initialize()
+        self.build_seconds += time.perf_counter() - t0
         return d
+
+    def _table(self, d):
+        if self.table is None:
+            self.table = BatchedFSI.slots(self.capacity, native=d.lbm.native, device=d.lbm.device,
+                                          n_substeps=self.n_substeps, substep_plan=self.substep_plan,
+                                          use_graph=self.use_graph)
+        return self.table
+
+    def _check(self, r: Request):
+        c = r.workload.config
+        if c.coupling_mode == "moving_boundary" and c.lbm_substeps_per_fsi_step != self.n_substeps:
+            raise ValueError(f"request {r.rid}: moving-boundary FSI step of {c.lbm_substeps_per_fsi_step} "
+                             f"substeps, the server runs exchanges of {self.n_substeps}")
+        r.bytes = estimate_bytes(c)
+        if r.bytes > self.budget:
+            raise ValueError(f"request {r.rid} needs {r.bytes} bytes, more than the whole budget")
 
     def run(self, requests: List[Request], sim_exchange_seconds: Optional[float] = None, sync=None, on_done=None):
         """Serve `requests` (arrival times in seconds from the start) to completion.
@@ -109,38 +141,68 @@ class FSIServer:
         pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         queue: deque = deque()
         active: List[Request] = []
+        slot = {}
         used = 0
-        batch = None
         t0 = time.perf_counter()
         sim_now = 0.0
+        changed = True
 
         def now():
             return sim_now if sim_exchange_seconds is not None else time.perf_counter() - t0
 
-        while pending or queue or active:
+        def build_ahead(limit=None):
+            """Build queued requests ahead of admission (at most `limit` now); returns how many."""
+            ready = sum(1 for q in queue if q.driver is not None)
+            n = 0
+            for q in queue:
+                if ready >= self.prebuild or (limit is not None and n >= limit):
+                    break
+                if q.driver is None:
+                    q.driver = self._build(q)
+                    ready += 1
+                    n += 1
+            return n
+
+        def overlap_exchange():
+            """While the exchange runs on the device: take in arrivals and build them."""
+            take_arrivals()
+            build_ahead()
+            if sim_exchange_seconds is not None or self.table.device.type != "cuda":
+                return
+            import torch
+
+            done = torch.cuda.Event()
+            done.record()
+            while not done.query():
+                take_arrivals()
+                if not build_ahead(limit=1):
+                    time.sleep(2e-4)
+
+        def take_arrivals():
             t = now()
             while pending and pending[0].arrival <= t:
-                queue.append(pending.popleft())
-            changed = False
-            while queue:
+                r = pending.popleft()
+                self._check(r)
+                queue.append(r)
+
+        while pending or queue or active:
+            take_arrivals()
+            while queue and len(active) < self.capacity:
                 r = queue[0]
-                r.bytes = estimate_bytes(r.workload.config)
-                c = r.workload.config
-                if c.coupling_mode == "moving_boundary" and c.lbm_substeps_per_fsi_step != self.n_substeps:
-                    raise ValueError(f"request {r.rid}: moving-boundary FSI step of {c.lbm_substeps_per_fsi_step} "
-                                     f"substeps, the server runs exchanges of {self.n_substeps}")
-                if r.bytes > self.budget:
-                    raise ValueError(f"request {r.rid} needs {r.bytes} bytes, more than the whole budget")
                 if used + r.bytes > self.budget:
                     break
                 queue.popleft()
-                r.driver = self._build(r)
+                if r.driver is None:
+                    r.driver = self._build(r)
+                    self.build_inline += 1
+                slot[r.rid] = self._table(r.driver).admit(r.driver)
                 r.admitted = now()
                 r.remaining = int(r.exchanges)
                 active.append(r)
                 used += r.bytes
                 changed = True
             if not active:
+                build_ahead()
                 if pending:
                     wait = pending[0].arrival - now()
                     if sim_exchange_seconds is not None:
@@ -148,11 +210,11 @@ class FSIServer:
                     elif wait > 0:
                         time.sleep(wait)
                 continue
-            if changed or batch is None:
-                batch = BatchedFSI(drivers=[r.driver for r in active], use_graph=self.use_graph,
-                                   n_substeps=self.n_substeps)
+            if changed:
                 self.regroups += 1
-            batch.step()
+                changed = False
+            self.table.step()  # queued on the device
+            overlap_exchange()
             if sync is not None:
                 sync()
             self.exchanges_run += 1
@@ -165,13 +227,29 @@ class FSIServer:
                 if r.remaining <= 0:
                     r.done = t
                     used -= r.bytes
+                    self.table.release(slot.pop(r.rid))
                     self._retire(r, on_done)
+                    changed = True
                 else:
                     still.append(r)
-            if len(still) != len(active):
-                batch = None
             active = still
         return requests
+
+    def stats(self) -> dict:
+        """Exchanges, how the exchange graph was run (built, rewritten in place
+        for a new instance set, replayed) and the host time spent building requests."""
+        t = self.table
+        replays = t.replays if t else 0
+        return {
+            "exchanges": self.exchanges_run,
+            "live_set_changes": self.regroups,
+            "graph_builds": t.graph_builds if t else 0,
+            "graph_updates": t.graph_updates if t else 0,
+            "graph_replays": replays,
+            "replay_fraction": round(replays / self.exchanges_run, 4) if self.exchanges_run else 0.0,
+            "build_seconds": round(self.build_seconds, 3),
+            "built_at_admission": self.build_inline,
+        }
 
     @staticmethod
     def _retire(r: Request, on_done):

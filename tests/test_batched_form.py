@@ -113,3 +113,80 @@ def test_substep_launch_plans_agree(emu, tmp_path):
             assert np.array_equal(getattr(da.solid, name).to_numpy(), getattr(db.solid, name).to_numpy()), name
     with pytest.raises(ValueError, match="substep_plan"):
         BatchedFSI(ws, out_dir=str(tmp_path / "bad"), native=emu, device="cpu", substep_plan="fast")
+
+
+def test_slot_table_admit_release_in_place(emu, tmp_path):
+    """A slot table admits and releases instances between exchanges: the
+    instance table keeps its address, the exchange graph of an unchanged
+    launch plan is rewritten in place (block maps, workgroup counts) instead
+    of built again, a new plan gets a graph of its own, and every instance
+    still matches its run alone bit for bit."""
+    ws = _mixed(tmp_path)
+    drivers = [_alone(w, tmp_path / "slot", emu, k) for k, w in enumerate(ws)]
+    alone = [_alone(w, tmp_path, emu, k) for k, w in enumerate(ws)]
+    table = BatchedFSI.slots(8, native=emu, device="cpu", n_substeps=6)
+    runs = [0] * len(ws)
+    s0 = table.admit(drivers[0])
+    s1 = table.admit(drivers[1])
+    table.step()
+    runs[0] += 1
+    runs[1] += 1
+    plan = table._plan
+    inst_addr = table._inst_dev.data_ptr()
+    s3 = table.admit(drivers[3])  # another penalty instance: same plan, the graph is updated
+    table.step()
+    for k in (0, 1, 3):
+        runs[k] += 1
+    assert table._plan == plan and (table.graph_builds, table.graph_updates, table.replays) == (1, 1, 1)
+    table.release(s0)
+    table.release(s1)
+    s2 = table.admit(drivers[2])  # moving-boundary with 3 MPM substeps: a new plan, a new graph
+    assert s2 in (s0, s1)
+    table.step()
+    runs[2] += 1
+    runs[3] += 1
+    assert table._plan != plan and table.graph_builds == 2
+    assert table._inst_dev.data_ptr() == inst_addr
+    table.release(s3)
+    table.release(s2)
+    assert table.live_slots == []
+    table.step()  # nothing live: nothing runs
+    table.close()
+    for k, (b, a) in enumerate(zip(drivers, alone)):
+        steps = runs[k] if a.config.coupling_mode == "moving_boundary" else runs[k] * 6
+        for _ in range(steps):
+            a.step_once()
+        assert b.total_lbm_substeps == a.total_lbm_substeps, k
+        for name in ("F", "rho", "v", "hydro_force"):
+            assert np.array_equal(getattr(b.lbm, name).to_numpy(), getattr(a.lbm, name).to_numpy()), (k, name)
+        for name in ("x", "v", "F", "C"):
+            assert np.array_equal(getattr(b.solid, name).to_numpy(), getattr(a.solid, name).to_numpy()), (k, name)
+
+
+def test_batch_graph_rejects_other_launch_lists(emu, tmp_path):
+    """fsi_batch_graph_update takes only the launch list the graph was built
+    from (same operations and substep indices); an operation without
+    workgroups is refused at build."""
+    import ctypes
+
+    import pytest
+
+    from mpm_lbm import _native
+
+    ws = _mixed(tmp_path)[:2]
+    batch = BatchedFSI(ws, out_dir=str(tmp_path / "g"), native=emu, device="cpu", substep_plan="split")
+    batch.step()
+    handle, _ = next(iter(batch._graphs.values()))
+    ops = batch._launch_list()
+    other = (_native.BatchOp * len(ops))(*ops)
+    other[3].t = other[3].t + 1
+    with pytest.raises(_native.FsiError, match="other operations"):
+        emu.call("fsi_batch_graph_update", handle, other, len(other))
+    with pytest.raises(_native.FsiError, match="number of operations"):
+        emu.call("fsi_batch_graph_update", handle, ops, len(ops) - 1)
+    empty = (_native.BatchOp * 1)()
+    empty[0].op = _native.BATCH_OP_PENALTY
+    h = ctypes.c_void_p()
+    with pytest.raises(_native.FsiError, match="without a block map"):
+        emu.call("fsi_batch_graph_create", empty, 1, _native.ptr(batch._inst_dev), batch.capacity, ctypes.byref(h))
+    batch.close()

@@ -51,3 +51,50 @@ def test_gpu_batched_matches_oracle(hip, tmp_path):
         errs = field_errors(d, o)
         print("batched instance", k, d.config.coupling_mode, errs, flush=True)
         check_history([errs], f"batched instance {k} ({d.config.coupling_mode})", TOL)
+
+
+def test_gpu_slot_table_replays_across_admissions(hip, tmp_path):
+    """Serving-form slot table on the GPU: instances admitted and released
+    between exchanges while one exchange graph is replayed, its block maps
+    and workgroup counts rewritten in place (the launch plan is unchanged);
+    each instance against its own oracle run for the exchanges it took part in."""
+    from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
+
+    ws = _mixed(tmp_path)[:4]
+    drivers, oracles = [], []
+    for k, w in enumerate(ws):
+        d = FSIDriver3D(w.config, str(tmp_path / f"slot{k}"), particle_cloud=w.cloud, native=hip,
+                        write_geometry_file=False, use_graph=False)
+        d.initialize()
+        flags = d.lbm.static_solid.to_numpy()
+        o = OracleFSI(w.config, flags, w.cloud, w.material, dtype=np.float32)
+        o.fp64 = OracleFSI(w.config, flags, w.cloud, w.material, dtype=np.float64)
+        drivers.append(d)
+        oracles.append(o)
+    table = BatchedFSI.slots(8, native=hip, n_substeps=8)
+    runs = [0] * 4
+    # MB (2 MPM substeps), penalty 20^3, MB (3)
+    slot = {k: table.admit(drivers[k]) for k in (0, 3, 2)}
+    table.step()
+    for k in (0, 2, 3):
+        runs[k] += 1
+    slot[1] = table.admit(drivers[1])  # a second, smaller penalty instance: same launch plan
+    table.step()
+    for k in (0, 1, 2, 3):
+        runs[k] += 1
+    table.release(slot.pop(3))
+    table.step()
+    for k in (0, 1, 2):
+        runs[k] += 1
+    print("slot table", {"builds": table.graph_builds, "updates": table.graph_updates, "replays": table.replays},
+          flush=True)
+    # one launch plan throughout: one graph, rewritten in place for each new instance set
+    assert table.graph_builds == 1 and table.replays == 2 and table.graph_updates == 2
+    for k, (d, o) in enumerate(zip(drivers, oracles)):
+        steps = runs[k] if d.config.coupling_mode == "moving_boundary" else runs[k] * 8
+        for _ in range(steps):
+            o.step_once()
+            o.fp64.step_once()
+        errs = field_errors(d, o)
+        print("slot instance", k, d.config.coupling_mode, runs[k], errs, flush=True)
+        check_history([errs], f"slot instance {k} ({d.config.coupling_mode})", TOL)

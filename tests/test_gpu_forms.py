@@ -94,6 +94,10 @@ def test_gpu_served_requests_match_oracle(hip, tmp_path):
 
     server.run(reqs, sim_exchange_seconds=1.0, on_done=keep)
     assert all(r.done is not None and r.driver is None for r in reqs) and server.regroups >= 3
+    st = server.stats()
+    print("served stats", st, flush=True)
+    # one slot table for the whole stream: a graph per launch plan, replayed across live-set changes
+    assert st["graph_builds"] + st["graph_replays"] == st["exchanges"]
     for r in reqs:
         w = r.workload
         fields, flags = kept[r.rid]
