@@ -224,6 +224,9 @@ typedef struct FsiMpmStats {
 #define FSI_BATCH_DOMAIN_PARTICLES 2 /* workgroups of FSI_PARTICLE_BLOCK particles */
 #define FSI_BATCH_DOMAIN_PLANES 3    /* substep tiles: nx * ceil(ny*nz / 256) workgroups */
 #define FSI_BATCH_DOMAIN_INSTANCES 4 /* one workgroup per instance */
+#define FSI_BATCH_DOMAIN_BOX 5       /* min(FSI_BATCH_BOX_BLOCKS, ceil(cells / 256)) workgroups per instance
+                                        that loop over the instance's tracked box (cells or grid nodes) */
+#define FSI_BATCH_BOX_BLOCKS 64
 
 typedef struct FsiBatchInstance {
   FsiLbmParams lp;
@@ -240,7 +243,23 @@ typedef struct FsiBatchInstance {
   int32_t cell_force; /* 1 = add cell_force (penalty) */
   int32_t pad0;
   double* react_ws;   /* FSI_COUPLED_WORKSPACE_DOUBLES(n_particles): reaction records of MPM_P2G_REACT */
+  int32_t* boxes;     /* FSI_BATCH_BOX_INTS, zeroed when the instance enters the batch (see below) */
 } FsiBatchInstance;
+
+/* Bounding boxes that confine the per-cell and per-node passes of the
+ * coupling to where the particles are.  Two boxes per instance, at
+ * FSI_BATCH_BOX_PROJ (lattice cells: projection fields and penalty forces)
+ * and FSI_BATCH_BOX_MPM (MPM grid nodes): int32 [0] valid, [1..3] lower and
+ * [4..6] upper corner (inclusive) of `cur`, [7..9] and [10..12] of `prev`.
+ * BOX_ROLL makes `cur` the new `prev` (the whole lattice / grid while not
+ * valid, as after zeroing) and empties `cur`; the particle passes then grow
+ * `cur` to their stencils (PROJ_PARTICLES, MPM_P2G, MPM_P2G_REACT); the
+ * clears cover `prev` (all the last pass wrote), NORMALIZE and the grid
+ * updates `cur`, PENALTY the box around both.  Cells and nodes outside hold
+ * zero before and after, so the results are those of whole-lattice passes. */
+#define FSI_BATCH_BOX_INTS 32
+#define FSI_BATCH_BOX_PROJ 0
+#define FSI_BATCH_BOX_MPM 16
 
 /* Operations of fsi_batch_launch. */
 #define FSI_BATCH_OP_LBM_SUBSTEP 0  /* one fused substep (t = substep index within the exchange) */
@@ -267,12 +286,13 @@ typedef struct FsiBatchInstance {
 #define FSI_BATCH_OP_LBM_SUBSTEP_PENALTY 18
 #define FSI_BATCH_OP_LBM_SUBSTEP_NONE 19
 #define FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD 20 /* after MB_FINALIZE: fold its records into the counters */
+#define FSI_BATCH_OP_BOX_ROLL 21 /* t = 0: roll the projection box, 1: the MPM box */
 
 /* Domain of each operation (the map passed to fsi_batch_launch must be built
- * for it): LBM_SUBSTEP, LBM_SUBSTEP_MB / _PENALTY / _NONE -> PLANES; REBUILD, MOMENTS, MB_BEGIN, MB_FINALIZE,
- * PROJ_CLEAR, PROJ_NORMALIZE, PENALTY -> CELLS; MPM_CLEAR, MPM_GRID_UPDATE,
- * MPM_GRID_UPDATE_REACT -> NODES; PROJ_PARTICLES, MPM_P2G, MPM_REACTION,
- * MPM_P2G_REACT, MPM_G2P -> PARTICLES; LBM_MB_FINALIZE_FOLD -> INSTANCES. */
+ * for it): LBM_SUBSTEP, LBM_SUBSTEP_MB / _PENALTY / _NONE -> PLANES; REBUILD, MOMENTS, MB_BEGIN,
+ * MB_FINALIZE -> CELLS; PROJ_CLEAR, PROJ_NORMALIZE, PENALTY, MPM_CLEAR, MPM_GRID_UPDATE,
+ * MPM_GRID_UPDATE_REACT -> BOX; PROJ_PARTICLES, MPM_P2G, MPM_REACTION,
+ * MPM_P2G_REACT, MPM_G2P -> PARTICLES; LBM_MB_FINALIZE_FOLD, BOX_ROLL -> INSTANCES. */
 
 /* Block map for one domain from the host copy of the descriptors: instances
  * that are active, whose mode bit is in `mode_mask` (bit m = 1 <<

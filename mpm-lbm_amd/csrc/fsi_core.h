@@ -715,7 +715,7 @@ FSI_HD Nbr nbr_of(const FsiLbmParams& p, int i, int j, int k) {
 template <typename Pop>
 FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const Pop& Fs, int cur, int parity,
                            uint32_t flags, int i, int j, int k, uint32_t N, uint32_t c, uint32_t info, const Nbr& nb,
-                           const FaceFlags& ff, const float* F);
+                           const FaceFlags& ff, const float* F, uint32_t outlet_flags);
 
 FSI_HD uint32_t cell_info(const FsiLbmState& st, uint32_t c) { return at32(st.cellinfo, c); }
 
@@ -789,9 +789,12 @@ FSI_HD void wall_links_accumulate(const FsiLbmState& st, uint32_t c, uint32_t in
 // by the upstream fluid neighbour, by the cell's own bounce-back, or by the BC.
 // Wall (solid) cells gather the previous substep's link forces instead.
 // All lattice addressing is 32-bit (at most 2^30 cells, checked by the ABI).
+// outlet_flags: the flags the pressure outlet reads the velocity behind it
+// with (~0u: `flags`); the batched penalty substep takes the stored moments
+// of some cells and the moments of F_src of the others.
 template <typename Pop, bool SPEC = false>
 FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int cur, int parity,
-                             uint32_t flags, int i, int j, int k) {
+                             uint32_t flags, int i, int j, int k, uint32_t outlet_flags = ~0u) {
   const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
   const uint32_t c = ((uint32_t)i * (uint32_t)p.ny + (uint32_t)j) * (uint32_t)p.nz + (uint32_t)k;
   const uint32_t info = at32(st.cellinfo, c);
@@ -830,7 +833,8 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
     wall_links_load<Pop>(st, Fs, parity, N, c, info, nb, ff, Fw, acc, c);
   }
   if (!solid) {
-    lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, flags, i, j, k, N, c, info, nb, ff, F);
+    lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, flags, i, j, k, N, c, info, nb, ff, F,
+                        outlet_flags == ~0u ? flags : outlet_flags);
     return;
   }
   wall_links_accumulate<Pop>(st, c, info, ff, Fw, acc);
@@ -841,7 +845,7 @@ FSI_HD void lbm_substep_cell(const FsiLbmParams& p, const FsiLbmState& st, int c
 template <typename Pop>
 FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const Pop& Fs, int cur, int parity,
                            uint32_t flags, int i, int j, int k, uint32_t N, uint32_t c, uint32_t info, const Nbr& nb,
-                           const FaceFlags& ff, const float* F) {
+                           const FaceFlags& ff, const float* F, uint32_t outlet_flags) {
   const bool mb = (flags & FSI_SUB_MOVING_WALL) != 0;
   const Pop Fd = pop_of<Pop>(st.F[1 - cur]);
   float gx = p.force[0], gy = p.force[1], gz = p.force[2];
@@ -857,7 +861,7 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
   const bool legacy_reset = p.open_bc == FSI_OPEN_BC_EQUILIBRIUM;
   const bool outlet_p = legacy_reset && i == p.nx - 1 && p.bc_x_right == FSI_BC_PRESSURE;
   float obx = 0.0f, oby = 0.0f, obz = 0.0f;
-  if (outlet_p) prev_velocity_flat(p, st, Fs, flags, c - (uint32_t)(p.ny * p.nz), obx, oby, obz);
+  if (outlet_p) prev_velocity_flat(p, st, Fs, outlet_flags, c - (uint32_t)(p.ny * p.nz), obx, oby, obz);
   float m[Q];
   moments_M(F, m);
   float rho, ux, uy, uz;
@@ -2424,6 +2428,10 @@ FSI_HD int batch_block_size(int domain) { return domain == FSI_BATCH_DOMAIN_PART
 
 FSI_HD long long batch_domain_blocks(const FsiBatchInstance& I, int domain) {
   if (domain == FSI_BATCH_DOMAIN_INSTANCES) return 1;
+  if (domain == FSI_BATCH_DOMAIN_BOX) {
+    const long long cb = ((long long)I.lp.nx * I.lp.ny * I.lp.nz + 255) / 256;
+    return cb < FSI_BATCH_BOX_BLOCKS ? cb : FSI_BATCH_BOX_BLOCKS;
+  }
   if (domain == FSI_BATCH_DOMAIN_PLANES)  // one i-plane per run of tiles, as the single-instance substep
     return (long long)I.lp.nx * (((long long)I.lp.ny * I.lp.nz + 255) / 256);
   long long n = domain == FSI_BATCH_DOMAIN_CELLS ? (long long)I.lp.nx * I.lp.ny * I.lp.nz
@@ -2449,7 +2457,7 @@ FSI_HD uint32_t batch_substep_flags(const FsiBatchInstance& I, int t) {
     f |= FSI_SUB_MOVING_WALL;
     f |= t == 0 ? FSI_SUB_STORED_MOMENTS : FSI_SUB_GATHER_PREV;
     if (t == I.n_substeps - 1) f |= FSI_SUB_DIAG;
-  } else {
+  } else if (t == 0) {
     f |= FSI_SUB_STORED_MOMENTS;
   }
   return f;
@@ -2465,6 +2473,57 @@ constexpr unsigned NXCD = 8;
 FSI_HD unsigned xcd_tile(unsigned b, unsigned n) {
   const unsigned per = n / NXCD, rem = n % NXCD, x = b % NXCD;
   return x * per + (x < rem ? x : rem) + b / NXCD;
+}
+
+// Bounding boxes of the batched coupling passes (FSI_BATCH_BOX_*): `b` points
+// at one box record, [0] valid, [1..6] cur (lo, hi), [7..12] prev (lo, hi).
+constexpr int BOX_CUR = 1, BOX_PREV = 7;
+
+FSI_HD void box_roll(int32_t* b, int nx, int ny, int nz) {
+  const int ext[3] = {nx, ny, nz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b[BOX_PREV + d] = b[0] ? b[BOX_CUR + d] : 0;
+    b[BOX_PREV + 3 + d] = b[0] ? b[BOX_CUR + 3 + d] : ext[d] - 1;
+    b[BOX_CUR + d] = 0x7fffffff;
+    b[BOX_CUR + 3 + d] = -0x7fffffff - 1;
+  }
+  b[0] = 1;
+}
+
+// Whether (i, j, k) lies in the box lo = r[0..2], hi = r[3..5].
+FSI_HD bool in_box(const int32_t* r, int i, int j, int k) {
+  return r[0] <= i && i <= r[3] && r[1] <= j && j <= r[4] && r[2] <= k && k <= r[5];
+}
+
+// The smallest box holding boxes r and q.
+FSI_HD void box_hull(const int32_t* r, const int32_t* q, int32_t* out) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    out[d] = r[d] < q[d] ? r[d] : q[d];
+    out[3 + d] = r[3 + d] > q[3 + d] ? r[3 + d] : q[3 + d];
+  }
+}
+
+// Whether the i-planes [i0, i1] meet box r.
+FSI_HD bool planes_meet_box(const int32_t* r, long long i0, long long i1) { return i1 >= r[0] && i0 <= r[3]; }
+
+// Where a penalty / none instance's substep t >= 1 takes rho and u from:
+// the batched exchange runs no moments pass between its LBM steps.  A
+// penalty instance's PENALTY pass has stored the moments (with the previous
+// step's cell force, as the pass after the previous step would have) in the
+// box it covers, the hull of the projection boxes; everywhere else the cell
+// force of both steps is zero and the moments of F_src are the same values.
+// A none instance takes the moments of F_src everywhere.  Sets the flags of
+// cell (i, j, k) and of the cell behind it (the pressure outlet reads that).
+FSI_HD void batch_moment_source(const FsiBatchInstance& I, int t, uint32_t& flags, uint32_t& outlet_flags, int i,
+                                int j, int k) {
+  outlet_flags = flags;
+  if (t == 0 || I.mode != FSI_BATCH_MODE_PENALTY) return;
+  int32_t both[6];
+  box_hull(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, both);
+  if (in_box(both, i, j, k)) flags |= FSI_SUB_STORED_MOMENTS;
+  if (in_box(both, i - 1, j, k)) outlet_flags |= FSI_SUB_STORED_MOMENTS;
 }
 
 }  // namespace fsi

@@ -398,7 +398,7 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
 int fsi_batch_block_map(const FsiBatchInstance* inst, int n_inst, int domain, int mode_mask, int min_mpm,
                         int32_t* map) {
   if (!inst || n_inst < 0) return fail(FSI_E_INVALID, "null batch arguments");
-  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_INSTANCES) return fail(FSI_E_INVALID, "unknown batch domain");
+  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_BOX) return fail(FSI_E_INVALID, "unknown batch domain");
   long long total = 0;
   for (int i = 0; i < n_inst; ++i) {
     if (inst[i].lp.open_bc != FSI_OPEN_BC_EQUILIBRIUM)
@@ -413,6 +413,23 @@ int fsi_batch_block_map(const FsiBatchInstance* inst, int n_inst, int domain, in
     total += nb;
   }
   return (int)total;
+}
+
+// Box tests of the batched passes (FSI_BATCH_BOX_*), as the kernels make them.
+static bool cell_in(const int32_t* r, long long e, int ny, int nz) {
+  const long long plane = (long long)ny * nz, i = e / plane, rem = e - i * plane;
+  return in_box(r, (int)i, (int)(rem / nz), (int)(rem % nz));
+}
+
+// Grows box record b's `cur` over particle q's stencil at spacing 1 / inv.
+static void grow_box(int32_t* b, const float* x, long long P, long long q, float inv) {
+  for (int d = 0; d < 3; ++d) {
+    int base;
+    float fx, w[3];
+    bspline(x[d * P + q] * inv, base, fx, w);
+    b[BOX_CUR + d] = base < b[BOX_CUR + d] ? base : b[BOX_CUR + d];
+    b[BOX_CUR + 3 + d] = base + 2 > b[BOX_CUR + 3 + d] ? base + 2 : b[BOX_CUR + 3 + d];
+  }
 }
 
 // Serial emulation of fsi_batch_launch: instance by instance (the instances
@@ -463,7 +480,11 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
           for (int a = 0; a < I.lp.nx; ++a)
             for (int b = 0; b < I.lp.ny; ++b)
               for (int k = 0; k < I.lp.nz; ++k)
-                lbm_substep_cell<Pop>(I.lp, I.ls, batch_cur(I, t), t & 1, batch_substep_flags(I, t), a, b, k);
+                {
+                  uint32_t fl = batch_substep_flags(I, t), ofl;
+                  batch_moment_source(I, t, fl, ofl, a, b, k);
+                  lbm_substep_cell<Pop>(I.lp, I.ls, batch_cur(I, t), t & 1, fl, a, b, k, ofl);
+                }
         });
         break;
       case FSI_BATCH_OP_LBM_REBUILD:
@@ -506,18 +527,29 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
       }
       case FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD:  // the serial finalize above sets the counters directly
         break;
+      case FSI_BATCH_OP_BOX_ROLL:
+        if (t == 0)
+          box_roll(I.boxes + FSI_BATCH_BOX_PROJ, I.lp.nx, I.lp.ny, I.lp.nz);
+        else
+          box_roll(I.boxes + FSI_BATCH_BOX_MPM, I.mp.gnx, I.mp.gny, I.mp.gnz);
+        break;
       case FSI_BATCH_OP_PROJ_CLEAR:
         for (long long c = 0; c < cells; ++c) {
+          if (!cell_in(I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, c, I.lp.ny, I.lp.nz)) continue;
           I.ls.solid_phi[c] = 0.0f;
           I.ls.solid_mass[c] = 0.0f;
           I.ls.solid_vel[c] = I.ls.solid_vel[cells + c] = I.ls.solid_vel[2 * cells + c] = 0.0f;
         }
         break;
       case FSI_BATCH_OP_PROJ_PARTICLES:
-        for (long long q = 0; q < parts; ++q) project_particle(I.lp, I.ls, I.mp, I.ms, I.cp, q, proj_sink(I.ls, cells));
+        for (long long q = 0; q < parts; ++q) {
+          grow_box(I.boxes + FSI_BATCH_BOX_PROJ, I.ms.x, parts, q, I.cp.inv_dx_norm);
+          project_particle(I.lp, I.ls, I.mp, I.ms, I.cp, q, proj_sink(I.ls, cells));
+        }
         break;
       case FSI_BATCH_OP_PROJ_NORMALIZE:
         for (long long c = 0; c < cells; ++c) {
+          if (!cell_in(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, c, I.lp.ny, I.lp.nz)) continue;
           ProjOut o = normalize_cell(I.lp, I.ls, I.cp, c);
           atomic_max_nonneg(&I.cc->max_phi_raw_bits, o.phi_raw > 0.0f ? o.phi_raw : 0.0f);
           I.cc->active_cell_count += o.active;
@@ -526,8 +558,16 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
           I.cc->projected_volume_clamped += (double)o.phi_clamped * I.cp.cell_volume_norm;
         }
         break;
-      case FSI_BATCH_OP_PENALTY:
+      case FSI_BATCH_OP_PENALTY: {
+        int32_t both[6];
+        box_hull(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, both);
         for (long long c = 0; c < cells; ++c) {
+          if (!cell_in(both, c, I.lp.ny, I.lp.nz)) continue;
+          if (t > 0)
+            by_storage(I.lp.storage, [&](auto tag) {
+              using Pop = decltype(tag);
+              lbm_moments_cell<Pop>(I.lp, I.ls, batch_cur(I, t), FSI_SUB_CELL_FORCE, c);
+            });
           float f[3];
           int act;
           penalty_cell(I.lp, I.ls, I.cp, c, f, act);
@@ -541,17 +581,25 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
           }
         }
         break;
+      }
       case FSI_BATCH_OP_MPM_CLEAR:
-        memset(I.ms.grid_v, 0, sizeof(float) * 3 * nodes);
-        memset(I.ms.grid_m, 0, sizeof(float) * nodes);
-        memset(I.ms.grid_f, 0, sizeof(float) * 3 * nodes);
+        for (long long n = 0; n < nodes; ++n) {
+          if (!cell_in(I.boxes + FSI_BATCH_BOX_MPM + BOX_PREV, n, I.mp.gny, I.mp.gnz)) continue;
+          I.ms.grid_v[n] = I.ms.grid_v[nodes + n] = I.ms.grid_v[2 * nodes + n] = 0.0f;
+          I.ms.grid_m[n] = 0.0f;
+          I.ms.grid_f[n] = I.ms.grid_f[nodes + n] = I.ms.grid_f[2 * nodes + n] = 0.0f;
+        }
         break;
       case FSI_BATCH_OP_MPM_P2G:
-        for (long long q = 0; q < parts; ++q) mpm_p2g_particle(I.mp, I.ms, q, p2g_sink(I.ms, nodes));
+        for (long long q = 0; q < parts; ++q) {
+          grow_box(I.boxes + FSI_BATCH_BOX_MPM, I.ms.x, parts, q, I.mp.inv_dx);
+          mpm_p2g_particle(I.mp, I.ms, q, p2g_sink(I.ms, nodes));
+        }
         break;
       case FSI_BATCH_OP_MPM_REACTION:
         if (I.mode == FSI_BATCH_MODE_NONE) break;
         for (long long q = 0; q < parts; ++q) {
+          grow_box(I.boxes + FSI_BATCH_BOX_MPM, I.ms.x, parts, q, I.cp.inv_dx_norm);
           ReactOut o = reaction_particle(I.lp, I.ls, I.mp, I.ms, I.cp, I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1, q,
                                           reaction_sink(I.ms, nodes));
           I.cc->active_reaction_particle_count += o.active;
@@ -564,7 +612,12 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
         }
         break;
       case FSI_BATCH_OP_MPM_P2G_REACT:
-        for (long long q = 0; q < parts; ++q) mpm_p2g_particle(I.mp, I.ms, q, p2g_sink(I.ms, nodes));
+        for (long long q = 0; q < parts; ++q) {
+          grow_box(I.boxes + FSI_BATCH_BOX_MPM, I.ms.x, parts, q, I.mp.inv_dx);
+          if (I.mode != FSI_BATCH_MODE_NONE && I.cp.inv_dx_norm != I.mp.inv_dx)
+            grow_box(I.boxes + FSI_BATCH_BOX_MPM, I.ms.x, parts, q, I.cp.inv_dx_norm);
+          mpm_p2g_particle(I.mp, I.ms, q, p2g_sink(I.ms, nodes));
+        }
         if (I.mode == FSI_BATCH_MODE_NONE) break;
         I.cc->active_reaction_particle_count = 0;
         I.cc->max_particle_reaction_bits = I.cc->max_grid_reaction_bits = 0;
@@ -583,7 +636,8 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
         break;
       case FSI_BATCH_OP_MPM_GRID_UPDATE:
       case FSI_BATCH_OP_MPM_GRID_UPDATE_REACT:
-        for (long long n = 0; n < nodes; ++n) mpm_grid_node(I.mp, I.ms, n);
+        for (long long n = 0; n < nodes; ++n)
+          if (cell_in(I.boxes + FSI_BATCH_BOX_MPM + BOX_CUR, n, I.mp.gny, I.mp.gnz)) mpm_grid_node(I.mp, I.ms, n);
         break;
       case FSI_BATCH_OP_MPM_G2P:
         for (long long q = 0; q < parts; ++q) mpm_g2p_particle(I.mp, I.ms, q);
@@ -606,7 +660,7 @@ struct EmuBatchGraph {
 int check_ops(const FsiBatchOp* ops, int n_ops, const EmuBatchGraph* g) {
   if (g && (size_t)n_ops != g->ops.size()) return fail(FSI_E_INVALID, "batch graph update with another number of operations");
   for (int i = 0; i < n_ops; ++i) {
-    if (ops[i].op < FSI_BATCH_OP_LBM_SUBSTEP || ops[i].op > FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD)
+    if (ops[i].op < FSI_BATCH_OP_LBM_SUBSTEP || ops[i].op > FSI_BATCH_OP_BOX_ROLL)
       return fail(FSI_E_INVALID, "unknown batch operation");
     if (ops[i].op != FSI_BATCH_OP_COUNTERS_CLEAR && (!ops[i].block_map || ops[i].n_blocks <= 0))
       return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");

@@ -593,7 +593,7 @@ __global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLb
   }
   if (fluid)
     lbm_fluid_cell<Pop>(p, st, Fs, cur, parity, fl, (int)i, (int)j, (int)k, N, c, info, nbr_of(p, (int)i, (int)j, (int)k),
-                        ff, F);
+                        ff, F, fl);
   if (queued) wall_links_accumulate<Pop>(st, wc, winfo, ff, Fw, acc);
 }
 
@@ -1484,9 +1484,69 @@ __global__ void __launch_bounds__(TPB) k_mpm_stats_partial(FsiMpmParams mp, FsiM
   const long long e_ = (long long)l_ * (BS) + threadIdx.x;                                 \
   const bool in_ = e_ < batch_domain_size(I, DOMAIN);
 
+// BOX-domain launches: each instance's workgroups (batch_domain_blocks) walk
+// the cells or nodes of one of its boxes, clipped to the domain, k fastest,
+// item b of the walk going to workgroup b / TPB mod that count.
+#define FSI_BATCH_BOX_PROLOGUE                                               \
+  const unsigned b_ = xcd_tile(blockIdx.x, gridDim.x);                       \
+  const int i_ = bmap[2 * b_], l_ = bmap[2 * b_ + 1];                         \
+  const FsiBatchInstance& I = inst[i_];                                      \
+  const long long stride_ = batch_domain_blocks(I, FSI_BATCH_DOMAIN_BOX) * TPB;
+
+struct BoxWalk {
+  int lo[3], e[3];
+  long long n;
+};
+
+__device__ __forceinline__ BoxWalk box_walk(const int32_t* r, int nx, int ny, int nz) {
+  const int ext[3] = {nx, ny, nz};
+  BoxWalk w;
+  w.n = 1;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int lo = r[d] > 0 ? r[d] : 0, hi = r[3 + d] < ext[d] - 1 ? r[3 + d] : ext[d] - 1;
+    w.lo[d] = lo;
+    w.e[d] = hi >= lo ? hi - lo + 1 : 0;
+    w.n *= w.e[d];
+  }
+  return w;
+}
+
+// Linear index (in the ny x nz domain) of item b of walk w.
+__device__ __forceinline__ long long box_item(const BoxWalk& w, long long b, int ny, int nz) {
+  const long long k = b % w.e[2], r = b / w.e[2], j = r % w.e[1], i = r / w.e[1];
+  return ((w.lo[0] + i) * ny + (w.lo[1] + j)) * (long long)nz + (w.lo[2] + k);
+}
+
+// Grows box record b's `cur` over the stencils (base .. base + 2 per axis)
+// of the wave's valid lanes: wave reductions, then one atomic per bound.
+__device__ __forceinline__ void wave_grow_box(int32_t* b, const int* base, bool valid) {
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = wave_min_i(valid ? base[d] : 0x7FFFFFFF);
+    hi[d] = wave_max_i(valid ? base[d] + 2 : -0x7FFFFFFF - 1);
+  }
+  if ((threadIdx.x & 63) == 0 && lo[0] <= hi[0]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(b + BOX_CUR + d, lo[d]);
+      atomicMax(b + BOX_CUR + 3 + d, hi[d]);
+    }
+  }
+}
+
+// Stencil base nodes of particle q (valid) at grid spacing 1 / inv.
+__device__ __forceinline__ void stencil_base(const float* x, long long P, long long q, float inv, int* base) {
+  float fx, w[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline(x[d * P + q] * inv, base[d], fx, w);
+}
+
 // FIXED: the substep flags common to the launch's instances (STORED_MOMENTS,
 // MOVING_WALL, CELL_FORCE), compiled in as in k_lbm_substep; ~0u = take them
-// from each instance at run time (any mix of modes).
+// from each instance at run time (any mix of modes).  A penalty instance's
+// substeps after the first take STORED_MOMENTS per cell (batch_moment_source).
 template <uint32_t FIXED>
 __global__ void __launch_bounds__(TPB, 5) kb_substep(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
   const unsigned b_ = xcd_tile(blockIdx.x, gridDim.x);
@@ -1503,14 +1563,16 @@ __global__ void __launch_bounds__(TPB, 5) kb_substep(const FsiBatchInstance* ins
   if (jk >= plane) return;
   const unsigned j = jk / (unsigned)p.nz, k = jk - j * (unsigned)p.nz;
   constexpr uint32_t VAR = FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE;
-  const uint32_t flags = FIXED == ~0u ? batch_substep_flags(I, t) : ((batch_substep_flags(I, t) & ~VAR) | FIXED);
+  uint32_t flags = FIXED == ~0u ? batch_substep_flags(I, t) : ((batch_substep_flags(I, t) & ~VAR) | FIXED);
+  uint32_t oflags;
+  batch_moment_source(I, t, flags, oflags, (int)i, (int)j, (int)k);
   const int cur = batch_cur(I, t), parity = t & 1;
   if (p.storage == FSI_STORE_F16_DEV)
-    lbm_substep_cell<PopF16, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
+    lbm_substep_cell<PopF16, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k, oflags);
   else if (p.storage == FSI_STORE_BF16_DEV)
-    lbm_substep_cell<PopBF16, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
+    lbm_substep_cell<PopBF16, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k, oflags);
   else
-    lbm_substep_cell<PopF32, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k);
+    lbm_substep_cell<PopF32, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k, oflags);
 }
 
 __global__ void __launch_bounds__(TPB) kb_rebuild(const FsiBatchInstance* inst, const int32_t* bmap) {
@@ -1566,6 +1628,16 @@ __global__ void __launch_bounds__(TPB) kb_finalize_fold(const FsiBatchInstance* 
   finalize_fold(I.ls.reduce_ws, n_rec, I.ls.counters, 1, I.n_substeps, sd);
 }
 
+// one workgroup per instance (INSTANCES domain); t = 0: projection box, 1: MPM box
+__global__ void __launch_bounds__(64) kb_box_roll(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
+  const FsiBatchInstance& I = inst[bmap[2 * blockIdx.x]];
+  if (threadIdx.x != 0) return;
+  if (t == 0)
+    box_roll(I.boxes + FSI_BATCH_BOX_PROJ, I.lp.nx, I.lp.ny, I.lp.nz);
+  else
+    box_roll(I.boxes + FSI_BATCH_BOX_MPM, I.mp.gnx, I.mp.gny, I.mp.gnz);
+}
+
 __global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* inst, int n_inst, int which) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n_inst || !inst[i].active) return;
@@ -1592,17 +1664,23 @@ __global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* 
 }
 
 __global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* inst, const int32_t* bmap) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
-  if (!in_) return;
+  FSI_BATCH_BOX_PROLOGUE
+  const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, I.lp.nx, I.lp.ny, I.lp.nz);
   const long long N = batch_domain_size(I, FSI_BATCH_DOMAIN_CELLS);
-  I.ls.solid_phi[e_] = 0.0f;
-  I.ls.solid_mass[e_] = 0.0f;
-  I.ls.solid_vel[e_] = I.ls.solid_vel[N + e_] = I.ls.solid_vel[2 * N + e_] = 0.0f;
+  for (long long b = (long long)l_ * TPB + threadIdx.x; b < w.n; b += stride_) {
+    const long long c = box_item(w, b, I.lp.ny, I.lp.nz);
+    I.ls.solid_phi[c] = 0.0f;
+    I.ls.solid_mass[c] = 0.0f;
+    I.ls.solid_vel[c] = I.ls.solid_vel[N + c] = I.ls.solid_vel[2 * N + c] = 0.0f;
+  }
 }
 
 __global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ __align__(16) float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
+  int base[3] = {0, 0, 0};
+  if (in_) stencil_base(I.ms.x, I.mp.n_particles, e_, I.cp.inv_dx_norm, base);
+  wave_grow_box(I.boxes + FSI_BATCH_BOX_PROJ, base, in_);
   proj_block(I.lp, I.ls, I.mp, I.ms, I.cp, e_, in_, tile);
 }
 
@@ -1610,16 +1688,24 @@ __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance*
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
-  ProjOut o;
-  o.phi_raw = o.phi_clamped = o.mass = 0.0f;
-  o.active = 0;
-  if (in_) o = normalize_cell(I.lp, I.ls, I.cp, e_);
-  const float mp = block_max_f(o.phi_raw > 0.0f ? o.phi_raw : 0.0f, sf);
-  const int act = block_sum_i(o.active, si);
-  const double m = block_reduce((double)o.mass, RED_SUM, sd);
-  const double vr = block_reduce((double)o.phi_raw * I.cp.cell_volume_norm, RED_SUM, sd);
-  const double vc = block_reduce((double)o.phi_clamped * I.cp.cell_volume_norm, RED_SUM, sd);
+  FSI_BATCH_BOX_PROLOGUE
+  const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, I.lp.nx, I.lp.ny, I.lp.nz);
+  float mp = 0.0f;
+  int act = 0;
+  double m = 0.0, vr = 0.0, vc = 0.0;
+  for (long long b = (long long)l_ * TPB + threadIdx.x; b < w.n; b += stride_) {
+    const ProjOut o = normalize_cell(I.lp, I.ls, I.cp, box_item(w, b, I.lp.ny, I.lp.nz));
+    mp = o.phi_raw > mp ? o.phi_raw : mp;
+    act += o.active;
+    m += (double)o.mass;
+    vr += (double)o.phi_raw * I.cp.cell_volume_norm;
+    vc += (double)o.phi_clamped * I.cp.cell_volume_norm;
+  }
+  mp = block_max_f(mp, sf);
+  act = block_sum_i(act, si);
+  m = block_reduce(m, RED_SUM, sd);
+  vr = block_reduce(vr, RED_SUM, sd);
+  vc = block_reduce(vc, RED_SUM, sd);
   if (threadIdx.x == 0) {
     FsiCouplingCounters* cc = I.cc;
     if (mp > 0.0f) atomicMax(&cc->max_phi_raw_bits, __float_as_uint(mp));
@@ -1630,18 +1716,34 @@ __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance*
   }
 }
 
-__global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, const int32_t* bmap) {
+// Penalty pass of LBM step t.  From t = 1 on it first stores the moments of
+// the cells it covers, from the populations after the last step and that
+// step's cell force, as the moments pass after that step would have.
+__global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
-  float f[3] = {0.0f, 0.0f, 0.0f};
-  int active = 0;
-  if (in_) penalty_cell(I.lp, I.ls, I.cp, e_, f, active);
-  const float mfn = block_max_f(norm3(f[0], f[1], f[2]), sf);
-  const int act = block_sum_i(active, si);
-  double net[3];
-  for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)f[d], RED_SUM, sd);
+  FSI_BATCH_BOX_PROLOGUE
+  int32_t both[6];
+  box_hull(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, both);
+  const BoxWalk w = box_walk(both, I.lp.nx, I.lp.ny, I.lp.nz);
+  float mfn = 0.0f;
+  int act = 0;
+  double net[3] = {0.0, 0.0, 0.0};
+  for (long long b = (long long)l_ * TPB + threadIdx.x; b < w.n; b += stride_) {
+    const long long c = box_item(w, b, I.lp.ny, I.lp.nz);
+    if (t > 0) FSI_BY_STORAGE(I.lp.storage, lbm_moments_cell<POP>(I.lp, I.ls, batch_cur(I, t), FSI_SUB_CELL_FORCE, c));
+    float f[3];
+    int active;
+    penalty_cell(I.lp, I.ls, I.cp, c, f, active);
+    const float fn = norm3(f[0], f[1], f[2]);
+    mfn = fn > mfn ? fn : mfn;
+    act += active;
+    for (int d = 0; d < 3; ++d) net[d] += (double)f[d];
+  }
+  mfn = block_max_f(mfn, sf);
+  act = block_sum_i(act, si);
+  for (int d = 0; d < 3; ++d) net[d] = block_reduce(net[d], RED_SUM, sd);
   if (threadIdx.x == 0) {
     FsiCouplingCounters* cc = I.cc;
     if (act) atomicAdd(&cc->active_force_cell_count, act);
@@ -1658,17 +1760,23 @@ __global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, 
 }
 
 __global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst, const int32_t* bmap) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES, TPB)
-  if (!in_) return;
+  FSI_BATCH_BOX_PROLOGUE
+  const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_MPM + BOX_PREV, I.mp.gnx, I.mp.gny, I.mp.gnz);
   const long long G = batch_domain_size(I, FSI_BATCH_DOMAIN_NODES);
-  I.ms.grid_v[e_] = I.ms.grid_v[G + e_] = I.ms.grid_v[2 * G + e_] = 0.0f;
-  I.ms.grid_m[e_] = 0.0f;
-  I.ms.grid_f[e_] = I.ms.grid_f[G + e_] = I.ms.grid_f[2 * G + e_] = 0.0f;
+  for (long long b = (long long)l_ * TPB + threadIdx.x; b < w.n; b += stride_) {
+    const long long n = box_item(w, b, I.mp.gny, I.mp.gnz);
+    I.ms.grid_v[n] = I.ms.grid_v[G + n] = I.ms.grid_v[2 * G + n] = 0.0f;
+    I.ms.grid_m[n] = 0.0f;
+    I.ms.grid_f[n] = I.ms.grid_f[G + n] = I.ms.grid_f[2 * G + n] = 0.0f;
+  }
 }
 
 __global__ void __launch_bounds__(PTPB) kb_mpm_p2g(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ float tile[TILE_COPIES * 4 * TILE_NODES];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
+  int base[3] = {0, 0, 0};
+  if (in_) stencil_base(I.ms.x, I.mp.n_particles, e_, I.mp.inv_dx, base);
+  wave_grow_box(I.boxes + FSI_BATCH_BOX_MPM, base, in_);
   p2g_block(I.mp, I.ms, e_, in_, tile);
 }
 
@@ -1679,6 +1787,9 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* 
   __shared__ float tile[TILE_COPIES * 3 * TILE_NODES];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   const int mode = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 2 : 1;
+  int base[3] = {0, 0, 0};
+  if (in_) stencil_base(I.ms.x, I.mp.n_particles, e_, I.cp.inv_dx_norm, base);
+  wave_grow_box(I.boxes + FSI_BATCH_BOX_MPM, base, in_ && I.mode != FSI_BATCH_MODE_NONE);
   const ReactOut o = reaction_block(I.lp, I.ls, I.mp, I.ms, I.cp, mode, e_, in_ && I.mode != FSI_BATCH_MODE_NONE,
                                     tile);
   const int act = block_sum_i(o.active, si);
@@ -1702,6 +1813,14 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_p2g_react(const FsiBatchInstance*
   __shared__ float sf[PTPB / 64];
   __shared__ int si[PTPB / 64];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
+  int base[3] = {0, 0, 0};
+  if (in_) stencil_base(I.ms.x, I.mp.n_particles, e_, I.mp.inv_dx, base);
+  wave_grow_box(I.boxes + FSI_BATCH_BOX_MPM, base, in_);
+  if (I.mode != FSI_BATCH_MODE_NONE && I.cp.inv_dx_norm != I.mp.inv_dx) {
+    // the reaction scatters on its own stencil into grid_f
+    if (in_) stencil_base(I.ms.x, I.mp.n_particles, e_, I.cp.inv_dx_norm, base);
+    wave_grow_box(I.boxes + FSI_BATCH_BOX_MPM, base, in_);
+  }
   double* rec = I.react_ws + (size_t)l_ * REACT_REC;
   if (I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY)
     p2g_react_block<2>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sd, sf, si);
@@ -1712,14 +1831,18 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_p2g_react(const FsiBatchInstance*
 }
 
 __global__ void __launch_bounds__(TPB) kb_mpm_grid_update(const FsiBatchInstance* inst, const int32_t* bmap) {
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES, TPB)
-  if (in_) mpm_grid_node(I.mp, I.ms, e_);
+  FSI_BATCH_BOX_PROLOGUE
+  const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_MPM + BOX_CUR, I.mp.gnx, I.mp.gny, I.mp.gnz);
+  for (long long b = (long long)l_ * TPB + threadIdx.x; b < w.n; b += stride_)
+    mpm_grid_node(I.mp, I.ms, box_item(w, b, I.mp.gny, I.mp.gnz));
 }
 
 __global__ void __launch_bounds__(TPB) kb_mpm_grid_update_react(const FsiBatchInstance* inst, const int32_t* bmap) {
   __shared__ double sd[TPB / 64];
-  FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_NODES, TPB)
-  if (in_) mpm_grid_node(I.mp, I.ms, e_);
+  FSI_BATCH_BOX_PROLOGUE
+  const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_MPM + BOX_CUR, I.mp.gnx, I.mp.gny, I.mp.gnz);
+  for (long long b = (long long)l_ * TPB + threadIdx.x; b < w.n; b += stride_)
+    mpm_grid_node(I.mp, I.ms, box_item(w, b, I.mp.gny, I.mp.gnz));
   if (l_ == 0 && I.mode != FSI_BATCH_MODE_NONE && I.mp.n_particles > 0)
     fold_react_records(I.react_ws, (int)((I.mp.n_particles + PTPB - 1) / PTPB), I.cc, sd);
 }
@@ -1750,8 +1873,13 @@ int batch_kernel(int op, int t, BatchKernel& k) {
       k.fn = t == 0 ? (const void*)&kb_substep<FSI_SUB_MOVING_WALL | FSI_SUB_STORED_MOMENTS>
                     : (const void*)&kb_substep<FSI_SUB_MOVING_WALL>;
       break;
-    case FSI_BATCH_OP_LBM_SUBSTEP_PENALTY: k.fn = (const void*)&kb_substep<FSI_SUB_CELL_FORCE | FSI_SUB_STORED_MOMENTS>; break;
-    case FSI_BATCH_OP_LBM_SUBSTEP_NONE: k.fn = (const void*)&kb_substep<FSI_SUB_STORED_MOMENTS>; break;
+    case FSI_BATCH_OP_LBM_SUBSTEP_PENALTY:
+      k.fn = t == 0 ? (const void*)&kb_substep<FSI_SUB_CELL_FORCE | FSI_SUB_STORED_MOMENTS>
+                    : (const void*)&kb_substep<FSI_SUB_CELL_FORCE>;
+      break;
+    case FSI_BATCH_OP_LBM_SUBSTEP_NONE:
+      k.fn = t == 0 ? (const void*)&kb_substep<FSI_SUB_STORED_MOMENTS> : (const void*)&kb_substep<0u>;
+      break;
     case FSI_BATCH_OP_LBM_REBUILD: k.fn = (const void*)&kb_rebuild; break;
     case FSI_BATCH_OP_LBM_MOMENTS: k.fn = (const void*)&kb_moments; break;
     case FSI_BATCH_OP_LBM_MB_BEGIN: k.fn = (const void*)&kb_mb_begin; break;
@@ -1768,6 +1896,7 @@ int batch_kernel(int op, int t, BatchKernel& k) {
     case FSI_BATCH_OP_MPM_G2P: k.fn = (const void*)&kb_mpm_g2p; k.block = PTPB; break;
     case FSI_BATCH_OP_MPM_P2G_REACT: k.fn = (const void*)&kb_mpm_p2g_react; k.block = PTPB; break;
     case FSI_BATCH_OP_MPM_GRID_UPDATE_REACT: k.fn = (const void*)&kb_mpm_grid_update_react; break;
+    case FSI_BATCH_OP_BOX_ROLL: k.fn = (const void*)&kb_box_roll; k.block = 64; break;
     default: return fail(FSI_E_INVALID, "unknown batch operation");
   }
   return FSI_OK;
@@ -2321,7 +2450,7 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
 int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domain, int mode_mask, int min_mpm,
                         int32_t* map_host) {
   if (!inst_host || n_inst < 0) return fail(FSI_E_INVALID, "null batch arguments");
-  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_INSTANCES)
+  if (domain < FSI_BATCH_DOMAIN_CELLS || domain > FSI_BATCH_DOMAIN_BOX)
     return fail(FSI_E_INVALID, "unknown batch domain");
   long long total = 0;
   for (int i = 0; i < n_inst; ++i) {

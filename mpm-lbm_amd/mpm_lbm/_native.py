@@ -181,11 +181,15 @@ BATCH_DOMAIN_NODES = 1
 BATCH_DOMAIN_PARTICLES = 2
 BATCH_DOMAIN_PLANES = 3
 BATCH_DOMAIN_INSTANCES = 4
+BATCH_DOMAIN_BOX = 5
+BATCH_BOX_BLOCKS = 64  # FSI_BATCH_BOX_BLOCKS
 (BATCH_OP_LBM_SUBSTEP, BATCH_OP_LBM_REBUILD, BATCH_OP_LBM_MOMENTS, BATCH_OP_LBM_MB_BEGIN, BATCH_OP_LBM_MB_FINALIZE,
  BATCH_OP_PROJ_CLEAR, BATCH_OP_PROJ_PARTICLES, BATCH_OP_PROJ_NORMALIZE, BATCH_OP_PENALTY, BATCH_OP_MPM_CLEAR,
  BATCH_OP_MPM_P2G, BATCH_OP_MPM_REACTION, BATCH_OP_MPM_GRID_UPDATE, BATCH_OP_MPM_G2P,
  BATCH_OP_COUNTERS_CLEAR, BATCH_OP_MPM_P2G_REACT, BATCH_OP_MPM_GRID_UPDATE_REACT, BATCH_OP_LBM_SUBSTEP_MB,
- BATCH_OP_LBM_SUBSTEP_PENALTY, BATCH_OP_LBM_SUBSTEP_NONE, BATCH_OP_LBM_MB_FINALIZE_FOLD) = range(21)
+ BATCH_OP_LBM_SUBSTEP_PENALTY, BATCH_OP_LBM_SUBSTEP_NONE, BATCH_OP_LBM_MB_FINALIZE_FOLD,
+ BATCH_OP_BOX_ROLL) = range(22)
+BATCH_BOX_INTS = 32  # FSI_BATCH_BOX_INTS
 
 # domain of each batched operation (the block map a launch takes is built for it)
 BATCH_OP_DOMAIN = {
@@ -193,14 +197,14 @@ BATCH_OP_DOMAIN = {
     BATCH_OP_LBM_SUBSTEP_PENALTY: BATCH_DOMAIN_PLANES, BATCH_OP_LBM_SUBSTEP_NONE: BATCH_DOMAIN_PLANES,
     BATCH_OP_LBM_REBUILD: BATCH_DOMAIN_CELLS, BATCH_OP_LBM_MOMENTS: BATCH_DOMAIN_CELLS,
     BATCH_OP_LBM_MB_BEGIN: BATCH_DOMAIN_CELLS, BATCH_OP_LBM_MB_FINALIZE: BATCH_DOMAIN_CELLS,
-    BATCH_OP_PROJ_CLEAR: BATCH_DOMAIN_CELLS, BATCH_OP_PROJ_NORMALIZE: BATCH_DOMAIN_CELLS,
-    BATCH_OP_PENALTY: BATCH_DOMAIN_CELLS,
-    BATCH_OP_MPM_CLEAR: BATCH_DOMAIN_NODES, BATCH_OP_MPM_GRID_UPDATE: BATCH_DOMAIN_NODES,
-    BATCH_OP_MPM_GRID_UPDATE_REACT: BATCH_DOMAIN_NODES,
+    BATCH_OP_PROJ_CLEAR: BATCH_DOMAIN_BOX, BATCH_OP_PROJ_NORMALIZE: BATCH_DOMAIN_BOX,
+    BATCH_OP_PENALTY: BATCH_DOMAIN_BOX,
+    BATCH_OP_MPM_CLEAR: BATCH_DOMAIN_BOX, BATCH_OP_MPM_GRID_UPDATE: BATCH_DOMAIN_BOX,
+    BATCH_OP_MPM_GRID_UPDATE_REACT: BATCH_DOMAIN_BOX,
     BATCH_OP_PROJ_PARTICLES: BATCH_DOMAIN_PARTICLES, BATCH_OP_MPM_P2G: BATCH_DOMAIN_PARTICLES,
     BATCH_OP_MPM_REACTION: BATCH_DOMAIN_PARTICLES, BATCH_OP_MPM_P2G_REACT: BATCH_DOMAIN_PARTICLES,
     BATCH_OP_MPM_G2P: BATCH_DOMAIN_PARTICLES,
-    BATCH_OP_LBM_MB_FINALIZE_FOLD: BATCH_DOMAIN_INSTANCES,
+    BATCH_OP_LBM_MB_FINALIZE_FOLD: BATCH_DOMAIN_INSTANCES, BATCH_OP_BOX_ROLL: BATCH_DOMAIN_INSTANCES,
 }
 
 
@@ -209,7 +213,7 @@ class BatchInstance(ctypes.Structure):
         ("lp", LbmParams), ("ls", LbmState), ("mp", MpmParams), ("ms", MpmState),
         ("cp", CouplingParams), ("cc", vp),
         ("mode", i32), ("cur0", i32), ("n_substeps", i32), ("mpm_substeps", i32), ("active", i32),
-        ("cell_force", i32), ("pad0", i32), ("react_ws", vp),
+        ("cell_force", i32), ("pad0", i32), ("react_ws", vp), ("boxes", vp),
     ]
 
 

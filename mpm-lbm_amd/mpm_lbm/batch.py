@@ -125,6 +125,7 @@ class BatchedFSI:
         self.drivers = [None] * self.capacity
         self._counters = [None] * self.capacity
         self._react_ws = [None] * self.capacity
+        self._boxes = [None] * self.capacity
         self._inst_host = (_native.BatchInstance * self.capacity)()
         nbytes = ctypes.sizeof(self._inst_host)
         self._inst_dev = torch.zeros(nbytes, dtype=torch.uint8, device=self.device)
@@ -159,6 +160,8 @@ class BatchedFSI:
         self._counters[k] = struct_buffer(_native.CouplingCounters, self.device)
         self._react_ws[k] = torch.zeros(max(1, _native.coupled_workspace_doubles(d.solid.n_particles)),
                                         dtype=torch.float64, device=self.device)
+        # zero: not valid, the first passes cover the whole lattice / grid
+        self._boxes[k] = torch.zeros(_native.BATCH_BOX_INTS, dtype=torch.int32, device=self.device)
         I = self._inst_host[k]
         I.lp = lbm._params
         I.ls = lbm._state
@@ -174,6 +177,7 @@ class BatchedFSI:
         I.active = 1
         I.cell_force = 1 if d.config.coupling_mode == "penalty" else 0
         I.react_ws = _native.ptr(self._react_ws[k])
+        I.boxes = _native.ptr(self._boxes[k])
         lbm._cell_force_active = bool(I.cell_force)
         self.drivers[k] = d
         self._dirty = True
@@ -202,6 +206,7 @@ class BatchedFSI:
         self.drivers[k] = None
         self._counters[k] = None
         self._react_ws[k] = None
+        self._boxes[k] = None
         self._dirty = True
 
     @property
@@ -274,6 +279,7 @@ class BatchedFSI:
         """MPM substeps with each instance's reaction (fsi_mpm_substep_coupled per instance)."""
         max_m = max(I.mpm_substeps for I in self._live_inst() if I.mode in modes)
         for j in range(max_m):
+            self._launch(_native.BATCH_OP_BOX_ROLL, modes, t=1, min_mpm=j)
             self._launch(_native.BATCH_OP_MPM_CLEAR, modes, min_mpm=j)
             self._launch(_native.BATCH_OP_MPM_P2G_REACT, modes, min_mpm=j)
             self._launch(_native.BATCH_OP_MPM_GRID_UPDATE_REACT, modes, min_mpm=j)
@@ -281,6 +287,7 @@ class BatchedFSI:
 
     def _project(self, modes):
         self._clear_counters(0)
+        self._launch(_native.BATCH_OP_BOX_ROLL, modes, t=0)
         self._launch(_native.BATCH_OP_PROJ_CLEAR, modes)
         self._launch(_native.BATCH_OP_PROJ_PARTICLES, modes)
         self._launch(_native.BATCH_OP_PROJ_NORMALIZE, modes)
@@ -298,11 +305,13 @@ class BatchedFSI:
             if has_one:
                 self._project(ONE)
                 self._clear_counters(1)
-                self._launch(_native.BATCH_OP_PENALTY, PEN)
+                self._launch(_native.BATCH_OP_PENALTY, PEN, t=t)  # stores the moments it reads (t >= 1)
             self._issue_substep(t)
             if has_one:
-                self._launch(_native.BATCH_OP_LBM_MOMENTS, ONE, t=t)
                 self._mpm_substeps(ONE)
+        if has_one:
+            # no moments pass between the steps (batch_moment_source): one after the last
+            self._launch(_native.BATCH_OP_LBM_MOMENTS, ONE, t=self.n_substeps - 1)
         if has_mb:
             self._launch(_native.BATCH_OP_LBM_MB_FINALIZE, MB)
             self._launch(_native.BATCH_OP_LBM_MB_FINALIZE_FOLD, MB)

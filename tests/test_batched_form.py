@@ -59,6 +59,8 @@ def _expected_map(inst, domain, modes, min_mpm):
             continue
         if domain == _native.BATCH_DOMAIN_INSTANCES:
             nb = 1
+        elif domain == _native.BATCH_DOMAIN_BOX:
+            nb = min(_native.BATCH_BOX_BLOCKS, -(-(I.lp.nx * I.lp.ny * I.lp.nz) // 256))
         elif domain == _native.BATCH_DOMAIN_PLANES:
             nb = I.lp.nx * (-(-(I.lp.ny * I.lp.nz) // 256))
         elif domain == _native.BATCH_DOMAIN_CELLS:
@@ -81,7 +83,7 @@ def test_block_map_helper_matches_python(emu, tmp_path):
     batch._inst_host[2].mpm_substeps = 1  # exercise the min_mpm filter
     addr = ctypes.addressof(batch._inst_host)
     for domain in (_native.BATCH_DOMAIN_CELLS, _native.BATCH_DOMAIN_NODES, _native.BATCH_DOMAIN_PARTICLES,
-                   _native.BATCH_DOMAIN_PLANES, _native.BATCH_DOMAIN_INSTANCES):
+                   _native.BATCH_DOMAIN_PLANES, _native.BATCH_DOMAIN_INSTANCES, _native.BATCH_DOMAIN_BOX):
         for modes in ((_native.BATCH_MODE_MOVING_BOUNDARY,), (_native.BATCH_MODE_PENALTY, _native.BATCH_MODE_NONE),
                       (0, 1, 2)):
             for min_mpm in (0, 1):
