@@ -2512,18 +2512,29 @@ FSI_HD bool planes_meet_box(const int32_t* r, long long i0, long long i1) { retu
 // the batched exchange runs no moments pass between its LBM steps.  A
 // penalty instance's PENALTY pass has stored the moments (with the previous
 // step's cell force, as the pass after the previous step would have) in the
-// box it covers, the hull of the projection boxes; everywhere else the cell
-// force of both steps is zero and the moments of F_src are the same values.
-// A none instance takes the moments of F_src everywhere.  Sets the flags of
-// cell (i, j, k) and of the cell behind it (the pressure outlet reads that).
+// box it covers, the hull of the projection boxes, and in the plane the
+// pressure outlet reads (i = nx - 2); everywhere else the cell force of both
+// steps is zero and the moments of F_src are the same values.  A none
+// instance takes the moments of F_src everywhere.  Whether penalty cell
+// (i, j, k) of substep t >= 1 takes the stored moments:
+FSI_HD bool batch_penalty_stored(const FsiBatchInstance& I, int i, int j, int k) {
+  // the hull's bounds one at a time (uniform values; no array to keep)
+  const int32_t* c = I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR;
+  const int32_t* q = I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV;
+  const int lo0 = c[0] < q[0] ? c[0] : q[0], hi0 = c[3] > q[3] ? c[3] : q[3];
+  const int lo1 = c[1] < q[1] ? c[1] : q[1], hi1 = c[4] > q[4] ? c[4] : q[4];
+  const int lo2 = c[2] < q[2] ? c[2] : q[2], hi2 = c[5] > q[5] ? c[5] : q[5];
+  return lo0 <= i && i <= hi0 && lo1 <= j && j <= hi1 && lo2 <= k && k <= hi2;
+}
+
+// The flags of cell (i, j, k) of instance I at substep t and the flags its
+// pressure-outlet reset reads the cell behind it with.
 FSI_HD void batch_moment_source(const FsiBatchInstance& I, int t, uint32_t& flags, uint32_t& outlet_flags, int i,
                                 int j, int k) {
   outlet_flags = flags;
   if (t == 0 || I.mode != FSI_BATCH_MODE_PENALTY) return;
-  int32_t both[6];
-  box_hull(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, both);
-  if (in_box(both, i, j, k)) flags |= FSI_SUB_STORED_MOMENTS;
-  if (in_box(both, i - 1, j, k)) outlet_flags |= FSI_SUB_STORED_MOMENTS;
+  if (batch_penalty_stored(I, i, j, k)) flags |= FSI_SUB_STORED_MOMENTS;
+  outlet_flags |= FSI_SUB_STORED_MOMENTS;
 }
 
 }  // namespace fsi

@@ -561,6 +561,15 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
       case FSI_BATCH_OP_PENALTY: {
         int32_t both[6];
         box_hull(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, both);
+        if (t > 0 && I.lp.bc_x_right == FSI_BC_PRESSURE) {  // the plane the pressure outlet reads
+          const long long plane = (long long)I.lp.ny * I.lp.nz, i0 = (long long)(I.lp.nx - 2) * plane;
+          for (long long b = 0; b < plane; ++b)
+            if (!batch_penalty_stored(I, I.lp.nx - 2, (int)(b / I.lp.nz), (int)(b % I.lp.nz)))
+              by_storage(I.lp.storage, [&](auto tag) {
+                using Pop = decltype(tag);
+                lbm_moments_cell<Pop>(I.lp, I.ls, batch_cur(I, t), FSI_SUB_CELL_FORCE, i0 + b);
+              });
+        }
         for (long long c = 0; c < cells; ++c) {
           if (!cell_in(both, c, I.lp.ny, I.lp.nz)) continue;
           if (t > 0)

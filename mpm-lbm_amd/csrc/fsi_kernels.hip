@@ -1472,7 +1472,10 @@ __global__ void __launch_bounds__(TPB) k_mpm_stats_partial(FsiMpmParams mp, FsiM
 }
 
 // ---------------------------------------------------------------------------
-// Batched kernels.  Dispatch slot blockIdx.x takes entry b of the block map
+// Batched kernels.  The instance table and the block map are read-only for
+// the launch (__restrict__): the descriptor fields a workgroup reads can stay
+// in scalar registers across the stores it makes, instead of being loaded
+// again after each one.  Dispatch slot blockIdx.x takes entry b of the block map
 // (b = xcd_tile(...): each XCD gets one contiguous run of the map, i.e. of
 // whole instances), which names the instance and the workgroup's index L in
 // that instance's domain; its local work item is L * (block size) + thread.
@@ -1548,7 +1551,7 @@ __device__ __forceinline__ void stencil_base(const float* x, long long P, long l
 // from each instance at run time (any mix of modes).  A penalty instance's
 // substeps after the first take STORED_MOMENTS per cell (batch_moment_source).
 template <uint32_t FIXED>
-__global__ void __launch_bounds__(TPB, 5) kb_substep(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
+__global__ void __launch_bounds__(TPB, 5) kb_substep(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap, int t) {
   const unsigned b_ = xcd_tile(blockIdx.x, gridDim.x);
   const int i_ = bmap[2 * b_], l_ = bmap[2 * b_ + 1];
   const FsiBatchInstance& I = inst[i_];
@@ -1564,8 +1567,13 @@ __global__ void __launch_bounds__(TPB, 5) kb_substep(const FsiBatchInstance* ins
   const unsigned j = jk / (unsigned)p.nz, k = jk - j * (unsigned)p.nz;
   constexpr uint32_t VAR = FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE;
   uint32_t flags = FIXED == ~0u ? batch_substep_flags(I, t) : ((batch_substep_flags(I, t) & ~VAR) | FIXED);
-  uint32_t oflags;
-  batch_moment_source(I, t, flags, oflags, (int)i, (int)j, (int)k);
+  uint32_t oflags = flags;
+  if (FIXED == ~0u) {
+    batch_moment_source(I, t, flags, oflags, (int)i, (int)j, (int)k);
+  } else if (FIXED == FSI_SUB_CELL_FORCE) {  // the penalty launch of substeps t >= 1
+    oflags = flags | FSI_SUB_STORED_MOMENTS;
+    if (batch_penalty_stored(I, (int)i, (int)j, (int)k)) flags |= FSI_SUB_STORED_MOMENTS;
+  }
   const int cur = batch_cur(I, t), parity = t & 1;
   if (p.storage == FSI_STORE_F16_DEV)
     lbm_substep_cell<PopF16, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k, oflags);
@@ -1575,14 +1583,14 @@ __global__ void __launch_bounds__(TPB, 5) kb_substep(const FsiBatchInstance* ins
     lbm_substep_cell<PopF32, true>(p, st, cur, parity, flags, (int)i, (int)j, (int)k, oflags);
 }
 
-__global__ void __launch_bounds__(TPB) kb_rebuild(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_rebuild(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   if (!in_) return;
   const int dyn = I.mode == FSI_BATCH_MODE_MOVING_BOUNDARY ? 1 : 0;
   FSI_BY_STORAGE(I.lp.storage, lbm_rebuild_cell<POP>(I.lp, I.ls, dyn, I.cur0, e_));
 }
 
-__global__ void __launch_bounds__(TPB) kb_moments(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
+__global__ void __launch_bounds__(TPB) kb_moments(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap, int t) {
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   if (!in_) return;
   // moments after substep t: the populations are in F[cur0 ^ ((t + 1) & 1)]
@@ -1590,7 +1598,7 @@ __global__ void __launch_bounds__(TPB) kb_moments(const FsiBatchInstance* inst, 
   FSI_BY_STORAGE(I.lp.storage, lbm_moments_cell<POP>(I.lp, I.ls, batch_cur(I, t + 1), flags, e_));
 }
 
-__global__ void __launch_bounds__(TPB) kb_mb_begin(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_mb_begin(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   if (e_ == 0) {
     FsiLbmCounters* k = I.ls.counters;
@@ -1604,7 +1612,7 @@ __global__ void __launch_bounds__(TPB) kb_mb_begin(const FsiBatchInstance* inst,
   if (in_) lbm_mb_begin_cell(I.lp, I.ls, e_);
 }
 
-__global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
@@ -1621,7 +1629,7 @@ __global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* inst,
 }
 
 // one workgroup per instance (INSTANCES domain): fold the finalize records
-__global__ void __launch_bounds__(TPB) kb_finalize_fold(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_finalize_fold(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   __shared__ double sd[TPB / 64];
   const FsiBatchInstance& I = inst[bmap[2 * blockIdx.x]];
   const int n_rec = (int)batch_domain_blocks(I, FSI_BATCH_DOMAIN_CELLS);
@@ -1629,7 +1637,7 @@ __global__ void __launch_bounds__(TPB) kb_finalize_fold(const FsiBatchInstance* 
 }
 
 // one workgroup per instance (INSTANCES domain); t = 0: projection box, 1: MPM box
-__global__ void __launch_bounds__(64) kb_box_roll(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
+__global__ void __launch_bounds__(64) kb_box_roll(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap, int t) {
   const FsiBatchInstance& I = inst[bmap[2 * blockIdx.x]];
   if (threadIdx.x != 0) return;
   if (t == 0)
@@ -1663,7 +1671,7 @@ __global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* 
   }
 }
 
-__global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   FSI_BATCH_BOX_PROLOGUE
   const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, I.lp.nx, I.lp.ny, I.lp.nz);
   const long long N = batch_domain_size(I, FSI_BATCH_DOMAIN_CELLS);
@@ -1675,7 +1683,7 @@ __global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* ins
   }
 }
 
-__global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   __shared__ __align__(16) float tile[TILE_COPIES * 5 * PROJ_TILE_NODES];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   int base[3] = {0, 0, 0};
@@ -1684,7 +1692,7 @@ __global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance
   proj_block(I.lp, I.ls, I.mp, I.ms, I.cp, e_, in_, tile);
 }
 
-__global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
@@ -1719,7 +1727,7 @@ __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance*
 // Penalty pass of LBM step t.  From t = 1 on it first stores the moments of
 // the cells it covers, from the populations after the last step and that
 // step's cell force, as the moments pass after that step would have.
-__global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, const int32_t* bmap, int t) {
+__global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap, int t) {
   __shared__ double sd[TPB / 64];
   __shared__ float sf[TPB / 64];
   __shared__ int si[TPB / 64];
@@ -1730,6 +1738,16 @@ __global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, 
   float mfn = 0.0f;
   int act = 0;
   double net[3] = {0.0, 0.0, 0.0};
+  if (t > 0 && I.lp.bc_x_right == FSI_BC_PRESSURE) {
+    // the plane the pressure outlet reads, outside the hull: its moments too
+    const long long plane = (long long)I.lp.ny * I.lp.nz, i0 = (long long)(I.lp.nx - 2) * plane;
+    for (long long b = (long long)l_ * TPB + threadIdx.x; b < plane; b += stride_) {
+      const int j = (int)(b / I.lp.nz), k = (int)(b % I.lp.nz);
+      if (!batch_penalty_stored(I, I.lp.nx - 2, j, k))
+        FSI_BY_STORAGE(I.lp.storage,
+                       lbm_moments_cell<POP>(I.lp, I.ls, batch_cur(I, t), FSI_SUB_CELL_FORCE, i0 + b));
+    }
+  }
   for (long long b = (long long)l_ * TPB + threadIdx.x; b < w.n; b += stride_) {
     const long long c = box_item(w, b, I.lp.ny, I.lp.nz);
     if (t > 0) FSI_BY_STORAGE(I.lp.storage, lbm_moments_cell<POP>(I.lp, I.ls, batch_cur(I, t), FSI_SUB_CELL_FORCE, c));
@@ -1759,7 +1777,7 @@ __global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* inst, 
   }
 }
 
-__global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   FSI_BATCH_BOX_PROLOGUE
   const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_MPM + BOX_PREV, I.mp.gnx, I.mp.gny, I.mp.gnz);
   const long long G = batch_domain_size(I, FSI_BATCH_DOMAIN_NODES);
@@ -1771,7 +1789,7 @@ __global__ void __launch_bounds__(TPB) kb_mpm_clear(const FsiBatchInstance* inst
   }
 }
 
-__global__ void __launch_bounds__(PTPB) kb_mpm_p2g(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(PTPB) kb_mpm_p2g(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   __shared__ float tile[TILE_COPIES * 4 * TILE_NODES];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   int base[3] = {0, 0, 0};
@@ -1780,7 +1798,7 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_p2g(const FsiBatchInstance* inst,
   p2g_block(I.mp, I.ms, e_, in_, tile);
 }
 
-__global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   __shared__ double sd[PTPB / 64];
   __shared__ float sf[PTPB / 64];
   __shared__ int si[PTPB / 64];
@@ -1807,7 +1825,7 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_reaction(const FsiBatchInstance* 
   }
 }
 
-__global__ void __launch_bounds__(PTPB) kb_mpm_p2g_react(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(PTPB) kb_mpm_p2g_react(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   __shared__ __align__(16) float tile[TILE_COPIES * 4 * FUSED_TILE_NODES];
   __shared__ double sd[PTPB / 64];
   __shared__ float sf[PTPB / 64];
@@ -1830,14 +1848,14 @@ __global__ void __launch_bounds__(PTPB) kb_mpm_p2g_react(const FsiBatchInstance*
     p2g_react_block<0>(I.lp, I.ls, I.mp, I.ms, I.cp, rec, e_, in_, tile, sd, sf, si);
 }
 
-__global__ void __launch_bounds__(TPB) kb_mpm_grid_update(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_mpm_grid_update(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   FSI_BATCH_BOX_PROLOGUE
   const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_MPM + BOX_CUR, I.mp.gnx, I.mp.gny, I.mp.gnz);
   for (long long b = (long long)l_ * TPB + threadIdx.x; b < w.n; b += stride_)
     mpm_grid_node(I.mp, I.ms, box_item(w, b, I.mp.gny, I.mp.gnz));
 }
 
-__global__ void __launch_bounds__(TPB) kb_mpm_grid_update_react(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(TPB) kb_mpm_grid_update_react(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   __shared__ double sd[TPB / 64];
   FSI_BATCH_BOX_PROLOGUE
   const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_MPM + BOX_CUR, I.mp.gnx, I.mp.gny, I.mp.gnz);
@@ -1847,7 +1865,7 @@ __global__ void __launch_bounds__(TPB) kb_mpm_grid_update_react(const FsiBatchIn
     fold_react_records(I.react_ws, (int)((I.mp.n_particles + PTPB - 1) / PTPB), I.cc, sd);
 }
 
-__global__ void __launch_bounds__(PTPB) kb_mpm_g2p(const FsiBatchInstance* inst, const int32_t* bmap) {
+__global__ void __launch_bounds__(PTPB) kb_mpm_g2p(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_PARTICLES, PTPB)
   if (in_) mpm_g2p_particle(I.mp, I.ms, e_);
 }
