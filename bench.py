@@ -333,10 +333,13 @@ def main():
     from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
 
     n, sub = args.n, args.substeps
-    w = workloads.moving_boundary_flap(n=n, steps=args.warmup + args.steps, substeps=sub, out_dir="/tmp/fsi_bench")
+    w = workloads.moving_boundary_flap(n=n, steps=args.warmup + args.steps + 1, substeps=sub, out_dir="/tmp/fsi_bench")
     log(f"[bench] workload {w.name}: {w.config.n_particles} particles, {sub} LBM substeps/FSI step")
     d = FSIDriver3D(w.config, "/tmp/fsi_bench/run", particle_cloud=w.cloud, write_geometry_file=False)
     d.initialize()
+    d.step_once()  # one eager FSI step: the flags and walls of the run
+    launch_tuning = d.tune_launch()
+    log(f"[bench] substep launch tuning: {launch_tuning}")
     for i in range(args.warmup):
         d.step_once()
     torch.cuda.synchronize()
@@ -426,6 +429,7 @@ def main():
             "particle_updates_per_s": round(w.config.n_particles * w.config.mpm_substeps_per_lbm_step * args.steps / secs, 1),
             "copy_ceiling_gbs": round(copy_gbs, 1),
             "substep_frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
+            "substep_launch_tuning": launch_tuning,
             "batched": batched,
             "served": served,
             "precision": precision,

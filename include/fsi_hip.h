@@ -81,6 +81,17 @@ typedef struct FsiLbmParams {
                                apply the population floor when it is set */
   int32_t ob_floor_enabled; /* open_boundary_population_floor is set */
   float ob_rho_min, ob_rho_max, ob_u_max, ob_noneq_cap, ob_floor;
+  /* Launch shape of fsi_lbm_substep (results do not depend on it; an
+   * autotuner picks it per lattice, all 0 = the built-in choice):
+   * launch_block  cells per workgroup, a run of the (j,k) plane of one
+   *               i-plane: 128, 256 or 512 (0 = 256; FP32 storage only
+   *               takes other than 256);
+   * launch_minw   waves per SIMD the kernel's register budget allows: 1 (no
+   *               cap) or 5 (capped, populations loaded with the flags);
+   *               0 = 1 when the grid fits the chip in one wave generation;
+   * launch_linear 1 = workgroups take tiles in dispatch order instead of
+   *               one contiguous slab of i-planes per XCD. */
+  int32_t launch_block, launch_minw, launch_linear, launch_pad;
 } FsiLbmParams;
 
 /* Device-side counters of the LBM (all reductions are order-independent:

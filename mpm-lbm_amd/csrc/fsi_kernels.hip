@@ -525,12 +525,12 @@ __global__ void __launch_bounds__(TPB) k_lbm_rebuild(FsiLbmParams p, FsiLbmState
 // index so that the four SIMDs share the wall work.  That wave issues the
 // queued cells' link loads together with its own cells' population loads,
 // so it still waits for memory once.
-template <typename Pop, uint32_t FIXED, int MINW>
-__global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
-                                                           uint32_t flags, unsigned tiles_per_plane) {
-  const unsigned L = xcd_tile(blockIdx.x, gridDim.x);
+template <typename Pop, uint32_t FIXED, int MINW, int BS>
+__global__ void __launch_bounds__(BS, MINW) k_lbm_substep(FsiLbmParams p, FsiLbmState st, int cur, int parity,
+                                                          uint32_t flags, unsigned tiles_per_plane) {
+  const unsigned L = p.launch_linear ? blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
   const unsigned i = L / tiles_per_plane;
-  const unsigned jk = (L - i * tiles_per_plane) * TPB + threadIdx.x;
+  const unsigned jk = (L - i * tiles_per_plane) * BS + threadIdx.x;
   const unsigned plane = (unsigned)p.ny * (unsigned)p.nz;
   const uint32_t fl = (flags & ~(FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE)) | FIXED;
   if (MINW > 1 || !(FIXED & FSI_SUB_MOVING_WALL) || !(fl & FSI_SUB_GATHER_PREV)) {
@@ -540,8 +540,8 @@ __global__ void __launch_bounds__(TPB, MINW) k_lbm_substep(FsiLbmParams p, FsiLb
     lbm_substep_cell<Pop, (MINW > 1)>(p, st, cur, parity, fl, (int)i, (int)j, (int)k);
     return;
   }
-  constexpr unsigned WAVES = TPB / 64;
-  __shared__ uint32_t q_jk[TPB], q_info[TPB];
+  constexpr unsigned WAVES = BS / 64;
+  __shared__ uint32_t q_jk[BS], q_info[BS];
   __shared__ uint32_t q_n[WAVES];
   const uint32_t N = (uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)p.nz;
   const bool in = jk < plane;
@@ -2107,18 +2107,34 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
   if (mb && (flags & FSI_SUB_DIAG) && !st->link_corr)
     return fail(FSI_E_INVALID, "diagnostic substep needs link_corr");
   if ((flags & FSI_SUB_CELL_FORCE) && !st->cell_force) return fail(FSI_E_INVALID, "cell_force is null");
-  const unsigned tiles = blocks_for((long long)p->ny * p->nz);
-  const dim3 grid(tiles * (unsigned)p->nx), block(TPB);
+  const int bs = p->launch_block ? p->launch_block : TPB;
+  if (bs != 128 && bs != 256 && bs != 512) return fail(FSI_E_INVALID, "launch_block must be 128, 256 or 512");
+  if (bs != TPB && p->storage != FSI_STORE_F32)
+    return fail(FSI_E_INVALID, "launch_block other than 256 is built for FP32 storage only");
+  if (p->launch_minw != 0 && p->launch_minw != 1 && p->launch_minw != 5)
+    return fail(FSI_E_INVALID, "launch_minw must be 0, 1 or 5");
+  const unsigned tiles = (unsigned)(((long long)p->ny * p->nz + bs - 1) / bs);
+  const dim3 grid(tiles * (unsigned)p->nx), block(bs);
   hipStream_t s = as_stream(stream);
   const uint32_t fixed = flags & (FSI_SUB_STORED_MOMENTS | FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE);
-  const bool one_generation = (long long)grid.x * (TPB / 64) <= 4LL * simds_on_device();
+  const bool one_generation = (long long)grid.x * (bs / 64) <= 4LL * simds_on_device();
+  const bool uncapped = p->launch_minw ? p->launch_minw == 1 : one_generation;
   switch (fixed) {
-#define FSI_CASE(V)                                                                   \
-  case (V):                                                                           \
-    if (one_generation)                                                               \
-      FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V), 1>), grid, block, 0, s, *p, *st, cur, parity, flags, tiles)); \
-    else                                                                              \
-      FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V), 5>), grid, block, 0, s, *p, *st, cur, parity, flags, tiles)); \
+#define FSI_LAUNCH(V, W)                                                                                         \
+  if (bs == 128)                                                                                                 \
+    hipLaunchKernelGGL((k_lbm_substep<PopF32, (V), W, 128>), grid, block, 0, s, *p, *st, cur, parity, flags, tiles); \
+  else if (bs == 512)                                                                                            \
+    hipLaunchKernelGGL((k_lbm_substep<PopF32, (V), W, 512>), grid, block, 0, s, *p, *st, cur, parity, flags, tiles); \
+  else                                                                                                           \
+    FSI_BY_STORAGE(p->storage, hipLaunchKernelGGL((k_lbm_substep<POP, (V), W, 256>), grid, block, 0, s, *p, *st, cur, \
+                                                  parity, flags, tiles));
+#define FSI_CASE(V)        \
+  case (V):                \
+    if (uncapped) {        \
+      FSI_LAUNCH(V, 1)     \
+    } else {               \
+      FSI_LAUNCH(V, 5)     \
+    }                      \
     break;
     FSI_CASE(0u)
     FSI_CASE(1u)
@@ -2129,6 +2145,7 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
     FSI_CASE(18u)
     FSI_CASE(19u)
 #undef FSI_CASE
+#undef FSI_LAUNCH
     default:
       return fail(FSI_E_INVALID, "unsupported substep flag combination");
   }

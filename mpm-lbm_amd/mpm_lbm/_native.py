@@ -62,6 +62,7 @@ class LbmParams(ctypes.Structure):
         ("storage", i32), ("collision", i32),
         ("open_bc", i32), ("ob_limiter", i32), ("ob_floor_enabled", i32),
         ("ob_rho_min", f32), ("ob_rho_max", f32), ("ob_u_max", f32), ("ob_noneq_cap", f32), ("ob_floor", f32),
+        ("launch_block", i32), ("launch_minw", i32), ("launch_linear", i32), ("launch_pad", i32),
     ]
 
 

@@ -416,6 +416,21 @@ class FSIDriver3D:
         self._advance_mpm_with_moving_boundary_reaction()
         self._eager_subcycled_steps += 1
 
+    def tune_launch(self, cache_path=None, launches: int = 20):
+        """Pick the fused substep's launch shape for this lattice by timing the
+        candidates on its live state (mpm_lbm.tuning.tune_substep; results do
+        not change).  Call after a first step, so the flags and walls are
+        those of the run; a captured step graph is dropped and captured again
+        with the new shape.  No reference counterpart."""
+        from ... import tuning
+
+        if not self.initialized:
+            self.initialize()
+        rec = tuning.tune_substep(self.lbm, tuning.driver_substep_flags(self), launches=launches,
+                                  cache_path=cache_path)
+        self._drop_step_graph()
+        return rec
+
     def _drop_step_graph(self):
         """Forget the captured FSI-step graph (its launches hold the state the
         driver had at capture); the next step runs eagerly and recaptures."""
