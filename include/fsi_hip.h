@@ -32,6 +32,7 @@ extern "C" {
 #define FSI_OK 0
 #define FSI_E_INVALID (-1)  /* bad argument or shape */
 #define FSI_E_HIP (-2)      /* HIP runtime error */
+#define FSI_E_UNSUPPORTED (-3) /* valid arguments outside what the entry point covers (the caller falls back) */
 
 /* Boundary-condition codes of LBMConfig.bc_x_* (lbm/config.py:61-66). */
 #define FSI_BC_PERIODIC 0
@@ -371,6 +372,32 @@ int fsi_lbm_rebuild(const FsiLbmParams* p, const FsiLbmState* st, int dynamic, i
  * deterministically.  `parity` selects link_c[parity] for writing. */
 int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int parity,
                     uint32_t flags, void* stream);
+
+/* The moving-wall subcycle of FSIDriver3D (fsi_driver.py:533-550: n_substeps
+ * x step_moving_bounceback, fluid.py:489-519, with the per-substep
+ * accumulate_moving_boundary_force_sample, :469-473) in one launch that
+ * keeps the lattice on chip: the same results as n calls of
+ * fsi_lbm_substep(cur + t, parity t % 2, flags_t) for t = 0 .. n-1, where
+ * flags_t = the given flags with FSI_SUB_STORED_MOMENTS on t = 0 only,
+ * FSI_SUB_GATHER_PREV on t > 0 only and FSI_SUB_DIAG on t = n-1 only.  Reads
+ * F[cur], writes F[(cur + n) % 2], and leaves the last substep's links for
+ * fsi_lbm_mb_finalize(cur + n, parity (n-1) % 2).  `flags` must hold
+ * FSI_SUB_MOVING_WALL.
+ *
+ * Covers FSI_STORE_F32, FSI_OPEN_BC_EQUILIBRIUM, nz = 64, nx and ny multiples
+ * of 4, and at most as many blocks of 4 x 4 x 64 cells as the device holds
+ * workgroups of 1024 work-items at once; returns FSI_E_UNSUPPORTED otherwise
+ * (use fsi_lbm_substep).  halo_dev: FSI_RES_HALO_FLOATS(N) floats of scratch.
+ * sync_dev: FSI_RES_SYNC_WORDS(nx, ny) words, zero before the first call and
+ * kept between calls: word 64 b counts the substeps block b has finished,
+ * word 64 * FSI_RES_BLOCKS(nx, ny) counts waits for a neighbour block that
+ * timed out (the blocks then go on without it: nonzero means the results of
+ * that call are invalid, and every word must be cleared before the next). */
+#define FSI_RES_BLOCKS(nx, ny) (((nx) / 4) * ((ny) / 4))
+#define FSI_RES_SYNC_WORDS(nx, ny) (64 * (FSI_RES_BLOCKS(nx, ny) + 1))
+#define FSI_RES_HALO_FLOATS(n_cells) (2 * 18 * (n_cells))
+int fsi_lbm_subcycle_resident(const FsiLbmParams* p, const FsiLbmState* st, int cur, int n_substeps,
+                              uint32_t flags, float* halo_dev, uint32_t* sync_dev, void* stream);
 
 /* Boundary_condition on the x faces (fluid.py:538-570, :927-958,
  * :1027-1111, :1349-1493) for F[buf] in place: p->open_bc selects the legacy

@@ -712,6 +712,129 @@ FSI_HD Nbr nbr_of(const FsiLbmParams& p, int i, int j, int k) {
   return n;
 }
 
+// colission (fluid.py:399-413) of fluid cell c from its post-stream
+// populations F: the force is LBMConfig.force plus cell_force under
+// FSI_SUB_CELL_FORCE; the cell collides with the stored rho, vel under
+// FSI_SUB_STORED_MOMENTS, else with the streaming3 moments (fluid.py:1795-1814)
+// taken from the moment vector as in moments_of.  Returns the post-collision
+// populations f and the density and velocity used.
+FSI_HD void collide_cell(const FsiLbmParams& p, const FsiLbmState& st, uint32_t flags, uint32_t N, uint32_t c,
+                         const float* F, float* f, float& rho, float& ux, float& uy, float& uz) {
+  float gx = p.force[0], gy = p.force[1], gz = p.force[2];
+  if (flags & FSI_SUB_CELL_FORCE) {
+    gx += at32(st.cell_force, c);
+    gy += at32(st.cell_force + N, c);
+    gz += at32(st.cell_force + 2 * (size_t)N, c);
+  }
+  const bool has_force = (gx != 0.0f) || (gy != 0.0f) || (gz != 0.0f);
+  float m[Q];
+  moments_M(F, m);
+  if (flags & FSI_SUB_STORED_MOMENTS) {
+    rho = at32(st.rho, c);
+    ux = at32(st.vel, c);
+    uy = at32(st.vel + N, c);
+    uz = at32(st.vel + 2 * (size_t)N, c);
+  } else {
+    rho = m[0];
+    const float inv = 1.0f / rho;
+    ux = (m[3] + 0.5f * gx) * inv;
+    uy = (m[5] + 0.5f * gy) * inv;
+    uz = (m[7] + 0.5f * gz) * inv;
+  }
+  if (p.collision == FSI_COLLIDE_REGULARIZED)
+    regularized_collide(F, rho, ux, uy, uz, has_force, gx, gy, gz, p.s_v, f);
+  else
+    mrt_collide_m(F, m, ux, uy, uz, has_force, gx, gy, gz, p.s_v, p.s_other, f);
+}
+
+// Ladd moving-wall corrections of fluid cell c's links (fluid.py:489-519):
+// cr[s] = -6 w_s rho (e_s . u_wall(c + e_s)) on each link s of a cell next to
+// a moving wall, 0 otherwise (wall_corr of wall_dot, in that order).
+FSI_HD float wall_dot(int s, float wx, float wy, float wz) {
+  return (float)EX[s] * wx + (float)EY[s] * wy + (float)EZ[s] * wz;
+}
+FSI_HD float wall_corr(int s, float rho, float dot) { return -6.0f * wgt(s) * rho * dot; }
+
+// e_s . u_wall(c + e_s) of each link s of fluid cell c, into dot[1..18] (0 on
+// the other directions).  The wall velocities are loaded 6 links at a time,
+// each load branch-free (a direction without a link reads the cell itself),
+// so a cell next to the moving solid waits for memory 3 times rather than
+// once per link.
+FSI_HD void moving_wall_dots(const FsiLbmState& st, uint32_t N, uint32_t c, uint32_t info, const Nbr& nb,
+                             float* dot) {
+#pragma unroll
+  for (int g0 = 1; g0 < Q; g0 += 6) {
+    float w3[6][3];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const uint32_t y = (info & (1u << (g0 + q))) ? c + (uint32_t)nb.d(g0 + q) : c;
+      w3[q][0] = at32(st.solid_vel, y);
+      w3[q][1] = at32(st.solid_vel + N, y);
+      w3[q][2] = at32(st.solid_vel + 2 * (size_t)N, y);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int s = g0 + q;
+      dot[s] = (info & (1u << s)) ? wall_dot(s, w3[q][0], w3[q][1], w3[q][2]) : 0.0f;
+    }
+  }
+}
+
+FSI_HD void moving_wall_corrections(const FsiLbmState& st, uint32_t N, uint32_t c, uint32_t info, const Nbr& nb,
+                                    bool moving, float rho, float* cr) {
+#pragma unroll
+  for (int s = 0; s < Q; ++s) cr[s] = 0.0f;
+  if (!moving) return;
+  float dot[Q];
+  moving_wall_dots(st, N, c, info, nb, dot);
+#pragma unroll
+  for (int s = 1; s < Q; ++s) cr[s] = wall_corr(s, rho, dot[s]);
+}
+
+// Boundary_condition_legacy (fluid.py:927-958) of BC-face cell (i,j,k): the
+// density and velocity its 19 populations are reset to.  (ux,uy,uz): the
+// cell's own velocity (a left pressure face keeps it); (obx,oby,obz): that of
+// cell (nx-2, j, k) behind a right pressure face.  The right face is applied
+// last.
+FSI_HD void face_reset_state(const FsiLbmParams& p, const FsiLbmState& st, int i, int j, int k, float ux, float uy,
+                             float uz, float obx, float oby, float obz, float& brho, float& bx, float& by,
+                             float& bz) {
+  const Grid g = make_grid(p);
+  brho = 1.0f;
+  bx = by = bz = 0.0f;
+  if (i == 0 && p.bc_x_left != FSI_BC_PERIODIC) {
+    if (p.bc_x_left == FSI_BC_VELOCITY) {
+      brho = 1.0f;
+      bx = p.vel_bc_x_left[0];
+      by = p.vel_bc_x_left[1];
+      bz = p.vel_bc_x_left[2];
+    } else {
+      brho = p.rho_bc_x_left;
+      const long long c1 = cell_of(g, (g.nx > 1 ? 1 : 0), j, k);
+      if (st.cellinfo[c1] & CI_SOLID) {
+        bx = by = bz = 0.0f;
+      } else {
+        bx = ux;
+        by = uy;
+        bz = uz;
+      }
+    }
+  }
+  if (i == g.nx - 1 && p.bc_x_right != FSI_BC_PERIODIC) {
+    if (p.bc_x_right == FSI_BC_VELOCITY) {
+      brho = 1.0f;
+      bx = p.vel_bc_x_right[0];
+      by = p.vel_bc_x_right[1];
+      bz = p.vel_bc_x_right[2];
+    } else {
+      brho = p.rho_bc_x_right;
+      bx = obx;
+      by = oby;
+      bz = obz;
+    }
+  }
+}
+
 template <typename Pop>
 FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const Pop& Fs, int cur, int parity,
                            uint32_t flags, int i, int j, int k, uint32_t N, uint32_t c, uint32_t info, const Nbr& nb,
@@ -848,13 +971,6 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
                            const FaceFlags& ff, const float* F, uint32_t outlet_flags) {
   const bool mb = (flags & FSI_SUB_MOVING_WALL) != 0;
   const Pop Fd = pop_of<Pop>(st.F[1 - cur]);
-  float gx = p.force[0], gy = p.force[1], gz = p.force[2];
-  if (flags & FSI_SUB_CELL_FORCE) {
-    gx += at32(st.cell_force, c);
-    gy += at32(st.cell_force + N, c);
-    gz += at32(st.cell_force + 2 * (size_t)N, c);
-  }
-  const bool has_force = (gx != 0.0f) || (gy != 0.0f) || (gz != 0.0f);
   // The pressure outlet's reset needs the velocity of the cell behind it
   // (fluid.py:944-951): its loads go out now, with nothing waiting on them
   // until the reset, instead of as a second memory round trip at the end.
@@ -862,27 +978,8 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
   const bool outlet_p = legacy_reset && i == p.nx - 1 && p.bc_x_right == FSI_BC_PRESSURE;
   float obx = 0.0f, oby = 0.0f, obz = 0.0f;
   if (outlet_p) prev_velocity_flat(p, st, Fs, outlet_flags, c - (uint32_t)(p.ny * p.nz), obx, oby, obz);
-  float m[Q];
-  moments_M(F, m);
-  float rho, ux, uy, uz;
-  if (flags & FSI_SUB_STORED_MOMENTS) {
-    rho = at32(st.rho, c);
-    ux = at32(st.vel, c);
-    uy = at32(st.vel + N, c);
-    uz = at32(st.vel + 2 * (size_t)N, c);
-  } else {
-    // streaming3 (fluid.py:1795-1814), from the moment vector as in moments_of
-    rho = m[0];
-    const float inv = 1.0f / rho;
-    ux = (m[3] + 0.5f * gx) * inv;
-    uy = (m[5] + 0.5f * gy) * inv;
-    uz = (m[7] + 0.5f * gz) * inv;
-  }
-  float f[Q];
-  if (p.collision == FSI_COLLIDE_REGULARIZED)
-    regularized_collide(F, rho, ux, uy, uz, has_force, gx, gy, gz, p.s_v, f);
-  else
-    mrt_collide_m(F, m, ux, uy, uz, has_force, gx, gy, gz, p.s_v, p.s_other, f);
+  float f[Q], rho, ux, uy, uz;
+  collide_cell(p, st, flags, N, c, F, f, rho, ux, uy, uz);
 
   // The legacy reset rewrites every population of a face cell here; the open
   // modes leave the face cells streaming like inner cells (their unknown
@@ -896,31 +993,9 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
   const bool moving = mb && (info & CI_MOVING);
   // walls cannot read these links back from F (a boundary rewrites them)
   const bool keep_links = mb && (moving || ff.o);
-  // Moving-wall corrections of the links, before the pushes: the wall
-  // velocities are loaded 6 links at a time, each load branch-free (a
-  // direction without a link reads the cell itself), so a cell next to the
-  // moving solid waits for memory 3 times rather than once per link.
+  // moving-wall corrections of the links, before the pushes
   float cr[Q];
-#pragma unroll
-  for (int s = 0; s < Q; ++s) cr[s] = 0.0f;
-  if (moving) {
-#pragma unroll
-    for (int g0 = 1; g0 < Q; g0 += 6) {
-      float w3[6][3];
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const uint32_t y = (info & (1u << (g0 + q))) ? c + (uint32_t)nb.d(g0 + q) : c;
-        w3[q][0] = at32(st.solid_vel, y);
-        w3[q][1] = at32(st.solid_vel + N, y);
-        w3[q][2] = at32(st.solid_vel + 2 * (size_t)N, y);
-      }
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const int s = g0 + q;
-        cr[s] = -6.0f * wgt(s) * rho * ((float)EX[s] * w3[q][0] + (float)EY[s] * w3[q][1] + (float)EZ[s] * w3[q][2]);
-      }
-    }
-  }
+  moving_wall_corrections(st, N, c, info, nb, moving, rho, cr);
   N = wave_uniform(N);
 #pragma unroll
   for (int s = 0; s < Q; ++s) {
@@ -937,43 +1012,195 @@ FSI_HD void lbm_fluid_cell(const FsiLbmParams& p, const FsiLbmState& st, const P
   }
 
   if (self_face) {
-    // Boundary_condition_legacy (fluid.py:927-958); right face applied last.
-    const Grid g = make_grid(p);
-    float brho = 1.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
-    if (i == 0 && p.bc_x_left != FSI_BC_PERIODIC) {
-      if (p.bc_x_left == FSI_BC_VELOCITY) {
-        brho = 1.0f;
-        bx = p.vel_bc_x_left[0];
-        by = p.vel_bc_x_left[1];
-        bz = p.vel_bc_x_left[2];
-      } else {
-        brho = p.rho_bc_x_left;
-        const long long c1 = cell_of(g, (g.nx > 1 ? 1 : 0), j, k);
-        if (st.cellinfo[c1] & CI_SOLID) {
-          bx = by = bz = 0.0f;
-        } else {
-          bx = ux;
-          by = uy;
-          bz = uz;
-        }
-      }
-    }
-    if (i == g.nx - 1 && p.bc_x_right != FSI_BC_PERIODIC) {
-      if (p.bc_x_right == FSI_BC_VELOCITY) {
-        brho = 1.0f;
-        bx = p.vel_bc_x_right[0];
-        by = p.vel_bc_x_right[1];
-        bz = p.vel_bc_x_right[2];
-      } else {
-        brho = p.rho_bc_x_right;
-        bx = obx;  // the velocity of cell (nx - 2, j, k), loaded above
-        by = oby;
-        bz = obz;
-      }
-    }
+    float brho, bx, by, bz;
+    face_reset_state(p, st, i, j, k, ux, uy, uz, obx, oby, obz, brho, bx, by, bz);
 #pragma unroll
     for (int s = 0; s < Q; ++s) Fd.st(N, s, c, feq(s, brho, bx, by, bz));
   }
+}
+
+// ---------------------------------------------------------------------------
+// Resident moving-wall subcycle (pull form).
+//
+// A whole subcycle of moving-wall substeps runs with the lattice on chip.
+// Each substep has two phases with an exchange between them.  Slot (s, x)
+// holds the value cell x + e_s takes as its population s:
+//   A. every fluid cell c collides (as lbm_fluid_cell) and fills slot (s, c)
+//      for each direction s = 1..18 with its post-collision f_s, or, on a link
+//      (the neighbour y = c + e_s is solid), with the link scalar
+//      c_s = f_s + (f_s + corr_s) that the wall y sums.  On a link it also
+//      fills slot (LR[s], y), which only c reads (c = y + e_LR[s]) and which
+//      the solid y leaves free, with its bounce-back f_s + corr_s;
+//   B. every fluid cell takes each population q = 1..18 from slot (q, c - e_q)
+//      and keeps its own f_0; a cell on a BC face resets all 19 to the
+//      boundary equilibrium; a wall cell y sums e_s c_s, c_s from slot
+//      (s, y - e_s), over its incoming links into its accumulator.
+// Values, bounce-backs, face resets and wall sums are the expressions of the
+// push form (lbm_substep_cell), so the two forms give the same bits.  A link
+// into a wall at rest has corr_s = +-0, so f_s + (f_s + corr_s) equals
+// 2 (f_s + corr_s), the value the push form's wall reads back from the
+// populations.  The wall sum of substep t is added during substep t instead
+// of t + 1, which keeps the order of the additions; the last substep's links
+// are left to the finalize, as after the push form.
+//
+// Block decomposition: a workgroup owns RES_BI x RES_BJ x RES_NZ cells
+// (i-planes x j-rows x a whole k-row of RES_NZ = nz cells), one cell per
+// work-item; wave w holds row (li, lj) = (w / RES_BJ, w % RES_BJ) and lane k
+// cell k of it.  Slots of cells in the block are in LDS, one plane per
+// direction; a slot (s, x) of a cell x in another block is element x of halo
+// plane s in device memory.
+// ---------------------------------------------------------------------------
+constexpr int RES_BI = 4, RES_BJ = 4, RES_NZ = 64;
+constexpr int RES_THREADS = RES_BI * RES_BJ * RES_NZ;
+
+FSI_HD bool res_shape_ok(int nx, int ny, int nz) {
+  return nz == RES_NZ && nx >= RES_BI && ny >= RES_BJ && nx % RES_BI == 0 && ny % RES_BJ == 0;
+}
+
+FSI_HD int res_blocks(const FsiLbmParams& p) { return (p.nx / RES_BI) * (p.ny / RES_BJ); }
+
+struct ResLane {
+  int li, lj, k;  // position in the block
+  int i, j;       // lattice position (k is shared)
+  uint32_t c;     // cell
+};
+
+FSI_HD ResLane res_lane(const FsiLbmParams& p, uint32_t block, uint32_t t) {
+  const uint32_t nbj = (uint32_t)p.ny / RES_BJ;
+  const uint32_t bi = block / nbj, bj = block - bi * nbj;
+  const uint32_t w = wave_uniform(t / RES_NZ);  // one row per wave: li, lj, i, j are wave-uniform
+  ResLane L;
+  L.k = (int)(t - w * RES_NZ);
+  L.li = (int)(w / RES_BJ);
+  L.lj = (int)(w - (uint32_t)L.li * RES_BJ);
+  L.i = (int)(bi * RES_BI) + L.li;
+  L.j = (int)(bj * RES_BJ) + L.lj;
+  L.c = ((uint32_t)L.i * (uint32_t)p.ny + (uint32_t)L.j) * RES_NZ + (uint32_t)L.k;
+  return L;
+}
+
+// Is the cell at (li + di, lj + dj) of the block's rows in the block?  (A
+// wave-uniform test: li, lj are the wave's row.)
+FSI_HD bool res_in_block(int li, int lj, int di, int dj) {
+  return (unsigned)(li + di) < (unsigned)RES_BI && (unsigned)(lj + dj) < (unsigned)RES_BJ;
+}
+
+// The same test for a row known only by its edge class in each axis (0: the
+// block's first row, 1: an inner row, 2: its last row), which decides it
+// because the block spans more than 2 rows (offsets are -1, 0, +1).
+FSI_HD constexpr bool res_class_in(int ci, int cj, int di, int dj) {
+  return (ci == 0 ? di >= 0 : (ci == 2 ? di <= 0 : true)) && (cj == 0 ? dj >= 0 : (cj == 2 ? dj <= 0 : true));
+}
+FSI_HD int res_row_class(int l, int rows) { return l == 0 ? 0 : (l == rows - 1 ? 2 : 1); }
+static_assert(RES_BI > 2 && RES_BJ > 2, "res_class_in needs blocks of at least 3 rows");
+
+// Block slot of the cell at (li + di, lj + dj, k + dk) of the block (k wraps
+// within the row, as nbr_of's periodic z offsets do for nz = RES_NZ).
+FSI_HD int res_slot(int li, int lj, int k, int di, int dj, int dk) {
+  return ((li + di) * RES_BJ + (lj + dj)) * RES_NZ + ((k + dk) & (RES_NZ - 1));
+}
+
+// Why the resident subcycle cannot run (nullptr: it can), apart from the
+// device's capacity, which the GPU entry point checks.
+inline const char* res_unsupported(const FsiLbmParams& p, const FsiLbmState& st, int cur, int n, uint32_t flags,
+                                   const float* halo, const uint32_t* sync) {
+  if (cur != 0 && cur != 1) return "cur must be 0 or 1";
+  if (n < 1) return "n_substeps must be positive";
+  if (!halo || !sync) return "halo and sync buffers are required";
+  if (!(flags & FSI_SUB_MOVING_WALL)) return "the resident subcycle is the moving-wall subcycle (FSI_SUB_MOVING_WALL)";
+  if (!st.link_c[0] || !st.link_c[1] || !st.link_corr || !st.hydro_accum)
+    return "moving-wall link buffers and the accumulator are required";
+  if (p.storage != FSI_STORE_F32) return "the resident subcycle covers FP32 populations";
+  if (p.open_bc != FSI_OPEN_BC_EQUILIBRIUM) return "the resident subcycle covers the legacy x-face reset";
+  if (!res_shape_ok(p.nx, p.ny, p.nz)) return "the resident subcycle covers nz = 64 with nx, ny multiples of 4";
+  return nullptr;
+}
+
+// Phase A of fluid cell c from its post-stream populations F.  put(s, v)
+// fills slot (s, c); bounce(s, v) fills slot (LR[s], c + e_s) on a link s.
+// dot(s): e_s . u_wall(c + e_s) of link s (moving_wall_dots), read only by a
+// cell next to a moving wall.
+// Returns the cell's f_0 in F[0] and the velocity the collision used in
+// (ux, uy, uz) (a pressure outlet behind the cell resets to it).  On the
+// subcycle's last substep (`last`) the cell leaves the link scalars of its
+// moving-wall and BC-face links in link_c[parity] and, with FSI_SUB_DIAG, its
+// corrections in link_corr, as the push form does, for the finalize.
+template <typename Dot, typename Put, typename Bounce>
+FSI_HD void res_collide_cell(const FsiLbmParams& p, const FsiLbmState& st, uint32_t flags, int parity, bool last,
+                             uint32_t N, uint32_t c, uint32_t info, bool face_o, float* F, float& ux, float& uy,
+                             float& uz, const Dot& dot, const Put& put, const Bounce& bounce) {
+  float f[Q], rho;
+  collide_cell(p, st, flags, N, c, F, f, rho, ux, uy, uz);
+  const bool moving = (info & CI_MOVING) != 0;
+  const bool keep = last && (moving || face_o);
+  const bool diag = last && (flags & FSI_SUB_DIAG) != 0;
+  float* link = st.link_c[parity];
+  F[0] = f[0];
+#pragma unroll
+  for (int s = 1; s < Q; ++s) {
+    if (info & (1u << s)) {
+      const float cr_s = moving ? wall_corr(s, rho, dot(s)) : 0.0f;
+      const float bounced = f[s] + cr_s;
+      put(s, f[s] + bounced);
+      bounce(s, bounced);
+      if (keep) at32(link + (size_t)(s - 1) * N, c) = f[s] + bounced;
+      if (diag) at32(st.link_corr + (size_t)(s - 1) * N, c) = cr_s;
+    } else {
+      put(s, f[s]);
+    }
+  }
+}
+
+// Phase B of cell L.c, a fluid cell or (when the substep gathers) a wall
+// cell.  Both kinds take the same 18 slots, v[q] = slot (q, c - e_q)
+// (res_pull_values): a fluid cell its populations 1..18 (F[0] is its own f_0
+// from phase A), a wall cell the link scalars c_s of its incoming links s (its
+// other slots hold nothing of this substep and are masked).  A fluid cell on a
+// BC face resets all 19 populations to the boundary equilibrium instead
+// (face_reset_state with its own velocity (ux, uy, uz) and, behind a right
+// pressure face, (obx, oby, obz), that of cell (nx - 2, j, k), 0 if solid).  A
+// wall cell sums e_s c_s in direction order with 0 on the directions without a
+// link, as wall_links_accumulate does, and adds it to its accumulator record
+// acc (loaded beforehand).
+template <typename Pull>
+FSI_HD void res_pull_values(float* v, const Pull& pull) {
+#pragma unroll
+  for (int q = 1; q < Q; ++q) v[q] = pull(q);
+}
+
+FSI_HD void res_take_values(const FsiLbmParams& p, const FsiLbmState& st, const ResLane& L, uint32_t info,
+                            bool face_o, float ux, float uy, float uz, float obx, float oby, float obz, float* F,
+                            const double* acc, const float* v) {
+  if (!(info & CI_SOLID)) {
+    if (face_o) {
+      float brho, bx, by, bz;
+      face_reset_state(p, st, L.i, L.j, L.k, ux, uy, uz, obx, oby, obz, brho, bx, by, bz);
+#pragma unroll
+      for (int s = 0; s < Q; ++s) F[s] = feq(s, brho, bx, by, bz);
+      return;
+    }
+#pragma unroll
+    for (int q = 1; q < Q; ++q) F[q] = v[q];
+    return;
+  }
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+  for (int s = 1; s < Q; ++s) {
+    const float cs = (info & (1u << s)) ? v[s] : 0.0f;
+    fx += (float)EX[s] * cs;
+    fy += (float)EY[s] * cs;
+    fz += (float)EZ[s] * cs;
+  }
+  wall_accumulate_loaded(st.hydro_accum, L.c, acc, fx, fy, fz);
+}
+
+template <typename Pull>
+FSI_HD void res_pull_cell(const FsiLbmParams& p, const FsiLbmState& st, const ResLane& L, uint32_t info,
+                          bool face_o, float ux, float uy, float uz, float obx, float oby, float obz, float* F,
+                          const double* acc, const Pull& pull) {
+  float v[Q];
+  res_pull_values(v, pull);
+  res_take_values(p, st, L, info, face_o, ux, uy, uz, obx, oby, obz, F, acc, v);
 }
 
 // Link statistics of direction s for wall cell y after the last moving-wall

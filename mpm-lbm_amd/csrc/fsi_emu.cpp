@@ -83,6 +83,108 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
   return FSI_OK;
 }
 
+// The resident subcycle as the GPU runs it: blocks of RES_THREADS cells, each
+// with its own copy of the LDS slot planes, the halo planes in `halo`; every
+// block finishes phase A before any block starts phase B (the GPU's
+// neighbour waits).  The per-lane registers are arrays indexed by cell.
+int fsi_lbm_subcycle_resident(const FsiLbmParams* p, const FsiLbmState* st, int cur, int n, uint32_t flags,
+                              float* halo, uint32_t* sync, void*) {
+  if (const char* why = res_unsupported(*p, *st, cur, n, flags, halo, sync)) return fail(FSI_E_UNSUPPORTED, why);
+  const uint32_t N = (uint32_t)ncells(p);
+  const int nblk = res_blocks(*p);
+  const size_t T = RES_THREADS;
+  std::vector<float> lds((size_t)nblk * NLINK * T), wd((size_t)nblk * NLINK * T),
+      ou((size_t)nblk * 3 * RES_BJ * RES_NZ);
+  std::vector<float> R((size_t)N * Q), U((size_t)N * 3);
+  const float* Fin = st->F[cur];
+  for (int b = 0; b < nblk; ++b)
+    for (uint32_t t = 0; t < T; ++t) {
+      const ResLane L = res_lane(*p, (uint32_t)b, t);
+      const uint32_t info = st->cellinfo[L.c];
+      if (info & CI_SOLID) continue;
+      for (int s = 0; s < Q; ++s) R[(size_t)L.c * Q + s] = Fin[(size_t)s * N + L.c];
+      if (info & CI_MOVING) {
+        float dot[Q];
+        moving_wall_dots(*st, N, L.c, info, nbr_of(*p, L.i, L.j, L.k), dot);
+        for (int s = 1; s < Q; ++s) wd[((size_t)b * NLINK + (s - 1)) * T + t] = dot[s];
+      }
+    }
+  for (int step = 0; step < n; ++step) {
+    uint32_t fl = flags & (FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE | FSI_SUB_DIAG);
+    if (step == 0) fl |= flags & FSI_SUB_STORED_MOMENTS;
+    const bool last = step == n - 1;
+    const bool gather = (flags & FSI_SUB_GATHER_PREV) && !last;
+    float* hp = halo + (size_t)(step & 1) * NLINK * N;
+    for (int b = 0; b < nblk; ++b) {
+      float* lv = lds.data() + (size_t)b * NLINK * T;
+      for (uint32_t t = 0; t < T; ++t) {
+        const ResLane L = res_lane(*p, (uint32_t)b, t);
+        const uint32_t info = st->cellinfo[L.c];
+        float* o = ou.data() + (size_t)b * 3 * RES_BJ * RES_NZ + (size_t)(L.lj * RES_NZ + L.k) * 3;
+        if (info & CI_SOLID) {
+          if (L.i == p->nx - 2) o[0] = o[1] = o[2] = 0.0f;
+          continue;
+        }
+        const Nbr nb = nbr_of(*p, L.i, L.j, L.k);
+        auto dot = [&](int s) { return wd[((size_t)b * NLINK + (s - 1)) * T + t]; };
+        auto put = [&](int s, float v) {
+          if (res_in_block(L.li, L.lj, EX[s], EY[s]))
+            lv[(size_t)(s - 1) * T + t] = v;
+          else
+            hp[(size_t)(s - 1) * N + L.c] = v;
+        };
+        auto bounce = [&](int s, float v) {
+          if (res_in_block(L.li, L.lj, EX[s], EY[s]))
+            lv[(size_t)(LR[s] - 1) * T + (size_t)res_slot(L.li, L.lj, L.k, EX[s], EY[s], EZ[s])] = v;
+          else
+            hp[(size_t)(LR[s] - 1) * N + (L.c + (uint32_t)nb.d(s))] = v;
+        };
+        float* u = U.data() + (size_t)L.c * 3;
+        res_collide_cell(*p, *st, fl, step & 1, last, N, L.c, info, face_flags(*p, L.i).o, R.data() + (size_t)L.c * Q,
+                         u[0], u[1], u[2], dot, put, bounce);
+        if (L.i == p->nx - 2) {
+          o[0] = u[0];
+          o[1] = u[1];
+          o[2] = u[2];
+        }
+      }
+    }
+    for (int b = 0; b < nblk; ++b) {
+      const float* lv = lds.data() + (size_t)b * NLINK * T;
+      for (uint32_t t = 0; t < T; ++t) {
+        const ResLane L = res_lane(*p, (uint32_t)b, t);
+        const uint32_t info = st->cellinfo[L.c];
+        const bool fluid = !(info & CI_SOLID);
+        if (!fluid && !(gather && (info & CI_MASK))) continue;
+        const Nbr nb = nbr_of(*p, L.i, L.j, L.k);
+        auto pull = [&](int q) {
+          if (res_in_block(L.li, L.lj, -EX[q], -EY[q]))
+            return lv[(size_t)(q - 1) * T + (size_t)res_slot(L.li, L.lj, L.k, -EX[q], -EY[q], -EZ[q])];
+          return hp[(size_t)(q - 1) * N + (L.c + (uint32_t)nb.d(LR[q]))];
+        };
+        const float* o = ou.data() + (size_t)b * 3 * RES_BJ * RES_NZ + (size_t)(L.lj * RES_NZ + L.k) * 3;
+        const float* u = U.data() + (size_t)L.c * 3;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!fluid) {
+          const double* a = st->hydro_accum + 4 * (size_t)L.c;
+          acc[0] = a[0];
+          acc[1] = a[1];
+          acc[2] = a[2];
+          acc[3] = a[3];
+        }
+        res_pull_cell(*p, *st, L, info, face_flags(*p, L.i).o, u[0], u[1], u[2], o[0], o[1], o[2],
+                      R.data() + (size_t)L.c * Q, acc, pull);
+      }
+    }
+  }
+  float* Fout = st->F[(cur + n) & 1];
+  for (uint32_t c = 0; c < N; ++c)
+    if (!(st->cellinfo[c] & CI_SOLID))
+      for (int s = 0; s < Q; ++s) Fout[(size_t)s * N + c] = R[(size_t)c * Q + s];
+  for (int b = 0; b < nblk; ++b) sync[64 * b] += (uint32_t)n;
+  return FSI_OK;
+}
+
 int fsi_lbm_open_bc(const FsiLbmParams* p, const FsiLbmState* st, int buf, uint32_t flags, void*) {
   if (p->nx < 3 || p->ny < 3 || p->nz < 3) return fail(FSI_E_INVALID, "lattice must be at least 3 cells along every axis");
   if (p->open_bc < FSI_OPEN_BC_EQUILIBRIUM || p->open_bc > FSI_OPEN_BC_CONVECTIVE)

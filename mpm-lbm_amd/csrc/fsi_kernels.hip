@@ -597,6 +597,198 @@ __global__ void __launch_bounds__(BS, MINW) k_lbm_substep(FsiLbmParams p, FsiLbm
   if (queued) wall_links_accumulate<Pop>(st, wc, winfo, ff, Fw, acc);
 }
 
+// ---------------------------------------------------------------------------
+// Resident moving-wall subcycle (fsi_lbm_subcycle_resident).
+//
+// One workgroup of RES_THREADS (1024) work-items per block of 4 x 4 x 64
+// cells, every block resident at once (one per CU at 64^3): each cell's 19
+// populations stay in VGPRs for the whole subcycle.  Per substep, phase A
+// fills each fluid cell's slots: in the block's LDS planes when the cell that
+// reads a slot is in the block, else in the halo planes in device memory;
+// phase B reads them (res_collide_cell / res_pull_cell in fsi_core.h).
+//
+// Between the phases a block waits for its 8 neighbour blocks (periodic in i
+// and j) to have published the substep.  Each block has a counter word in
+// `sync`.  The halo values and the counters are accessed as agent-scope
+// relaxed atomics, which are coherent across the XCDs' L2 caches, so no cache
+// write-back or invalidation is needed per substep (as fences they measured
+// +3.2 us per substep at 64^3).  A block's counter store is ordered after its
+// halo stores: every wave waits until its stores have completed
+// (s_waitcnt vmcnt(0), the wait a release fence ends with) before the barrier
+// after which the counter is stored; a reader issues its halo loads only
+// after its poll has returned the new count and a barrier.  Counters only grow
+// (each counts every substep its block has run, over all calls), so a launch
+// needs no reset and can be replayed from a graph.  Every wait is bounded: a
+// neighbour not heard from within RES_MAX_POLLS polls is counted in the error
+// word and the block goes on without it, so no launch can hang.
+// ---------------------------------------------------------------------------
+constexpr uint32_t RES_SYNC_STRIDE = 64;
+constexpr int RES_MAX_POLLS = 1 << 16;
+
+// Element c of a halo plane (uniform base, 32-bit offset, as at32) as an
+// agent-scope relaxed atomic.
+__device__ __forceinline__ void res_put(float* plane, uint32_t c, float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __hip_atomic_store(&at32(plane, c), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+__device__ __forceinline__ float res_get(float* plane, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __hip_atomic_load(&at32(plane, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+  return 0.0f;
+#endif
+}
+
+template <bool FORCE>
+__global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p, FsiLbmState st, int cur, int n,
+                                                                  uint32_t flags, float* __restrict__ halo,
+                                                                  uint32_t* __restrict__ sync) {
+  __shared__ float lv[NLINK * RES_THREADS];  // slot planes, directions 1..18
+  __shared__ float wd[NLINK * RES_THREADS];  // e_s . u_wall of the links of cells next to a moving wall
+  __shared__ float ou[3 * RES_BJ * RES_NZ];  // velocity of the cells at i = nx - 2
+  __shared__ uint32_t s_base;
+  __shared__ int bail;
+  const uint32_t b = blockIdx.x, t = threadIdx.x;
+  const ResLane L = res_lane(p, b, t);
+  const uint32_t N = wave_uniform((uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)RES_NZ);
+  const uint32_t info = cell_info(st, L.c);
+  const bool fluid = !(info & CI_SOLID);
+  const bool wall = !fluid && (info & CI_MASK) != 0;
+  const bool face_o = bc_face(p, L.i);
+  const PopF32 Fin = pop_of<PopF32>(st.F[cur]), Fout = pop_of<PopF32>(st.F[(cur + n) & 1]);
+  uint32_t* own = sync + RES_SYNC_STRIDE * b;
+  if (t == 0) {
+    s_base = __hip_atomic_load(own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bail = 0;
+  }
+  // lanes 1..8 of wave 0 each watch one neighbour block (the 3 x 3 around it)
+  uint32_t* watch = nullptr;
+  if (t >= 1 && t <= 8) {
+    const int d = (int)t <= 4 ? (int)t - 1 : (int)t;
+    const int nbi = p.nx / RES_BI, nbj = p.ny / RES_BJ;
+    const int bi = (int)b / nbj, bj = (int)b - bi * nbj;
+    const int wi = (bi + d / 3 - 1 + nbi) % nbi, wj = (bj + d % 3 - 1 + nbj) % nbj;
+    watch = sync + RES_SYNC_STRIDE * (uint32_t)(wi * nbj + wj);
+  }
+  float F[Q];
+  if (fluid) {
+#pragma unroll
+    for (int s = 0; s < Q; ++s) F[s] = Fin.ld(N, s, L.c);
+    // the wall velocities are fixed for the subcycle: their dot products
+    // with the link directions are formed once
+    if (info & CI_MOVING) {
+      float dot[Q];
+      moving_wall_dots(st, N, L.c, info, nbr_of(p, L.i, L.j, L.k), dot);
+#pragma unroll
+      for (int s = 1; s < Q; ++s) wd[(s - 1) * RES_THREADS + t] = dot[s];
+    }
+  }
+  __syncthreads();
+  const uint32_t base = s_base;
+  const uint32_t fixed = FSI_SUB_MOVING_WALL | (FORCE ? FSI_SUB_CELL_FORCE : 0u) | (flags & FSI_SUB_DIAG);
+  const bool accumulate = wall && (flags & FSI_SUB_GATHER_PREV) != 0;
+  float* ob = ou + (L.lj * RES_NZ + L.k) * 3;
+  // the wave's row by edge class (a block's first, inner or last row in i and
+  // j), for which the pulls are compiled with the LDS-or-halo choice of each
+  // direction fixed: one straight run of loads per row
+  const int row_class = res_row_class(L.li, RES_BI) * 3 + res_row_class(L.lj, RES_BJ);
+  float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+  for (int step = 0; step < n; ++step) {
+    const uint32_t fl = step == 0 ? fixed | (flags & FSI_SUB_STORED_MOMENTS) : fixed;
+    const bool last = step == n - 1;
+    const bool gather = accumulate && !last;
+    float* hp = halo + (size_t)(step & 1) * NLINK * N;
+    // The lane's k, opaque to the optimiser: the cell index, neighbour offsets
+    // and slot addresses derived from it are formed in every substep instead
+    // of being held across the loop in registers (which spilled).
+    int kk = L.k;
+    asm volatile("" : "+v"(kk));
+    ResLane Lc = L;
+    Lc.k = kk;
+    Lc.c = L.c - (uint32_t)L.k + (uint32_t)kk;
+    const uint32_t c = Lc.c;
+    const Nbr nb = nbr_of(p, Lc.i, Lc.j, kk);
+    const uint32_t tt = (uint32_t)(Lc.li * RES_BJ + Lc.lj) * RES_NZ + (uint32_t)kk;
+    if (fluid) {
+      // slot (s, c) and, on a link, slot (LR[s], c + e_s): in LDS when the
+      // cell reading it is in the block (a wave-uniform test), else in halo
+      auto dot = [&](int s) { return wd[(s - 1) * RES_THREADS + tt]; };
+      auto put = [&](int s, float v) {
+        if (res_in_block(Lc.li, Lc.lj, EX[s], EY[s]))
+          lv[(s - 1) * RES_THREADS + tt] = v;
+        else
+          res_put(hp + (size_t)(s - 1) * N, c, v);
+      };
+      auto bounce = [&](int s, float v) {
+        if (res_in_block(Lc.li, Lc.lj, EX[s], EY[s]))
+          lv[(LR[s] - 1) * RES_THREADS + res_slot(Lc.li, Lc.lj, kk, EX[s], EY[s], EZ[s])] = v;
+        else
+          res_put(hp + (size_t)(LR[s] - 1) * N, c + (uint32_t)nb.d(s), v);
+      };
+      res_collide_cell(p, st, fl, step & 1, last, N, c, info, face_o, F, ux, uy, uz, dot, put, bounce);
+    }
+    if (L.i == p.nx - 2) {
+      ob[0] = fluid ? ux : 0.0f;
+      ob[1] = fluid ? uy : 0.0f;
+      ob[2] = fluid ? uz : 0.0f;
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (gather) {
+      const double* a = st.hydro_accum + 4 * (size_t)c;
+      acc[0] = a[0];
+      acc[1] = a[1];
+      acc[2] = a[2];
+      acc[3] = a[3];
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's halo stores (and loads) have completed
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(own, base + (uint32_t)step + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (watch && !bail) {
+      const uint32_t want = base + (uint32_t)step + 1u;
+      int polls = 0;
+      while ((int32_t)(__hip_atomic_load(watch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - want) < 0) {
+        if (++polls > RES_MAX_POLLS) {
+          atomicAdd(sync + RES_SYNC_STRIDE * (uint32_t)res_blocks(p), 1u);
+          bail = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    if (fluid || gather) {
+      float v[Q];
+      switch (row_class) {
+#define FSI_RES_PULL(CI, CJ)                                                                        \
+  case (CI) * 3 + (CJ):                                                                             \
+    res_pull_values(v, [&](int q) -> float {                                                        \
+      if (res_class_in((CI), (CJ), -EX[q], -EY[q]))                                                 \
+        return lv[(q - 1) * RES_THREADS + res_slot(Lc.li, Lc.lj, kk, -EX[q], -EY[q], -EZ[q])];     \
+      return res_get(hp + (size_t)(q - 1) * N, c + (uint32_t)nb.d(LR[q]));                          \
+    });                                                                                             \
+    break;
+        FSI_RES_PULL(0, 0)
+        FSI_RES_PULL(0, 1)
+        FSI_RES_PULL(0, 2)
+        FSI_RES_PULL(1, 0)
+        FSI_RES_PULL(1, 1)
+        FSI_RES_PULL(1, 2)
+        FSI_RES_PULL(2, 0)
+        FSI_RES_PULL(2, 1)
+        FSI_RES_PULL(2, 2)
+#undef FSI_RES_PULL
+      }
+      res_take_values(p, st, Lc, info, face_o, ux, uy, uz, ob[0], ob[1], ob[2], F, acc, v);
+    }
+    __syncthreads();
+  }
+  if (fluid) {
+#pragma unroll
+    for (int s = 0; s < Q; ++s) Fout.st(N, s, L.c, F[s]);
+  }
+}
+
 __global__ void __launch_bounds__(64) k_open_bc_clear(FsiLbmState st) {
   if (threadIdx.x < 5) st.counters->ob_step[threadIdx.x] = 0u;
 }
@@ -2150,6 +2342,43 @@ int fsi_lbm_substep(const FsiLbmParams* p, const FsiLbmState* st, int cur, int p
       return fail(FSI_E_INVALID, "unsupported substep flag combination");
   }
   return check_launch("k_lbm_substep");
+}
+
+int fsi_lbm_subcycle_resident(const FsiLbmParams* p, const FsiLbmState* st, int cur, int n, uint32_t flags,
+                              float* halo, uint32_t* sync, void* stream) {
+  int rc = check_lbm(p, st);
+  if (rc) return rc;
+  if (const char* why = res_unsupported(*p, *st, cur, n, flags, halo, sync)) return fail(FSI_E_UNSUPPORTED, why);
+  if (!st->solid_vel) return fail(FSI_E_INVALID, "moving-wall subcycle needs solid_vel");
+  const bool force = (flags & FSI_SUB_CELL_FORCE) != 0;
+  if (force && !st->cell_force) return fail(FSI_E_INVALID, "cell_force is null");
+  // every block must be resident at once: at most (CUs x workgroups per CU)
+  static thread_local int dev_cached = -1;
+  static thread_local long long capacity[2] = {0, 0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(FSI_E_HIP, "hipGetDevice failed");
+  if (dev != dev_cached) {
+    int cus = 0, per_cu[2] = {0, 0};
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], k_lbm_resident<false>, RES_THREADS, 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], k_lbm_resident<true>, RES_THREADS, 0) != hipSuccess)
+      return fail(FSI_E_HIP, "occupancy query of the resident subcycle failed");
+    capacity[0] = (long long)cus * per_cu[0];
+    capacity[1] = (long long)cus * per_cu[1];
+    dev_cached = dev;
+  }
+  const int nblk = res_blocks(*p);
+  if (nblk > capacity[force ? 1 : 0])
+    return fail(FSI_E_UNSUPPORTED, "the lattice needs " + std::to_string(nblk) +
+                                       " resident workgroups, the device holds " +
+                                       std::to_string(capacity[force ? 1 : 0]));
+  hipStream_t s = as_stream(stream);
+  if (force)
+    hipLaunchKernelGGL(k_lbm_resident<true>, dim3(nblk), dim3(RES_THREADS), 0, s, *p, *st, cur, n, flags, halo, sync);
+  else
+    hipLaunchKernelGGL(k_lbm_resident<false>, dim3(nblk), dim3(RES_THREADS), 0, s, *p, *st, cur, n, flags, halo, sync);
+  return check_launch("k_lbm_resident");
 }
 
 int fsi_lbm_open_bc(const FsiLbmParams* p, const FsiLbmState* st, int buf, uint32_t flags, void* stream) {

@@ -47,6 +47,8 @@ SUB_GATHER_PREV = 4
 SUB_DIAG = 8
 SUB_CELL_FORCE = 16
 
+E_UNSUPPORTED = -3  # FSI_E_UNSUPPORTED
+
 REDUCE_BLOCKS = 256
 
 
@@ -234,7 +236,7 @@ ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, 
 EXPORTED_SYMBOLS = [
     "fsi_last_error", "fsi_version", "fsi_abi_sizes", "fsi_sync",
     "fsi_lbm_init", "fsi_lbm_rebuild", "fsi_lbm_substep", "fsi_lbm_open_bc", "fsi_lbm_link_stats", "fsi_lbm_moments",
-    "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats",
+    "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats", "fsi_lbm_subcycle_resident",
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
     "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_substep_coupled_noalloc",
     "fsi_mpm_reduce_stats",
@@ -260,6 +262,21 @@ def bin_count(nx: int, ny: int, nz: int) -> int:
 def bin_work_ints(n_bins: int) -> int:
     """FSI_BIN_WS_INTS(n_bins)."""
     return int(n_bins) + 1 + (int(n_bins) + 1 + 1023) // 1024
+
+
+def res_blocks(nx: int, ny: int) -> int:
+    """FSI_RES_BLOCKS(nx, ny): workgroups of the resident subcycle."""
+    return (int(nx) // 4) * (int(ny) // 4)
+
+
+def res_sync_words(nx: int, ny: int) -> int:
+    """FSI_RES_SYNC_WORDS(nx, ny)."""
+    return 64 * (res_blocks(nx, ny) + 1)
+
+
+def res_halo_floats(n_cells: int) -> int:
+    """FSI_RES_HALO_FLOATS(n_cells)."""
+    return 2 * 18 * int(n_cells)
 
 
 def coupled_workspace_doubles(n_particles: int) -> int:
@@ -297,6 +314,8 @@ class Native:
         L.fsi_lbm_mb_finalize.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, vp]
         L.fsi_lbm_reduce_stats.argtypes = [P(LbmParams), P(LbmState), vp, vp, vp]
+        L.fsi_lbm_subcycle_resident.argtypes = [P(LbmParams), P(LbmState), ctypes.c_int, ctypes.c_int, u32, vp, vp,
+                                                vp]
         L.fsi_project.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState), P(CouplingParams), vp, vp]
         L.fsi_penalty_force.argtypes = [P(LbmParams), P(LbmState), P(CouplingParams), vp, vp]
         L.fsi_mpm_clear_grid.argtypes = [P(MpmParams), P(MpmState), vp]
@@ -335,6 +354,17 @@ class Native:
                 raise FsiError(
                     f"{self.path}: struct {cls.__name__} is {size} bytes in C but {ctypes.sizeof(cls)} in ctypes"
                 )
+
+    def try_call(self, name: str, *args) -> bool:
+        """Call an entry point that may decline its arguments: False when it
+        returns FSI_E_UNSUPPORTED, True on success; raise on other errors."""
+        rc = getattr(self.lib, name)(*args)
+        if rc == E_UNSUPPORTED:
+            return False
+        if rc != 0:
+            msg = self.lib.fsi_last_error().decode("utf-8", "replace")
+            raise FsiError(f"{name} failed ({rc}): {msg}")
+        return True
 
     def call(self, name: str, *args):
         rc = getattr(self.lib, name)(*args)

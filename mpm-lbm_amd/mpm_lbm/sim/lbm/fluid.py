@@ -174,6 +174,13 @@ class LBMFluid3D:
         self.mb_sample_count = 0
         self._last_link_parity = None  # link_c parity of the last moving-wall substep, while its links are current
         self._link_stats_ws = None
+        # The moving-wall subcycle runs as one resident launch with the lattice
+        # on chip (fsi_lbm_subcycle_resident) where that entry point covers the
+        # lattice; False keeps one launch per substep.  Both give the same bits.
+        self.resident_subcycle = True
+        self._res_declined = False
+        self._res_halo = None
+        self._res_sync = None
         self.tau_f = None
         self.s_v = None
         self.s_other = None
@@ -387,7 +394,13 @@ class LBMFluid3D:
         self.update_dynamic_solid(threshold)
         base = _native.SUB_MOVING_WALL | (_native.SUB_CELL_FORCE if self._cell_force_active else 0)
         parity = 0
-        for t in range(n):
+        launches = n
+        if self._run_resident(n, base, accumulate):
+            self.cur = (self.cur + n) % 2
+            self._moments_fresh = False
+            parity = n % 2
+            launches = 0
+        for t in range(launches):
             flags = base
             if t == 0 and self._moments_fresh:
                 flags |= _native.SUB_STORED_MOMENTS
@@ -414,6 +427,44 @@ class LBMFluid3D:
             self._pending_accumulate = False
             self._hydro_from_links = True
         self._sync_moments()
+
+    def _resident_shape_ok(self):
+        return (self.resident_subcycle and not self._res_declined and self._storage_code == _native.STORE_F32
+                and self._open_bc == _native.OPEN_BC_EQUILIBRIUM and self.nz == 64 and self.nx % 4 == 0
+                and self.ny % 4 == 0)
+
+    def _run_resident(self, n, base, accumulate):
+        """The subcycle's n substeps as one fsi_lbm_subcycle_resident launch;
+        False (nothing run) where it does not cover the lattice or device."""
+        if not self._resident_shape_ok():
+            return False
+        if self._res_sync is None:
+            self._res_halo = torch.empty((_native.res_halo_floats(self.N),), dtype=torch.float32, device=self.device)
+            self._res_sync = torch.zeros((_native.res_sync_words(self.nx, self.ny),), dtype=torch.int32,
+                                         device=self.device)
+        flags = base | _native.SUB_DIAG
+        if self._moments_fresh:
+            flags |= _native.SUB_STORED_MOMENTS
+        if accumulate:
+            flags |= _native.SUB_GATHER_PREV
+        ok = self.native.try_call("fsi_lbm_subcycle_resident", ctypes.byref(self._params), ctypes.byref(self._state),
+                                  self.cur, int(n), flags, _native.ptr(self._res_halo), _native.ptr(self._res_sync),
+                                  self._stream())
+        if not ok:
+            self._res_declined = True
+        return ok
+
+    def check_resident_waits(self):
+        """Raise if a resident subcycle went on without a neighbour block
+        (a wait that timed out: its results are invalid); the sync words are
+        cleared so the next call starts clean."""
+        if self._res_sync is None:
+            return
+        nblk = _native.res_blocks(self.nx, self.ny)
+        timeouts = int(self._res_sync[64 * nblk].item())
+        if timeouts:
+            self._res_sync.zero_()
+            raise RuntimeError(f"resident LBM subcycle: {timeouts} neighbour waits timed out; results are invalid")
 
     # --------------------------------------------------------- open boundaries
     def Boundary_condition(self):
