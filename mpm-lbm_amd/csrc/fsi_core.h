@@ -144,6 +144,15 @@ FSI_HD uint32_t wave_uniform(uint32_t v) {
 #endif
 }
 
+// An instruction-scheduling fence (device): the code before it is scheduled
+// before the code after it, which bounds how many loads are in flight (and
+// registers live) at once.
+FSI_HD void sched_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 // Element c of a population plane through a 32-bit byte offset.  A lattice
 // holds at most 2^30 cells (checked by the C ABI), so c * sizeof(T) < 2^32:
 // the plane base (uniform: buffer + s * N) stays in scalar registers and each
@@ -650,15 +659,19 @@ FSI_HD void wall_accumulate(double* A, uint32_t y, float fx, float fy, float fz)
   if (n > a[3]) a[3] = n;
 }
 
-// The same update from an accumulator record `a` already loaded into registers.
-FSI_HD void wall_accumulate_loaded(double* A, uint32_t y, const double* a, float fx, float fy, float fz) {
-  double* o = A + 4 * (size_t)y;
+// The same update from an accumulator record `a` already loaded, written to
+// record o.
+FSI_HD void wall_accumulate_into(double* o, const double* a, float fx, float fy, float fz) {
   const double ax = a[0] + (double)fx, ay = a[1] + (double)fy, az = a[2] + (double)fz;
   const double n = (double)norm3((float)ax, (float)ay, (float)az);
   o[0] = ax;
   o[1] = ay;
   o[2] = az;
   o[3] = n > a[3] ? n : a[3];
+}
+
+FSI_HD void wall_accumulate_loaded(double* A, uint32_t y, const double* a, float fx, float fy, float fz) {
+  wall_accumulate_into(A + 4 * (size_t)y, a, fx, fy, fz);
 }
 
 // Previous-step velocity of cell c as the reference's `v` field holds it at
@@ -1100,6 +1113,27 @@ FSI_HD int res_slot(int li, int lj, int k, int di, int dj, int dk) {
   return ((li + di) * RES_BJ + (lj + dj)) * RES_NZ + ((k + dk) & (RES_NZ - 1));
 }
 
+// The block's rows and one row around them in i and j (periodic), whole
+// k-rows: the solid velocities a block's cells next to a moving wall read,
+// held for the subcycle (they are fixed during it).
+constexpr int RES_RI = RES_BI + 2, RES_RJ = RES_BJ + 2;
+constexpr int RES_RING = RES_RI * RES_RJ * RES_NZ;
+
+// Ring entry of the cell at (li + di, lj + dj, k + dk) of the block.
+FSI_HD int res_ring_slot(int li, int lj, int k, int di, int dj, int dk) {
+  return ((li + 1 + di) * RES_RJ + (lj + 1 + dj)) * RES_NZ + ((k + dk) & (RES_NZ - 1));
+}
+
+// Lattice cell of ring entry r of `block`.
+FSI_HD uint32_t res_ring_cell(const FsiLbmParams& p, uint32_t block, int r) {
+  const uint32_t nbj = (uint32_t)p.ny / RES_BJ;
+  const int bi = (int)(block / nbj), bj = (int)(block - (uint32_t)bi * nbj);
+  const int row = r / RES_NZ, k = r - row * RES_NZ;
+  const int ri = row / RES_RJ, rj = row - ri * RES_RJ;
+  const int i = wrap(bi * RES_BI + ri - 1, p.nx), j = wrap(bj * RES_BJ + rj - 1, p.ny);
+  return ((uint32_t)i * (uint32_t)p.ny + (uint32_t)j) * RES_NZ + (uint32_t)k;
+}
+
 // Why the resident subcycle cannot run (nullptr: it can), apart from the
 // device's capacity, which the GPU entry point checks.
 inline const char* res_unsupported(const FsiLbmParams& p, const FsiLbmState& st, int cur, int n, uint32_t flags,
@@ -1116,8 +1150,9 @@ inline const char* res_unsupported(const FsiLbmParams& p, const FsiLbmState& st,
   return nullptr;
 }
 
-// Phase A of fluid cell c from its post-stream populations F.  put(s, v)
-// fills slot (s, c); bounce(s, v) fills slot (LR[s], c + e_s) on a link s.
+// Phase A of fluid cell c from its post-stream populations F.  put(out)
+// fills slot (s, c) with out[s] for s = 1..18; bounce(s, v) fills slot
+// (LR[s], c + e_s) on a link s.
 // dot(s): e_s . u_wall(c + e_s) of link s (moving_wall_dots), read only by a
 // cell next to a moving wall.
 // Returns the cell's f_0 in F[0] and the velocity the collision used in
@@ -1136,19 +1171,27 @@ FSI_HD void res_collide_cell(const FsiLbmParams& p, const FsiLbmState& st, uint3
   const bool diag = last && (flags & FSI_SUB_DIAG) != 0;
   float* link = st.link_c[parity];
   F[0] = f[0];
+  // six links at a time (a cell next to a moving wall reads its wall
+  // velocities for them), the next six scheduled only after
 #pragma unroll
-  for (int s = 1; s < Q; ++s) {
-    if (info & (1u << s)) {
-      const float cr_s = moving ? wall_corr(s, rho, dot(s)) : 0.0f;
-      const float bounced = f[s] + cr_s;
-      put(s, f[s] + bounced);
-      bounce(s, bounced);
-      if (keep) at32(link + (size_t)(s - 1) * N, c) = f[s] + bounced;
-      if (diag) at32(st.link_corr + (size_t)(s - 1) * N, c) = cr_s;
-    } else {
-      put(s, f[s]);
+  for (int g0 = 1; g0 < Q; g0 += 6) {
+    float cr[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) cr[q] = moving && (info & (1u << (g0 + q))) ? wall_corr(g0 + q, rho, dot(g0 + q)) : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int s = g0 + q;
+      if (info & (1u << s)) {
+        const float bounced = f[s] + cr[q];
+        bounce(s, bounced);
+        if (keep) at32(link + (size_t)(s - 1) * N, c) = f[s] + bounced;
+        if (diag) at32(st.link_corr + (size_t)(s - 1) * N, c) = cr[q];
+        f[s] = f[s] + bounced;
+      }
     }
+    sched_fence();
   }
+  put(f);
 }
 
 // Phase B of cell L.c, a fluid cell or (when the substep gathers) a wall
@@ -1161,7 +1204,7 @@ FSI_HD void res_collide_cell(const FsiLbmParams& p, const FsiLbmState& st, uint3
 // pressure face, (obx, oby, obz), that of cell (nx - 2, j, k), 0 if solid).  A
 // wall cell sums e_s c_s in direction order with 0 on the directions without a
 // link, as wall_links_accumulate does, and adds it to its accumulator record
-// acc (loaded beforehand).
+// acc (x, y, z and the running max of the norm, updated in place).
 template <typename Pull>
 FSI_HD void res_pull_values(float* v, const Pull& pull) {
 #pragma unroll
@@ -1170,7 +1213,7 @@ FSI_HD void res_pull_values(float* v, const Pull& pull) {
 
 FSI_HD void res_take_values(const FsiLbmParams& p, const FsiLbmState& st, const ResLane& L, uint32_t info,
                             bool face_o, float ux, float uy, float uz, float obx, float oby, float obz, float* F,
-                            const double* acc, const float* v) {
+                            double* acc, const float* v) {
   if (!(info & CI_SOLID)) {
     if (face_o) {
       float brho, bx, by, bz;
@@ -1191,13 +1234,13 @@ FSI_HD void res_take_values(const FsiLbmParams& p, const FsiLbmState& st, const 
     fy += (float)EY[s] * cs;
     fz += (float)EZ[s] * cs;
   }
-  wall_accumulate_loaded(st.hydro_accum, L.c, acc, fx, fy, fz);
+  wall_accumulate_into(acc, acc, fx, fy, fz);
 }
 
 template <typename Pull>
 FSI_HD void res_pull_cell(const FsiLbmParams& p, const FsiLbmState& st, const ResLane& L, uint32_t info,
                           bool face_o, float ux, float uy, float uz, float obx, float oby, float obz, float* F,
-                          const double* acc, const Pull& pull) {
+                          double* acc, const Pull& pull) {
   float v[Q];
   res_pull_values(v, pull);
   res_take_values(p, st, L, info, face_o, ux, uy, uz, obx, oby, obz, F, acc, v);

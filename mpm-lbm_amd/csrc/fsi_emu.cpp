@@ -93,22 +93,18 @@ int fsi_lbm_subcycle_resident(const FsiLbmParams* p, const FsiLbmState* st, int 
   const uint32_t N = (uint32_t)ncells(p);
   const int nblk = res_blocks(*p);
   const size_t T = RES_THREADS;
-  std::vector<float> lds((size_t)nblk * NLINK * T), wd((size_t)nblk * NLINK * T),
+  std::vector<float> lds((size_t)nblk * NLINK * T), uw((size_t)nblk * 3 * RES_RING),
       ou((size_t)nblk * 3 * RES_BJ * RES_NZ);
   std::vector<float> R((size_t)N * Q), U((size_t)N * 3);
   const float* Fin = st->F[cur];
   for (int b = 0; b < nblk; ++b)
-    for (uint32_t t = 0; t < T; ++t) {
-      const ResLane L = res_lane(*p, (uint32_t)b, t);
-      const uint32_t info = st->cellinfo[L.c];
-      if (info & CI_SOLID) continue;
-      for (int s = 0; s < Q; ++s) R[(size_t)L.c * Q + s] = Fin[(size_t)s * N + L.c];
-      if (info & CI_MOVING) {
-        float dot[Q];
-        moving_wall_dots(*st, N, L.c, info, nbr_of(*p, L.i, L.j, L.k), dot);
-        for (int s = 1; s < Q; ++s) wd[((size_t)b * NLINK + (s - 1)) * T + t] = dot[s];
-      }
+    for (int r = 0; r < RES_RING; ++r) {
+      const uint32_t y = res_ring_cell(*p, (uint32_t)b, r);
+      for (int d = 0; d < 3; ++d) uw[((size_t)b * 3 + d) * RES_RING + r] = st->solid_vel[(size_t)d * N + y];
     }
+  for (uint32_t c = 0; c < N; ++c)
+    if (!(st->cellinfo[c] & CI_SOLID))
+      for (int s = 0; s < Q; ++s) R[(size_t)c * Q + s] = Fin[(size_t)s * N + c];
   for (int step = 0; step < n; ++step) {
     uint32_t fl = flags & (FSI_SUB_MOVING_WALL | FSI_SUB_CELL_FORCE | FSI_SUB_DIAG);
     if (step == 0) fl |= flags & FSI_SUB_STORED_MOMENTS;
@@ -126,12 +122,17 @@ int fsi_lbm_subcycle_resident(const FsiLbmParams* p, const FsiLbmState* st, int 
           continue;
         }
         const Nbr nb = nbr_of(*p, L.i, L.j, L.k);
-        auto dot = [&](int s) { return wd[((size_t)b * NLINK + (s - 1)) * T + t]; };
-        auto put = [&](int s, float v) {
-          if (res_in_block(L.li, L.lj, EX[s], EY[s]))
-            lv[(size_t)(s - 1) * T + t] = v;
-          else
-            hp[(size_t)(s - 1) * N + L.c] = v;
+        auto dot = [&](int s) {
+          const float* w = uw.data() + (size_t)b * 3 * RES_RING + res_ring_slot(L.li, L.lj, L.k, EX[s], EY[s], EZ[s]);
+          return wall_dot(s, w[0], w[RES_RING], w[2 * RES_RING]);
+        };
+        auto put = [&](const float* out) {
+          for (int s = 1; s < Q; ++s) {
+            if (res_in_block(L.li, L.lj, EX[s], EY[s]))
+              lv[(size_t)(s - 1) * T + t] = out[s];
+            else
+              hp[(size_t)(s - 1) * N + L.c] = out[s];
+          }
         };
         auto bounce = [&](int s, float v) {
           if (res_in_block(L.li, L.lj, EX[s], EY[s]))
@@ -164,16 +165,8 @@ int fsi_lbm_subcycle_resident(const FsiLbmParams* p, const FsiLbmState* st, int 
         };
         const float* o = ou.data() + (size_t)b * 3 * RES_BJ * RES_NZ + (size_t)(L.lj * RES_NZ + L.k) * 3;
         const float* u = U.data() + (size_t)L.c * 3;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        if (!fluid) {
-          const double* a = st->hydro_accum + 4 * (size_t)L.c;
-          acc[0] = a[0];
-          acc[1] = a[1];
-          acc[2] = a[2];
-          acc[3] = a[3];
-        }
         res_pull_cell(*p, *st, L, info, face_flags(*p, L.i).o, u[0], u[1], u[2], o[0], o[1], o[2],
-                      R.data() + (size_t)L.c * Q, acc, pull);
+                      R.data() + (size_t)L.c * Q, st->hydro_accum + 4 * (size_t)L.c, pull);
       }
     }
   }

@@ -644,14 +644,15 @@ template <bool FORCE>
 __global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p, FsiLbmState st, int cur, int n,
                                                                   uint32_t flags, float* __restrict__ halo,
                                                                   uint32_t* __restrict__ sync) {
-  __shared__ float lv[NLINK * RES_THREADS];  // slot planes, directions 1..18
-  __shared__ float wd[NLINK * RES_THREADS];  // e_s . u_wall of the links of cells next to a moving wall
-  __shared__ float ou[3 * RES_BJ * RES_NZ];  // velocity of the cells at i = nx - 2
+  __shared__ float lv[NLINK * RES_THREADS];     // slot planes, directions 1..18
+  __shared__ float uw[3 * RES_RING];            // solid velocities of the block and the ring around it
+  __shared__ double wacc[4 * RES_THREADS];      // wall cells' accumulator records
+  __shared__ float ou[3 * RES_BJ * RES_NZ];     // velocity of the cells at i = nx - 2
   __shared__ uint32_t s_base;
   __shared__ int bail;
   const uint32_t b = blockIdx.x, t = threadIdx.x;
   const ResLane L = res_lane(p, b, t);
-  const uint32_t N = wave_uniform((uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)RES_NZ);
+  const uint32_t Nk = wave_uniform((uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)RES_NZ);
   const uint32_t info = cell_info(st, L.c);
   const bool fluid = !(info & CI_SOLID);
   const bool wall = !fluid && (info & CI_MASK) != 0;
@@ -674,20 +675,22 @@ __global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p,
   float F[Q];
   if (fluid) {
 #pragma unroll
-    for (int s = 0; s < Q; ++s) F[s] = Fin.ld(N, s, L.c);
-    // the wall velocities are fixed for the subcycle: their dot products
-    // with the link directions are formed once
-    if (info & CI_MOVING) {
-      float dot[Q];
-      moving_wall_dots(st, N, L.c, info, nbr_of(p, L.i, L.j, L.k), dot);
+    for (int s = 0; s < Q; ++s) F[s] = Fin.ld(Nk, s, L.c);
+  }
+  // the wall velocities are fixed for the subcycle
+  for (int r = (int)t; r < RES_RING; r += RES_THREADS) {
+    const uint32_t y = res_ring_cell(p, b, r);
 #pragma unroll
-      for (int s = 1; s < Q; ++s) wd[(s - 1) * RES_THREADS + t] = dot[s];
-    }
+    for (int d = 0; d < 3; ++d) uw[d * RES_RING + r] = st.solid_vel[(size_t)d * Nk + y];
+  }
+  const bool accumulate = wall && (flags & FSI_SUB_GATHER_PREV) != 0;
+  if (accumulate) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) wacc[4 * t + d] = st.hydro_accum[4 * (size_t)L.c + d];
   }
   __syncthreads();
   const uint32_t base = s_base;
   const uint32_t fixed = FSI_SUB_MOVING_WALL | (FORCE ? FSI_SUB_CELL_FORCE : 0u) | (flags & FSI_SUB_DIAG);
-  const bool accumulate = wall && (flags & FSI_SUB_GATHER_PREV) != 0;
   float* ob = ou + (L.lj * RES_NZ + L.k) * 3;
   // the wave's row by edge class (a block's first, inner or last row in i and
   // j), for which the pulls are compiled with the LDS-or-halo choice of each
@@ -698,27 +701,37 @@ __global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p,
     const uint32_t fl = step == 0 ? fixed | (flags & FSI_SUB_STORED_MOMENTS) : fixed;
     const bool last = step == n - 1;
     const bool gather = accumulate && !last;
-    float* hp = halo + (size_t)(step & 1) * NLINK * N;
     // The lane's k, opaque to the optimiser: the cell index, neighbour offsets
     // and slot addresses derived from it are formed in every substep instead
     // of being held across the loop in registers (which spilled).
     int kk = L.k;
     asm volatile("" : "+v"(kk));
+    // likewise the lattice size, from which every plane base is formed
+    uint32_t N = Nk;
+    asm volatile("" : "+s"(N));
     ResLane Lc = L;
     Lc.k = kk;
     Lc.c = L.c - (uint32_t)L.k + (uint32_t)kk;
     const uint32_t c = Lc.c;
     const Nbr nb = nbr_of(p, Lc.i, Lc.j, kk);
+    float* hp = halo + (size_t)(step & 1) * NLINK * N;
     const uint32_t tt = (uint32_t)(Lc.li * RES_BJ + Lc.lj) * RES_NZ + (uint32_t)kk;
     if (fluid) {
       // slot (s, c) and, on a link, slot (LR[s], c + e_s): in LDS when the
       // cell reading it is in the block (a wave-uniform test), else in halo
-      auto dot = [&](int s) { return wd[(s - 1) * RES_THREADS + tt]; };
-      auto put = [&](int s, float v) {
-        if (res_in_block(Lc.li, Lc.lj, EX[s], EY[s]))
-          lv[(s - 1) * RES_THREADS + tt] = v;
-        else
-          res_put(hp + (size_t)(s - 1) * N, c, v);
+      auto dot = [&](int s) {
+        const float* w = uw + res_ring_slot(Lc.li, Lc.lj, kk, EX[s], EY[s], EZ[s]);
+        return wall_dot(s, w[0], w[RES_RING], w[2 * RES_RING]);
+      };
+      // the values that leave the block first, so that their stores are under
+      // way while the block's own go to LDS
+      auto put = [&](const float* out) {
+#pragma unroll
+        for (int s = 1; s < Q; ++s)
+          if (!res_in_block(Lc.li, Lc.lj, EX[s], EY[s])) res_put(hp + (size_t)(s - 1) * N, c, out[s]);
+#pragma unroll
+        for (int s = 1; s < Q; ++s)
+          if (res_in_block(Lc.li, Lc.lj, EX[s], EY[s])) lv[(s - 1) * RES_THREADS + tt] = out[s];
       };
       auto bounce = [&](int s, float v) {
         if (res_in_block(Lc.li, Lc.lj, EX[s], EY[s]))
@@ -732,14 +745,6 @@ __global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p,
       ob[0] = fluid ? ux : 0.0f;
       ob[1] = fluid ? uy : 0.0f;
       ob[2] = fluid ? uz : 0.0f;
-    }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (gather) {
-      const double* a = st.hydro_accum + 4 * (size_t)c;
-      acc[0] = a[0];
-      acc[1] = a[1];
-      acc[2] = a[2];
-      acc[3] = a[3];
     }
     __builtin_amdgcn_s_waitcnt(0);  // this wave's halo stores (and loads) have completed
     __syncthreads();
@@ -779,13 +784,17 @@ __global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p,
         FSI_RES_PULL(2, 2)
 #undef FSI_RES_PULL
       }
-      res_take_values(p, st, Lc, info, face_o, ux, uy, uz, ob[0], ob[1], ob[2], F, acc, v);
+      res_take_values(p, st, Lc, info, face_o, ux, uy, uz, ob[0], ob[1], ob[2], F, wacc + 4 * tt, v);
     }
     __syncthreads();
   }
   if (fluid) {
 #pragma unroll
-    for (int s = 0; s < Q; ++s) Fout.st(N, s, L.c, F[s]);
+    for (int s = 0; s < Q; ++s) Fout.st(Nk, s, L.c, F[s]);
+  }
+  if (accumulate) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) st.hydro_accum[4 * (size_t)L.c + d] = wacc[4 * t + d];
   }
 }
 
