@@ -11,10 +11,13 @@ back between two synchronisations; each FSI step is projection, flag
 rebuild, 120 fused LBM substeps, finalize, moments and 10 MPM substeps).
 
 Also reported:
-  roofline     the fused LBM substep kernel: algorithmic bytes per launch
-               (152 B per fluid cell + 4 B cell word per cell) over its
-               average duration, timed live with HIP events on the stream
-               the kernels run on, against 8 TB/s;
+  roofline     the resident subcycle kernel (k_lbm_resident: one launch runs
+               the FSI step's 120 LBM substeps with the lattice on chip):
+               algorithmic bytes per launch (120 x (152 B per fluid cell +
+               4 B cell word per cell), what the substeps would move through
+               HBM one launch each) over its average duration, timed live
+               with HIP events on the stream the kernels run on, against
+               8 TB/s; the launch-per-substep kernel is timed beside it;
   copy_ceiling device-to-device copy bandwidth measured in the same run;
   cpu_baseline the reference-structured C++ oracle (OpenMP) on the host
                cores: median of 3 FSI steps of the same workload after one
@@ -49,10 +52,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2, help="untimed FSI steps")
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--substeps", type=int, default=120)
-    ap.add_argument("--kernel-launches", type=int, default=600, help="substep launches timed for the roofline")
+    ap.add_argument("--kernel-launches", type=int, default=600,
+                    help="launch-per-substep kernel launches timed beside the resident subcycle")
+    ap.add_argument("--resident-launches", type=int, default=10, help="resident subcycle launches timed for the roofline")
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="timed FSI steps of the CPU baseline sample, after one warm-up step; median (0 = skip)")
-    ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "lbm_substep_64cube_traffic.json"))
+    ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "lbm_resident_64cube_traffic.json"))
+    ap.add_argument("--push-traffic-json", default=str(ROOT / "profiles" / "lbm_substep_64cube_traffic.json"))
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched sweep (0 = skip)")
     ap.add_argument("--batch-exchanges", type=int, default=3, help="timed batched exchanges")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the served form (0 = skip)")
@@ -119,6 +125,31 @@ def time_substep_kernel(torch, lbm, launches):
     torch.cuda.synchronize()
     lbm._moments_fresh = False
     return s.elapsed_time(e) / launches
+
+
+def time_resident_kernel(torch, lbm, launches, n_sub, flags):
+    """Average duration of one resident subcycle launch (k_lbm_resident: the
+    FSI step's n_sub moving-wall substeps with the lattice on chip), issued
+    with the FSI step's flags on the live lattice: HIP events around each of
+    `launches` launches, on the stream the kernel runs on."""
+    import ctypes
+
+    from mpm_lbm import _native
+
+    stream = lbm._stream()
+    durations = []
+    for _ in range(launches + 1):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        lbm.native.call("fsi_lbm_subcycle_resident", ctypes.byref(lbm._params), ctypes.byref(lbm._state), lbm.cur,
+                        n_sub, flags, _native.ptr(lbm._res_halo), _native.ptr(lbm._res_sync), stream)
+        e.record()
+        torch.cuda.synchronize()
+        lbm.cur = (lbm.cur + n_sub) % 2
+        durations.append(s.elapsed_time(e))
+    lbm._moments_fresh = False
+    lbm.check_resident_waits()
+    return sum(durations[1:]) / launches  # the first launch is a warm-up
 
 
 def _timed(torch, fn):
@@ -374,15 +405,28 @@ def main():
         raise RuntimeError("non-finite diagnostics after the timed steps")
 
     fluid = int(np.count_nonzero(d.lbm.solid.to_numpy() == 0))
-    algo_bytes = fluid * 152 + n ** 3 * 4
+    algo_bytes = fluid * 152 + n ** 3 * 4  # per substep
+    resident = d.lbm._res_sync is not None and not d.lbm._res_declined
+    if not resident:
+        raise RuntimeError("the 64^3 subcycle did not take the resident kernel")
+    from mpm_lbm import _native
+
+    res_flags = _native.SUB_MOVING_WALL | _native.SUB_GATHER_PREV | _native.SUB_DIAG
+    r_ms = time_resident_kernel(torch, d.lbm, args.resident_launches, sub, res_flags)
+    achieved = sub * algo_bytes / (r_ms * 1e-3) / 1e9
+    log(f"[bench] resident subcycle: {r_ms * 1e3:.1f} us avg over {args.resident_launches} launches = "
+        f"{r_ms * 1e3 / sub:.3f} us per substep, {achieved:.0f} GB/s")
     k_ms = time_substep_kernel(torch, d.lbm, args.kernel_launches)
-    achieved = algo_bytes / (k_ms * 1e-3) / 1e9
-    log(f"[bench] fused substep: {k_ms * 1e3:.2f} us avg over {args.kernel_launches} launches, {achieved:.0f} GB/s")
+    push_gbs = algo_bytes / (k_ms * 1e-3) / 1e9
+    log(f"[bench] launch-per-substep kernel: {k_ms * 1e3:.2f} us avg over {args.kernel_launches} launches, "
+        f"{push_gbs:.0f} GB/s")
     copy_gbs = copy_ceiling_gbs(torch)
-    traffic = None
-    tp = Path(args.traffic_json)
-    if tp.is_file():
-        traffic = json.loads(tp.read_text()).get("hbm_bytes_per_launch")
+
+    def _traffic(path):
+        tp = Path(path)
+        return json.loads(tp.read_text()).get("hbm_bytes_per_launch") if tp.is_file() else None
+
+    traffic, push_traffic = _traffic(args.traffic_json), _traffic(args.push_traffic_json)
 
     batched = None
     if args.batch > 0:
@@ -428,8 +472,15 @@ def main():
             "mfluops": round(fluid * sub * args.steps / secs / 1e6, 2),
             "particle_updates_per_s": round(w.config.n_particles * w.config.mpm_substeps_per_lbm_step * args.steps / secs, 1),
             "copy_ceiling_gbs": round(copy_gbs, 1),
-            "substep_frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
-            "substep_launch_tuning": launch_tuning,
+            "resident_us_per_substep": round(r_ms * 1e3 / sub, 3),
+            "launch_per_substep_kernel": {
+                "kernel": "k_lbm_substep (fused collide+stream+bounce-back+BC, one launch per substep)",
+                "avg_launch_us": round(k_ms * 1e3, 3),
+                "achieved_gbs": round(push_gbs, 1),
+                "frac": round(push_gbs / HBM_PEAK_GBS, 4),
+                "traffic": push_traffic,
+                "launch_tuning": launch_tuning,
+            },
             "batched": batched,
             "served": served,
             "precision": precision,
@@ -441,9 +492,10 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "k_lbm_substep (fused collide+stream+bounce-back+BC)",
-            "algorithmic_bytes_per_launch": algo_bytes,
-            "avg_launch_us": round(k_ms * 1e3, 3),
+            "kernel": f"k_lbm_resident ({sub} moving-wall substeps per launch, the lattice on chip)",
+            "algorithmic_bytes_per_launch": sub * algo_bytes,
+            "algorithmic_bytes_per_substep": algo_bytes,
+            "avg_launch_us": round(r_ms * 1e3, 3),
         },
         "cpu_baseline": cpu,
     }

@@ -536,6 +536,7 @@ class FSIDriver3D:
     def collect_diagnostics(self, step: int):
         """fsi_driver.py:597-669: the same 25 columns, reduced on the device."""
         t0 = time.perf_counter()
+        self.lbm.check_resident_waits()
         ls = self.lbm.reduce_stats()
         ms = self.solid.reduce_stats()
         fluid = FSIDiagnostics3D.lbm_fluid_stats(self.lbm, stats=ls)
