@@ -33,9 +33,15 @@
 // the oracle's plain C++ does: the library is built with
 // -ffp-contract=fast-honor-pragmas, so these turn a*b+c -> fma fusion off
 // (inside a function body, or from BEGIN to END at file scope) and back on.
+// The build without fma fusion (FSI_EXACT_BUILD, -ffp-contract=off) keeps it
+// off after END as well, so no kernel of that build fuses anything.
 #define FSI_NO_CONTRACT _Pragma("clang fp contract(off)")
 #define FSI_NO_CONTRACT_BEGIN _Pragma("clang fp contract(off)")
+#ifdef FSI_EXACT_BUILD
+#define FSI_NO_CONTRACT_END _Pragma("clang fp contract(off)")
+#else
 #define FSI_NO_CONTRACT_END _Pragma("clang fp contract(fast)")
+#endif
 #else
 #define FSI_HD static inline
 #define FSI_HDM inline
@@ -1610,6 +1616,14 @@ FSI_HD Stencil particle_stencil(const float* x, long long P, long long p, float 
   return s;
 }
 
+// The stencil of a particle at position x (held by the caller).
+FSI_HD Stencil stencil_at(const float* x, float inv) {
+  Stencil s;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline(x[d] * inv, s.base[d], s.fx[d], s.w[d]);
+  return s;
+}
+
 FSI_HD float det3(const float* A) {
   return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
          A[2] * (A[3] * A[7] - A[4] * A[6]);
@@ -1925,7 +1939,26 @@ FSI_HD void mpm_p2g_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
   mpm_p2g_particle(mp, ms, p, particle_stencil(ms.x, mp.n_particles, p, mp.inv_dx), sink);
 }
 
-// grid_update + apply_boundary_at_node (solid.py:252-259, :97-111).
+// grid_update + apply_boundary_at_node (solid.py:252-259, :97-111) of a
+// node (I, J, K) of mass m > 0: its velocity from momentum mom and force f.
+FSI_HD void grid_node_velocity(const FsiMpmParams& mp, float m, const float* mom, const float* f, int I, int J, int K,
+                               float* v) {
+  const int idx[3] = {I, J, K};
+  const int ext[3] = {mp.gnx, mp.gny, mp.gnz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float vd = mom[d] / m;
+    vd += mp.dt * mp.gravity[d];
+    vd += mp.dt * f[d] / m;
+    v[d] = vd;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (idx[d] < mp.bound && v[d] < 0.0f) v[d] = 0.0f;
+    if (idx[d] > ext[d] - mp.bound && v[d] > 0.0f) v[d] = 0.0f;
+  }
+}
+
 FSI_HD void mpm_grid_node(const FsiMpmParams& mp, const FsiMpmState& ms, long long n) {
   const MpmGrid g = make_mgrid(mp);
   const float m = ms.grid_m[n];
@@ -1933,21 +1966,96 @@ FSI_HD void mpm_grid_node(const FsiMpmParams& mp, const FsiMpmState& ms, long lo
   const int K = (int)(n % g.nz);
   const int Jn = (int)((n / g.nz) % g.ny);
   const int I = (int)(n / ((long long)g.nz * g.ny));
+  const float mom[3] = {ms.grid_v[n], ms.grid_v[g.G + n], ms.grid_v[2 * g.G + n]};
+  const float f[3] = {ms.grid_f[n], ms.grid_f[g.G + n], ms.grid_f[2 * g.G + n]};
   float v[3];
-  const int idx[3] = {I, Jn, K};
-  const int ext[3] = {g.nx, g.ny, g.nz};
+  grid_node_velocity(mp, m, mom, f, I, Jn, K, v);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    float vd = ms.grid_v[d * g.G + n] / m;
-    vd += mp.dt * mp.gravity[d];
-    vd += mp.dt * ms.grid_f[d * g.G + n] / m;
-    v[d] = vd;
+  for (int d = 0; d < 3; ++d) ms.grid_v[d * g.G + n] = v[d];
+}
+
+// The per-particle part of g2p + clamp + dynamics controls + fixed pin
+// (solid.py:262-315, :114-124): from the node velocities gv of particle p's
+// stencil (node k = 9a + 3b + c; nodes outside the grid cleared in inm), its
+// position x0 and deformation gradient Fm, the new x, v, C, F and det F.
+FSI_HD void g2p_update(const FsiMpmParams& mp, const FsiMpmState& ms, long long p, const float* fx,
+                       const float (&w)[3][3], const float (&gv)[27][3], uint32_t inm, const float* x0,
+                       const float* Fm, float* x, float* v, float* nC, float* Fn, float& Jn) {
+  FSI_NO_CONTRACT
+  const long long P = mp.n_particles;
+  float nv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int e = 0; e < 9; ++e) nC[e] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    if (!((inm >> k) & 1u)) continue;
+    const int a = k / 9, b = (k / 3) % 3, cc = k % 3;
+    const float dp[3] = {((float)a - fx[0]) * mp.dx, ((float)b - fx[1]) * mp.dx, ((float)cc - fx[2]) * mp.dx};
+    const float wt = w[0][a] * w[1][b] * w[2][cc];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      nv[r] += wt * gv[k][r];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) nC[r * 3 + c] += 4.0f * wt * gv[k][r] * dp[c] * mp.inv_dx * mp.inv_dx;
+    }
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (idx[d] < mp.bound && v[d] < 0.0f) v[d] = 0.0f;
-    if (idx[d] > ext[d] - mp.bound && v[d] > 0.0f) v[d] = 0.0f;
-    ms.grid_v[d * g.G + n] = v[d];
+    v[d] = nv[d];
+    x[d] = x0[d] + mp.dt * nv[d];
+  }
+  const float eps = mp.dx;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (x[d] < eps) {
+      x[d] = eps;
+      if (v[d] < 0.0f) v[d] = 0.0f;
+    }
+    if (x[d] > mp.domain_max[d] - eps) {
+      x[d] = mp.domain_max[d] - eps;
+      if (v[d] > 0.0f) v[d] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float a0 = (r == 0 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 0];
+      const float a1 = (r == 1 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 1];
+      const float a2 = (r == 2 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 2];
+      Fn[r * 3 + c] = a0 * Fm[0 * 3 + c] + a1 * Fm[1 * 3 + c] + a2 * Fm[2 * 3 + c];
+    }
+  Jn = det3(Fn);
+  if (mp.damping_enabled) {
+    float df = 1.0f - mp.velocity_damping * mp.dt;
+    df = df > 0.0f ? df : 0.0f;
+    v[0] *= df; v[1] *= df; v[2] *= df;
+  }
+  if (mp.lock_z_enabled) {
+    x[2] = ms.fixed_x[2 * P + p];
+    v[2] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      nC[2 * 3 + d] = 0.0f;
+      nC[d * 3 + 2] = 0.0f;
+      Fn[2 * 3 + d] = 0.0f;
+      Fn[d * 3 + 2] = 0.0f;
+    }
+    Fn[8] = 1.0f;
+    Jn = det3(Fn);
+  }
+  if (ms.fixed_mask[p] != 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[d] = ms.fixed_x[d * P + p];
+      v[d] = 0.0f;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      nC[e] = 0.0f;
+      Fn[e] = (e == 0 || e == 4 || e == 8) ? 1.0f : 0.0f;
+    }
+    Jn = 1.0f;
   }
 }
 
@@ -1962,8 +2070,6 @@ FSI_HD void mpm_g2p_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
   for (int d = 0; d < 3; ++d) bspline(ms.x[d * P + p] * mp.inv_dx, base[d], fx[d], w[d]);
   float Fm[9];
   load_mat(ms.F, P, p, Fm);
-  float nv[3] = {0.0f, 0.0f, 0.0f};
-  float nC[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   // Loads first, then the sums in node order: 9 rows of 3 nodes along k, one
   // 3-float load per row and channel from a row start kept inside the grid
   // (a row outside in i or j reads row 0), nodes outside flagged off.
@@ -1996,79 +2102,9 @@ FSI_HD void mpm_g2p_particle(const FsiMpmParams& mp, const FsiMpmState& ms, long
       inm |= (row_ok[r] && 0 <= K && K < g.nz) ? (1u << (3 * r + c)) : 0u;
     }
   }
-#pragma unroll
-  for (int k = 0; k < 27; ++k) {
-    if (!((inm >> k) & 1u)) continue;
-    const int a = k / 9, b = (k / 3) % 3, cc = k % 3;
-    const float dp[3] = {((float)a - fx[0]) * mp.dx, ((float)b - fx[1]) * mp.dx, ((float)cc - fx[2]) * mp.dx};
-    const float wt = w[0][a] * w[1][b] * w[2][cc];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      nv[r] += wt * gv[k][r];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) nC[r * 3 + c] += 4.0f * wt * gv[k][r] * dp[c] * mp.inv_dx * mp.inv_dx;
-    }
-  }
-  float x[3], v[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    v[d] = nv[d];
-    x[d] = ms.x[d * P + p] + mp.dt * nv[d];
-  }
-  const float eps = mp.dx;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    if (x[d] < eps) {
-      x[d] = eps;
-      if (v[d] < 0.0f) v[d] = 0.0f;
-    }
-    if (x[d] > mp.domain_max[d] - eps) {
-      x[d] = mp.domain_max[d] - eps;
-      if (v[d] > 0.0f) v[d] = 0.0f;
-    }
-  }
-  float Fn[9];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float a0 = (r == 0 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 0];
-      const float a1 = (r == 1 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 1];
-      const float a2 = (r == 2 ? 1.0f : 0.0f) + mp.dt * nC[r * 3 + 2];
-      Fn[r * 3 + c] = a0 * Fm[0 * 3 + c] + a1 * Fm[1 * 3 + c] + a2 * Fm[2 * 3 + c];
-    }
-  float Jn = det3(Fn);
-  if (mp.damping_enabled) {
-    float df = 1.0f - mp.velocity_damping * mp.dt;
-    df = df > 0.0f ? df : 0.0f;
-    v[0] *= df; v[1] *= df; v[2] *= df;
-  }
-  if (mp.lock_z_enabled) {
-    x[2] = ms.fixed_x[2 * P + p];
-    v[2] = 0.0f;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      nC[2 * 3 + d] = 0.0f;
-      nC[d * 3 + 2] = 0.0f;
-      Fn[2 * 3 + d] = 0.0f;
-      Fn[d * 3 + 2] = 0.0f;
-    }
-    Fn[8] = 1.0f;
-    Jn = det3(Fn);
-  }
-  if (ms.fixed_mask[p] != 0) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      x[d] = ms.fixed_x[d * P + p];
-      v[d] = 0.0f;
-    }
-#pragma unroll
-    for (int e = 0; e < 9; ++e) {
-      nC[e] = 0.0f;
-      Fn[e] = (e == 0 || e == 4 || e == 8) ? 1.0f : 0.0f;
-    }
-    Jn = 1.0f;
-  }
+  const float x0[3] = {ms.x[p], ms.x[P + p], ms.x[2 * P + p]};
+  float x[3], v[3], nC[9], Fn[9], Jn;
+  g2p_update(mp, ms, p, fx, w, gv, inm, x0, Fm, x, v, nC, Fn, Jn);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     ms.x[d * P + p] = x[d];

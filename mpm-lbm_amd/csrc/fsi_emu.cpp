@@ -374,6 +374,110 @@ int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* l
   return fsi_mpm_substep_coupled(lp, ls, mp, ms, cp, cc, mode, mode ? scratch.data() : nullptr, stream);
 }
 
+// The resident MPM subcycle as the kernel runs it, phase by phase (every
+// workgroup's scatter, the barrier, every workgroup's gather): substep s
+// scatters into grid buffer (n - 1 - s) mod 3 (0: the state's grid, 1 and 2
+// in scratch), each particle block clears the nodes of its stencil box of
+// substep s - 2 in buffer (n + 1 - s) mod 3, and g2p reads node velocities
+// computed from the buffer.  The reaction goes to grid_f (the emulation's
+// p2g and reaction are separate passes); the last buffer is converted to
+// velocities at the end.
+int fsi_mpm_subcycle_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                             const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode,
+                             int n, double* workspace, float* scratch, uint32_t* sync, void* stream) {
+  if (!mp || !ms) return fail(FSI_E_INVALID, "null MPM params/state");
+  if (mode < 0 || mode > 2) return fail(FSI_E_INVALID, "mode must be 0 (none), 1 (penalty) or 2 (moving boundary)");
+  if (n < 1) return fail(FSI_E_INVALID, "n must be at least 1");
+  if (!scratch || !sync) return fail(FSI_E_INVALID, "the resident MPM subcycle needs scratch and sync");
+  if (mode && (!lp || !ls || !cp || !cc || !workspace))
+    return fail(FSI_E_INVALID, "the reaction needs coupling params, counters, workspace, hydro_force, solid_phi");
+  if (mode && cp->inv_dx_norm != mp->inv_dx)
+    return fail(FSI_E_UNSUPPORTED, "the reaction samples the lattice at another spacing than the MPM grid");
+  if (mp->n_particles <= 0) return fail(FSI_E_UNSUPPORTED, "no particles");
+  if (mp->gnx < 3 || mp->gny < 3 || mp->gnz < 3)
+    return fail(FSI_E_INVALID, "MPM grid must be at least 3 nodes along every axis");
+  const long long G = (long long)mp->gnx * mp->gny * mp->gnz, P = mp->n_particles;
+  const long long nb = (P + FSI_PARTICLE_BLOCK - 1) / FSI_PARTICLE_BLOCK;
+  auto buf = [&](int k, FsiMpmState& b) {
+    b = *ms;
+    if (k) {
+      b.grid_v = scratch + (size_t)(k - 1) * 4 * G;
+      b.grid_m = b.grid_v + 3 * G;
+    }
+  };
+  FsiMpmState b0;
+  for (int k = 0; k < 3; ++k) {
+    buf(k, b0);
+    memset(b0.grid_v, 0, sizeof(float) * 3 * G);
+    memset(b0.grid_m, 0, sizeof(float) * G);
+  }
+  std::vector<int> box(2 * 6 * nb, 0);  // per block: [s - 1] lo, hi; [s - 2] lo, hi
+  for (long long b = 0; b < nb; ++b)
+    for (int k = 0; k < 2; ++k)
+      for (int d = 0; d < 3; ++d) {
+        box[(b * 2 + k) * 6 + d] = 0;
+        box[(b * 2 + k) * 6 + 3 + d] = -1;
+      }
+  std::vector<float> vel(3 * (size_t)G);
+  for (int s = 0; s < n; ++s) {
+    FsiMpmState cur;
+    buf((n - 1 - s) % 3, cur);
+    memset(ms->grid_f, 0, sizeof(float) * 3 * G);
+    int rc = fsi_mpm_p2g(mp, &cur, stream);
+    if (!rc && mode) rc = fsi_mpm_reaction(lp, ls, mp, &cur, cp, cc, mode, stream);
+    if (rc) return rc;
+    FsiMpmState old;
+    buf((n + 1 - s) % 3, old);
+    for (long long b = 0; b < nb; ++b) {
+      int* bx = &box[b * 12];
+      if (s >= 2) {
+        const int* lo = bx + 6;
+        const int* hi = bx + 9;
+        for (int I = lo[0] < 0 ? 0 : lo[0]; I <= hi[0] && I < mp->gnx; ++I)
+          for (int J = lo[1] < 0 ? 0 : lo[1]; J <= hi[1] && J < mp->gny; ++J)
+            for (int K = lo[2] < 0 ? 0 : lo[2]; K <= hi[2] && K < mp->gnz; ++K) {
+              const long long nn = ((long long)I * mp->gny + J) * mp->gnz + K;
+              for (int d = 0; d < 3; ++d) old.grid_v[d * G + nn] = 0.0f;
+              old.grid_m[nn] = 0.0f;
+            }
+      }
+      for (int e = 0; e < 6; ++e) bx[6 + e] = bx[e];
+      int lo[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, hi[3] = {-0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF};
+      for (long long q = b * FSI_PARTICLE_BLOCK; q < P && q < (b + 1) * FSI_PARTICLE_BLOCK; ++q) {
+        const Stencil sn = particle_stencil(ms->x, P, q, mp->inv_dx);
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = sn.base[d] < lo[d] ? sn.base[d] : lo[d];
+          hi[d] = sn.base[d] + 2 > hi[d] ? sn.base[d] + 2 : hi[d];
+        }
+      }
+      for (int d = 0; d < 3; ++d) {
+        bx[d] = lo[d];
+        bx[3 + d] = hi[d];
+      }
+    }
+    // the gather's node velocities
+    for (long long nn = 0; nn < G; ++nn) {
+      const float m = cur.grid_m[nn];
+      const float mom[3] = {cur.grid_v[nn], cur.grid_v[G + nn], cur.grid_v[2 * G + nn]};
+      float v[3] = {mom[0], mom[1], mom[2]};
+      if (m > 0.0f) {
+        const int K = (int)(nn % mp->gnz), J = (int)((nn / mp->gnz) % mp->gny),
+                  I = (int)(nn / ((long long)mp->gnz * mp->gny));
+        const float f[3] = {ms->grid_f[nn], ms->grid_f[G + nn], ms->grid_f[2 * G + nn]};
+        grid_node_velocity(*mp, m, mom, f, I, J, K, v);
+      }
+      for (int d = 0; d < 3; ++d) vel[d * G + nn] = v[d];
+    }
+    FsiMpmState gat = *ms;
+    gat.grid_v = vel.data();
+    rc = fsi_mpm_g2p(mp, &gat, stream);
+    if (rc) return rc;
+  }
+  // buffer 0 (the state's grid) to velocities
+  for (long long nn = 0; nn < G; ++nn) mpm_grid_node(*mp, *ms, nn);
+  return FSI_OK;
+}
+
 // Binning, serially: keys in particle order, counts, exclusive scan, and
 // slots taken in particle order (so each bin's list is already increasing).
 static void bin_serial(const float* x, int32_t P, float inv, const BinGrid& bg, const FsiBins& b) {

@@ -1436,15 +1436,23 @@ __global__ void __launch_bounds__(PTPB) k_mpm_p2g_react(FsiLbmParams lp, FsiLbmS
                         sd, sf, si);
 }
 
-// Fold of the per-block reaction records (in block order: deterministic) into
-// the counters, by one whole workgroup of TPB threads.
+// Fold of the per-block reaction records into the counters, by one whole
+// workgroup: lane l of its first wave sums records l, l + 64, ... and the
+// workgroup reduction adds the lanes, so the order (and the result) is the
+// same for every workgroup size and launch.
 __device__ __forceinline__ void fold_react_records(const double* rec, int n_rec, FsiCouplingCounters* cc,
                                                    double* sd) {
   double a[REACT_REC];
 #pragma unroll
   for (int e = 0; e < REACT_REC; ++e) a[e] = 0.0;
-  for (int k = threadIdx.x; k < n_rec; k += (int)blockDim.x) {
-    const double* r = rec + (size_t)k * REACT_REC;
+  for (int k = threadIdx.x; threadIdx.x < 64 && k < n_rec; k += 64) {
+    // agent-scope loads: the records may come from other workgroups of the
+    // same launch (the resident MPM subcycle)
+    double r[REACT_REC];
+#pragma unroll
+    for (int e = 0; e < REACT_REC; ++e)
+      r[e] = __hip_atomic_load(const_cast<double*>(rec) + (size_t)k * REACT_REC + e, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     a[0] += r[0];
     a[1] = r[1] > a[1] ? r[1] : a[1];
     a[2] = r[2] > a[2] ? r[2] : a[2];
@@ -1472,6 +1480,358 @@ __global__ void __launch_bounds__(TPB) k_mpm_grid_update_react(FsiMpmParams mp, 
   const long long n = (long long)blockIdx.x * TPB + threadIdx.x;
   if (n < (long long)mp.gnx * mp.gny * mp.gnz) mpm_grid_node(mp, ms, n);
   if (rec && blockIdx.x == 0) fold_react_records(rec, n_rec, cc, sd);
+}
+
+// ---------------------------------------------------------------------------
+// Resident MPM subcycle (fsi_mpm_subcycle_coupled): n substeps of
+// fsi_mpm_substep_coupled in one launch.  One PTPB workgroup per particle
+// block, every workgroup resident at once; each lane keeps its particle's x,
+// v, C and F in registers for the whole subcycle.
+//
+// Substep s scatters into grid buffer buf(s) = (n - 1 - s) mod 3 (buffer 0 is
+// the state's grid_v / grid_m, so the last substep's grid is the state's;
+// buffers 1 and 2 live in the caller's scratch) and G2P reads it back, with
+// the grid update applied per gathered node.  With three buffers one grid
+// barrier per substep, between the scatter and the gather, suffices: during
+// its scatter of substep s a workgroup also clears the nodes it scattered to
+// in substep s - 2 (the box of its stencils then), in buf(s + 1), which every
+// gather of substep s - 2 finished reading before the barrier of substep s - 1
+// and which the scatter of substep s + 1 writes only after the barrier of s.
+// A first barrier follows the clear of all three buffers (and grid_f) at the
+// start; a last one precedes the fold of the last substep's reaction records
+// and the conversion of buffer 0 to velocities, so the state after the call
+// is that of n calls of fsi_mpm_substep_coupled.
+//
+// Grid values cross workgroups (and XCDs) only through agent-scope atomics:
+// the scatter's float adds, the clears as atomic stores, and the gather's
+// atomic loads, which read the coherent level and not an XCD's L2.  Before a
+// workgroup counts itself in at a barrier its wave waits for its own stores
+// and adds to complete (s_waitcnt vmcnt(0)).  The barrier counts arrivals in
+// eight words (workgroup b in word b mod 8); lanes 0..7 poll them, at most
+// MPM_RES_MAX_POLLS times (then the timeout word is counted and the
+// workgroup goes on: no launch can hang).  The last workgroup to leave resets
+// the arrival words, so every launch starts from zero.
+// ---------------------------------------------------------------------------
+constexpr uint32_t MPM_RES_STRIDE = 64;  // words between counters (separate lines)
+constexpr int MPM_RES_MAX_POLLS = 1 << 18;
+constexpr uint32_t MPM_RES_EXIT = 8, MPM_RES_TIMEOUT = 9;
+
+struct MpmResBuf {
+  float* mom;  // momentum x, y, z (G apart)
+  float* m;    // mass
+};
+
+__device__ __forceinline__ MpmResBuf mpm_res_buf(const FsiMpmState& ms, float* scratch, long long G, int k) {
+  MpmResBuf b;
+  if (k == 0) {
+    b.mom = ms.grid_v;
+    b.m = ms.grid_m;
+  } else {
+    b.mom = scratch + (size_t)(k - 1) * 4 * G;
+    b.m = b.mom + 3 * G;
+  }
+  return b;
+}
+
+__device__ __forceinline__ void agent_store(float* a, float v) {
+  __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float agent_load(const float* a) {
+  return __hip_atomic_load(const_cast<float*>(a), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier k (1, 2, ... within the launch) over nb workgroups.
+__device__ __forceinline__ void mpm_res_barrier(uint32_t* sync, int nb, uint32_t k, int& bail) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's adds, stores and loads have completed
+  __syncthreads();
+  const uint32_t lane = threadIdx.x;
+  if (lane == 0)
+    __hip_atomic_fetch_add(sync + MPM_RES_STRIDE * (blockIdx.x & 7u), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!bail) {
+    // word w counts the workgroups b = w mod 8 of [0, nb)
+    const uint32_t want = lane < 8 ? k * (((uint32_t)nb + 7u - lane) / 8u) : 0u;
+    int polls = 0;
+    while (true) {
+      const uint32_t got =
+          lane < 8 ? __hip_atomic_load(sync + MPM_RES_STRIDE * lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+      if (__ballot(lane < 8 && (int32_t)(got - want) < 0) == 0) break;
+      if (++polls > MPM_RES_MAX_POLLS) {
+        if (lane == 0) atomicAdd(sync + MPM_RES_STRIDE * MPM_RES_TIMEOUT, 1u);
+        bail = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+}
+
+// Agent-scope zero of nodes [lo, hi] (clipped to the grid) of buffer b.
+__device__ __forceinline__ void mpm_res_clear_box(const MpmResBuf& b, const int* lo, const int* hi, int gnx, int gny,
+                                                  int gnz, long long G) {
+  int l[3], e[3];
+  const int g[3] = {gnx, gny, gnz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    l[d] = lo[d] < 0 ? 0 : lo[d];
+    const int h = hi[d] > g[d] - 1 ? g[d] - 1 : hi[d];
+    e[d] = h - l[d] + 1;
+  }
+  if (e[0] <= 0 || e[1] <= 0 || e[2] <= 0) return;
+  const int V = e[0] * e[1] * e[2], plane = e[1] * e[2];
+  for (int i = threadIdx.x; i < V; i += PTPB) {
+    const int a = i / plane, r = i - a * plane, c = r / e[2], f = r - c * e[2];
+    const long long n = ((long long)(l[0] + a) * gny + (l[1] + c)) * gnz + (l[2] + f);
+    agent_store(b.mom + n, 0.0f);
+    agent_store(b.mom + G + n, 0.0f);
+    agent_store(b.mom + 2 * G + n, 0.0f);
+    agent_store(b.m + n, 0.0f);
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmState ls, FsiMpmParams mp,
+                                                       FsiMpmState ms, FsiCouplingParams cp, int n,
+                                                       float* __restrict__ scratch, double* __restrict__ rec,
+                                                       FsiCouplingCounters* cc, uint32_t* __restrict__ sync) {
+  __shared__ __align__(16) float tile[TILE_COPIES * 4 * FUSED_TILE_NODES];
+  __shared__ double sd[PTPB / 64];
+  __shared__ float sf[PTPB / 64];
+  __shared__ int si[PTPB / 64];
+  const int nb = (int)gridDim.x;
+  const long long P = mp.n_particles, G = (long long)mp.gnx * mp.gny * mp.gnz;
+  const long long q = (long long)blockIdx.x * PTPB + threadIdx.x;
+  const bool valid = q < P;
+  const long long qs = valid ? q : 0;
+  int bail = 0;
+  // the start: every buffer and grid_f cleared, a slice per workgroup
+  {
+    const long long per = (G + nb - 1) / nb, n0 = per * blockIdx.x, n1 = n0 + per < G ? n0 + per : G;
+    for (long long i = n0 + threadIdx.x; i < n1; i += PTPB) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        agent_store(ms.grid_v + d * G + i, 0.0f);
+        agent_store(ms.grid_f + d * G + i, 0.0f);
+      }
+      agent_store(ms.grid_m + i, 0.0f);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) agent_store(scratch + e * G + i, 0.0f);
+    }
+  }
+  // the particle's state, held for the subcycle
+  float x[3], v[3], C[9], F[9];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    x[d] = ms.x[d * P + qs];
+    v[d] = ms.v[d * P + qs];
+  }
+  load_mat(ms.C, P, qs, C);
+  load_mat(ms.F, P, qs, F);
+  const float mass = ms.mass[qs], vol0 = ms.vol0[qs];
+  // boxes of the last two substeps' stencils (uniform): [0] substep s - 1, [1] s - 2
+  int blo[2][3], bhi[2][3];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      blo[k][d] = 0;
+      bhi[k][d] = -1;
+    }
+  uint32_t bar = 0;
+  mpm_res_barrier(sync, nb, ++bar, bail);
+  const bool same = cp.inv_dx_norm == mp.inv_dx;  // checked by the host
+  for (int s = 0; s < n; ++s) {
+    const int kb = (n - 1 - s) % 3;
+    const MpmResBuf cur = mpm_res_buf(ms, scratch, G, kb);
+    const GlobalSink gp = {{cur.mom, cur.mom + G, cur.mom + 2 * G, cur.m, nullptr}};
+    // ---- P2G (+ reaction) into buf(s), as p2g_react_block
+    Stencil sn = stencil_at(x, mp.inv_dx);
+    if (!valid) sn.base[0] = sn.base[1] = sn.base[2] = 0;
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = wave_min_i(valid ? sn.base[d] : 0x7FFFFFFF);
+      hi[d] = wave_max_i(valid ? sn.base[d] + 2 : -0x7FFFFFFF);
+    }
+    const TileBox b = block_tile_box(sn.base, valid, FUSED_TILE_NODES);
+    ReactOut o;
+    o.active = 0;
+    o.particle_norm = o.max_contrib = 0.0f;
+    o.f[0] = o.f[1] = o.f[2] = 0.0f;
+    o.g[0] = o.g[1] = o.g[2] = 0.0f;
+    int kind = SCATTER_GLOBAL;
+    bool checked = true;
+    float4* mine = reinterpret_cast<float4*>(tile) + (threadIdx.x & (TILE_COPIES - 1)) * b.V;
+    if (b.V > 0) {
+      kind = bases_exclusive(sn, valid, b, tile) ? SCATTER_TILE_RMW : SCATTER_TILE_ATOMIC;
+      tile_clear_n(tile, TILE_COPIES * 4 * b.V);
+      const bool in_grid = b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= mp.gnx && b.ly + b.ey <= mp.gny &&
+                           b.lz + b.ez <= mp.gnz;
+      const bool in_lat = b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= lp.nx && b.ly + b.ey <= lp.ny &&
+                          b.lz + b.ez <= lp.nz;
+      checked = !(in_grid && (!(MODE && same) || in_lat));
+    }
+    kind = __builtin_amdgcn_readfirstlane(kind);
+    checked = __builtin_amdgcn_readfirstlane((int)checked) != 0;
+    if (valid) {
+      P2gIn in;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        in.F[e] = F[e];
+        in.C[e] = C[e];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) in.v[d] = v[d];
+      in.mass = mass;
+      in.vol0 = vol0;
+      o = p2g_react_particle<MODE>(lp, ls, mp, ms, cp, q, in, sn, same, checked, kind, mine, b, gp);
+    }
+    if (b.V > 0) tile_flush_fused(reinterpret_cast<const float4*>(tile), b, gp, mp.gnx, mp.gny, mp.gnz);
+    // the nodes this workgroup scattered to two substeps ago, in buf(s + 1)
+    if (s >= 2)
+      mpm_res_clear_box(mpm_res_buf(ms, scratch, G, (n + 1 - s) % 3), blo[1], bhi[1], mp.gnx, mp.gny, mp.gnz, G);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      blo[1][d] = blo[0][d];
+      bhi[1][d] = bhi[0][d];
+      blo[0][d] = lo[d];
+      bhi[0][d] = hi[d];
+    }
+    if (MODE && s == n - 1) {  // the counters hold the last substep's reaction
+      const int act = block_sum_i(o.active, si);
+      const float mpn = block_max_f(o.particle_norm, sf);
+      const float mgc = block_max_f(o.max_contrib, sf);
+      double net[3], netg[3];
+      for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)o.f[d], RED_SUM, sd);
+      for (int d = 0; d < 3; ++d) netg[d] = block_reduce((double)o.g[d], RED_SUM, sd);
+      if (threadIdx.x == 0) {  // read by workgroup 0: agent-scope stores
+        double* r = rec + (size_t)blockIdx.x * REACT_REC;
+        const double vals[REACT_REC] = {(double)act, (double)mpn, (double)mgc, net[0], net[1], net[2],
+                                        netg[0], netg[1], netg[2]};
+        for (int e = 0; e < REACT_REC; ++e) __hip_atomic_store(r + e, vals[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    mpm_res_barrier(sync, nb, ++bar, bail);
+    // ---- G2P from buf(s), the grid update applied per node.  The workgroup's
+    // stencils are those of its scatter (the particles have not moved), so
+    // with a tile the box's nodes are loaded once, coalesced, and their
+    // velocities kept in LDS (the scatter's tile is free after its flush).
+    const float zf[3] = {0.0f, 0.0f, 0.0f};  // grid_f stays zero: the reaction is in the momentum
+    float4* vt = reinterpret_cast<float4*>(tile);
+    if (b.V > 0) {
+      const int plane = b.ey * b.ez;
+      for (int i = threadIdx.x; i < b.V; i += PTPB) {
+        const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
+        const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
+        float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (0 <= I && I < mp.gnx && 0 <= J && J < mp.gny && 0 <= K && K < mp.gnz) {
+          const long long nn = ((long long)I * mp.gny + J) * mp.gnz + K;
+          float mom[3] = {agent_load(cur.mom + nn), agent_load(cur.mom + G + nn), agent_load(cur.mom + 2 * G + nn)};
+          const float m = agent_load(cur.m + nn);
+          float vel[3] = {mom[0], mom[1], mom[2]};
+          if (m > 0.0f) grid_node_velocity(mp, m, mom, zf, I, J, K, vel);
+          t = make_float4(vel[0], vel[1], vel[2], 0.0f);
+        }
+        vt[i] = t;
+      }
+      __syncthreads();
+    }
+    if (valid) {
+      int base[3];
+      float fx[3], w[3][3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) bspline(x[d] * mp.inv_dx, base[d], fx[d], w[d]);
+      float gv[27][3];
+      uint32_t inm = 0u;
+#pragma unroll
+      for (int k = 0; k < 27; ++k) {
+        const int I = base[0] + k / 9, J = base[1] + (k / 3) % 3, K = base[2] + k % 3;
+        inm |= (0 <= I && I < mp.gnx && 0 <= J && J < mp.gny && 0 <= K && K < mp.gnz) ? (1u << k) : 0u;
+      }
+      if (b.V > 0) {
+        // the nodes' velocities from the workgroup's tile, one LDS read per node
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+          const float4 t = vt[((base[0] + k / 9 - b.lx) * b.ey + (base[1] + (k / 3) % 3 - b.ly)) * b.ez +
+                              (base[2] + k % 3 - b.lz)];
+          gv[k][0] = t.x;
+          gv[k][1] = t.y;
+          gv[k][2] = t.z;
+        }
+      } else {
+        // no tile: the particle's own nodes, all 108 loads in flight together
+        float gm[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+          const long long nn = ((inm >> k) & 1u) ? ((long long)(base[0] + k / 9) * mp.gny + (base[1] + (k / 3) % 3)) *
+                                                          mp.gnz + (base[2] + k % 3)
+                                                    : 0;
+          gv[k][0] = agent_load(cur.mom + nn);
+          gv[k][1] = agent_load(cur.mom + G + nn);
+          gv[k][2] = agent_load(cur.mom + 2 * G + nn);
+          gm[k] = agent_load(cur.m + nn);
+        }
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+          if (gm[k] > 0.0f) {
+            const float mom[3] = {gv[k][0], gv[k][1], gv[k][2]};
+            grid_node_velocity(mp, gm[k], mom, zf, base[0] + k / 9, base[1] + (k / 3) % 3, base[2] + k % 3, gv[k]);
+          }
+        }
+      }
+      float xn[3], vn[3], Cn[9], Fn[9], Jn;
+      g2p_update(mp, ms, q, fx, w, gv, inm, x, F, xn, vn, Cn, Fn, Jn);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        x[d] = xn[d];
+        v[d] = vn[d];
+      }
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        C[e] = Cn[e];
+        F[e] = Fn[e];
+      }
+      if (s == n - 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          ms.x[d * P + q] = x[d];
+          ms.v[d * P + q] = v[d];
+        }
+        store_mat(ms.C, P, q, C);
+        store_mat(ms.F, P, q, F);
+        ms.Jp[q] = Jn;
+      }
+    }
+  }
+  mpm_res_barrier(sync, nb, ++bar, bail);
+  // buffer 0 (the state's grid) to velocities, a slice per workgroup; the
+  // scratch buffers are cleared by the next call
+  {
+    const long long per = (G + nb - 1) / nb, n0 = per * blockIdx.x, n1 = n0 + per < G ? n0 + per : G;
+    for (long long i = n0 + threadIdx.x; i < n1; i += PTPB) {
+      const float m = agent_load(ms.grid_m + i);
+      if (!(m > 0.0f)) continue;
+      const int K = (int)(i % mp.gnz), J = (int)((i / mp.gnz) % mp.gny), I = (int)(i / ((long long)mp.gnz * mp.gny));
+      const float mom[3] = {agent_load(ms.grid_v + i), agent_load(ms.grid_v + G + i), agent_load(ms.grid_v + 2 * G + i)};
+      const float zf[3] = {0.0f, 0.0f, 0.0f};
+      float vv[3];
+      grid_node_velocity(mp, m, mom, zf, I, J, K, vv);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) agent_store(ms.grid_v + d * G + i, vv[d]);
+    }
+  }
+  if (MODE && blockIdx.x == 0) fold_react_records(rec, nb, cc, sd);
+  // the last workgroup out resets the arrival words
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t out =
+        __hip_atomic_fetch_add(sync + MPM_RES_STRIDE * MPM_RES_EXIT, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (out == (uint32_t)nb - 1u) {
+      for (uint32_t w = 0; w < 8; ++w) __hip_atomic_store(sync + MPM_RES_STRIDE * w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(sync + MPM_RES_STRIDE * MPM_RES_EXIT, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2590,6 +2950,66 @@ int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* l
     if (rc) return rc;
   }
   return fsi_mpm_substep_coupled(lp, ls, mp, ms, cp, cc, mode, ws, stream);
+}
+
+int fsi_mpm_subcycle_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
+                             const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode,
+                             int n, double* workspace, float* scratch, uint32_t* sync, void* stream) {
+  static_assert(MPM_RES_STRIDE * MPM_RES_TIMEOUT == FSI_MPM_RES_TIMEOUT_WORD, "timeout word");
+  static_assert(MPM_RES_STRIDE * (MPM_RES_TIMEOUT + 1) == FSI_MPM_RES_SYNC_WORDS, "sync words");
+  int rc = check_mpm(mp, ms);
+  if (rc) return rc;
+  if (mode < 0 || mode > 2) return fail(FSI_E_INVALID, "mode must be 0 (none), 1 (penalty) or 2 (moving boundary)");
+  if (n < 1) return fail(FSI_E_INVALID, "n must be at least 1");
+  if (!scratch || !sync) return fail(FSI_E_INVALID, "the resident MPM subcycle needs scratch and sync");
+  FsiLbmParams lpv;
+  FsiLbmState lsv;
+  FsiCouplingParams cpv;
+  memset(&lpv, 0, sizeof(lpv));
+  memset(&lsv, 0, sizeof(lsv));
+  memset(&cpv, 0, sizeof(cpv));
+  if (mode) {
+    rc = check_lbm(lp, ls);
+    if (rc) return rc;
+    if (!cp || !cc || !workspace || !ls->hydro_force || !ls->solid_phi)
+      return fail(FSI_E_INVALID, "the reaction needs coupling params, counters, workspace, hydro_force, solid_phi");
+    if (cp->inv_dx_norm != mp->inv_dx)
+      return fail(FSI_E_UNSUPPORTED, "the reaction samples the lattice at another spacing than the MPM grid");
+    lpv = *lp;
+    lsv = *ls;
+    cpv = *cp;
+  }
+  if (mp->n_particles <= 0) return fail(FSI_E_UNSUPPORTED, "no particles");
+  // every workgroup must be resident at once
+  static thread_local int dev_cached = -1;
+  static thread_local long long capacity[3] = {0, 0, 0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(FSI_E_HIP, "hipGetDevice failed");
+  if (dev != dev_cached) {
+    int cus = 0, per_cu[3] = {0, 0, 0};
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], k_mpm_resident<0>, PTPB, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], k_mpm_resident<1>, PTPB, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[2], k_mpm_resident<2>, PTPB, 0) != hipSuccess)
+      return fail(FSI_E_HIP, "occupancy query of the resident MPM subcycle failed");
+    for (int m = 0; m < 3; ++m) capacity[m] = (long long)cus * per_cu[m];
+    dev_cached = dev;
+  }
+  const unsigned pb = pblocks_for(mp->n_particles);
+  if ((long long)pb > capacity[mode])
+    return fail(FSI_E_UNSUPPORTED, std::to_string(pb) + " particle workgroups exceed the " +
+                                       std::to_string(capacity[mode]) + " the device holds at once");
+  hipStream_t s = as_stream(stream);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_mpm_resident<1>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, n, scratch, workspace,
+                       cc, sync);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_mpm_resident<2>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, n, scratch, workspace,
+                       cc, sync);
+  else
+    hipLaunchKernelGGL(k_mpm_resident<0>, dim3(pb), dim3(PTPB), 0, s, lpv, lsv, *mp, *ms, cpv, n, scratch,
+                       (double*)nullptr, (FsiCouplingCounters*)nullptr, sync);
+  return check_launch("k_mpm_resident");
 }
 
 int fsi_bin_particles(const float* x, int32_t n_particles, float inv_dx, int32_t nx, int32_t ny, int32_t nz,

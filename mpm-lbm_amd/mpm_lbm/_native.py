@@ -47,6 +47,7 @@ SUB_GATHER_PREV = 4
 SUB_DIAG = 8
 SUB_CELL_FORCE = 16
 
+E_INVALID = -1  # FSI_E_INVALID
 E_UNSUPPORTED = -3  # FSI_E_UNSUPPORTED
 
 REDUCE_BLOCKS = 256
@@ -239,7 +240,7 @@ EXPORTED_SYMBOLS = [
     "fsi_lbm_mb_begin", "fsi_lbm_mb_finalize", "fsi_lbm_reduce_stats", "fsi_lbm_subcycle_resident",
     "fsi_project", "fsi_penalty_force", "fsi_mpm_clear_grid", "fsi_mpm_p2g", "fsi_mpm_reaction",
     "fsi_mpm_grid_update", "fsi_mpm_g2p", "fsi_mpm_substep_coupled", "fsi_mpm_substep_coupled_noalloc",
-    "fsi_mpm_reduce_stats",
+    "fsi_mpm_subcycle_coupled", "fsi_mpm_reduce_stats",
     "fsi_bin_particles", "fsi_mpm_substep_ordered", "fsi_project_ordered",
     "fsi_batch_block_map", "fsi_batch_launch",
     "fsi_batch_graph_create", "fsi_batch_graph_update", "fsi_batch_graph_launch", "fsi_batch_graph_destroy",
@@ -277,6 +278,15 @@ def res_sync_words(nx: int, ny: int) -> int:
 def res_halo_floats(n_cells: int) -> int:
     """FSI_RES_HALO_FLOATS(n_cells)."""
     return 2 * 18 * int(n_cells)
+
+
+MPM_RES_SYNC_WORDS = 640  # FSI_MPM_RES_SYNC_WORDS
+MPM_RES_TIMEOUT_WORD = 576  # FSI_MPM_RES_TIMEOUT_WORD
+
+
+def mpm_res_scratch_floats(grid_nodes: int) -> int:
+    """FSI_MPM_RES_SCRATCH_FLOATS(grid_nodes)."""
+    return 8 * int(grid_nodes)
 
 
 def coupled_workspace_doubles(n_particles: int) -> int:
@@ -328,6 +338,8 @@ class Native:
                                               P(CouplingParams), vp, ctypes.c_int, vp, vp]
         L.fsi_mpm_substep_coupled_noalloc.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),
                                                       P(CouplingParams), vp, ctypes.c_int, vp]
+        L.fsi_mpm_subcycle_coupled.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),
+                                               P(CouplingParams), vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         L.fsi_mpm_reduce_stats.argtypes = [P(MpmParams), P(MpmState), vp, vp, vp]
         L.fsi_bin_particles.argtypes = [vp, i32, ctypes.c_float, i32, i32, i32, P(Bins), vp]
         L.fsi_mpm_substep_ordered.argtypes = [P(LbmParams), P(LbmState), P(MpmParams), P(MpmState),

@@ -356,9 +356,8 @@ class FSIDriver3D:
         self.lbm.step()
         self.timing["lbm_step_time"] += time.perf_counter() - t0
         t0 = time.perf_counter()
-        for _ in range(self.config.mpm_substeps_per_lbm_step):
-            self.solid.substep_coupled()
-            self.total_mpm_substeps += 1
+        self.solid.subcycle_coupled(self.config.mpm_substeps_per_lbm_step)
+        self.total_mpm_substeps += self.config.mpm_substeps_per_lbm_step
         self.timing["mpm_substep_time"] += time.perf_counter() - t0
 
     def _step_penalty(self):
@@ -373,9 +372,8 @@ class FSIDriver3D:
         self.timing["lbm_step_time"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         # clear_grid, p2g, add_lbm_reaction_to_mpm_grid, grid_update, g2p - fused per substep
-        for _ in range(self.config.mpm_substeps_per_lbm_step):
-            self.solid.substep_coupled(self.lbm, self.penalty_coupler, 1)
-            self.total_mpm_substeps += 1
+        self.solid.subcycle_coupled(self.config.mpm_substeps_per_lbm_step, self.lbm, self.penalty_coupler, 1)
+        self.total_mpm_substeps += self.config.mpm_substeps_per_lbm_step
         self.timing["mpm_substep_time"] += time.perf_counter() - t0
 
     def _step_moving_boundary(self):
@@ -466,9 +464,8 @@ class FSIDriver3D:
             coupler = self.link_area_coupler
         # clear_grid, p2g, clear_reaction_diagnostics, add_*_reaction_to_mpm_grid,
         # grid_update, g2p - as one fused device sequence per substep
-        for _ in range(self.config.mpm_substeps_per_lbm_step):
-            self.solid.substep_coupled(self.lbm, coupler, 2)
-            self.total_mpm_substeps += 1
+        self.solid.subcycle_coupled(self.config.mpm_substeps_per_lbm_step, self.lbm, coupler, 2)
+        self.total_mpm_substeps += self.config.mpm_substeps_per_lbm_step
         self.timing["mpm_substep_time"] += time.perf_counter() - t0
 
     def run(self, checkpoint_dir=None, checkpoint_every: int = 0, resume: bool = False,
@@ -537,6 +534,7 @@ class FSIDriver3D:
         """fsi_driver.py:597-669: the same 25 columns, reduced on the device."""
         t0 = time.perf_counter()
         self.lbm.check_resident_waits()
+        self.solid.check_resident_waits()
         ls = self.lbm.reduce_stats()
         ms = self.solid.reduce_stats()
         fluid = FSIDiagnostics3D.lbm_fluid_stats(self.lbm, stats=ls)

@@ -53,6 +53,12 @@ class MPMSolid3D:
         # float atomics; fastest) or "particle" (binned, in particle order;
         # deterministic)
         self.transfer_order = "atomic"
+        # subcycle_coupled in the "atomic" order runs as one resident launch
+        # (fsi_mpm_subcycle_coupled) when the device holds it
+        self.resident_subcycle = True
+        self._res_declined = False
+        self._res_scratch = None
+        self._res_sync = None
         self.gravity_x, self.gravity_y, self.gravity_z = config.gravity
         self.box_min_x, self.box_min_y, self.box_min_z = config.box_min
         self.box_max_x, self.box_max_y, self.box_max_z = config.box_max
@@ -263,6 +269,50 @@ class MPMSolid3D:
         else:
             self.native.call("fsi_mpm_substep_coupled", None, None, ctypes.byref(self._params),
                              ctypes.byref(self._state), None, None, 0, None, self._stream())
+
+    def subcycle_coupled(self, n: int, lbm=None, coupler=None, mode: int = 0):
+        """n calls of substep_coupled (the MPM substeps of one FSI step, the
+        lattice unchanged in between).  In the "atomic" order they run as one
+        launch that keeps the particles on chip (fsi_mpm_subcycle_coupled)
+        when the device holds every particle workgroup at once; otherwise, or
+        when resident_subcycle is off, as n substeps."""
+        if self.resident_subcycle and not self._res_declined and self.transfer_order == "atomic" and n > 0:
+            if self._run_resident(int(n), lbm, coupler, mode):
+                return
+            self._res_declined = True
+        for _ in range(int(n)):
+            self.substep_coupled(lbm, coupler, mode)
+
+    def _run_resident(self, n, lbm, coupler, mode) -> bool:
+        if getattr(self, "_coupled_ws", None) is None:
+            self._coupled_ws = torch.zeros(_native.coupled_workspace_doubles(self.n_particles), dtype=torch.float64,
+                                           device=self.device)
+        if self._res_sync is None:
+            G = int(np.prod(self.grid_shape))
+            self._res_scratch = torch.zeros(_native.mpm_res_scratch_floats(G), dtype=torch.float32, device=self.device)
+            self._res_sync = torch.zeros(_native.MPM_RES_SYNC_WORDS, dtype=torch.int32, device=self.device)
+        scratch, sync = _native.ptr(self._res_scratch), _native.ptr(self._res_sync)
+        if mode:
+            stream = coupler._bind(lbm)
+            lbm._sync_hydro()
+            return self.native.try_call("fsi_mpm_subcycle_coupled", ctypes.byref(lbm._params),
+                                        ctypes.byref(lbm._state), ctypes.byref(self._params),
+                                        ctypes.byref(self._state), ctypes.byref(coupler.params),
+                                        _native.ptr(coupler._counters_buf), int(mode), n,
+                                        _native.ptr(self._coupled_ws), scratch, sync, stream)
+        return self.native.try_call("fsi_mpm_subcycle_coupled", None, None, ctypes.byref(self._params),
+                                    ctypes.byref(self._state), None, None, 0, n, None, scratch, sync, self._stream())
+
+    def check_resident_waits(self):
+        """Raise if a resident MPM subcycle went on past a barrier without
+        every workgroup (a wait that timed out: its results are invalid); the
+        sync words are cleared so the next call starts clean."""
+        if self._res_sync is None:
+            return
+        timeouts = int(self._res_sync[_native.MPM_RES_TIMEOUT_WORD].item())
+        if timeouts:
+            self._res_sync.zero_()
+            raise RuntimeError(f"resident MPM subcycle: {timeouts} barrier waits timed out; results are invalid")
 
     def _substep_ordered(self, lbm, coupler, mode):
         """substep_coupled with every grid-node sum in particle order
