@@ -2261,6 +2261,7 @@ struct HydroGather {
   Row3 h[9][4];
   int d;
   uint32_t in;
+  bool shifted;  // rows loaded from a start moved inside the lattice (d != 0 possible)
 };
 
 // Issues the loads and returns at once: no loaded value is used here, so the
@@ -2273,6 +2274,7 @@ FSI_HD void hydro_gather_load(const FsiLbmParams& lp, const FsiLbmState& ls, int
   const int k0 = checked ? row_start(base[2], g.nz) : base[2];
   G.d = k0 - base[2];
   G.in = 0u;
+  G.shifted = checked;
 #pragma unroll
   for (int r = 0; r < 9; ++r) {
     const int I = base[0] + r / 3, Jn = base[1] + r % 3;
@@ -2307,8 +2309,11 @@ FSI_HD ReactOut reaction_from_gather(const FsiCouplingParams& cp, int mode, floa
   float hx = 0.0f, hy = 0.0f, hz = 0.0f, sphi = 0.0f;
 #pragma unroll
   for (int r = 0; r < 9; ++r) {
-    const Row3 rx = row_shift(G.h[r][0], G.d), ry = row_shift(G.h[r][1], G.d), rz = row_shift(G.h[r][2], G.d),
-               rp = row_shift(G.h[r][3], G.d);
+    // unchecked rows start at the stencil: row_shift by 0 would return them bit for bit
+    const Row3 rx = G.shifted ? row_shift(G.h[r][0], G.d) : G.h[r][0];
+    const Row3 ry = G.shifted ? row_shift(G.h[r][1], G.d) : G.h[r][1];
+    const Row3 rz = G.shifted ? row_shift(G.h[r][2], G.d) : G.h[r][2];
+    const Row3 rp = G.shifted ? row_shift(G.h[r][3], G.d) : G.h[r][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int k = 3 * r + c;

@@ -333,28 +333,41 @@ __device__ __forceinline__ void wave_order() {
 // come from different iterations of the 27-node loop, which the wave issues in
 // order.  In a lattice-ordered cloud (8 particles per cell, lane parity = the
 // particle's z half of the cell, which is also its tile copy) the condition
-// always holds; otherwise the workgroup takes the atomic path.
+// always holds.  Otherwise the lanes are split into rounds (scatter_rounds):
+// within a round no two lanes of a copy share a base, and the rounds run one
+// after the other, so the read-modify-write stays exact in any particle order.
 
-// Whether the read-modify-write scatter is exact for this workgroup.  Scratch:
-// the tile, before it is cleared.
-__device__ __forceinline__ bool bases_exclusive(const Stencil& sn, bool valid, const TileBox& b, float* tile) {
+// The read-modify-write rounds of the workgroup's lanes: returns this lane's
+// round (-1 for an invalid lane) and in `nrounds` their number (uniform).
+// Round r holds the lanes that win their (copy, base) slot once the lanes of
+// rounds < r are taken out, so no two lanes of a round and copy share a base.
+// Each pass settles at least one lane, so there are at most 64.  Scratch: the
+// tile, before it is cleared.
+__device__ __forceinline__ int scatter_rounds(const Stencil& sn, bool valid, const TileBox& b, float* tile,
+                                              int& nrounds) {
   int* own = reinterpret_cast<int*>(tile);
-  int key = 0;
-  if (valid) {
-    key = (int)(threadIdx.x & (TILE_COPIES - 1)) * b.V +
-          ((sn.base[0] - b.lx) * b.ey + (sn.base[1] - b.ly)) * b.ez + (sn.base[2] - b.lz);
-    own[key] = (int)threadIdx.x;
+  const int key = valid ? (int)(threadIdx.x & (TILE_COPIES - 1)) * b.V +
+                              ((sn.base[0] - b.lx) * b.ey + (sn.base[1] - b.ly)) * b.ez + (sn.base[2] - b.lz)
+                        : 0;
+  int round = -1, r = 0;
+  bool pending = valid;
+  while (__syncthreads_or(pending)) {
+    if (pending) own[key] = (int)threadIdx.x;
+    __syncthreads();
+    if (pending && own[key] == (int)threadIdx.x) {
+      round = r;
+      pending = false;
+    }
+    ++r;
   }
-  __syncthreads();
-  const int clash = valid && own[key] != (int)threadIdx.x;
-  return !__syncthreads_or(clash);
+  nrounds = r;
+  return round;
 }
 
 // Where the fused P2G scatter puts a node's 4 floats.
 enum FusedScatter {
-  SCATTER_TILE_RMW = 0,     // LDS tile, read-modify-write (bases exclusive per copy)
-  SCATTER_TILE_ATOMIC = 1,  // LDS tile, float atomics
-  SCATTER_GLOBAL = 2        // grid arrays, float atomics (box too large for a tile)
+  SCATTER_TILE_RMW = 0,  // LDS tile, read-modify-write, in rounds of lanes with distinct bases per copy
+  SCATTER_GLOBAL = 2     // grid arrays, float atomics (box too large for a tile)
 };
 
 __device__ __forceinline__ float pick3(int i, const float (&v)[3]) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
@@ -403,13 +416,6 @@ __device__ __forceinline__ void proj_scatter(const FsiLbmParams& lp, const FsiCo
         t4[row + c] = v;
         t1[row + c] = u;
         wave_order();
-      } else if (KIND == SCATTER_TILE_ATOMIC) {
-        float* f = reinterpret_cast<float*>(t4 + row + c);
-        __hip_atomic_fetch_add(f + 0, vf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(f + 1, mc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(f + 2, p2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(f + 3, p3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(t1 + row + c, p4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       } else if (row_in && zin[c]) {
         atomic_add_f32(g.ch[0] + crow + c, vf);
         atomic_add_f32(g.ch[1] + crow + c, mc);
@@ -444,8 +450,10 @@ __device__ __forceinline__ void proj_block(const FsiLbmParams& lp, const FsiLbmS
   float4* t4 = reinterpret_cast<float4*>(tile);
   float* t1 = reinterpret_cast<float*>(t4 + TILE_COPIES * b.V);
   const int copy = (int)(threadIdx.x & (TILE_COPIES - 1));
+  int round = -1, nrounds = 0;
   if (b.V > 0) {
-    kind = bases_exclusive(sn, valid, b, tile) ? SCATTER_TILE_RMW : SCATTER_TILE_ATOMIC;
+    kind = SCATTER_TILE_RMW;
+    round = scatter_rounds(sn, valid, b, tile, nrounds);
     tile_clear_n(tile, TILE_COPIES * 5 * b.V);
     checked = !(b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= lp.nx && b.ly + b.ey <= lp.ny &&
                 b.lz + b.ez <= lp.nz);
@@ -455,12 +463,12 @@ __device__ __forceinline__ void proj_block(const FsiLbmParams& lp, const FsiLbmS
   if (valid) {
     float4* m4 = t4 + copy * b.V;
     float* m1 = t1 + copy * b.V;
-    if (kind == SCATTER_TILE_RMW)
-      proj_scatter<SCATTER_TILE_RMW>(lp, cp, sn, vol, mass, vx, vy, vz, checked, m4, m1, b, g);
-    else if (kind == SCATTER_TILE_ATOMIC)
-      proj_scatter<SCATTER_TILE_ATOMIC>(lp, cp, sn, vol, mass, vx, vy, vz, checked, m4, m1, b, g);
-    else
+    if (kind == SCATTER_TILE_RMW) {
+      for (int r = 0; r < nrounds; ++r)
+        if (round == r) proj_scatter<SCATTER_TILE_RMW>(lp, cp, sn, vol, mass, vx, vy, vz, checked, m4, m1, b, g);
+    } else {
       proj_scatter<SCATTER_GLOBAL>(lp, cp, sn, vol, mass, vx, vy, vz, checked, m4, m1, b, g);
+    }
   }
   if (b.V > 0) {
     // flush: the copies summed per node, one global atomic per touched node
@@ -1262,12 +1270,6 @@ __device__ __forceinline__ float fused_scatter(const FsiMpmParams& mp, const Ste
         v.w += am;
         row[c] = v;
         wave_order();
-      } else if (KIND == SCATTER_TILE_ATOMIC) {
-        float* f = reinterpret_cast<float*>(row + c);
-        __hip_atomic_fetch_add(f + 0, ax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(f + 1, ay, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(f + 2, az, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(f + 3, am, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       } else if (in) {
         atomic_add_f32(g.ch[0] + nrow + c, ax);
         atomic_add_f32(g.ch[1] + nrow + c, ay);
@@ -1299,7 +1301,8 @@ __device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, c
                                                        const FsiMpmParams& mp, const FsiMpmState& ms,
                                                        const FsiCouplingParams& cp, long long q, const P2gIn& in,
                                                        const Stencil& sn, bool fold, bool checked, int kind,
-                                                       float4* tile, const TileBox& b, const GlobalSink& g) {
+                                                       int round, int nrounds, float4* tile, const TileBox& b,
+                                                       const GlobalSink& g) {
   ReactOut o;
   o.active = 0;
   o.particle_norm = o.max_contrib = 0.0f;
@@ -1319,13 +1322,13 @@ __device__ __forceinline__ ReactOut p2g_react_particle(const FsiLbmParams& lp, c
       o.g[d] = o.f[d] * win;
     }
   }
-  float wmax;
-  if (kind == SCATTER_TILE_RMW)
-    wmax = fused_scatter<SCATTER_TILE_RMW>(mp, sn, t, mv, checked, tile, b, g);
-  else if (kind == SCATTER_TILE_ATOMIC)
-    wmax = fused_scatter<SCATTER_TILE_ATOMIC>(mp, sn, t, mv, checked, tile, b, g);
-  else
+  float wmax = 0.0f;
+  if (kind == SCATTER_TILE_RMW) {
+    for (int r = 0; r < nrounds; ++r)
+      if (round == r) wmax = fused_scatter<SCATTER_TILE_RMW>(mp, sn, t, mv, checked, tile, b, g);
+  } else {
     wmax = fused_scatter<SCATTER_GLOBAL>(mp, sn, t, mv, checked, tile, b, g);
+  }
   if (MODE && fold) {
     const float cn = norm3(wmax * o.f[0], wmax * o.f[1], wmax * o.f[2]);
     o.max_contrib = cn > 0.0f ? cn : 0.0f;
@@ -1387,8 +1390,10 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
   int kind = SCATTER_GLOBAL;
   bool checked = true;
   float4* mine = reinterpret_cast<float4*>(tile) + (threadIdx.x & (TILE_COPIES - 1)) * b.V;
+  int round = -1, nrounds = 0;
   if (b.V > 0) {
-    kind = bases_exclusive(sn, valid, b, tile) ? SCATTER_TILE_RMW : SCATTER_TILE_ATOMIC;
+    kind = SCATTER_TILE_RMW;
+    round = scatter_rounds(sn, valid, b, tile, nrounds);
     tile_clear_n(tile, TILE_COPIES * 4 * b.V);
     const bool in_grid = b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= mp.gnx && b.ly + b.ey <= mp.gny &&
                          b.lz + b.ez <= mp.gnz;
@@ -1400,7 +1405,7 @@ __device__ __forceinline__ void p2g_react_block(const FsiLbmParams& lp, const Fs
   kind = __builtin_amdgcn_readfirstlane(kind);
   checked = __builtin_amdgcn_readfirstlane((int)checked) != 0;
   if (valid) {
-    o = p2g_react_particle<MODE>(lp, ls, mp, ms, cp, q, in, sn, same, checked, kind, mine, b, gp);
+    o = p2g_react_particle<MODE>(lp, ls, mp, ms, cp, q, in, sn, same, checked, kind, round, nrounds, mine, b, gp);
     // a reaction on another spacing scatters on its own stencil into grid_f
     if (MODE && !same) o = reaction_particle(lp, ls, mp, ms, cp, MODE, q, gr);
   }
@@ -1515,6 +1520,7 @@ __global__ void __launch_bounds__(TPB) k_mpm_grid_update_react(FsiMpmParams mp, 
 constexpr uint32_t MPM_RES_STRIDE = 64;  // words between counters (separate lines)
 constexpr int MPM_RES_MAX_POLLS = 1 << 18;
 constexpr uint32_t MPM_RES_EXIT = 8, MPM_RES_TIMEOUT = 9;
+constexpr int MPM_RES_FILL = 4;  // tile nodes per lane whose loads the gather issues together
 
 struct MpmResBuf {
   float* mom;  // momentum x, y, z (G apart)
@@ -1662,8 +1668,10 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
     int kind = SCATTER_GLOBAL;
     bool checked = true;
     float4* mine = reinterpret_cast<float4*>(tile) + (threadIdx.x & (TILE_COPIES - 1)) * b.V;
+    int round = -1, nrounds = 0;
     if (b.V > 0) {
-      kind = bases_exclusive(sn, valid, b, tile) ? SCATTER_TILE_RMW : SCATTER_TILE_ATOMIC;
+      kind = SCATTER_TILE_RMW;
+      round = scatter_rounds(sn, valid, b, tile, nrounds);
       tile_clear_n(tile, TILE_COPIES * 4 * b.V);
       const bool in_grid = b.lx >= 0 && b.ly >= 0 && b.lz >= 0 && b.lx + b.ex <= mp.gnx && b.ly + b.ey <= mp.gny &&
                            b.lz + b.ez <= mp.gnz;
@@ -1684,7 +1692,7 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
       for (int d = 0; d < 3; ++d) in.v[d] = v[d];
       in.mass = mass;
       in.vol0 = vol0;
-      o = p2g_react_particle<MODE>(lp, ls, mp, ms, cp, q, in, sn, same, checked, kind, mine, b, gp);
+      o = p2g_react_particle<MODE>(lp, ls, mp, ms, cp, q, in, sn, same, checked, kind, round, nrounds, mine, b, gp);
     }
     if (b.V > 0) tile_flush_fused(reinterpret_cast<const float4*>(tile), b, gp, mp.gnx, mp.gny, mp.gnz);
     // the nodes this workgroup scattered to two substeps ago, in buf(s + 1)
@@ -1719,20 +1727,35 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
     const float zf[3] = {0.0f, 0.0f, 0.0f};  // grid_f stays zero: the reaction is in the momentum
     float4* vt = reinterpret_cast<float4*>(tile);
     if (b.V > 0) {
+      // MPM_RES_FILL nodes per lane per pass, their loads all in flight
+      // together (a node outside the grid, or past the box, reads node 0)
       const int plane = b.ey * b.ez;
-      for (int i = threadIdx.x; i < b.V; i += PTPB) {
-        const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
-        const int I = b.lx + a, J = b.ly + e, K = b.lz + f;
-        float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (0 <= I && I < mp.gnx && 0 <= J && J < mp.gny && 0 <= K && K < mp.gnz) {
-          const long long nn = ((long long)I * mp.gny + J) * mp.gnz + K;
-          float mom[3] = {agent_load(cur.mom + nn), agent_load(cur.mom + G + nn), agent_load(cur.mom + 2 * G + nn)};
-          const float m = agent_load(cur.m + nn);
-          float vel[3] = {mom[0], mom[1], mom[2]};
-          if (m > 0.0f) grid_node_velocity(mp, m, mom, zf, I, J, K, vel);
-          t = make_float4(vel[0], vel[1], vel[2], 0.0f);
+      for (int i0 = 0; i0 < b.V; i0 += MPM_RES_FILL * PTPB) {
+        float nm[MPM_RES_FILL], nv[MPM_RES_FILL][3];
+        int ijk[MPM_RES_FILL][3];
+        bool use[MPM_RES_FILL];
+#pragma unroll
+        for (int u = 0; u < MPM_RES_FILL; ++u) {
+          const int i = i0 + u * PTPB + (int)threadIdx.x;
+          const int a = i / plane, r = i - a * plane, e = r / b.ez, f = r - e * b.ez;
+          ijk[u][0] = b.lx + a;
+          ijk[u][1] = b.ly + e;
+          ijk[u][2] = b.lz + f;
+          use[u] = i < b.V && 0 <= ijk[u][0] && ijk[u][0] < mp.gnx && 0 <= ijk[u][1] && ijk[u][1] < mp.gny &&
+                   0 <= ijk[u][2] && ijk[u][2] < mp.gnz;
+          const long long nn = use[u] ? ((long long)ijk[u][0] * mp.gny + ijk[u][1]) * mp.gnz + ijk[u][2] : 0;
+          nv[u][0] = agent_load(cur.mom + nn);
+          nv[u][1] = agent_load(cur.mom + G + nn);
+          nv[u][2] = agent_load(cur.mom + 2 * G + nn);
+          nm[u] = agent_load(cur.m + nn);
         }
-        vt[i] = t;
+#pragma unroll
+        for (int u = 0; u < MPM_RES_FILL; ++u) {
+          const int i = i0 + u * PTPB + (int)threadIdx.x;
+          float vel[3] = {nv[u][0], nv[u][1], nv[u][2]};
+          if (use[u] && nm[u] > 0.0f) grid_node_velocity(mp, nm[u], nv[u], zf, ijk[u][0], ijk[u][1], ijk[u][2], vel);
+          if (i < b.V) vt[i] = use[u] ? make_float4(vel[0], vel[1], vel[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
       }
       __syncthreads();
     }

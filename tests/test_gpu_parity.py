@@ -70,7 +70,7 @@ def _reordered(w, how, seed=7):
     """`w` with its particles in another storage order.  "local" shuffles each
     64-particle workgroup's particles among themselves: the tile boxes stay
     those of the lattice order, but lanes of one tile copy then share stencil
-    bases, so the scatters take their LDS-atomic path.  "global" shuffles the
+    bases, so the scatters run in several read-modify-write rounds.  "global" shuffles the
     whole cloud: the boxes outgrow the tile and the scatters go to the grid
     with global atomics."""
     import dataclasses
@@ -107,7 +107,7 @@ def test_gpu_scatter_fallback_paths(hip, tmp_path, how, mode):
     w = _reordered(w, how)
     g = _groups(w, 32)
     if how == "local":
-        assert all(v <= 512 for v, _ in g) and not all(ex for _, ex in g)  # LDS tile, atomic updates
+        assert all(v <= 512 for v, _ in g) and not all(ex for _, ex in g)  # LDS tile, several rounds
     else:
         assert sum(v > 512 for v, _ in g) > len(g) // 2  # mostly too large for a tile
     d, o = make_pair(w, tmp_path / "run", hip, None, with_fp64=True)
