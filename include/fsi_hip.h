@@ -513,21 +513,23 @@ int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* l
  * as one launch that keeps the particles on chip: one FSI_PARTICLE_BLOCK
  * workgroup per particle block, all resident at once, each holding its
  * particles' x, v, C and F in registers, with one grid-wide barrier per
- * substep (three grid buffers: the state's grid and two in `scratch`) and the
- * grid update applied where g2p gathers.  The state after the call is that of
- * the n calls: particles, counters (the last substep's reaction), grid_v /
- * grid_m (the last substep's updated grid), grid_f zero.
+ * substep (three grid buffers in `scratch`) and the grid update applied where
+ * g2p gathers.  The state after the call is that of the n calls: particles,
+ * counters (the last substep's reaction), grid_v / grid_m (the last
+ * substep's updated grid), grid_f zero.
  * Returns FSI_E_UNSUPPORTED (nothing launched) when the workgroups would not
  * all be resident on the device, when there are no particles, or when the
  * reaction samples the lattice at another spacing than the MPM grid
  * (cp->inv_dx_norm != mp->inv_dx): the caller then runs the n calls.
  * workspace: FSI_COUPLED_WORKSPACE_DOUBLES(n_particles); scratch:
- * FSI_MPM_RES_SCRATCH_FLOATS(grid nodes) floats; sync: FSI_MPM_RES_SYNC_WORDS
- * words, zeroed before the first call (each call leaves its barrier words
- * zero; word FSI_MPM_RES_TIMEOUT_WORD counts barrier waits that gave up, which
- * never happens while every workgroup runs, and is left for the caller).
- * Calls sharing `sync` or `scratch` must not overlap. */
-#define FSI_MPM_RES_SCRATCH_FLOATS(grid_nodes) (8 * (long long)(grid_nodes))
+ * FSI_MPM_RES_SCRATCH_FLOATS(grid nodes) floats and sync:
+ * FSI_MPM_RES_SYNC_WORDS words, both zeroed before the first call; each call
+ * leaves its scratch and barrier words zero again.  Word
+ * FSI_MPM_RES_TIMEOUT_WORD of sync counts barrier waits that gave up (never,
+ * while every workgroup runs): after one the results are invalid and the
+ * caller zeroes scratch and sync.  Calls sharing `sync` or `scratch` must not
+ * overlap. */
+#define FSI_MPM_RES_SCRATCH_FLOATS(grid_nodes) (12 * (long long)(grid_nodes))
 #define FSI_MPM_RES_SYNC_WORDS 640
 #define FSI_MPM_RES_TIMEOUT_WORD 576
 int fsi_mpm_subcycle_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,

@@ -376,12 +376,12 @@ int fsi_mpm_substep_coupled_noalloc(const FsiLbmParams* lp, const FsiLbmState* l
 
 // The resident MPM subcycle as the kernel runs it, phase by phase (every
 // workgroup's scatter, the barrier, every workgroup's gather): substep s
-// scatters into grid buffer (n - 1 - s) mod 3 (0: the state's grid, 1 and 2
-// in scratch), each particle block clears the nodes of its stencil box of
-// substep s - 2 in buffer (n + 1 - s) mod 3, and g2p reads node velocities
-// computed from the buffer.  The reaction goes to grid_f (the emulation's
-// p2g and reaction are separate passes); the last buffer is converted to
-// velocities at the end.
+// scatters into scratch buffer s mod 3, each particle block clears the nodes
+// of its stencil box of substep s - 2 in buffer (s + 1) mod 3, and g2p reads
+// node velocities computed from the buffer.  The reaction goes to grid_f (the
+// emulation's p2g and reaction are separate passes).  At the end the state's
+// grid takes the last buffer's updated velocities and masses, and the
+// scratch is left zero (the kernel's workgroups clear their last two boxes).
 int fsi_mpm_subcycle_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, const FsiMpmParams* mp,
                              const FsiMpmState* ms, const FsiCouplingParams* cp, FsiCouplingCounters* cc, int mode,
                              int n, double* workspace, float* scratch, uint32_t* sync, void* stream) {
@@ -400,17 +400,11 @@ int fsi_mpm_subcycle_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, cons
   const long long nb = (P + FSI_PARTICLE_BLOCK - 1) / FSI_PARTICLE_BLOCK;
   auto buf = [&](int k, FsiMpmState& b) {
     b = *ms;
-    if (k) {
-      b.grid_v = scratch + (size_t)(k - 1) * 4 * G;
-      b.grid_m = b.grid_v + 3 * G;
-    }
+    b.grid_v = scratch + (size_t)k * 4 * G;
+    b.grid_m = b.grid_v + 3 * G;
   };
-  FsiMpmState b0;
-  for (int k = 0; k < 3; ++k) {
-    buf(k, b0);
-    memset(b0.grid_v, 0, sizeof(float) * 3 * G);
-    memset(b0.grid_m, 0, sizeof(float) * G);
-  }
+  for (long long i = 0; i < 12 * G; ++i)
+    if (scratch[i] != 0.0f) return fail(FSI_E_INVALID, "the resident MPM scratch is not zero on entry");
   std::vector<int> box(2 * 6 * nb, 0);  // per block: [s - 1] lo, hi; [s - 2] lo, hi
   for (long long b = 0; b < nb; ++b)
     for (int k = 0; k < 2; ++k)
@@ -421,13 +415,13 @@ int fsi_mpm_subcycle_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, cons
   std::vector<float> vel(3 * (size_t)G);
   for (int s = 0; s < n; ++s) {
     FsiMpmState cur;
-    buf((n - 1 - s) % 3, cur);
+    buf(s % 3, cur);
     memset(ms->grid_f, 0, sizeof(float) * 3 * G);
     int rc = fsi_mpm_p2g(mp, &cur, stream);
     if (!rc && mode) rc = fsi_mpm_reaction(lp, ls, mp, &cur, cp, cc, mode, stream);
     if (rc) return rc;
     FsiMpmState old;
-    buf((n + 1 - s) % 3, old);
+    buf((s + 1) % 3, old);
     for (long long b = 0; b < nb; ++b) {
       int* bx = &box[b * 12];
       if (s >= 2) {
@@ -473,8 +467,13 @@ int fsi_mpm_subcycle_coupled(const FsiLbmParams* lp, const FsiLbmState* ls, cons
     rc = fsi_mpm_g2p(mp, &gat, stream);
     if (rc) return rc;
   }
-  // buffer 0 (the state's grid) to velocities
+  // the state's grid: the last buffer's updated velocities and masses
+  FsiMpmState last;
+  buf((n - 1) % 3, last);
+  memcpy(ms->grid_v, last.grid_v, sizeof(float) * 3 * G);
+  memcpy(ms->grid_m, last.grid_m, sizeof(float) * G);
   for (long long nn = 0; nn < G; ++nn) mpm_grid_node(*mp, *ms, nn);
+  memset(scratch, 0, sizeof(float) * 12 * G);
   return FSI_OK;
 }
 

@@ -1493,19 +1493,19 @@ __global__ void __launch_bounds__(TPB) k_mpm_grid_update_react(FsiMpmParams mp, 
 // block, every workgroup resident at once; each lane keeps its particle's x,
 // v, C and F in registers for the whole subcycle.
 //
-// Substep s scatters into grid buffer buf(s) = (n - 1 - s) mod 3 (buffer 0 is
-// the state's grid_v / grid_m, so the last substep's grid is the state's;
-// buffers 1 and 2 live in the caller's scratch) and G2P reads it back, with
-// the grid update applied per gathered node.  With three buffers one grid
-// barrier per substep, between the scatter and the gather, suffices: during
-// its scatter of substep s a workgroup also clears the nodes it scattered to
-// in substep s - 2 (the box of its stencils then), in buf(s + 1), which every
-// gather of substep s - 2 finished reading before the barrier of substep s - 1
-// and which the scatter of substep s + 1 writes only after the barrier of s.
-// A first barrier follows the clear of all three buffers (and grid_f) at the
-// start; a last one precedes the fold of the last substep's reaction records
-// and the conversion of buffer 0 to velocities, so the state after the call
-// is that of n calls of fsi_mpm_substep_coupled.
+// Substep s scatters into grid buffer buf(s) = s mod 3 of the caller's scratch
+// and G2P reads it back, with the grid update applied per gathered node.
+// With three buffers one grid barrier per substep, between the scatter and
+// the gather, suffices: during its scatter of substep s a workgroup also
+// clears the nodes it scattered to in substep s - 2 (the box of its stencils
+// then), in buf(s + 1), which every gather of substep s - 2 finished reading
+// before the barrier of substep s - 1 and which the scatter of substep s + 1
+// writes only after the barrier of s.  The scratch is zero on entry and on
+// exit: after a last barrier each workgroup clears its boxes of the last two
+// substeps.  The state's grid is cleared at the start and receives, from the
+// last gather, the velocity and mass of every node a workgroup's stencils
+// cover (workgroups that share a node write the same values), so the state
+// after the call is that of n calls of fsi_mpm_substep_coupled.
 //
 // Grid values cross workgroups (and XCDs) only through agent-scope atomics:
 // the scatter's float adds, the clears as atomic stores, and the gather's
@@ -1527,15 +1527,10 @@ struct MpmResBuf {
   float* m;    // mass
 };
 
-__device__ __forceinline__ MpmResBuf mpm_res_buf(const FsiMpmState& ms, float* scratch, long long G, int k) {
+__device__ __forceinline__ MpmResBuf mpm_res_buf(float* scratch, long long G, int k) {
   MpmResBuf b;
-  if (k == 0) {
-    b.mom = ms.grid_v;
-    b.m = ms.grid_m;
-  } else {
-    b.mom = scratch + (size_t)(k - 1) * 4 * G;
-    b.m = b.mom + 3 * G;
-  }
+  b.mom = scratch + (size_t)k * 4 * G;
+  b.m = b.mom + 3 * G;
   return b;
 }
 
@@ -1610,7 +1605,8 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
   const bool valid = q < P;
   const long long qs = valid ? q : 0;
   int bail = 0;
-  // the start: every buffer and grid_f cleared, a slice per workgroup
+  // the start: the state's grid cleared, a slice per workgroup (the last
+  // gather writes into it after n barriers)
   {
     const long long per = (G + nb - 1) / nb, n0 = per * blockIdx.x, n1 = n0 + per < G ? n0 + per : G;
     for (long long i = n0 + threadIdx.x; i < n1; i += PTPB) {
@@ -1620,8 +1616,6 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
         agent_store(ms.grid_f + d * G + i, 0.0f);
       }
       agent_store(ms.grid_m + i, 0.0f);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) agent_store(scratch + e * G + i, 0.0f);
     }
   }
   // the particle's state, held for the subcycle
@@ -1644,11 +1638,10 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
       bhi[k][d] = -1;
     }
   uint32_t bar = 0;
-  mpm_res_barrier(sync, nb, ++bar, bail);
   const bool same = cp.inv_dx_norm == mp.inv_dx;  // checked by the host
   for (int s = 0; s < n; ++s) {
-    const int kb = (n - 1 - s) % 3;
-    const MpmResBuf cur = mpm_res_buf(ms, scratch, G, kb);
+    const MpmResBuf cur = mpm_res_buf(scratch, G, s % 3);
+    const bool last = s == n - 1;
     const GlobalSink gp = {{cur.mom, cur.mom + G, cur.mom + 2 * G, cur.m, nullptr}};
     // ---- P2G (+ reaction) into buf(s), as p2g_react_block
     Stencil sn = stencil_at(x, mp.inv_dx);
@@ -1696,8 +1689,7 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
     }
     if (b.V > 0) tile_flush_fused(reinterpret_cast<const float4*>(tile), b, gp, mp.gnx, mp.gny, mp.gnz);
     // the nodes this workgroup scattered to two substeps ago, in buf(s + 1)
-    if (s >= 2)
-      mpm_res_clear_box(mpm_res_buf(ms, scratch, G, (n + 1 - s) % 3), blo[1], bhi[1], mp.gnx, mp.gny, mp.gnz, G);
+    if (s >= 2) mpm_res_clear_box(mpm_res_buf(scratch, G, (s + 1) % 3), blo[1], bhi[1], mp.gnx, mp.gny, mp.gnz, G);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       blo[1][d] = blo[0][d];
@@ -1705,7 +1697,7 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
       blo[0][d] = lo[d];
       bhi[0][d] = hi[d];
     }
-    if (MODE && s == n - 1) {  // the counters hold the last substep's reaction
+    if (MODE && last) {  // the counters hold the last substep's reaction
       const int act = block_sum_i(o.active, si);
       const float mpn = block_max_f(o.particle_norm, sf);
       const float mgc = block_max_f(o.max_contrib, sf);
@@ -1755,6 +1747,12 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
           float vel[3] = {nv[u][0], nv[u][1], nv[u][2]};
           if (use[u] && nm[u] > 0.0f) grid_node_velocity(mp, nm[u], nv[u], zf, ijk[u][0], ijk[u][1], ijk[u][2], vel);
           if (i < b.V) vt[i] = use[u] ? make_float4(vel[0], vel[1], vel[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if (last && use[u]) {  // the state's grid after the call: grid_update's output
+            const long long nn = ((long long)ijk[u][0] * mp.gny + ijk[u][1]) * mp.gnz + ijk[u][2];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) agent_store(ms.grid_v + d * G + nn, vel[d]);
+            agent_store(ms.grid_m + nn, nm[u]);
+          }
         }
       }
       __syncthreads();
@@ -1801,6 +1799,17 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
             grid_node_velocity(mp, gm[k], mom, zf, base[0] + k / 9, base[1] + (k / 3) % 3, base[2] + k % 3, gv[k]);
           }
         }
+        if (last) {  // the state's grid after the call
+#pragma unroll
+          for (int k = 0; k < 27; ++k) {
+            if (!((inm >> k) & 1u)) continue;
+            const long long nn =
+                ((long long)(base[0] + k / 9) * mp.gny + (base[1] + (k / 3) % 3)) * mp.gnz + (base[2] + k % 3);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) agent_store(ms.grid_v + d * G + nn, gv[k][d]);
+            agent_store(ms.grid_m + nn, gm[k]);
+          }
+        }
       }
       float xn[3], vn[3], Cn[9], Fn[9], Jn;
       g2p_update(mp, ms, q, fx, w, gv, inm, x, F, xn, vn, Cn, Fn, Jn);
@@ -1814,7 +1823,7 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
         C[e] = Cn[e];
         F[e] = Fn[e];
       }
-      if (s == n - 1) {
+      if (last) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           ms.x[d * P + q] = x[d];
@@ -1826,23 +1835,11 @@ __global__ void __launch_bounds__(PTPB) k_mpm_resident(FsiLbmParams lp, FsiLbmSt
       }
     }
   }
+  // every gather of the last substep done: this workgroup's boxes of the last
+  // two substeps cleared, so the scratch is zero again
   mpm_res_barrier(sync, nb, ++bar, bail);
-  // buffer 0 (the state's grid) to velocities, a slice per workgroup; the
-  // scratch buffers are cleared by the next call
-  {
-    const long long per = (G + nb - 1) / nb, n0 = per * blockIdx.x, n1 = n0 + per < G ? n0 + per : G;
-    for (long long i = n0 + threadIdx.x; i < n1; i += PTPB) {
-      const float m = agent_load(ms.grid_m + i);
-      if (!(m > 0.0f)) continue;
-      const int K = (int)(i % mp.gnz), J = (int)((i / mp.gnz) % mp.gny), I = (int)(i / ((long long)mp.gnz * mp.gny));
-      const float mom[3] = {agent_load(ms.grid_v + i), agent_load(ms.grid_v + G + i), agent_load(ms.grid_v + 2 * G + i)};
-      const float zf[3] = {0.0f, 0.0f, 0.0f};
-      float vv[3];
-      grid_node_velocity(mp, m, mom, zf, I, J, K, vv);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) agent_store(ms.grid_v + d * G + i, vv[d]);
-    }
-  }
+  mpm_res_clear_box(mpm_res_buf(scratch, G, (n - 1) % 3), blo[0], bhi[0], mp.gnx, mp.gny, mp.gnz, G);
+  if (n >= 2) mpm_res_clear_box(mpm_res_buf(scratch, G, (n - 2) % 3), blo[1], bhi[1], mp.gnx, mp.gny, mp.gnz, G);
   if (MODE && blockIdx.x == 0) fold_react_records(rec, nb, cc, sd);
   // the last workgroup out resets the arrival words
   __builtin_amdgcn_s_waitcnt(0);

@@ -286,7 +286,7 @@ MPM_RES_TIMEOUT_WORD = 576  # FSI_MPM_RES_TIMEOUT_WORD
 
 def mpm_res_scratch_floats(grid_nodes: int) -> int:
     """FSI_MPM_RES_SCRATCH_FLOATS(grid_nodes)."""
-    return 8 * int(grid_nodes)
+    return 12 * int(grid_nodes)
 
 
 def coupled_workspace_doubles(n_particles: int) -> int:

@@ -54,8 +54,13 @@ class MPMSolid3D:
         # deterministic)
         self.transfer_order = "atomic"
         # subcycle_coupled in the "atomic" order runs as one resident launch
-        # (fsi_mpm_subcycle_coupled) when the device holds it
+        # (fsi_mpm_subcycle_coupled) when the device holds it and there are at
+        # least resident_min_substeps substeps: with fewer, the launch's fixed
+        # costs (its clear of the state's grid by the particle workgroups
+        # alone, two barriers) outweigh what it saves (measured: 1 substep per
+        # FSI step is slower resident, 10 faster)
         self.resident_subcycle = True
+        self.resident_min_substeps = 4
         self._res_declined = False
         self._res_scratch = None
         self._res_sync = None
@@ -274,9 +279,11 @@ class MPMSolid3D:
         """n calls of substep_coupled (the MPM substeps of one FSI step, the
         lattice unchanged in between).  In the "atomic" order they run as one
         launch that keeps the particles on chip (fsi_mpm_subcycle_coupled)
-        when the device holds every particle workgroup at once; otherwise, or
-        when resident_subcycle is off, as n substeps."""
-        if self.resident_subcycle and not self._res_declined and self.transfer_order == "atomic" and n > 0:
+        when the device holds every particle workgroup at once and n is at
+        least resident_min_substeps; otherwise, or when resident_subcycle is
+        off, as n substeps."""
+        if (self.resident_subcycle and not self._res_declined and self.transfer_order == "atomic"
+                and n >= max(1, self.resident_min_substeps)):
             if self._run_resident(int(n), lbm, coupler, mode):
                 return
             self._res_declined = True
@@ -306,12 +313,14 @@ class MPMSolid3D:
     def check_resident_waits(self):
         """Raise if a resident MPM subcycle went on past a barrier without
         every workgroup (a wait that timed out: its results are invalid); the
-        sync words are cleared so the next call starts clean."""
+        sync words and the grid scratch are cleared so the next call starts
+        clean."""
         if self._res_sync is None:
             return
         timeouts = int(self._res_sync[_native.MPM_RES_TIMEOUT_WORD].item())
         if timeouts:
             self._res_sync.zero_()
+            self._res_scratch.zero_()
             raise RuntimeError(f"resident MPM subcycle: {timeouts} barrier waits timed out; results are invalid")
 
     def _substep_ordered(self, lbm, coupler, mode):

@@ -63,6 +63,7 @@ def _driver(w, out, native, resident, cfg=None):
                     write_geometry_file=False)
     d.initialize()
     d.solid.resident_subcycle = resident
+    d.solid.resident_min_substeps = 1
     return d
 
 

@@ -26,6 +26,7 @@ def _driver(w, out, emu, resident, mpm_substeps):
     d = FSIDriver3D(cfg, str(out), particle_cloud=w.cloud, native=emu, device="cpu", write_geometry_file=False)
     d.initialize()
     d.solid.resident_subcycle = resident
+    d.solid.resident_min_substeps = 1
     return d
 
 
