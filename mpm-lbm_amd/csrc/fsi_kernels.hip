@@ -994,13 +994,16 @@ __global__ void __launch_bounds__(TPB) k_lbm_stats_partial(FsiLbmParams p, FsiLb
 
 __global__ void __launch_bounds__(64) k_stats_final(const double* part, int nparts, int nfields, int which,
                                                     double* out) {
-  // one work-item per field, fixed order over the partials: deterministic
-  const int f = threadIdx.x;
+  // one wave per field: lane l folds partials l, l + 64, ..., then the wave
+  // reduction combines the lanes, both in a fixed order: deterministic
+  __shared__ double sd[1];
+  const int f = (int)blockIdx.x;
   if (f >= nfields) return;
   const int op = which == 0 ? lbm_stats_op(f) : mpm_stats_op(f);
   double r = red_identity(op);
-  for (int b = 0; b < nparts; ++b) r = red_combine(op, r, part[(long long)b * nfields + f]);
-  out[f] = r;
+  for (int b = (int)threadIdx.x; b < nparts; b += 64) r = red_combine(op, r, part[(long long)b * nfields + f]);
+  r = block_reduce(r, op, sd);
+  if (threadIdx.x == 0) out[f] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -1050,7 +1053,9 @@ __global__ void __launch_bounds__(TPB) k_proj_normalize(FsiLbmParams lp, FsiLbmS
   const double vr = block_reduce((double)o.phi_raw * cp.cell_volume_norm, RED_SUM, sd);
   const double vc = block_reduce((double)o.phi_clamped * cp.cell_volume_norm, RED_SUM, sd);
   if (threadIdx.x == 0) {
-    atomicMax(&cc->max_phi_raw_bits, __float_as_uint(mp));
+    // a zero maximum (most workgroups) cannot raise the counter: no atomic on
+    // the shared word
+    if (__float_as_uint(mp) != 0u) atomicMax(&cc->max_phi_raw_bits, __float_as_uint(mp));
     if (act) atomicAdd(&cc->active_cell_count, act);
     if (m != 0.0) atomicAdd(&cc->projected_mass, m);
     if (vr != 0.0) atomicAdd(&cc->projected_volume_raw, vr);
@@ -1086,8 +1091,10 @@ __global__ void __launch_bounds__(TPB) k_penalty(FsiLbmParams lp, FsiLbmState ls
   for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)f[d], RED_SUM, sd);
   if (threadIdx.x == 0) {
     if (act) atomicAdd(&cc->active_force_cell_count, act);
-    atomicMax(&cc->max_cell_force_bits, __float_as_uint(mfn));
-    atomicMax(&cc->max_hydro_force_bits, __float_as_uint(mfn));
+    if (__float_as_uint(mfn) != 0u) {
+      atomicMax(&cc->max_cell_force_bits, __float_as_uint(mfn));
+      atomicMax(&cc->max_hydro_force_bits, __float_as_uint(mfn));
+    }
     for (int d = 0; d < 3; ++d)
       if (net[d] != 0.0) {
         atomicAdd(&cc->net_cell_force[d], net[d]);
@@ -1193,8 +1200,8 @@ __global__ void __launch_bounds__(PTPB) k_mpm_reaction(FsiLbmParams lp, FsiLbmSt
   for (int d = 0; d < 3; ++d) netg[d] = block_reduce((double)o.g[d], RED_SUM, sd);
   if (threadIdx.x == 0) {
     if (act) atomicAdd(&cc->active_reaction_particle_count, act);
-    atomicMax(&cc->max_particle_reaction_bits, __float_as_uint(mpn));
-    atomicMax(&cc->max_grid_reaction_bits, __float_as_uint(mgc));
+    if (__float_as_uint(mpn) != 0u) atomicMax(&cc->max_particle_reaction_bits, __float_as_uint(mpn));
+    if (__float_as_uint(mgc) != 0u) atomicMax(&cc->max_grid_reaction_bits, __float_as_uint(mgc));
     for (int d = 0; d < 3; ++d) {
       if (net[d] != 0.0) atomicAdd(&cc->net_particle_reaction[d], net[d]);
       if (netg[d] != 0.0) atomicAdd(&cc->net_grid_reaction[d], netg[d]);
@@ -2846,7 +2853,7 @@ int fsi_lbm_reduce_stats(const FsiLbmParams* p, const FsiLbmState* st, FsiLbmSta
     return fail(FSI_E_INVALID, "reduce_stats needs workspace, out, solid_phi, cell_force, hydro_force");
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(k_lbm_stats_partial, dim3(FSI_REDUCE_BLOCKS), dim3(TPB), 0, s, *p, *st, workspace_dev);
-  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, reinterpret_cast<const double*>(workspace_dev),
+  hipLaunchKernelGGL(k_stats_final, dim3(LBM_STATS_N), dim3(64), 0, s, reinterpret_cast<const double*>(workspace_dev),
                      FSI_REDUCE_BLOCKS, LBM_STATS_N, 0, reinterpret_cast<double*>(out_dev));
   return check_launch("k_lbm_stats");
 }
@@ -3154,7 +3161,7 @@ int fsi_mpm_reduce_stats(const FsiMpmParams* mp, const FsiMpmState* ms, FsiMpmSt
   if (!workspace_dev || !out_dev) return fail(FSI_E_INVALID, "reduce_stats needs workspace and out");
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(k_mpm_stats_partial, dim3(FSI_REDUCE_BLOCKS), dim3(TPB), 0, s, *mp, *ms, workspace_dev);
-  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, reinterpret_cast<const double*>(workspace_dev),
+  hipLaunchKernelGGL(k_stats_final, dim3(MPM_STATS_N), dim3(64), 0, s, reinterpret_cast<const double*>(workspace_dev),
                      FSI_REDUCE_BLOCKS, MPM_STATS_N, 1, reinterpret_cast<double*>(out_dev));
   return check_launch("k_mpm_stats");
 }
