@@ -256,6 +256,12 @@ typedef struct FsiBatchInstance {
   int32_t pad0;
   double* react_ws;   /* FSI_COUPLED_WORKSPACE_DOUBLES(n_particles): reaction records of MPM_P2G_REACT */
   int32_t* boxes;     /* FSI_BATCH_BOX_INTS, zeroed when the instance enters the batch (see below) */
+  float* res_halo;    /* resident != 0: FSI_RES_HALO_FLOATS(cells) floats (fsi_lbm_subcycle_resident's halo) */
+  uint32_t* res_sync; /* resident != 0: FSI_RES_SYNC_WORDS(nx, ny) words, zero before the first exchange, kept */
+  int32_t resident;   /* 1: a moving-boundary instance whose subcycle runs as LBM_RESIDENT (its PLANES domain is
+                         empty, so the LBM_SUBSTEP* launches leave it out); it must meet what
+                         fsi_lbm_subcycle_resident covers, and the device must hold its blocks at once */
+  int32_t pad1;
 } FsiBatchInstance;
 
 /* Bounding boxes that confine the per-cell and per-node passes of the
@@ -299,6 +305,13 @@ typedef struct FsiBatchInstance {
 #define FSI_BATCH_OP_LBM_SUBSTEP_NONE 19
 #define FSI_BATCH_OP_LBM_MB_FINALIZE_FOLD 20 /* after MB_FINALIZE: fold its records into the counters */
 #define FSI_BATCH_OP_BOX_ROLL 21 /* t = 0: roll the projection box, 1: the MPM box */
+/* The whole moving-wall subcycle of ONE instance with `resident` set, in one
+ * launch with its lattice on chip (fsi_lbm_subcycle_resident from cur0 over
+ * n_substeps substeps, with the flags of the LBM_SUBSTEP_MB launches it
+ * replaces).  Its block map lists (instance, b) for b = 0 ..
+ * FSI_RES_BLOCKS(nx, ny) - 1 of that one instance, in order: a launch must
+ * not hold the blocks of two instances, which could not all be resident. */
+#define FSI_BATCH_OP_LBM_RESIDENT 22
 
 /* Domain of each operation (the map passed to fsi_batch_launch must be built
  * for it): LBM_SUBSTEP, LBM_SUBSTEP_MB / _PENALTY / _NONE -> PLANES; REBUILD, MOMENTS, MB_BEGIN,

@@ -1156,6 +1156,21 @@ inline const char* res_unsupported(const FsiLbmParams& p, const FsiLbmState& st,
   return nullptr;
 }
 
+// The flags of a batched instance's LBM_RESIDENT launch: those of the
+// LBM_SUBSTEP_MB launches it replaces (stored moments at t = 0, the previous
+// links gathered for t > 0, statistics at the last substep).
+constexpr uint32_t BATCH_RESIDENT_FLAGS =
+    FSI_SUB_MOVING_WALL | FSI_SUB_STORED_MOMENTS | FSI_SUB_GATHER_PREV | FSI_SUB_DIAG;
+
+// Why an active batched instance marked resident cannot take LBM_RESIDENT
+// (nullptr: it can, up to the device's capacity, which the launch checks).
+inline const char* batch_resident_invalid(const FsiBatchInstance& I) {
+  if (!I.resident) return nullptr;
+  if (I.mode != FSI_BATCH_MODE_MOVING_BOUNDARY || I.cell_force)
+    return "a resident batched instance must be a moving-boundary instance without cell force";
+  return res_unsupported(I.lp, I.ls, I.cur0, I.n_substeps, BATCH_RESIDENT_FLAGS, I.res_halo, I.res_sync);
+}
+
 // Phase A of fluid cell c from its post-stream populations F.  put(out)
 // fills slot (s, c) with out[s] for s = 1..18; bounce(s, v) fills slot
 // (LR[s], c + e_s) on a link s.
@@ -2744,7 +2759,7 @@ FSI_HD long long batch_domain_blocks(const FsiBatchInstance& I, int domain) {
     return cb < FSI_BATCH_BOX_BLOCKS ? cb : FSI_BATCH_BOX_BLOCKS;
   }
   if (domain == FSI_BATCH_DOMAIN_PLANES)  // one i-plane per run of tiles, as the single-instance substep
-    return (long long)I.lp.nx * (((long long)I.lp.ny * I.lp.nz + 255) / 256);
+    return I.resident ? 0 : (long long)I.lp.nx * (((long long)I.lp.ny * I.lp.nz + 255) / 256);
   long long n = domain == FSI_BATCH_DOMAIN_CELLS ? (long long)I.lp.nx * I.lp.ny * I.lp.nz
               : domain == FSI_BATCH_DOMAIN_NODES ? (long long)I.mp.gnx * I.mp.gny * I.mp.gnz
                                                  : (long long)I.mp.n_particles;

@@ -601,6 +601,8 @@ int fsi_batch_block_map(const FsiBatchInstance* inst, int n_inst, int domain, in
   for (int i = 0; i < n_inst; ++i) {
     if (inst[i].lp.open_bc != FSI_OPEN_BC_EQUILIBRIUM)
       return fail(FSI_E_INVALID, "the batched form runs the legacy x-face reset only (open_bc 0)");
+    if (inst[i].active)
+      if (const char* why = batch_resident_invalid(inst[i])) return fail(FSI_E_INVALID, why);
     if (!inst[i].active || !(mode_mask & (1 << inst[i].mode)) || inst[i].mpm_substeps <= min_mpm) continue;
     const long long nb = batch_domain_blocks(inst[i], domain);
     if (map)
@@ -659,6 +661,12 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int
     return FSI_OK;
   }
   if (!map) return fail(FSI_E_INVALID, "null block map");
+  if (op == FSI_BATCH_OP_LBM_RESIDENT) {  // the one instance of the map, as fsi_lbm_subcycle_resident
+    if (n_blocks <= 0) return FSI_OK;
+    const FsiBatchInstance& I = inst[map[0]];
+    return fsi_lbm_subcycle_resident(&I.lp, &I.ls, I.cur0, I.n_substeps, BATCH_RESIDENT_FLAGS, I.res_halo, I.res_sync,
+                                     nullptr);
+  }
   std::vector<char> sel((size_t)n_inst, 0);
   for (int b = 0; b < n_blocks; ++b) sel[(size_t)map[2 * b]] = 1;
   for (int i = 0; i < n_inst; ++i) {
@@ -867,7 +875,7 @@ struct EmuBatchGraph {
 int check_ops(const FsiBatchOp* ops, int n_ops, const EmuBatchGraph* g) {
   if (g && (size_t)n_ops != g->ops.size()) return fail(FSI_E_INVALID, "batch graph update with another number of operations");
   for (int i = 0; i < n_ops; ++i) {
-    if (ops[i].op < FSI_BATCH_OP_LBM_SUBSTEP || ops[i].op > FSI_BATCH_OP_BOX_ROLL)
+    if (ops[i].op < FSI_BATCH_OP_LBM_SUBSTEP || ops[i].op > FSI_BATCH_OP_LBM_RESIDENT)
       return fail(FSI_E_INVALID, "unknown batch operation");
     if (ops[i].op != FSI_BATCH_OP_COUNTERS_CLEAR && (!ops[i].block_map || ops[i].n_blocks <= 0))
       return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");

@@ -648,17 +648,19 @@ __device__ __forceinline__ float res_get(float* plane, uint32_t c) {
 #endif
 }
 
+// The subcycle of block b of the lattice (p, st); the workgroup is that
+// block's 1024 work-items.
 template <bool FORCE>
-__global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p, FsiLbmState st, int cur, int n,
-                                                                  uint32_t flags, float* __restrict__ halo,
-                                                                  uint32_t* __restrict__ sync) {
+__device__ __forceinline__ void res_subcycle_block(const FsiLbmParams& p, const FsiLbmState& st, int cur, int n,
+                                                   uint32_t flags, float* __restrict__ halo,
+                                                   uint32_t* __restrict__ sync, uint32_t b) {
   __shared__ float lv[NLINK * RES_THREADS];     // slot planes, directions 1..18
   __shared__ float uw[3 * RES_RING];            // solid velocities of the block and the ring around it
   __shared__ double wacc[4 * RES_THREADS];      // wall cells' accumulator records
   __shared__ float ou[3 * RES_BJ * RES_NZ];     // velocity of the cells at i = nx - 2
   __shared__ uint32_t s_base;
   __shared__ int bail;
-  const uint32_t b = blockIdx.x, t = threadIdx.x;
+  const uint32_t t = threadIdx.x;
   const ResLane L = res_lane(p, b, t);
   const uint32_t Nk = wave_uniform((uint32_t)p.nx * (uint32_t)p.ny * (uint32_t)RES_NZ);
   const uint32_t info = cell_info(st, L.c);
@@ -804,6 +806,13 @@ __global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p,
 #pragma unroll
     for (int d = 0; d < 4; ++d) st.hydro_accum[4 * (size_t)L.c + d] = wacc[4 * t + d];
   }
+}
+
+template <bool FORCE>
+__global__ void __launch_bounds__(RES_THREADS, 1) k_lbm_resident(FsiLbmParams p, FsiLbmState st, int cur, int n,
+                                                                  uint32_t flags, float* __restrict__ halo,
+                                                                  uint32_t* __restrict__ sync) {
+  res_subcycle_block<FORCE>(p, st, cur, n, flags, halo, sync, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(64) k_open_bc_clear(FsiLbmState st) {
@@ -2234,6 +2243,15 @@ __global__ void __launch_bounds__(64) kb_box_roll(const FsiBatchInstance* __rest
     box_roll(I.boxes + FSI_BATCH_BOX_MPM, I.mp.gnx, I.mp.gny, I.mp.gnz);
 }
 
+// LBM_RESIDENT: the moving-wall subcycle of the one instance the block map
+// names, block b of it at map entry b, as k_lbm_resident.
+__global__ void __launch_bounds__(RES_THREADS, 1) kb_lbm_resident(const FsiBatchInstance* __restrict__ inst,
+                                                                  const int32_t* __restrict__ bmap, int) {
+  const FsiBatchInstance& I = inst[bmap[2 * blockIdx.x]];
+  res_subcycle_block<false>(I.lp, I.ls, I.cur0, I.n_substeps, BATCH_RESIDENT_FLAGS, I.res_halo, I.res_sync,
+                            (uint32_t)bmap[2 * blockIdx.x + 1]);
+}
+
 __global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* inst, int n_inst, int which) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n_inst || !inst[i].active) return;
@@ -2503,8 +2521,46 @@ int batch_kernel(int op, int t, BatchKernel& k) {
     case FSI_BATCH_OP_MPM_P2G_REACT: k.fn = (const void*)&kb_mpm_p2g_react; k.block = PTPB; break;
     case FSI_BATCH_OP_MPM_GRID_UPDATE_REACT: k.fn = (const void*)&kb_mpm_grid_update_react; break;
     case FSI_BATCH_OP_BOX_ROLL: k.fn = (const void*)&kb_box_roll; k.block = 64; break;
+    case FSI_BATCH_OP_LBM_RESIDENT: k.fn = (const void*)&kb_lbm_resident; k.block = RES_THREADS; break;
     default: return fail(FSI_E_INVALID, "unknown batch operation");
   }
+  return FSI_OK;
+}
+
+// How many workgroups of the resident subcycle the current device holds at
+// once (CUs x workgroups per CU), cached per device.
+int res_capacity(bool force, long long& capacity) {
+  static thread_local int dev_cached = -1;
+  static thread_local long long cap[2] = {0, 0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(FSI_E_HIP, "hipGetDevice failed");
+  if (dev != dev_cached) {
+    int cus = 0, per_cu[3] = {0, 0, 0};
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], k_lbm_resident<false>, RES_THREADS, 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], k_lbm_resident<true>, RES_THREADS, 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[2], kb_lbm_resident, RES_THREADS, 0) != hipSuccess)
+      return fail(FSI_E_HIP, "occupancy query of the resident subcycle failed");
+    // the batched kernel runs the unforced subcycle: both hold the smaller count
+    cap[0] = (long long)cus * (per_cu[0] < per_cu[2] ? per_cu[0] : per_cu[2]);
+    cap[1] = (long long)cus * per_cu[1];
+    dev_cached = dev;
+  }
+  capacity = cap[force ? 1 : 0];
+  return FSI_OK;
+}
+
+// An LBM_RESIDENT launch must fit the device at once (its block map holds one
+// instance's blocks).
+int check_resident_blocks(int op, int n_blocks) {
+  if (op != FSI_BATCH_OP_LBM_RESIDENT) return FSI_OK;
+  long long capacity = 0;
+  if (const int rc = res_capacity(false, capacity)) return rc;
+  if (n_blocks > capacity)
+    return fail(FSI_E_UNSUPPORTED, "LBM_RESIDENT with " + std::to_string(n_blocks) +
+                                       " workgroups, the device holds " + std::to_string(capacity));
   return FSI_OK;
 }
 
@@ -2568,6 +2624,7 @@ int check_graph_ops(const BatchGraph* g, const FsiBatchOp* ops, int n_ops, bool 
     if (batch_kernel(ops[i].op, ops[i].t, k) != FSI_OK) return FSI_E_INVALID;
     if (ops[i].op != FSI_BATCH_OP_COUNTERS_CLEAR && (!ops[i].block_map || ops[i].n_blocks <= 0))
       return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");
+    if (const int rc = check_resident_blocks(ops[i].op, ops[i].n_blocks)) return rc;
     if (update && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t))
       return fail(FSI_E_INVALID, "batch graph update with other operations");
   }
@@ -2748,27 +2805,12 @@ int fsi_lbm_subcycle_resident(const FsiLbmParams* p, const FsiLbmState* st, int 
   if (!st->solid_vel) return fail(FSI_E_INVALID, "moving-wall subcycle needs solid_vel");
   const bool force = (flags & FSI_SUB_CELL_FORCE) != 0;
   if (force && !st->cell_force) return fail(FSI_E_INVALID, "cell_force is null");
-  // every block must be resident at once: at most (CUs x workgroups per CU)
-  static thread_local int dev_cached = -1;
-  static thread_local long long capacity[2] = {0, 0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return fail(FSI_E_HIP, "hipGetDevice failed");
-  if (dev != dev_cached) {
-    int cus = 0, per_cu[2] = {0, 0};
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[0], k_lbm_resident<false>, RES_THREADS, 0) !=
-            hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[1], k_lbm_resident<true>, RES_THREADS, 0) != hipSuccess)
-      return fail(FSI_E_HIP, "occupancy query of the resident subcycle failed");
-    capacity[0] = (long long)cus * per_cu[0];
-    capacity[1] = (long long)cus * per_cu[1];
-    dev_cached = dev;
-  }
+  long long capacity = 0;
+  if ((rc = res_capacity(force, capacity)) != FSI_OK) return rc;
   const int nblk = res_blocks(*p);
-  if (nblk > capacity[force ? 1 : 0])
+  if (nblk > capacity)
     return fail(FSI_E_UNSUPPORTED, "the lattice needs " + std::to_string(nblk) +
-                                       " resident workgroups, the device holds " +
-                                       std::to_string(capacity[force ? 1 : 0]));
+                                       " resident workgroups, the device holds " + std::to_string(capacity));
   hipStream_t s = as_stream(stream);
   if (force)
     hipLaunchKernelGGL(k_lbm_resident<true>, dim3(nblk), dim3(RES_THREADS), 0, s, *p, *st, cur, n, flags, halo, sync);
@@ -3177,6 +3219,8 @@ int fsi_batch_block_map(const FsiBatchInstance* inst_host, int n_inst, int domai
     const FsiBatchInstance& I = inst_host[i];
     if (I.lp.open_bc != FSI_OPEN_BC_EQUILIBRIUM)
       return fail(FSI_E_INVALID, "the batched form runs the legacy x-face reset only (open_bc 0)");
+    if (I.active)
+      if (const char* why = batch_resident_invalid(I)) return fail(FSI_E_INVALID, why);
     if (!I.active || !(mode_mask & (1 << I.mode)) || I.mpm_substeps <= min_mpm) continue;
     const long long nb = batch_domain_blocks(I, domain);
     if (total + nb > 0x7fffffffLL) return fail(FSI_E_INVALID, "batch too large for one launch");
@@ -3195,6 +3239,7 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const
   if (!inst_dev || n_inst <= 0) return fail(FSI_E_INVALID, "empty batch");
   BatchKernel k;
   if (batch_kernel(op, t, k) != FSI_OK) return FSI_E_INVALID;
+  if (const int rc = check_resident_blocks(op, n_blocks)) return rc;
   if (op != FSI_BATCH_OP_COUNTERS_CLEAR) {
     if (!block_map_dev) return fail(FSI_E_INVALID, "null block map");
     if (n_blocks <= 0) return FSI_OK;

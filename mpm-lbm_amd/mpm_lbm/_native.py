@@ -192,7 +192,7 @@ BATCH_BOX_BLOCKS = 64  # FSI_BATCH_BOX_BLOCKS
  BATCH_OP_MPM_P2G, BATCH_OP_MPM_REACTION, BATCH_OP_MPM_GRID_UPDATE, BATCH_OP_MPM_G2P,
  BATCH_OP_COUNTERS_CLEAR, BATCH_OP_MPM_P2G_REACT, BATCH_OP_MPM_GRID_UPDATE_REACT, BATCH_OP_LBM_SUBSTEP_MB,
  BATCH_OP_LBM_SUBSTEP_PENALTY, BATCH_OP_LBM_SUBSTEP_NONE, BATCH_OP_LBM_MB_FINALIZE_FOLD,
- BATCH_OP_BOX_ROLL) = range(22)
+ BATCH_OP_BOX_ROLL, BATCH_OP_LBM_RESIDENT) = range(23)
 BATCH_BOX_INTS = 32  # FSI_BATCH_BOX_INTS
 
 # domain of each batched operation (the block map a launch takes is built for it)
@@ -218,6 +218,7 @@ class BatchInstance(ctypes.Structure):
         ("cp", CouplingParams), ("cc", vp),
         ("mode", i32), ("cur0", i32), ("n_substeps", i32), ("mpm_substeps", i32), ("active", i32),
         ("cell_force", i32), ("pad0", i32), ("react_ws", vp), ("boxes", vp),
+        ("res_halo", vp), ("res_sync", vp), ("resident", i32), ("pad1", i32),
     ]
 
 

@@ -73,7 +73,7 @@ def check_batchable(d):
 
 class BatchedFSI:
     def __init__(self, workloads=None, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True,
-                 drivers=None, n_substeps=None, substep_plan=None, capacity=None):
+                 drivers=None, n_substeps=None, substep_plan=None, capacity=None, resident_lbm=True):
         """Batch `workloads` (a driver is built for each), or batch initialized
         `drivers` that already exist (they keep their state).  n_substeps: LBM
         substeps per exchange (default: the moving-boundary instances' FSI
@@ -84,7 +84,11 @@ class BatchedFSI:
         it on the first exchange (`SUBSTEP_PLANS`).  capacity: slots of the
         table (default: exactly the instances given; more leaves free slots
         for `admit`).  use_graph: run each exchange as a native graph (the
-        tuning exchange runs eagerly)."""
+        tuning exchange runs eagerly).  resident_lbm: moving-boundary instances
+        whose lattice the resident subcycle covers (FP32, legacy x faces,
+        nz = 64, nx and ny multiples of 4, every block resident at once) run
+        their subcycle as one LBM_RESIDENT launch each instead of inside the
+        batched substep launches."""
         if substep_plan is not None and substep_plan not in SUBSTEP_PLANS:
             raise ValueError(f"substep_plan must be one of {SUBSTEP_PLANS} or None")
         self.substep_plan = substep_plan
@@ -123,6 +127,10 @@ class BatchedFSI:
         if self.n_substeps % 2:
             raise ValueError("the batched exchange needs an even number of LBM substeps")
         self.drivers = [None] * self.capacity
+        self.resident_lbm = bool(resident_lbm)
+        self._res = [None] * self.capacity  # (halo, sync) of resident instances
+        self._cus = (torch.cuda.get_device_properties(self.device).multi_processor_count
+                     if self.device.type == "cuda" else None)
         self._counters = [None] * self.capacity
         self._react_ws = [None] * self.capacity
         self._boxes = [None] * self.capacity
@@ -145,9 +153,11 @@ class BatchedFSI:
 
     @classmethod
     def slots(cls, capacity, native=None, device=None, n_substeps=120, substep_plan="split", use_graph=True):
-        """An empty slot table of `capacity` instances (the serving form)."""
+        """An empty slot table of `capacity` instances (the serving form).  Its
+        instances stay in the batched substep launches (no LBM_RESIDENT), so the
+        exchange's launch list keeps its shape as requests come and go."""
         return cls(drivers=[], native=native, device=device, n_substeps=n_substeps, substep_plan=substep_plan,
-                   use_graph=use_graph, capacity=capacity)
+                   use_graph=use_graph, capacity=capacity, resident_lbm=False)
 
     # ----------------------------------------------------------------- slots
     def _fill_slot(self, k, d):
@@ -178,6 +188,12 @@ class BatchedFSI:
         I.cell_force = 1 if d.config.coupling_mode == "penalty" else 0
         I.react_ws = _native.ptr(self._react_ws[k])
         I.boxes = _native.ptr(self._boxes[k])
+        I.resident = 0
+        if self.resident_lbm and d.config.coupling_mode == "moving_boundary" and self._resident_ok(lbm):
+            halo = torch.empty((_native.res_halo_floats(lbm.N),), dtype=torch.float32, device=self.device)
+            sync = torch.zeros((_native.res_sync_words(lbm.nx, lbm.ny),), dtype=torch.int32, device=self.device)
+            self._res[k] = (halo, sync)
+            I.res_halo, I.res_sync, I.resident = _native.ptr(halo), _native.ptr(sync), 1
         lbm._cell_force_active = bool(I.cell_force)
         self.drivers[k] = d
         self._dirty = True
@@ -207,7 +223,29 @@ class BatchedFSI:
         self._counters[k] = None
         self._react_ws[k] = None
         self._boxes[k] = None
+        self._res[k] = None
         self._dirty = True
+
+    def _resident_ok(self, lbm):
+        """Whether a moving-boundary instance's subcycle can run as LBM_RESIDENT:
+        the lattice the resident subcycle covers, with every block (one per CU)
+        resident at once."""
+        if not lbm._resident_shape_ok():
+            return False
+        return self._cus is None or _native.res_blocks(lbm.nx, lbm.ny) <= self._cus
+
+    def check_resident_waits(self):
+        """Raise if an LBM_RESIDENT launch went on without a neighbour block
+        (a wait that timed out: that instance's results are invalid)."""
+        for k in self.live_slots:
+            if self._res[k] is None:
+                continue
+            lbm = self.drivers[k].lbm
+            sync = self._res[k][1]
+            timeouts = int(sync[64 * _native.res_blocks(lbm.nx, lbm.ny)].item())
+            if timeouts:
+                sync.zero_()
+                raise RuntimeError(f"slot {k}: {timeouts} neighbour waits of the resident subcycle timed out")
 
     @property
     def live_slots(self):
@@ -229,6 +267,19 @@ class BatchedFSI:
     def _write_map(self, key):
         """Block map of `key` for the live instances, into its device buffer
         (grown when too small: the graphs take the new address on update)."""
+        if key[0] == "resident":  # the blocks of one resident instance, in order
+            k = key[1]
+            d = self.drivers[k]
+            n = _native.res_blocks(d.lbm.nx, d.lbm.ny) if d is not None and self._res[k] is not None else 0
+            host = np.stack([np.full(n, k, dtype=np.int32), np.arange(n, dtype=np.int32)], axis=1)
+            entry = self._maps.get(key)
+            if entry is None or entry[0].shape[0] < n:
+                entry = [torch.empty((max(n, 1), 2), dtype=torch.int32, device=self.device), 0]
+                self._maps[key] = entry
+            if n:
+                entry[0][:n].copy_(torch.from_numpy(host))
+            entry[1] = n
+            return
         domain, modes, min_mpm = key
         mask = sum(1 << m for m in modes)
         addr = ctypes.addressof(self._inst_host)
@@ -264,6 +315,20 @@ class BatchedFSI:
             return
         self.native.call("fsi_batch_launch", op, _native.ptr(self._inst_dev), self.capacity, _native.ptr(table),
                          live, t, stream_handle(self.device))
+
+    def _launch_resident(self, k):
+        """The subcycle of resident slot k as one LBM_RESIDENT launch."""
+        key = ("resident", k)
+        if key not in self._maps:
+            self._write_map(key)
+        table, live = self._maps[key]
+        if live == 0:
+            return
+        if self._recording is not None:
+            self._recording.append((_native.BATCH_OP_LBM_RESIDENT, 0, key))
+            return
+        self.native.call("fsi_batch_launch", _native.BATCH_OP_LBM_RESIDENT, _native.ptr(self._inst_dev),
+                         self.capacity, _native.ptr(table), live, 0, stream_handle(self.device))
 
     def _clear_counters(self, which):
         if self._recording is not None:
@@ -301,6 +366,9 @@ class BatchedFSI:
             self._project(MB)
             self._launch(_native.BATCH_OP_LBM_MB_BEGIN, MB)
             self._launch(_native.BATCH_OP_LBM_REBUILD, MB)
+            for k in self.live_slots:  # their subcycles; the substep launches leave them out
+                if self._inst_host[k].resident:
+                    self._launch_resident(k)
         for t in range(self.n_substeps):
             if has_one:
                 self._project(ONE)

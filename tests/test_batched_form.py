@@ -49,6 +49,42 @@ def test_batched_equals_alone(emu, tmp_path):
         assert np.array_equal(b.lbm.solid.to_numpy(), a.lbm.solid.to_numpy())
 
 
+def test_batched_resident_subcycle_equals_alone(emu, tmp_path):
+    """A moving-boundary instance the resident subcycle covers (16 x 12 x 64)
+    runs its subcycle as one LBM_RESIDENT launch; the batch still equals each
+    instance run alone, and the same batch without that launch, bit for bit."""
+    from mpm_lbm import _native
+
+    ws = [workloads.moving_boundary_flap(shape=(16, 12, 64), steps=2, substeps=6, flap=(2, 6, 6), mpm_substeps=2,
+                                         out_dir=str(tmp_path)),
+          *_mixed(tmp_path)[:2]]
+    batch = BatchedFSI(ws, out_dir=str(tmp_path / "batch"), native=emu, device="cpu")
+    flat = BatchedFSI(ws, out_dir=str(tmp_path / "flat"), native=emu, device="cpu", resident_lbm=False)
+    assert [I.resident for I in batch._live_inst()] == [1, 0, 0]
+    assert [I.resident for I in flat._live_inst()] == [0, 0, 0]
+    alone = [_alone(w, tmp_path, emu, k) for k, w in enumerate(ws)]
+    for _ in range(2):
+        batch.step()
+        flat.step()
+        for d in alone:
+            steps = 1 if d.config.coupling_mode == "moving_boundary" else batch.n_substeps
+            for _ in range(steps):
+                d.step_once()
+    batch.check_resident_waits()
+    assert batch._maps[("resident", 0)][1] == _native.res_blocks(16, 12)
+    # the resident instance has no workgroup in the substep launches
+    planes = [v for key, v in batch._maps.items() if key[0] == _native.BATCH_DOMAIN_PLANES]
+    assert planes and all(0 not in v[0][:v[1], 0].tolist() for v in planes)
+    for k in range(3):
+        for other in (flat.drivers[k], alone[k]):
+            b = batch.drivers[k]
+            for name, fb, fo in (("F", b.lbm.F, other.lbm.F), ("rho", b.lbm.rho, other.lbm.rho),
+                                 ("v", b.lbm.v, other.lbm.v), ("hydro", b.lbm.hydro_force, other.lbm.hydro_force),
+                                 ("x", b.solid.x, other.solid.x), ("pv", b.solid.v, other.solid.v)):
+                assert rel_max_err(fb.to_numpy(), fo.to_numpy()) == 0.0, (k, name)
+            assert np.array_equal(b.lbm.solid.to_numpy(), other.lbm.solid.to_numpy())
+
+
 def _expected_map(inst, domain, modes, min_mpm):
     """Block map restated in Python: (instance, workgroup) pairs, instances in index order."""
     from mpm_lbm import _native

@@ -115,8 +115,13 @@ def test_gpu_c4_full_sweep_one_exchange(hip, tmp_path):
     batch = BatchedFSI(ws, out_dir=str(tmp_path / "batch"), native=hip)
     print("C4 instances", len(ws), "cells", batch.total_cells, "subset", [(k, sizes[k], modes[k]) for k in subset],
           flush=True)
+    # the 64^3 moving-boundary instances run their subcycle as LBM_RESIDENT launches
+    resident = [k for k in range(len(ws)) if batch._inst_host[k].resident]
+    assert resident and all(sizes[k] == 64 and modes[k] == "moving_boundary" for k in resident)
+    assert any(k in resident for k in subset)
     oracles = {k: _oracles(ws[k], batch.drivers[k]) for k in subset}
     batch.step()
+    batch.check_resident_waits()
     for d in batch.drivers:
         _finite_and_sane(d)
     for k in subset:
