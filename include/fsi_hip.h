@@ -294,7 +294,7 @@ typedef struct FsiBatchInstance {
 #define FSI_BATCH_OP_MPM_REACTION 11
 #define FSI_BATCH_OP_MPM_GRID_UPDATE 12
 #define FSI_BATCH_OP_MPM_G2P 13
-#define FSI_BATCH_OP_COUNTERS_CLEAR 14 /* zero the coupling counters (one work item per instance) */
+#define FSI_BATCH_OP_COUNTERS_CLEAR 14 /* zero counters `t` of the map's instances (INSTANCES domain) */
 #define FSI_BATCH_OP_MPM_P2G_REACT 15  /* P2G + the instance's reaction in one particle pass (as
                                           fsi_mpm_substep_coupled), records into react_ws */
 #define FSI_BATCH_OP_MPM_GRID_UPDATE_REACT 16 /* grid update; folds react_ws into the counters */
@@ -336,12 +336,12 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const
                      int n_blocks, int t, void* stream);
 
 /* A batched exchange as a HIP graph built from its launch list (no stream
- * capture): one kernel node per operation, in order.  An operation is what
- * fsi_batch_launch takes (n_blocks > 0; COUNTERS_CLEAR ignores block_map and
- * n_blocks).  When instances come and go, fsi_batch_graph_update rewrites the
- * block maps and workgroup counts of the instantiated graph in place
- * (hipGraphExecKernelNodeSetParams) for a launch list of the same operations
- * and substep indices; it fails (FSI_E_INVALID) on any other list.  The
+ * capture): one kernel node per operation, on the chain its `branch` names.
+ * An operation is what fsi_batch_launch takes (n_blocks > 0).  When
+ * instances come and go, fsi_batch_graph_update rewrites the block maps and
+ * workgroup counts of the instantiated graph in place
+ * (hipGraphExecKernelNodeSetParams) for a launch list of the same operations,
+ * substep indices and branches; it fails (FSI_E_INVALID) on any other list.  The
  * instance table `inst_dev` and its slot count are fixed for the graph's
  * life: rewrite the table's contents, not its address. */
 typedef struct FsiBatchOp {
@@ -349,8 +349,18 @@ typedef struct FsiBatchOp {
   int32_t t;
   const int32_t* block_map; /* device */
   int32_t n_blocks;
-  int32_t pad0;
+  int32_t branch; /* FSI_BATCH_BRANCH_*; fsi_batch_launch and the serial emulation run the list in order */
 } FsiBatchOp;
+
+/* Graph dependencies: a MAIN node follows the previous MAIN (or JOIN) node; a
+ * SIDE node follows the previous SIDE node, the first one of a side chain the
+ * main node before it; a JOIN node follows both chains' last nodes and ends
+ * the side chain.  The two chains may run at the same time: the list puts on
+ * the side chain only operations on other instances than the main chain's
+ * until the join, and the list order is a valid serial order. */
+#define FSI_BATCH_BRANCH_MAIN 0
+#define FSI_BATCH_BRANCH_SIDE 1
+#define FSI_BATCH_BRANCH_JOIN 2
 
 int fsi_batch_graph_create(const FsiBatchOp* ops, int n_ops, const FsiBatchInstance* inst_dev, int n_inst,
                            void** graph_out);

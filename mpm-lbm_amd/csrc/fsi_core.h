@@ -1164,6 +1164,30 @@ constexpr uint32_t BATCH_RESIDENT_FLAGS =
 
 // Why an active batched instance marked resident cannot take LBM_RESIDENT
 // (nullptr: it can, up to the device's capacity, which the launch checks).
+// COUNTERS_CLEAR of one instance: `which` 0 projection, 1 penalty, 2
+// reaction coupling counters, 3 the lattice's bounce-back counters.
+FSI_HD void batch_clear_counters(const FsiBatchInstance& I, int which) {
+  FsiCouplingCounters* cc = I.cc;
+  if (which == 0) {
+    cc->projected_mass = cc->projected_volume_raw = cc->projected_volume_clamped = 0.0;
+    cc->max_phi_raw_bits = 0;
+    cc->active_cell_count = 0;
+  } else if (which == 1) {
+    cc->active_force_cell_count = 0;
+    cc->max_cell_force_bits = cc->max_hydro_force_bits = 0;
+    for (int d = 0; d < 3; ++d) cc->net_cell_force[d] = cc->net_hydro_force[d] = 0.0;
+  } else if (which == 2) {
+    cc->active_reaction_particle_count = 0;
+    cc->max_particle_reaction_bits = cc->max_grid_reaction_bits = 0;
+    for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] = cc->net_grid_reaction[d] = 0.0;
+  } else {
+    FsiLbmCounters* k = I.ls.counters;
+    k->bb_link_count = 0;
+    k->bb_max_correction_bits = 0;
+    for (int d = 0; d < 3; ++d) k->bb_net_solid_force[d] = 0.0;
+  }
+}
+
 inline const char* batch_resident_invalid(const FsiBatchInstance& I) {
   if (!I.resident) return nullptr;
   if (I.mode != FSI_BATCH_MODE_MOVING_BOUNDARY || I.cell_force)

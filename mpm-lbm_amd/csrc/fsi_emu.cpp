@@ -635,32 +635,11 @@ static void grow_box(int32_t* b, const float* x, long long P, long long q, float
 // Serial emulation of fsi_batch_launch: instance by instance (the instances
 // named in the block map, in index order), work item by work item.
 int fsi_batch_launch(int op, const FsiBatchInstance* inst, int n_inst, const int32_t* map, int n_blocks, int t, void*) {
-  if (op == FSI_BATCH_OP_COUNTERS_CLEAR) {
-    for (int i = 0; i < n_inst; ++i) {
-      if (!inst[i].active) continue;
-      FsiCouplingCounters* cc = inst[i].cc;
-      if (t == 0) {
-        cc->projected_mass = cc->projected_volume_raw = cc->projected_volume_clamped = 0.0;
-        cc->max_phi_raw_bits = 0;
-        cc->active_cell_count = 0;
-      } else if (t == 1) {
-        cc->active_force_cell_count = 0;
-        cc->max_cell_force_bits = cc->max_hydro_force_bits = 0;
-        for (int d = 0; d < 3; ++d) cc->net_cell_force[d] = cc->net_hydro_force[d] = 0.0;
-      } else if (t == 2) {
-        cc->active_reaction_particle_count = 0;
-        cc->max_particle_reaction_bits = cc->max_grid_reaction_bits = 0;
-        for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] = cc->net_grid_reaction[d] = 0.0;
-      } else {
-        FsiLbmCounters* k = inst[i].ls.counters;
-        k->bb_link_count = 0;
-        k->bb_max_correction_bits = 0;
-        for (int d = 0; d < 3; ++d) k->bb_net_solid_force[d] = 0.0;
-      }
-    }
+  if (!map) return fail(FSI_E_INVALID, "null block map");
+  if (op == FSI_BATCH_OP_COUNTERS_CLEAR) {  // the instances of the map (INSTANCES domain)
+    for (int b = 0; b < n_blocks; ++b) batch_clear_counters(inst[map[2 * b]], t);
     return FSI_OK;
   }
-  if (!map) return fail(FSI_E_INVALID, "null block map");
   if (op == FSI_BATCH_OP_LBM_RESIDENT) {  // the one instance of the map, as fsi_lbm_subcycle_resident
     if (n_blocks <= 0) return FSI_OK;
     const FsiBatchInstance& I = inst[map[0]];
@@ -877,9 +856,12 @@ int check_ops(const FsiBatchOp* ops, int n_ops, const EmuBatchGraph* g) {
   for (int i = 0; i < n_ops; ++i) {
     if (ops[i].op < FSI_BATCH_OP_LBM_SUBSTEP || ops[i].op > FSI_BATCH_OP_LBM_RESIDENT)
       return fail(FSI_E_INVALID, "unknown batch operation");
-    if (ops[i].op != FSI_BATCH_OP_COUNTERS_CLEAR && (!ops[i].block_map || ops[i].n_blocks <= 0))
+    if (!ops[i].block_map || ops[i].n_blocks <= 0)
       return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");
-    if (g && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t))
+    if (ops[i].branch < FSI_BATCH_BRANCH_MAIN || ops[i].branch > FSI_BATCH_BRANCH_JOIN)
+      return fail(FSI_E_INVALID, "unknown batch graph branch");
+    if (g && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t ||
+              ops[i].branch != g->ops[(size_t)i].branch))
       return fail(FSI_E_INVALID, "batch graph update with other operations");
   }
   return FSI_OK;

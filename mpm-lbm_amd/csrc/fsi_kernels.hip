@@ -2252,29 +2252,9 @@ __global__ void __launch_bounds__(RES_THREADS, 1) kb_lbm_resident(const FsiBatch
                             (uint32_t)bmap[2 * blockIdx.x + 1]);
 }
 
-__global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* inst, int n_inst, int which) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= n_inst || !inst[i].active) return;
-  const FsiBatchInstance& I = inst[i];
-  FsiCouplingCounters* cc = I.cc;
-  if (which == 0) {
-    cc->projected_mass = cc->projected_volume_raw = cc->projected_volume_clamped = 0.0;
-    cc->max_phi_raw_bits = 0;
-    cc->active_cell_count = 0;
-  } else if (which == 1) {
-    cc->active_force_cell_count = 0;
-    cc->max_cell_force_bits = cc->max_hydro_force_bits = 0;
-    for (int d = 0; d < 3; ++d) cc->net_cell_force[d] = cc->net_hydro_force[d] = 0.0;
-  } else if (which == 2) {
-    cc->active_reaction_particle_count = 0;
-    cc->max_particle_reaction_bits = cc->max_grid_reaction_bits = 0;
-    for (int d = 0; d < 3; ++d) cc->net_particle_reaction[d] = cc->net_grid_reaction[d] = 0.0;
-  } else {
-    FsiLbmCounters* k = I.ls.counters;
-    k->bb_link_count = 0;
-    k->bb_max_correction_bits = 0;
-    k->bb_net_solid_force[0] = k->bb_net_solid_force[1] = k->bb_net_solid_force[2] = 0.0;
-  }
+// One workgroup per instance of the map (INSTANCES domain), its first lane.
+__global__ void __launch_bounds__(64) kb_counters_clear(const FsiBatchInstance* inst, const int32_t* bmap, int which) {
+  if (threadIdx.x == 0) batch_clear_counters(inst[bmap[2 * blockIdx.x]], which);
 }
 
 __global__ void __launch_bounds__(TPB) kb_proj_clear(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
@@ -2564,23 +2544,18 @@ int check_resident_blocks(int op, int n_blocks) {
   return FSI_OK;
 }
 
-dim3 batch_grid(int op, int n_inst, int n_blocks) {
-  return op == FSI_BATCH_OP_COUNTERS_CLEAR ? dim3((unsigned)(n_inst + 63) / 64) : dim3((unsigned)n_blocks);
-}
-
 // Argument values of a batched kernel and the pointer array a launch takes:
-// (inst, map[, t]) or, for COUNTERS_CLEAR, (inst, n_inst, which).
+// (inst, map[, t]).
 struct BatchArgs {
   const FsiBatchInstance* inst;
   const int32_t* map;
-  int n_inst, t;
+  int t;
   void* ptrs[3];
-  BatchArgs(const FsiBatchInstance* i, int n, const int32_t* m, int t_) : inst(i), map(m), n_inst(n), t(t_) {
+  BatchArgs(const FsiBatchInstance* i, const int32_t* m, int t_) : inst(i), map(m), t(t_) {
     ptrs[0] = &inst;
     ptrs[1] = &map;
     ptrs[2] = &t;
   }
-  void counters_layout() { ptrs[1] = &n_inst; }
 };
 
 struct BatchGraph {
@@ -2599,14 +2574,29 @@ void destroy_graph(BatchGraph* g) {
   delete g;
 }
 
+// Dependencies of a graph node on `branch` (FsiBatchOp.branch), given the
+// last node of the main and the side chain so far: a main node follows the
+// main chain, a side node the side chain (or, as its first node, the main
+// chain), a join node both.
+int batch_node_deps(int branch, hipGraphNode_t main_last, hipGraphNode_t side_last, hipGraphNode_t* deps) {
+  int n = 0;
+  if (branch == FSI_BATCH_BRANCH_SIDE) {
+    if (side_last) deps[n++] = side_last;
+    else if (main_last) deps[n++] = main_last;
+    return n;
+  }
+  if (main_last) deps[n++] = main_last;
+  if (branch == FSI_BATCH_BRANCH_JOIN && side_last) deps[n++] = side_last;
+  return n;
+}
+
 // Kernel node parameters of op `o`; `a` holds the argument values (and must
 // outlive the call that takes `p`).
-void node_params(const FsiBatchOp& o, int n_inst, BatchArgs& a, hipKernelNodeParams& p) {
+void node_params(const FsiBatchOp& o, BatchArgs& a, hipKernelNodeParams& p) {
   BatchKernel k;
   batch_kernel(o.op, o.t, k);
-  if (o.op == FSI_BATCH_OP_COUNTERS_CLEAR) a.counters_layout();
   p.func = const_cast<void*>(k.fn);
-  p.gridDim = batch_grid(o.op, n_inst, o.n_blocks);
+  p.gridDim = dim3((unsigned)o.n_blocks);
   p.blockDim = dim3(k.block);
   p.sharedMemBytes = 0;
   p.kernelParams = a.ptrs;
@@ -2622,10 +2612,13 @@ int check_graph_ops(const BatchGraph* g, const FsiBatchOp* ops, int n_ops, bool 
   for (int i = 0; i < n_ops; ++i) {
     BatchKernel k;
     if (batch_kernel(ops[i].op, ops[i].t, k) != FSI_OK) return FSI_E_INVALID;
-    if (ops[i].op != FSI_BATCH_OP_COUNTERS_CLEAR && (!ops[i].block_map || ops[i].n_blocks <= 0))
+    if (!ops[i].block_map || ops[i].n_blocks <= 0)
       return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");
     if (const int rc = check_resident_blocks(ops[i].op, ops[i].n_blocks)) return rc;
-    if (update && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t))
+    if (ops[i].branch < FSI_BATCH_BRANCH_MAIN || ops[i].branch > FSI_BATCH_BRANCH_JOIN)
+      return fail(FSI_E_INVALID, "unknown batch graph branch");
+    if (update && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t ||
+                   ops[i].branch != g->ops[(size_t)i].branch))
       return fail(FSI_E_INVALID, "batch graph update with other operations");
   }
   return FSI_OK;
@@ -3240,13 +3233,10 @@ int fsi_batch_launch(int op, const FsiBatchInstance* inst_dev, int n_inst, const
   BatchKernel k;
   if (batch_kernel(op, t, k) != FSI_OK) return FSI_E_INVALID;
   if (const int rc = check_resident_blocks(op, n_blocks)) return rc;
-  if (op != FSI_BATCH_OP_COUNTERS_CLEAR) {
-    if (!block_map_dev) return fail(FSI_E_INVALID, "null block map");
-    if (n_blocks <= 0) return FSI_OK;
-  }
-  BatchArgs a(inst_dev, n_inst, block_map_dev, t);
-  const hipError_t e = hipLaunchKernel(k.fn, batch_grid(op, n_inst, n_blocks), dim3(k.block), a.ptrs, 0,
-                                       as_stream(stream));
+  if (!block_map_dev) return fail(FSI_E_INVALID, "null block map");
+  if (n_blocks <= 0) return FSI_OK;
+  BatchArgs a(inst_dev, block_map_dev, t);
+  const hipError_t e = hipLaunchKernel(k.fn, dim3((unsigned)n_blocks), dim3(k.block), a.ptrs, 0, as_stream(stream));
   if (e != hipSuccess) return fail(FSI_E_HIP, hipGetErrorString(e));
   return check_launch("fsi_batch_launch");
 }
@@ -3263,13 +3253,23 @@ int fsi_batch_graph_create(const FsiBatchOp* ops, int n_ops, const FsiBatchInsta
   int rc = check_graph_ops(g, ops, n_ops, false);
   hipError_t e = hipSuccess;
   if (rc == FSI_OK) e = hipGraphCreate(&g->graph, 0);
+  hipGraphNode_t main_last = nullptr, side_last = nullptr;
   for (int i = 0; rc == FSI_OK && e == hipSuccess && i < n_ops; ++i) {
     hipKernelNodeParams p;
-    BatchArgs a(inst_dev, n_inst, ops[i].block_map, ops[i].t);
-    node_params(ops[i], n_inst, a, p);
+    BatchArgs a(inst_dev, ops[i].block_map, ops[i].t);
+    node_params(ops[i], a, p);
+    hipGraphNode_t deps[2];
+    const int nd = batch_node_deps(ops[i].branch, main_last, side_last, deps);
     hipGraphNode_t node;
-    e = hipGraphAddKernelNode(&node, g->graph, i ? &g->nodes.back() : nullptr, i ? 1 : 0, &p);
-    if (e == hipSuccess) g->nodes.push_back(node);
+    e = hipGraphAddKernelNode(&node, g->graph, nd ? deps : nullptr, (size_t)nd, &p);
+    if (e != hipSuccess) break;
+    g->nodes.push_back(node);
+    if (ops[i].branch == FSI_BATCH_BRANCH_SIDE) {
+      side_last = node;
+    } else {
+      main_last = node;
+      if (ops[i].branch == FSI_BATCH_BRANCH_JOIN) side_last = nullptr;
+    }
   }
   if (rc == FSI_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
   if (rc == FSI_OK && e != hipSuccess) rc = fail(FSI_E_HIP, hipGetErrorString(e));
@@ -3290,8 +3290,8 @@ int fsi_batch_graph_update(void* graph, const FsiBatchOp* ops, int n_ops) {
     const FsiBatchOp& o = g->ops[(size_t)i];
     if (o.block_map == ops[i].block_map && o.n_blocks == ops[i].n_blocks) continue;
     hipKernelNodeParams p;
-    BatchArgs a(g->inst, g->n_inst, ops[i].block_map, ops[i].t);
-    node_params(ops[i], g->n_inst, a, p);
+    BatchArgs a(g->inst, ops[i].block_map, ops[i].t);
+    node_params(ops[i], a, p);
     const hipError_t e = hipGraphExecKernelNodeSetParams(g->exec, g->nodes[(size_t)i], &p);
     if (e != hipSuccess) return fail(FSI_E_HIP, hipGetErrorString(e));
     g->ops[(size_t)i] = ops[i];

@@ -209,6 +209,7 @@ BATCH_OP_DOMAIN = {
     BATCH_OP_MPM_REACTION: BATCH_DOMAIN_PARTICLES, BATCH_OP_MPM_P2G_REACT: BATCH_DOMAIN_PARTICLES,
     BATCH_OP_MPM_G2P: BATCH_DOMAIN_PARTICLES,
     BATCH_OP_LBM_MB_FINALIZE_FOLD: BATCH_DOMAIN_INSTANCES, BATCH_OP_BOX_ROLL: BATCH_DOMAIN_INSTANCES,
+    BATCH_OP_COUNTERS_CLEAR: BATCH_DOMAIN_INSTANCES,
 }
 
 
@@ -229,7 +230,11 @@ class Bins(ctypes.Structure):
 
 class BatchOp(ctypes.Structure):
     """FsiBatchOp: one operation of a batched exchange graph."""
-    _fields_ = [("op", i32), ("t", i32), ("block_map", vp), ("n_blocks", i32), ("pad0", i32)]
+    _fields_ = [("op", i32), ("t", i32), ("block_map", vp), ("n_blocks", i32), ("branch", i32)]
+
+
+# FsiBatchOp.branch: graph chains of a batched exchange
+BATCH_BRANCH_MAIN, BATCH_BRANCH_SIDE, BATCH_BRANCH_JOIN = 0, 1, 2
 
 
 ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, CouplingParams,

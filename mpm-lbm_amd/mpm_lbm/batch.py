@@ -21,6 +21,20 @@ in place (fsi_batch_graph_update) as long as the launch plan - which
 operations run, over which instance groups - is the same; one graph is kept
 per launch plan.
 
+Moving-boundary instances run their whole subcycle before the particle
+passes, so their substeps need no other instance's data.  They are not put in
+the per-substep launches with the penalty instances: they run in groups whose
+populations fit in the device's last-level cache (`subcycle_group_bytes`),
+one group after the other, each through all its substeps.  What a substep
+writes is then still in that cache when the next one reads it.  A 96^3
+lattice batched with three others costs 46 us per substep, against 34 us
+alone or with one other.  In the exchange graph these subcycles and the rest
+of the moving-boundary exchange form a side chain that runs beside the
+penalty instances' steps (`FSI_BATCH_BRANCH_SIDE`): the two touch no common
+instance, and the penalty chain's short, latency-bound launches leave the
+device room for the substeps.  The resident subcycles run before the fork,
+alone on the device, since each needs all its workgroups resident at once.
+
 Results are identical to running each instance alone: the kernels and their
 per-instance arguments are the same, only the launch is shared.
 """
@@ -41,6 +55,14 @@ TPB = 256
 # faster depends on the mix (one launch per mode compiles the mode's flags in
 # but leaves each launch a share of the chip).
 SUBSTEP_PLANS = ("split", "mixed")
+# One copy of the populations per subcycle group, at most.  On MI355X (256 MB
+# of last-level cache) a batched substep over one or two 96^3 lattices (67
+# and 134 MB) costs 34 us per lattice, over three or four 46 us.  Run one
+# after the other, groups of one 96^3 lattice and a few small ones gave the
+# shortest exchange of the 64-instance sweep (179 ms; 188 ms at 160 MiB, 192
+# ms without groups); beside the penalty chain, 48 to 96 MiB gave the same.
+SUBCYCLE_GROUP_BYTES = 72 << 20
+Q = 19  # D3Q19 populations per cell
 # plan chosen per batch signature (instance shapes, modes, storage), this process
 _TUNED = {}
 _MODE = {"none": _native.BATCH_MODE_NONE, "penalty": _native.BATCH_MODE_PENALTY,
@@ -73,7 +95,8 @@ def check_batchable(d):
 
 class BatchedFSI:
     def __init__(self, workloads=None, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True,
-                 drivers=None, n_substeps=None, substep_plan=None, capacity=None, resident_lbm=True):
+                 drivers=None, n_substeps=None, substep_plan=None, capacity=None, resident_lbm=True,
+                 subcycle_group_bytes=SUBCYCLE_GROUP_BYTES):
         """Batch `workloads` (a driver is built for each), or batch initialized
         `drivers` that already exist (they keep their state).  n_substeps: LBM
         substeps per exchange (default: the moving-boundary instances' FSI
@@ -88,7 +111,11 @@ class BatchedFSI:
         whose lattice the resident subcycle covers (FP32, legacy x faces,
         nz = 64, nx and ny multiples of 4, every block resident at once) run
         their subcycle as one LBM_RESIDENT launch each instead of inside the
-        batched substep launches."""
+        batched substep launches.  subcycle_group_bytes: the other
+        moving-boundary instances run their subcycles in groups whose
+        populations (one copy) take at most this many bytes, one group at a
+        time (an instance larger than that is a group of its own); 0 = inside
+        the per-substep launches of the penalty instances."""
         if substep_plan is not None and substep_plan not in SUBSTEP_PLANS:
             raise ValueError(f"substep_plan must be one of {SUBSTEP_PLANS} or None")
         self.substep_plan = substep_plan
@@ -128,6 +155,7 @@ class BatchedFSI:
             raise ValueError("the batched exchange needs an even number of LBM substeps")
         self.drivers = [None] * self.capacity
         self.resident_lbm = bool(resident_lbm)
+        self.subcycle_group_bytes = int(subcycle_group_bytes or 0)
         self._res = [None] * self.capacity  # (halo, sync) of resident instances
         self._cus = (torch.cuda.get_device_properties(self.device).multi_processor_count
                      if self.device.type == "cuda" else None)
@@ -148,16 +176,19 @@ class BatchedFSI:
             self._fill_slot(self.drivers.index(None), d)
         self.use_graph = bool(use_graph)
         self._recording = None
+        self._side = False  # launches recorded now go on the side chain
+        self._join = False  # the next recorded main-chain launch waits for the side chain
         self._plan_for = None
         self.exchanges = 0
 
     @classmethod
     def slots(cls, capacity, native=None, device=None, n_substeps=120, substep_plan="split", use_graph=True):
         """An empty slot table of `capacity` instances (the serving form).  Its
-        instances stay in the batched substep launches (no LBM_RESIDENT), so the
-        exchange's launch list keeps its shape as requests come and go."""
+        instances stay in the batched substep launches (no LBM_RESIDENT, no
+        subcycle groups), so the exchange's launch list keeps its shape as
+        requests come and go."""
         return cls(drivers=[], native=native, device=device, n_substeps=n_substeps, substep_plan=substep_plan,
-                   use_graph=use_graph, capacity=capacity, resident_lbm=False)
+                   use_graph=use_graph, capacity=capacity, resident_lbm=False, subcycle_group_bytes=0)
 
     # ----------------------------------------------------------------- slots
     def _fill_slot(self, k, d):
@@ -247,6 +278,29 @@ class BatchedFSI:
                 sync.zero_()
                 raise RuntimeError(f"slot {k}: {timeouts} neighbour waits of the resident subcycle timed out")
 
+    def subcycle_groups(self):
+        """The live moving-boundary slots not run resident, in subcycle groups:
+        first fit, largest lattice first, within subcycle_group_bytes of
+        populations per group."""
+        if not self.subcycle_group_bytes:
+            return []
+        items = []
+        for k in self.live_slots:
+            I = self._inst_host[k]
+            if I.mode == _native.BATCH_MODE_MOVING_BOUNDARY and not I.resident:
+                size = 4 if I.lp.storage == _native.STORE_F32 else 2
+                items.append((Q * size * I.lp.nx * I.lp.ny * I.lp.nz, k))
+        groups = []  # [bytes, slots]
+        for nbytes, k in sorted(items, key=lambda e: (-e[0], e[1])):
+            for g in groups:
+                if g[0] + nbytes <= self.subcycle_group_bytes:
+                    g[0] += nbytes
+                    g[1].append(k)
+                    break
+            else:
+                groups.append([nbytes, [k]])
+        return [sorted(g[1]) for g in groups]
+
     @property
     def live_slots(self):
         return [k for k, d in enumerate(self.drivers) if d is not None]
@@ -280,6 +334,23 @@ class BatchedFSI:
                 entry[0][:n].copy_(torch.from_numpy(host))
             entry[1] = n
             return
+        if key[0] == "group":  # the substep tiles of one subcycle group's instances
+            groups = self.subcycle_groups()
+            members = groups[key[1]] if key[1] < len(groups) else []
+            sub = (_native.BATCH_DOMAIN_PLANES, MB, 0)
+            self._write_map(sub)
+            table, n = self._maps[sub]
+            rows = table[:n]
+            host = rows[torch.isin(rows[:, 0], torch.tensor(members, dtype=torch.int32, device=rows.device))]
+            n = int(host.shape[0])
+            entry = self._maps.get(key)
+            if entry is None or entry[0].shape[0] < n:
+                entry = [torch.empty((max(n, 1), 2), dtype=torch.int32, device=self.device), 0]
+                self._maps[key] = entry
+            if n:
+                entry[0][:n].copy_(host)
+            entry[1] = n
+            return
         domain, modes, min_mpm = key
         mask = sum(1 << m for m in modes)
         addr = ctypes.addressof(self._inst_host)
@@ -311,7 +382,7 @@ class BatchedFSI:
         if live == 0:
             return
         if self._recording is not None:
-            self._recording.append((op, t, (_native.BATCH_OP_DOMAIN[op], tuple(sorted(modes)), min_mpm)))
+            self._record(op, t, (_native.BATCH_OP_DOMAIN[op], tuple(sorted(modes)), min_mpm))
             return
         self.native.call("fsi_batch_launch", op, _native.ptr(self._inst_dev), self.capacity, _native.ptr(table),
                          live, t, stream_handle(self.device))
@@ -325,17 +396,39 @@ class BatchedFSI:
         if live == 0:
             return
         if self._recording is not None:
-            self._recording.append((_native.BATCH_OP_LBM_RESIDENT, 0, key))
+            self._record(_native.BATCH_OP_LBM_RESIDENT, 0, key)
             return
         self.native.call("fsi_batch_launch", _native.BATCH_OP_LBM_RESIDENT, _native.ptr(self._inst_dev),
                          self.capacity, _native.ptr(table), live, 0, stream_handle(self.device))
 
-    def _clear_counters(self, which):
-        if self._recording is not None:
-            self._recording.append((_native.BATCH_OP_COUNTERS_CLEAR, which, None))
+    def _launch_group(self, g, t):
+        """Substep t of subcycle group g."""
+        key = ("group", g)
+        if key not in self._maps:
+            self._write_map(key)
+        table, live = self._maps[key]
+        if live == 0:
             return
-        self.native.call("fsi_batch_launch", _native.BATCH_OP_COUNTERS_CLEAR, _native.ptr(self._inst_dev),
-                         self.capacity, None, 0, which, stream_handle(self.device))
+        if self._recording is not None:
+            self._record(_native.BATCH_OP_LBM_SUBSTEP_MB, t, key)
+            return
+        self.native.call("fsi_batch_launch", _native.BATCH_OP_LBM_SUBSTEP_MB, _native.ptr(self._inst_dev),
+                         self.capacity, _native.ptr(table), live, t, stream_handle(self.device))
+
+    def _record(self, op, t, key):
+        """Append a launch to the plan being recorded, on the graph's side
+        chain (while `_side` is set) or its main chain."""
+        if self._side:
+            branch = _native.BATCH_BRANCH_SIDE
+        elif self._join:
+            branch, self._join = _native.BATCH_BRANCH_JOIN, False
+        else:
+            branch = _native.BATCH_BRANCH_MAIN
+        self._recording.append((op, t, key, branch))
+
+    def _clear_counters(self, which, modes):
+        """Counters `which` (0 projection, 1 penalty) of the instances of `modes`."""
+        self._launch(_native.BATCH_OP_COUNTERS_CLEAR, modes, t=which)
 
     def _live_inst(self):
         return [self._inst_host[k] for k in self.live_slots]
@@ -351,7 +444,7 @@ class BatchedFSI:
             self._launch(_native.BATCH_OP_MPM_G2P, modes, min_mpm=j)
 
     def _project(self, modes):
-        self._clear_counters(0)
+        self._clear_counters(0, modes)
         self._launch(_native.BATCH_OP_BOX_ROLL, modes, t=0)
         self._launch(_native.BATCH_OP_PROJ_CLEAR, modes)
         self._launch(_native.BATCH_OP_PROJ_PARTICLES, modes)
@@ -362,31 +455,51 @@ class BatchedFSI:
         live = self._live_inst()
         has_mb = any(I.mode in MB for I in live)
         has_one = any(I.mode in ONE for I in live)
+        groups = self.subcycle_groups() if has_mb else []
+        self._join = False
         if has_mb:
             self._project(MB)
             self._launch(_native.BATCH_OP_LBM_MB_BEGIN, MB)
             self._launch(_native.BATCH_OP_LBM_REBUILD, MB)
-            for k in self.live_slots:  # their subcycles; the substep launches leave them out
+            # resident subcycles alone on the device: each needs all its workgroups at once
+            for k in self.live_slots:  # the substep launches leave them out
                 if self._inst_host[k].resident:
                     self._launch_resident(k)
+            if groups:
+                # the rest of the moving-boundary exchange on a side chain of the
+                # graph, beside the penalty instances' steps (other instances);
+                # the launch after those joins it
+                self._side = True
+                for g in range(len(groups)):
+                    for t in range(self.n_substeps):
+                        self._launch_group(g, t)
+                self._mb_exchange_end()
+                self._side = False
+        per_substep = ONE if self.subcycle_group_bytes else tuple(_MODE.values())
         for t in range(self.n_substeps):
             if has_one:
                 self._project(ONE)
-                self._clear_counters(1)
+                self._clear_counters(1, PEN)
                 self._launch(_native.BATCH_OP_PENALTY, PEN, t=t)  # stores the moments it reads (t >= 1)
-            self._issue_substep(t)
+            self._issue_substep(t, per_substep)
             if has_one:
                 self._mpm_substeps(ONE)
+        self._join = bool(groups)
         if has_one:
             # no moments pass between the steps (batch_moment_source): one after the last
             self._launch(_native.BATCH_OP_LBM_MOMENTS, ONE, t=self.n_substeps - 1)
-        if has_mb:
-            self._launch(_native.BATCH_OP_LBM_MB_FINALIZE, MB)
-            self._launch(_native.BATCH_OP_LBM_MB_FINALIZE_FOLD, MB)
-            self._launch(_native.BATCH_OP_LBM_MOMENTS, MB, t=self.n_substeps - 1)
-            self._mpm_substeps(MB)
+        if has_mb and not groups:
+            self._mb_exchange_end()
 
-    def _issue_substep(self, t):
+    def _mb_exchange_end(self):
+        """After the moving-boundary subcycles: wall force, moments, MPM substeps."""
+        self._launch(_native.BATCH_OP_LBM_MB_FINALIZE, MB)
+        self._launch(_native.BATCH_OP_LBM_MB_FINALIZE_FOLD, MB)
+        self._launch(_native.BATCH_OP_LBM_MOMENTS, MB, t=self.n_substeps - 1)
+        self._mpm_substeps(MB)
+
+    def _issue_substep(self, t, modes):
+        """The batched substep t of the instances of `modes`."""
         plan = self.substep_plan
         timer = None
         if plan is None:
@@ -397,9 +510,10 @@ class BatchedFSI:
                 timer = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 timer[0].record()
         if plan == "mixed":
-            self._launch(_native.BATCH_OP_LBM_SUBSTEP, tuple(_MODE.values()), t=t)
+            self._launch(_native.BATCH_OP_LBM_SUBSTEP, modes, t=t)
         else:
-            self._launch(_native.BATCH_OP_LBM_SUBSTEP_MB, (_native.BATCH_MODE_MOVING_BOUNDARY,), t=t)
+            if _native.BATCH_MODE_MOVING_BOUNDARY in modes:
+                self._launch(_native.BATCH_OP_LBM_SUBSTEP_MB, (_native.BATCH_MODE_MOVING_BOUNDARY,), t=t)
             self._launch(_native.BATCH_OP_LBM_SUBSTEP_PENALTY, (_native.BATCH_MODE_PENALTY,), t=t)
             self._launch(_native.BATCH_OP_LBM_SUBSTEP_NONE, (_native.BATCH_MODE_NONE,), t=t)
         if timer is not None:
@@ -420,8 +534,8 @@ class BatchedFSI:
     def _launch_list(self):
         """The plan's launches with the current block maps (FsiBatchOp array)."""
         ops = (_native.BatchOp * len(self._plan))()
-        for o, (op, t, key) in zip(ops, self._plan):
-            o.op, o.t = op, t
+        for o, (op, t, key, branch) in zip(ops, self._plan):
+            o.op, o.t, o.branch = op, t, branch
             if key is not None:
                 table, live = self._maps[key]
                 o.block_map, o.n_blocks = _native.ptr(table), live
@@ -465,8 +579,9 @@ class BatchedFSI:
     # --------------------------------------------------------------- tuning
     def _signature(self):
         """What the substep timings depend on: per instance its lattice, mode,
-        storage, collision, particle count and MPM substeps, and the exchange length."""
-        return (self.n_substeps,) + tuple(
+        storage, collision, particle count and MPM substeps, the exchange length
+        and which instances the per-substep launches hold."""
+        return (self.n_substeps, self.resident_lbm, self.subcycle_group_bytes) + tuple(
             (I.lp.nx, I.lp.ny, I.lp.nz, int(I.mode), int(I.lp.storage), int(I.lp.collision), int(I.mp.n_particles),
              int(I.mpm_substeps)) for I in self._live_inst())
 

@@ -2,8 +2,9 @@
 and modes gives the same results as running each instance alone (checked on
 the serial emulation of the kernels; the GPU version is in test_gpu_batched)."""
 import numpy as np
+import pytest
 
-from mpm_lbm import workloads
+from mpm_lbm import _native, workloads
 from mpm_lbm.batch import BatchedFSI
 from mpm_lbm.sim.drivers.fsi_driver import FSIDriver3D
 from tests.parity import rel_max_err
@@ -47,6 +48,12 @@ def test_batched_equals_alone(emu, tmp_path):
             err = rel_max_err(fb.to_numpy(), fa.to_numpy())
             assert err == 0.0, (k, name, err)  # same kernels, same order: bit-identical
         assert np.array_equal(b.lbm.solid.to_numpy(), a.lbm.solid.to_numpy())
+        # the projection counters of the last projection: a moving-boundary
+        # instance's survive the penalty instances' projections after it
+        got, want = batch.coupling_stats(k), a.projector._counters()
+        assert got.active_cell_count == want.active_cell_count > 0, k
+        assert got.max_phi_raw_bits == want.max_phi_raw_bits, k
+        assert got.projected_mass == pytest.approx(want.projected_mass, rel=1e-12), k
 
 
 def test_batched_resident_subcycle_equals_alone(emu, tmp_path):
@@ -135,8 +142,6 @@ def test_substep_launch_plans_agree(emu, tmp_path):
     """The two launch plans of the batched substep (one launch per coupling
     mode, flags compiled in; or one launch over every instance, flags read
     per instance) give bit-identical instances; an unknown plan is refused."""
-    import pytest
-
     ws = _mixed(tmp_path)
     runs = {}
     for plan in ("split", "mixed"):
@@ -207,10 +212,6 @@ def test_batch_graph_rejects_other_launch_lists(emu, tmp_path):
     workgroups is refused at build."""
     import ctypes
 
-    import pytest
-
-    from mpm_lbm import _native
-
     ws = _mixed(tmp_path)[:2]
     batch = BatchedFSI(ws, out_dir=str(tmp_path / "g"), native=emu, device="cpu", substep_plan="split")
     batch.step()
@@ -228,3 +229,55 @@ def test_batch_graph_rejects_other_launch_lists(emu, tmp_path):
     with pytest.raises(_native.FsiError, match="without a block map"):
         emu.call("fsi_batch_graph_create", empty, 1, _native.ptr(batch._inst_dev), batch.capacity, ctypes.byref(h))
     batch.close()
+
+
+def test_subcycle_groups_equal_alone(emu, tmp_path):
+    """Moving-boundary instances run their subcycles in groups (here one
+    instance per group, and all in one group) outside the per-substep
+    launches of the penalty instances; every instance still equals the same
+    batch with the moving-boundary instances inside those launches, bit for bit."""
+    ws = _mixed(tmp_path)
+    runs = {}
+    for name, budget in (("own", 1), ("one", 1 << 30), ("flat", 0)):
+        b = BatchedFSI(ws, out_dir=str(tmp_path / name), native=emu, device="cpu", subcycle_group_bytes=budget)
+        b.step()
+        b.step()
+        runs[name] = b
+    assert runs["own"].subcycle_groups() == [[0], [2]]
+    assert runs["one"].subcycle_groups() == [[0, 2]]
+    assert runs["flat"].subcycle_groups() == []
+    # the grouped batches' per-substep launches hold the penalty instances alone
+    for name in ("own", "one"):
+        plan = runs[name]._plan
+        mb = _native.BATCH_MODE_MOVING_BOUNDARY
+        assert not [key for _, _, key, _ in plan if key[0] == _native.BATCH_DOMAIN_PLANES and mb in key[1]]
+        # the groups' substeps and the rest of the moving-boundary exchange form
+        # the graph's side chain (moving-boundary instances only), joined by one
+        # later launch
+        side = [i for i, (op, _, key, br) in enumerate(plan) if br == _native.BATCH_BRANCH_SIDE]
+        groups = [i for i in side if plan[i][2][0] == "group"]
+        assert {plan[i][0] for i in groups} == {_native.BATCH_OP_LBM_SUBSTEP_MB}
+        assert len(groups) == len(runs[name].subcycle_groups()) * runs[name].n_substeps == side.index(groups[-1]) + 1
+        assert all(plan[i][2][0] == "group" or plan[i][2][1] == (mb,) for i in side)
+        assert _native.BATCH_OP_MPM_G2P in {plan[i][0] for i in side}
+        joins = [i for i, (_, _, _, br) in enumerate(plan) if br == _native.BATCH_BRANCH_JOIN]
+        assert len(joins) == 1 and joins[0] > side[-1]
+        assert plan[joins[0]][0] == _native.BATCH_OP_LBM_MOMENTS  # the penalty instances' last moments
+    assert {br for *_, br in runs["flat"]._plan} == {_native.BATCH_BRANCH_MAIN}
+    for k in range(4):
+        for name in ("own", "one"):
+            a, b = runs["flat"].drivers[k], runs[name].drivers[k]
+            for field in ("F", "rho", "v", "hydro_force"):
+                assert np.array_equal(getattr(a.lbm, field).to_numpy(), getattr(b.lbm, field).to_numpy()), (k, field)
+            for field in ("x", "v", "F", "C"):
+                assert np.array_equal(getattr(a.solid, field).to_numpy(), getattr(b.solid, field).to_numpy()), (k, field)
+
+
+def test_subcycle_groups_first_fit(emu, tmp_path):
+    """Groups are packed first fit, largest lattice first, within the budget."""
+    ws = _mixed(tmp_path)
+    b = BatchedFSI(ws, out_dir=str(tmp_path / "g"), native=emu, device="cpu",
+                   subcycle_group_bytes=19 * 4 * 16 ** 3 + 19 * 4 * 12 ** 3 - 1)
+    assert b.subcycle_groups() == [[0], [2]]
+    b.subcycle_group_bytes = 19 * 4 * (16 ** 3 + 12 ** 3)
+    assert b.subcycle_groups() == [[0, 2]]

@@ -5,7 +5,7 @@ against its own oracle run to the single-instance parity bar."""
 import numpy as np
 import pytest
 
-from mpm_lbm import workloads
+from mpm_lbm import _native, workloads
 from mpm_lbm.batch import BatchedFSI
 from oracle.oracle import OracleFSI
 from tests.parity import check_history, field_errors
@@ -43,6 +43,11 @@ def test_gpu_batched_matches_oracle(hip, tmp_path):
 
     print("substep plan tuning", batch.tuning, flush=True)
     assert batch.substep_plan in SUBSTEP_PLANS and batch.tuning["plan"] == batch.substep_plan
+    # the replayed graphs ran the moving-boundary subcycles on a side chain
+    # beside the penalty instances' steps
+    assert batch.subcycle_groups() == [[0, 2]] and batch.replays + batch.graph_builds == n_exchanges - 1
+    assert {br for *_, br in batch._plan} == {_native.BATCH_BRANCH_MAIN, _native.BATCH_BRANCH_SIDE,
+                                             _native.BATCH_BRANCH_JOIN}
     for k, (d, o) in enumerate(zip(batch.drivers, oracles)):
         steps = n_exchanges if d.config.coupling_mode == "moving_boundary" else n_exchanges * batch.n_substeps
         for _ in range(steps):

@@ -15,7 +15,7 @@
 import numpy as np
 import pytest
 
-from mpm_lbm import workloads
+from mpm_lbm import _native, workloads
 from oracle.oracle import OracleFSI
 from tests.parity import check_history, driver_fields, field_errors, fields_errors
 
@@ -112,15 +112,21 @@ def test_gpu_c4_full_sweep_one_exchange(hip, tmp_path):
         seen.add(key)
         subset.append(k)
     assert any(sizes[k] == 96 and modes[k] == "moving_boundary" for k in subset)
-    batch = BatchedFSI(ws, out_dir=str(tmp_path / "batch"), native=hip)
+    # a fixed substep plan: the one exchange runs as the graph (no eager tuning exchange)
+    batch = BatchedFSI(ws, out_dir=str(tmp_path / "batch"), native=hip, substep_plan="split")
     print("C4 instances", len(ws), "cells", batch.total_cells, "subset", [(k, sizes[k], modes[k]) for k in subset],
           flush=True)
     # the 64^3 moving-boundary instances run their subcycle as LBM_RESIDENT launches
     resident = [k for k in range(len(ws)) if batch._inst_host[k].resident]
     assert resident and all(sizes[k] == 64 and modes[k] == "moving_boundary" for k in resident)
     assert any(k in resident for k in subset)
+    # the other moving-boundary instances in subcycle groups, on the graph's side chain
+    grouped = sorted(k for g in batch.subcycle_groups() for k in g)
+    assert grouped == [k for k in range(len(ws)) if modes[k] == "moving_boundary" and k not in resident]
+    assert any(k in grouped for k in subset)
     oracles = {k: _oracles(ws[k], batch.drivers[k]) for k in subset}
     batch.step()
+    assert batch.graph_builds == 1 and _native.BATCH_BRANCH_SIDE in {br for *_, br in batch._plan}
     batch.check_resident_waits()
     for d in batch.drivers:
         _finite_and_sane(d)
