@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--push-traffic-json", default=str(ROOT / "profiles" / "lbm_substep_64cube_traffic.json"))
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched sweep (0 = skip)")
     ap.add_argument("--batch-exchanges", type=int, default=3, help="timed batched exchanges")
+    ap.add_argument("--batch-f16", type=int, default=1,
+                    help="also time the batched sweep with FP16-deviation storage (0 = skip)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window of the served form (0 = skip)")
     ap.add_argument("--repeats", type=int, default=5,
                     help="timed blocks the K timed steps are split into; value = their median")
@@ -161,23 +163,31 @@ def _timed(torch, fn):
     return s.elapsed_time(e) / 1e3
 
 
-def batched_form(torch, count, exchanges):
+def _batched_exchanges(torch, ws, out_dir, exchanges):
+    """A BatchedFSI of `ws` after its tuning and graph-building exchanges,
+    and the seconds of `exchanges` timed exchanges."""
+    from mpm_lbm.batch import BatchedFSI
+
+    t0 = time.perf_counter()
+    b = BatchedFSI(ws, out_dir=out_dir)
+    setup = time.perf_counter() - t0
+    b.step()  # eager: builds the block tables, tunes the substep plan
+    b.step()  # builds the exchange graph and runs it
+    torch.cuda.synchronize()
+    return b, setup, _timed(torch, lambda: [b.step() for _ in range(exchanges)])
+
+
+def batched_form(torch, count, exchanges, f16=True):
     """Batched sweep (seeded mix of 32/48/64/96^3 moving-boundary and penalty
     flap instances): aggregate MLUPS of one-launch-per-operation exchanges
     replayed as a HIP graph, against the same instances run one at a time
     through FSIDriver3D (graph-captured FSI step for moving-boundary instances,
-    120 one-LBM-step FSI steps for penalty instances)."""
+    120 one-LBM-step FSI steps for penalty instances).  With `f16`, the same
+    sweep with FP16-deviation storage is timed too (reported beside it)."""
     from mpm_lbm import workloads
-    from mpm_lbm.batch import BatchedFSI
 
     ws = workloads.batch_sweep(count=count, seed=4, out_dir="/tmp/fsi_bench_batch")
-    t0 = time.perf_counter()
-    b = BatchedFSI(ws, out_dir="/tmp/fsi_bench_batch/run")
-    setup = time.perf_counter() - t0
-    b.step()  # eager: builds the block tables
-    b.step()  # captures the exchange graph and replays it
-    torch.cuda.synchronize()
-    secs = _timed(torch, lambda: [b.step() for _ in range(exchanges)])
+    b, setup, secs = _batched_exchanges(torch, ws, "/tmp/fsi_bench_batch/run", exchanges)
     cells = b.total_cells
     batched = cells * b.n_substeps * exchanges / secs / 1e6
     log(f"[bench] batched: {count} instances, {cells} cells, {exchanges} exchanges in {secs * 1e3:.1f} ms "
@@ -197,17 +207,35 @@ def batched_form(torch, count, exchanges):
     plain = cells * b.n_substeps / plain_s / 1e6
     log(f"[bench] plain (one instance at a time): {plain_s * 1e3:.1f} ms per exchange -> {plain:.1f} MLUPS")
     modes = [d.config.coupling_mode for d in b.drivers]
+    groups, tuning, n_sub = len(b.subcycle_groups()), b.tuning, b.n_substeps
+    b.close()
+    del b
+    half = None
+    if f16:
+        ws16 = workloads.batch_sweep(count=count, seed=4, out_dir="/tmp/fsi_bench_batch16", storage="f16")
+        b16, _, secs16 = _batched_exchanges(torch, ws16, "/tmp/fsi_bench_batch16/run", exchanges)
+        rate16 = cells * b16.n_substeps * exchanges / secs16 / 1e6
+        half = {"batched_mlups": round(rate16, 1), "ms_per_exchange": round(secs16 * 1e3 / exchanges, 3),
+                "speedup_vs_plain_f32": round(rate16 / plain, 3), "subcycle_groups": len(b16.subcycle_groups()),
+                "substep_plan": b16.tuning["plan"] if b16.tuning else b16.substep_plan,
+                "error": "FP16-deviation storage: the precision form's bounds against FP64 "
+                         "(tests/test_gpu_batched.py::test_gpu_batched_fp16_storage_vs_fp64)"}
+        log(f"[bench] batched FP16 storage: {secs16 * 1e3 / exchanges:.1f} ms per exchange -> {rate16:.1f} MLUPS")
+        b16.close()
+        del b16
     return {
         "instances": count,
         "moving_boundary": modes.count("moving_boundary"),
         "penalty": modes.count("penalty"),
         "total_cells": cells,
-        "substeps_per_exchange": b.n_substeps,
+        "substeps_per_exchange": n_sub,
         "batched_mlups": round(batched, 1),
         "plain_mlups": round(plain, 1),
         "speedup_vs_plain": round(batched / plain, 3),
         "ms_per_exchange": round(secs * 1e3 / exchanges, 3),
-        "substep_plan_tuning": b.tuning,
+        "substep_plan_tuning": tuning,
+        "subcycle_groups": groups,
+        "f16": half,
         "equal_error": "same FP32 kernels and per-instance arguments; batched runs checked against the oracle at "
                        "the plain path's bar (tests/test_gpu_batched.py)",
     }
@@ -430,7 +458,7 @@ def main():
 
     batched = None
     if args.batch > 0:
-        batched = batched_form(torch, args.batch, args.batch_exchanges)
+        batched = batched_form(torch, args.batch, args.batch_exchanges, f16=args.batch_f16 > 0)
 
     served = None
     if args.serve_seconds > 0:

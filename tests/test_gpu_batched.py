@@ -14,16 +14,15 @@ pytestmark = pytest.mark.gpu
 TOL = 1.0e-5
 
 
-def _mixed(tmp_path):
+def _mixed(tmp_path, storage="f32"):
+    kw = {"out_dir": str(tmp_path), "storage": storage}
     return [
-        workloads.flap_case("moving_boundary", 24, substeps=8, thickness=2, seed=1, mpm_substeps=2,
-                            out_dir=str(tmp_path)),
-        workloads.flap_case("penalty", 16, substeps=8, thickness=2, seed=2, out_dir=str(tmp_path)),
-        workloads.flap_case("moving_boundary", 16, substeps=8, thickness=4, seed=3, mpm_substeps=3, E_mod=3e4,
-                            out_dir=str(tmp_path)),
-        workloads.flap_case("penalty", 20, substeps=8, thickness=4, seed=4, E_mod=2e5, out_dir=str(tmp_path)),
+        workloads.flap_case("moving_boundary", 24, substeps=8, thickness=2, seed=1, mpm_substeps=2, **kw),
+        workloads.flap_case("penalty", 16, substeps=8, thickness=2, seed=2, **kw),
+        workloads.flap_case("moving_boundary", 16, substeps=8, thickness=4, seed=3, mpm_substeps=3, E_mod=3e4, **kw),
+        workloads.flap_case("penalty", 20, substeps=8, thickness=4, seed=4, E_mod=2e5, **kw),
         # coupling "none": the split plan's NONE launch
-        workloads.flap_case("none", 12, substeps=8, thickness=2, seed=5, out_dir=str(tmp_path)),
+        workloads.flap_case("none", 12, substeps=8, thickness=2, seed=5, **kw),
     ]
 
 
@@ -56,6 +55,33 @@ def test_gpu_batched_matches_oracle(hip, tmp_path):
         errs = field_errors(d, o)
         print("batched instance", k, d.config.coupling_mode, errs, flush=True)
         check_history([errs], f"batched instance {k} ({d.config.coupling_mode})", TOL)
+
+
+def test_gpu_batched_fp16_storage_vs_fp64(hip, tmp_path):
+    """The mixed batch with FP16-deviation storage (the precision form's
+    storage inside the batched launches and the graph's side chain): each
+    instance against its own FP64 oracle run, to the FP16 bounds of the
+    precision form."""
+    from tests.test_precision_forms import BOUNDS
+
+    ws = _mixed(tmp_path, storage="f16")
+    batch = BatchedFSI(ws, out_dir=str(tmp_path / "batch16"), native=hip)
+    assert {int(I.lp.storage) for I in batch._live_inst()} == {_native.STORE_F16_DEV}
+    oracles = [OracleFSI(w.config, d.lbm.static_solid.to_numpy(), w.cloud, w.material, dtype=np.float64)
+               for w, d in zip(ws, batch.drivers)]
+    n_exchanges = 3
+    for _ in range(n_exchanges):
+        batch.step()
+    assert batch.subcycle_groups() == [[0, 2]] and batch.graph_builds == 1
+    for k, (d, o) in enumerate(zip(batch.drivers, oracles)):
+        steps = n_exchanges if d.config.coupling_mode == "moving_boundary" else n_exchanges * batch.n_substeps
+        for _ in range(steps):
+            o.step_once()
+        errs = field_errors(d, o)
+        print("batched f16 instance", k, d.config.coupling_mode, {q: errs[q] for q in BOUNDS["f16"]}, flush=True)
+        assert errs["solid_equal"], k
+        for q, bound in BOUNDS["f16"].items():
+            assert errs[q] <= bound, (k, d.config.coupling_mode, q, errs[q])
 
 
 def test_gpu_slot_table_replays_across_admissions(hip, tmp_path):
