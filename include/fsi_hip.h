@@ -349,18 +349,20 @@ typedef struct FsiBatchOp {
   int32_t t;
   const int32_t* block_map; /* device */
   int32_t n_blocks;
-  int32_t branch; /* FSI_BATCH_BRANCH_*; fsi_batch_launch and the serial emulation run the list in order */
+  int32_t branch; /* graph chain: FSI_BATCH_BRANCH_MAIN, a side chain 1..FSI_BATCH_SIDE_CHAINS, or
+                     FSI_BATCH_BRANCH_JOIN; fsi_batch_launch and the serial emulation run the list in order */
 } FsiBatchOp;
 
 /* Graph dependencies: a MAIN node follows the previous MAIN (or JOIN) node; a
- * SIDE node follows the previous SIDE node, the first one of a side chain the
- * main node before it; a JOIN node follows both chains' last nodes and ends
- * the side chain.  The two chains may run at the same time: the list puts on
- * the side chain only operations on other instances than the main chain's
- * until the join, and the list order is a valid serial order. */
+ * node of side chain k follows the previous node of chain k, the first one of
+ * a side chain the main node before it; a JOIN node follows the last nodes of
+ * the main and every open side chain and ends those side chains.  The chains
+ * may run at the same time: the list puts on each chain only operations on
+ * other instances than the other chains' until the join, and the list order is
+ * a valid serial order. */
 #define FSI_BATCH_BRANCH_MAIN 0
-#define FSI_BATCH_BRANCH_SIDE 1
-#define FSI_BATCH_BRANCH_JOIN 2
+#define FSI_BATCH_SIDE_CHAINS 3
+#define FSI_BATCH_BRANCH_JOIN (-1)
 
 int fsi_batch_graph_create(const FsiBatchOp* ops, int n_ops, const FsiBatchInstance* inst_dev, int n_inst,
                            void** graph_out);

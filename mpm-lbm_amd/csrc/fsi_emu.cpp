@@ -858,7 +858,7 @@ int check_ops(const FsiBatchOp* ops, int n_ops, const EmuBatchGraph* g) {
       return fail(FSI_E_INVALID, "unknown batch operation");
     if (!ops[i].block_map || ops[i].n_blocks <= 0)
       return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");
-    if (ops[i].branch < FSI_BATCH_BRANCH_MAIN || ops[i].branch > FSI_BATCH_BRANCH_JOIN)
+    if (ops[i].branch < FSI_BATCH_BRANCH_JOIN || ops[i].branch > FSI_BATCH_SIDE_CHAINS)
       return fail(FSI_E_INVALID, "unknown batch graph branch");
     if (g && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t ||
               ops[i].branch != g->ops[(size_t)i].branch))

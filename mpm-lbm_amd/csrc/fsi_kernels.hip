@@ -2575,18 +2575,20 @@ void destroy_graph(BatchGraph* g) {
 }
 
 // Dependencies of a graph node on `branch` (FsiBatchOp.branch), given the
-// last node of the main and the side chain so far: a main node follows the
-// main chain, a side node the side chain (or, as its first node, the main
-// chain), a join node both.
-int batch_node_deps(int branch, hipGraphNode_t main_last, hipGraphNode_t side_last, hipGraphNode_t* deps) {
+// last node of the main chain (last[0]) and of each side chain so far: a main
+// node follows the main chain, a side node its side chain (or, as its first
+// node, the main chain), a join node all of them.
+int batch_node_deps(int branch, const hipGraphNode_t* last, hipGraphNode_t* deps) {
   int n = 0;
-  if (branch == FSI_BATCH_BRANCH_SIDE) {
-    if (side_last) deps[n++] = side_last;
-    else if (main_last) deps[n++] = main_last;
+  if (branch > FSI_BATCH_BRANCH_MAIN) {
+    if (last[branch]) deps[n++] = last[branch];
+    else if (last[0]) deps[n++] = last[0];
     return n;
   }
-  if (main_last) deps[n++] = main_last;
-  if (branch == FSI_BATCH_BRANCH_JOIN && side_last) deps[n++] = side_last;
+  if (last[0]) deps[n++] = last[0];
+  if (branch == FSI_BATCH_BRANCH_JOIN)
+    for (int c = 1; c <= FSI_BATCH_SIDE_CHAINS; ++c)
+      if (last[c]) deps[n++] = last[c];
   return n;
 }
 
@@ -2615,7 +2617,7 @@ int check_graph_ops(const BatchGraph* g, const FsiBatchOp* ops, int n_ops, bool 
     if (!ops[i].block_map || ops[i].n_blocks <= 0)
       return fail(FSI_E_INVALID, "batch graph operation without a block map or workgroups");
     if (const int rc = check_resident_blocks(ops[i].op, ops[i].n_blocks)) return rc;
-    if (ops[i].branch < FSI_BATCH_BRANCH_MAIN || ops[i].branch > FSI_BATCH_BRANCH_JOIN)
+    if (ops[i].branch < FSI_BATCH_BRANCH_JOIN || ops[i].branch > FSI_BATCH_SIDE_CHAINS)
       return fail(FSI_E_INVALID, "unknown batch graph branch");
     if (update && (ops[i].op != g->ops[(size_t)i].op || ops[i].t != g->ops[(size_t)i].t ||
                    ops[i].branch != g->ops[(size_t)i].branch))
@@ -3253,22 +3255,23 @@ int fsi_batch_graph_create(const FsiBatchOp* ops, int n_ops, const FsiBatchInsta
   int rc = check_graph_ops(g, ops, n_ops, false);
   hipError_t e = hipSuccess;
   if (rc == FSI_OK) e = hipGraphCreate(&g->graph, 0);
-  hipGraphNode_t main_last = nullptr, side_last = nullptr;
+  hipGraphNode_t last[FSI_BATCH_SIDE_CHAINS + 1] = {};
   for (int i = 0; rc == FSI_OK && e == hipSuccess && i < n_ops; ++i) {
     hipKernelNodeParams p;
     BatchArgs a(inst_dev, ops[i].block_map, ops[i].t);
     node_params(ops[i], a, p);
-    hipGraphNode_t deps[2];
-    const int nd = batch_node_deps(ops[i].branch, main_last, side_last, deps);
+    hipGraphNode_t deps[FSI_BATCH_SIDE_CHAINS + 1];
+    const int nd = batch_node_deps(ops[i].branch, last, deps);
     hipGraphNode_t node;
     e = hipGraphAddKernelNode(&node, g->graph, nd ? deps : nullptr, (size_t)nd, &p);
     if (e != hipSuccess) break;
     g->nodes.push_back(node);
-    if (ops[i].branch == FSI_BATCH_BRANCH_SIDE) {
-      side_last = node;
+    if (ops[i].branch > FSI_BATCH_BRANCH_MAIN) {
+      last[ops[i].branch] = node;
     } else {
-      main_last = node;
-      if (ops[i].branch == FSI_BATCH_BRANCH_JOIN) side_last = nullptr;
+      last[0] = node;
+      if (ops[i].branch == FSI_BATCH_BRANCH_JOIN)
+        for (int c = 1; c <= FSI_BATCH_SIDE_CHAINS; ++c) last[c] = nullptr;
     }
   }
   if (rc == FSI_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);

@@ -233,8 +233,8 @@ class BatchOp(ctypes.Structure):
     _fields_ = [("op", i32), ("t", i32), ("block_map", vp), ("n_blocks", i32), ("branch", i32)]
 
 
-# FsiBatchOp.branch: graph chains of a batched exchange
-BATCH_BRANCH_MAIN, BATCH_BRANCH_SIDE, BATCH_BRANCH_JOIN = 0, 1, 2
+# FsiBatchOp.branch: graph chains of a batched exchange (side chains 1..BATCH_SIDE_CHAINS)
+BATCH_BRANCH_MAIN, BATCH_SIDE_CHAINS, BATCH_BRANCH_JOIN = 0, 3, -1
 
 
 ABI_STRUCTS = [LbmParams, LbmState, LbmCounters, LbmStats, MpmParams, MpmState, CouplingParams,

@@ -30,9 +30,12 @@ writes is then still in that cache when the next one reads it.  A 96^3
 lattice batched with three others costs 46 us per substep, against 34 us
 alone or with one other.  In the exchange graph these subcycles and the rest
 of the moving-boundary exchange form a side chain that runs beside the
-penalty instances' steps (`FSI_BATCH_BRANCH_SIDE`): the two touch no common
-instance, and the penalty chain's short, latency-bound launches leave the
-device room for the substeps.  The resident subcycles run before the fork,
+penalty instances' steps (side chain 1 of `FsiBatchOp.branch`): the two touch
+no common instance, and the penalty chain's short, latency-bound launches
+leave the device room for the substeps.  The penalty instances themselves are
+split into `penalty_chains` sets of about equal lattice size, each set's 120
+steps on a chain of its own, so that one set's short launches overlap the
+other's substeps.  The resident subcycles run before the fork,
 alone on the device, since each needs all its workgroups resident at once.
 
 Results are identical to running each instance alone: the kernels and their
@@ -96,7 +99,7 @@ def check_batchable(d):
 class BatchedFSI:
     def __init__(self, workloads=None, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True,
                  drivers=None, n_substeps=None, substep_plan=None, capacity=None, resident_lbm=True,
-                 subcycle_group_bytes=SUBCYCLE_GROUP_BYTES):
+                 subcycle_group_bytes=SUBCYCLE_GROUP_BYTES, penalty_chains=2):
         """Batch `workloads` (a driver is built for each), or batch initialized
         `drivers` that already exist (they keep their state).  n_substeps: LBM
         substeps per exchange (default: the moving-boundary instances' FSI
@@ -115,7 +118,9 @@ class BatchedFSI:
         moving-boundary instances run their subcycles in groups whose
         populations (one copy) take at most this many bytes, one group at a
         time (an instance larger than that is a group of its own); 0 = inside
-        the per-substep launches of the penalty instances."""
+        the per-substep launches of the penalty instances.  penalty_chains:
+        graph chains the penalty / none instances are split over (with
+        subcycle groups only; 1 = one chain)."""
         if substep_plan is not None and substep_plan not in SUBSTEP_PLANS:
             raise ValueError(f"substep_plan must be one of {SUBSTEP_PLANS} or None")
         self.substep_plan = substep_plan
@@ -156,6 +161,9 @@ class BatchedFSI:
         self.drivers = [None] * self.capacity
         self.resident_lbm = bool(resident_lbm)
         self.subcycle_group_bytes = int(subcycle_group_bytes or 0)
+        if not 1 <= int(penalty_chains) <= _native.BATCH_SIDE_CHAINS - 1:
+            raise ValueError(f"penalty_chains must be in 1..{_native.BATCH_SIDE_CHAINS - 1}")
+        self.penalty_chains = int(penalty_chains)
         self._res = [None] * self.capacity  # (halo, sync) of resident instances
         self._cus = (torch.cuda.get_device_properties(self.device).multi_processor_count
                      if self.device.type == "cuda" else None)
@@ -176,7 +184,8 @@ class BatchedFSI:
             self._fill_slot(self.drivers.index(None), d)
         self.use_graph = bool(use_graph)
         self._recording = None
-        self._side = False  # launches recorded now go on the side chain
+        self._chain = 0  # graph chain of the launches recorded now (0: main)
+        self._set = None  # penalty set the ONE-mode launches issued now cover (None: all)
         self._join = False  # the next recorded main-chain launch waits for the side chain
         self._plan_for = None
         self.exchanges = 0
@@ -188,7 +197,8 @@ class BatchedFSI:
         subcycle groups), so the exchange's launch list keeps its shape as
         requests come and go."""
         return cls(drivers=[], native=native, device=device, n_substeps=n_substeps, substep_plan=substep_plan,
-                   use_graph=use_graph, capacity=capacity, resident_lbm=False, subcycle_group_bytes=0)
+                   use_graph=use_graph, capacity=capacity, resident_lbm=False, subcycle_group_bytes=0,
+                   penalty_chains=1)
 
     # ----------------------------------------------------------------- slots
     def _fill_slot(self, k, d):
@@ -301,6 +311,20 @@ class BatchedFSI:
                 groups.append([nbytes, [k]])
         return [sorted(g[1]) for g in groups]
 
+    def penalty_sets(self):
+        """The live penalty / none slots split into the sets whose steps run on
+        chains of their own: largest lattice first, each to the set with the
+        fewest cells so far.  One set unless subcycle groups are on."""
+        one = [k for k in self.live_slots if self._inst_host[k].mode in ONE]
+        n = self.penalty_chains if self.subcycle_group_bytes else 1
+        sets = [[0, []] for _ in range(min(n, max(1, len(one))))]
+        cells = lambda k: self._inst_host[k].lp.nx * self._inst_host[k].lp.ny * self._inst_host[k].lp.nz
+        for k in sorted(one, key=lambda k: (-cells(k), k)):
+            s = min(sets, key=lambda e: e[0])
+            s[0] += cells(k)
+            s[1].append(k)
+        return [sorted(s[1]) for s in sets]
+
     @property
     def live_slots(self):
         return [k for k, d in enumerate(self.drivers) if d is not None]
@@ -334,10 +358,12 @@ class BatchedFSI:
                 entry[0][:n].copy_(torch.from_numpy(host))
             entry[1] = n
             return
-        if key[0] == "group":  # the substep tiles of one subcycle group's instances
-            groups = self.subcycle_groups()
+        if key[0] in ("group", "set"):
+            if key[0] == "group":  # the substep tiles of one subcycle group's instances
+                groups, sub = self.subcycle_groups(), (_native.BATCH_DOMAIN_PLANES, MB, 0)
+            else:  # ("set", c, domain, modes, min_mpm): that map cut to penalty set c
+                groups, sub = self.penalty_sets(), key[2:]
             members = groups[key[1]] if key[1] < len(groups) else []
-            sub = (_native.BATCH_DOMAIN_PLANES, MB, 0)
             self._write_map(sub)
             table, n = self._maps[sub]
             rows = table[:n]
@@ -378,11 +404,18 @@ class BatchedFSI:
 
     # ------------------------------------------------------------ launches
     def _launch(self, op, modes, t=0, min_mpm=0):
-        table, live = self._block_map(_native.BATCH_OP_DOMAIN[op], modes, min_mpm)
+        key = (_native.BATCH_OP_DOMAIN[op], tuple(sorted(modes)), min_mpm)
+        if self._set is not None and any(m in ONE for m in modes):
+            key = ("set", self._set) + key
+            if key not in self._maps:
+                self._write_map(key)
+            table, live = self._maps[key]
+        else:
+            table, live = self._block_map(*key)
         if live == 0:
             return
         if self._recording is not None:
-            self._record(op, t, (_native.BATCH_OP_DOMAIN[op], tuple(sorted(modes)), min_mpm))
+            self._record(op, t, key)
             return
         self.native.call("fsi_batch_launch", op, _native.ptr(self._inst_dev), self.capacity, _native.ptr(table),
                          live, t, stream_handle(self.device))
@@ -416,10 +449,10 @@ class BatchedFSI:
                          self.capacity, _native.ptr(table), live, t, stream_handle(self.device))
 
     def _record(self, op, t, key):
-        """Append a launch to the plan being recorded, on the graph's side
-        chain (while `_side` is set) or its main chain."""
-        if self._side:
-            branch = _native.BATCH_BRANCH_SIDE
+        """Append a launch to the plan being recorded, on the graph chain
+        `_chain` (0: the main chain)."""
+        if self._chain:
+            branch = self._chain
         elif self._join:
             branch, self._join = _native.BATCH_BRANCH_JOIN, False
         else:
@@ -435,7 +468,8 @@ class BatchedFSI:
 
     def _mpm_substeps(self, modes):
         """MPM substeps with each instance's reaction (fsi_mpm_substep_coupled per instance)."""
-        max_m = max(I.mpm_substeps for I in self._live_inst() if I.mode in modes)
+        slots = self.live_slots if self._set is None else self.penalty_sets()[self._set]
+        max_m = max(self._inst_host[k].mpm_substeps for k in slots if self._inst_host[k].mode in modes)
         for j in range(max_m):
             self._launch(_native.BATCH_OP_BOX_ROLL, modes, t=1, min_mpm=j)
             self._launch(_native.BATCH_OP_MPM_CLEAR, modes, min_mpm=j)
@@ -466,25 +500,32 @@ class BatchedFSI:
                 if self._inst_host[k].resident:
                     self._launch_resident(k)
             if groups:
-                # the rest of the moving-boundary exchange on a side chain of the
+                # the rest of the moving-boundary exchange on side chain 1 of the
                 # graph, beside the penalty instances' steps (other instances);
                 # the launch after those joins it
-                self._side = True
+                self._chain = 1
                 for g in range(len(groups)):
                     for t in range(self.n_substeps):
                         self._launch_group(g, t)
                 self._mb_exchange_end()
-                self._side = False
+                self._chain = 0
         per_substep = ONE if self.subcycle_group_bytes else tuple(_MODE.values())
-        for t in range(self.n_substeps):
-            if has_one:
-                self._project(ONE)
-                self._clear_counters(1, PEN)
-                self._launch(_native.BATCH_OP_PENALTY, PEN, t=t)  # stores the moments it reads (t >= 1)
-            self._issue_substep(t, per_substep)
-            if has_one:
-                self._mpm_substeps(ONE)
-        self._join = bool(groups)
+        sets = self.penalty_sets() if has_one else [[]]
+        for c in reversed(range(len(sets))):
+            # penalty set c on side chain c + 1, set 0 on the main chain: the side
+            # chains are recorded first, so that each forks where the main chain
+            # leaves off (after the resident subcycles)
+            self._set, self._chain = (c if len(sets) > 1 else None), (c + 1 if c else 0)
+            for t in range(self.n_substeps):
+                if has_one:
+                    self._project(ONE)
+                    self._clear_counters(1, PEN)
+                    self._launch(_native.BATCH_OP_PENALTY, PEN, t=t)  # stores the moments it reads (t >= 1)
+                self._issue_substep(t, per_substep)
+                if has_one:
+                    self._mpm_substeps(ONE)
+        self._set, self._chain = None, 0
+        self._join = bool(groups) or len(sets) > 1
         if has_one:
             # no moments pass between the steps (batch_moment_source): one after the last
             self._launch(_native.BATCH_OP_LBM_MOMENTS, ONE, t=self.n_substeps - 1)

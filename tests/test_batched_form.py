@@ -231,46 +231,62 @@ def test_batch_graph_rejects_other_launch_lists(emu, tmp_path):
     batch.close()
 
 
-def test_subcycle_groups_equal_alone(emu, tmp_path):
+def test_subcycle_groups_and_chains_equal_flat(emu, tmp_path):
     """Moving-boundary instances run their subcycles in groups (here one
     instance per group, and all in one group) outside the per-substep
-    launches of the penalty instances; every instance still equals the same
-    batch with the moving-boundary instances inside those launches, bit for bit."""
+    launches of the penalty instances, on side chain 1 of the exchange graph;
+    the penalty instances are split over the main chain and side chain 2 (or
+    all on the main chain).  Every instance still equals the same batch with
+    every instance in the per-substep launches of one chain, bit for bit."""
     ws = _mixed(tmp_path)
     runs = {}
-    for name, budget in (("own", 1), ("one", 1 << 30), ("flat", 0)):
-        b = BatchedFSI(ws, out_dir=str(tmp_path / name), native=emu, device="cpu", subcycle_group_bytes=budget)
+    for name, budget, chains in (("own", 1, 2), ("one", 1 << 30, 2), ("one1", 1 << 30, 1), ("flat", 0, 2)):
+        b = BatchedFSI(ws, out_dir=str(tmp_path / name), native=emu, device="cpu", subcycle_group_bytes=budget,
+                       penalty_chains=chains)
         b.step()
         b.step()
         runs[name] = b
     assert runs["own"].subcycle_groups() == [[0], [2]]
     assert runs["one"].subcycle_groups() == [[0, 2]]
     assert runs["flat"].subcycle_groups() == []
-    # the grouped batches' per-substep launches hold the penalty instances alone
-    for name in ("own", "one"):
+    assert runs["one"].penalty_sets() == [[3], [1]]  # 16^3 and 12^3
+    assert runs["one1"].penalty_sets() == [[1, 3]] and runs["flat"].penalty_sets() == [[1, 3]]
+    mb = _native.BATCH_MODE_MOVING_BOUNDARY
+    for name in ("own", "one", "one1"):
         plan = runs[name]._plan
-        mb = _native.BATCH_MODE_MOVING_BOUNDARY
+        # the grouped batches' per-substep launches hold the penalty instances alone
         assert not [key for _, _, key, _ in plan if key[0] == _native.BATCH_DOMAIN_PLANES and mb in key[1]]
-        # the groups' substeps and the rest of the moving-boundary exchange form
-        # the graph's side chain (moving-boundary instances only), joined by one
-        # later launch
-        side = [i for i, (op, _, key, br) in enumerate(plan) if br == _native.BATCH_BRANCH_SIDE]
-        groups = [i for i in side if plan[i][2][0] == "group"]
+        # side chain 1: the groups' substeps, then the rest of the moving-boundary exchange
+        c1 = [i for i, (*_, br) in enumerate(plan) if br == 1]
+        groups = [i for i in c1 if plan[i][2][0] == "group"]
         assert {plan[i][0] for i in groups} == {_native.BATCH_OP_LBM_SUBSTEP_MB}
-        assert len(groups) == len(runs[name].subcycle_groups()) * runs[name].n_substeps == side.index(groups[-1]) + 1
-        assert all(plan[i][2][0] == "group" or plan[i][2][1] == (mb,) for i in side)
-        assert _native.BATCH_OP_MPM_G2P in {plan[i][0] for i in side}
-        joins = [i for i, (_, _, _, br) in enumerate(plan) if br == _native.BATCH_BRANCH_JOIN]
-        assert len(joins) == 1 and joins[0] > side[-1]
+        assert len(groups) == len(runs[name].subcycle_groups()) * runs[name].n_substeps == c1.index(groups[-1]) + 1
+        assert all(plan[i][2][0] == "group" or plan[i][2][1] == (mb,) for i in c1)
+        assert _native.BATCH_OP_MPM_G2P in {plan[i][0] for i in c1}
+        # side chain 2 and the main chain after the fork: penalty sets 1 and 0
+        c2 = [i for i, (*_, br) in enumerate(plan) if br == 2]
+        fork = [i for i, (_, _, key, br) in enumerate(plan) if br == 0 and key[0] == "set"]
+        if name == "one1":
+            assert not c2 and not fork
+        else:
+            assert c2 and {plan[i][2][:2] for i in c2} == {("set", 1)}
+            assert fork and {plan[i][2][:2] for i in fork} == {("set", 0)} and fork[0] > c2[-1] > c1[-1]
+            assert len(c2) == len(fork)  # the same launches over the other set
+        joins = [i for i, (*_, br) in enumerate(plan) if br == _native.BATCH_BRANCH_JOIN]
+        assert len(joins) == 1 and joins[0] == len(plan) - 1
         assert plan[joins[0]][0] == _native.BATCH_OP_LBM_MOMENTS  # the penalty instances' last moments
     assert {br for *_, br in runs["flat"]._plan} == {_native.BATCH_BRANCH_MAIN}
     for k in range(4):
-        for name in ("own", "one"):
+        for name in ("own", "one", "one1"):
             a, b = runs["flat"].drivers[k], runs[name].drivers[k]
-            for field in ("F", "rho", "v", "hydro_force"):
+            for field in ("F", "rho", "v", "hydro_force", "cell_force"):
                 assert np.array_equal(getattr(a.lbm, field).to_numpy(), getattr(b.lbm, field).to_numpy()), (k, field)
             for field in ("x", "v", "F", "C"):
                 assert np.array_equal(getattr(a.solid, field).to_numpy(), getattr(b.solid, field).to_numpy()), (k, field)
+            ca, cb = runs["flat"].coupling_stats(k), runs[name].coupling_stats(k)
+            assert bytes(ca) == bytes(cb), (k, name)
+    with pytest.raises(ValueError, match="penalty_chains"):
+        BatchedFSI(ws, out_dir=str(tmp_path / "bad"), native=emu, device="cpu", penalty_chains=3)
 
 
 def test_subcycle_groups_first_fit(emu, tmp_path):

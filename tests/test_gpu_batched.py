@@ -42,11 +42,11 @@ def test_gpu_batched_matches_oracle(hip, tmp_path):
 
     print("substep plan tuning", batch.tuning, flush=True)
     assert batch.substep_plan in SUBSTEP_PLANS and batch.tuning["plan"] == batch.substep_plan
-    # the replayed graphs ran the moving-boundary subcycles on a side chain
-    # beside the penalty instances' steps
+    # the replayed graphs ran the moving-boundary subcycles on side chain 1,
+    # and the penalty / none instances in two sets on the main chain and side chain 2
     assert batch.subcycle_groups() == [[0, 2]] and batch.replays + batch.graph_builds == n_exchanges - 1
-    assert {br for *_, br in batch._plan} == {_native.BATCH_BRANCH_MAIN, _native.BATCH_BRANCH_SIDE,
-                                             _native.BATCH_BRANCH_JOIN}
+    assert len(batch.penalty_sets()) == 2
+    assert {br for *_, br in batch._plan} == {_native.BATCH_BRANCH_MAIN, 1, 2, _native.BATCH_BRANCH_JOIN}
     for k, (d, o) in enumerate(zip(batch.drivers, oracles)):
         steps = n_exchanges if d.config.coupling_mode == "moving_boundary" else n_exchanges * batch.n_substeps
         for _ in range(steps):
