@@ -107,6 +107,10 @@ int check_mpm(const FsiMpmParams* mp, const FsiMpmState* ms) {
   return FSI_OK;
 }
 
+// Kinds of block_reduce_n beside RED_SUM / RED_MIN / RED_MAX: the maximum
+// of floats, at least 0 (block_max_f), and the sum of ints (block_sum_i).
+constexpr int RED_MAXF0 = 3, RED_SUMI = 4;
+
 // ---------------------------------------------------------------------------
 // Block reductions: a wave64 reduction through DPP lane moves (register to
 // register: quad permutes, row mirrors, then the row broadcasts of lanes 15
@@ -167,6 +171,55 @@ __device__ __forceinline__ float block_max_f(float v, float* smem) {
   if (threadIdx.x == 0)
     for (int w = 0; w < (int)(blockDim.x / 64); ++w) r = smem[w] > r ? smem[w] : r;
   return r;
+}
+
+// N reductions of one workgroup at once, results valid in thread 0.  Each
+// value goes through the wave reduction of its kind (block_reduce,
+// block_max_f or block_sum_i); lane 0 of each wave stores the N wave results,
+// and thread 0 combines the waves in wave order from the same starting value.
+// The results therefore carry the same bits as N separate reductions, with
+// 2 barriers instead of 2 N.  smem holds N * (waves per workgroup) doubles.
+template <int N>
+__device__ __forceinline__ void block_reduce_n(double (&v)[N], const int (&op)[N], double* smem) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = (int)(blockDim.x / 64);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (op[i] == RED_MAXF0)
+      v[i] = (double)lane63(wave_dpp_reduce((float)v[i], -__builtin_inff(), [](float a, float b) { return b > a ? b : a; }));
+    else if (op[i] == RED_SUMI)
+      v[i] = (double)lane63(wave_dpp_reduce((int)v[i], 0, [](int a, int b) { return a + b; }));
+    else
+      v[i] = wave_reduce(v[i], op[i]);
+  }
+  if (blockDim.x == 64) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (op[i] == RED_MAXF0) v[i] = v[i] > 0.0 ? v[i] : 0.0;
+    return;
+  }
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) smem[i * waves + wave] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (op[i] == RED_MAXF0) {
+        float r = 0.0f;
+        for (int w = 0; w < waves; ++w) r = (float)smem[i * waves + w] > r ? (float)smem[i * waves + w] : r;
+        v[i] = (double)r;
+      } else if (op[i] == RED_SUMI) {
+        int r = 0;
+        for (int w = 0; w < waves; ++w) r += (int)smem[i * waves + w];
+        v[i] = (double)r;
+      } else {
+        double r = red_identity(op[i]);
+        for (int w = 0; w < waves; ++w) r = red_combine(op[i], r, smem[i * waves + w]);
+        v[i] = r;
+      }
+    }
+  }
 }
 
 __device__ __forceinline__ int block_sum_i(int v, int* smem) {
@@ -907,23 +960,14 @@ __global__ void __launch_bounds__(TPB) k_lbm_mb_begin(FsiLbmParams p, FsiLbmStat
 // a thousand workgroups would otherwise serialise on.
 constexpr int FIN_REC = 8;
 
-__device__ __forceinline__ void finalize_block_record(const FinalizeOut& o, double* rec, double* sd, float* sf,
-                                                      int* si) {
-  const int links = block_sum_i(o.links, si);
-  const float mc = block_max_f(o.max_corr, sf);
-  const float mn = block_max_f(o.mean_norm, sf);
-  const float rm = block_max_f(o.runmax, sf);
-  const double hx = block_reduce(o.hx, RED_SUM, sd);
-  const double hy = block_reduce(o.hy, RED_SUM, sd);
-  const double hz = block_reduce(o.hz, RED_SUM, sd);
+// sd: 7 doubles per wave of the workgroup
+__device__ __forceinline__ void finalize_block_record(const FinalizeOut& o, double* rec, double* sd) {
+  double v[7] = {(double)o.links, (double)o.max_corr, (double)o.mean_norm, (double)o.runmax, o.hx, o.hy, o.hz};
+  constexpr int op[7] = {RED_SUMI, RED_MAXF0, RED_MAXF0, RED_MAXF0, RED_SUM, RED_SUM, RED_SUM};
+  block_reduce_n(v, op, sd);
   if (threadIdx.x == 0) {
-    rec[0] = (double)links;
-    rec[1] = (double)mc;
-    rec[2] = (double)mn;
-    rec[3] = (double)rm;
-    rec[4] = hx;
-    rec[5] = hy;
-    rec[6] = hz;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) rec[i] = v[i];
     rec[7] = 0.0;
   }
 }
@@ -967,9 +1011,7 @@ __device__ __forceinline__ void finalize_fold(const double* ws, int n_rec, FsiLb
 template <typename Pop>
 __global__ void __launch_bounds__(TPB) k_lbm_finalize(FsiLbmParams p, FsiLbmState st, int cur, int parity,
                                                       int n_samples, int subcycled) {
-  __shared__ double sd[TPB / 64];
-  __shared__ float sf[TPB / 64];
-  __shared__ int si[TPB / 64];
+  __shared__ double sd[7 * TPB / 64];
   const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
   FinalizeOut o;
   o.links = 0;
@@ -978,7 +1020,7 @@ __global__ void __launch_bounds__(TPB) k_lbm_finalize(FsiLbmParams p, FsiLbmStat
   o.runmax = 0.0f;
   o.hx = o.hy = o.hz = 0.0;
   if (c < (long long)p.nx * p.ny * p.nz) o = lbm_finalize_cell<Pop>(p, st, cur, parity, n_samples, subcycled, c);
-  finalize_block_record(o, st.reduce_ws + (size_t)blockIdx.x * FIN_REC, sd, sf, si);
+  finalize_block_record(o, st.reduce_ws + (size_t)blockIdx.x * FIN_REC, sd);
 }
 
 __global__ void __launch_bounds__(TPB) k_lbm_finalize_fold(FsiLbmState st, int n_rec, int subcycled, int n_samples) {
@@ -1046,9 +1088,7 @@ __global__ void __launch_bounds__(PTPB) k_proj_particles(FsiLbmParams lp, FsiLbm
 
 __global__ void __launch_bounds__(TPB) k_proj_normalize(FsiLbmParams lp, FsiLbmState ls, FsiCouplingParams cp,
                                                         FsiCouplingCounters* cc) {
-  __shared__ double sd[TPB / 64];
-  __shared__ float sf[TPB / 64];
-  __shared__ int si[TPB / 64];
+  __shared__ double sd[5 * TPB / 64];
   const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
   ProjOut o;
   o.phi_raw = 0.0f;
@@ -1056,11 +1096,13 @@ __global__ void __launch_bounds__(TPB) k_proj_normalize(FsiLbmParams lp, FsiLbmS
   o.mass = 0.0f;
   o.active = 0;
   if (c < (long long)lp.nx * lp.ny * lp.nz) o = normalize_cell(lp, ls, cp, c);
-  const float mp = block_max_f(o.phi_raw > 0.0f ? o.phi_raw : 0.0f, sf);
-  const int act = block_sum_i(o.active, si);
-  const double m = block_reduce((double)o.mass, RED_SUM, sd);
-  const double vr = block_reduce((double)o.phi_raw * cp.cell_volume_norm, RED_SUM, sd);
-  const double vc = block_reduce((double)o.phi_clamped * cp.cell_volume_norm, RED_SUM, sd);
+  double v[5] = {(double)(o.phi_raw > 0.0f ? o.phi_raw : 0.0f), (double)o.active, (double)o.mass,
+                 (double)o.phi_raw * cp.cell_volume_norm, (double)o.phi_clamped * cp.cell_volume_norm};
+  constexpr int op[5] = {RED_MAXF0, RED_SUMI, RED_SUM, RED_SUM, RED_SUM};
+  block_reduce_n(v, op, sd);
+  const float mp = (float)v[0];
+  const int act = (int)v[1];
+  const double m = v[2], vr = v[3], vc = v[4];
   if (threadIdx.x == 0) {
     // a zero maximum (most workgroups) cannot raise the counter: no atomic on
     // the shared word
@@ -1086,18 +1128,17 @@ __global__ void k_penalty_clear(FsiCouplingCounters* cc) {
 
 __global__ void __launch_bounds__(TPB) k_penalty(FsiLbmParams lp, FsiLbmState ls, FsiCouplingParams cp,
                                                  FsiCouplingCounters* cc) {
-  __shared__ double sd[TPB / 64];
-  __shared__ float sf[TPB / 64];
-  __shared__ int si[TPB / 64];
+  __shared__ double sd[5 * TPB / 64];
   const long long c = (long long)blockIdx.x * TPB + threadIdx.x;
   float f[3] = {0.0f, 0.0f, 0.0f};
   int active = 0;
   if (c < (long long)lp.nx * lp.ny * lp.nz) penalty_cell(lp, ls, cp, c, f, active);
-  const float fn = norm3(f[0], f[1], f[2]);
-  const float mfn = block_max_f(fn, sf);
-  const int act = block_sum_i(active, si);
-  double net[3];
-  for (int d = 0; d < 3; ++d) net[d] = block_reduce((double)f[d], RED_SUM, sd);
+  double v[5] = {(double)norm3(f[0], f[1], f[2]), (double)active, (double)f[0], (double)f[1], (double)f[2]};
+  constexpr int op[5] = {RED_MAXF0, RED_SUMI, RED_SUM, RED_SUM, RED_SUM};
+  block_reduce_n(v, op, sd);
+  const float mfn = (float)v[0];
+  const int act = (int)v[1];
+  const double net[3] = {v[2], v[3], v[4]};
   if (threadIdx.x == 0) {
     if (act) atomicAdd(&cc->active_force_cell_count, act);
     if (__float_as_uint(mfn) != 0u) {
@@ -2210,9 +2251,7 @@ __global__ void __launch_bounds__(TPB) kb_mb_begin(const FsiBatchInstance* __res
 }
 
 __global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
-  __shared__ double sd[TPB / 64];
-  __shared__ float sf[TPB / 64];
-  __shared__ int si[TPB / 64];
+  __shared__ double sd[7 * TPB / 64];
   FSI_BATCH_PROLOGUE(FSI_BATCH_DOMAIN_CELLS, TPB)
   FinalizeOut o;
   o.links = 0;
@@ -2222,7 +2261,7 @@ __global__ void __launch_bounds__(TPB) kb_finalize(const FsiBatchInstance* __res
   if (in_)
     FSI_BY_STORAGE(I.lp.storage, o = lbm_finalize_cell<POP>(I.lp, I.ls, batch_cur(I, I.n_substeps), last & 1,
                                                             I.n_substeps, 1, e_));
-  finalize_block_record(o, I.ls.reduce_ws + (size_t)l_ * FIN_REC, sd, sf, si);
+  finalize_block_record(o, I.ls.reduce_ws + (size_t)l_ * FIN_REC, sd);
 }
 
 // one workgroup per instance (INSTANCES domain): fold the finalize records
@@ -2279,9 +2318,7 @@ __global__ void __launch_bounds__(PTPB) kb_proj_particles(const FsiBatchInstance
 }
 
 __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap) {
-  __shared__ double sd[TPB / 64];
-  __shared__ float sf[TPB / 64];
-  __shared__ int si[TPB / 64];
+  __shared__ double sd[5 * TPB / 64];
   FSI_BATCH_BOX_PROLOGUE
   const BoxWalk w = box_walk(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, I.lp.nx, I.lp.ny, I.lp.nz);
   float mp = 0.0f;
@@ -2295,11 +2332,14 @@ __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance*
     vr += (double)o.phi_raw * I.cp.cell_volume_norm;
     vc += (double)o.phi_clamped * I.cp.cell_volume_norm;
   }
-  mp = block_max_f(mp, sf);
-  act = block_sum_i(act, si);
-  m = block_reduce(m, RED_SUM, sd);
-  vr = block_reduce(vr, RED_SUM, sd);
-  vc = block_reduce(vc, RED_SUM, sd);
+  double v[5] = {(double)mp, (double)act, m, vr, vc};
+  constexpr int op[5] = {RED_MAXF0, RED_SUMI, RED_SUM, RED_SUM, RED_SUM};
+  block_reduce_n(v, op, sd);
+  mp = (float)v[0];
+  act = (int)v[1];
+  m = v[2];
+  vr = v[3];
+  vc = v[4];
   if (threadIdx.x == 0) {
     FsiCouplingCounters* cc = I.cc;
     if (mp > 0.0f) atomicMax(&cc->max_phi_raw_bits, __float_as_uint(mp));
@@ -2314,9 +2354,7 @@ __global__ void __launch_bounds__(TPB) kb_proj_normalize(const FsiBatchInstance*
 // the cells it covers, from the populations after the last step and that
 // step's cell force, as the moments pass after that step would have.
 __global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* __restrict__ inst, const int32_t* __restrict__ bmap, int t) {
-  __shared__ double sd[TPB / 64];
-  __shared__ float sf[TPB / 64];
-  __shared__ int si[TPB / 64];
+  __shared__ double sd[5 * TPB / 64];
   FSI_BATCH_BOX_PROLOGUE
   int32_t both[6];
   box_hull(I.boxes + FSI_BATCH_BOX_PROJ + BOX_CUR, I.boxes + FSI_BATCH_BOX_PROJ + BOX_PREV, both);
@@ -2345,9 +2383,12 @@ __global__ void __launch_bounds__(TPB) kb_penalty(const FsiBatchInstance* __rest
     act += active;
     for (int d = 0; d < 3; ++d) net[d] += (double)f[d];
   }
-  mfn = block_max_f(mfn, sf);
-  act = block_sum_i(act, si);
-  for (int d = 0; d < 3; ++d) net[d] = block_reduce(net[d], RED_SUM, sd);
+  double v[5] = {(double)mfn, (double)act, net[0], net[1], net[2]};
+  constexpr int op[5] = {RED_MAXF0, RED_SUMI, RED_SUM, RED_SUM, RED_SUM};
+  block_reduce_n(v, op, sd);
+  mfn = (float)v[0];
+  act = (int)v[1];
+  for (int d = 0; d < 3; ++d) net[d] = v[2 + d];
   if (threadIdx.x == 0) {
     FsiCouplingCounters* cc = I.cc;
     if (act) atomicAdd(&cc->active_force_cell_count, act);
