@@ -32,10 +32,11 @@ alone or with one other.  In the exchange graph these subcycles and the rest
 of the moving-boundary exchange form a side chain that runs beside the
 penalty instances' steps (side chain 1 of `FsiBatchOp.branch`): the two touch
 no common instance, and the penalty chain's short, latency-bound launches
-leave the device room for the substeps.  The penalty instances themselves are
-split into `penalty_chains` sets of about equal lattice size, each set's 120
-steps on a chain of its own, so that one set's short launches overlap the
-other's substeps.  The resident subcycles run before the fork,
+leave the device room for the substeps.  The penalty instances can be split
+further (`penalty_chains`): sets of about equal lattice size, each set's 120
+steps on a chain of its own.  On the 64-instance sweep that measured no faster
+(164.4, 165.9 and 167.5 ms per exchange for one, two and three sets,
+interleaved repeats), so one set is the default.  The resident subcycles run before the fork,
 alone on the device, since each needs all its workgroups resident at once.
 
 Results are identical to running each instance alone: the kernels and their
@@ -99,7 +100,7 @@ def check_batchable(d):
 class BatchedFSI:
     def __init__(self, workloads=None, out_dir="/tmp/fsi_batch", native=None, device=None, use_graph=True,
                  drivers=None, n_substeps=None, substep_plan=None, capacity=None, resident_lbm=True,
-                 subcycle_group_bytes=SUBCYCLE_GROUP_BYTES, penalty_chains=2):
+                 subcycle_group_bytes=SUBCYCLE_GROUP_BYTES, penalty_chains=1):
         """Batch `workloads` (a driver is built for each), or batch initialized
         `drivers` that already exist (they keep their state).  n_substeps: LBM
         substeps per exchange (default: the moving-boundary instances' FSI
@@ -161,8 +162,9 @@ class BatchedFSI:
         self.drivers = [None] * self.capacity
         self.resident_lbm = bool(resident_lbm)
         self.subcycle_group_bytes = int(subcycle_group_bytes or 0)
-        if not 1 <= int(penalty_chains) <= _native.BATCH_SIDE_CHAINS - 1:
-            raise ValueError(f"penalty_chains must be in 1..{_native.BATCH_SIDE_CHAINS - 1}")
+        # set c > 0 runs on side chain c + 1 (chain 1 is the subcycle groups')
+        if not 1 <= int(penalty_chains) <= _native.BATCH_SIDE_CHAINS:
+            raise ValueError(f"penalty_chains must be in 1..{_native.BATCH_SIDE_CHAINS}")
         self.penalty_chains = int(penalty_chains)
         self._res = [None] * self.capacity  # (halo, sync) of resident instances
         self._cus = (torch.cuda.get_device_properties(self.device).multi_processor_count

@@ -286,7 +286,7 @@ def test_subcycle_groups_and_chains_equal_flat(emu, tmp_path):
             ca, cb = runs["flat"].coupling_stats(k), runs[name].coupling_stats(k)
             assert bytes(ca) == bytes(cb), (k, name)
     with pytest.raises(ValueError, match="penalty_chains"):
-        BatchedFSI(ws, out_dir=str(tmp_path / "bad"), native=emu, device="cpu", penalty_chains=3)
+        BatchedFSI(ws, out_dir=str(tmp_path / "bad"), native=emu, device="cpu", penalty_chains=4)
 
 
 def test_subcycle_groups_first_fit(emu, tmp_path):

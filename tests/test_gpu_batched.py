@@ -28,7 +28,7 @@ def _mixed(tmp_path, storage="f32"):
 
 def test_gpu_batched_matches_oracle(hip, tmp_path):
     ws = _mixed(tmp_path)
-    batch = BatchedFSI(ws, out_dir=str(tmp_path / "batch"), native=hip)
+    batch = BatchedFSI(ws, out_dir=str(tmp_path / "batch"), native=hip, penalty_chains=2)
     oracles = []
     for w, d in zip(ws, batch.drivers):
         flags = d.lbm.static_solid.to_numpy()

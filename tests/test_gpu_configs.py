@@ -121,13 +121,13 @@ def test_gpu_c4_full_sweep_one_exchange(hip, tmp_path):
     assert resident and all(sizes[k] == 64 and modes[k] == "moving_boundary" for k in resident)
     assert any(k in resident for k in subset)
     # the other moving-boundary instances in subcycle groups, on the graph's side
-    # chain 1; the penalty instances in two sets, on the main chain and side chain 2
+    # chain 1, beside the penalty instances on the main chain
     grouped = sorted(k for g in batch.subcycle_groups() for k in g)
     assert grouped == [k for k in range(len(ws)) if modes[k] == "moving_boundary" and k not in resident]
     assert any(k in grouped for k in subset)
     oracles = {k: _oracles(ws[k], batch.drivers[k]) for k in subset}
     batch.step()
-    assert batch.graph_builds == 1 and {1, 2} <= {br for *_, br in batch._plan}
+    assert batch.graph_builds == 1 and 1 in {br for *_, br in batch._plan}
     batch.check_resident_waits()
     for d in batch.drivers:
         _finite_and_sane(d)
