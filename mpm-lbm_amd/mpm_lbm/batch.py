@@ -194,12 +194,12 @@ class BatchedFSI:
 
     @classmethod
     def slots(cls, capacity, native=None, device=None, n_substeps=120, substep_plan="split", use_graph=True):
-        """An empty slot table of `capacity` instances (the serving form).  Its
-        instances stay in the batched substep launches (no LBM_RESIDENT, no
-        subcycle groups), so the exchange's launch list keeps its shape as
-        requests come and go."""
+        """An empty slot table of `capacity` instances (the serving form).  No
+        instance runs LBM_RESIDENT, and the moving-boundary instances form one
+        subcycle group (on the graph's side chain), so the exchange's launch
+        list keeps its shape as requests come and go."""
         return cls(drivers=[], native=native, device=device, n_substeps=n_substeps, substep_plan=substep_plan,
-                   use_graph=use_graph, capacity=capacity, resident_lbm=False, subcycle_group_bytes=0,
+                   use_graph=use_graph, capacity=capacity, resident_lbm=False, subcycle_group_bytes=1 << 62,
                    penalty_chains=1)
 
     # ----------------------------------------------------------------- slots
